@@ -1,0 +1,205 @@
+/*
+ * reffine_hip.h — C ABI of the MI355X (gfx950) execution path for reffine
+ * queries.
+ *
+ * Every function returns an int status (RF_OK == 0). On failure the
+ * calling thread's last error message is available from rf_last_error().
+ * Nothing in this library calls exit() or abort() across the ABI.
+ *
+ * The reference launches GPU code through CudaEngine
+ * (include/reffine/engine/cuda_engine.h:24-35, src/engine/cuda_engine.cpp:
+ * 25-87) and runs queries compiled by compile_op<T>/compile_loop<T>
+ * (include/reffine/utils/utils.h:20-54), whose generated entry point is
+ * `extern "C" void <name>(OutT* output_addr, In0 a0, ...)`
+ * (src/pass/canonpass.cpp:50-64, src/pass/cemitter.cpp:227-252).
+ * Each declaration below names the reference interface it replaces.
+ */
+#ifndef REFFINE_HIP_H_
+#define REFFINE_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Arrow C data interface (identical to include/reffine/arrow/abi.h:18-49). */
+#ifndef ARROW_C_DATA_INTERFACE
+#define ARROW_C_DATA_INTERFACE
+#define ARROW_FLAG_DICTIONARY_ORDERED 1
+#define ARROW_FLAG_NULLABLE 2
+#define ARROW_FLAG_MAP_KEYS_SORTED 4
+struct ArrowSchema {
+    const char* format;
+    const char* name;
+    const char* metadata;
+    int64_t flags;
+    int64_t n_children;
+    struct ArrowSchema** children;
+    struct ArrowSchema* dictionary;
+    void (*release)(struct ArrowSchema*);
+    void* private_data;
+};
+struct ArrowArray {
+    int64_t length;
+    int64_t null_count;
+    int64_t offset;
+    int64_t n_buffers;
+    int64_t n_children;
+    const void** buffers;
+    struct ArrowArray** children;
+    struct ArrowArray* dictionary;
+    void (*release)(struct ArrowArray*);
+    void* private_data;
+};
+#endif /* ARROW_C_DATA_INTERFACE */
+
+enum rf_status {
+    RF_OK = 0,
+    RF_ERR_INVALID = 1,     /* bad argument, malformed IR, type error      */
+    RF_ERR_UNSUPPORTED = 2, /* IR shape the GPU lowering does not accept   */
+    RF_ERR_HIP = 3,         /* HIP runtime / hiprtc failure                */
+    RF_ERR_NOMEM = 4,       /* device memory or admission limit exceeded   */
+    RF_ERR_ARROW = 5,       /* Arrow input the device layer cannot take    */
+    RF_ERR_BUSY = 6         /* server queue full / no free slot            */
+};
+
+/* Opaque handles. */
+typedef struct rf_table rf_table;   /* device-resident Arrow record batch   */
+typedef struct rf_query rf_query;   /* lowered + JIT-specialised query plan */
+
+/* One argument of a compiled query, in Func::inputs order. A VECTOR input
+ * is a device table; a primitive input is passed by value (int64 for all
+ * integer / idx / bool types, double for float types), matching the by-value
+ * primitives of the generated entry point (include/reffine/base/type.h:
+ * 212-249). */
+typedef struct rf_arg {
+    int32_t kind; /* 0 = table, 1 = int64, 2 = double */
+    int32_t pad;
+    union {
+        rf_table* table;
+        int64_t i64;
+        double f64;
+    } u;
+} rf_arg;
+
+/* Result of a query. A reduction writes its state struct (C layout, fields
+ * in declaration order, natural alignment — the ScalarPass output layout,
+ * src/pass/scalarpass.cpp:103-227) to `output_addr`. An Op writes a device
+ * table handle into `*out_table` (the reference writes an ArrowTable* into
+ * its ArrowTable** output, src/pass/loopgen.cpp:113-204). */
+
+/* ---------------------------------------------------------------- device */
+
+/* Replaces CudaEngine::Get() (src/engine/cuda_engine.cpp:25-43): selects
+ * the HIP device and creates the library's stream. Idempotent. */
+int rf_init(int device);
+
+/* Thread-local message of the last failing call ("" when none). */
+const char* rf_last_error(void);
+
+/* Device properties: CU count, HBM bytes, free bytes. */
+int rf_device_info(int32_t* num_cus, int64_t* total_bytes, int64_t* free_bytes);
+
+/* Blocks until all work issued by the library on its stream is finished
+ * (replaces cuCtxSynchronize, main.cpp:229). */
+int rf_sync(void);
+
+/* ----------------------------------------------------------------- tables */
+
+/* Uploads a host Arrow record batch (struct array "+s", one child per
+ * column: formats c,C,s,S,i,I,l,L,f,g,b? and "+r" run-end children) into
+ * HBM. Replaces the host ArrowTable2 built by load_arrow_file
+ * (src/utils/utils.cpp:11-34, include/reffine/arrow/table.h:24-60): value
+ * buffers and validity bitmaps are copied to 256-byte aligned device
+ * memory. `dim` is the table's iterator dimension (reffine's
+ * ArrowTable::dim, include/reffine/arrow/abi.h:56-69). Children with
+ * offset != 0 are rejected with RF_ERR_ARROW. */
+int rf_table_upload(const struct ArrowSchema* schema,
+                    const struct ArrowArray* array, int64_t dim,
+                    rf_table** out);
+
+/* Creates a device table over column buffers that are already in device
+ * memory (borrowed; the caller keeps them alive). `formats[i]` is the Arrow
+ * format string of column i; `validity[i]` may be NULL (all valid). */
+int rf_table_wrap_device(int64_t dim, int64_t n_cols, int64_t n_rows,
+                         const char* const* formats, void* const* values,
+                         void* const* validity, rf_table** out);
+
+/* Number of rows, columns, and a column's device value / validity pointers.
+ * Replaces the vinstr accessors get_vector_len / get_vector_data_buf /
+ * get_vector_bit_buf (include/reffine/vinstr/vinstr.h:43-69). */
+int rf_table_shape(const rf_table* t, int64_t* n_rows, int64_t* n_cols);
+int rf_table_column(const rf_table* t, int64_t col, void** values,
+                    void** validity, int64_t* length, char* format_out,
+                    size_t format_cap);
+
+/* Copies column `col` values (and, if `validity_out` is non-NULL, its
+ * validity bitmap, (length+7)/8 bytes, all-ones when the column has none)
+ * to host memory. */
+int rf_table_download_column(const rf_table* t, int64_t col, void* values_out,
+                             size_t values_cap, uint8_t* validity_out,
+                             size_t validity_cap);
+
+/* Builds the device hash index over int64 column 0 (key -> first valid
+ * row). Replaces ArrowTable2::build_index / build_vector_index
+ * (include/reffine/arrow/table.h:107-138, src/vinstr/external.cpp:11-16). */
+int rf_table_build_index(rf_table* t);
+
+/* Looks up `n` keys in the index built above; writes the row or -1
+ * (vector_locate, include/reffine/vinstr/vinstr.h:137-146). Device
+ * pointers. */
+int rf_table_locate(const rf_table* t, const int64_t* d_keys, int64_t n,
+                    int64_t* d_rows_out);
+
+int rf_table_free(rf_table* t);
+
+/* ---------------------------------------------------------------- queries */
+
+/* Compiles a reffine Func given as IR text (the S-expression form written
+ * by reffine_amd.reffiner, one node per reffine IR node). Runs the GPU
+ * lowering (the counterpart of LoopGen, src/pass/loopgen.cpp:44-269) and
+ * specialises the hand-written kernels for the query's row expressions.
+ * Replaces compile_op<T>(func, vectorize) (include/reffine/utils/utils.h:
+ * 47-54). `flags`: RF_COMPILE_DETERMINISTIC selects the fixed-order
+ * reduction mode. */
+#define RF_COMPILE_DETERMINISTIC 1u
+int rf_compile(const char* ir_text, uint32_t flags, rf_query** out);
+
+/* Lowers and JIT-compiles a Func for gfx950 without a device (no module
+ * load): the build-time check that a query shape is accepted. `flags` bit 0
+ * as rf_compile; bit 1 compiles the variant with validity bitmaps on every
+ * column. Writes the lowering summary and kernel source to `buf`. No
+ * reference counterpart (the reference compiles only inside compile_op). */
+int rf_compile_check(const char* ir_text, uint32_t flags, char* buf,
+                     size_t cap);
+
+/* Byte size of the query's reduction output struct (0 for Op outputs), and
+ * whether the output is a table. */
+int rf_query_output_info(const rf_query* q, int64_t* out_bytes,
+                         int32_t* out_is_table);
+
+/* Human-readable lowering summary (kernel kind, columns, fields). */
+int rf_query_describe(const rf_query* q, char* buf, size_t cap);
+
+/* Replaces the call of the generated entry point `void name(OutT*, ...)`:
+ * runs the query on device tables and synchronises. `output_addr` receives
+ * the reduction struct (host memory, rf_query_output_info bytes); for Op
+ * queries `out_table` receives a new device table (free with
+ * rf_table_free). */
+int rf_query_run(rf_query* q, const rf_arg* args, int64_t n_args,
+                 void* output_addr, rf_table** out_table);
+
+/* Times `iters` back-to-back launches of the query's main kernel with HIP
+ * events on the library stream; writes the mean milliseconds per run. */
+int rf_query_time(rf_query* q, const rf_arg* args, int64_t n_args,
+                  int32_t iters, double* ms_per_run);
+
+int rf_query_free(rf_query* q);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* REFFINE_HIP_H_ */

@@ -1,0 +1,250 @@
+// extern "C" entry points declared in include/reffine_hip.h. Every call
+// catches exceptions and turns them into a status code plus the thread's
+// last-error text; nothing crosses the ABI as an exception or exit().
+#include <cstring>
+#include <string>
+
+#include "engine.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guard(F&& f) {
+    try {
+        g_last_error.clear();
+        f();
+        return RF_OK;
+    } catch (const rf::Unsupported& e) {
+        g_last_error = e.what();
+        return RF_ERR_UNSUPPORTED;
+    } catch (const rf::IRError& e) {
+        g_last_error = e.what();
+        return RF_ERR_INVALID;
+    } catch (const rf::NoMem& e) {
+        g_last_error = e.what();
+        return RF_ERR_NOMEM;
+    } catch (const rf::ArrowError& e) {
+        g_last_error = e.what();
+        return RF_ERR_ARROW;
+    } catch (const rf::HipError& e) {
+        g_last_error = e.what();
+        return RF_ERR_HIP;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return RF_ERR_INVALID;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return RF_ERR_INVALID;
+    }
+}
+
+void need(bool c, const char* msg) {
+    if (!c) throw rf::IRError(msg);
+}
+
+}  // namespace
+
+extern "C" {
+
+int rf_init(int device) {
+    return guard([&] { rf::init_device(device); });
+}
+
+const char* rf_last_error(void) { return g_last_error.c_str(); }
+
+int rf_device_info(int32_t* num_cus, int64_t* total_bytes, int64_t* free_bytes) {
+    return guard([&] {
+        rf::Device& d = rf::device();
+        size_t fr = 0, tot = 0;
+        RF_HIP(hipMemGetInfo(&fr, &tot));
+        if (num_cus) *num_cus = d.num_cus;
+        if (total_bytes) *total_bytes = (int64_t)tot;
+        if (free_bytes) *free_bytes = (int64_t)fr;
+    });
+}
+
+int rf_sync(void) {
+    return guard([&] {
+        RF_HIP(hipStreamSynchronize(rf::device().stream));
+        RF_HIP(hipDeviceSynchronize());
+    });
+}
+
+int rf_table_upload(const struct ArrowSchema* schema, const struct ArrowArray* array,
+                    int64_t dim, rf_table** out) {
+    return guard([&] {
+        need(out != nullptr, "out is NULL");
+        *out = rf::table_upload(schema, array, dim);
+    });
+}
+
+int rf_table_wrap_device(int64_t dim, int64_t n_cols, int64_t n_rows,
+                         const char* const* formats, void* const* values,
+                         void* const* validity, rf_table** out) {
+    return guard([&] {
+        need(out && formats && values, "NULL argument");
+        *out = rf::table_wrap(dim, n_cols, n_rows, formats, values, validity);
+    });
+}
+
+int rf_table_shape(const rf_table* t, int64_t* n_rows, int64_t* n_cols) {
+    return guard([&] {
+        need(t != nullptr, "table is NULL");
+        if (n_rows) *n_rows = t->rows();
+        if (n_cols) *n_cols = (int64_t)t->cols.size();
+    });
+}
+
+int rf_table_column(const rf_table* t, int64_t col, void** values, void** validity,
+                    int64_t* length, char* format_out, size_t format_cap) {
+    return guard([&] {
+        need(t != nullptr, "table is NULL");
+        need(col >= 0 && col < (int64_t)t->cols.size(), "column out of range");
+        const rf::Column& c = t->cols[col];
+        if (values) *values = c.values;
+        if (validity) *validity = c.validity;
+        if (length) *length = c.length;
+        if (format_out && format_cap) {
+            std::strncpy(format_out, c.format.c_str(), format_cap - 1);
+            format_out[format_cap - 1] = 0;
+        }
+    });
+}
+
+int rf_table_download_column(const rf_table* t, int64_t col, void* values_out,
+                             size_t values_cap, uint8_t* validity_out, size_t validity_cap) {
+    return guard([&] {
+        need(t != nullptr, "table is NULL");
+        need(col >= 0 && col < (int64_t)t->cols.size(), "column out of range");
+        const rf::Column& c = t->cols[col];
+        rf::Device& d = rf::device();
+        size_t bytes = (size_t)c.length * c.type.width();
+        need(values_cap >= bytes, "values buffer too small");
+        if (bytes)
+            RF_HIP(hipMemcpyAsync(values_out, c.values, bytes, hipMemcpyDeviceToHost, d.stream));
+        if (validity_out) {
+            size_t vb = ((size_t)c.length + 7) / 8;
+            need(validity_cap >= vb, "validity buffer too small");
+            if (c.validity)
+                RF_HIP(hipMemcpyAsync(validity_out, c.validity, vb, hipMemcpyDeviceToHost, d.stream));
+            else
+                std::memset(validity_out, 0xff, vb);
+        }
+        RF_HIP(hipStreamSynchronize(d.stream));
+    });
+}
+
+int rf_table_build_index(rf_table* t) {
+    return guard([&] {
+        need(t != nullptr, "table is NULL");
+        rf::table_build_index(t);
+    });
+}
+
+int rf_table_locate(const rf_table* t, const int64_t* d_keys, int64_t n, int64_t* d_rows_out) {
+    return guard([&] {
+        need(t != nullptr, "table is NULL");
+        need(t->has_index, "table has no index (call rf_table_build_index)");
+        rf::Device& d = rf::device();
+        RF_HIP(rf::aot_index_lookup(rf::probe_of(t), d_keys, n, d_rows_out, d.stream));
+        RF_HIP(hipStreamSynchronize(d.stream));
+    });
+}
+
+int rf_table_free(rf_table* t) {
+    return guard([&] { rf::table_free(t); });
+}
+
+int rf_compile(const char* ir_text, uint32_t flags, rf_query** out) {
+    return guard([&] {
+        need(ir_text && out, "NULL argument");
+        rf::device();
+        rf::N f = rf::parse_func(ir_text);
+        auto q = std::make_unique<rf_query>();
+        q->plan = rf::lower_func(f, false);
+        q->det = (flags & RF_COMPILE_DETERMINISTIC) != 0;
+        q->tune = rf::default_tuning(q->plan);
+        // Compile the all-valid specialisation now so compile errors surface
+        // here, as compile_op's do.
+        std::vector<std::vector<bool>> nl(q->plan.inputs.size());
+        for (size_t i = 0; i < q->plan.inputs.size(); i++)
+            nl[i].assign(q->plan.inputs[i]->type.dtypes.size(), false);
+        rf::Spec sp;
+        sp.ks = rf::gen_scan(q->plan, nl, q->tune, q->det);
+        sp.mod = &rf::jit_module(sp.ks);
+        std::string sig;
+        for (size_t i = 0; i < q->plan.inputs.size(); i++) {
+            if (!q->plan.inputs[i]->type.is_vector()) continue;
+            sig += std::string(q->plan.inputs[i]->type.dtypes.size(), '0') + "|";
+        }
+        q->specs.emplace(sig, std::move(sp));
+        *out = q.release();
+    });
+}
+
+int rf_compile_check(const char* ir_text, uint32_t flags, char* buf, size_t cap) {
+    return guard([&] {
+        need(ir_text != nullptr, "NULL argument");
+        rf::N f = rf::parse_func(ir_text);
+        rf::Plan plan = rf::lower_func(f, false);
+        rf::Tuning tune = rf::default_tuning(plan);
+        std::vector<std::vector<bool>> nl(plan.inputs.size());
+        for (size_t i = 0; i < plan.inputs.size(); i++)
+            nl[i].assign(plan.inputs[i]->type.dtypes.size(), (flags & 2u) != 0);
+        rf::KernelSource ks = rf::gen_scan(plan, nl, tune, (flags & RF_COMPILE_DETERMINISTIC) != 0);
+        std::vector<char> code = rf::jit_compile(ks);
+        if (buf && cap) {
+            std::string s = plan.describe() + "code object bytes: " + std::to_string(code.size()) +
+                            "\n--- kernel ---\n" + ks.code;
+            std::strncpy(buf, s.c_str(), cap - 1);
+            buf[cap - 1] = 0;
+        }
+    });
+}
+
+int rf_query_output_info(const rf_query* q, int64_t* out_bytes, int32_t* out_is_table) {
+    return guard([&] {
+        need(q != nullptr, "query is NULL");
+        size_t sz = 0, al = 0;
+        rf::c_layout(q->plan.output_type, sz, al);
+        if (out_bytes) *out_bytes = q->plan.output_type.is_vector() ? 0 : (int64_t)sz;
+        if (out_is_table) *out_is_table = q->plan.output_type.is_vector() ? 1 : 0;
+    });
+}
+
+int rf_query_describe(const rf_query* q, char* buf, size_t cap) {
+    return guard([&] {
+        need(q && buf && cap, "NULL argument");
+        std::string s = q->plan.describe();
+        auto tn = q->tune;
+        s += "tuning: block " + std::to_string(tn.block) + " items " + std::to_string(tn.items) +
+             " sub " + std::to_string(tn.sub) + (q->det ? " deterministic" : "") + "\n";
+        if (!q->specs.empty()) s += "--- kernel ---\n" + q->specs.begin()->second.ks.code;
+        std::strncpy(buf, s.c_str(), cap - 1);
+        buf[cap - 1] = 0;
+    });
+}
+
+int rf_query_run(rf_query* q, const rf_arg* args, int64_t n_args, void* output_addr,
+                 rf_table** out_table) {
+    return guard([&] {
+        need(q != nullptr, "query is NULL");
+        rf::run_query(q, args, n_args, output_addr, out_table);
+    });
+}
+
+int rf_query_time(rf_query* q, const rf_arg* args, int64_t n_args, int32_t iters,
+                  double* ms_per_run) {
+    return guard([&] {
+        need(q && ms_per_run, "NULL argument");
+        *ms_per_run = rf::time_query(q, args, n_args, iters);
+    });
+}
+
+int rf_query_free(rf_query* q) {
+    return guard([&] { delete q; });
+}
+
+}  // extern "C"

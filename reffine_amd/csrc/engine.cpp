@@ -1,0 +1,915 @@
+// HIP runtime layer (see engine.hpp).
+#include "engine.hpp"
+
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <sstream>
+
+namespace rf {
+
+#include "rf_embed.inc"  // kRfAbiSrc, kRfDeviceSrc: kernel header texts
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static Device g_dev;
+
+Device& device() {
+    if (g_dev.id < 0) throw HipError("rf_init has not been called");
+    return g_dev;
+}
+
+Device& init_device(int id) {
+    std::lock_guard<std::mutex> lk(g_dev.mu);
+    if (g_dev.id == id) return g_dev;
+    if (g_dev.id >= 0) throw HipError("library already bound to another device");
+    int n = 0;
+    RF_HIP(hipGetDeviceCount(&n));
+    if (id < 0 || id >= n) throw HipError("no HIP device " + std::to_string(id));
+    RF_HIP(hipSetDevice(id));
+    hipDeviceProp_t prop;
+    RF_HIP(hipGetDeviceProperties(&prop, id));
+    std::string arch = prop.gcnArchName;
+    if (arch.rfind("gfx950", 0) != 0)
+        throw HipError("device " + std::to_string(id) + " is " + arch +
+                       "; this library targets gfx950 (MI355X) only");
+    g_dev.num_cus = prop.multiProcessorCount;
+    g_dev.arch = "gfx950";
+    RF_HIP(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+    g_dev.id = id;
+    return g_dev;
+}
+
+void* dev_alloc(size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    bytes = (bytes + 255) & ~size_t(255);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw NoMem("device allocation of " + std::to_string(bytes) + " bytes failed: " +
+                    hipGetErrorString(e));
+    }
+    return p;
+}
+
+void dev_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+void dev_fill_u64(void* p, uint64_t v, int64_t n) {
+    RF_HIP(aot_fill_u64(p, v, n, device().stream));
+}
+
+// ------------------------------------------------------------------ tables
+
+static bool format_type(const std::string& f, Type& t) {
+    static const std::map<std::string, BT> m = {
+        {"c", BT::INT8},   {"C", BT::UINT8},  {"s", BT::INT16}, {"S", BT::UINT16},
+        {"i", BT::INT32},  {"I", BT::UINT32}, {"l", BT::INT64}, {"L", BT::UINT64},
+        {"f", BT::FLOAT32}, {"g", BT::FLOAT64}};
+    auto it = m.find(f);
+    if (it == m.end()) return false;
+    t = Type(it->second);
+    return true;
+}
+
+}  // namespace rf
+
+rf::Type rf_table::type() const {
+    std::vector<rf::Type> d;
+    std::vector<rf::Enc> e;
+    for (auto& c : cols) {
+        d.push_back(c.type);
+        e.push_back(c.enc);
+    }
+    return rf::Type(rf::BT::VECTOR, d, (int)dim, e);
+}
+
+namespace rf {
+
+rf_table* table_upload(const ArrowSchema* schema, const ArrowArray* array, int64_t dim) {
+    if (!schema || !array) throw ArrowError("null schema or array");
+    if (!schema->format || std::string(schema->format) != "+s")
+        throw ArrowError("record batch must be a struct array (format \"+s\")");
+    if (schema->n_children != array->n_children)
+        throw ArrowError("schema/array child count mismatch");
+    if (array->offset != 0) throw ArrowError("struct array offset != 0 is not supported");
+    if (dim < 1 || dim > schema->n_children) throw ArrowError("bad dim");
+    Device& d = device();
+    auto t = std::make_unique<rf_table>();
+    t->dim = dim;
+    struct Piece {
+        const void* src;
+        size_t bytes;
+        size_t off;
+        void** dst;
+    };
+    std::vector<Piece> pieces;
+    size_t total = 0;
+    auto add = [&](const void* src, size_t bytes, void** dst) {
+        pieces.push_back({src, bytes, total, dst});
+        total += (bytes + 255) & ~size_t(255);
+        if (bytes == 0) total += 256;
+    };
+    t->cols.resize(schema->n_children);
+    for (int64_t i = 0; i < schema->n_children; i++) {
+        const ArrowSchema* cs = schema->children[i];
+        const ArrowArray* ca = array->children[i];
+        Column& c = t->cols[i];
+        c.format = cs->format ? cs->format : "";
+        const ArrowArray* va = ca;
+        const ArrowSchema* vs = cs;
+        if (ca->offset != 0)
+            throw ArrowError("column " + std::to_string(i) + " has offset != 0");
+        if (c.format == "+r") {
+            if (ca->n_children != 2 || cs->n_children != 2)
+                throw ArrowError("run-end column needs run_ends and values children");
+            if (std::string(cs->children[0]->format) != "i")
+                throw ArrowError("run ends must be int32");
+            const ArrowArray* re = ca->children[0];
+            va = ca->children[1];
+            vs = cs->children[1];
+            if (re->offset != 0 || va->offset != 0)
+                throw ArrowError("run-end children with offset != 0");
+            c.enc = Enc::RUNEND;
+            c.logical_length = ca->length;
+            add(re->buffers[1], (size_t)re->length * 4, &c.run_ends);
+        }
+        if (!format_type(vs->format ? vs->format : "", c.type))
+            throw ArrowError("unsupported Arrow format '" + std::string(vs->format) + "'");
+        c.length = va->length;
+        if (c.enc == Enc::FLAT) c.logical_length = c.length;
+        c.null_count = va->null_count;
+        if (va->n_buffers < 2) throw ArrowError("primitive array without a values buffer");
+        add(va->buffers[1], (size_t)c.length * c.type.width(), &c.values);
+        if (va->buffers[0] && va->null_count != 0) {
+            size_t bytes = ((size_t)c.length + 7) / 8;
+            add(va->buffers[0], bytes, &c.validity);
+        }
+    }
+    char* arena = (char*)dev_alloc(total);
+    t->owned.push_back(arena);
+    for (auto& p : pieces) {
+        *p.dst = arena + p.off;
+        if (p.bytes && p.src)
+            RF_HIP(hipMemcpyAsync(arena + p.off, p.src, p.bytes, hipMemcpyHostToDevice, d.stream));
+        // Bitmaps are read as uint16 words: zero the padding byte(s).
+        size_t padded = (p.bytes + 255) & ~size_t(255);
+        if (padded > p.bytes)
+            RF_HIP(hipMemsetAsync(arena + p.off + p.bytes, 0, padded - p.bytes, d.stream));
+    }
+    RF_HIP(hipStreamSynchronize(d.stream));
+    return t.release();
+}
+
+rf_table* table_wrap(int64_t dim, int64_t n_cols, int64_t n_rows, const char* const* formats,
+                     void* const* values, void* const* validity) {
+    if (n_cols < 1 || dim < 1 || dim > n_cols) throw ArrowError("bad table shape");
+    auto t = std::make_unique<rf_table>();
+    t->dim = dim;
+    t->cols.resize(n_cols);
+    for (int64_t i = 0; i < n_cols; i++) {
+        Column& c = t->cols[i];
+        c.format = formats[i];
+        if (!format_type(c.format, c.type))
+            throw ArrowError("unsupported Arrow format '" + c.format + "'");
+        c.length = c.logical_length = n_rows;
+        c.values = values[i];
+        c.validity = validity ? validity[i] : nullptr;
+        if ((uintptr_t)c.values % 16 != 0) throw ArrowError("device column not 16-byte aligned");
+    }
+    return t.release();
+}
+
+static uint64_t next_pow2(uint64_t x) {
+    uint64_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+void table_build_index(rf_table* t) {
+    if (t->cols.empty()) throw ArrowError("empty table");
+    const Column& c = t->cols[0];
+    if (c.type != types::I64 || c.enc != Enc::FLAT)
+        throw ArrowError("index needs a flat int64 column 0 "
+                         "(include/reffine/arrow/table.h:107-138)");
+    Device& d = device();
+    if (t->has_index) return;
+    const uint64_t cap = next_pow2(std::max<uint64_t>(16, 2 * (uint64_t)c.length));
+    void* keys = dev_alloc(cap * 8);
+    t->owned.push_back(keys);
+    void* rows = dev_alloc(cap * 8);
+    t->owned.push_back(rows);
+    void* mk = dev_alloc(8);
+    dev_fill_u64(keys, (uint64_t)RF_EMPTY_KEY, (int64_t)cap);
+    dev_fill_u64(rows, (uint64_t)INT64_MAX, (int64_t)cap);
+    dev_fill_u64(mk, (uint64_t)INT64_MAX, 1);
+    RF_HIP(aot_index_build((const int64_t*)c.values, (const uint16_t*)c.validity, c.length, keys,
+                           rows, cap - 1, mk, d.stream));
+    int64_t mkr = 0;
+    RF_HIP(hipMemcpyAsync(&mkr, mk, 8, hipMemcpyDeviceToHost, d.stream));
+    RF_HIP(hipStreamSynchronize(d.stream));
+    dev_free(mk);
+    t->index.keys = keys;
+    t->index.rows = rows;
+    t->index.mask = cap - 1;
+    t->index.min_key_row = (mkr == INT64_MAX) ? -1 : mkr;
+    t->has_index = true;
+}
+
+RfProbeDev probe_of(const rf_table* t) {
+    RfProbeDev h{};
+    h.keys = (const int64_t*)t->index.keys;
+    h.rows = (const int64_t*)t->index.rows;
+    h.mask = t->index.mask;
+    h.min_key_row = t->index.min_key_row;
+    return h;
+}
+
+// -------------------------------------------------------------------- JIT
+
+static std::mutex g_jit_mu;
+static std::map<std::string, std::unique_ptr<Module>> g_modules;
+
+static std::string full_source(const KernelSource& ks) {
+    return std::string("#include \"rf_device.h\"\n") + ks.code;
+}
+
+std::vector<char> jit_compile(const KernelSource& ks) {
+    const std::string src = full_source(ks);
+    const char* headers[] = {kRfAbiSrc, kRfDeviceSrc};
+    const char* names[] = {"rf_abi.h", "rf_device.h"};
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "rf_query.hip", 2, headers, names) !=
+        HIPRTC_SUCCESS)
+        throw HipError("hiprtcCreateProgram failed");
+    // gfx950 only; no FMA contraction, no packed FP32.
+    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+                          "-munsafe-fp-atomics", "-Xclang", "-target-feature", "-Xclang",
+                          "-packed-fp32-ops"};
+    hiprtcResult r = hiprtcCompileProgram(prog, sizeof(opts) / sizeof(opts[0]), opts);
+    size_t log_size = 0;
+    hiprtcGetProgramLogSize(prog, &log_size);
+    std::string log(log_size, '\0');
+    if (log_size) hiprtcGetProgramLog(prog, &log[0]);
+    if (r != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        throw HipError("hiprtc compile failed: " + log + "\n--- source ---\n" + ks.code);
+    }
+    size_t code_size = 0;
+    hiprtcGetCodeSize(prog, &code_size);
+    std::vector<char> code(code_size);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    return code;
+}
+
+Module& jit_module(const KernelSource& ks) {
+    device();
+    const std::string src = full_source(ks);
+    std::lock_guard<std::mutex> lk(g_jit_mu);
+    auto it = g_modules.find(src);
+    if (it != g_modules.end()) return *it->second;
+    std::vector<char> code = jit_compile(ks);
+    auto m = std::make_unique<Module>();
+    RF_HIP(hipModuleLoadData(&m->mod, code.data()));
+    RF_HIP(hipModuleGetFunction(&m->fn, m->mod, ks.entry.c_str()));
+    int nb = 0;
+    RF_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, m->fn, ks.tune.block, 0));
+    m->max_blocks_per_cu = std::max(1, nb);
+    Module& ref = *m;
+    g_modules[src] = std::move(m);
+    return ref;
+}
+
+// ------------------------------------------------------- host evaluation
+
+namespace {
+
+struct EvalCtx {
+    const Plan* plan;
+    const rf_arg* args;
+    int64_t n_args;
+    std::map<std::string, HostVal> comps;  // component symbol -> value
+    std::map<std::string, N> tbl;
+};
+
+rf_table* arg_table(const EvalCtx& c, int i) {
+    if (i < 0 || i >= c.n_args || c.args[i].kind != 0 || !c.args[i].u.table)
+        throw IRError("argument " + std::to_string(i) + " must be a table");
+    return c.args[i].u.table;
+}
+
+HostVal hv_int(const Type& t, int64_t v) {
+    HostVal h;
+    h.type = t;
+    h.i = v;
+    return h;
+}
+HostVal hv_float(const Type& t, double v) {
+    HostVal h;
+    h.type = t;
+    h.is_float = true;
+    h.f = v;
+    return h;
+}
+
+int64_t wrap_int(const Type& t, int64_t v) {
+    switch (t.bt) {
+        case BT::BOOL: return v != 0;
+        case BT::INT8: return (int8_t)v;
+        case BT::UINT8: return (uint8_t)v;
+        case BT::INT16: return (int16_t)v;
+        case BT::UINT16: return (uint16_t)v;
+        case BT::INT32: return (int32_t)v;
+        case BT::UINT32: return (uint32_t)v;
+        default: return v;
+    }
+}
+
+HostVal convert(const HostVal& v, const Type& t) {
+    if (t.is_float()) {
+        double x = v.is_float ? v.f
+                              : (v.type.bt == BT::UINT64 ? (double)(uint64_t)v.i : (double)v.i);
+        if (t.bt == BT::FLOAT32) x = (float)x;
+        return hv_float(t, x);
+    }
+    int64_t x;
+    if (v.is_float) {
+        x = t.bt == BT::UINT64 ? (int64_t)(uint64_t)v.f : (int64_t)v.f;
+        if (t.bt == BT::BOOL) x = v.f != 0;
+    } else {
+        x = v.i;
+    }
+    return hv_int(t, wrap_int(t, x));
+}
+
+HostVal eval(const N& e, EvalCtx& c);
+
+HostVal eval_leaf(const N& e, EvalCtx& c) {
+    const std::string n = e->name;
+    if (n.rfind("$arg.", 0) == 0) {
+        int i = std::stoi(n.substr(5));
+        if (i >= c.n_args) throw IRError("missing argument " + std::to_string(i));
+        const rf_arg& a = c.args[i];
+        if (a.kind == 1) return convert(hv_int(types::I64, a.u.i64), e->type);
+        if (a.kind == 2) return convert(hv_float(types::F64, a.u.f64), e->type);
+        throw IRError("argument " + std::to_string(i) + " must be a scalar");
+    }
+    if (n.rfind("$len.", 0) == 0) {
+        auto dot = n.find('.', 5);
+        int t = std::stoi(n.substr(5, dot - 5));
+        int col = std::stoi(n.substr(dot + 1));
+        rf_table* tb = arg_table(c, t);
+        if (col >= (int)tb->cols.size()) throw IRError("length of a missing column");
+        return hv_int(types::IDX_T, tb->cols[col].length);
+    }
+    throw IRError("leaf " + n + " is not host-evaluable");
+}
+
+HostVal eval(const N& e, EvalCtx& c) {
+    switch (e->kind) {
+        case K::Const:
+            if (e->type.is_float()) return convert(hv_float(types::F64, e->val), e->type);
+            if (e->type.is_unsigned()) return hv_int(e->type, wrap_int(e->type, (int64_t)(uint64_t)e->val));
+            return hv_int(e->type, wrap_int(e->type, (int64_t)e->val));
+        case K::Sym: {
+            if (!leaf_name(e).empty()) return eval_leaf(e, c);
+            auto cm = c.comps.find(e->name);
+            if (cm != c.comps.end()) return cm->second;
+            for (size_t i = 0; i < c.plan->inputs.size(); i++) {
+                if (c.plan->inputs[i]->name == e->name) {
+                    if (e->type.is_vector()) throw IRError("vector input in a host expression");
+                    N l = mk_sym("$arg." + std::to_string(i), e->type);
+                    return eval_leaf(l, c);
+                }
+            }
+            auto t = c.tbl.find(e->name);
+            if (t != c.tbl.end()) return eval(t->second, c);
+            throw IRError("undefined symbol " + e->name);
+        }
+        case K::Length: {
+            N v = e->args[0];
+            while (v->kind == K::Sym) {
+                bool found = false;
+                for (size_t i = 0; i < c.plan->inputs.size(); i++)
+                    if (c.plan->inputs[i]->name == v->name) {
+                        rf_table* tb = arg_table(c, (int)i);
+                        if (e->idx >= (int64_t)tb->cols.size()) throw IRError("length column");
+                        return hv_int(types::IDX_T, tb->cols[e->idx].length);
+                    }
+                auto t = c.tbl.find(v->name);
+                if (t == c.tbl.end()) break;
+                v = t->second;
+                found = true;
+                (void)found;
+            }
+            throw Unsupported("length of a computed vector on the host");
+        }
+        case K::Cast:
+            return convert(eval(e->args[0], c), e->type);
+        case K::Get: {
+            HostVal v = eval(e->args[0], c);
+            if (!v.type.is_struct()) return v;
+            return v.fields.at(e->idx);
+        }
+        case K::New: {
+            HostVal h;
+            h.type = e->type;
+            for (auto& a : e->args) h.fields.push_back(eval(a, c));
+            return h;
+        }
+        case K::Select: {
+            HostVal cv = eval(e->args[0], c);
+            return (cv.is_float ? cv.f != 0 : cv.i != 0) ? eval(e->args[1], c)
+                                                         : eval(e->args[2], c);
+        }
+        case K::Nary: {
+            std::vector<HostVal> a;
+            for (auto& x : e->args) a.push_back(eval(x, c));
+            const Type& at = e->args[0]->type;
+            bool fl = at.is_float();
+            auto F = [&](int i) { return convert(a[i], types::F64).f; };
+            auto I = [&](int i) { return a[i].is_float ? (int64_t)a[i].f : a[i].i; };
+            auto B = [&](bool b) { return hv_int(types::BOOL_T, b ? 1 : 0); };
+            switch (e->op) {
+                case MathOp::ADD: return fl ? convert(hv_float(types::F64, F(0) + F(1)), e->type) : hv_int(e->type, wrap_int(e->type, (int64_t)((uint64_t)I(0) + (uint64_t)I(1))));
+                case MathOp::SUB: return fl ? convert(hv_float(types::F64, F(0) - F(1)), e->type) : hv_int(e->type, wrap_int(e->type, (int64_t)((uint64_t)I(0) - (uint64_t)I(1))));
+                case MathOp::MUL: return fl ? convert(hv_float(types::F64, F(0) * F(1)), e->type) : hv_int(e->type, wrap_int(e->type, (int64_t)((uint64_t)I(0) * (uint64_t)I(1))));
+                case MathOp::DIV:
+                    if (fl) return convert(hv_float(types::F64, F(0) / F(1)), e->type);
+                    if (I(1) == 0) throw IRError("integer division by zero");
+                    return hv_int(e->type, wrap_int(e->type, I(0) / I(1)));
+                case MathOp::MOD:
+                    if (I(1) == 0) throw IRError("integer modulo by zero");
+                    return hv_int(e->type, wrap_int(e->type, I(0) % I(1)));
+                case MathOp::MAX: return fl ? ((F(0) < F(1)) ? a[1] : a[0]) : ((I(0) < I(1)) ? a[1] : a[0]);
+                case MathOp::MIN: return fl ? ((F(1) < F(0)) ? a[1] : a[0]) : ((I(1) < I(0)) ? a[1] : a[0]);
+                case MathOp::ABS: return fl ? convert(hv_float(types::F64, std::fabs(F(0))), e->type) : hv_int(e->type, I(0) < 0 ? -I(0) : I(0));
+                case MathOp::NEG: return fl ? convert(hv_float(types::F64, -F(0)), e->type) : hv_int(e->type, wrap_int(e->type, -I(0)));
+                case MathOp::SQRT: return convert(hv_float(types::F64, std::sqrt(F(0))), e->type);
+                case MathOp::POW: return convert(hv_float(types::F64, std::pow(F(0), F(1))), e->type);
+                case MathOp::CEIL: return convert(hv_float(types::F64, std::ceil(F(0))), e->type);
+                case MathOp::FLOOR: return convert(hv_float(types::F64, std::floor(F(0))), e->type);
+                case MathOp::LT: return B(fl ? F(0) < F(1) : I(0) < I(1));
+                case MathOp::LTE: return B(fl ? F(0) <= F(1) : I(0) <= I(1));
+                case MathOp::GT: return B(fl ? F(0) > F(1) : I(0) > I(1));
+                case MathOp::GTE: return B(fl ? F(0) >= F(1) : I(0) >= I(1));
+                case MathOp::EQ: return B(fl ? F(0) == F(1) : I(0) == I(1));
+                case MathOp::NOT: return B(I(0) == 0);
+                case MathOp::AND: return B(I(0) != 0 && I(1) != 0);
+                case MathOp::OR: return B(I(0) != 0 || I(1) != 0);
+            }
+            break;
+        }
+        default:
+            break;
+    }
+    throw Unsupported("expression not host-evaluable: " + to_text(e));
+}
+
+int64_t eval_idx(const N& e, EvalCtx& c) {
+    HostVal v = eval(e, c);
+    return v.is_float ? (int64_t)v.f : v.i;
+}
+
+HostVal combine_host(AggOp op, const HostVal& a, const HostVal& b, const Type& t) {
+    bool fl = t.is_float();
+    HostVal r;
+    switch (op) {
+        case AggOp::ADD:
+            r = fl ? hv_float(t, a.f + b.f) : hv_int(t, (int64_t)((uint64_t)a.i + (uint64_t)b.i));
+            break;
+        case AggOp::MUL:
+            r = fl ? hv_float(t, a.f * b.f) : hv_int(t, (int64_t)((uint64_t)a.i * (uint64_t)b.i));
+            break;
+        case AggOp::MIN:
+            r = fl ? ((b.f < a.f) ? b : a) : ((b.i < a.i) ? b : a);
+            break;
+        case AggOp::MAX:
+            r = fl ? ((a.f < b.f) ? b : a) : ((a.i < b.i) ? b : a);
+            break;
+        case AggOp::AND: r = hv_int(t, (a.i != 0 && b.i != 0) ? 1 : 0); break;
+        case AggOp::OR: r = hv_int(t, (a.i != 0 || b.i != 0) ? 1 : 0); break;
+    }
+    if (fl && t.bt == BT::FLOAT32) r.f = (float)r.f;
+    if (!fl) r.i = wrap_int(t, r.i);
+    r.type = t;
+    return r;
+}
+
+HostVal read_prim(const Type& t, const char* p) {
+    switch (t.bt) {
+        case BT::BOOL: return hv_int(t, *(const uint8_t*)p != 0);
+        case BT::INT8: return hv_int(t, *(const int8_t*)p);
+        case BT::UINT8: return hv_int(t, *(const uint8_t*)p);
+        case BT::INT16: return hv_int(t, *(const int16_t*)p);
+        case BT::UINT16: return hv_int(t, *(const uint16_t*)p);
+        case BT::INT32: return hv_int(t, *(const int32_t*)p);
+        case BT::UINT32: return hv_int(t, *(const uint32_t*)p);
+        case BT::INT64: case BT::IDX: return hv_int(t, *(const int64_t*)p);
+        case BT::UINT64: return hv_int(t, (int64_t) * (const uint64_t*)p);
+        case BT::FLOAT32: return hv_float(t, *(const float*)p);
+        case BT::FLOAT64: return hv_float(t, *(const double*)p);
+        default: throw IRError("bad primitive");
+    }
+}
+
+void write_prim(const HostVal& v, const Type& t, char* p) {
+    HostVal c = convert(v, t);
+    switch (t.bt) {
+        case BT::BOOL: *(uint8_t*)p = c.i != 0; break;
+        case BT::INT8: *(int8_t*)p = (int8_t)c.i; break;
+        case BT::UINT8: *(uint8_t*)p = (uint8_t)c.i; break;
+        case BT::INT16: *(int16_t*)p = (int16_t)c.i; break;
+        case BT::UINT16: *(uint16_t*)p = (uint16_t)c.i; break;
+        case BT::INT32: *(int32_t*)p = (int32_t)c.i; break;
+        case BT::UINT32: *(uint32_t*)p = (uint32_t)c.i; break;
+        case BT::INT64: case BT::IDX: *(int64_t*)p = c.i; break;
+        case BT::UINT64: *(uint64_t*)p = (uint64_t)c.i; break;
+        case BT::FLOAT32: *(float*)p = (float)c.f; break;
+        case BT::FLOAT64: *(double*)p = c.f; break;
+        default: throw IRError("bad primitive");
+    }
+}
+
+void write_value(const HostVal& v, const Type& t, char* dst) {
+    if (t.is_struct()) {
+        size_t off = 0;
+        for (size_t i = 0; i < t.dtypes.size(); i++) {
+            size_t sz, al;
+            c_layout(t.dtypes[i], sz, al);
+            off = (off + al - 1) / al * al;
+            write_value(v.fields.at(i), t.dtypes[i], dst + off);
+            off += sz;
+        }
+        return;
+    }
+    if (t.is_vector()) {
+        *(void**)dst = v.table;
+        return;
+    }
+    write_prim(v, t, dst);
+}
+
+}  // namespace
+
+void c_layout(const Type& t, size_t& size, size_t& align) {
+    if (t.is_struct()) {
+        size_t off = 0, al = 1;
+        for (auto& d : t.dtypes) {
+            size_t s, a;
+            c_layout(d, s, a);
+            off = (off + a - 1) / a * a + s;
+            al = std::max(al, a);
+        }
+        size = (off + al - 1) / al * al;
+        align = al;
+        return;
+    }
+    if (t.is_vector() || t.bt == BT::PTR) {
+        size = align = 8;
+        return;
+    }
+    size = align = (size_t)t.width();
+}
+
+// ----------------------------------------------------------- execution
+
+static std::string null_signature(const Plan& plan, const rf_arg* args, int64_t n_args) {
+    std::string s;
+    for (size_t i = 0; i < plan.inputs.size(); i++) {
+        if (!plan.inputs[i]->type.is_vector()) continue;
+        if ((int64_t)i >= n_args || args[i].kind != 0 || !args[i].u.table)
+            throw IRError("argument " + std::to_string(i) + " must be a table");
+        const rf_table* t = args[i].u.table;
+        const Type& want = plan.inputs[i]->type;
+        Type have = t->type();
+        if (have != want)
+            throw IRError("argument " + std::to_string(i) + " has type " + have.str() +
+                          ", the function expects " + want.str());
+        for (auto& c : t->cols) s += c.validity ? '1' : '0';
+        s += '|';
+    }
+    return s;
+}
+
+static Spec& get_spec(rf_query* q, const rf_arg* args, int64_t n_args) {
+    std::string sig = null_signature(q->plan, args, n_args);
+    auto it = q->specs.find(sig);
+    if (it != q->specs.end()) return it->second;
+    std::vector<std::vector<bool>> nl(q->plan.inputs.size());
+    for (size_t i = 0; i < q->plan.inputs.size(); i++) {
+        if (!q->plan.inputs[i]->type.is_vector()) continue;
+        for (auto& c : args[i].u.table->cols) nl[i].push_back(c.validity != nullptr);
+    }
+    Spec sp;
+    sp.ks = gen_scan(q->plan, nl, q->tune, q->det);
+    sp.mod = &jit_module(sp.ks);
+    auto r = q->specs.emplace(sig, std::move(sp));
+    return r.first->second;
+}
+
+}  // namespace rf
+
+rf_query::~rf_query() {
+    rf::dev_free(partials);
+    rf::dev_free(result);
+    rf::dev_free(counters);
+    rf::dev_free(status);
+    rf::dev_free(out_count);
+    for (void* p : outs) rf::dev_free(p);
+    if (host_stage) (void)hipHostFree(host_stage);
+}
+
+namespace rf {
+
+void table_free(rf_table* t) {
+    if (!t) return;
+    for (void* p : t->owned) dev_free(p);
+    delete t;
+}
+
+// Status words carry a 22-bit launch epoch; they are cleared when it wraps
+// so that a word left by an old launch can never match.
+uint64_t next_epoch(rf_query* q) {
+    q->epoch++;
+    if ((q->epoch & ((1ull << 22) - 1)) == 0) {
+        dev_fill_u64(q->status, 0, (int64_t)q->status_tiles);
+        q->epoch++;
+    }
+    return q->epoch;
+}
+
+Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/) {
+    const Plan& plan = q->plan;
+    if (n_args != (int64_t)plan.inputs.size())
+        throw IRError("function '" + plan.func_name + "' takes " +
+                      std::to_string(plan.inputs.size()) + " arguments, got " +
+                      std::to_string(n_args));
+    for (size_t i = 0; i < plan.inputs.size(); i++) {
+        bool vec = plan.inputs[i]->type.is_vector();
+        if (vec != (args[i].kind == 0))
+            throw IRError("argument " + std::to_string(i) + (vec ? " must be a table" : " must be a scalar"));
+    }
+    Launch l;
+    l.spec = &get_spec(q, args, n_args);
+    const KernelSource& ks = l.spec->ks;
+    EvalCtx c{&plan, args, n_args, {}, {}};
+    int64_t lo = eval_idx(plan.start, c), end = eval_idx(plan.end, c);
+    if (plan.source == Plan::TABLE_ROWS) {
+        const rf_table* dt = args[plan.driver].u.table;
+        // Rows past the scanned table's end are never read (the reference
+        // does not check; reading them is undefined there).
+        int64_t n = dt->rows();
+        if (end > n) end = n;
+        if (lo < 0) lo = 0;
+    }
+    l.lo = lo;
+    l.end = end;
+    RfScanArgs& A = l.args;
+    std::memset(&A, 0, sizeof A);
+    const int64_t items = ks.tune.items;
+    const int64_t tile = (int64_t)ks.tune.sub * ks.tune.block * items;
+    int64_t base = lo >= 0 ? lo - (lo % items) : -(((-lo) + items - 1) / items) * items;
+    A.start = base;
+    A.lo = lo;
+    A.end = end;
+    A.ntiles = end > lo ? (end - base + tile - 1) / tile : 0;
+    l.empty = A.ntiles == 0;
+    // Columns.
+    for (size_t s = 0; s < ks.cols.size(); s++) {
+        const ColSlot& cs = ks.cols[s];
+        const rf_table* t = args[cs.table].u.table;
+        const Column& col = t->cols.at(cs.col);
+        A.col[s] = col.values;
+        A.valid[s] = (const uint16_t*)col.validity;
+    }
+    for (size_t j = 0; j < plan.probes.size(); j++) {
+        rf_table* t = args[plan.probes[j].table].u.table;
+        if (!t->has_index) table_build_index(t);
+        if ((int)j >= RF_MAXPROBES) throw Unsupported("more than 4 probed tables");
+        A.probe[j] = probe_of(t);
+    }
+    // Parameters.
+    for (size_t i = 0; i < ks.params.size(); i++) {
+        const ParamSlot& p = ks.params[i];
+        HostVal v;
+        if (p.kind == ParamSlot::CONST) {
+            v = eval(mk_const(p.type, p.value), c);
+        } else if (p.kind == ParamSlot::ARG) {
+            v = eval_leaf(mk_sym("$arg." + std::to_string(p.arg), p.type), c);
+        } else {
+            v = eval_leaf(mk_sym("$len." + std::to_string(p.arg) + "." + std::to_string(p.col),
+                                 types::IDX_T), c);
+        }
+        if (p.type.is_float()) A.p[i].f = convert(v, types::F64).f;
+        else A.p[i].i = convert(v, types::I64).i;
+    }
+    // Workspace.
+    Device& d = device();
+    const Module& m = *l.spec->mod;
+    int64_t grid = std::min<int64_t>(A.ntiles, (int64_t)d.num_cus * m.max_blocks_per_cu);
+    if (grid < 1) grid = 1;
+    l.grid = (int)grid;
+    size_t nparts = ks.det ? (size_t)std::max<int64_t>(A.ntiles, 1) : (size_t)grid;
+    size_t need = std::max<size_t>(nparts * std::max(ks.acc_bytes, 8), 256);
+    if (need > q->partials_bytes) {
+        dev_free(q->partials);
+        q->partials = dev_alloc(need);
+        q->partials_bytes = need;
+    }
+    if (!q->result) q->result = dev_alloc(4096);
+    if (!q->counters) {
+        q->counters = dev_alloc(256);
+        dev_fill_u64(q->counters, 0, 32);
+    }
+    if (!q->out_count) q->out_count = dev_alloc(256);
+    if (!q->host_stage) RF_HIP(hipHostMalloc(&q->host_stage, 8192, hipHostMallocDefault));
+    if (ks.has_sel) {
+        if ((size_t)A.ntiles > q->status_tiles) {
+            dev_free(q->status);
+            q->status_tiles = std::max<size_t>(A.ntiles, 1024);
+            q->status = dev_alloc(q->status_tiles * 8);
+            dev_fill_u64(q->status, 0, (int64_t)q->status_tiles);
+            q->epoch = 0;
+        }
+        size_t rows = (size_t)std::max<int64_t>(end - lo, 1);
+        if (rows > q->out_rows_cap || q->outs.size() != ks.out_types.size()) {
+            for (void* p : q->outs) dev_free(p);
+            q->outs.clear();
+            for (auto& t : ks.out_types) q->outs.push_back(dev_alloc(rows * t.width()));
+            q->out_rows_cap = rows;
+        }
+        for (size_t j = 0; j < q->outs.size(); j++) A.out[j] = q->outs[j];
+        A.tile_status = (uint64_t*)q->status;
+        A.epoch = next_epoch(q);
+    }
+    A.partials = q->partials;
+    A.result = q->result;
+    A.counters = (uint32_t*)q->counters;
+    A.out_count = (int64_t*)q->out_count;
+    A.spin_limit = 1u << 24;
+    return l;
+}
+
+void launch(rf_query* /*q*/, Launch& l, hipStream_t s) {
+    if (l.empty) return;
+    void* params[] = {&l.args};
+    RF_HIP(hipModuleLaunchKernel(l.spec->mod->fn, l.grid, 1, 1, l.spec->ks.tune.block, 1, 1, 0,
+                                 s, params, nullptr));
+}
+
+static HostVal finish_reduce(const Plan& plan, const KernelSource& ks, size_t comp,
+                             const char* acc_bytes, bool empty, EvalCtx& c) {
+    const ReduceComp& rc = plan.reduces[comp];
+    std::vector<HostVal> fields;
+    for (size_t f = 0; f < rc.fields.size(); f++) {
+        const Field& fl = rc.fields[f];
+        HostVal v = convert(eval(fl.init, c), fl.type);
+        if (fl.updated && !empty) {
+            for (size_t a = 0; a < ks.acc.size(); a++) {
+                if (ks.acc[a].comp == (int)comp && ks.acc[a].field == (int)f) {
+                    HostVal dv = read_prim(fl.type, acc_bytes + ks.acc_offset[a]);
+                    v = combine_host(fl.op, v, dv, fl.type);
+                }
+            }
+        }
+        fields.push_back(v);
+    }
+    // Re-nest into the state struct.
+    std::function<HostVal(const Type&, size_t&)> nest = [&](const Type& t, size_t& k) {
+        if (t.is_struct()) {
+            HostVal h;
+            h.type = t;
+            for (auto& d : t.dtypes) h.fields.push_back(nest(d, k));
+            return h;
+        }
+        return fields.at(k++);
+    };
+    size_t k = 0;
+    return nest(rc.state_type, k);
+}
+
+void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_addr,
+               rf_table** out_table) {
+    std::lock_guard<std::mutex> lk(q->mu);
+    Device& d = device();
+    Launch l = prepare(q, args, n_args, nullptr);
+    const KernelSource& ks = l.spec->ks;
+    launch(q, l, d.stream);
+    char* stage = (char*)q->host_stage;
+    if (!l.empty) {
+        if (ks.acc_bytes)
+            RF_HIP(hipMemcpyAsync(stage, q->result, ks.acc_bytes, hipMemcpyDeviceToHost, d.stream));
+        if (ks.has_sel) {
+            RF_HIP(hipMemcpyAsync(stage + 4096, q->out_count, 8, hipMemcpyDeviceToHost, d.stream));
+            RF_HIP(hipMemcpyAsync(stage + 4104, (char*)q->counters + 8, 4, hipMemcpyDeviceToHost,
+                                  d.stream));
+        }
+    }
+    RF_HIP(hipStreamSynchronize(d.stream));
+    if (!l.empty && ks.has_sel && *(uint32_t*)(stage + 4104) != 0) {
+        dev_fill_u64(q->counters, 0, 32);
+        RF_HIP(hipStreamSynchronize(d.stream));
+        throw HipError("selection look-back exceeded its wait bound; the run was abandoned");
+    }
+    const Plan& plan = q->plan;
+    EvalCtx c{&plan, args, n_args, {}, {}};
+    for (auto& kv : plan.tbl) c.tbl[kv.first->name] = kv.second;
+    for (size_t r = 0; r < plan.reduces.size(); r++)
+        c.comps[plan.reduces[r].sym_name] = finish_reduce(plan, ks, r, stage, l.empty, c);
+    rf_table* sel = nullptr;
+    if (!plan.ops.empty()) {
+        int64_t count = l.empty ? 0 : *(int64_t*)(stage + 4096);
+        const OpComp& oc = plan.ops[0];
+        auto t = std::make_unique<rf_table>();
+        t->dim = oc.n_iters;
+        size_t total = 0;
+        std::vector<size_t> offs;
+        for (auto& ty : ks.out_types) {
+            offs.push_back(total);
+            total += ((size_t)std::max<int64_t>(count, 1) * ty.width() + 255) & ~size_t(255);
+        }
+        char* arena = (char*)dev_alloc(total);
+        t->owned.push_back(arena);
+        for (size_t j = 0; j < ks.out_types.size(); j++) {
+            Column col;
+            col.type = ks.out_types[j];
+            col.format = "";
+            switch (col.type.bt) {
+                case BT::BOOL: case BT::INT8: col.format = "c"; break;
+                case BT::UINT8: col.format = "C"; break;
+                case BT::INT16: col.format = "s"; break;
+                case BT::UINT16: col.format = "S"; break;
+                case BT::INT32: col.format = "i"; break;
+                case BT::UINT32: col.format = "I"; break;
+                case BT::INT64: case BT::IDX: col.format = "l"; col.type = types::I64; break;
+                case BT::UINT64: col.format = "L"; break;
+                case BT::FLOAT32: col.format = "f"; break;
+                case BT::FLOAT64: col.format = "g"; break;
+                default: throw IRError("output column type");
+            }
+            if (ks.out_types[j].bt == BT::BOOL) col.type = types::I8;
+            col.length = col.logical_length = count;
+            col.values = arena + offs[j];
+            if (count > 0)
+                RF_HIP(hipMemcpyAsync(col.values, q->outs[j], (size_t)count * ks.out_types[j].width(),
+                                      hipMemcpyDeviceToDevice, d.stream));
+            t->cols.push_back(col);
+        }
+        RF_HIP(hipStreamSynchronize(d.stream));
+        sel = t.release();
+        HostVal hv;
+        hv.type = oc.vec_type;
+        hv.table = sel;
+        c.comps[oc.sym_name] = hv;
+    }
+    // Func output: evaluated over the component results (a struct output
+    // that holds an Op result carries the table handle in its field).
+    HostVal out = eval(plan.output, c);
+    if (plan.output_type.is_vector()) {
+        if (!out_table) {
+            if (sel) table_free(sel);
+            throw IRError("an Op output needs out_table");
+        }
+        *out_table = (rf_table*)out.table;
+        if (output_addr) *(void**)output_addr = out.table;
+        return;
+    }
+    if (output_addr) write_value(out, plan.output_type, (char*)output_addr);
+    if (out_table) *out_table = sel;
+}
+
+double time_query(rf_query* q, const rf_arg* args, int64_t n_args, int iters) {
+    std::lock_guard<std::mutex> lk(q->mu);
+    Device& d = device();
+    Launch l = prepare(q, args, n_args, nullptr);
+    if (l.empty) return 0.0;
+    hipEvent_t e0, e1;
+    RF_HIP(hipEventCreate(&e0));
+    RF_HIP(hipEventCreate(&e1));
+    launch(q, l, d.stream);  // warm
+    RF_HIP(hipEventRecord(e0, d.stream));
+    for (int i = 0; i < iters; i++) {
+        // Every launch needs a fresh epoch for the look-back status words.
+        if (l.spec->ks.has_sel) l.args.epoch = next_epoch(q);
+        launch(q, l, d.stream);
+    }
+    RF_HIP(hipEventRecord(e1, d.stream));
+    RF_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    RF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return iters > 0 ? ms / iters : 0.0;
+}
+
+}  // namespace rf

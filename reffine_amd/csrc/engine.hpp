@@ -1,0 +1,162 @@
+// HIP runtime layer: device context, device-resident Arrow tables, the
+// hiprtc module cache and query execution. Replaces the reference's
+// CudaEngine (src/engine/cuda_engine.cpp:25-87) and the host-side
+// ArrowTable2 / memman output registry (include/reffine/arrow/table.h,
+// src/engine/memory.cpp:5-16).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/reffine_hip.h"
+#include "hipgen.hpp"
+#include "kernels/rf_abi.h"
+#include "lower.hpp"
+
+namespace rf {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct ArrowError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct NoMem : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what);
+#define RF_HIP(x) ::rf::hip_check((x), #x)
+
+struct Device {
+    int id = -1;
+    hipStream_t stream = nullptr;
+    int num_cus = 0;
+    std::string arch;
+    std::mutex mu;
+};
+
+// Ahead-of-time kernels (kernels/rf_aot.hip).
+hipError_t aot_fill_u64(void* p, uint64_t v, int64_t n, hipStream_t s);
+hipError_t aot_index_build(const int64_t* keys, const uint16_t* valid, int64_t n,
+                           void* slot_keys, void* slot_rows, uint64_t mask, void* min_key_row,
+                           hipStream_t s);
+hipError_t aot_index_lookup(const RfProbeDev& h, const int64_t* keys, int64_t n, int64_t* rows,
+                            hipStream_t s);
+hipError_t aot_copy(const void* src, void* dst, int64_t n16, hipStream_t s);
+
+Device& device();          // initialised device (throws if rf_init not called)
+Device& init_device(int id);
+
+void* dev_alloc(size_t bytes);  // 256-byte aligned, throws NoMem
+void dev_free(void* p);
+void dev_fill_u64(void* p, uint64_t v, int64_t n);  // kernel fill on the stream
+
+struct Column {
+    std::string format;
+    Type type;
+    Enc enc = Enc::FLAT;
+    int64_t length = 0;       // rows (REE: number of runs)
+    int64_t null_count = 0;
+    void* values = nullptr;   // device
+    void* validity = nullptr; // device bitmap (uint16 words, LSB first) or null
+    void* run_ends = nullptr; // REE: device int32 run ends
+    int64_t logical_length = 0;
+};
+
+struct HashIndex {
+    void* keys = nullptr;
+    void* rows = nullptr;
+    uint64_t mask = 0;
+    int64_t min_key_row = -1;
+};
+
+}  // namespace rf
+
+struct rf_table {
+    int64_t dim = 1;
+    std::vector<rf::Column> cols;
+    std::vector<void*> owned;  // device allocations freed with the table
+    rf::HashIndex index;
+    bool has_index = false;
+    rf::Type type() const;
+    int64_t rows() const { return cols.empty() ? 0 : cols[0].length; }
+};
+
+namespace rf {
+
+struct Module {
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+    int max_blocks_per_cu = 1;
+};
+
+// Compiles (or fetches from the cache) the specialised scan kernel.
+Module& jit_module(const KernelSource& ks);
+// hiprtc compile for gfx950 only (no device needed); returns the code object.
+std::vector<char> jit_compile(const KernelSource& ks);
+
+struct Spec {
+    KernelSource ks;
+    Module* mod = nullptr;
+};
+
+}  // namespace rf
+
+struct rf_query {
+    rf::Plan plan;
+    bool det = false;
+    rf::Tuning tune;
+    std::map<std::string, rf::Spec> specs;  // by nullability signature
+    // Workspace (grow-only, reused across runs).
+    void* partials = nullptr;
+    size_t partials_bytes = 0;
+    void* result = nullptr;
+    void* counters = nullptr;
+    void* status = nullptr;
+    size_t status_tiles = 0;
+    void* out_count = nullptr;
+    std::vector<void*> outs;
+    size_t out_rows_cap = 0;
+    uint64_t epoch = 0;
+    void* host_stage = nullptr;  // pinned: result Acc + count
+    std::mutex mu;
+    ~rf_query();
+};
+
+namespace rf {
+
+rf_table* table_upload(const ArrowSchema* schema, const ArrowArray* array, int64_t dim);
+rf_table* table_wrap(int64_t dim, int64_t n_cols, int64_t n_rows, const char* const* formats,
+                     void* const* values, void* const* validity);
+void table_build_index(rf_table* t);
+void table_free(rf_table* t);
+RfProbeDev probe_of(const rf_table* t);
+uint64_t next_epoch(rf_query* q);
+
+struct Launch {
+    Spec* spec = nullptr;
+    RfScanArgs args{};
+    int grid = 0;
+    bool empty = false;
+    int64_t lo = 0, end = 0;
+};
+
+// Prepares arguments and workspace for one run (no launch).
+Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* ok);
+void launch(rf_query* q, Launch& l, hipStream_t s);
+// Launch + download + host-side finish; returns the output.
+void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_addr,
+               rf_table** out_table);
+double time_query(rf_query* q, const rf_arg* args, int64_t n_args, int iters);
+
+// Output struct layout (natural alignment, ScalarPass field order).
+void c_layout(const Type& t, size_t& size, size_t& align);
+
+}  // namespace rf

@@ -1,0 +1,425 @@
+// Row-code emitter for the scan skeleton (see hipgen.hpp).
+#include "hipgen.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <set>
+#include <sstream>
+
+namespace rf {
+
+namespace {
+
+std::string cty(const Type& t) { return t.cstr(); }
+
+std::vector<std::string> split(const std::string& s, char d) {
+    std::vector<std::string> out;
+    std::string cur;
+    for (char c : s) {
+        if (c == d) {
+            out.push_back(cur);
+            cur.clear();
+        } else {
+            cur += c;
+        }
+    }
+    out.push_back(cur);
+    return out;
+}
+
+enum class Mode { ROWS, EMIT };
+
+struct Gen {
+    const Plan& plan;
+    const std::vector<std::vector<bool>>& nullable;
+    KernelSource& ks;
+    std::map<std::string, int> param_ix;
+    std::map<std::pair<int, int>, int> col_ix;
+
+    Gen(const Plan& p, const std::vector<std::vector<bool>>& nl, KernelSource& k)
+        : plan(p), nullable(nl), ks(k) {}
+
+    bool is_nullable(int t, int c) const {
+        return t < (int)nullable.size() && c < (int)nullable[t].size() && nullable[t][c];
+    }
+
+    int col_slot(int t, int c, bool vec) {
+        auto key = std::make_pair(t, c);
+        auto it = col_ix.find(key);
+        if (it != col_ix.end()) {
+            if (vec) ks.cols[it->second].vector = true;
+            return it->second;
+        }
+        if ((int)ks.cols.size() >= 32) throw Unsupported("query reads more than 32 columns");
+        ColSlot s;
+        s.table = t;
+        s.col = c;
+        s.type = plan.inputs[t]->type.dtypes[c];
+        s.nullable = is_nullable(t, c);
+        s.vector = vec;
+        ks.cols.push_back(s);
+        int ix = (int)ks.cols.size() - 1;
+        col_ix[key] = ix;
+        return ix;
+    }
+
+    std::string param(const ParamSlot& p, const std::string& key) {
+        int ix;
+        auto it = param_ix.find(key);
+        if (it != param_ix.end()) {
+            ix = it->second;
+        } else {
+            if ((int)ks.params.size() >= 96) throw Unsupported("more than 96 constants");
+            ks.params.push_back(p);
+            ix = (int)ks.params.size() - 1;
+            param_ix[key] = ix;
+        }
+        const Type& t = p.type;
+        std::string s = "L.p[" + std::to_string(ix) + "]";
+        if (t.bt == BT::BOOL) return "(" + s + ".i != 0)";
+        if (t.is_float()) return "((" + cty(t) + ")" + s + ".f)";
+        return "((" + cty(t) + ")" + s + ".i)";
+    }
+
+    std::string const_param(const N& c) {
+        char buf[96];
+        std::snprintf(buf, sizeof buf, "c:%s:%a", c->type.str().c_str(), c->val);
+        ParamSlot p{ParamSlot::CONST, c->type, c->val, 0, 0};
+        return param(p, buf);
+    }
+};
+
+// Expression emitter with structural CSE into `const auto` temporaries.
+struct Emitter {
+    Gen& g;
+    Mode mode;
+    std::ostringstream& os;
+    std::string indent;
+    std::map<std::string, std::string> memo;
+    std::set<int> probes_done;
+    int ntmp = 0;
+
+    Emitter(Gen& gen, Mode m, std::ostringstream& o, std::string ind)
+        : g(gen), mode(m), os(o), indent(std::move(ind)) {}
+
+    std::string tmp(const std::string& rhs) {
+        std::string v = "t" + std::to_string(ntmp++);
+        os << indent << "const auto " << v << " = " << rhs << ";\n";
+        return v;
+    }
+
+    std::string probe(int j) {
+        std::string h = "h" + std::to_string(j);
+        if (!probes_done.count(j)) {
+            std::string k = expr(g.plan.probes[j].key);
+            os << indent << "const int64_t " << h << " = rf_probe(L.probe[" << j
+               << "], (int64_t)" << k << ");\n";
+            probes_done.insert(j);
+        }
+        return h;
+    }
+
+    std::string leaf_text(const N& e) {
+        const std::string name = leaf_name(e);
+        auto parts = split(name, '.');
+        const std::string& kind = parts[0];
+        if (kind == "$col") {
+            int t = std::stoi(parts[1]), c = std::stoi(parts[2]);
+            if (t != g.plan.driver) throw IRError("column leaf of a non-scanned table");
+            int s = g.col_slot(t, c, mode == Mode::ROWS);
+            const std::string T = cty(g.ks.cols[s].type);
+            if (mode == Mode::ROWS) return "c" + std::to_string(s) + ".v[k]";
+            return "((const " + T + "*)L.col[" + std::to_string(s) + "])[row]";
+        }
+        if (kind == "$valid") {
+            int t = std::stoi(parts[1]), c = std::stoi(parts[2]);
+            if (t != g.plan.driver) throw IRError("validity leaf of a non-scanned table");
+            if (!g.is_nullable(t, c)) return "true";
+            int s = g.col_slot(t, c, mode == Mode::ROWS);
+            if (mode == Mode::ROWS) return "(((vb" + std::to_string(s) + " >> k) & 1u) != 0u)";
+            return "rf_valid_bit(L.valid[" + std::to_string(s) + "], row)";
+        }
+        if (kind == "$pcol") {
+            int j = std::stoi(parts[1]), c = std::stoi(parts[2]);
+            int t = g.plan.probes[j].table;
+            int s = g.col_slot(t, c, false);
+            std::string h = probe(j);
+            const std::string T = cty(g.ks.cols[s].type);
+            return tmp("(" + h + " >= 0 ? ((const " + T + "*)L.col[" + std::to_string(s) +
+                       "])[" + h + "] : (" + T + ")0)");
+        }
+        if (kind == "$found") {
+            int j = std::stoi(parts[1]);
+            return "(" + probe(j) + " >= 0)";
+        }
+        if (kind == "$row") return "row";
+        if (kind == "$arg") {
+            ParamSlot p{ParamSlot::ARG, e->type, 0, std::stoi(parts[1]), 0};
+            return g.param(p, name + ":" + e->type.str());
+        }
+        if (kind == "$len") {
+            ParamSlot p{ParamSlot::LEN, e->type, 0, std::stoi(parts[1]), std::stoi(parts[2])};
+            return g.param(p, name);
+        }
+        throw IRError("unexpected leaf " + name + " in row code");
+    }
+
+    std::string expr(const N& e) {
+        if (e->kind == K::Sym) {
+            auto it = memo.find(e->name);
+            if (it != memo.end()) return it->second;
+            std::string r = leaf_text(e);
+            memo[e->name] = r;
+            return r;
+        }
+        if (e->kind == K::Const) return g.const_param(e);
+        std::string key = to_text(e);
+        auto it = memo.find(key);
+        if (it != memo.end()) return it->second;
+        std::string r;
+        switch (e->kind) {
+            case K::Cast:
+                r = tmp("(" + cty(e->type) + ") " + expr(e->args[0]));
+                break;
+            case K::Select:
+                r = tmp("(" + expr(e->args[0]) + " ? " + expr(e->args[1]) + " : " +
+                        expr(e->args[2]) + ")");
+                break;
+            case K::Nary:
+                r = tmp(nary(e));
+                break;
+            default:
+                throw Unsupported("node in row code: " + key);
+        }
+        memo[key] = r;
+        return r;
+    }
+
+    std::string nary(const N& e) {
+        auto a = [&](int i) { return expr(e->args[i]); };
+        const std::string T = cty(e->type);
+        switch (e->op) {
+            case MathOp::ADD: return "(" + a(0) + " + " + a(1) + ")";
+            case MathOp::SUB: return "(" + a(0) + " - " + a(1) + ")";
+            case MathOp::MUL: return "(" + a(0) + " * " + a(1) + ")";
+            case MathOp::DIV: return "(" + a(0) + " / " + a(1) + ")";
+            case MathOp::MOD: return "(" + a(0) + " % " + a(1) + ")";
+            case MathOp::MAX:
+                return "rf_max<" + T + ">((" + T + ")" + a(0) + ", (" + T + ")" + a(1) + ")";
+            case MathOp::MIN:
+                return "rf_min<" + T + ">((" + T + ")" + a(0) + ", (" + T + ")" + a(1) + ")";
+            case MathOp::ABS: return "rf_abs<" + T + ">((" + T + ")" + a(0) + ")";
+            case MathOp::NEG: return "(-" + a(0) + ")";
+            case MathOp::SQRT: return "sqrt(" + a(0) + ")";
+            case MathOp::POW: return "pow(" + a(0) + ", " + a(1) + ")";
+            case MathOp::CEIL: return "ceil(" + a(0) + ")";
+            case MathOp::FLOOR: return "floor(" + a(0) + ")";
+            case MathOp::LT: return "(" + a(0) + " < " + a(1) + ")";
+            case MathOp::LTE: return "(" + a(0) + " <= " + a(1) + ")";
+            case MathOp::GT: return "(" + a(0) + " > " + a(1) + ")";
+            case MathOp::GTE: return "(" + a(0) + " >= " + a(1) + ")";
+            case MathOp::EQ: return "(" + a(0) + " == " + a(1) + ")";
+            case MathOp::NOT: return "(!" + a(0) + ")";
+            case MathOp::AND: return "(" + a(0) + " && " + a(1) + ")";
+            case MathOp::OR: return "(" + a(0) + " || " + a(1) + ")";
+        }
+        throw IRError("bad operator");
+    }
+};
+
+std::string combine_expr(AggOp op, const Type& t, const std::string& a,
+                         const std::string& b) {
+    const std::string T = cty(t);
+    switch (op) {
+        case AggOp::ADD: return "(" + a + " + " + b + ")";
+        case AggOp::MUL: return "(" + a + " * " + b + ")";
+        case AggOp::MIN: return "rf_min<" + T + ">(" + a + ", (" + T + ")" + b + ")";
+        case AggOp::MAX: return "rf_max<" + T + ">(" + a + ", (" + T + ")" + b + ")";
+        case AggOp::AND: return "(" + a + " && " + b + ")";
+        case AggOp::OR: return "(" + a + " || " + b + ")";
+    }
+    return "?";
+}
+
+}  // namespace
+
+std::string fold_identity(AggOp op, const Type& t) {
+    const std::string T = t.cstr();
+    switch (op) {
+        case AggOp::ADD: return "(" + T + ")0";
+        case AggOp::MUL: return "(" + T + ")1";
+        case AggOp::AND: return "true";
+        case AggOp::OR: return "false";
+        case AggOp::MIN:
+        case AggOp::MAX: {
+            bool mx = op == AggOp::MAX;  // identity of max is the lowest value
+            if (t.is_float()) return mx ? "(" + T + ")(-__builtin_inf())" : "(" + T + ")__builtin_inf()";
+            switch (t.bt) {
+                case BT::BOOL: return mx ? "false" : "true";
+                case BT::INT8: return mx ? "(int8_t)(-128)" : "(int8_t)127";
+                case BT::INT16: return mx ? "(int16_t)(-32768)" : "(int16_t)32767";
+                case BT::INT32: return mx ? "(int32_t)(-2147483647 - 1)" : "(int32_t)2147483647";
+                case BT::INT64: case BT::IDX:
+                    return mx ? "(int64_t)(-9223372036854775807LL - 1)" : "(int64_t)9223372036854775807LL";
+                case BT::UINT8: return mx ? "(uint8_t)0" : "(uint8_t)255";
+                case BT::UINT16: return mx ? "(uint16_t)0" : "(uint16_t)65535";
+                case BT::UINT32: return mx ? "(uint32_t)0" : "(uint32_t)4294967295u";
+                case BT::UINT64: return mx ? "(uint64_t)0" : "(uint64_t)18446744073709551615ull";
+                default: break;
+            }
+        }
+    }
+    throw IRError("no identity for fold");
+}
+
+static bool collect_driver_cols(const N& e, const Plan& plan, std::set<int>& widths) {
+    if (!e) return false;
+    if (is_leaf(e, "$col.")) {
+        auto parts = split(leaf_name(e), '.');
+        if (std::stoi(parts[1]) == plan.driver) widths.insert(e->type.width());
+    }
+    for (auto& a : e->args) collect_driver_cols(a, plan, widths);
+    return true;
+}
+
+Tuning default_tuning(const Plan& plan) {
+    std::set<int> widths;
+    for (auto& r : plan.reduces)
+        for (auto& f : r.fields)
+            if (f.updated) {
+                collect_driver_cols(f.pred, plan, widths);
+                collect_driver_cols(f.contrib, plan, widths);
+            }
+    for (auto& o : plan.ops) collect_driver_cols(o.pred, plan, widths);
+    Tuning t;
+    int wmin = widths.empty() ? 8 : *widths.begin();
+    t.items = wmin >= 8 ? 2 : (wmin >= 4 ? 4 : 8);
+    if (plan.source == Plan::RANGE) t.items = 4;
+    // Tile of about 4096 rows; the selection scan packs SUB 16-bit counters.
+    t.sub = std::max(1, 4096 / (t.block * t.items));
+    if (!plan.ops.empty()) t.sub = std::min(t.sub, 4);
+    return t;
+}
+
+KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nullable,
+                      const Tuning& tune, bool det) {
+    KernelSource ks;
+    ks.tune = tune;
+    ks.det = det;
+    Gen g(plan, nullable, ks);
+    if (tune.block % 64 != 0 || tune.block > 1024) throw IRError("block size");
+    if (16 % tune.items != 0) throw IRError("ITEMS must divide 16");
+    ks.has_sel = !plan.ops.empty();
+    if (ks.has_sel && tune.sub > 4) throw IRError("SUB > 4 with a selection");
+
+    // Accumulator layout: one field per updated state field.
+    for (size_t c = 0; c < plan.reduces.size(); c++)
+        for (size_t f = 0; f < plan.reduces[c].fields.size(); f++) {
+            const Field& fl = plan.reduces[c].fields[f];
+            if (!fl.updated) continue;
+            ks.acc.push_back({(int)c, (int)f, fl.type, fl.op});
+        }
+    // Struct layout with natural alignment.
+    int off = 0, maxal = 1;
+    for (auto& a : ks.acc) {
+        int w = a.type.width();
+        off = (off + w - 1) / w * w;
+        ks.acc_offset.push_back(off);
+        off += w;
+        maxal = std::max(maxal, w);
+    }
+    ks.acc_bytes = ks.acc.empty() ? 0 : (off + maxal - 1) / maxal * maxal;
+
+    // ---- rows(): row code
+    std::ostringstream body;
+    Emitter em(g, Mode::ROWS, body, "            ");
+    std::vector<std::string> acc_pred, acc_contrib;
+    for (auto& a : ks.acc) {
+        const Field& fl = plan.reduces[a.comp].fields[a.field];
+        acc_pred.push_back(em.expr(fl.pred));
+        acc_contrib.push_back(em.expr(fl.contrib));
+    }
+    std::string sel_pred;
+    if (ks.has_sel) sel_pred = em.expr(plan.ops[0].pred);
+
+    std::ostringstream o;
+    o << "// Generated row code for reffine function '" << plan.func_name << "'.\n";
+    o << "struct Q {\n";
+    o << "    enum { BLOCK = " << tune.block << ", ITEMS = " << tune.items
+      << ", SUB = " << tune.sub << ", NACC = " << ks.acc.size()
+      << ", HAS_SEL = " << (ks.has_sel ? 1 : 0) << ", DET = " << (det ? 1 : 0) << " };\n";
+    o << "    struct Acc {";
+    for (size_t i = 0; i < ks.acc.size(); i++) o << " " << cty(ks.acc[i].type) << " a" << i << ";";
+    if (ks.acc.empty()) o << " int unused;";
+    o << " };\n";
+    o << "    static __device__ __forceinline__ void acc_zero(Acc& a) {";
+    for (size_t i = 0; i < ks.acc.size(); i++)
+        o << " a.a" << i << " = " << fold_identity(ks.acc[i].op, ks.acc[i].type) << ";";
+    o << " }\n";
+    o << "    static __device__ __forceinline__ void acc_combine(Acc& a, const Acc& b) {";
+    for (size_t i = 0; i < ks.acc.size(); i++) {
+        std::string ai = "a.a" + std::to_string(i), bi = "b.a" + std::to_string(i);
+        o << " " << ai << " = " << combine_expr(ks.acc[i].op, ks.acc[i].type, ai, bi) << ";";
+    }
+    o << " }\n";
+    o << "    static __device__ __forceinline__ void acc_shfl_xor(Acc& a, int o) {\n"
+      << "        Acc b;";
+    for (size_t i = 0; i < ks.acc.size(); i++)
+        o << " b.a" << i << " = rf_shfl_xor(a.a" << i << ", o);";
+    o << "\n        acc_combine(a, b);\n    }\n";
+
+    o << "    template <bool FULL>\n"
+      << "    static __device__ __forceinline__ uint32_t rows(Acc& a, const RfScanArgs& L, int64_t r) {\n";
+    for (size_t s = 0; s < ks.cols.size(); s++) {
+        const ColSlot& c = ks.cols[s];
+        if (!c.vector) continue;
+        const std::string T = cty(c.type);
+        o << "        const RfVec<" << T << ", ITEMS> c" << s << " = FULL ? rf_ldv<" << T
+          << ", ITEMS>((const " << T << "*)L.col[" << s << "] + r) : rf_ldv_tail<" << T
+          << ", ITEMS>((const " << T << "*)L.col[" << s << "], r, L.end);\n";
+        if (c.nullable)
+            o << "        const uint32_t vb" << s << " = rf_valid_bits<ITEMS>(L.valid[" << s
+              << "], r);\n";
+    }
+    o << "        uint32_t sel = 0;\n";
+    o << "#pragma unroll\n        for (int k = 0; k < ITEMS; k++) {\n";
+    o << "            const int64_t row = r + k;\n";
+    o << "            const bool inb = FULL || (row >= L.lo && row < L.end);\n";
+    o << "            (void)row;\n";
+    o << body.str();
+    for (size_t i = 0; i < ks.acc.size(); i++) {
+        std::string ai = "a.a" + std::to_string(i);
+        o << "            " << ai << " = (inb && " << acc_pred[i] << ") ? "
+          << combine_expr(ks.acc[i].op, ks.acc[i].type, ai, acc_contrib[i]) << " : " << ai
+          << ";\n";
+    }
+    if (ks.has_sel) o << "            sel |= (uint32_t)(inb && " << sel_pred << ") << k;\n";
+    o << "        }\n        return sel;\n    }\n";
+
+    // ---- emit(): selection outputs of one qualifying row
+    o << "    static __device__ __forceinline__ void emit(const RfScanArgs& L, int64_t row, int64_t pos) {\n";
+    if (ks.has_sel) {
+        std::ostringstream eb;
+        Emitter ee(g, Mode::EMIT, eb, "        ");
+        std::vector<std::string> vals;
+        for (auto& v : plan.ops[0].values) vals.push_back(ee.expr(v));
+        o << eb.str();
+        for (size_t j = 0; j < vals.size(); j++) {
+            const std::string T = cty(plan.ops[0].types[j]);
+            o << "        ((" << T << "*)L.out[" << j << "])[pos] = (" << T << ")" << vals[j]
+              << ";\n";
+            ks.out_types.push_back(plan.ops[0].types[j]);
+        }
+        if (vals.size() > 16) throw Unsupported("more than 16 output columns");
+    } else {
+        o << "        (void)L; (void)row; (void)pos;\n";
+    }
+    o << "    }\n};\n\n";
+    o << "extern \"C\" __global__ __launch_bounds__(" << tune.block
+      << ") void rf_q(RfScanArgs L) { rf_scan<Q>(L); }\n";
+    ks.code = o.str();
+    return ks;
+}
+
+}  // namespace rf

@@ -1,0 +1,70 @@
+// Emits the per-query row code (`struct Q`) for the scan skeleton in
+// kernels/rf_device.h. The emitted expressions keep the C text semantics of
+// the reference's CEmitter (src/pass/cemitter.cpp:80-170): operators are
+// printed infix on values of the IR's C types, so integer promotion, casts
+// and comparisons behave as in the CPU JIT; kernels are compiled with
+// -ffp-contract=off so that no multiply-add is fused.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "lower.hpp"
+
+namespace rf {
+
+struct ParamSlot {
+    enum Kind { CONST, ARG, LEN } kind;
+    Type type;
+    double value = 0;  // CONST
+    int arg = 0;       // ARG: func input index; LEN: table input index
+    int col = 0;       // LEN: column
+};
+
+struct ColSlot {
+    int table;       // func input index
+    int col;         // column in that table
+    Type type;
+    bool nullable;   // validity bitmap present (specialisation)
+    bool vector;     // loaded with the scan's vector loads (driver column)
+};
+
+struct AccField {
+    int comp;        // reduce component
+    int field;       // field index within the component
+    Type type;
+    AggOp op;
+};
+
+struct Tuning {
+    int block = 256;
+    int items = 4;
+    int sub = 4;
+};
+
+struct KernelSource {
+    std::string code;              // Q struct + kernel entry
+    std::string entry = "rf_q";
+    std::vector<ParamSlot> params;
+    std::vector<ColSlot> cols;
+    std::vector<AccField> acc;     // Acc field order
+    std::vector<int> acc_offset;   // byte offset of each Acc field
+    int acc_bytes = 0;
+    bool has_sel = false;
+    std::vector<Type> out_types;   // selection output columns
+    Tuning tune;
+    bool det = false;
+};
+
+// `nullable[t][c]` says whether column c of func input t has a bitmap.
+KernelSource gen_scan(const Plan& plan,
+                      const std::vector<std::vector<bool>>& nullable,
+                      const Tuning& tune, bool det);
+
+// Picks ITEMS from the narrowest driver column the row code loads.
+Tuning default_tuning(const Plan& plan);
+
+// Identity of a fold as a C literal of `t`.
+std::string fold_identity(AggOp op, const Type& t);
+
+}  // namespace rf

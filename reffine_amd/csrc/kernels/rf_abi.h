@@ -1,0 +1,58 @@
+// Launch ABI shared by the host runtime and the device skeleton.
+// Compiled by the host compiler, by hipcc (ahead-of-time kernels) and by
+// hiprtc (query-specialised kernels); layouts must not depend on which.
+#pragma once
+
+#ifdef __HIPCC_RTC__
+typedef __INT8_TYPE__ int8_t;
+typedef __INT16_TYPE__ int16_t;
+typedef __INT32_TYPE__ int32_t;
+typedef __INT64_TYPE__ int64_t;
+typedef __UINT8_TYPE__ uint8_t;
+typedef __UINT16_TYPE__ uint16_t;
+typedef __UINT32_TYPE__ uint32_t;
+typedef __UINT64_TYPE__ uint64_t;
+#else
+#include <stdint.h>
+#endif
+
+#define RF_MAXCOLS 32
+#define RF_MAXPARAMS 96
+#define RF_MAXPROBES 4
+#define RF_MAXOUT 16
+
+#define RF_EMPTY_KEY ((int64_t)0x8000000000000000ull)
+
+union RfParam {
+    int64_t i;
+    double f;
+};
+
+// Open-addressing table over an int64 key column: first valid row per key
+// (the reference's build_index keeps the first occurrence,
+// include/reffine/arrow/table.h:107-138).
+struct RfProbeDev {
+    const int64_t* keys;   // slot keys, RF_EMPTY_KEY marks a free slot
+    const int64_t* rows;   // slot row (minimum row holding the key)
+    uint64_t mask;         // capacity - 1 (capacity a power of two)
+    int64_t min_key_row;   // row of key == RF_EMPTY_KEY, -1 if none
+};
+
+struct RfScanArgs {
+    int64_t start;        // first row of tile 0 (lo rounded down to ITEMS)
+    int64_t lo, end;      // driver rows [lo, end)
+    int64_t ntiles;
+    const void* col[RF_MAXCOLS];
+    const uint16_t* valid[RF_MAXCOLS];
+    struct RfProbeDev probe[RF_MAXPROBES];
+    void* partials;       // Acc per tile (deterministic) or per block
+    void* result;         // final Acc
+    uint32_t* counters;   // [0] tile ticket, [1] finished blocks, [2] error
+    uint64_t* tile_status;
+    uint64_t epoch;
+    uint32_t spin_limit;  // bound on look-back polls (RF_SPIN_LIMIT)
+    uint32_t pad0;
+    void* out[RF_MAXOUT];
+    int64_t* out_count;
+    union RfParam p[RF_MAXPARAMS];
+};

@@ -1,0 +1,102 @@
+// Ahead-of-time gfx950 kernels that do not depend on a query's row code:
+// the device hash index of reffine's build_index
+// (include/reffine/arrow/table.h:107-138), its point lookup (vector_locate,
+// include/reffine/vinstr/vinstr.h:137-146), fills used instead of
+// hipMemset so that launches stay capturable, and a copy kernel of known
+// size used to calibrate HBM byte counters.
+#include <hip/hip_runtime.h>
+
+#include "rf_device.h"
+
+extern "C" __global__ void rf_fill_u64(uint64_t* p, uint64_t v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Inserts every valid key of column 0 with its row; equal keys keep the
+// smallest row (the first occurrence).
+extern "C" __global__ void rf_index_build(const int64_t* __restrict__ keys,
+                                          const uint16_t* __restrict__ valid,
+                                          int64_t n, unsigned long long* slot_keys,
+                                          unsigned long long* slot_rows, uint64_t mask,
+                                          unsigned long long* min_key_row) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (valid && !rf_valid_bit(valid, i)) continue;
+        const int64_t key = keys[i];
+        if (key == RF_EMPTY_KEY) {
+            atomicMin(min_key_row, (unsigned long long)i);
+            continue;
+        }
+        uint64_t s = rf_hash64((uint64_t)key) & mask;
+        for (;;) {
+            const unsigned long long prev =
+                atomicCAS(&slot_keys[s], (unsigned long long)RF_EMPTY_KEY,
+                          (unsigned long long)key);
+            if (prev == (unsigned long long)RF_EMPTY_KEY || prev == (unsigned long long)key) {
+                atomicMin(&slot_rows[s], (unsigned long long)i);
+                break;
+            }
+            s = (s + 1) & mask;
+        }
+    }
+}
+
+extern "C" __global__ void rf_index_lookup(RfProbeDev h, const int64_t* __restrict__ keys,
+                                           int64_t n, int64_t* __restrict__ rows) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        rows[i] = rf_probe(h, keys[i]);
+}
+
+// Plain 16-byte-per-lane copy; its byte counters calibrate FETCH/WRITE_SIZE.
+extern "C" __global__ void rf_copy_calib(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                         int64_t n16) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// ------------------------------------------------------- host launchers
+
+namespace rf {
+
+static int grid_for(int64_t n, int block) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > 65536) g = 65536;
+    return (int)g;
+}
+
+hipError_t aot_fill_u64(void* p, uint64_t v, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rf_fill_u64, dim3(grid_for(n, 256)), dim3(256), 0, s, (uint64_t*)p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t aot_index_build(const int64_t* keys, const uint16_t* valid, int64_t n,
+                           void* slot_keys, void* slot_rows, uint64_t mask, void* min_key_row,
+                           hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rf_index_build, dim3(grid_for(n, 256)), dim3(256), 0, s, keys, valid, n,
+                       (unsigned long long*)slot_keys, (unsigned long long*)slot_rows, mask,
+                       (unsigned long long*)min_key_row);
+    return hipGetLastError();
+}
+
+hipError_t aot_index_lookup(const RfProbeDev& h, const int64_t* keys, int64_t n, int64_t* rows,
+                            hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rf_index_lookup, dim3(grid_for(n, 256)), dim3(256), 0, s, h, keys, n, rows);
+    return hipGetLastError();
+}
+
+hipError_t aot_copy(const void* src, void* dst, int64_t n16, hipStream_t s) {
+    if (n16 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rf_copy_calib, dim3(grid_for(n16, 256)), dim3(256), 0, s,
+                       (const uint4*)src, (uint4*)dst, n16);
+    return hipGetLastError();
+}
+
+}  // namespace rf

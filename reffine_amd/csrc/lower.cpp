@@ -1,0 +1,668 @@
+// GPU lowering: Func -> Plan.  See lower.hpp for the contract.
+#include "lower.hpp"
+
+#include <algorithm>
+#include <functional>
+#include <map>
+#include <set>
+#include <sstream>
+
+namespace rf {
+
+std::string leaf_name(const N& n) {
+    if (n && n->kind == K::Sym && !n->name.empty() && n->name[0] == '$')
+        return n->name;
+    return "";
+}
+
+bool is_leaf(const N& n, const char* prefix) {
+    auto s = leaf_name(n);
+    return !s.empty() && s.rfind(prefix, 0) == 0;
+}
+
+static N leaf(const std::string& name, const Type& t) { return mk_sym(name, t); }
+
+static bool same(const N& a, const N& b) {
+    if (a == b) return true;
+    if (!a || !b) return false;
+    if (a->kind == K::Sym && b->kind == K::Sym) return a->name == b->name && a->type == b->type;
+    return false;
+}
+
+static bool mentions(const N& e, const std::function<bool(const N&)>& p) {
+    if (!e) return false;
+    if (p(e)) return true;
+    for (auto& a : e->args)
+        if (mentions(a, p)) return true;
+    for (auto& a : e->iters)
+        if (mentions(a, p)) return true;
+    return false;
+}
+
+static bool mentions_state(const N& e) {
+    return mentions(e, [](const N& n) { return is_leaf(n, "$s."); });
+}
+
+static N mk_and(N a, N b) {
+    if (!a) return b;
+    if (!b) return a;
+    if (a->kind == K::Const && a->val != 0) return b;
+    if (b->kind == K::Const && b->val != 0) return a;
+    return mk_nary(MathOp::AND, {a, b});
+}
+
+static N true_c() { return mk_const(types::BOOL_T, 1); }
+
+// ---------------------------------------------------------------- context
+
+struct Lowerer {
+    N func;
+    bool vectorize;
+    std::map<std::string, N> tbl;        // table symbol name -> expr
+    std::map<std::string, int> input_ix; // input symbol name -> index
+    Plan plan;
+
+    // Row binding: what the scan's row means.
+    N row_val_sym;      // Reduce value symbol bound to a SubVector row
+    N iter_sym;         // Op iterator symbol (VecSpace / range form)
+    int driver = -1;
+    int driver_dim = 1;
+    std::map<std::string, int> probe_of;  // probe key text -> probe index
+
+    int input_index(const N& sym) {
+        auto it = input_ix.find(sym->name);
+        return it == input_ix.end() ? -1 : it->second;
+    }
+
+    N col_leaf(int table, int col) {
+        const Type& vt = plan.inputs[table]->type;
+        if (col < 0 || col >= (int)vt.dtypes.size())
+            throw IRError("column index out of range");
+        if (col < (int)vt.enc.size() && vt.enc[col] == Enc::RUNEND)
+            throw Unsupported("run-end encoded column in a row expression");
+        return leaf("$col." + std::to_string(table) + "." + std::to_string(col),
+                    vt.dtypes[col]);
+    }
+    N valid_leaf(int table, int col) {
+        return leaf("$valid." + std::to_string(table) + "." + std::to_string(col),
+                    types::BOOL_T);
+    }
+
+    // Row of a table as the struct reffine's Element/SubVector produce:
+    // columns dim.. of a dim-1 table (type.h:132-142 valty).
+    N row_struct(int table, int first_col) {
+        const Type& vt = plan.inputs[table]->type;
+        std::vector<N> cols;
+        for (int c = first_col; c < (int)vt.dtypes.size(); c++) {
+            if (c < (int)vt.enc.size() && vt.enc[c] == Enc::RUNEND) {
+                cols.push_back(leaf("$ree", vt.dtypes[c]));  // poisoned lazily
+            } else {
+                cols.push_back(col_leaf(table, c));
+            }
+        }
+        return mk_new(cols);
+    }
+
+    int probe_for(int table, N key) {
+        std::string k = std::to_string(table) + "|" + to_text(key);
+        auto it = probe_of.find(k);
+        if (it != probe_of.end()) return it->second;
+        const Type& vt = plan.inputs[table]->type;
+        if (vt.dim != 1) throw Unsupported("probe into a multi-dimensional vector");
+        if (vt.dtypes[0] != types::I64)
+            throw Unsupported("hash probe needs an int64 iterator column "
+                              "(build_index, include/reffine/arrow/table.h:107-138)");
+        if (key->type != types::I64) throw IRError("probe key must be i64");
+        int j = (int)plan.probes.size();
+        plan.probes.push_back({table, key});
+        probe_of[k] = j;
+        return j;
+    }
+
+    // Probed row: columns of table T at the row holding `key` in column 0.
+    N probe_row(int table, N key) {
+        int j = probe_for(table, key);
+        const Type& vt = plan.inputs[table]->type;
+        std::vector<N> cols;
+        for (int c = vt.dim; c < (int)vt.dtypes.size(); c++)
+            cols.push_back(leaf("$pcol." + std::to_string(j) + "." + std::to_string(c),
+                                vt.dtypes[c]));
+        return mk_new(cols);
+    }
+
+    // ------------------------------------------------------ the rewriter
+
+    std::map<Node*, N> memo;
+    std::map<std::string, N> subst;  // symbol name -> replacement
+
+    N rw(const N& e) {
+        auto m = memo.find(e.get());
+        if (m != memo.end()) return m->second;
+        N r = rw_(e);
+        memo[e.get()] = r;
+        return r;
+    }
+
+    N rw_(const N& e) {
+        switch (e->kind) {
+            case K::Sym: {
+                if (!leaf_name(e).empty()) return e;
+                auto s = subst.find(e->name);
+                if (s != subst.end()) return s->second;
+                int ix = input_index(e);
+                if (ix >= 0) {
+                    if (e->type.is_vector())
+                        throw Unsupported("vector input '" + e->name +
+                                          "' used as a row value");
+                    return leaf("$arg." + std::to_string(ix), e->type);
+                }
+                auto t = tbl.find(e->name);
+                if (t != tbl.end()) return rw(t->second);
+                throw IRError("undefined symbol " + e->name);
+            }
+            case K::Const:
+                return e;
+            case K::Cast:
+                return mk_cast(e->type, rw(e->args[0]));
+            case K::Select: {
+                N c = rw(e->args[0]), t = rw(e->args[1]), f = rw(e->args[2]);
+                if (t->kind == K::New && f->kind == K::New) {
+                    std::vector<N> v;
+                    for (size_t i = 0; i < t->args.size(); i++)
+                        v.push_back(simplify_select(c, t->args[i], f->args[i]));
+                    return mk_new(v);
+                }
+                return simplify_select(c, t, f);
+            }
+            case K::Get: {
+                N v = rw(e->args[0]);
+                return get_of(v, e->idx);
+            }
+            case K::New: {
+                std::vector<N> v;
+                for (auto& a : e->args) v.push_back(rw(a));
+                return mk_new(v);
+            }
+            case K::Nary: {
+                std::vector<N> v;
+                for (auto& a : e->args) v.push_back(rw(a));
+                return mk_nary(e->op, v);
+            }
+            case K::Element:
+                return rw_element(e);
+            case K::In:
+                return rw_in(e);
+            case K::Length: {
+                N vec = resolve_vec(e->args[0]);
+                int t = input_index(vec);
+                if (t < 0) throw Unsupported("length of a computed vector");
+                return leaf("$len." + std::to_string(t) + "." + std::to_string(e->idx),
+                            types::IDX_T);
+            }
+            default:
+                throw Unsupported("node kind inside a row expression: " + to_text(e));
+        }
+    }
+
+    N simplify_select(N c, N t, N f) {
+        if (t->type.is_struct()) {
+            // Select of structs that are not literal New nodes.
+            std::vector<N> v;
+            for (size_t i = 0; i < t->type.dtypes.size(); i++)
+                v.push_back(simplify_select(c, get_of(t, i), get_of(f, i)));
+            return mk_new(v);
+        }
+        if (same(t, f)) return t;
+        return mk_select(c, t, f);
+    }
+
+    N get_of(N v, int64_t k) {
+        if (v->kind == K::New) {
+            N r = v->args.at(k);
+            if (is_leaf(r, "$ree")) throw Unsupported("run-end encoded column read per row");
+            return r;
+        }
+        if (v->kind == K::Select) return simplify_select(v->args[0], get_of(v->args[1], k),
+                                                         get_of(v->args[2], k));
+        if (v->type.is_primitive() && k == 0) return v;  // canonpass.cpp:22-29
+        return mk_get(v, k);
+    }
+
+    N resolve_vec(const N& e) {
+        if (e->kind == K::Sym) {
+            if (input_index(e) >= 0) return e;
+            auto t = tbl.find(e->name);
+            if (t != tbl.end()) return resolve_vec(t->second);
+        }
+        return e;
+    }
+
+    N rw_element(const N& e) {
+        N vec = resolve_vec(e->args[0]);
+        int t = input_index(vec);
+        if (t < 0) throw Unsupported("Element of a computed vector");
+        const N& it = e->args[1];
+        if (iter_sym && it->kind == K::Sym && it->name == iter_sym->name &&
+            t == driver) {
+            // Iterator-bound element: direct row (loopgen.cpp:16-17).
+            if (plan.inputs[t]->type.dim != 1)
+                throw Unsupported("element of a multi-dimensional vector");
+            return row_struct(t, 1);
+        }
+        // Otherwise a hash probe (loopgen.cpp:18-19, vector_locate).
+        return probe_row(t, rw(it));
+    }
+
+    N rw_in(const N& e) {
+        N vec = resolve_vec(e->args[1]);
+        int t = input_index(vec);
+        if (t < 0) throw Unsupported("In over a computed vector");
+        if (t == driver && iter_sym && e->args[0]->kind == K::Sym &&
+            e->args[0]->name == iter_sym->name)
+            return true_c();  // the scan itself
+        int j = probe_for(t, rw(e->args[0]));
+        return leaf("$found." + std::to_string(j), types::BOOL_T);
+    }
+
+    // -------------------------------------------- state-field decomposition
+
+    struct Dec {
+        bool upd = false;
+        AggOp op = AggOp::ADD;
+        N pred, contrib;
+    };
+
+    Dec decompose(const N& E, const std::string& fname) {
+        Dec d;
+        if (E->kind == K::Sym && E->name == fname) return d;  // no update
+        if (E->kind == K::Select) {
+            const N& c = E->args[0];
+            if (mentions_state(c))
+                throw Unsupported("reduction predicate depends on the state");
+            Dec t = decompose(E->args[1], fname), f = decompose(E->args[2], fname);
+            if (!t.upd && !f.upd) return d;
+            if (!t.upd) {
+                f.pred = mk_and(mk_nary(MathOp::NOT, {c}), f.pred);
+                return f;
+            }
+            if (!f.upd) {
+                t.pred = mk_and(c, t.pred);
+                return t;
+            }
+            if (t.op != f.op) throw Unsupported("select between different folds");
+            d.upd = true;
+            d.op = t.op;
+            N tp = t.pred ? t.pred : true_c(), fp = f.pred ? f.pred : true_c();
+            d.pred = (t.pred || f.pred)
+                         ? mk_select(c, tp, fp)
+                         : nullptr;
+            d.contrib = mk_select(c, t.contrib, f.contrib);
+            return d;
+        }
+        if (E->kind == K::Nary && E->args.size() == 2) {
+            const N& a = E->args[0];
+            const N& b = E->args[1];
+            bool a_s = a->kind == K::Sym && a->name == fname;
+            bool b_s = b->kind == K::Sym && b->name == fname;
+            AggOp op;
+            bool comm = true;
+            switch (E->op) {
+                case MathOp::ADD: op = AggOp::ADD; break;
+                case MathOp::SUB: op = AggOp::ADD; comm = false; break;
+                case MathOp::MUL: op = AggOp::MUL; break;
+                case MathOp::MAX: op = AggOp::MAX; break;
+                case MathOp::MIN: op = AggOp::MIN; break;
+                case MathOp::AND: op = AggOp::AND; break;
+                case MathOp::OR: op = AggOp::OR; break;
+                default:
+                    throw Unsupported("state field updated by " + mathop_name(E->op));
+            }
+            if (a_s && !mentions_state(b)) {
+                d.upd = true;
+                d.op = op;
+                d.contrib = comm ? b : mk_nary(MathOp::NEG, {b});
+                return d;
+            }
+            if (b_s && comm && !mentions_state(a)) {
+                d.upd = true;
+                d.op = op;
+                d.contrib = a;
+                return d;
+            }
+        }
+        throw Unsupported("reduction state field is not an associative fold: " +
+                          to_text(E));
+    }
+
+    // Flattened state leaves for a state type; returns New-tree of leaves.
+    N state_tree(const Type& t, std::vector<int>& path, std::vector<Field>& fields) {
+        if (t.is_struct()) {
+            std::vector<N> v;
+            for (size_t i = 0; i < t.dtypes.size(); i++) {
+                path.push_back((int)i);
+                v.push_back(state_tree(t.dtypes[i], path, fields));
+                path.pop_back();
+            }
+            return mk_new(v);
+        }
+        if (!t.is_primitive()) throw Unsupported("non-primitive reduction state field");
+        Field f;
+        f.type = t;
+        f.path = path;
+        std::string name = "$s." + std::to_string(fields.size());
+        fields.push_back(f);
+        return leaf(name, t);
+    }
+
+    static void flatten(const N& e, std::vector<N>& out) {
+        if (e->kind == K::New) {
+            for (auto& a : e->args) flatten(a, out);
+        } else {
+            if (e->type.is_struct()) throw Unsupported("unflattenable struct value");
+            out.push_back(e);
+        }
+    }
+
+    // ------------------------------------------------ scan source setup
+
+    // Binds the scan to a SubVector (loopgen.cpp:221-242) or an Op
+    // (loopgen.cpp:44-111). Returns the row value struct and the row
+    // predicate.
+    struct Source {
+        N row_value;  // struct value delivered to acc / Op outputs
+        N pred;       // row predicate (validity, Op predicate)
+    };
+
+    Source bind_subvec(const N& sv) {
+        N vec = resolve_vec(sv->args[0]);
+        int t = input_index(vec);
+        if (t < 0) throw Unsupported("SubVector of a computed vector");
+        set_driver(t);
+        plan.source = Plan::TABLE_ROWS;
+        plan.start = sv->args[1];
+        plan.end = sv->args[2];
+        const Type& vt = plan.inputs[t]->type;
+        if (vt.dtypes.size() < 2) throw Unsupported("SubVector of a one-column vector");
+        Source s;
+        // Outputs are ReadData of columns 1.. and the body condition is the
+        // validity of column 1 (loopgen.cpp:231-238).
+        s.row_value = row_struct(t, 1);
+        s.pred = valid_leaf(t, 1);
+        return s;
+    }
+
+    void set_driver(int t) {
+        if (driver >= 0 && driver != t)
+            throw Unsupported("components scan different tables");
+        driver = t;
+        plan.driver = t;
+    }
+
+    // Splits an Op predicate into conjuncts.
+    static void conjuncts(const N& e, std::vector<N>& out) {
+        if (e->kind == K::Nary && e->op == MathOp::AND) {
+            conjuncts(e->args[0], out);
+            conjuncts(e->args[1], out);
+        } else {
+            out.push_back(e);
+        }
+    }
+
+    N resolve_bool_sym(const N& e) {
+        if (e->kind == K::Sym && leaf_name(e).empty() && input_index(e) < 0) {
+            auto t = tbl.find(e->name);
+            if (t != tbl.end()) return resolve_bool_sym(t->second);
+        }
+        return e;
+    }
+
+    Source bind_op(const N& op) {
+        if (op->iters.size() != 1)
+            throw Unsupported("Op with more than one iterator");
+        iter_sym = op->iters[0];
+        std::vector<N> cj;
+        conjuncts(resolve_bool_sym(op->args[0]), cj);
+        // The first In(t, T) over an input table drives the scan (VecSpace,
+        // iter_space.cpp:55-91); further In's are hash semi-joins
+        // (InterSpace, iter_space.cpp:318-364).
+        int drv = -1;
+        for (auto& c0 : cj) {
+            N c = resolve_bool_sym(c0);
+            if (c->kind == K::In && c->args[0]->kind == K::Sym &&
+                c->args[0]->name == iter_sym->name) {
+                N vec = resolve_vec(c->args[1]);
+                int t = input_index(vec);
+                if (t >= 0) {
+                    drv = t;
+                    break;
+                }
+            }
+        }
+        Source s;
+        N pred;
+        if (drv >= 0) {
+            set_driver(drv);
+            plan.source = Plan::TABLE_ROWS;
+            const Type& vt = plan.inputs[drv]->type;
+            if (vt.dim != 1) throw Unsupported("Op over a multi-dimensional vector");
+            if (vt.dtypes[0] != iter_sym->type) throw IRError("iterator type mismatch");
+            plan.start = mk_const(types::IDX_T, 0);
+            plan.end = leaf("$len." + std::to_string(drv) + ".0", types::IDX_T);
+            subst[iter_sym->name] = col_leaf(drv, 0);
+            // VecSpace::_iter_cond: validity of column 0 (iter_space.cpp:60-66)
+            pred = valid_leaf(drv, 0);
+        } else {
+            // Range form: iterator bounded by lb <= t < ub (gen_table_op,
+            // test/src/test_utils.cpp:32-46).
+            plan.source = Plan::RANGE;
+            if (!iter_sym->type.is_int() && !iter_sym->type.is_idx())
+                throw Unsupported("range Op over a non-integer iterator");
+            N lb, ub;
+            std::vector<N> rest;
+            for (auto& c0 : cj) {
+                N c = resolve_bool_sym(c0);
+                if (c->kind == K::Nary && c->args.size() == 2 &&
+                    (c->op == MathOp::GTE || c->op == MathOp::GT ||
+                     c->op == MathOp::LT || c->op == MathOp::LTE)) {
+                    bool l_it = c->args[0]->kind == K::Sym && c->args[0]->name == iter_sym->name;
+                    bool r_it = c->args[1]->kind == K::Sym && c->args[1]->name == iter_sym->name;
+                    auto one = mk_const(iter_sym->type, 1);
+                    if (l_it && !r_it) {
+                        N b = c->args[1];
+                        if (c->op == MathOp::GTE && !lb) { lb = b; continue; }
+                        if (c->op == MathOp::GT && !lb) { lb = mk_nary(MathOp::ADD, {b, one}); continue; }
+                        if (c->op == MathOp::LT && !ub) { ub = b; continue; }
+                        if (c->op == MathOp::LTE && !ub) { ub = mk_nary(MathOp::ADD, {b, one}); continue; }
+                    } else if (r_it && !l_it) {
+                        N b = c->args[0];
+                        if (c->op == MathOp::LTE && !lb) { lb = b; continue; }
+                        if (c->op == MathOp::LT && !lb) { lb = mk_nary(MathOp::ADD, {b, one}); continue; }
+                        if (c->op == MathOp::GT && !ub) { ub = b; continue; }
+                        if (c->op == MathOp::GTE && !ub) { ub = mk_nary(MathOp::ADD, {b, one}); continue; }
+                    }
+                }
+                rest.push_back(c);
+            }
+            if (!lb || !ub) throw Unsupported("Op iterator is not bounded on both sides");
+            plan.start = mk_cast(types::IDX_T, lb);
+            plan.end = mk_cast(types::IDX_T, ub);
+            subst[iter_sym->name] = mk_cast(iter_sym->type, leaf("$row", types::IDX_T));
+            cj = rest;
+        }
+        for (auto& c : cj) pred = mk_and(pred, rw(c));
+        s.pred = pred ? pred : true_c();
+        std::vector<N> vals;
+        for (auto& it : op->iters) vals.push_back(rw(it));
+        for (size_t i = 1; i < op->args.size(); i++) vals.push_back(rw(op->args[i]));
+        s.row_value = mk_new(vals);
+        return s;
+    }
+
+    // ------------------------------------------------------ components
+
+    ReduceComp lower_reduce(const N& red, const std::string& sym_name) {
+        const N& vec = red->args[0];
+        const N& init = red->args[1];
+        const N& ssym = red->args[2];
+        const N& vsym = red->args[3];
+        const N& acc = red->args[4];
+        if (vec->type.dim != 1)  // loopgen.cpp:209
+            throw Unsupported("reduction over a multi-dimensional vector");
+        N rv = resolve_vec(vec);
+        Source src;
+        if (rv->kind == K::SubVector) {
+            src = bind_subvec(rv);
+        } else if (rv->kind == K::Op) {
+            src = bind_op(rv);
+        } else {
+            throw Unsupported("reduction over " + to_text(rv));
+        }
+        ReduceComp rc;
+        rc.state_type = red->type;
+        rc.sym_name = sym_name;
+        std::vector<int> path;
+        N stree = state_tree(red->type, path, rc.fields);
+        subst[ssym->name] = stree;
+        subst[vsym->name] = src.row_value;
+        memo.clear();
+        N accr = rw(acc);
+        std::vector<N> flat;
+        flatten(accr, flat);
+        if (flat.size() != rc.fields.size()) throw IRError("accumulator shape");
+        std::vector<N> inits;
+        flatten(fold_host(init), inits);
+        for (size_t f = 0; f < rc.fields.size(); f++) {
+            Dec d = decompose(flat[f], "$s." + std::to_string(f));
+            Field& fl = rc.fields[f];
+            fl.init = inits.at(f);
+            fl.updated = d.upd;
+            if (d.upd) {
+                fl.op = d.op;
+                // vectorize: state = body_cond ? acc : state; otherwise
+                // if (body_cond) state = acc -- the same fold
+                // (loopgen.cpp:253-262).
+                fl.pred = mk_and(src.pred, d.pred);
+                if (!fl.pred) fl.pred = true_c();
+                fl.contrib = d.contrib;
+                if (mentions_state(fl.contrib) || mentions_state(fl.pred))
+                    throw Unsupported("state field depends on another field");
+            }
+        }
+        return rc;
+    }
+
+    // Structural inlining for host-side expressions (init values, bounds).
+    N fold_host(const N& e) {
+        memo.clear();
+        return rw(e);
+    }
+
+    OpComp lower_op(const N& op, const std::string& sym_name) {
+        Source src = bind_op(op);
+        OpComp oc;
+        oc.sym_name = sym_name;
+        oc.pred = src.pred;
+        oc.values = src.row_value->args;
+        oc.n_iters = (int)op->iters.size();
+        for (auto& v : oc.values) {
+            if (!v->type.is_primitive()) throw Unsupported("non-primitive Op output");
+            oc.types.push_back(v->type);
+        }
+        oc.vec_type = op->type;
+        return oc;
+    }
+
+    Plan run() {
+        plan.func_name = func->name;
+        for (size_t i = 1; i < func->args.size(); i++) {
+            plan.inputs.push_back(func->args[i]);
+            input_ix[func->args[i]->name] = (int)i - 1;
+        }
+        for (auto& kv : func->tbl) tbl[kv.first->name] = kv.second;
+        plan.tbl = func->tbl;
+        plan.vectorize = vectorize;
+        N out = func->args[0];
+        plan.output_type = out->type;
+        // Collect the components the output needs: Reduce / Op symbols.
+        std::vector<std::pair<std::string, N>> comps;
+        std::set<std::string> seen;
+        std::function<void(const N&)> walk = [&](const N& e) {
+            if (e->kind == K::Sym) {
+                auto t = tbl.find(e->name);
+                if (t == tbl.end()) return;
+                if (t->second->kind == K::Reduce || t->second->kind == K::Op) {
+                    if (seen.insert(e->name).second) comps.push_back({e->name, t->second});
+                    return;
+                }
+                walk(t->second);
+                return;
+            }
+            if (e->kind == K::Reduce || e->kind == K::Op)
+                throw Unsupported("unnamed Reduce/Op in the output expression");
+            for (auto& a : e->args) walk(a);
+        };
+        walk(out);
+        if (comps.empty()) throw Unsupported("function output has no Reduce or Op");
+        for (auto& [name, node] : comps) {
+            if (node->kind == K::Reduce) {
+                plan.reduces.push_back(lower_reduce(node, name));
+            } else {
+                if (!plan.ops.empty()) throw Unsupported("more than one Op output");
+                plan.ops.push_back(lower_op(node, name));
+            }
+        }
+        plan.output = out;
+        return plan;
+    }
+};
+
+Plan lower_func(const N& func, bool vectorize) {
+    if (!func || func->kind != K::Func) throw IRError("expected a Func");
+    Lowerer L;
+    L.func = func;
+    L.vectorize = vectorize;
+    return L.run();
+}
+
+static const char* aggop_name(AggOp o) {
+    switch (o) {
+        case AggOp::ADD: return "add";
+        case AggOp::MUL: return "mul";
+        case AggOp::MIN: return "min";
+        case AggOp::MAX: return "max";
+        case AggOp::AND: return "and";
+        case AggOp::OR: return "or";
+    }
+    return "?";
+}
+
+std::string Plan::describe() const {
+    std::ostringstream o;
+    o << "func " << func_name << ": scan "
+      << (source == RANGE ? "range" : "table input " + std::to_string(driver))
+      << " rows [" << to_text(start) << ", " << to_text(end) << ")\n";
+    for (size_t j = 0; j < probes.size(); j++)
+        o << "  probe " << j << ": table input " << probes[j].table << " key "
+          << to_text(probes[j].key) << "\n";
+    for (auto& r : reduces) {
+        o << "  reduce " << r.sym_name << " -> " << r.state_type.str() << "\n";
+        for (size_t f = 0; f < r.fields.size(); f++) {
+            auto& fl = r.fields[f];
+            o << "    field " << f << " " << fl.type.str();
+            if (fl.updated)
+                o << " " << aggop_name(fl.op) << " if " << to_text(fl.pred)
+                  << " of " << to_text(fl.contrib);
+            else
+                o << " (not updated)";
+            o << "\n";
+        }
+    }
+    for (auto& op : ops) {
+        o << "  select " << op.sym_name << " -> " << op.vec_type.str() << " if "
+          << to_text(op.pred) << "\n";
+        for (auto& v : op.values) o << "    value " << to_text(v) << "\n";
+    }
+    return o.str();
+}
+
+}  // namespace rf

@@ -1,0 +1,100 @@
+// GPU lowering of a reffine Func onto the scan kernel skeleton.
+//
+// The reference lowers Reduce/Op nodes to a sequential Loop
+// (src/pass/loopgen.cpp:44-269) that folds one row at a time into a struct
+// state. The GPU counterpart keeps the same row semantics but re-associates
+// the fold: every flattened state field must update as
+//     s_f' = pred_f(row) ? s_f (+|*|min|max|&&|||) contrib_f(row) : s_f
+// with pred_f and contrib_f free of state, so rows can be combined in a
+// tree. Such fields are produced by the reference's own idioms: _sel(pred,
+// new_state, s) (benchmark/include/tpch.h:174-208), per-field _sel
+// (tpch.h:412-478) and plain accumulation (test/src/test_reduce.cpp:61-122).
+// A field that does not decompose makes the query RF_ERR_UNSUPPORTED.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "ir.hpp"
+
+namespace rf {
+
+// Row-level leaf symbols produced by the lowering (Sym nodes whose names
+// start with '$'):
+//   $col.T.C    value of column C of func input T at the current row
+//               (T is the scanned table) or at the probed row (T probed)
+//   $valid.T.C  validity bit of that column
+//   $found.T    the probe into table T found the key
+//   $row        the row index (or the iterator value of a range scan)
+//   $arg.I      scalar func input I
+//   $len.T.C    length of column C of table T (host-known)
+//   $s.F        state field F (only inside the decomposition)
+std::string leaf_name(const N& n);  // "" if not a leaf
+bool is_leaf(const N& n, const char* prefix);
+
+enum class AggOp { ADD, MUL, MIN, MAX, AND, OR };
+
+struct Field {
+    Type type;           // primitive type of the flattened state field
+    AggOp op = AggOp::ADD;
+    bool updated = false;
+    N pred;              // bool row predicate (never null when updated)
+    N contrib;           // contribution (field type)
+    N init;              // init expression of the field (host-evaluated)
+    std::vector<int> path;  // position inside the nested state struct
+};
+
+struct ReduceComp {
+    Type state_type;
+    std::vector<Field> fields;
+    std::string sym_name;  // table symbol holding this reduction
+};
+
+struct OpComp {
+    std::vector<N> values;   // iterator values then outputs (row leaves)
+    std::vector<Type> types;
+    int n_iters = 1;
+    N pred;                  // row qualifies (bool)
+    Type vec_type;           // reffine type of the output vector
+    std::string sym_name;
+};
+
+struct Probe {
+    int table;  // func input index of the probed table
+    N key;      // int64 row expression giving the key
+};
+
+struct Plan {
+    enum Source { TABLE_ROWS, RANGE } source = TABLE_ROWS;
+    std::string func_name;
+    std::vector<N> inputs;     // func inputs (syms)
+    int driver = -1;           // func input index of the scanned table
+    N start, end;              // host expressions: rows [start, end)
+    N row_pred;                // common row predicate (validity, bounds)
+    std::vector<Probe> probes;
+    std::vector<ReduceComp> reduces;
+    std::vector<OpComp> ops;   // at most one
+    N output;                  // func output expression over component syms
+    Type output_type;
+    std::vector<std::pair<N, N>> tbl;  // the func's symbol table
+    bool vectorize = false;
+
+    std::string describe() const;
+};
+
+// Lowers `func` (a parsed IR Func) into a plan; throws Unsupported for
+// shapes the kernels do not cover and IRError for ill-formed IR.
+Plan lower_func(const N& func, bool vectorize);
+
+// Evaluation of host-side expressions (bounds, init values, the func output
+// over reduction results).
+struct HostVal {
+    Type type;
+    bool is_float = false;
+    double f = 0;
+    int64_t i = 0;
+    std::vector<HostVal> fields;  // struct
+    void* table = nullptr;        // vector-typed output (device table)
+};
+
+}  // namespace rf

@@ -1,0 +1,84 @@
+"""Query builders for the scan workloads, written in reffine's own idioms.
+
+The scan-filter-aggregate form is reffine's TPC-H Q6 idiom
+(benchmark/include/tpch.h:174-208): `_red(_subvec(T, 0, _len(T, 1)), init,
+acc)` with `acc = _sel(pred, new_state, s)` over a 2-dimensional table whose
+fold skips column 0. The selection form is an `_op` over `_in(t, T)`
+(test/src/test_transform.cpp:9-28) whose iterator column is an int32 row
+id, so its output column 0 is the int32 selection vector.
+"""
+from __future__ import annotations
+
+from .reffiner import (_add, _cast, _f64, _f64_t, _func, _gte, _i32, _i32_t, _i64, _i64_t,
+                       _idx, _in, _len, _lt, _lte, _mul, _new, _op, _red, _sel, _subvec, _sym,
+                       _vec_t, _get)
+
+
+def scan_agg_c1(lo: int = 365, hi: int = 730, vmax: float = 0.5):
+    """COUNT(*), SUM(v*k) WHERE lo <= d < hi AND v < vmax over
+    T(rowid i64, k i64, v f64, d i32), dim 2."""
+    T = _sym("T", _vec_t(2, _i64_t, _i64_t, _f64_t, _i32_t))
+
+    def acc(s, v):
+        k, x, d = _get(v, 0), _get(v, 1), _get(v, 2)
+        pred = _gte(d, _i32(lo)) & _lt(d, _i32(hi)) & _lt(x, _f64(vmax))
+        new_s = _new([_add(_get(s, 0), _i64(1)),
+                      _add(_get(s, 1), _mul(x, _cast(_f64_t, k)))])
+        return _sel(pred, new_s, s)
+
+    red = _red(_subvec(T, _idx(0), _len(T, 1)), lambda: _new([_i64(0), _f64(0)]), acc)
+    rs = _sym("red", red)
+    fn = _func("scan_c1", rs, [T])
+    fn.tbl[rs] = red
+    return fn
+
+
+C2_TYPE = _vec_t(1, _i32_t, _i64_t, _f64_t, _i32_t, _i64_t)  # rowid, k, v, d, q
+
+
+def _c2_pred(T, t, d0, vlo, vhi, qmax):
+    e = T[t]
+    v, d, q = e[1], e[2], e[3]
+    return (_in(t, T) & _gte(d, _i32(d0)) & _lt(d, _i32(d0 + 365)) & _gte(v, _f64(vlo)) &
+            _lte(v, _f64(vhi)) & _lt(q, _i64(qmax))), e
+
+
+def scan_select_c2(d0: int = 1000, vlo: float = 0.05, vhi: float = 0.07, qmax: int = 24,
+                   with_selection: bool = True):
+    """COUNT, SUM(q) i64, SUM(v*q) f64 and the int32 selection vector of
+    rows with d in [d0, d0+365), vlo <= v <= vhi, q < qmax, over
+    T(rowid i32, k i64, v f64, d i32, q i64). Output struct
+    {{i64, i64, f64}, selection-vector} (the selection is an Op whose only
+    column is the iterator)."""
+    T = _sym("T", C2_TYPE)
+    t = _sym("t", _i32_t)
+    pred, e = _c2_pred(T, t, d0, vlo, vhi, qmax)
+    rows = _op([t], pred, [e[3], e[1]])  # iterator, q, v
+
+    def acc(s, r):
+        q, v = r[1], r[2]
+        return _new([_add(_get(s, 0), _i64(1)), _add(_get(s, 1), q),
+                     _add(_get(s, 2), _mul(v, _cast(_f64_t, q)))])
+
+    red = _red(rows, lambda: _new([_i64(0), _i64(0), _f64(0)]), acc)
+    rs = _sym("agg", red)
+    if not with_selection:
+        fn = _func("scan_c2_agg", rs, [T])
+        fn.tbl[rs] = red
+        return fn
+    t2 = _sym("t", _i32_t)
+    pred2, _ = _c2_pred(T, t2, d0, vlo, vhi, qmax)
+    sel = _op([t2], pred2, [])
+    ss = _sym("sel", sel)
+    out = _new([rs, ss])
+    os_ = _sym("out", out)
+    fn = _func("scan_c2", os_, [T])
+    fn.tbl[rs] = red
+    fn.tbl[ss] = sel
+    fn.tbl[os_] = out
+    return fn
+
+
+def c2_bytes_per_row() -> int:
+    """Algorithmic bytes per scanned row: d (4) + v (8) + q (8)."""
+    return 20

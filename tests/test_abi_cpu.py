@@ -1,0 +1,68 @@
+"""C ABI checks that need no GPU: the library loads, exports exactly what
+include/reffine_hip.h declares, rejects bad input with status codes, and
+every workload query lowers and compiles for gfx950."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from reffine_amd import runtime
+from reffine_amd.reffiner import (_add, _f64, _f64_t, _func, _get, _i64, _i64_t, _idx, _len,
+                                  _mul, _new, _red, _sel, _subvec, _sym, _vec_t)
+from reffine_amd import workloads
+
+HEADER = os.path.join(os.path.dirname(__file__), "..", "include", "reffine_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\w[\w\s\*]*?\b(rf_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = runtime.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in reffine_hip.h but not exported"
+    assert sorted(runtime.EXPORTED) == names
+
+
+def test_errors_are_status_codes_without_device():
+    lib = runtime.lib()
+    h = ctypes.c_void_p()
+    # No rf_init: compile needs a device -> HIP error status, no crash.
+    st = lib.rf_compile(workloads.scan_agg_c1().ir().encode(), 0, ctypes.byref(h))
+    assert st != 0
+    assert b"rf_init" in lib.rf_last_error()
+    # Malformed IR is RF_ERR_INVALID.
+    buf = ctypes.create_string_buffer(256)
+    assert lib.rf_compile_check(b"(func", 0, buf, 256) == 1
+    assert lib.rf_compile_check(b"(func f (sym o#1 i64) () ())", 0, buf, 256) == 2
+
+
+def test_unsupported_fold_is_rejected():
+    T = _sym("T", _vec_t(2, _i64_t, _i64_t, _f64_t))
+    # s * 0.5 + v is not an associative fold of the state.
+    red = _red(_subvec(T, _idx(0), _len(T, 1)), lambda: _f64(0),
+               lambda s, v: _add(_mul(s, _f64(0.5)), _get(v, 1)))
+    rs = _sym("r", red)
+    fn = _func("bad", rs, [T])
+    fn.tbl[rs] = red
+    with pytest.raises(runtime.ReffineError) as e:
+        runtime.compile_check(fn)
+    assert e.value.status == 2
+
+
+@pytest.mark.parametrize("det", [False, True])
+@pytest.mark.parametrize("build", [workloads.scan_agg_c1, workloads.scan_select_c2,
+                                   lambda: workloads.scan_select_c2(with_selection=False)])
+def test_workloads_compile_for_gfx950(build, det):
+    s = runtime.compile_check(build(), deterministic=det)
+    assert "code object bytes" in s
+
+
+def test_nullable_variant_compiles():
+    s = runtime.compile_check(workloads.scan_agg_c1(), nullable=True)
+    assert "rf_valid_bits" in s
