@@ -1,0 +1,112 @@
+"""The reference test suite's queries, rebuilt with the mirrored DSL.
+
+Each builder cites the reference test it restates; the expected values are
+in tests/golden/expected.json."""
+import json
+import os
+
+import numpy as np
+
+from reffine_amd.reffiner import (_add, _func, _get, _gt, _gte, _i64, _i64_t, _idx, _in, _len,
+                                  _lt, _lte, _new, _op, _red, _subvec, _sym, _vec_t)
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+STUDENT_COLS = ["id", "hours_studied", "attendance", "sleep_hours", "previous_scores",
+                "exam_score"]
+
+
+def expected():
+    with open(os.path.join(GOLDEN, "expected.json")) as f:
+        return json.load(f)
+
+
+def students():
+    d = np.load(os.path.join(GOLDEN, "students.npz"))
+    return [d[c] for c in STUDENT_COLS]
+
+
+def students_type(dim=1):
+    return _vec_t(dim, *([_i64_t] * len(STUDENT_COLS)))
+
+
+def sum_hours_subvec():
+    """test/src/test_reduce.cpp:9-59 sums column 1 with a hand-written Loop;
+    the same fold in the Reduce/SubVector idiom (2-dim table: the fold reads
+    columns 1..)."""
+    T = _sym("vec", students_type(2))
+    red = _red(_subvec(T, _idx(0), _len(T, 1)), lambda: _i64(0),
+               lambda s, v: _add(s, _get(v, 0)))
+    rs = _sym("sum", red)
+    fn = _func("foo", rs, [T])
+    fn.tbl[rs] = red
+    return fn
+
+
+def vector_op():
+    """test/src/test_reduce.cpp:61-115 (aggregate_op_test)."""
+    t = _sym("t", _i64_t)
+    vec = _sym("vec_in", students_type(1))
+    op = _op([t], _in(t, vec) & _lte(t, _i64(48)) & _gte(t, _i64(10)),
+             [vec[t][2], _new([vec[t][1], vec[t][2], vec[t][0], vec[t][3]]), vec[t][3]])
+
+    def acc(s, v):
+        v0, v1 = _get(_get(v, 2), 0), _get(_get(v, 2), 1)
+        v2, v3 = _get(_get(v, 2), 2), _get(_get(v, 2), 3)
+        s0, s1 = _get(_get(s, 0), 0), _get(_get(s, 0), 1)
+        s2, s3 = _get(_get(s, 1), 0), _get(_get(s, 1), 1)
+        return _new([_new([_add(s0, v0), _add(s1, v1)]), _new([_add(s2, v2), _add(s3, v3)])])
+
+    sm = _red(op, lambda: _new([_new([_i64(0), _i64(0)]), _new([_i64(0), _i64(0)])]), acc)
+    sum_sym = _sym("sum", sm)
+    res = _get(_get(sum_sym, 1), 0)
+    res_sym = _sym("res", res)
+    fn = _func("foo", res_sym, [vec])
+    fn.tbl[sum_sym] = sm
+    fn.tbl[res_sym] = res
+    return fn
+
+
+def transform_op(lb=5, n=10):
+    """test/src/test_transform.cpp:9-28."""
+    t = _sym("t", _i64_t)
+    vec = _sym("vec_in", students_type(1))
+    elem = vec[t]
+    elem_sym = _sym("elem", elem)
+    out = _add(elem_sym[0], _i64(n))
+    out_sym = _sym("out", out)
+    op = _op([t], _in(t, vec) & _gt(t, _i64(lb)), [out_sym])
+    op_sym = _sym("op", op)
+    fn = _func("foo", op_sym, [vec])
+    fn.tbl[elem_sym] = elem
+    fn.tbl[out_sym] = out
+    fn.tbl[op_sym] = op
+    return fn
+
+
+def gen_table_op():
+    """test/src/test_utils.cpp:32-46: Op over lb <= t < ub with output {t}."""
+    t = _sym("t", _i64_t)
+    lb = _sym("lb", _i64_t)
+    ub = _sym("ub", _i64_t)
+    op = _op([t], _gte(t, lb) & _lt(t, ub), [t])
+    op_sym = _sym("op", op)
+    fn = _func("gen_table", op_sym, [lb, ub])
+    fn.tbl[op_sym] = op
+    return fn
+
+
+def join_op():
+    """test/src/test_join.cpp:9-35: 3-way intersection on t."""
+    ty = _vec_t(1, _i64_t, _i64_t)
+    a, b, c = _sym("avec", ty), _sym("bvec", ty), _sym("cvec", ty)
+    t = _sym("t", _i64_t)
+    ae, be, ce = a[t][0], b[t][0], c[t][0]
+    a_s, b_s, c_s = _sym("a", ae), _sym("b", be), _sym("c", ce)
+    op = _op([t], _in(t, a) & _in(t, b) & _in(t, c), [_add(_add(a_s, b_s), c_s)])
+    op_sym = _sym("op", op)
+    fn = _func("join", op_sym, [a, b, c])
+    fn.tbl[a_s] = ae
+    fn.tbl[b_s] = be
+    fn.tbl[c_s] = ce
+    fn.tbl[op_sym] = op
+    return fn

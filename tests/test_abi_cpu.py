@@ -66,3 +66,16 @@ def test_workloads_compile_for_gfx950(build, det):
 def test_nullable_variant_compiles():
     s = runtime.compile_check(workloads.scan_agg_c1(), nullable=True)
     assert "rf_valid_bits" in s
+
+
+def test_reference_test_queries_compile_for_gfx950():
+    import ref_queries as rq
+    for build in [rq.sum_hours_subvec, rq.vector_op, rq.transform_op, rq.gen_table_op,
+                  rq.join_op]:
+        assert "code object bytes" in runtime.compile_check(build())
+
+
+def test_range_bounds_are_extracted():
+    import ref_queries as rq
+    s = runtime.compile_check(rq.gen_table_op())
+    assert "scan range rows [(cast x (sym lb" in s and "(cast x (sym ub" in s

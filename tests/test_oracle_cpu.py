@@ -1,0 +1,116 @@
+"""Pins the oracle before it is trusted: the CPU JIT restatement (oracle.cjit)
+and the brute-force restatement (oracle.brute) against the reference test
+suite's golden values and against closed forms / pure-Python loops; the
+fixed-order restatement (oracle.detorder) against exact sums."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import brute, cjit, detorder
+from reffine_amd import datagen, workloads
+import ref_queries as rq
+
+
+def students_input():
+    return {"cols": rq.students()}
+
+
+@pytest.mark.parametrize("vectorize", [False, True])
+def test_cjit_reduce_golden(vectorize):
+    exp = rq.expected()
+    (s,) = cjit.compile_op(rq.sum_hours_subvec(), vectorize)(students_input())
+    assert s == (exp["aggregate_loop_test"],)
+    (st,) = cjit.compile_op(rq.vector_op(), vectorize)(students_input())
+    assert st[2] == exp["aggregate_op_test"]  # res = get(get(sum, 1), 0)
+
+
+def test_cjit_transform_golden():
+    exp = rq.expected()
+    (cols,) = cjit.compile_op(rq.transform_op())(students_input())
+    ids, hours = rq.students()[0], rq.students()[1]
+    assert len(cols[0]) == exp["transform_op_test_len"]
+    np.testing.assert_array_equal(cols[0], ids[5:])
+    np.testing.assert_array_equal(cols[1], hours[5:] + 10)
+
+
+def test_cjit_gen_table_and_join_golden():
+    exp = rq.expected()
+    gen = cjit.compile_op(rq.gen_table_op())
+    tabs = []
+    for lb, ub in [(0, 10), (5, 15), (7, 12)]:
+        (cols,) = gen(lb, ub, cap=ub - lb)
+        np.testing.assert_array_equal(cols[0], np.arange(lb, ub))
+        tabs.append({"cols": [cols[0], cols[1]]})
+    (out,) = cjit.compile_op(rq.join_op())(*tabs)
+    np.testing.assert_array_equal(out[0], exp["join_op_test_keys"])
+    np.testing.assert_array_equal(out[1], 3 * np.array(exp["join_op_test_keys"]))
+
+
+def test_brute_scan_against_python_loops():
+    n = 3000
+    c = datagen.scan_table(n, 77)
+    cnt, s = brute.scan_c1(c["k"], c["v"], c["d"])
+    pc, ps = 0, []
+    for k, v, d in zip(c["k"].tolist(), c["v"].tolist(), c["d"].tolist()):
+        if 365 <= d < 730 and v < 0.5:
+            pc += 1
+            ps.append(v * float(k))
+    assert cnt == pc and s == math.fsum(ps)
+    cnt2, sq, svq, sel = brute.scan_c2(c["v"], c["d"], c["q"], d0=0, vlo=0.0, vhi=0.5, qmax=40)
+    idx = [i for i in range(n) if 0 <= c["d"][i] < 365 and 0.0 <= c["v"][i] <= 0.5 and
+           c["q"][i] < 40]
+    assert cnt2 == len(idx) and list(sel) == idx
+    assert sq == sum(int(c["q"][i]) for i in idx)
+
+
+def test_cjit_matches_brute_on_workloads():
+    n = 50_000
+    c = datagen.scan_table(n, 5, with_q=True, rowid_dtype=np.int32)
+    run = cjit.c2_query(d0=100, vlo=0.0, vhi=0.3, qmax=30)
+    cnt, sq, svq, sel = run(c["rowid"], c["v"], c["d"], c["q"])
+    w = brute.scan_c2(c["v"], c["d"], c["q"], d0=100, vlo=0.0, vhi=0.3, qmax=30, rowid=c["rowid"])
+    assert (cnt, sq) == w[:2] and brute.rel_err(svq, w[2]) <= 1e-13
+    np.testing.assert_array_equal(sel, w[3])
+    c1 = datagen.scan_table(n, 6, with_q=False)
+    (st,) = cjit.compile_op(workloads.scan_agg_c1())({"cols": [c1["rowid"], c1["k"], c1["v"], c1["d"]]})
+    w1 = brute.scan_c1(c1["k"], c1["v"], c1["d"])
+    assert st[0] == w1[0] and brute.rel_err(st[1], w1[1]) <= 1e-13
+
+
+def test_cjit_prints_float_constants_like_cemitter():
+    # std::to_string keeps 6 decimals (src/pass/cemitter.cpp:101).
+    from reffine_amd.reffiner import _f64_t
+    assert cjit.cpp_const(_f64_t, 0.05) == "double(0.050000)"
+    assert cjit.cpp_const(_f64_t, 1e-7) == "double(0.000000)"
+
+
+def test_detorder_integers_exact_and_fp_close():
+    rng = np.random.default_rng(0)
+    n = 100_000
+    x = rng.integers(0, 1000, n).astype(np.int64)
+    p = rng.random(n) < 0.3
+    assert int(detorder.det_sum(x, p, 0, n, 256, 4, 4)) == int(x[p].sum())
+    xf = rng.random(n)
+    got = float(detorder.det_sum(xf, p, 0, n, 256, 4, 4))
+    assert brute.rel_err(got, brute.fsum(xf[p])) < 1e-14
+    # a deliberately wrong order (reversed rows) must not be bit-identical
+    # for typical data, while a single-element input is exact in any order
+    one = np.zeros(n)
+    one[123] = 0.1
+    assert float(detorder.det_sum(one, np.ones(n, bool), 0, n, 256, 4, 4)) == 0.1
+    # unaligned start and tail handling
+    got = float(detorder.det_sum(xf, p, 3, n - 5, 256, 4, 4))
+    assert brute.rel_err(got, brute.fsum(xf[3:n - 5][p[3:n - 5]])) < 1e-14
+
+
+def test_datagen_is_a_pure_function_of_seed_col_row():
+    a = datagen.uniform(3, 2, 1000)
+    b = datagen.uniform(3, 2, 600, start=400)
+    np.testing.assert_array_equal(a[400:], b)
+    assert not np.array_equal(a, datagen.uniform(4, 2, 1000))
+    k = datagen.randint(1, 1, 100_000, 0, 1000, np.int64)
+    assert k.min() == 0 and k.max() == 999
+    g = datagen.group_table(100_000)
+    f1 = np.bincount(g["f1"]) / 100_000
+    assert abs(f1[0] - 0.25) < 0.01 and abs(f1[2] - 0.5) < 0.01
