@@ -33,6 +33,30 @@ def scan_c1(k, v, d, lo=365, hi=730, vmax=0.5):
     return int(m.sum()), fsum(v[m] * k[m].astype(np.float64))
 
 
+def group_c3_contribs(c):
+    """Per-row values of the grouped aggregate, each one IEEE operation as in
+    the C text: qty, price, price*(1-disc), (price*(1-disc))*(1+tax), disc."""
+    disc_price = c["price"] * (1.0 - c["disc"])
+    charge = disc_price * (1.0 + c["tax"])
+    return [c["qty"], c["price"], disc_price, charge, c["disc"]]
+
+
+def group_c3(c, dmax=2526, groups=((0, 0), (0, 1), (1, 0), (1, 1), (2, 0), (2, 1))):
+    """Per group (f1, f2): SUM(qty) i64, SUM(price), SUM(price*(1-disc)),
+    SUM(price*(1-disc)*(1+tax)), SUM(disc) f64, COUNT i64 over d <= dmax
+    (benchmark/include/tpch.h:412-478 idiom); flat list, group-major."""
+    keep = c["d"] <= dmax
+    vals = group_c3_contribs(c)
+    out = []
+    for a, b in groups:
+        m = keep & (c["f1"] == a) & (c["f2"] == b)
+        out.append(int(vals[0][m].sum(dtype=np.int64)))
+        for x in vals[1:]:
+            out.append(fsum(x[m]))
+        out.append(int(m.sum()))
+    return out
+
+
 def scan_c2(v, d, q, d0=1000, vlo=0.05, vhi=0.07, qmax=24, rowid=None):
     """COUNT, SUM(q), SUM(v*q) and the ascending selection vector."""
     m = (d >= d0) & (d < d0 + 365) & (v >= vlo) & (v <= vhi) & (q < qmax)

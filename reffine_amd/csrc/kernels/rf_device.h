@@ -278,11 +278,19 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
 
     Acc run;
     Q::acc_zero(run);
+    // Loop structure: every iteration starts with the ticket and ends with a
+    // barrier, and the tile index is made scalar, so the loop exit is a
+    // uniform scalar branch. Without the closing barrier the compiler may
+    // merge the thread-0 region at the end of one iteration with the
+    // thread-0 ticket at the start of the next, sending lane 0 and lanes
+    // 1..63 of wave 0 around different paths through the barriers.
     for (;;) {
         if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(&L.counters[0], 1u);
         __syncthreads();
-        const int64_t tile = s_tile;
-        __syncthreads();
+        const int64_t tile_v = s_tile;
+        const int64_t tile =
+            (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(tile_v >> 32)) << 32) |
+                      (uint32_t)__builtin_amdgcn_readfirstlane((int)tile_v));
         if (tile >= L.ntiles) break;
         const int64_t row0 = L.start + tile * TILE;
         Acc a;
@@ -337,8 +345,9 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
                 }
                 sbase += (total >> (16 * s)) & 0xffffull;
             }
-            __syncthreads();
         }
+        // Closes the tile: s_tile, s_prefix and s_scan may be rewritten next.
+        __syncthreads();
     }
 
     if constexpr (Q::NACC > 0 && !Q::DET) {

@@ -9,7 +9,7 @@ id, so its output column 0 is the int32 selection vector.
 """
 from __future__ import annotations
 
-from .reffiner import (_add, _cast, _f64, _f64_t, _func, _gte, _i32, _i32_t, _i64, _i64_t,
+from .reffiner import (_add, _cast, _eq, _f64, _f64_t, _func, _gte, _i8, _i8_t, _i32, _i32_t, _i64, _i64_t, _sub,
                        _idx, _in, _len, _lt, _lte, _mul, _new, _op, _red, _sel, _subvec, _sym,
                        _vec_t, _get)
 
@@ -82,3 +82,49 @@ def scan_select_c2(d0: int = 1000, vlo: float = 0.05, vhi: float = 0.07, qmax: i
 def c2_bytes_per_row() -> int:
     """Algorithmic bytes per scanned row: d (4) + v (8) + q (8)."""
     return 20
+
+
+# rowid, f1, f2, qty, price, disc, tax, d (2-dim: the fold reads columns 1..)
+C3_TYPE = _vec_t(2, _i64_t, _i8_t, _i8_t, _i64_t, _f64_t, _f64_t, _f64_t, _i32_t)
+C3_GROUPS = [(a, b) for a in range(3) for b in range(2)]
+C3_FIELDS = ["sum_qty", "sum_price", "sum_disc_price", "sum_charge", "sum_disc", "count"]
+
+
+def group_agg_c3(dmax: int = 2526):
+    """Low-cardinality grouped aggregate in the TPC-H Q1 idiom
+    (benchmark/include/tpch.h:412-478): one unrolled
+    `_sel(f1 == a & f2 == b & d <= dmax, s + x, s)` per (group, field).
+    State: for each of the 6 groups (f1, f2) in order (0,0), (0,1), ...,
+    (2,1): SUM(qty) i64, SUM(price), SUM(price*(1-disc)),
+    SUM(price*(1-disc)*(1+tax)), SUM(disc) f64, COUNT i64. AVG(qty),
+    AVG(price), AVG(disc) are the sums over COUNT."""
+    T = _sym("T", C3_TYPE)
+
+    def acc(s, v):
+        f1, f2, qty, price, disc, tax, d = (_get(v, i) for i in range(7))
+        disc_price = _mul(price, _sub(_f64(1), disc))
+        charge = _mul(disc_price, _add(_f64(1), tax))
+        keep = _lte(d, _i32(dmax))
+        vals = [qty, price, disc_price, charge, disc, _i64(1)]
+        out = []
+        for g, (a, b) in enumerate(C3_GROUPS):
+            p = keep & _eq(f1, _i8(a)) & _eq(f2, _i8(b))
+            for k, x in enumerate(vals):
+                sk = _get(s, g * len(vals) + k)
+                out.append(_sel(p, _add(sk, x), sk))
+        return _new(out)
+
+    def init():
+        return _new([_i64(0) if k in (0, 5) else _f64(0)
+                     for _ in C3_GROUPS for k in range(len(C3_FIELDS))])
+
+    red = _red(_subvec(T, _idx(0), _len(T, 1)), init, acc)
+    rs = _sym("q1", red)
+    fn = _func("group_c3", rs, [T])
+    fn.tbl[rs] = red
+    return fn
+
+
+def c3_bytes_per_row() -> int:
+    """f1 (1) + f2 (1) + qty (8) + price (8) + disc (8) + tax (8) + d (4)."""
+    return 38
