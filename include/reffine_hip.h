@@ -198,6 +198,12 @@ int rf_query_time(rf_query* q, const rf_arg* args, int64_t n_args,
 
 int rf_query_free(rf_query* q);
 
+/* Times a plain 16-bytes-per-lane device copy of `bytes` (read + write)
+ * over `iters` launches; the measured HBM ceiling the scan kernels are
+ * compared with, and the reference size for calibrating byte counters. No
+ * reference counterpart. */
+int rf_calib_copy(int64_t bytes, int32_t iters, double* ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -243,6 +243,37 @@ int rf_query_time(rf_query* q, const rf_arg* args, int64_t n_args, int32_t iters
     });
 }
 
+int rf_calib_copy(int64_t bytes, int32_t iters, double* ms_per_copy) {
+    return guard([&] {
+        need(ms_per_copy && bytes >= 16 && iters > 0, "bad argument");
+        rf::Device& d = rf::device();
+        int64_t n16 = bytes / 16;
+        void* src = rf::dev_alloc((size_t)n16 * 16);
+        void* dst = nullptr;
+        try {
+            dst = rf::dev_alloc((size_t)n16 * 16);
+        } catch (...) {
+            rf::dev_free(src);
+            throw;
+        }
+        hipEvent_t e0, e1;
+        RF_HIP(hipEventCreate(&e0));
+        RF_HIP(hipEventCreate(&e1));
+        RF_HIP(rf::aot_copy(src, dst, n16, d.stream));
+        RF_HIP(hipEventRecord(e0, d.stream));
+        for (int i = 0; i < iters; i++) RF_HIP(rf::aot_copy(src, dst, n16, d.stream));
+        RF_HIP(hipEventRecord(e1, d.stream));
+        RF_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        RF_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        rf::dev_free(src);
+        rf::dev_free(dst);
+        *ms_per_copy = ms / iters;
+    });
+}
+
 int rf_query_free(rf_query* q) {
     return guard([&] { delete q; });
 }

@@ -273,31 +273,37 @@ std::string fold_identity(AggOp op, const Type& t) {
     throw IRError("no identity for fold");
 }
 
-static bool collect_driver_cols(const N& e, const Plan& plan, std::set<int>& widths) {
-    if (!e) return false;
+static void collect_driver_cols(const N& e, const Plan& plan, std::map<int, int>& cols) {
+    if (!e) return;
     if (is_leaf(e, "$col.")) {
         auto parts = split(leaf_name(e), '.');
-        if (std::stoi(parts[1]) == plan.driver) widths.insert(e->type.width());
+        if (std::stoi(parts[1]) == plan.driver) cols[std::stoi(parts[2])] = e->type.width();
     }
-    for (auto& a : e->args) collect_driver_cols(a, plan, widths);
-    return true;
+    for (auto& a : e->args) collect_driver_cols(a, plan, cols);
 }
 
 Tuning default_tuning(const Plan& plan) {
-    std::set<int> widths;
+    std::map<int, int> cols;  // driver column -> width, loaded per row
     for (auto& r : plan.reduces)
         for (auto& f : r.fields)
             if (f.updated) {
-                collect_driver_cols(f.pred, plan, widths);
-                collect_driver_cols(f.contrib, plan, widths);
+                collect_driver_cols(f.pred, plan, cols);
+                collect_driver_cols(f.contrib, plan, cols);
             }
-    for (auto& o : plan.ops) collect_driver_cols(o.pred, plan, widths);
+    for (auto& o : plan.ops) collect_driver_cols(o.pred, plan, cols);
     Tuning t;
-    int wmin = widths.empty() ? 8 : *widths.begin();
+    int wmin = 8, row_bytes = 0;
+    for (auto& kv : cols) {
+        wmin = std::min(wmin, kv.second);
+        row_bytes += kv.second;
+    }
     t.items = wmin >= 8 ? 2 : (wmin >= 4 ? 4 : 8);
     if (plan.source == Plan::RANGE) t.items = 4;
-    // Tile of about 4096 rows; the selection scan packs SUB 16-bit counters.
+    // Tiles of about 4096 rows, with the loads of one tile (issued together)
+    // within about 512 bytes of registers per thread; the selection scan
+    // packs SUB 16-bit counters into 64 bits.
     t.sub = std::max(1, 4096 / (t.block * t.items));
+    if (row_bytes > 0) t.sub = std::max(1, std::min(t.sub, 512 / (t.items * row_bytes)));
     if (!plan.ops.empty()) t.sub = std::min(t.sub, 4);
     return t;
 }
@@ -369,6 +375,22 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
         o << " b.a" << i << " = rf_shfl_xor(a.a" << i << ", o);";
     o << "\n        acc_combine(a, b);\n    }\n";
 
+    // Vector-loaded columns become parameters of row_block.
+    std::string vparams, vargs, vargs_s;
+    for (size_t s = 0; s < ks.cols.size(); s++) {
+        const ColSlot& c = ks.cols[s];
+        if (!c.vector) continue;
+        const std::string T = cty(c.type), S = std::to_string(s);
+        vparams += ", const RfVec<" + T + ", ITEMS>& c" + S;
+        vargs += ", c" + S;
+        vargs_s += ", c" + S + "[s]";
+        if (c.nullable) {
+            vparams += ", uint32_t vb" + S;
+            vargs += ", vb" + S;
+            vargs_s += ", vb" + S + "[s]";
+        }
+    }
+    // rows(): one sub-step with bounds (edge tiles).
     o << "    template <bool FULL>\n"
       << "    static __device__ __forceinline__ uint32_t rows(Acc& a, const RfScanArgs& L, int64_t r) {\n";
     for (size_t s = 0; s < ks.cols.size(); s++) {
@@ -382,6 +404,35 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
             o << "        const uint32_t vb" << s << " = rf_valid_bits<ITEMS>(L.valid[" << s
               << "], r);\n";
     }
+    o << "        return row_block<FULL>(a, L, r" << vargs << ");\n    }\n";
+    // tile_full(): an interior tile, every sub-step's loads issued first.
+    o << "    static __device__ __forceinline__ void tile_full(Acc& a, const RfScanArgs& L, "
+         "int64_t row0, uint32_t* bits) {\n"
+      << "        const int64_t rb = row0 + (int64_t)threadIdx.x * ITEMS;\n";
+    for (size_t s = 0; s < ks.cols.size(); s++) {
+        const ColSlot& c = ks.cols[s];
+        if (!c.vector) continue;
+        o << "        RfVec<" << cty(c.type) << ", ITEMS> c" << s << "[SUB];\n";
+        if (c.nullable) o << "        uint32_t vb" << s << "[SUB];\n";
+    }
+    o << "#pragma unroll\n        for (int s = 0; s < SUB; s++) {\n"
+      << "            const int64_t r = rb + (int64_t)s * BLOCK * ITEMS;\n";
+    for (size_t s = 0; s < ks.cols.size(); s++) {
+        const ColSlot& c = ks.cols[s];
+        if (!c.vector) continue;
+        const std::string T = cty(c.type);
+        o << "            c" << s << "[s] = rf_ldv<" << T << ", ITEMS>((const " << T << "*)L.col["
+          << s << "] + r);\n";
+        if (c.nullable)
+            o << "            vb" << s << "[s] = rf_valid_bits<ITEMS>(L.valid[" << s << "], r);\n";
+    }
+    o << "        }\n#pragma unroll\n        for (int s = 0; s < SUB; s++) {\n"
+      << "            const int64_t r = rb + (int64_t)s * BLOCK * ITEMS;\n"
+      << "            bits[s] = row_block<true>(a, L, r" << vargs_s << ");\n        }\n    }\n";
+    // row_block(): the query's row code over ITEMS loaded rows.
+    o << "    template <bool FULL>\n"
+      << "    static __device__ __forceinline__ uint32_t row_block(Acc& a, const RfScanArgs& L, "
+         "int64_t r" << vparams << ") {\n";
     o << "        uint32_t sel = 0;\n";
     o << "#pragma unroll\n        for (int k = 0; k < ITEMS; k++) {\n";
     o << "            const int64_t row = r + k;\n";

@@ -197,12 +197,15 @@ __device__ __forceinline__ uint64_t rf_st_make(uint64_t flag, uint64_t epoch,
     return (flag << 62) | ((epoch & RF_ST_EPOCH_MASK) << 40) | (val & RF_ST_VAL_MASK);
 }
 
+// A status word carries flag, epoch and value in one 64-bit atomic and no
+// other data is published through it, so relaxed device-scope atomics
+// suffice (no L2 writeback / invalidate per tile).
 __device__ __forceinline__ void rf_st_store(uint64_t* p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ uint64_t rf_st_load(const uint64_t* p) {
-    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_ACQUIRE,
+    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -296,15 +299,21 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
         Acc a;
         Q::acc_zero(a);
         uint32_t bits[SUB];
+        if (row0 >= L.lo && row0 + TILE <= L.end) {
+            // Interior tile: all SUB sub-steps' loads are issued before any
+            // row is evaluated.
+            Q::tile_full(a, L, row0, bits);
+        } else {
 #pragma unroll
-        for (int s = 0; s < SUB; s++) {
-            const int64_t r = row0 + ((int64_t)s * BLOCK + threadIdx.x) * ITEMS;
-            if (r >= L.lo && r + ITEMS <= L.end)
-                bits[s] = Q::template rows<true>(a, L, r);
-            else if (r + ITEMS > L.lo && r < L.end)
-                bits[s] = Q::template rows<false>(a, L, r);
-            else
-                bits[s] = 0;
+            for (int s = 0; s < SUB; s++) {
+                const int64_t r = row0 + ((int64_t)s * BLOCK + threadIdx.x) * ITEMS;
+                if (r >= L.lo && r + ITEMS <= L.end)
+                    bits[s] = Q::template rows<true>(a, L, r);
+                else if (r + ITEMS > L.lo && r < L.end)
+                    bits[s] = Q::template rows<false>(a, L, r);
+                else
+                    bits[s] = 0;
+            }
         }
         if constexpr (Q::NACC > 0) {
             if constexpr (Q::DET) {

@@ -114,6 +114,8 @@ _SIGS = [
     ("rf_query_time", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RfArg), ctypes.c_int64,
                                      ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
     ("rf_query_free", ctypes.c_int, [ctypes.c_void_p]),
+    ("rf_calib_copy", ctypes.c_int, [ctypes.c_int64, ctypes.c_int32,
+                                     ctypes.POINTER(ctypes.c_double)]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]
@@ -426,6 +428,13 @@ class Query:
 def compile_op(func: Func, vectorize: bool = False, deterministic: bool = False) -> Query:
     """compile_op<T>(func, vectorize) (include/reffine/utils/utils.h:47-54)."""
     return Query(func, vectorize, deterministic)
+
+
+def calib_copy(nbytes: int, iters: int = 20) -> float:
+    """Mean ms of a plain device copy of nbytes (read nbytes + write nbytes)."""
+    ms = ctypes.c_double()
+    _ck(lib().rf_calib_copy(nbytes, iters, ctypes.byref(ms)))
+    return ms.value
 
 
 def compile_check(func: Func, deterministic: bool = False, nullable: bool = False) -> str:
