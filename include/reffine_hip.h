@@ -180,6 +180,13 @@ int rf_compile_check(const char* ir_text, uint32_t flags, char* buf,
 int rf_query_output_info(const rf_query* q, int64_t* out_bytes,
                          int32_t* out_is_table);
 
+/* Launch shape of the query's scan kernel: threads per block, rows per
+ * thread per sub-step (ITEMS) and sub-steps per tile (SUB). In the
+ * deterministic mode these fix the reduction order (tiles of
+ * SUB*BLOCK*ITEMS rows). No reference counterpart. */
+int rf_query_tuning(const rf_query* q, int32_t* block, int32_t* items,
+                    int32_t* sub);
+
 /* Human-readable lowering summary (kernel kind, columns, fields). */
 int rf_query_describe(const rf_query* q, char* buf, size_t cap);
 

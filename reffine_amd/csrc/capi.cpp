@@ -214,6 +214,15 @@ int rf_query_output_info(const rf_query* q, int64_t* out_bytes, int32_t* out_is_
     });
 }
 
+int rf_query_tuning(const rf_query* q, int32_t* block, int32_t* items, int32_t* sub) {
+    return guard([&] {
+        need(q != nullptr, "query is NULL");
+        if (block) *block = q->tune.block;
+        if (items) *items = q->tune.items;
+        if (sub) *sub = q->tune.sub;
+    });
+}
+
 int rf_query_describe(const rf_query* q, char* buf, size_t cap) {
     return guard([&] {
         need(q && buf && cap, "NULL argument");

@@ -109,6 +109,9 @@ _SIGS = [
     ("rf_query_output_info", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
                                             ctypes.POINTER(ctypes.c_int32)]),
     ("rf_query_describe", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]),
+    ("rf_query_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.POINTER(ctypes.c_int32)]),
     ("rf_query_run", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RfArg), ctypes.c_int64,
                                     ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     ("rf_query_time", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RfArg), ctypes.c_int64,
@@ -411,6 +414,13 @@ class Query:
         ms = ctypes.c_double()
         _ck(lib().rf_query_time(self.handle, arr, len(inputs), iters, ctypes.byref(ms)))
         return ms.value
+
+    def tuning(self) -> tuple[int, int, int]:
+        """(BLOCK, ITEMS, SUB) of the scan kernel; with deterministic=True
+        they fix the reduction order."""
+        b, i, s = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _ck(lib().rf_query_tuning(self.handle, ctypes.byref(b), ctypes.byref(i), ctypes.byref(s)))
+        return b.value, i.value, s.value
 
     def describe(self) -> str:
         buf = ctypes.create_string_buffer(1 << 20)

@@ -36,13 +36,14 @@ def test_c3_deterministic_bit_exact(device):
     c, t = c3_table(n, seed=8)
     q = compile_op(workloads.group_agg_c3(), deterministic=True)
     got = q(t)
+    block, items, sub = q.tuning()
     keep = c["d"] <= 2526
     vals = brute.group_c3_contribs(c)
     i = 0
     for a, b in workloads.C3_GROUPS:
         m = keep & (c["f1"] == a) & (c["f2"] == b)
         for x in vals + [np.ones(n, np.int64)]:
-            want = detorder.det_sum(x, m, 0, n, 256, 8, 2)
+            want = detorder.det_sum(x, m, 0, n, block, items, sub)
             assert got[i] == want.item(), (i, got[i], want)
             i += 1
     assert q(t) == got
