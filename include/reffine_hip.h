@@ -167,6 +167,12 @@ int rf_table_free(rf_table* t);
 #define RF_COMPILE_DETERMINISTIC 1u
 int rf_compile(const char* ir_text, uint32_t flags, rf_query** out);
 
+/* rf_compile with an explicit launch shape `tuning` = {BLOCK (64..1024,
+ * multiple of 64), ITEMS (1, 2, 4, 8 or 16), SUB (1..64; at most 4 for a
+ * query with a selection)}: the autotuner's entry point. */
+int rf_compile_tuned(const char* ir_text, uint32_t flags,
+                     const int32_t* tuning, rf_query** out);
+
 /* Lowers and JIT-compiles a Func for gfx950 without a device (no module
  * load): the build-time check that a query shape is accepted. `flags` bit 0
  * as rf_compile; bit 1 compiles the variant with validity bitmaps on every

@@ -157,30 +157,49 @@ int rf_table_free(rf_table* t) {
     return guard([&] { rf::table_free(t); });
 }
 
+static void compile_impl(const char* ir_text, uint32_t flags, const int32_t* tuning,
+                         rf_query** out) {
+    need(ir_text && out, "NULL argument");
+    rf::device();
+    rf::N f = rf::parse_func(ir_text);
+    auto q = std::make_unique<rf_query>();
+    q->plan = rf::lower_func(f, false);
+    q->det = (flags & RF_COMPILE_DETERMINISTIC) != 0;
+    q->tune = rf::default_tuning(q->plan);
+    if (tuning) {
+        rf::Tuning t{tuning[0], tuning[1], tuning[2]};
+        need(t.block >= 64 && t.block <= 1024 && t.block % 64 == 0, "block must be 64..1024, x64");
+        need(t.items == 1 || t.items == 2 || t.items == 4 || t.items == 8 || t.items == 16,
+             "items must be 1, 2, 4, 8 or 16");
+        need(t.sub >= 1 && t.sub <= 64, "sub must be 1..64");
+        need(q->plan.ops.empty() || t.sub <= 4, "a selection allows sub <= 4");
+        q->tune = t;
+    }
+    // Compile the all-valid specialisation now so compile errors surface
+    // here, as compile_op's do.
+    std::vector<std::vector<bool>> nl(q->plan.inputs.size());
+    for (size_t i = 0; i < q->plan.inputs.size(); i++)
+        nl[i].assign(q->plan.inputs[i]->type.dtypes.size(), false);
+    rf::Spec sp;
+    sp.ks = rf::gen_scan(q->plan, nl, q->tune, q->det);
+    sp.mod = &rf::jit_module(sp.ks);
+    std::string sig;
+    for (size_t i = 0; i < q->plan.inputs.size(); i++) {
+        if (!q->plan.inputs[i]->type.is_vector()) continue;
+        sig += std::string(q->plan.inputs[i]->type.dtypes.size(), '0') + "|";
+    }
+    q->specs.emplace(sig, std::move(sp));
+    *out = q.release();
+}
+
 int rf_compile(const char* ir_text, uint32_t flags, rf_query** out) {
+    return guard([&] { compile_impl(ir_text, flags, nullptr, out); });
+}
+
+int rf_compile_tuned(const char* ir_text, uint32_t flags, const int32_t* tuning, rf_query** out) {
     return guard([&] {
-        need(ir_text && out, "NULL argument");
-        rf::device();
-        rf::N f = rf::parse_func(ir_text);
-        auto q = std::make_unique<rf_query>();
-        q->plan = rf::lower_func(f, false);
-        q->det = (flags & RF_COMPILE_DETERMINISTIC) != 0;
-        q->tune = rf::default_tuning(q->plan);
-        // Compile the all-valid specialisation now so compile errors surface
-        // here, as compile_op's do.
-        std::vector<std::vector<bool>> nl(q->plan.inputs.size());
-        for (size_t i = 0; i < q->plan.inputs.size(); i++)
-            nl[i].assign(q->plan.inputs[i]->type.dtypes.size(), false);
-        rf::Spec sp;
-        sp.ks = rf::gen_scan(q->plan, nl, q->tune, q->det);
-        sp.mod = &rf::jit_module(sp.ks);
-        std::string sig;
-        for (size_t i = 0; i < q->plan.inputs.size(); i++) {
-            if (!q->plan.inputs[i]->type.is_vector()) continue;
-            sig += std::string(q->plan.inputs[i]->type.dtypes.size(), '0') + "|";
-        }
-        q->specs.emplace(sig, std::move(sp));
-        *out = q.release();
+        need(tuning != nullptr, "tuning is NULL");
+        compile_impl(ir_text, flags, tuning, out);
     });
 }
 

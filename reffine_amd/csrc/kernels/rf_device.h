@@ -275,26 +275,32 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
     constexpr int64_t TILE = (int64_t)SUB * BLOCK * ITEMS;
     __shared__ Acc s_waves[BLOCK / 64];
     __shared__ uint64_t s_scan[BLOCK / 64];
-    __shared__ int64_t s_tile;
+    __shared__ int64_t s_tile[2];
     __shared__ uint64_t s_prefix;
     __shared__ uint32_t s_last;
 
     Acc run;
     Q::acc_zero(run);
-    // Loop structure: every iteration starts with the ticket and ends with a
-    // barrier, and the tile index is made scalar, so the loop exit is a
-    // uniform scalar branch. Without the closing barrier the compiler may
-    // merge the thread-0 region at the end of one iteration with the
-    // thread-0 ticket at the start of the next, sending lane 0 and lanes
-    // 1..63 of wave 0 around different paths through the barriers.
-    for (;;) {
-        if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(&L.counters[0], 1u);
-        __syncthreads();
-        const int64_t tile_v = s_tile;
+    // Loop structure: every iteration ends with a barrier and the tile index
+    // is made scalar, so the loop exit is a uniform scalar branch. Without
+    // the closing barrier the compiler may merge the thread-0 region at the
+    // end of one iteration with the thread-0 region at the start of the
+    // next, sending lane 0 and lanes 1..63 of wave 0 around different paths
+    // through the barriers.
+    // The next tile's ticket is fetched while the current tile is processed
+    // (its latency hides behind the tile's loads). Tickets alternate between
+    // two slots: iteration i reads slot i&1 and writes slot (i+1)&1, and the
+    // closing barrier orders every read before the slot is written again.
+    if (threadIdx.x == 0) s_tile[0] = (int64_t)atomicAdd(&L.counters[0], 1u);
+    __syncthreads();
+    for (int par = 0;; par ^= 1) {
+        const int64_t tile_v = s_tile[par];
         const int64_t tile =
             (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(tile_v >> 32)) << 32) |
                       (uint32_t)__builtin_amdgcn_readfirstlane((int)tile_v));
         if (tile >= L.ntiles) break;
+        uint32_t next_ticket = 0;
+        if (threadIdx.x == 0) next_ticket = atomicAdd(&L.counters[0], 1u);
         const int64_t row0 = L.start + tile * TILE;
         Acc a;
         Q::acc_zero(a);
@@ -355,7 +361,9 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
                 sbase += (total >> (16 * s)) & 0xffffull;
             }
         }
-        // Closes the tile: s_tile, s_prefix and s_scan may be rewritten next.
+        // Closes the tile: s_prefix and s_scan may be rewritten next; s_tile
+        // takes the prefetched ticket (read by all only after the barrier).
+        if (threadIdx.x == 0) s_tile[par ^ 1] = (int64_t)next_ticket;
         __syncthreads();
     }
 

@@ -104,6 +104,9 @@ _SIGS = [
     ("rf_table_free", ctypes.c_int, [ctypes.c_void_p]),
     ("rf_compile", ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint32,
                                   ctypes.POINTER(ctypes.c_void_p)]),
+    ("rf_compile_tuned", ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint32,
+                                        ctypes.POINTER(ctypes.c_int32),
+                                        ctypes.POINTER(ctypes.c_void_p)]),
     ("rf_compile_check", ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_char_p,
                                         ctypes.c_size_t]),
     ("rf_query_output_info", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
@@ -359,14 +362,19 @@ def _decode(buf: bytes, t: DataType, off: int = 0):
 class Query:
     """A compiled query: the GPU counterpart of compile_op's function pointer."""
 
-    def __init__(self, func: Func, vectorize: bool = False, deterministic: bool = False):
+    def __init__(self, func: Func, vectorize: bool = False, deterministic: bool = False,
+                 tuning: tuple[int, int, int] | None = None):
         self.func = func
         self.vectorize = vectorize
         self.deterministic = deterministic
         self.text = func.ir()
         h = ctypes.c_void_p()
         flags = RF_COMPILE_DETERMINISTIC if deterministic else 0
-        _ck(lib().rf_compile(self.text.encode(), flags, ctypes.byref(h)))
+        if tuning is None:
+            _ck(lib().rf_compile(self.text.encode(), flags, ctypes.byref(h)))
+        else:
+            tv = (ctypes.c_int32 * 3)(*tuning)
+            _ck(lib().rf_compile_tuned(self.text.encode(), flags, tv, ctypes.byref(h)))
         self.handle = h
         nb = ctypes.c_int64()
         it = ctypes.c_int32()
@@ -435,9 +443,11 @@ class Query:
             pass
 
 
-def compile_op(func: Func, vectorize: bool = False, deterministic: bool = False) -> Query:
-    """compile_op<T>(func, vectorize) (include/reffine/utils/utils.h:47-54)."""
-    return Query(func, vectorize, deterministic)
+def compile_op(func: Func, vectorize: bool = False, deterministic: bool = False,
+               tuning: tuple[int, int, int] | None = None) -> Query:
+    """compile_op<T>(func, vectorize) (include/reffine/utils/utils.h:47-54).
+    `tuning` = (BLOCK, ITEMS, SUB) overrides the default launch shape."""
+    return Query(func, vectorize, deterministic, tuning)
 
 
 def calib_copy(nbytes: int, iters: int = 20) -> float:
