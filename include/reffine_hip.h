@@ -204,10 +204,12 @@ int rf_query_describe(const rf_query* q, char* buf, size_t cap);
 int rf_query_run(rf_query* q, const rf_arg* args, int64_t n_args,
                  void* output_addr, rf_table** out_table);
 
-/* Times `iters` back-to-back launches of the query's main kernel with HIP
- * events on the library stream; writes the mean milliseconds per run. */
+/* Times `iters` back-to-back runs of the query's kernels with HIP events on
+ * the library stream (no host work): `ms_per_run` is the mean device time of
+ * a whole run (scan + selection gather), `ms_scan_kernel` (optional) that of
+ * the scan kernel alone. */
 int rf_query_time(rf_query* q, const rf_arg* args, int64_t n_args,
-                  int32_t iters, double* ms_per_run);
+                  int32_t iters, double* ms_per_run, double* ms_scan_kernel);
 
 int rf_query_free(rf_query* q);
 

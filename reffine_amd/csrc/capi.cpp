@@ -264,10 +264,12 @@ int rf_query_run(rf_query* q, const rf_arg* args, int64_t n_args, void* output_a
 }
 
 int rf_query_time(rf_query* q, const rf_arg* args, int64_t n_args, int32_t iters,
-                  double* ms_per_run) {
+                  double* ms_per_run, double* ms_scan_kernel) {
     return guard([&] {
         need(q && ms_per_run, "NULL argument");
-        *ms_per_run = rf::time_query(q, args, n_args, iters);
+        double scan = 0;
+        rf::time_query(q, args, n_args, iters, ms_per_run, &scan);
+        if (ms_scan_kernel) *ms_scan_kernel = scan;
     });
 }
 

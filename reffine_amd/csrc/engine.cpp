@@ -281,6 +281,7 @@ Module& jit_module(const KernelSource& ks) {
     auto m = std::make_unique<Module>();
     RF_HIP(hipModuleLoadData(&m->mod, code.data()));
     RF_HIP(hipModuleGetFunction(&m->fn, m->mod, ks.entry.c_str()));
+    RF_HIP(hipModuleGetFunction(&m->gather, m->mod, "rf_q_gather"));
     int nb = 0;
     RF_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, m->fn, ks.tune.block, 0));
     m->max_blocks_per_cu = std::max(1, nb);
@@ -623,9 +624,10 @@ rf_query::~rf_query() {
     rf::dev_free(partials);
     rf::dev_free(result);
     rf::dev_free(counters);
-    rf::dev_free(status);
+    rf::dev_free(tile_arrays);
     rf::dev_free(out_count);
     for (void* p : outs) rf::dev_free(p);
+    for (void* p : stages) rf::dev_free(p);
     if (host_stage) (void)hipHostFree(host_stage);
 }
 
@@ -635,17 +637,6 @@ void table_free(rf_table* t) {
     if (!t) return;
     for (void* p : t->owned) dev_free(p);
     delete t;
-}
-
-// Status words carry a 22-bit launch epoch; they are cleared when it wraps
-// so that a word left by an old launch can never match.
-uint64_t next_epoch(rf_query* q) {
-    q->epoch++;
-    if ((q->epoch & ((1ull << 22) - 1)) == 0) {
-        dev_fill_u64(q->status, 0, (int64_t)q->status_tiles);
-        q->epoch++;
-    }
-    return q->epoch;
 }
 
 Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/) {
@@ -734,37 +725,54 @@ Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/
     if (!q->out_count) q->out_count = dev_alloc(256);
     if (!q->host_stage) RF_HIP(hipHostMalloc(&q->host_stage, 8192, hipHostMallocDefault));
     if (ks.has_sel) {
-        if ((size_t)A.ntiles > q->status_tiles) {
-            dev_free(q->status);
-            q->status_tiles = std::max<size_t>(A.ntiles, 1024);
-            q->status = dev_alloc(q->status_tiles * 8);
-            dev_fill_u64(q->status, 0, (int64_t)q->status_tiles);
-            q->epoch = 0;
+        // Per-block staging regions cover every tile a block can own.
+        const int64_t stage_rows = ((A.ntiles + grid - 1) / grid) * tile;
+        const size_t stage_total = (size_t)std::max<int64_t>(stage_rows * grid, 1);
+        const size_t rows = (size_t)std::max<int64_t>(end - lo, 1);
+        if ((size_t)A.ntiles > q->tiles_cap) {
+            dev_free(q->tile_arrays);
+            q->tiles_cap = std::max<size_t>(A.ntiles, 1024);
+            q->tile_arrays = dev_alloc(q->tiles_cap * 3 * 8);
         }
-        size_t rows = (size_t)std::max<int64_t>(end - lo, 1);
-        if (rows > q->out_rows_cap || q->outs.size() != ks.out_types.size()) {
+        if (stage_total > q->stage_rows_cap || rows > q->out_rows_cap ||
+            q->outs.size() != ks.out_types.size()) {
             for (void* p : q->outs) dev_free(p);
+            for (void* p : q->stages) dev_free(p);
             q->outs.clear();
-            for (auto& t : ks.out_types) q->outs.push_back(dev_alloc(rows * t.width()));
+            q->stages.clear();
+            for (auto& t : ks.out_types) {
+                q->outs.push_back(dev_alloc(rows * t.width()));
+                q->stages.push_back(dev_alloc(stage_total * t.width()));
+            }
             q->out_rows_cap = rows;
+            q->stage_rows_cap = stage_total;
         }
-        for (size_t j = 0; j < q->outs.size(); j++) A.out[j] = q->outs[j];
-        A.tile_status = (uint64_t*)q->status;
-        A.epoch = next_epoch(q);
+        for (size_t j = 0; j < q->outs.size(); j++) {
+            A.out[j] = q->outs[j];
+            A.stage[j] = q->stages[j];
+        }
+        A.stage_rows = stage_rows;
+        A.tile_src = (int64_t*)q->tile_arrays;
+        A.tile_cnt = A.tile_src + q->tiles_cap;
+        A.tile_dst = A.tile_cnt + q->tiles_cap;
     }
     A.partials = q->partials;
     A.result = q->result;
     A.counters = (uint32_t*)q->counters;
     A.out_count = (int64_t*)q->out_count;
-    A.spin_limit = 1u << 24;
     return l;
 }
 
-void launch(rf_query* /*q*/, Launch& l, hipStream_t s) {
+void launch(rf_query* /*q*/, Launch& l, hipStream_t s, bool scan_only) {
     if (l.empty) return;
     void* params[] = {&l.args};
     RF_HIP(hipModuleLaunchKernel(l.spec->mod->fn, l.grid, 1, 1, l.spec->ks.tune.block, 1, 1, 0,
                                  s, params, nullptr));
+    if (l.spec->ks.has_sel && !scan_only) {
+        const int g = (int)std::min<int64_t>(l.args.ntiles, 8192);
+        RF_HIP(hipModuleLaunchKernel(l.spec->mod->gather, g, 1, 1, 256, 1, 1, 0, s, params,
+                                     nullptr));
+    }
 }
 
 static HostVal finish_reduce(const Plan& plan, const KernelSource& ks, size_t comp,
@@ -804,23 +812,15 @@ void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_add
     Device& d = device();
     Launch l = prepare(q, args, n_args, nullptr);
     const KernelSource& ks = l.spec->ks;
-    launch(q, l, d.stream);
+    launch(q, l, d.stream, false);
     char* stage = (char*)q->host_stage;
     if (!l.empty) {
         if (ks.acc_bytes)
             RF_HIP(hipMemcpyAsync(stage, q->result, ks.acc_bytes, hipMemcpyDeviceToHost, d.stream));
-        if (ks.has_sel) {
+        if (ks.has_sel)
             RF_HIP(hipMemcpyAsync(stage + 4096, q->out_count, 8, hipMemcpyDeviceToHost, d.stream));
-            RF_HIP(hipMemcpyAsync(stage + 4104, (char*)q->counters + 8, 4, hipMemcpyDeviceToHost,
-                                  d.stream));
-        }
     }
     RF_HIP(hipStreamSynchronize(d.stream));
-    if (!l.empty && ks.has_sel && *(uint32_t*)(stage + 4104) != 0) {
-        dev_fill_u64(q->counters, 0, 32);
-        RF_HIP(hipStreamSynchronize(d.stream));
-        throw HipError("selection look-back exceeded its wait bound; the run was abandoned");
-    }
     const Plan& plan = q->plan;
     EvalCtx c{&plan, args, n_args, {}, {}};
     for (auto& kv : plan.tbl) c.tbl[kv.first->name] = kv.second;
@@ -888,28 +888,28 @@ void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_add
     if (out_table) *out_table = sel;
 }
 
-double time_query(rf_query* q, const rf_arg* args, int64_t n_args, int iters) {
+void time_query(rf_query* q, const rf_arg* args, int64_t n_args, int iters, double* ms_all,
+                double* ms_scan) {
     std::lock_guard<std::mutex> lk(q->mu);
     Device& d = device();
     Launch l = prepare(q, args, n_args, nullptr);
-    if (l.empty) return 0.0;
-    hipEvent_t e0, e1;
-    RF_HIP(hipEventCreate(&e0));
-    RF_HIP(hipEventCreate(&e1));
-    launch(q, l, d.stream);  // warm
-    RF_HIP(hipEventRecord(e0, d.stream));
-    for (int i = 0; i < iters; i++) {
-        // Every launch needs a fresh epoch for the look-back status words.
-        if (l.spec->ks.has_sel) l.args.epoch = next_epoch(q);
-        launch(q, l, d.stream);
-    }
-    RF_HIP(hipEventRecord(e1, d.stream));
-    RF_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    RF_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return iters > 0 ? ms / iters : 0.0;
+    *ms_all = *ms_scan = 0.0;
+    if (l.empty || iters <= 0) return;
+    hipEvent_t e[4];
+    for (auto& x : e) RF_HIP(hipEventCreate(&x));
+    launch(q, l, d.stream, false);  // warm
+    RF_HIP(hipEventRecord(e[0], d.stream));
+    for (int i = 0; i < iters; i++) launch(q, l, d.stream, false);
+    RF_HIP(hipEventRecord(e[1], d.stream));
+    for (int i = 0; i < iters; i++) launch(q, l, d.stream, true);
+    RF_HIP(hipEventRecord(e[2], d.stream));
+    RF_HIP(hipEventSynchronize(e[2]));
+    float a = 0, b = 0;
+    RF_HIP(hipEventElapsedTime(&a, e[0], e[1]));
+    RF_HIP(hipEventElapsedTime(&b, e[1], e[2]));
+    for (auto& x : e) (void)hipEventDestroy(x);
+    *ms_all = a / iters;
+    *ms_scan = b / iters;
 }
 
 }  // namespace rf

@@ -93,7 +93,8 @@ namespace rf {
 
 struct Module {
     hipModule_t mod = nullptr;
-    hipFunction_t fn = nullptr;
+    hipFunction_t fn = nullptr;      // rf_q: the scan
+    hipFunction_t gather = nullptr;  // rf_q_gather: selection placement
     int max_blocks_per_cu = 1;
 };
 
@@ -119,12 +120,13 @@ struct rf_query {
     size_t partials_bytes = 0;
     void* result = nullptr;
     void* counters = nullptr;
-    void* status = nullptr;
-    size_t status_tiles = 0;
+    void* tile_arrays = nullptr;  // tile_src | tile_cnt | tile_dst
+    size_t tiles_cap = 0;
     void* out_count = nullptr;
-    std::vector<void*> outs;
+    std::vector<void*> outs;      // selection columns in output order
+    std::vector<void*> stages;    // per-block staging of the same columns
     size_t out_rows_cap = 0;
-    uint64_t epoch = 0;
+    size_t stage_rows_cap = 0;
     void* host_stage = nullptr;  // pinned: result Acc + count
     std::mutex mu;
     ~rf_query();
@@ -138,7 +140,6 @@ rf_table* table_wrap(int64_t dim, int64_t n_cols, int64_t n_rows, const char* co
 void table_build_index(rf_table* t);
 void table_free(rf_table* t);
 RfProbeDev probe_of(const rf_table* t);
-uint64_t next_epoch(rf_query* q);
 
 struct Launch {
     Spec* spec = nullptr;
@@ -150,11 +151,13 @@ struct Launch {
 
 // Prepares arguments and workspace for one run (no launch).
 Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* ok);
-void launch(rf_query* q, Launch& l, hipStream_t s);
+void launch(rf_query* q, Launch& l, hipStream_t s, bool scan_only);
 // Launch + download + host-side finish; returns the output.
 void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_addr,
                rf_table** out_table);
-double time_query(rf_query* q, const rf_arg* args, int64_t n_args, int iters);
+// Mean ms per run of all the query's kernels, and of its scan kernel alone.
+void time_query(rf_query* q, const rf_arg* args, int64_t n_args, int iters, double* ms_all,
+                double* ms_scan);
 
 // Output struct layout (natural alignment, ScalarPass field order).
 void c_layout(const Type& t, size_t& size, size_t& align);

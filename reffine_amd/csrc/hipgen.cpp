@@ -448,27 +448,36 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
     if (ks.has_sel) o << "            sel |= (uint32_t)(inb && " << sel_pred << ") << k;\n";
     o << "        }\n        return sel;\n    }\n";
 
-    // ---- emit(): selection outputs of one qualifying row
+    // ---- emit(): selection outputs of one qualifying row, into staging;
+    // move(): one staged row to its output position.
     o << "    static __device__ __forceinline__ void emit(const RfScanArgs& L, int64_t row, int64_t pos) {\n";
+    std::ostringstream mv;
     if (ks.has_sel) {
         std::ostringstream eb;
         Emitter ee(g, Mode::EMIT, eb, "        ");
         std::vector<std::string> vals;
         for (auto& v : plan.ops[0].values) vals.push_back(ee.expr(v));
+        if (vals.size() > 16) throw Unsupported("more than 16 output columns");
         o << eb.str();
         for (size_t j = 0; j < vals.size(); j++) {
-            const std::string T = cty(plan.ops[0].types[j]);
-            o << "        ((" << T << "*)L.out[" << j << "])[pos] = (" << T << ")" << vals[j]
+            const std::string T = cty(plan.ops[0].types[j]), J = std::to_string(j);
+            o << "        ((" << T << "*)L.stage[" << J << "])[pos] = (" << T << ")" << vals[j]
               << ";\n";
+            mv << "        ((" << T << "*)L.out[" << J << "])[dst] = ((const " << T << "*)L.stage["
+               << J << "])[src];\n";
             ks.out_types.push_back(plan.ops[0].types[j]);
         }
-        if (vals.size() > 16) throw Unsupported("more than 16 output columns");
     } else {
         o << "        (void)L; (void)row; (void)pos;\n";
+        mv << "        (void)L; (void)src; (void)dst;\n";
     }
-    o << "    }\n};\n\n";
+    o << "    }\n";
+    o << "    static __device__ __forceinline__ void move(const RfScanArgs& L, int64_t src, int64_t dst) {\n"
+      << mv.str() << "    }\n};\n\n";
     o << "extern \"C\" __global__ __launch_bounds__(" << tune.block
       << ") void rf_q(RfScanArgs L) { rf_scan<Q>(L); }\n";
+    o << "extern \"C\" __global__ __launch_bounds__(256) void rf_q_gather(RfScanArgs L) { "
+         "rf_gather<Q>(L); }\n";
     ks.code = o.str();
     return ks;
 }

@@ -47,11 +47,14 @@ struct RfScanArgs {
     struct RfProbeDev probe[RF_MAXPROBES];
     void* partials;       // Acc per tile (deterministic) or per block
     void* result;         // final Acc
-    uint32_t* counters;   // [0] tile ticket, [1] finished blocks, [2] error
-    uint64_t* tile_status;
-    uint64_t epoch;
-    uint32_t spin_limit;  // bound on look-back polls (RF_SPIN_LIMIT)
-    uint32_t pad0;
+    uint32_t* counters;   // [1] finished blocks (reset by the last block)
+    // Selection: staging (per-block regions of stage_rows rows), per-tile
+    // staging offset / count / output offset, final columns and count.
+    int64_t stage_rows;
+    int64_t* tile_src;
+    int64_t* tile_cnt;
+    int64_t* tile_dst;
+    void* stage[RF_MAXOUT];
     void* out[RF_MAXOUT];
     int64_t* out_count;
     union RfParam p[RF_MAXPARAMS];

@@ -3,10 +3,11 @@
 // This header is compiled by hiprtc together with a generated `struct Q`
 // that supplies the per-row code of one query (column loads, predicate,
 // per-field contributions, selection outputs). Everything that is not
-// row-specific lives here: the dynamic tile scheduler, 128-bit coalesced
+// row-specific lives here: the static tile schedule, 128-bit coalesced
 // column loads, the fixed-order wave/block/grid reductions, the ordered
-// stream compaction (ballot/popcount + decoupled look-back) and the hash
-// probe of reffine's build_index (include/reffine/arrow/table.h:107-138).
+// stream compaction (per-block staging in tile order + a gather in tile
+// order) and the hash probe of reffine's build_index
+// (include/reffine/arrow/table.h:107-138).
 //
 // Row semantics follow the reference's generated loop
 // (src/pass/loopgen.cpp:206-269): a row contributes to state field f iff
@@ -184,90 +185,28 @@ __device__ __forceinline__ uint64_t rf_block_excl_scan(uint64_t x, uint64_t* s_t
     return wave_base + inc - x;
 }
 
-// ------------------------------------------- decoupled look-back status
-// Status word: [63:62] flag (1 aggregate, 2 inclusive prefix), [61:40]
-// launch epoch, [39:0] value.
-#define RF_ST_AGG 1ull
-#define RF_ST_PRE 2ull
-#define RF_ST_VAL_MASK ((1ull << 40) - 1)
-#define RF_ST_EPOCH_MASK ((1ull << 22) - 1)
-
-__device__ __forceinline__ uint64_t rf_st_make(uint64_t flag, uint64_t epoch,
-                                               uint64_t val) {
-    return (flag << 62) | ((epoch & RF_ST_EPOCH_MASK) << 40) | (val & RF_ST_VAL_MASK);
-}
-
-// A status word carries flag, epoch and value in one 64-bit atomic and no
-// other data is published through it, so relaxed device-scope atomics
-// suffice (no L2 writeback / invalidate per tile).
-__device__ __forceinline__ void rf_st_store(uint64_t* p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ uint64_t rf_st_load(const uint64_t* p) {
-    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Upper bound on polls of one predecessor's status word. Predecessors are
-// always resident (tiles are handed out in order by an atomic ticket), so
-// the bound is only a safety net: past it the wave stops waiting, raises
-// the error word and the host reports the run as failed.
-#define RF_SPIN_LIMIT (1u << 24)
-
-// Wave 0 computes the exclusive prefix of tile `tile` given its aggregate;
-// publishes aggregate then inclusive prefix. Returns the exclusive prefix
-// (valid in all lanes of wave 0).
-__device__ __forceinline__ uint64_t rf_lookback(uint64_t* status, int64_t tile,
-                                                uint64_t agg, uint64_t epoch,
-                                                uint32_t* err, uint32_t spin_limit) {
-    const uint32_t lane = threadIdx.x & 63;
-    const uint64_t ep = epoch & RF_ST_EPOCH_MASK;
-    if (tile == 0) {
-        if (lane == 0) rf_st_store(status, rf_st_make(RF_ST_PRE, ep, agg));
-        return 0;
-    }
-    if (lane == 0) rf_st_store(status + tile, rf_st_make(RF_ST_AGG, ep, agg));
-    uint64_t excl = 0;
-    int64_t base = tile - 1;
-    for (;;) {
-        const int64_t t = base - (int64_t)lane;
-        uint64_t st = 0;
-        uint64_t flag = 0;
-        if (t >= 0) {
-            for (uint32_t spins = 0;; spins++) {
-                st = rf_st_load(status + t);
-                flag = st >> 62;
-                if (flag != 0 && ((st >> 40) & RF_ST_EPOCH_MASK) == ep) break;
-                if (spins >= spin_limit) {
-                    atomicOr(err, 1u);
-                    flag = RF_ST_PRE;
-                    st = 0;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-        } else {
-            flag = RF_ST_PRE;  // before tile 0: prefix 0
-            st = 0;
-        }
-        const uint64_t val = (t >= 0) ? (st & RF_ST_VAL_MASK) : 0;
-        const uint64_t pre_mask = __ballot(flag == RF_ST_PRE);
-        // Closest predecessor holding an inclusive prefix.
-        const int stop = pre_mask ? __builtin_ctzll(pre_mask) : 64;
-        uint64_t contrib = ((int)lane <= stop) ? val : 0;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) contrib += __shfl_xor(contrib, o, 64);
-        excl += contrib;
-        if (pre_mask) break;
-        base -= 64;
-    }
-    if (lane == 0) rf_st_store(status + tile, rf_st_make(RF_ST_PRE, ep, excl + agg));
-    return excl;
-}
-
 // ------------------------------------------------------------ the kernel
 
+// Thread t's exclusive share of a chunked sum: thread t owns entries
+// [t*per, (t+1)*per) of `v`; returns the prefix of the entries before its
+// chunk and writes the total. Used by the completing block to scan tile
+// counts in tile order.
+template <int BLOCK>
+__device__ __forceinline__ uint64_t rf_chunk_prefix(const int64_t* v, int64_t n, int64_t per,
+                                                    uint64_t* s_tmp, uint64_t& total) {
+    uint64_t mine = 0;
+    const int64_t lo = (int64_t)threadIdx.x * per;
+    for (int64_t i = lo; i < lo + per && i < n; i++) mine += (uint64_t)v[i];
+    return rf_block_excl_scan<BLOCK>(mine, s_tmp, total);
+}
+
+// Scan kernel. Tiles are assigned statically (tile = blockIdx.x + k *
+// gridDim.x), so no tile waits on another and no atomics are taken per
+// tile. A selection is compacted per block: block b appends the selected
+// rows of its tiles, in tile order, to its own staging region
+// [b * stage_rows, (b + 1) * stage_rows) and records each tile's staging
+// offset and count; the completing block turns the counts into output
+// offsets (in tile order) and rf_gather moves the rows into place.
 template <class Q>
 __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
     typedef typename Q::Acc Acc;
@@ -275,32 +214,12 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
     constexpr int64_t TILE = (int64_t)SUB * BLOCK * ITEMS;
     __shared__ Acc s_waves[BLOCK / 64];
     __shared__ uint64_t s_scan[BLOCK / 64];
-    __shared__ int64_t s_tile[2];
-    __shared__ uint64_t s_prefix;
     __shared__ uint32_t s_last;
 
     Acc run;
     Q::acc_zero(run);
-    // Loop structure: every iteration ends with a barrier and the tile index
-    // is made scalar, so the loop exit is a uniform scalar branch. Without
-    // the closing barrier the compiler may merge the thread-0 region at the
-    // end of one iteration with the thread-0 region at the start of the
-    // next, sending lane 0 and lanes 1..63 of wave 0 around different paths
-    // through the barriers.
-    // The next tile's ticket is fetched while the current tile is processed
-    // (its latency hides behind the tile's loads). Tickets alternate between
-    // two slots: iteration i reads slot i&1 and writes slot (i+1)&1, and the
-    // closing barrier orders every read before the slot is written again.
-    if (threadIdx.x == 0) s_tile[0] = (int64_t)atomicAdd(&L.counters[0], 1u);
-    __syncthreads();
-    for (int par = 0;; par ^= 1) {
-        const int64_t tile_v = s_tile[par];
-        const int64_t tile =
-            (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(tile_v >> 32)) << 32) |
-                      (uint32_t)__builtin_amdgcn_readfirstlane((int)tile_v));
-        if (tile >= L.ntiles) break;
-        uint32_t next_ticket = 0;
-        if (threadIdx.x == 0) next_ticket = atomicAdd(&L.counters[0], 1u);
+    int64_t cursor = (int64_t)blockIdx.x * L.stage_rows;  // block-uniform
+    for (int64_t tile = blockIdx.x; tile < L.ntiles; tile += gridDim.x) {
         const int64_t row0 = L.start + tile * TILE;
         Acc a;
         Q::acc_zero(a);
@@ -330,40 +249,38 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
             }
         }
         if constexpr (Q::HAS_SEL) {
+            // Rows of a tile in ascending order are (sub-step, thread, item):
+            // one 16-bit counter per sub-step, packed, scanned across the
+            // block in a single pass.
             uint64_t packed = 0;
 #pragma unroll
             for (int s = 0; s < SUB; s++)
                 packed |= (uint64_t)__builtin_popcount(bits[s]) << (16 * s);
             uint64_t total;
             const uint64_t excl = rf_block_excl_scan<BLOCK>(packed, s_scan, total);
-            uint64_t tile_count = 0;
-#pragma unroll
-            for (int s = 0; s < SUB; s++) tile_count += (total >> (16 * s)) & 0xffffull;
-            if (threadIdx.x < 64) {
-                const uint64_t pre = rf_lookback(L.tile_status, tile, tile_count, L.epoch,
-                                                 &L.counters[2], L.spin_limit);
-                if (threadIdx.x == 0) s_prefix = pre;
-            }
-            __syncthreads();
-            const uint64_t prefix = s_prefix;
             uint64_t sbase = 0;
 #pragma unroll
             for (int s = 0; s < SUB; s++) {
                 uint32_t b = bits[s];
-                uint64_t pos = prefix + sbase + ((excl >> (16 * s)) & 0xffffull);
+                int64_t pos = cursor + (int64_t)(sbase + ((excl >> (16 * s)) & 0xffffull));
                 const int64_t r = row0 + ((int64_t)s * BLOCK + threadIdx.x) * ITEMS;
                 while (b) {
                     const int k = __builtin_ctz(b);
                     b &= b - 1;
-                    Q::emit(L, r + k, (int64_t)pos);
+                    Q::emit(L, r + k, pos);
                     pos++;
                 }
                 sbase += (total >> (16 * s)) & 0xffffull;
             }
+            if (threadIdx.x == 0) {
+                L.tile_src[tile] = cursor;
+                L.tile_cnt[tile] = (int64_t)sbase;
+            }
+            cursor += (int64_t)sbase;
         }
-        // Closes the tile: s_prefix and s_scan may be rewritten next; s_tile
-        // takes the prefetched ticket (read by all only after the barrier).
-        if (threadIdx.x == 0) s_tile[par ^ 1] = (int64_t)next_ticket;
+        // Closes the tile. Keeps every thread-0 region of one iteration apart
+        // from the next iteration's, so the compiler cannot thread them
+        // together around the barriers.
         __syncthreads();
     }
 
@@ -372,8 +289,9 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
         if (threadIdx.x == 0) reinterpret_cast<Acc*>(L.partials)[blockIdx.x] = run;
     }
 
-    // Completion: the last block to finish combines the partials, writes the
-    // selection count and resets the counters for the next launch.
+    // Completion: the last block to finish combines the partials, turns the
+    // tile counts into output offsets and resets the counter for the next
+    // launch.
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -392,14 +310,29 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
         rf_block_reduce<Q>(a, s_waves);
         if (threadIdx.x == 0) *reinterpret_cast<Acc*>(L.result) = a;
     }
-    if (threadIdx.x == 0) {
-        if constexpr (Q::HAS_SEL) {
-            uint64_t cnt = 0;
-            if (L.ntiles > 0) cnt = rf_st_load(L.tile_status + (L.ntiles - 1)) & RF_ST_VAL_MASK;
-            *L.out_count = (int64_t)cnt;
+    if constexpr (Q::HAS_SEL) {
+        const int64_t per = (L.ntiles + BLOCK - 1) / BLOCK;
+        uint64_t total;
+        uint64_t off = rf_chunk_prefix<BLOCK>(L.tile_cnt, L.ntiles, per, s_scan, total);
+        const int64_t lo = (int64_t)threadIdx.x * per;
+        for (int64_t i = lo; i < lo + per && i < L.ntiles; i++) {
+            L.tile_dst[i] = (int64_t)off;
+            off += (uint64_t)L.tile_cnt[i];
         }
-        L.counters[0] = 0;
+        if (threadIdx.x == 0) *L.out_count = (int64_t)total;
+    }
+    if (threadIdx.x == 0) {
         L.counters[1] = 0;
         __threadfence();
+    }
+}
+
+// Moves each tile's staged rows to its output offset (one tile per block
+// iteration, threads over the rows).
+template <class Q>
+__device__ __forceinline__ void rf_gather(const RfScanArgs& L) {
+    for (int64_t tile = blockIdx.x; tile < L.ntiles; tile += gridDim.x) {
+        const int64_t src = L.tile_src[tile], dst = L.tile_dst[tile], cnt = L.tile_cnt[tile];
+        for (int64_t i = threadIdx.x; i < cnt; i += blockDim.x) Q::move(L, src + i, dst + i);
     }
 }

@@ -118,7 +118,8 @@ _SIGS = [
     ("rf_query_run", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RfArg), ctypes.c_int64,
                                     ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     ("rf_query_time", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RfArg), ctypes.c_int64,
-                                     ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
+                                     ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double)]),
     ("rf_query_free", ctypes.c_int, [ctypes.c_void_p]),
     ("rf_calib_copy", ctypes.c_int, [ctypes.c_int64, ctypes.c_int32,
                                      ctypes.POINTER(ctypes.c_double)]),
@@ -418,10 +419,17 @@ class Query:
         return val
 
     def time(self, *inputs, iters: int = 20) -> float:
+        """Mean device ms of a run (all of the query's kernels)."""
+        return self.time_kernels(*inputs, iters=iters)[0]
+
+    def time_kernels(self, *inputs, iters: int = 20) -> tuple[float, float]:
+        """(mean device ms of a run, mean ms of the scan kernel alone), from
+        HIP events on the library stream."""
         arr = self._args(inputs)
-        ms = ctypes.c_double()
-        _ck(lib().rf_query_time(self.handle, arr, len(inputs), iters, ctypes.byref(ms)))
-        return ms.value
+        ms, scan = ctypes.c_double(), ctypes.c_double()
+        _ck(lib().rf_query_time(self.handle, arr, len(inputs), iters, ctypes.byref(ms),
+                                ctypes.byref(scan)))
+        return ms.value, scan.value
 
     def tuning(self) -> tuple[int, int, int]:
         """(BLOCK, ITEMS, SUB) of the scan kernel; with deterministic=True
