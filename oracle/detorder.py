@@ -2,14 +2,15 @@
 (test infrastructure).
 
 Order (kernels/rf_device.h, deterministic mode):
-  * rows [base, end) are cut into tiles of SUB*BLOCK*ITEMS rows; in
-    sub-step s thread t owns rows tile0 + (s*BLOCK + t)*ITEMS + [0, ITEMS);
-  * a thread folds its rows in ascending order from the identity:
+  * rows [base, end) are cut into wave-tiles of SUB*64*ITEMS rows; in
+    sub-step s lane l owns rows tile0 + (s*64 + l)*ITEMS + [0, ITEMS);
+  * a lane folds its rows in ascending order from the identity:
     acc = pred ? acc (+) x : acc;
-  * a wave combines lanes by xor-butterflies o = 1, 2, 4, ..., 32 with
-    lane_i <- lane_i (+) lane_{i^o}; waves are folded in order w0, w1, ...;
+  * the wave combines lanes by xor-butterflies o = 1, 2, 4, ..., 32 with
+    lane_i <- lane_i (+) lane_{i^o}; lane 0 holds the tile partial;
   * the completing block's thread t folds tile partials t, t+BLOCK, ... in
-    order, then the same wave and block combine gives the device total;
+    order; each wave combines its lanes by the same butterflies and the
+    block folds its waves in order w0, w1, ...;
   * the host returns init (+) total.
 numpy float64 arithmetic is IEEE binary64 with round-to-nearest, so this
 reproduces the kernel's result bit for bit (for + on FP64 and int64).
@@ -42,7 +43,7 @@ def det_sum(x: np.ndarray, pred: np.ndarray, lo: int, end: int, block: int, item
     x and pred index absolute rows (length >= end)."""
     dtype = x.dtype
     base = lo - (lo % items)
-    tile = sub * block * items
+    tile = sub * 64 * items
     ntiles = (end - base + tile - 1) // tile if end > lo else 0
     if ntiles == 0:
         return np.array(init, dtype=dtype) + np.zeros((), dtype)
@@ -53,13 +54,13 @@ def det_sum(x: np.ndarray, pred: np.ndarray, lo: int, end: int, block: int, item
     ok = (rows >= lo) & (rows < end)
     xs[ok] = x[rows[ok]]
     ps[ok] = pred[rows[ok]]
-    xs = xs.reshape(ntiles, sub, block, items)
-    ps = ps.reshape(ntiles, sub, block, items)
-    acc = np.zeros((ntiles, block), dtype=dtype)
+    xs = xs.reshape(ntiles, sub, 64, items)
+    ps = ps.reshape(ntiles, sub, 64, items)
+    acc = np.zeros((ntiles, 64), dtype=dtype)
     for s in range(sub):
         for k in range(items):
             acc = np.where(ps[:, s, :, k], acc + xs[:, s, :, k], acc)
-    parts = _block(acc, block)  # [ntiles]
+    parts = _butterfly(acc)  # [ntiles]
     # completing block: thread t folds parts t, t+block, ...
     nt = np.zeros(block, dtype=dtype)
     for i in range(0, ntiles, block):

@@ -16,13 +16,13 @@ _cache: dict[tuple, tuple[int, int, int]] = {}
 
 
 def candidates(has_selection: bool) -> list[tuple[int, int, int]]:
+    """Launch shapes to try: wave-tiles of SUB*64*ITEMS rows, up to 64 rows
+    per lane per tile."""
     out = []
     for block in (256, 512):
         for items in (2, 4, 8):
-            for sub in (1, 2, 4, 8):
-                if has_selection and sub > 4:
-                    continue
-                if sub * block * items > 16384:
+            for sub in (2, 4, 8, 16):
+                if items * sub > 64:
                     continue
                 out.append((block, items, sub))
     return out

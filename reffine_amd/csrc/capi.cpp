@@ -172,7 +172,6 @@ static void compile_impl(const char* ir_text, uint32_t flags, const int32_t* tun
         need(t.items == 1 || t.items == 2 || t.items == 4 || t.items == 8 || t.items == 16,
              "items must be 1, 2, 4, 8 or 16");
         need(t.sub >= 1 && t.sub <= 64, "sub must be 1..64");
-        need(q->plan.ops.empty() || t.sub <= 4, "a selection allows sub <= 4");
         q->tune = t;
     }
     // Compile the all-valid specialisation now so compile errors surface

@@ -624,7 +624,7 @@ rf_query::~rf_query() {
     rf::dev_free(partials);
     rf::dev_free(result);
     rf::dev_free(counters);
-    rf::dev_free(tile_arrays);
+    rf::dev_free(wave_arrays);
     rf::dev_free(out_count);
     for (void* p : outs) rf::dev_free(p);
     for (void* p : stages) rf::dev_free(p);
@@ -668,7 +668,7 @@ Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/
     RfScanArgs& A = l.args;
     std::memset(&A, 0, sizeof A);
     const int64_t items = ks.tune.items;
-    const int64_t tile = (int64_t)ks.tune.sub * ks.tune.block * items;
+    const int64_t tile = (int64_t)ks.tune.sub * 64 * items;  // rows per wave-tile
     int64_t base = lo >= 0 ? lo - (lo % items) : -(((-lo) + items - 1) / items) * items;
     A.start = base;
     A.lo = lo;
@@ -710,7 +710,8 @@ Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/
     int64_t grid = std::min<int64_t>(A.ntiles, (int64_t)d.num_cus * m.max_blocks_per_cu);
     if (grid < 1) grid = 1;
     l.grid = (int)grid;
-    size_t nparts = ks.det ? (size_t)std::max<int64_t>(A.ntiles, 1) : (size_t)grid;
+    const int64_t nw = grid * (ks.tune.block / 64);  // waves in the launch
+    size_t nparts = ks.det ? (size_t)std::max<int64_t>(A.ntiles, 1) : (size_t)nw;
     size_t need = std::max<size_t>(nparts * std::max(ks.acc_bytes, 8), 256);
     if (need > q->partials_bytes) {
         dev_free(q->partials);
@@ -725,14 +726,14 @@ Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/
     if (!q->out_count) q->out_count = dev_alloc(256);
     if (!q->host_stage) RF_HIP(hipHostMalloc(&q->host_stage, 8192, hipHostMallocDefault));
     if (ks.has_sel) {
-        // Per-block staging regions cover every tile a block can own.
-        const int64_t stage_rows = ((A.ntiles + grid - 1) / grid) * tile;
-        const size_t stage_total = (size_t)std::max<int64_t>(stage_rows * grid, 1);
+        // Per-wave staging regions cover every wave-tile a wave can own.
+        const int64_t stage_rows = ((A.ntiles + nw - 1) / nw) * tile;
+        const size_t stage_total = (size_t)std::max<int64_t>(stage_rows * nw, 1);
         const size_t rows = (size_t)std::max<int64_t>(end - lo, 1);
-        if ((size_t)A.ntiles > q->tiles_cap) {
-            dev_free(q->tile_arrays);
-            q->tiles_cap = std::max<size_t>(A.ntiles, 1024);
-            q->tile_arrays = dev_alloc(q->tiles_cap * 3 * 8);
+        if ((size_t)nw > q->waves_cap) {
+            dev_free(q->wave_arrays);
+            q->waves_cap = std::max<size_t>(nw, 1024);
+            q->wave_arrays = dev_alloc(q->waves_cap * 2 * 8);
         }
         if (stage_total > q->stage_rows_cap || rows > q->out_rows_cap ||
             q->outs.size() != ks.out_types.size()) {
@@ -752,9 +753,9 @@ Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/
             A.stage[j] = q->stages[j];
         }
         A.stage_rows = stage_rows;
-        A.tile_src = (int64_t*)q->tile_arrays;
-        A.tile_cnt = A.tile_src + q->tiles_cap;
-        A.tile_dst = A.tile_cnt + q->tiles_cap;
+        A.nwaves = nw;
+        A.wave_cnt = (int64_t*)q->wave_arrays;
+        A.wave_dst = A.wave_cnt + q->waves_cap;
     }
     A.partials = q->partials;
     A.result = q->result;
@@ -769,7 +770,7 @@ void launch(rf_query* /*q*/, Launch& l, hipStream_t s, bool scan_only) {
     RF_HIP(hipModuleLaunchKernel(l.spec->mod->fn, l.grid, 1, 1, l.spec->ks.tune.block, 1, 1, 0,
                                  s, params, nullptr));
     if (l.spec->ks.has_sel && !scan_only) {
-        const int g = (int)std::min<int64_t>(l.args.ntiles, 8192);
+        const int g = (int)std::min<int64_t>(l.args.nwaves, 8192);
         RF_HIP(hipModuleLaunchKernel(l.spec->mod->gather, g, 1, 1, 256, 1, 1, 0, s, params,
                                      nullptr));
     }

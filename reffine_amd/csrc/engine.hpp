@@ -120,8 +120,8 @@ struct rf_query {
     size_t partials_bytes = 0;
     void* result = nullptr;
     void* counters = nullptr;
-    void* tile_arrays = nullptr;  // tile_src | tile_cnt | tile_dst
-    size_t tiles_cap = 0;
+    void* wave_arrays = nullptr;  // wave_cnt | wave_dst
+    size_t waves_cap = 0;
     void* out_count = nullptr;
     std::vector<void*> outs;      // selection columns in output order
     std::vector<void*> stages;    // per-block staging of the same columns

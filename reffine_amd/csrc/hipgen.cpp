@@ -299,12 +299,10 @@ Tuning default_tuning(const Plan& plan) {
     }
     t.items = wmin >= 8 ? 2 : (wmin >= 4 ? 4 : 8);
     if (plan.source == Plan::RANGE) t.items = 4;
-    // Tiles of about 4096 rows, with the loads of one tile (issued together)
-    // within about 512 bytes of registers per thread; the selection scan
-    // packs SUB 16-bit counters into 64 bits.
-    t.sub = std::max(1, 4096 / (t.block * t.items));
-    if (row_bytes > 0) t.sub = std::max(1, std::min(t.sub, 512 / (t.items * row_bytes)));
-    if (!plan.ops.empty()) t.sub = std::min(t.sub, 4);
+    // Wave-tiles of SUB*64*ITEMS rows whose loads (issued together) fit in
+    // about 512 bytes of registers per lane.
+    t.sub = 8;
+    if (row_bytes > 0) t.sub = std::max(1, std::min(16, 512 / (t.items * row_bytes)));
     return t;
 }
 
@@ -317,7 +315,6 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
     if (tune.block % 64 != 0 || tune.block > 1024) throw IRError("block size");
     if (16 % tune.items != 0) throw IRError("ITEMS must divide 16");
     ks.has_sel = !plan.ops.empty();
-    if (ks.has_sel && tune.sub > 4) throw IRError("SUB > 4 with a selection");
 
     // Accumulator layout: one field per updated state field.
     for (size_t c = 0; c < plan.reduces.size(); c++)
@@ -407,8 +404,8 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
     o << "        return row_block<FULL>(a, L, r" << vargs << ");\n    }\n";
     // tile_full(): an interior tile, every sub-step's loads issued first.
     o << "    static __device__ __forceinline__ void tile_full(Acc& a, const RfScanArgs& L, "
-         "int64_t row0, uint32_t* bits) {\n"
-      << "        const int64_t rb = row0 + (int64_t)threadIdx.x * ITEMS;\n";
+         "int64_t row0, uint32_t lane, uint32_t* bits) {\n"
+      << "        const int64_t rb = row0 + (int64_t)lane * ITEMS;\n";
     for (size_t s = 0; s < ks.cols.size(); s++) {
         const ColSlot& c = ks.cols[s];
         if (!c.vector) continue;
@@ -416,7 +413,7 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
         if (c.nullable) o << "        uint32_t vb" << s << "[SUB];\n";
     }
     o << "#pragma unroll\n        for (int s = 0; s < SUB; s++) {\n"
-      << "            const int64_t r = rb + (int64_t)s * BLOCK * ITEMS;\n";
+      << "            const int64_t r = rb + (int64_t)s * 64 * ITEMS;\n";
     for (size_t s = 0; s < ks.cols.size(); s++) {
         const ColSlot& c = ks.cols[s];
         if (!c.vector) continue;
@@ -427,7 +424,7 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
             o << "            vb" << s << "[s] = rf_valid_bits<ITEMS>(L.valid[" << s << "], r);\n";
     }
     o << "        }\n#pragma unroll\n        for (int s = 0; s < SUB; s++) {\n"
-      << "            const int64_t r = rb + (int64_t)s * BLOCK * ITEMS;\n"
+      << "            const int64_t r = rb + (int64_t)s * 64 * ITEMS;\n"
       << "            bits[s] = row_block<true>(a, L, r" << vargs_s << ");\n        }\n    }\n";
     // row_block(): the query's row code over ITEMS loaded rows.
     o << "    template <bool FULL>\n"

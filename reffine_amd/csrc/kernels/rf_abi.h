@@ -45,15 +45,15 @@ struct RfScanArgs {
     const void* col[RF_MAXCOLS];
     const uint16_t* valid[RF_MAXCOLS];
     struct RfProbeDev probe[RF_MAXPROBES];
-    void* partials;       // Acc per tile (deterministic) or per block
+    void* partials;       // Acc per wave-tile (deterministic) or per wave
     void* result;         // final Acc
     uint32_t* counters;   // [1] finished blocks (reset by the last block)
-    // Selection: staging (per-block regions of stage_rows rows), per-tile
-    // staging offset / count / output offset, final columns and count.
+    // Selection: staging (per-wave regions of stage_rows rows), per-wave
+    // count and output offset, final columns and count.
     int64_t stage_rows;
-    int64_t* tile_src;
-    int64_t* tile_cnt;
-    int64_t* tile_dst;
+    int64_t nwaves;
+    int64_t* wave_cnt;
+    int64_t* wave_dst;
     void* stage[RF_MAXOUT];
     void* out[RF_MAXOUT];
     int64_t* out_count;

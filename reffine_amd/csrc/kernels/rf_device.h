@@ -3,27 +3,28 @@
 // This header is compiled by hiprtc together with a generated `struct Q`
 // that supplies the per-row code of one query (column loads, predicate,
 // per-field contributions, selection outputs). Everything that is not
-// row-specific lives here: the static tile schedule, 128-bit coalesced
-// column loads, the fixed-order wave/block/grid reductions, the ordered
-// stream compaction (per-block staging in tile order + a gather in tile
-// order) and the hash probe of reffine's build_index
-// (include/reffine/arrow/table.h:107-138).
+// row-specific lives here: the wave-level tile schedule, 128-bit coalesced
+// column loads, the fixed-order wave/grid reductions, wavefront
+// ballot/popcount stream compaction, and the hash probe of reffine's
+// build_index (include/reffine/arrow/table.h:107-138).
 //
 // Row semantics follow the reference's generated loop
 // (src/pass/loopgen.cpp:206-269): a row contributes to state field f iff
 // its predicate holds, and the field folds with +, *, min, max, && or ||.
 //
-// Fixed reduction order (the deterministic mode; independent of grid size
-// and of which block runs which tile):
-//   * a tile is SUB x BLOCK x ITEMS consecutive rows; in sub-step s thread
-//     t owns rows tile0 + (s*BLOCK + t)*ITEMS + [0, ITEMS);
-//   * each thread folds its rows in ascending order, starting from the
-//     fold's identity;
-//   * a wave combines its 64 lanes with xor-butterflies 1,2,4,...,32
-//     (lane value v_i <- v_i (+) v_{i^o}); the block folds its waves in
-//     order w0 (+) w1 (+) ...; the result is tile partial[tile];
+// Work unit: a wave-tile of SUB x 64 x ITEMS consecutive rows; in sub-step
+// s lane l owns rows tile0 + (s*64 + l)*ITEMS + [0, ITEMS). Each wave owns a
+// contiguous range of wave-tiles, so the steady state has no block barrier.
+//
+// Fixed reduction order (the deterministic mode; independent of the grid
+// and of which wave runs which tile):
+//   * each lane folds its rows of a tile in ascending order, starting from
+//     the fold's identity;
+//   * the wave combines its 64 lanes with xor-butterflies 1,2,4,...,32
+//     (lane value v_i <- v_i (+) v_{i^o}); lane 0's value is partial[tile];
 //   * the completing block: thread t folds partial[t], partial[t+BLOCK],
-//     ... in order, then the same wave/block combine gives the total.
+//     ... in order; each wave combines its lanes by the same butterflies and
+//     the block folds its waves in order w0 (+) w1 (+) ...
 #pragma once
 
 #include "rf_abi.h"
@@ -187,39 +188,48 @@ __device__ __forceinline__ uint64_t rf_block_excl_scan(uint64_t x, uint64_t* s_t
 
 // ------------------------------------------------------------ the kernel
 
-// Thread t's exclusive share of a chunked sum: thread t owns entries
-// [t*per, (t+1)*per) of `v`; returns the prefix of the entries before its
-// chunk and writes the total. Used by the completing block to scan tile
-// counts in tile order.
-template <int BLOCK>
-__device__ __forceinline__ uint64_t rf_chunk_prefix(const int64_t* v, int64_t n, int64_t per,
-                                                    uint64_t* s_tmp, uint64_t& total) {
-    uint64_t mine = 0;
-    const int64_t lo = (int64_t)threadIdx.x * per;
-    for (int64_t i = lo; i < lo + per && i < n; i++) mine += (uint64_t)v[i];
-    return rf_block_excl_scan<BLOCK>(mine, s_tmp, total);
+__device__ __forceinline__ int64_t rf_scalar64(int64_t v) {
+    return (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
+                     (uint32_t)__builtin_amdgcn_readfirstlane((int)v));
 }
 
-// Scan kernel. Tiles are assigned statically (tile = blockIdx.x + k *
-// gridDim.x), so no tile waits on another and no atomics are taken per
-// tile. A selection is compacted per block: block b appends the selected
-// rows of its tiles, in tile order, to its own staging region
-// [b * stage_rows, (b + 1) * stage_rows) and records each tile's staging
-// offset and count; the completing block turns the counts into output
-// offsets (in tile order) and rf_gather moves the rows into place.
+// Count of set bits of `mask` in lanes below this lane.
+__device__ __forceinline__ uint32_t rf_lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Scan kernel. Wave w (of W = gridDim.x * BLOCK/64) owns wave-tiles
+// [w*per, min((w+1)*per, ntiles)) with per = ceil(ntiles / W).
+//   * reductions fold into registers (per wave-tile partials in the
+//     deterministic mode);
+//   * a selection is compacted per wave with ballot/popcount: the wave
+//     appends its qualifying rows, in row order, to its staging region
+//     [w*stage_rows, (w+1)*stage_rows) and records its count; as waves own
+//     contiguous ascending ranges, the output is the concatenation of the
+//     waves' staged rows in wave order. The completing block scans the W
+//     counts into offsets and rf_gather moves each wave's rows into place.
 template <class Q>
 __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
     typedef typename Q::Acc Acc;
     constexpr int BLOCK = Q::BLOCK, ITEMS = Q::ITEMS, SUB = Q::SUB;
-    constexpr int64_t TILE = (int64_t)SUB * BLOCK * ITEMS;
-    __shared__ Acc s_waves[BLOCK / 64];
-    __shared__ uint64_t s_scan[BLOCK / 64];
+    constexpr int WAVES = BLOCK / 64;
+    constexpr int64_t TILE = (int64_t)SUB * 64 * ITEMS;
+    __shared__ Acc s_waves[WAVES];
+    __shared__ uint64_t s_scan[WAVES];
     __shared__ uint32_t s_last;
+
+    const uint32_t lane = __lane_id();
+    const int64_t gw = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+    const int64_t nw = (int64_t)gridDim.x * WAVES;
+    const int64_t per = (L.ntiles + nw - 1) / nw;
+    const int64_t t_begin = rf_scalar64(gw * per);
+    const int64_t t_end = rf_scalar64(t_begin + per < L.ntiles ? t_begin + per : L.ntiles);
 
     Acc run;
     Q::acc_zero(run);
-    int64_t cursor = (int64_t)blockIdx.x * L.stage_rows;  // block-uniform
-    for (int64_t tile = blockIdx.x; tile < L.ntiles; tile += gridDim.x) {
+    int64_t cursor = 0;  // rows this wave has staged (wave-uniform)
+    for (int64_t tile = t_begin; tile < t_end; tile++) {
         const int64_t row0 = L.start + tile * TILE;
         Acc a;
         Q::acc_zero(a);
@@ -227,11 +237,11 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
         if (row0 >= L.lo && row0 + TILE <= L.end) {
             // Interior tile: all SUB sub-steps' loads are issued before any
             // row is evaluated.
-            Q::tile_full(a, L, row0, bits);
+            Q::tile_full(a, L, row0, lane, bits);
         } else {
 #pragma unroll
             for (int s = 0; s < SUB; s++) {
-                const int64_t r = row0 + ((int64_t)s * BLOCK + threadIdx.x) * ITEMS;
+                const int64_t r = row0 + ((int64_t)s * 64 + lane) * ITEMS;
                 if (r >= L.lo && r + ITEMS <= L.end)
                     bits[s] = Q::template rows<true>(a, L, r);
                 else if (r + ITEMS > L.lo && r < L.end)
@@ -242,55 +252,53 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
         }
         if constexpr (Q::NACC > 0) {
             if constexpr (Q::DET) {
-                rf_block_reduce<Q>(a, s_waves);
-                if (threadIdx.x == 0) reinterpret_cast<Acc*>(L.partials)[tile] = a;
+                rf_wave_reduce<Q>(a);
+                if (lane == 0) reinterpret_cast<Acc*>(L.partials)[tile] = a;
             } else {
                 Q::acc_combine(run, a);
             }
         }
         if constexpr (Q::HAS_SEL) {
-            // Rows of a tile in ascending order are (sub-step, thread, item):
-            // one 16-bit counter per sub-step, packed, scanned across the
-            // block in a single pass.
-            uint64_t packed = 0;
-#pragma unroll
-            for (int s = 0; s < SUB; s++)
-                packed |= (uint64_t)__builtin_popcount(bits[s]) << (16 * s);
-            uint64_t total;
-            const uint64_t excl = rf_block_excl_scan<BLOCK>(packed, s_scan, total);
-            uint64_t sbase = 0;
+            // Rows of a sub-step in ascending order are (lane, item): the
+            // position of (lane, k) is the number of qualifying rows in lower
+            // lanes (one ballot per item) plus those of items < k in the lane.
+            const int64_t wave_base = gw * L.stage_rows;
 #pragma unroll
             for (int s = 0; s < SUB; s++) {
-                uint32_t b = bits[s];
-                int64_t pos = cursor + (int64_t)(sbase + ((excl >> (16 * s)) & 0xffffull));
-                const int64_t r = row0 + ((int64_t)s * BLOCK + threadIdx.x) * ITEMS;
-                while (b) {
-                    const int k = __builtin_ctz(b);
-                    b &= b - 1;
-                    Q::emit(L, r + k, pos);
-                    pos++;
+                const uint32_t b = bits[s];
+                uint32_t below = 0, total = 0;
+#pragma unroll
+                for (int k = 0; k < ITEMS; k++) {
+                    const uint64_t m = __ballot((b >> k) & 1u);
+                    below += rf_lanes_below(m);
+                    total += (uint32_t)__builtin_popcountll(m);
                 }
-                sbase += (total >> (16 * s)) & 0xffffull;
+                if (b) {
+                    const int64_t r = row0 + ((int64_t)s * 64 + lane) * ITEMS;
+                    int64_t pos = wave_base + cursor + below;
+                    uint32_t bb = b;
+                    while (bb) {
+                        const int k = __builtin_ctz(bb);
+                        bb &= bb - 1;
+                        Q::emit(L, r + k, pos);
+                        pos++;
+                    }
+                }
+                cursor += total;
             }
-            if (threadIdx.x == 0) {
-                L.tile_src[tile] = cursor;
-                L.tile_cnt[tile] = (int64_t)sbase;
-            }
-            cursor += (int64_t)sbase;
         }
-        // Closes the tile. Keeps every thread-0 region of one iteration apart
-        // from the next iteration's, so the compiler cannot thread them
-        // together around the barriers.
-        __syncthreads();
     }
 
+    if constexpr (Q::HAS_SEL) {
+        if (lane == 0) L.wave_cnt[gw] = cursor;
+    }
     if constexpr (Q::NACC > 0 && !Q::DET) {
-        rf_block_reduce<Q>(run, s_waves);
-        if (threadIdx.x == 0) reinterpret_cast<Acc*>(L.partials)[blockIdx.x] = run;
+        rf_wave_reduce<Q>(run);
+        if (lane == 0) reinterpret_cast<Acc*>(L.partials)[gw] = run;
     }
 
     // Completion: the last block to finish combines the partials, turns the
-    // tile counts into output offsets and resets the counter for the next
+    // wave counts into output offsets and resets the counter for the next
     // launch.
     __threadfence();
     __syncthreads();
@@ -302,7 +310,7 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
     if (!s_last) return;
     __threadfence();
     if constexpr (Q::NACC > 0) {
-        const int64_t nparts = Q::DET ? L.ntiles : (int64_t)gridDim.x;
+        const int64_t nparts = Q::DET ? L.ntiles : nw;
         const Acc* parts = reinterpret_cast<const Acc*>(L.partials);
         Acc a;
         Q::acc_zero(a);
@@ -311,13 +319,17 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
         if (threadIdx.x == 0) *reinterpret_cast<Acc*>(L.result) = a;
     }
     if constexpr (Q::HAS_SEL) {
-        const int64_t per = (L.ntiles + BLOCK - 1) / BLOCK;
+        // Exclusive scan of the nw wave counts: thread t takes a contiguous
+        // chunk of `pt` of them.
+        const int64_t pt = (nw + BLOCK - 1) / BLOCK;
+        const int64_t lo = (int64_t)threadIdx.x * pt;
+        uint64_t mine = 0;
+        for (int64_t i = lo; i < lo + pt && i < nw; i++) mine += (uint64_t)L.wave_cnt[i];
         uint64_t total;
-        uint64_t off = rf_chunk_prefix<BLOCK>(L.tile_cnt, L.ntiles, per, s_scan, total);
-        const int64_t lo = (int64_t)threadIdx.x * per;
-        for (int64_t i = lo; i < lo + per && i < L.ntiles; i++) {
-            L.tile_dst[i] = (int64_t)off;
-            off += (uint64_t)L.tile_cnt[i];
+        uint64_t off = rf_block_excl_scan<BLOCK>(mine, s_scan, total);
+        for (int64_t i = lo; i < lo + pt && i < nw; i++) {
+            L.wave_dst[i] = (int64_t)off;
+            off += (uint64_t)L.wave_cnt[i];
         }
         if (threadIdx.x == 0) *L.out_count = (int64_t)total;
     }
@@ -327,12 +339,11 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
     }
 }
 
-// Moves each tile's staged rows to its output offset (one tile per block
-// iteration, threads over the rows).
+// Moves each wave's staged rows to its output offset.
 template <class Q>
 __device__ __forceinline__ void rf_gather(const RfScanArgs& L) {
-    for (int64_t tile = blockIdx.x; tile < L.ntiles; tile += gridDim.x) {
-        const int64_t src = L.tile_src[tile], dst = L.tile_dst[tile], cnt = L.tile_cnt[tile];
+    for (int64_t w = blockIdx.x; w < L.nwaves; w += gridDim.x) {
+        const int64_t src = w * L.stage_rows, dst = L.wave_dst[w], cnt = L.wave_cnt[w];
         for (int64_t i = threadIdx.x; i < cnt; i += blockDim.x) Q::move(L, src + i, dst + i);
     }
 }
