@@ -168,8 +168,9 @@ int rf_table_free(rf_table* t);
 int rf_compile(const char* ir_text, uint32_t flags, rf_query** out);
 
 /* rf_compile with an explicit launch shape `tuning` = {BLOCK (64..1024,
- * multiple of 64), ITEMS (1, 2, 4, 8 or 16), SUB (1..64; at most 4 for a
- * query with a selection)}: the autotuner's entry point. */
+ * multiple of 64), ITEMS (1, 2, 4, 8 or 16), SUB (1..64), WAVES_PER_CU
+ * (1..64, 0 = 16)}: the autotuner's entry point. A wave-tile is
+ * SUB*64*ITEMS rows. */
 int rf_compile_tuned(const char* ir_text, uint32_t flags,
                      const int32_t* tuning, rf_query** out);
 
@@ -186,12 +187,12 @@ int rf_compile_check(const char* ir_text, uint32_t flags, char* buf,
 int rf_query_output_info(const rf_query* q, int64_t* out_bytes,
                          int32_t* out_is_table);
 
-/* Launch shape of the query's scan kernel: threads per block, rows per
- * thread per sub-step (ITEMS) and sub-steps per tile (SUB). In the
- * deterministic mode these fix the reduction order (tiles of
- * SUB*BLOCK*ITEMS rows). No reference counterpart. */
-int rf_query_tuning(const rf_query* q, int32_t* block, int32_t* items,
-                    int32_t* sub);
+/* Launch shape of the query's scan kernel, written to tuning[0..3]:
+ * threads per block, rows per lane per sub-step (ITEMS), sub-steps per
+ * wave-tile (SUB) and resident waves per CU. In the deterministic mode
+ * ITEMS and SUB fix the reduction order (wave-tiles of SUB*64*ITEMS rows,
+ * the completion combine over BLOCK threads). No reference counterpart. */
+int rf_query_tuning(const rf_query* q, int32_t* tuning);
 
 /* Human-readable lowering summary (kernel kind, columns, fields). */
 int rf_query_describe(const rf_query* q, char* buf, size_t cap);

@@ -167,11 +167,12 @@ static void compile_impl(const char* ir_text, uint32_t flags, const int32_t* tun
     q->det = (flags & RF_COMPILE_DETERMINISTIC) != 0;
     q->tune = rf::default_tuning(q->plan);
     if (tuning) {
-        rf::Tuning t{tuning[0], tuning[1], tuning[2]};
+        rf::Tuning t{tuning[0], tuning[1], tuning[2], tuning[3] > 0 ? tuning[3] : 16};
         need(t.block >= 64 && t.block <= 1024 && t.block % 64 == 0, "block must be 64..1024, x64");
         need(t.items == 1 || t.items == 2 || t.items == 4 || t.items == 8 || t.items == 16,
              "items must be 1, 2, 4, 8 or 16");
         need(t.sub >= 1 && t.sub <= 64, "sub must be 1..64");
+        need(t.waves_per_cu >= 1 && t.waves_per_cu <= 64, "waves_per_cu must be 1..64");
         q->tune = t;
     }
     // Compile the all-valid specialisation now so compile errors surface
@@ -232,12 +233,13 @@ int rf_query_output_info(const rf_query* q, int64_t* out_bytes, int32_t* out_is_
     });
 }
 
-int rf_query_tuning(const rf_query* q, int32_t* block, int32_t* items, int32_t* sub) {
+int rf_query_tuning(const rf_query* q, int32_t* tuning) {
     return guard([&] {
-        need(q != nullptr, "query is NULL");
-        if (block) *block = q->tune.block;
-        if (items) *items = q->tune.items;
-        if (sub) *sub = q->tune.sub;
+        need(q && tuning, "NULL argument");
+        tuning[0] = q->tune.block;
+        tuning[1] = q->tune.items;
+        tuning[2] = q->tune.sub;
+        tuning[3] = q->tune.waves_per_cu;
     });
 }
 
@@ -247,7 +249,8 @@ int rf_query_describe(const rf_query* q, char* buf, size_t cap) {
         std::string s = q->plan.describe();
         auto tn = q->tune;
         s += "tuning: block " + std::to_string(tn.block) + " items " + std::to_string(tn.items) +
-             " sub " + std::to_string(tn.sub) + (q->det ? " deterministic" : "") + "\n";
+             " sub " + std::to_string(tn.sub) + " waves/CU " + std::to_string(tn.waves_per_cu) +
+             (q->det ? " deterministic" : "") + "\n";
         if (!q->specs.empty()) s += "--- kernel ---\n" + q->specs.begin()->second.ks.code;
         std::strncpy(buf, s.c_str(), cap - 1);
         buf[cap - 1] = 0;

@@ -707,7 +707,13 @@ Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/
     // Workspace.
     Device& d = device();
     const Module& m = *l.spec->mod;
-    int64_t grid = std::min<int64_t>(A.ntiles, (int64_t)d.num_cus * m.max_blocks_per_cu);
+    // Launch width: waves_per_cu resident waves per CU (bounded by occupancy),
+    // each owning a contiguous range of wave-tiles.
+    const int waves_per_block = ks.tune.block / 64;
+    const int64_t blocks_per_cu = std::max<int64_t>(
+        1, std::min<int64_t>(m.max_blocks_per_cu, ks.tune.waves_per_cu / waves_per_block));
+    int64_t grid = std::min<int64_t>((A.ntiles + waves_per_block - 1) / waves_per_block,
+                                     (int64_t)d.num_cus * blocks_per_cu);
     if (grid < 1) grid = 1;
     l.grid = (int)grid;
     const int64_t nw = grid * (ks.tune.block / 64);  // waves in the launch

@@ -292,17 +292,20 @@ Tuning default_tuning(const Plan& plan) {
             }
     for (auto& o : plan.ops) collect_driver_cols(o.pred, plan, cols);
     Tuning t;
-    int wmin = 8, row_bytes = 0;
+    int wmax = 0, row_bytes = 0;
     for (auto& kv : cols) {
-        wmin = std::min(wmin, kv.second);
+        wmax = std::max(wmax, kv.second);
         row_bytes += kv.second;
     }
-    t.items = wmin >= 8 ? 2 : (wmin >= 4 ? 4 : 8);
-    if (plan.source == Plan::RANGE) t.items = 4;
-    // Wave-tiles of SUB*64*ITEMS rows whose loads (issued together) fit in
-    // about 512 bytes of registers per lane.
-    t.sub = 8;
-    if (row_bytes > 0) t.sub = std::max(1, std::min(16, 512 / (t.items * row_bytes)));
+    // ITEMS rows per lane make the widest column one dense 16-byte load per
+    // lane per sub-step (narrower columns get dense 8/4/2-byte loads); SUB
+    // sub-steps put about 256 bytes per lane in flight per wave-tile. With
+    // contiguous wave ranges at 16 waves per CU this is the streaming shape
+    // that saturates HBM on gfx950.
+    t.items = wmax >= 8 ? 2 : (wmax == 4 ? 4 : (wmax == 2 ? 8 : 16));
+    if (wmax == 0 || plan.source == Plan::RANGE) t.items = 4;
+    t.sub = 4;
+    if (row_bytes > 0) t.sub = std::max(1, std::min(16, 256 / (t.items * row_bytes)));
     return t;
 }
 

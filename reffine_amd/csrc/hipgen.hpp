@@ -40,6 +40,7 @@ struct Tuning {
     int block = 256;
     int items = 4;
     int sub = 4;
+    int waves_per_cu = 16;  // launch width: resident waves aimed at per CU
 };
 
 struct KernelSource {

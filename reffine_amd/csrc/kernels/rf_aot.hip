@@ -50,10 +50,27 @@ extern "C" __global__ void rf_index_lookup(RfProbeDev h, const int64_t* __restri
         rows[i] = rf_probe(h, keys[i]);
 }
 
-// Plain 16-byte-per-lane copy; its byte counters calibrate FETCH/WRITE_SIZE.
+// Device copy in the streaming shape of the scan kernels (contiguous range
+// per wave, 8 nontemporal 16-byte loads per lane in flight): the measured
+// HBM ceiling, and a kernel of known size to calibrate FETCH/WRITE_SIZE.
 extern "C" __global__ void rf_copy_calib(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                          int64_t n16) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16;
+    constexpr int U = 8;
+    constexpr int64_t CHUNK = 64 * U;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / 64);
+    const int64_t gw = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int64_t lane = threadIdx.x & 63;
+    const int64_t nchunks = n16 / CHUNK;
+    const int64_t per = (nchunks + nw - 1) / nw;
+    const int64_t c0 = gw * per, c1 = (c0 + per < nchunks) ? c0 + per : nchunks;
+    for (int64_t c = c0; c < c1; c++) {
+        uint4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = __builtin_nontemporal_load(src + c * CHUNK + u * 64 + lane);
+#pragma unroll
+        for (int u = 0; u < U; u++) __builtin_nontemporal_store(v[u], dst + c * CHUNK + u * 64 + lane);
+    }
+    for (int64_t i = nchunks * CHUNK + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16;
          i += (int64_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
 }
@@ -94,8 +111,11 @@ hipError_t aot_index_lookup(const RfProbeDev& h, const int64_t* keys, int64_t n,
 
 hipError_t aot_copy(const void* src, void* dst, int64_t n16, hipStream_t s) {
     if (n16 <= 0) return hipSuccess;
-    hipLaunchKernelGGL(rf_copy_calib, dim3(grid_for(n16, 256)), dim3(256), 0, s,
-                       (const uint4*)src, (uint4*)dst, n16);
+    int dev = 0, cus = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    hipLaunchKernelGGL(rf_copy_calib, dim3(cus * 4), dim3(256), 0, s, (const uint4*)src,
+                       (uint4*)dst, n16);
     return hipGetLastError();
 }
 

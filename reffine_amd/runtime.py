@@ -112,9 +112,7 @@ _SIGS = [
     ("rf_query_output_info", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
                                             ctypes.POINTER(ctypes.c_int32)]),
     ("rf_query_describe", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]),
-    ("rf_query_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
-                                       ctypes.POINTER(ctypes.c_int32),
-                                       ctypes.POINTER(ctypes.c_int32)]),
+    ("rf_query_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     ("rf_query_run", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RfArg), ctypes.c_int64,
                                     ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     ("rf_query_time", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RfArg), ctypes.c_int64,
@@ -364,7 +362,7 @@ class Query:
     """A compiled query: the GPU counterpart of compile_op's function pointer."""
 
     def __init__(self, func: Func, vectorize: bool = False, deterministic: bool = False,
-                 tuning: tuple[int, int, int] | None = None):
+                 tuning: tuple | None = None):
         self.func = func
         self.vectorize = vectorize
         self.deterministic = deterministic
@@ -374,7 +372,7 @@ class Query:
         if tuning is None:
             _ck(lib().rf_compile(self.text.encode(), flags, ctypes.byref(h)))
         else:
-            tv = (ctypes.c_int32 * 3)(*tuning)
+            tv = (ctypes.c_int32 * 4)(*(list(tuning) + [0] * (4 - len(tuning))))
             _ck(lib().rf_compile_tuned(self.text.encode(), flags, tv, ctypes.byref(h)))
         self.handle = h
         nb = ctypes.c_int64()
@@ -431,12 +429,12 @@ class Query:
                                 ctypes.byref(scan)))
         return ms.value, scan.value
 
-    def tuning(self) -> tuple[int, int, int]:
-        """(BLOCK, ITEMS, SUB) of the scan kernel; with deterministic=True
-        they fix the reduction order."""
-        b, i, s = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-        _ck(lib().rf_query_tuning(self.handle, ctypes.byref(b), ctypes.byref(i), ctypes.byref(s)))
-        return b.value, i.value, s.value
+    def tuning(self) -> tuple[int, int, int, int]:
+        """(BLOCK, ITEMS, SUB, WAVES_PER_CU) of the scan kernel; with
+        deterministic=True the first three fix the reduction order."""
+        t = (ctypes.c_int32 * 4)()
+        _ck(lib().rf_query_tuning(self.handle, t))
+        return tuple(t)
 
     def describe(self) -> str:
         buf = ctypes.create_string_buffer(1 << 20)
@@ -452,9 +450,10 @@ class Query:
 
 
 def compile_op(func: Func, vectorize: bool = False, deterministic: bool = False,
-               tuning: tuple[int, int, int] | None = None) -> Query:
+               tuning: tuple | None = None) -> Query:
     """compile_op<T>(func, vectorize) (include/reffine/utils/utils.h:47-54).
-    `tuning` = (BLOCK, ITEMS, SUB) overrides the default launch shape."""
+    `tuning` = (BLOCK, ITEMS, SUB[, WAVES_PER_CU]) overrides the default
+    launch shape."""
     return Query(func, vectorize, deterministic, tuning)
 
 
