@@ -169,7 +169,8 @@ int rf_compile(const char* ir_text, uint32_t flags, rf_query** out);
 
 /* rf_compile with an explicit launch shape `tuning` = {BLOCK (64..1024,
  * multiple of 64), ITEMS (1, 2, 4, 8 or 16), SUB (1..64), WAVES_PER_CU
- * (1..64, 0 = 16)}: the autotuner's entry point. A wave-tile is
+ * (1..64, 0 = 16), PIPELINE (0/1: overlap each wave's next tile loads with
+ * the current tile)}: the autotuner's entry point. A wave-tile is
  * SUB*64*ITEMS rows. */
 int rf_compile_tuned(const char* ir_text, uint32_t flags,
                      const int32_t* tuning, rf_query** out);
@@ -187,9 +188,10 @@ int rf_compile_check(const char* ir_text, uint32_t flags, char* buf,
 int rf_query_output_info(const rf_query* q, int64_t* out_bytes,
                          int32_t* out_is_table);
 
-/* Launch shape of the query's scan kernel, written to tuning[0..3]:
+/* Launch shape of the query's scan kernel, written to tuning[0..4]:
  * threads per block, rows per lane per sub-step (ITEMS), sub-steps per
- * wave-tile (SUB) and resident waves per CU. In the deterministic mode
+ * wave-tile (SUB), resident waves per CU and the pipelining flag. In the
+ * deterministic mode
  * ITEMS and SUB fix the reduction order (wave-tiles of SUB*64*ITEMS rows,
  * the completion combine over BLOCK threads). No reference counterpart. */
 int rf_query_tuning(const rf_query* q, int32_t* tuning);

@@ -15,17 +15,15 @@ from .runtime import Query, ReffineError, compile_op
 _cache: dict[tuple, tuple[int, int, int]] = {}
 
 
-def candidates(has_selection: bool) -> list[tuple[int, int, int, int]]:
-    """Launch shapes to try (BLOCK, ITEMS, SUB, WAVES_PER_CU): wave-tiles of
-    SUB*64*ITEMS rows, up to 64 rows per lane per tile."""
+def candidates(has_selection: bool) -> list[tuple[int, int, int, int, int]]:
+    """Launch shapes to try (BLOCK, ITEMS, SUB, WAVES_PER_CU, PIPELINE):
+    wave-tiles of SUB*64*ITEMS rows, up to 32 rows per lane per tile."""
     out = []
-    for block in (256, 512):
-        for items in (2, 4, 8):
-            for sub in (2, 4, 8, 16):
-                if items * sub > 64:
-                    continue
-                for wpc in (8, 16, 24):
-                    out.append((block, items, sub, wpc))
+    for items in (2, 4):
+        for sub in (1, 2, 4, 8):
+            for wpc in (8, 16):
+                for pipe in (0, 1):
+                    out.append((256, items, sub, wpc, pipe))
     return out
 
 

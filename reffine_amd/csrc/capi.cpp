@@ -167,7 +167,8 @@ static void compile_impl(const char* ir_text, uint32_t flags, const int32_t* tun
     q->det = (flags & RF_COMPILE_DETERMINISTIC) != 0;
     q->tune = rf::default_tuning(q->plan);
     if (tuning) {
-        rf::Tuning t{tuning[0], tuning[1], tuning[2], tuning[3] > 0 ? tuning[3] : 16};
+        rf::Tuning t{tuning[0], tuning[1], tuning[2], tuning[3] > 0 ? tuning[3] : 16,
+                     tuning[4] != 0 ? 1 : 0};
         need(t.block >= 64 && t.block <= 1024 && t.block % 64 == 0, "block must be 64..1024, x64");
         need(t.items == 1 || t.items == 2 || t.items == 4 || t.items == 8 || t.items == 16,
              "items must be 1, 2, 4, 8 or 16");
@@ -240,6 +241,7 @@ int rf_query_tuning(const rf_query* q, int32_t* tuning) {
         tuning[1] = q->tune.items;
         tuning[2] = q->tune.sub;
         tuning[3] = q->tune.waves_per_cu;
+        tuning[4] = q->tune.pipeline;
     });
 }
 
@@ -250,6 +252,7 @@ int rf_query_describe(const rf_query* q, char* buf, size_t cap) {
         auto tn = q->tune;
         s += "tuning: block " + std::to_string(tn.block) + " items " + std::to_string(tn.items) +
              " sub " + std::to_string(tn.sub) + " waves/CU " + std::to_string(tn.waves_per_cu) +
+             (tn.pipeline ? " pipelined" : "") +
              (q->det ? " deterministic" : "") + "\n";
         if (!q->specs.empty()) s += "--- kernel ---\n" + q->specs.begin()->second.ks.code;
         std::strncpy(buf, s.c_str(), cap - 1);

@@ -354,7 +354,8 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
     o << "struct Q {\n";
     o << "    enum { BLOCK = " << tune.block << ", ITEMS = " << tune.items
       << ", SUB = " << tune.sub << ", NACC = " << ks.acc.size()
-      << ", HAS_SEL = " << (ks.has_sel ? 1 : 0) << ", DET = " << (det ? 1 : 0) << " };\n";
+      << ", HAS_SEL = " << (ks.has_sel ? 1 : 0) << ", DET = " << (det ? 1 : 0)
+      << ", PIPE = " << (tune.pipeline ? 1 : 0) << " };\n";
     o << "    struct Acc {";
     for (size_t i = 0; i < ks.acc.size(); i++) o << " " << cty(ks.acc[i].type) << " a" << i << ";";
     if (ks.acc.empty()) o << " int unused;";
@@ -405,30 +406,48 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
               << "], r);\n";
     }
     o << "        return row_block<FULL>(a, L, r" << vargs << ");\n    }\n";
-    // tile_full(): an interior tile, every sub-step's loads issued first.
-    o << "    static __device__ __forceinline__ void tile_full(Acc& a, const RfScanArgs& L, "
-         "int64_t row0, uint32_t lane, uint32_t* bits) {\n"
-      << "        const int64_t rb = row0 + (int64_t)lane * ITEMS;\n";
+    // Tile: the registers of one interior wave-tile; load() issues every
+    // sub-step's loads, compute() evaluates the rows (the skeleton overlaps
+    // the next tile's load() with this tile's compute()).
+    o << "    struct Tile {";
+    bool any_vec = false;
     for (size_t s = 0; s < ks.cols.size(); s++) {
         const ColSlot& c = ks.cols[s];
         if (!c.vector) continue;
-        o << "        RfVec<" << cty(c.type) << ", ITEMS> c" << s << "[SUB];\n";
-        if (c.nullable) o << "        uint32_t vb" << s << "[SUB];\n";
+        any_vec = true;
+        o << " RfVec<" << cty(c.type) << ", ITEMS> c" << s << "[SUB];";
+        if (c.nullable) o << " uint32_t vb" << s << "[SUB];";
     }
-    o << "#pragma unroll\n        for (int s = 0; s < SUB; s++) {\n"
-      << "            const int64_t r = rb + (int64_t)s * 64 * ITEMS;\n";
+    if (!any_vec) o << " int unused;";
+    o << " };\n";
+    o << "    static __device__ __forceinline__ void load(Tile& T, const RfScanArgs& L, "
+         "int64_t row0, uint32_t lane) {\n"
+      << "        const int64_t rb = row0 + (int64_t)lane * ITEMS;\n        (void)rb; (void)T;\n"
+      << "#pragma unroll\n        for (int s = 0; s < SUB; s++) {\n"
+      << "            const int64_t r = rb + (int64_t)s * 64 * ITEMS;\n            (void)r;\n";
     for (size_t s = 0; s < ks.cols.size(); s++) {
         const ColSlot& c = ks.cols[s];
         if (!c.vector) continue;
         const std::string T = cty(c.type);
-        o << "            c" << s << "[s] = rf_ldv<" << T << ", ITEMS>((const " << T << "*)L.col["
+        o << "            T.c" << s << "[s] = rf_ldv<" << T << ", ITEMS>((const " << T << "*)L.col["
           << s << "] + r);\n";
         if (c.nullable)
-            o << "            vb" << s << "[s] = rf_valid_bits<ITEMS>(L.valid[" << s << "], r);\n";
+            o << "            T.vb" << s << "[s] = rf_valid_bits<ITEMS>(L.valid[" << s << "], r);\n";
     }
-    o << "        }\n#pragma unroll\n        for (int s = 0; s < SUB; s++) {\n"
+    o << "        }\n    }\n";
+    std::string targs_s;
+    for (size_t s = 0; s < ks.cols.size(); s++) {
+        const ColSlot& c = ks.cols[s];
+        if (!c.vector) continue;
+        targs_s += ", T.c" + std::to_string(s) + "[s]";
+        if (c.nullable) targs_s += ", T.vb" + std::to_string(s) + "[s]";
+    }
+    o << "    static __device__ __forceinline__ void compute(Acc& a, const Tile& T, "
+         "const RfScanArgs& L, int64_t row0, uint32_t lane, uint32_t* bits) {\n"
+      << "        const int64_t rb = row0 + (int64_t)lane * ITEMS;\n        (void)T;\n"
+      << "#pragma unroll\n        for (int s = 0; s < SUB; s++) {\n"
       << "            const int64_t r = rb + (int64_t)s * 64 * ITEMS;\n"
-      << "            bits[s] = row_block<true>(a, L, r" << vargs_s << ");\n        }\n    }\n";
+      << "            bits[s] = row_block<true>(a, L, r" << targs_s << ");\n        }\n    }\n";
     // row_block(): the query's row code over ITEMS loaded rows.
     o << "    template <bool FULL>\n"
       << "    static __device__ __forceinline__ uint32_t row_block(Acc& a, const RfScanArgs& L, "

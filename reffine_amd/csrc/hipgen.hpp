@@ -41,6 +41,7 @@ struct Tuning {
     int items = 4;
     int sub = 4;
     int waves_per_cu = 16;  // launch width: resident waves aimed at per CU
+    int pipeline = 0;       // 1: overlap the next wave-tile's loads with this one's rows
 };
 
 struct KernelSource {

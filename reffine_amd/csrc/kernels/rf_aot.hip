@@ -53,7 +53,9 @@ extern "C" __global__ void rf_index_lookup(RfProbeDev h, const int64_t* __restri
 // Device copy in the streaming shape of the scan kernels (contiguous range
 // per wave, 8 nontemporal 16-byte loads per lane in flight): the measured
 // HBM ceiling, and a kernel of known size to calibrate FETCH/WRITE_SIZE.
-extern "C" __global__ void rf_copy_calib(const uint4* __restrict__ src, uint4* __restrict__ dst,
+typedef unsigned int rf_u4 __attribute__((ext_vector_type(4)));
+
+extern "C" __global__ void rf_copy_calib(const rf_u4* __restrict__ src, rf_u4* __restrict__ dst,
                                          int64_t n16) {
     constexpr int U = 8;
     constexpr int64_t CHUNK = 64 * U;
@@ -64,7 +66,7 @@ extern "C" __global__ void rf_copy_calib(const uint4* __restrict__ src, uint4* _
     const int64_t per = (nchunks + nw - 1) / nw;
     const int64_t c0 = gw * per, c1 = (c0 + per < nchunks) ? c0 + per : nchunks;
     for (int64_t c = c0; c < c1; c++) {
-        uint4 v[U];
+        rf_u4 v[U];
 #pragma unroll
         for (int u = 0; u < U; u++) v[u] = __builtin_nontemporal_load(src + c * CHUNK + u * 64 + lane);
 #pragma unroll
@@ -112,10 +114,12 @@ hipError_t aot_index_lookup(const RfProbeDev& h, const int64_t* keys, int64_t n,
 hipError_t aot_copy(const void* src, void* dst, int64_t n16, hipStream_t s) {
     if (n16 <= 0) return hipSuccess;
     int dev = 0, cus = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    hipLaunchKernelGGL(rf_copy_calib, dim3(cus * 4), dim3(256), 0, s, (const uint4*)src,
-                       (uint4*)dst, n16);
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(rf_copy_calib, dim3(cus * 4), dim3(256), 0, s, (const rf_u4*)src,
+                       (rf_u4*)dst, n16);
     return hipGetLastError();
 }
 

@@ -229,15 +229,19 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
     Acc run;
     Q::acc_zero(run);
     int64_t cursor = 0;  // rows this wave has staged (wave-uniform)
-    for (int64_t tile = t_begin; tile < t_end; tile++) {
+
+    // One wave-tile: rows from the loaded registers `T` (interior tiles) or
+    // with bounds checks (the edge tiles), then its fold and compaction.
+    auto process = [&](const typename Q::Tile& T, int64_t tile, bool interior) {
         const int64_t row0 = L.start + tile * TILE;
-        Acc a;
-        Q::acc_zero(a);
+        // The deterministic mode folds each tile from the identity; otherwise
+        // rows fold straight into the wave's running state.
+        Acc a_tile;
+        Acc& a = Q::DET ? a_tile : run;
+        if constexpr (Q::DET) Q::acc_zero(a);
         uint32_t bits[SUB];
-        if (row0 >= L.lo && row0 + TILE <= L.end) {
-            // Interior tile: all SUB sub-steps' loads are issued before any
-            // row is evaluated.
-            Q::tile_full(a, L, row0, lane, bits);
+        if (interior) {
+            Q::compute(a, T, L, row0, lane, bits);
         } else {
 #pragma unroll
             for (int s = 0; s < SUB; s++) {
@@ -250,13 +254,9 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
                     bits[s] = 0;
             }
         }
-        if constexpr (Q::NACC > 0) {
-            if constexpr (Q::DET) {
-                rf_wave_reduce<Q>(a);
-                if (lane == 0) reinterpret_cast<Acc*>(L.partials)[tile] = a;
-            } else {
-                Q::acc_combine(run, a);
-            }
+        if constexpr (Q::NACC > 0 && Q::DET) {
+            rf_wave_reduce<Q>(a);
+            if (lane == 0) reinterpret_cast<Acc*>(L.partials)[tile] = a;
         }
         if constexpr (Q::HAS_SEL) {
             // Rows of a sub-step in ascending order are (lane, item): the
@@ -286,6 +286,36 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
                 }
                 cursor += total;
             }
+        }
+    };
+    auto interior = [&](int64_t tile) {
+        const int64_t row0 = L.start + tile * TILE;
+        return row0 >= L.lo && row0 + TILE <= L.end;
+    };
+
+    if constexpr (Q::PIPE) {
+        // Software pipeline over the wave's tiles, two register sets: the
+        // next tile's loads are in flight while the current tile is
+        // evaluated.
+        typename Q::Tile TA, TB;
+        if (t_begin < t_end && interior(t_begin)) Q::load(TA, L, L.start + t_begin * TILE, lane);
+        for (int64_t tile = t_begin; tile < t_end; tile += 2) {
+            const bool in0 = interior(tile);
+            const bool has1 = tile + 1 < t_end;
+            const bool in1 = has1 && interior(tile + 1);
+            if (in1) Q::load(TB, L, L.start + (tile + 1) * TILE, lane);
+            process(TA, tile, in0);
+            if (!has1) break;
+            if (tile + 2 < t_end && interior(tile + 2))
+                Q::load(TA, L, L.start + (tile + 2) * TILE, lane);
+            process(TB, tile + 1, in1);
+        }
+    } else {
+        for (int64_t tile = t_begin; tile < t_end; tile++) {
+            typename Q::Tile T;
+            const bool in = interior(tile);
+            if (in) Q::load(T, L, L.start + tile * TILE, lane);
+            process(T, tile, in);
         }
     }
 

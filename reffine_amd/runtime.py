@@ -372,7 +372,7 @@ class Query:
         if tuning is None:
             _ck(lib().rf_compile(self.text.encode(), flags, ctypes.byref(h)))
         else:
-            tv = (ctypes.c_int32 * 4)(*(list(tuning) + [0] * (4 - len(tuning))))
+            tv = (ctypes.c_int32 * 5)(*(list(tuning) + [0] * (5 - len(tuning))))
             _ck(lib().rf_compile_tuned(self.text.encode(), flags, tv, ctypes.byref(h)))
         self.handle = h
         nb = ctypes.c_int64()
@@ -429,10 +429,10 @@ class Query:
                                 ctypes.byref(scan)))
         return ms.value, scan.value
 
-    def tuning(self) -> tuple[int, int, int, int]:
-        """(BLOCK, ITEMS, SUB, WAVES_PER_CU) of the scan kernel; with
-        deterministic=True the first three fix the reduction order."""
-        t = (ctypes.c_int32 * 4)()
+    def tuning(self) -> tuple[int, int, int, int, int]:
+        """(BLOCK, ITEMS, SUB, WAVES_PER_CU, PIPELINE) of the scan kernel;
+        with deterministic=True BLOCK, ITEMS and SUB fix the reduction order."""
+        t = (ctypes.c_int32 * 5)()
         _ck(lib().rf_query_tuning(self.handle, t))
         return tuple(t)
 
@@ -452,8 +452,8 @@ class Query:
 def compile_op(func: Func, vectorize: bool = False, deterministic: bool = False,
                tuning: tuple | None = None) -> Query:
     """compile_op<T>(func, vectorize) (include/reffine/utils/utils.h:47-54).
-    `tuning` = (BLOCK, ITEMS, SUB[, WAVES_PER_CU]) overrides the default
-    launch shape."""
+    `tuning` = (BLOCK, ITEMS, SUB[, WAVES_PER_CU[, PIPELINE]]) overrides the
+    default launch shape."""
     return Query(func, vectorize, deterministic, tuning)
 
 
