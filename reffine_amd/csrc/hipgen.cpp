@@ -304,8 +304,17 @@ Tuning default_tuning(const Plan& plan) {
     // that saturates HBM on gfx950.
     t.items = wmax >= 8 ? 2 : (wmax == 4 ? 4 : (wmax == 2 ? 8 : 16));
     if (wmax == 0 || plan.source == Plan::RANGE) t.items = 4;
+    // With a small state, wave-tiles are double-buffered (the next tile's
+    // loads overlap this tile's rows) at 8 waves per CU; a large state
+    // (many accumulators) keeps one register set at 16 waves per CU.
+    int nacc = 0;
+    for (auto& r : plan.reduces)
+        for (auto& f : r.fields) nacc += f.updated ? 1 : 0;
+    t.pipeline = nacc <= 8 ? 1 : 0;
+    t.waves_per_cu = t.pipeline ? 8 : 16;
+    const int budget = t.pipeline ? 640 : 320;  // load bytes per lane per tile
     t.sub = 4;
-    if (row_bytes > 0) t.sub = std::max(1, std::min(16, 256 / (t.items * row_bytes)));
+    if (row_bytes > 0) t.sub = std::max(1, std::min(16, budget / (t.items * row_bytes)));
     return t;
 }
 
