@@ -64,3 +64,59 @@ def scan_c2(v, d, q, d0=1000, vlo=0.05, vhi=0.07, qmax=24, rowid=None):
     sel = (rowid[idx] if rowid is not None else idx).astype(np.int32)
     return (int(m.sum()), int(q[m].sum(dtype=np.int64)),
             fsum(v[m] * q[m].astype(np.float64)), sel)
+
+
+def group_join_c4(build, probe, bf_eq=1, pq_gt=10):
+    """Hash join + grouped aggregate, brute force. For every build key with
+    bf == bf_eq, in ascending key order: SUM(pv*pq) over probe rows with
+    pk == key and pq > pq_gt, folded from 0.0 in ascending probe-row order
+    (np.add.at applies its updates one index at a time, in order), and the
+    COUNT of those rows. Returns (keys, sums, counts, ranks, values) where
+    ranks/values are the joined rows in probe-row order."""
+    bk = np.asarray(build["bk"], dtype=np.int64)
+    keys = np.sort(bk[np.asarray(build["bf"]) == bf_eq])
+    pk = np.asarray(probe["pk"], dtype=np.int64)
+    pq = np.asarray(probe["pq"], dtype=np.int64)
+    pv = np.asarray(probe["pv"], dtype=np.float64)
+    m = pq > pq_gt
+    if len(keys) == 0:
+        e = np.zeros(0)
+        return keys, e, e.astype(np.int64), e.astype(np.int64), e
+    pos = np.searchsorted(keys, pk[m])
+    ok = (pos < len(keys)) & (keys[np.minimum(pos, len(keys) - 1)] == pk[m])
+    ranks = pos[ok]
+    vals = (pv[m] * pq[m].astype(np.float64))[ok]
+    sums = np.zeros(len(keys), dtype=np.float64)
+    np.add.at(sums, ranks, vals)
+    counts = np.bincount(ranks, minlength=len(keys)).astype(np.int64)
+    return keys, sums, counts, ranks, vals
+
+
+def group_fsums(n_groups, ranks, vals):
+    """Correctly rounded per-group sums (math.fsum per group)."""
+    order = np.argsort(ranks, kind="stable")
+    r, v = ranks[order], vals[order]
+    bounds = np.searchsorted(r, np.arange(n_groups + 1))
+    return np.array([math.fsum(v[bounds[g]:bounds[g + 1]].tolist()) for g in range(n_groups)])
+
+
+def group_join_nested(build, probe, bf_eq=1, pq_gt=10):
+    """The nested loops of the reffine form, in pure Python (small inputs
+    only): for each build row in row order whose bf == bf_eq, fold the probe
+    rows with pk == bk and pq > pq_gt, in row order, into (0.0, 0) with
+    s0 = s0 + pv*pq, s1 = s1 + 1. Returns the rows sorted by key."""
+    out = []
+    bk, bf = list(map(int, build["bk"])), list(map(int, build["bf"]))
+    pk, pq = list(map(int, probe["pk"])), list(map(int, probe["pq"]))
+    pv = [float(x) for x in probe["pv"]]
+    for r in range(len(bk)):
+        if bf[r] != bf_eq:
+            continue
+        t = bk[r]
+        s0, s1 = 0.0, 0
+        for i in range(len(pk)):
+            if pk[i] == t and pq[i] > pq_gt:
+                s0 = s0 + pv[i] * float(pq[i])
+                s1 = s1 + 1
+        out.append((t, s0, s1))
+    return sorted(out)

@@ -23,7 +23,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 HOST_SOURCES = ["ir.cpp", "lower.cpp", "hipgen.cpp", "engine.cpp", "capi.cpp"]
-DEVICE_SOURCES = ["kernels/rf_aot.hip"]
+DEVICE_SOURCES = ["kernels/rf_aot.hip", "kernels/rf_group.hip"]
 EMBED = {"kRfAbiSrc": "kernels/rf_abi.h", "kRfDeviceSrc": "kernels/rf_device.h"}
 
 COMMON = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",

@@ -176,6 +176,11 @@ static void compile_impl(const char* ir_text, uint32_t flags, const int32_t* tun
         need(t.waves_per_cu >= 1 && t.waves_per_cu <= 64, "waves_per_cu must be 1..64");
         q->tune = t;
     }
+    if (q->plan.gj) {
+        rf::setup_group_join(q.get(), tuning ? &q->tune : nullptr);
+        *out = q.release();
+        return;
+    }
     // Compile the all-valid specialisation now so compile errors surface
     // here, as compile_op's do.
     std::vector<std::vector<bool>> nl(q->plan.inputs.size());
@@ -213,11 +218,19 @@ int rf_compile_check(const char* ir_text, uint32_t flags, char* buf, size_t cap)
         std::vector<std::vector<bool>> nl(plan.inputs.size());
         for (size_t i = 0; i < plan.inputs.size(); i++)
             nl[i].assign(plan.inputs[i]->type.dtypes.size(), (flags & 2u) != 0);
-        rf::KernelSource ks = rf::gen_scan(plan, nl, tune, (flags & RF_COMPILE_DETERMINISTIC) != 0);
-        std::vector<char> code = rf::jit_compile(ks);
+        std::vector<rf::KernelSource> kss;
+        if (plan.gj)
+            kss = rf::group_join_sources(plan, nl);
+        else
+            kss.push_back(rf::gen_scan(plan, nl, tune, (flags & RF_COMPILE_DETERMINISTIC) != 0));
+        size_t bytes = 0;
+        std::string text;
+        for (auto& ks : kss) {
+            bytes += rf::jit_compile(ks).size();
+            text += "\n--- kernel ---\n" + ks.code;
+        }
         if (buf && cap) {
-            std::string s = plan.describe() + "code object bytes: " + std::to_string(code.size()) +
-                            "\n--- kernel ---\n" + ks.code;
+            std::string s = plan.describe() + "code object bytes: " + std::to_string(bytes) + text;
             std::strncpy(buf, s.c_str(), cap - 1);
             buf[cap - 1] = 0;
         }

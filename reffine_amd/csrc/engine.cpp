@@ -281,7 +281,7 @@ Module& jit_module(const KernelSource& ks) {
     auto m = std::make_unique<Module>();
     RF_HIP(hipModuleLoadData(&m->mod, code.data()));
     RF_HIP(hipModuleGetFunction(&m->fn, m->mod, ks.entry.c_str()));
-    RF_HIP(hipModuleGetFunction(&m->gather, m->mod, "rf_q_gather"));
+    if (ks.entry == "rf_q") RF_HIP(hipModuleGetFunction(&m->gather, m->mod, "rf_q_gather"));
     int nb = 0;
     RF_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, m->fn, ks.tune.block, 0));
     m->max_blocks_per_cu = std::max(1, nb);
@@ -639,8 +639,24 @@ void table_free(rf_table* t) {
     delete t;
 }
 
-Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/) {
-    const Plan& plan = q->plan;
+static void fill_params(const KernelSource& ks, EvalCtx& c, RfParam* out) {
+    for (size_t i = 0; i < ks.params.size(); i++) {
+        const ParamSlot& p = ks.params[i];
+        HostVal v;
+        if (p.kind == ParamSlot::CONST) {
+            v = eval(mk_const(p.type, p.value), c);
+        } else if (p.kind == ParamSlot::ARG) {
+            v = eval_leaf(mk_sym("$arg." + std::to_string(p.arg), p.type), c);
+        } else {
+            v = eval_leaf(mk_sym("$len." + std::to_string(p.arg) + "." + std::to_string(p.col),
+                                 types::IDX_T), c);
+        }
+        if (p.type.is_float()) out[i].f = convert(v, types::F64).f;
+        else out[i].i = convert(v, types::I64).i;
+    }
+}
+
+static void check_args(const Plan& plan, const rf_arg* args, int64_t n_args) {
     if (n_args != (int64_t)plan.inputs.size())
         throw IRError("function '" + plan.func_name + "' takes " +
                       std::to_string(plan.inputs.size()) + " arguments, got " +
@@ -650,6 +666,47 @@ Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/
         if (vec != (args[i].kind == 0))
             throw IRError("argument " + std::to_string(i) + (vec ? " must be a table" : " must be a scalar"));
     }
+}
+
+// A device table of `count` rows with fresh columns of `types` (one arena).
+static rf_table* alloc_table(const std::vector<Type>& types, int64_t count, int64_t dim) {
+    auto t = std::make_unique<rf_table>();
+    t->dim = dim;
+    size_t total = 0;
+    std::vector<size_t> offs;
+    for (auto& ty : types) {
+        offs.push_back(total);
+        total += ((size_t)std::max<int64_t>(count, 1) * ty.width() + 255) & ~size_t(255);
+    }
+    char* arena = (char*)dev_alloc(std::max<size_t>(total, 256));
+    t->owned.push_back(arena);
+    for (size_t j = 0; j < types.size(); j++) {
+        Column col;
+        col.type = types[j];
+        switch (col.type.bt) {
+            case BT::BOOL: case BT::INT8: col.format = "c"; break;
+            case BT::UINT8: col.format = "C"; break;
+            case BT::INT16: col.format = "s"; break;
+            case BT::UINT16: col.format = "S"; break;
+            case BT::INT32: col.format = "i"; break;
+            case BT::UINT32: col.format = "I"; break;
+            case BT::INT64: case BT::IDX: col.format = "l"; col.type = types::I64; break;
+            case BT::UINT64: col.format = "L"; break;
+            case BT::FLOAT32: col.format = "f"; break;
+            case BT::FLOAT64: col.format = "g"; break;
+            default: throw IRError("output column type");
+        }
+        if (types[j].bt == BT::BOOL) col.type = types::I8;
+        col.length = col.logical_length = count;
+        col.values = arena + offs[j];
+        t->cols.push_back(col);
+    }
+    return t.release();
+}
+
+Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/) {
+    const Plan& plan = q->plan;
+    check_args(plan, args, n_args);
     Launch l;
     l.spec = &get_spec(q, args, n_args);
     const KernelSource& ks = l.spec->ks;
@@ -684,26 +741,17 @@ Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/
         A.valid[s] = (const uint16_t*)col.validity;
     }
     for (size_t j = 0; j < plan.probes.size(); j++) {
+        if ((int)j >= RF_MAXPROBES) throw Unsupported("more than 4 probed tables");
+        if (plan.probes[j].external) {
+            if (j >= q->probe_override.size()) throw IRError("external probe without an index");
+            A.probe[j] = q->probe_override[j];
+            continue;
+        }
         rf_table* t = args[plan.probes[j].table].u.table;
         if (!t->has_index) table_build_index(t);
-        if ((int)j >= RF_MAXPROBES) throw Unsupported("more than 4 probed tables");
         A.probe[j] = probe_of(t);
     }
-    // Parameters.
-    for (size_t i = 0; i < ks.params.size(); i++) {
-        const ParamSlot& p = ks.params[i];
-        HostVal v;
-        if (p.kind == ParamSlot::CONST) {
-            v = eval(mk_const(p.type, p.value), c);
-        } else if (p.kind == ParamSlot::ARG) {
-            v = eval_leaf(mk_sym("$arg." + std::to_string(p.arg), p.type), c);
-        } else {
-            v = eval_leaf(mk_sym("$len." + std::to_string(p.arg) + "." + std::to_string(p.col),
-                                 types::IDX_T), c);
-        }
-        if (p.type.is_float()) A.p[i].f = convert(v, types::F64).f;
-        else A.p[i].i = convert(v, types::I64).i;
-    }
+    fill_params(ks, c, A.p);
     // Workspace.
     Device& d = device();
     const Module& m = *l.spec->mod;
@@ -782,6 +830,14 @@ void launch(rf_query* /*q*/, Launch& l, hipStream_t s, bool scan_only) {
     }
 }
 
+// The selection placement kernel alone (after a scan-only launch).
+static void launch_gather(Launch& l, hipStream_t s) {
+    if (l.empty || !l.spec->ks.has_sel) return;
+    void* params[] = {&l.args};
+    const int g = (int)std::min<int64_t>(l.args.nwaves, 8192);
+    RF_HIP(hipModuleLaunchKernel(l.spec->mod->gather, g, 1, 1, 256, 1, 1, 0, s, params, nullptr));
+}
+
 static HostVal finish_reduce(const Plan& plan, const KernelSource& ks, size_t comp,
                              const char* acc_bytes, bool empty, EvalCtx& c) {
     const ReduceComp& rc = plan.reduces[comp];
@@ -813,9 +869,272 @@ static HostVal finish_reduce(const Plan& plan, const KernelSource& ks, size_t co
     return nest(rc.state_type, k);
 }
 
+// ---------------------------------------------------------- group join
+
+GroupJoinState::~GroupJoinState() {
+    dev_free(small);
+    dev_free(keys2);
+    dev_free(rows2);
+    dev_free(hkeys);
+    dev_free(hrows);
+    dev_free(hist);
+    dev_free(rank2);
+    for (void* p : pay2) dev_free(p);
+    dev_free(seg);
+}
+
+static void grow(void*& p, size_t& cap, size_t need) {
+    if (need <= cap && p) return;
+    dev_free(p);
+    p = nullptr;
+    p = dev_alloc(std::max<size_t>(need, 256));
+    cap = need;
+}
+
+static std::vector<std::vector<bool>> all_valid(const Plan& plan) {
+    std::vector<std::vector<bool>> nl(plan.inputs.size());
+    for (size_t i = 0; i < plan.inputs.size(); i++)
+        nl[i].assign(plan.inputs[i]->type.dtypes.size(), false);
+    return nl;
+}
+
+static std::string all_valid_sig(const Plan& plan) {
+    std::string sig;
+    for (size_t i = 0; i < plan.inputs.size(); i++) {
+        if (!plan.inputs[i]->type.is_vector()) continue;
+        sig += std::string(plan.inputs[i]->type.dtypes.size(), '0') + "|";
+    }
+    return sig;
+}
+
+static std::unique_ptr<rf_query> sub_query(const Plan& plan, const Tuning& tune) {
+    auto q = std::make_unique<rf_query>();
+    q->plan = plan;
+    q->det = false;
+    q->tune = tune;
+    Spec sp;
+    sp.ks = gen_scan(plan, all_valid(plan), tune, false);
+    sp.mod = &jit_module(sp.ks);
+    q->specs.emplace(all_valid_sig(plan), std::move(sp));
+    return q;
+}
+
+void setup_group_join(rf_query* q, const Tuning* probe_tune) {
+    const GroupJoinPlan& gp = *q->plan.gj;
+    auto G = std::make_unique<GroupJoinState>();
+    G->build_q = sub_query(gp.build_plan, default_tuning(gp.build_plan));
+    G->probe_q = sub_query(gp.probe_plan, probe_tune ? *probe_tune : default_tuning(gp.probe_plan));
+    G->probe_q->probe_override.resize(gp.probe_plan.probes.size());
+    Spec fs;
+    fs.ks = gen_group_fold(gp, all_valid(q->plan));
+    fs.mod = &jit_module(fs.ks);
+    G->fold_specs.emplace(all_valid_sig(q->plan), std::move(fs));
+    q->tune = G->probe_q->tune;
+    q->gj = std::move(G);
+}
+
+std::vector<KernelSource> group_join_sources(const Plan& plan,
+                                             const std::vector<std::vector<bool>>& nullable) {
+    const GroupJoinPlan& gp = *plan.gj;
+    return {gen_scan(gp.build_plan, nullable, default_tuning(gp.build_plan), false),
+            gen_scan(gp.probe_plan, nullable, default_tuning(gp.probe_plan), false),
+            gen_group_fold(gp, nullable)};
+}
+
+static Spec& fold_spec(rf_query* q, const rf_arg* args, int64_t n_args) {
+    std::string sig = null_signature(q->plan, args, n_args);
+    auto& specs = q->gj->fold_specs;
+    auto it = specs.find(sig);
+    if (it != specs.end()) return it->second;
+    std::vector<std::vector<bool>> nl(q->plan.inputs.size());
+    for (size_t i = 0; i < q->plan.inputs.size(); i++) {
+        if (!q->plan.inputs[i]->type.is_vector()) continue;
+        for (auto& c : args[i].u.table->cols) nl[i].push_back(c.validity != nullptr);
+    }
+    Spec sp;
+    sp.ks = gen_group_fold(*q->plan.gj, nl);
+    sp.mod = &jit_module(sp.ks);
+    return specs.emplace(sig, std::move(sp)).first->second;
+}
+
+struct GjTiming {
+    hipEvent_t scan0 = nullptr, scan1 = nullptr;  // around the probe scan kernel
+};
+
+static int bits_of(uint64_t range) {
+    int b = 0;
+    while (b < 64 && (range >> b) != 0) b++;
+    return b;
+}
+
+// Runs a group join (see GroupJoinPlan). Three host synchronisations: after
+// the build selection (group count and key range), after the probe scan
+// (joined row count) and at the end.
+static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
+                                GjTiming* tm) {
+    Device& d = device();
+    hipStream_t s = d.stream;
+    GroupJoinState& G = *q->gj;
+    const GroupJoinPlan& gp = *q->plan.gj;
+    check_args(q->plan, args, n_args);
+    rf_query* bq = G.build_q.get();
+    rf_query* pq = G.probe_q.get();
+    if (!G.small) G.small = dev_alloc(256);
+    uint64_t* small = (uint64_t*)G.small;  // [0..1] key range, [2] duplicate flag, [3] scratch
+
+    // 1. Build selection: (key, build row) of the qualifying build rows.
+    Launch lb = prepare(bq, args, n_args, nullptr);
+    char* hs = (char*)bq->host_stage;  // pinned, allocated by prepare
+    dev_fill_u64(small, ~0ull, 1);
+    dev_fill_u64(small + 1, 0, 3);
+    launch(bq, lb, s, false);
+    int64_t K = 0;
+    uint64_t mm[2] = {~0ull, 0};
+    if (!lb.empty) {
+        const int64_t nb = args[gp.build].u.table->rows();
+        RF_HIP(aot_minmax_i64((const int64_t*)bq->outs[0], (const int64_t*)bq->out_count, nb, small,
+                              s));
+        RF_HIP(hipMemcpyAsync(hs, bq->out_count, 8, hipMemcpyDeviceToHost, s));
+        RF_HIP(hipMemcpyAsync(hs + 8, small, 16, hipMemcpyDeviceToHost, s));
+        RF_HIP(hipStreamSynchronize(s));
+        K = *(int64_t*)hs;
+        std::memcpy(mm, hs + 8, 16);
+    }
+    const Spec& fs = fold_spec(q, args, n_args);
+    if (K == 0) return alloc_table(gp.out_types, 0, 1);
+    if (K >= ((int64_t)1 << 31)) throw Unsupported("more than 2^31 groups");
+    const int64_t kmin = (int64_t)(mm[0] ^ 0x8000000000000000ull);
+    const int64_t kmax = (int64_t)(mm[1] ^ 0x8000000000000000ull);
+
+    // 2. Sort the build keys (ranks = positions in key order).
+    if ((size_t)K * 8 > G.keys_cap || !G.keys2) {
+        dev_free(G.keys2);
+        dev_free(G.rows2);
+        G.keys2 = G.rows2 = nullptr;
+        G.keys2 = dev_alloc((size_t)K * 8);
+        G.rows2 = dev_alloc((size_t)K * 8);
+        G.keys_cap = (size_t)K * 8;
+    }
+    const size_t hist_need = (size_t)aot_sort_hist_entries(std::max<int64_t>(K, 1)) * 4;
+    grow(G.hist, G.hist_cap, hist_need);
+    void* kb[2] = {bq->outs[0], G.keys2};
+    void* pb[2][8] = {{bq->outs[1]}, {G.rows2}};
+    const int32_t pw8[8] = {8};
+    int which = 0;
+    RF_HIP(aot_radix_sort(kb, pb, pw8, 1, 8, K, bits_of((uint64_t)kmax - (uint64_t)kmin),
+                          (uint64_t)kmin, (uint32_t*)G.hist, &which, s));
+    const int64_t* skeys = (const int64_t*)kb[which];
+    const int64_t* srows = (const int64_t*)pb[which][0];
+
+    // 3. Index key -> rank (capacity >= table_scale * K, a power of two).
+    uint64_t cap = 16;
+    while (cap < (uint64_t)G.table_scale * (uint64_t)K) cap <<= 1;
+    if (cap * 8 > G.hcap || !G.hkeys) {
+        dev_free(G.hkeys);
+        dev_free(G.hrows);
+        G.hkeys = nullptr;
+        G.hrows = nullptr;
+        G.hkeys = dev_alloc(cap * 8);
+        G.hrows = dev_alloc(cap * 8);
+        G.hcap = cap * 8;
+    }
+    dev_fill_u64(G.hkeys, (uint64_t)RF_EMPTY_KEY, (int64_t)cap);
+    dev_fill_u64(G.hrows, (uint64_t)INT64_MAX, (int64_t)cap);
+    dev_fill_u64(small + 3, (uint64_t)INT64_MAX, 1);
+    RF_HIP(aot_index_build(skeys, nullptr, K, G.hkeys, G.hrows, cap - 1, small + 3, s));
+    RF_HIP(aot_adjacent_dups(skeys, K, small + 2, s));
+    RfProbeDev h{};
+    h.keys = (const int64_t*)G.hkeys;
+    h.rows = (const int64_t*)G.hrows;
+    h.mask = cap - 1;
+    h.min_key_row = (kmin == RF_EMPTY_KEY) ? 0 : -1;  // the smallest key has rank 0
+    for (size_t j = 0; j < gp.probe_plan.probes.size(); j++)
+        if (gp.probe_plan.probes[j].external) pq->probe_override[j] = h;
+
+    // 4. Probe scan: (rank, payload) of joined probe rows, in row order.
+    Launch lp = prepare(pq, args, n_args, nullptr);
+    if (tm) RF_HIP(hipEventRecord(tm->scan0, s));
+    launch(pq, lp, s, true);
+    if (tm) RF_HIP(hipEventRecord(tm->scan1, s));
+    launch_gather(lp, s);
+    RF_HIP(hipMemcpyAsync(hs, pq->out_count, 8, hipMemcpyDeviceToHost, s));
+    RF_HIP(hipMemcpyAsync(hs + 8, small + 2, 8, hipMemcpyDeviceToHost, s));
+    RF_HIP(hipStreamSynchronize(s));
+    const int64_t J = lp.empty ? 0 : *(int64_t*)hs;
+    if (*(uint64_t*)(hs + 8) != 0)
+        throw IRError("group join: the build table holds a qualifying key more than once "
+                      "(each group key must occur in one build row)");
+
+    // 5. Stable sort of the joined rows by rank.
+    const int np = (int)gp.pay_types.size();
+    if (J > 0) {
+        if ((size_t)J > G.join_cap || G.pay2.size() != (size_t)np) {
+            dev_free(G.rank2);
+            G.rank2 = nullptr;
+            for (void* p : G.pay2) dev_free(p);
+            G.pay2.clear();
+            G.rank2 = dev_alloc((size_t)J * 4);
+            for (int j = 0; j < np; j++) G.pay2.push_back(dev_alloc((size_t)J * gp.pay_types[j].width()));
+            G.join_cap = (size_t)J;
+        }
+        grow(G.hist, G.hist_cap, (size_t)aot_sort_hist_entries(J) * 4);
+    }
+    void* kj[2] = {pq->outs[0], G.rank2};
+    void* pj[2][8] = {};
+    int32_t pwj[8] = {};
+    for (int j = 0; j < np; j++) {
+        pj[0][j] = pq->outs[1 + j];
+        pj[1][j] = (size_t)j < G.pay2.size() ? G.pay2[j] : nullptr;  // allocated once J > 0
+        pwj[j] = gp.pay_types[j].width();
+    }
+    int wj = 0;
+    if (J > 1)
+        RF_HIP(aot_radix_sort(kj, pj, pwj, np, 4, J, bits_of((uint64_t)(K - 1)), 0,
+                              (uint32_t*)G.hist, &wj, s));
+
+    // 6. Segment bounds per group.
+    grow(G.seg, G.seg_cap, (size_t)K * 16);
+    int64_t* lo = (int64_t*)G.seg;
+    int64_t* hi = lo + K;
+    dev_fill_u64(lo, 0, 2 * K);
+    if (J > 0) RF_HIP(aot_seg_bounds((const uint32_t*)kj[wj], J, lo, hi, s));
+
+    // 7. Fold each group and evaluate the outputs.
+    std::unique_ptr<rf_table> out(alloc_table(gp.out_types, K, 1));
+    RfFoldArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.groups = K;
+    A.keys = skeys;
+    A.brow = srows;
+    A.seg_lo = lo;
+    A.seg_hi = hi;
+    for (int j = 0; j < np; j++) A.pay[j] = pj[wj][j];
+    for (size_t c = 0; c < fs.ks.cols.size(); c++) {
+        const ColSlot& cs = fs.ks.cols[c];
+        const Column& col = args[cs.table].u.table->cols.at(cs.col);
+        A.col[c] = col.values;
+        A.valid[c] = (const uint16_t*)col.validity;
+    }
+    for (size_t j = 0; j < out->cols.size(); j++) A.out[j] = out->cols[j].values;
+    EvalCtx c{&q->plan, args, n_args, {}, {}};
+    fill_params(fs.ks, c, A.p);
+    void* params[] = {&A};
+    const int g = (int)std::min<int64_t>((K + 255) / 256, 16384);
+    RF_HIP(hipModuleLaunchKernel(fs.mod->fn, g, 1, 1, 256, 1, 1, 0, s, params, nullptr));
+    RF_HIP(hipStreamSynchronize(s));
+    return out.release();
+}
+
 void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_addr,
                rf_table** out_table) {
     std::lock_guard<std::mutex> lk(q->mu);
+    if (q->gj) {
+        if (!out_table) throw IRError("an Op output needs out_table");
+        rf_table* t = run_group_join(q, args, n_args, nullptr);
+        *out_table = t;
+        if (output_addr) *(void**)output_addr = t;
+        return;
+    }
     Device& d = device();
     Launch l = prepare(q, args, n_args, nullptr);
     const KernelSource& ks = l.spec->ks;
@@ -837,41 +1156,12 @@ void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_add
     if (!plan.ops.empty()) {
         int64_t count = l.empty ? 0 : *(int64_t*)(stage + 4096);
         const OpComp& oc = plan.ops[0];
-        auto t = std::make_unique<rf_table>();
-        t->dim = oc.n_iters;
-        size_t total = 0;
-        std::vector<size_t> offs;
-        for (auto& ty : ks.out_types) {
-            offs.push_back(total);
-            total += ((size_t)std::max<int64_t>(count, 1) * ty.width() + 255) & ~size_t(255);
-        }
-        char* arena = (char*)dev_alloc(total);
-        t->owned.push_back(arena);
-        for (size_t j = 0; j < ks.out_types.size(); j++) {
-            Column col;
-            col.type = ks.out_types[j];
-            col.format = "";
-            switch (col.type.bt) {
-                case BT::BOOL: case BT::INT8: col.format = "c"; break;
-                case BT::UINT8: col.format = "C"; break;
-                case BT::INT16: col.format = "s"; break;
-                case BT::UINT16: col.format = "S"; break;
-                case BT::INT32: col.format = "i"; break;
-                case BT::UINT32: col.format = "I"; break;
-                case BT::INT64: case BT::IDX: col.format = "l"; col.type = types::I64; break;
-                case BT::UINT64: col.format = "L"; break;
-                case BT::FLOAT32: col.format = "f"; break;
-                case BT::FLOAT64: col.format = "g"; break;
-                default: throw IRError("output column type");
-            }
-            if (ks.out_types[j].bt == BT::BOOL) col.type = types::I8;
-            col.length = col.logical_length = count;
-            col.values = arena + offs[j];
+        std::unique_ptr<rf_table> t(alloc_table(ks.out_types, count, oc.n_iters));
+        for (size_t j = 0; j < ks.out_types.size(); j++)
             if (count > 0)
-                RF_HIP(hipMemcpyAsync(col.values, q->outs[j], (size_t)count * ks.out_types[j].width(),
+                RF_HIP(hipMemcpyAsync(t->cols[j].values, q->outs[j],
+                                      (size_t)count * ks.out_types[j].width(),
                                       hipMemcpyDeviceToDevice, d.stream));
-            t->cols.push_back(col);
-        }
         RF_HIP(hipStreamSynchronize(d.stream));
         sel = t.release();
         HostVal hv;
@@ -899,8 +1189,37 @@ void time_query(rf_query* q, const rf_arg* args, int64_t n_args, int iters, doub
                 double* ms_scan) {
     std::lock_guard<std::mutex> lk(q->mu);
     Device& d = device();
-    Launch l = prepare(q, args, n_args, nullptr);
     *ms_all = *ms_scan = 0.0;
+    if (q->gj) {
+        // Whole group joins (host synchronisations included) and the probe
+        // scan kernel of each.
+        if (iters <= 0) return;
+        GjTiming tm;
+        hipEvent_t e[2];
+        RF_HIP(hipEventCreate(&tm.scan0));
+        RF_HIP(hipEventCreate(&tm.scan1));
+        for (auto& x : e) RF_HIP(hipEventCreate(&x));
+        table_free(run_group_join(q, args, n_args, nullptr));  // warm
+        double scan = 0;
+        RF_HIP(hipEventRecord(e[0], d.stream));
+        for (int i = 0; i < iters; i++) {
+            table_free(run_group_join(q, args, n_args, &tm));
+            float ms = 0;
+            RF_HIP(hipEventElapsedTime(&ms, tm.scan0, tm.scan1));
+            scan += ms;
+        }
+        RF_HIP(hipEventRecord(e[1], d.stream));
+        RF_HIP(hipEventSynchronize(e[1]));
+        float a = 0;
+        RF_HIP(hipEventElapsedTime(&a, e[0], e[1]));
+        for (auto& x : e) (void)hipEventDestroy(x);
+        (void)hipEventDestroy(tm.scan0);
+        (void)hipEventDestroy(tm.scan1);
+        *ms_all = a / iters;
+        *ms_scan = scan / iters;
+        return;
+    }
+    Launch l = prepare(q, args, n_args, nullptr);
     if (l.empty || iters <= 0) return;
     hipEvent_t e[4];
     for (auto& x : e) RF_HIP(hipEventCreate(&x));

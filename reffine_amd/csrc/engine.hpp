@@ -50,6 +50,16 @@ hipError_t aot_index_build(const int64_t* keys, const uint16_t* valid, int64_t n
 hipError_t aot_index_lookup(const RfProbeDev& h, const int64_t* keys, int64_t n, int64_t* rows,
                             hipStream_t s);
 hipError_t aot_copy(const void* src, void* dst, int64_t n16, hipStream_t s);
+// Group join kernels (kernels/rf_group.hip).
+int64_t aot_sort_hist_entries(int64_t n);
+hipError_t aot_radix_sort(void* const keys[2], void* const pay[2][8], const int32_t* pw, int np,
+                          int key_bytes, int64_t n, int bits, uint64_t bias, uint32_t* hist,
+                          int* which, hipStream_t s);
+hipError_t aot_minmax_i64(const int64_t* keys, const int64_t* n_dev, int64_t n_max, void* out,
+                          hipStream_t s);
+hipError_t aot_adjacent_dups(const int64_t* keys, int64_t n, void* flag, hipStream_t s);
+hipError_t aot_seg_bounds(const uint32_t* rank, int64_t n, int64_t* lo, int64_t* hi,
+                          hipStream_t s);
 
 Device& device();          // initialised device (throws if rf_init not called)
 Device& init_device(int id);
@@ -93,8 +103,8 @@ namespace rf {
 
 struct Module {
     hipModule_t mod = nullptr;
-    hipFunction_t fn = nullptr;      // rf_q: the scan
-    hipFunction_t gather = nullptr;  // rf_q_gather: selection placement
+    hipFunction_t fn = nullptr;      // the entry: rf_q (scan) or rf_gj_fold
+    hipFunction_t gather = nullptr;  // rf_q_gather: selection placement (scan modules)
     int max_blocks_per_cu = 1;
 };
 
@@ -110,8 +120,14 @@ struct Spec {
 
 }  // namespace rf
 
+namespace rf {
+struct GroupJoinState;
+}
+
 struct rf_query {
     rf::Plan plan;
+    std::unique_ptr<rf::GroupJoinState> gj;  // group join: sub-queries + workspace
+    std::vector<RfProbeDev> probe_override;  // indexes for external probes
     bool det = false;
     rf::Tuning tune;
     std::map<std::string, rf::Spec> specs;  // by nullability signature
@@ -161,5 +177,35 @@ void time_query(rf_query* q, const rf_arg* args, int64_t n_args, int iters, doub
 
 // Output struct layout (natural alignment, ScalarPass field order).
 void c_layout(const Type& t, size_t& size, size_t& align);
+
+// Group join: sub-queries for the build and probe scans, the per-group
+// fold module and a grow-only workspace.
+struct GroupJoinState {
+    std::unique_ptr<rf_query> build_q, probe_q;
+    std::map<std::string, Spec> fold_specs;  // by nullability signature
+    int table_scale = 2;                     // index capacity >= scale * groups
+    void* small = nullptr;                   // key range, duplicate flag, scratch
+    void* keys2 = nullptr;                   // sort ping-pong buffers (build side)
+    void* rows2 = nullptr;
+    size_t keys_cap = 0;
+    void* hkeys = nullptr;                   // key -> rank index
+    void* hrows = nullptr;
+    size_t hcap = 0;
+    void* hist = nullptr;
+    size_t hist_cap = 0;
+    void* rank2 = nullptr;                   // sort ping-pong buffers (joined rows)
+    std::vector<void*> pay2;
+    size_t join_cap = 0;
+    void* seg = nullptr;                     // lo | hi per group
+    size_t seg_cap = 0;
+    ~GroupJoinState();
+};
+
+// Creates the sub-queries of a group join and compiles its all-valid
+// kernels; `probe_tune` (optional) sets the probe scan's launch shape.
+void setup_group_join(rf_query* q, const Tuning* probe_tune);
+// Kernel sources of a group join for a nullability (compile checks).
+std::vector<KernelSource> group_join_sources(const Plan& plan,
+                                             const std::vector<std::vector<bool>>& nullable);
 
 }  // namespace rf

@@ -28,7 +28,10 @@ std::vector<std::string> split(const std::string& s, char d) {
     return out;
 }
 
-enum class Mode { ROWS, EMIT };
+// ROWS: the scan's row code over vector-loaded rows; EMIT: one qualifying
+// row re-read by index; FINAL: one group of a group join (build-table
+// columns at the group's build row).
+enum class Mode { ROWS, EMIT, FINAL };
 
 struct Gen {
     const Plan& plan;
@@ -112,9 +115,16 @@ struct Emitter {
     std::string probe(int j) {
         std::string h = "h" + std::to_string(j);
         if (!probes_done.count(j)) {
-            std::string k = expr(g.plan.probes[j].key);
-            os << indent << "const int64_t " << h << " = rf_probe(L.probe[" << j
-               << "], (int64_t)" << k << ");\n";
+            const Probe& pr = g.plan.probes[j];
+            std::string k = expr(pr.key);
+            if (pr.guard) {
+                std::string gd = expr(pr.guard);
+                os << indent << "const int64_t " << h << " = (" << gd << ") ? rf_probe(L.probe["
+                   << j << "], (int64_t)" << k << ") : (int64_t)-1;\n";
+            } else {
+                os << indent << "const int64_t " << h << " = rf_probe(L.probe[" << j
+                   << "], (int64_t)" << k << ");\n";
+            }
             probes_done.insert(j);
         }
         return h;
@@ -130,6 +140,7 @@ struct Emitter {
             int s = g.col_slot(t, c, mode == Mode::ROWS);
             const std::string T = cty(g.ks.cols[s].type);
             if (mode == Mode::ROWS) return "c" + std::to_string(s) + ".v[k]";
+            if (mode == Mode::FINAL) return "((const " + T + "*)L.col[" + std::to_string(s) + "])[brow]";
             return "((const " + T + "*)L.col[" + std::to_string(s) + "])[row]";
         }
         if (kind == "$valid") {
@@ -138,6 +149,7 @@ struct Emitter {
             if (!g.is_nullable(t, c)) return "true";
             int s = g.col_slot(t, c, mode == Mode::ROWS);
             if (mode == Mode::ROWS) return "(((vb" + std::to_string(s) + " >> k) & 1u) != 0u)";
+            if (mode == Mode::FINAL) return "rf_valid_bit(L.valid[" + std::to_string(s) + "], brow)";
             return "rf_valid_bit(L.valid[" + std::to_string(s) + "], row)";
         }
         if (kind == "$pcol") {
@@ -153,7 +165,13 @@ struct Emitter {
             int j = std::stoi(parts[1]);
             return "(" + probe(j) + " >= 0)";
         }
-        if (kind == "$row") return "row";
+        if (kind == "$prow") return probe(std::stoi(parts[1]));
+        if (kind == "$row") {
+            if (mode == Mode::FINAL) throw IRError("row index in a group join output");
+            return "row";
+        }
+        if (kind == "$key" && mode == Mode::FINAL) return "key";
+        if (kind == "$acc" && mode == Mode::FINAL) return "s" + parts[1];
         if (kind == "$arg") {
             ParamSlot p{ParamSlot::ARG, e->type, 0, std::stoi(parts[1]), 0};
             return g.param(p, name + ":" + e->type.str());
@@ -291,6 +309,10 @@ Tuning default_tuning(const Plan& plan) {
                 collect_driver_cols(f.contrib, plan, cols);
             }
     for (auto& o : plan.ops) collect_driver_cols(o.pred, plan, cols);
+    for (auto& p : plan.probes) {
+        collect_driver_cols(p.key, plan, cols);
+        collect_driver_cols(p.guard, plan, cols);
+    }
     Tuning t;
     int wmax = 0, row_bytes = 0;
     for (auto& kv : cols) {
@@ -506,6 +528,72 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
       << ") void rf_q(RfScanArgs L) { rf_scan<Q>(L); }\n";
     o << "extern \"C\" __global__ __launch_bounds__(256) void rf_q_gather(RfScanArgs L) { "
          "rf_gather<Q>(L); }\n";
+    ks.code = o.str();
+    return ks;
+}
+
+KernelSource gen_group_fold(const GroupJoinPlan& gj, const std::vector<std::vector<bool>>& nullable) {
+    KernelSource ks;
+    ks.entry = "rf_gj_fold";
+    ks.tune.block = 256;
+    // Build-table columns are read by index at the group's build row.
+    Gen g(gj.build_plan, nullable, ks);
+    std::ostringstream o;
+    o << "// Generated group fold and outputs for group join '" << gj.sym_name << "'.\n";
+    o << "extern \"C\" __global__ __launch_bounds__(256) void rf_gj_fold(RfFoldArgs L) {\n";
+    o << "    for (int64_t grp = blockIdx.x * (int64_t)256 + threadIdx.x; grp < L.groups;\n"
+         "         grp += (int64_t)gridDim.x * 256) {\n";
+    o << "        const int64_t i0 = L.seg_lo[grp], i1 = L.seg_hi[grp];\n";
+    // Field state, initialised from the reduction's init (host values).
+    std::ostringstream init;
+    Emitter ei(g, Mode::FINAL, init, "        ");
+    std::vector<std::string> iv;
+    for (auto& f : gj.fields) iv.push_back(ei.expr(f.init));
+    o << init.str();
+    for (size_t f = 0; f < gj.fields.size(); f++) {
+        const std::string T = cty(gj.fields[f].type);
+        o << "        " << T << " s" << f << " = (" << T << ")" << iv[f] << ";\n";
+    }
+    // Sequential fold of the group's joined rows in ascending probe-row order.
+    std::ostringstream cst;
+    Emitter ec(g, Mode::FINAL, cst, "        ");
+    ec.ntmp = ei.ntmp;  // one scope: temporaries stay distinct
+    std::vector<std::string> cv(gj.fields.size());
+    for (size_t f = 0; f < gj.fields.size(); f++)
+        if (gj.fields[f].updated && gj.fields[f].const_contrib) cv[f] = ec.expr(gj.fields[f].const_contrib);
+    o << cst.str();
+    o << "        for (int64_t i = i0; i < i1; i++) {\n";
+    for (size_t f = 0; f < gj.fields.size(); f++) {
+        const GField& gf = gj.fields[f];
+        if (!gf.updated) continue;
+        const std::string T = cty(gf.type), S = "s" + std::to_string(f);
+        std::string v = gf.const_contrib
+                            ? "(" + T + ")" + cv[f]
+                            : "((const " + cty(gj.pay_types[gf.pay]) + "*)L.pay[" +
+                                  std::to_string(gf.pay) + "])[i]";
+        std::string upd = combine_expr(gf.op, gf.type, S, v);
+        if (gf.pred_pay >= 0)
+            o << "            " << S << " = ((const bool*)L.pay[" << gf.pred_pay << "])[i] ? "
+              << upd << " : " << S << ";\n";
+        else
+            o << "            " << S << " = " << upd << ";\n";
+    }
+    o << "        }\n";
+    o << "        const int64_t key = L.keys[grp];\n        const int64_t brow = L.brow[grp];\n"
+         "        (void)key; (void)brow;\n";
+    std::ostringstream body;
+    Emitter em(g, Mode::FINAL, body, "        ");
+    em.ntmp = ec.ntmp;
+    std::vector<std::string> ov;
+    for (auto& v : gj.out_values) ov.push_back(em.expr(v));
+    if (ov.size() > 16) throw Unsupported("more than 16 group join output columns");
+    o << body.str();
+    for (size_t j = 0; j < ov.size(); j++) {
+        const std::string T = cty(gj.out_types[j]);
+        o << "        ((" << T << "*)L.out[" << j << "])[grp] = (" << T << ")" << ov[j] << ";\n";
+        ks.out_types.push_back(gj.out_types[j]);
+    }
+    o << "    }\n}\n";
     ks.code = o.str();
     return ks;
 }

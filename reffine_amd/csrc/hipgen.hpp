@@ -63,6 +63,11 @@ KernelSource gen_scan(const Plan& plan,
                       const std::vector<std::vector<bool>>& nullable,
                       const Tuning& tune, bool det);
 
+// Per-group fold and output kernel (rf_gj_fold) of a group join;
+// `nullable` describes the function inputs as for gen_scan.
+KernelSource gen_group_fold(const GroupJoinPlan& gj,
+                            const std::vector<std::vector<bool>>& nullable);
+
 // Picks ITEMS from the narrowest driver column the row code loads.
 Tuning default_tuning(const Plan& plan);
 

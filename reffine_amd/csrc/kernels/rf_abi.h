@@ -59,3 +59,18 @@ struct RfScanArgs {
     int64_t* out_count;
     union RfParam p[RF_MAXPARAMS];
 };
+
+// One group join's per-group fold (generated rf_gj_fold): joined rows
+// sorted by group, [seg_lo[g], seg_hi[g]) those of group g.
+struct RfFoldArgs {
+    int64_t groups;
+    const int64_t* keys;     // group key (sorted)
+    const int64_t* brow;     // build row of the group
+    const int64_t* seg_lo;
+    const int64_t* seg_hi;
+    const void* pay[RF_MAXOUT];   // sorted payload columns (after the rank)
+    const void* col[RF_MAXCOLS];  // build table columns read by the outputs
+    const uint16_t* valid[RF_MAXCOLS];
+    void* out[RF_MAXOUT];         // output columns, one row per group
+    union RfParam p[RF_MAXPARAMS];
+};

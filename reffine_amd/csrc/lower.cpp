@@ -134,6 +134,21 @@ struct Lowerer {
 
     std::map<Node*, N> memo;
     std::map<std::string, N> subst;  // symbol name -> replacement
+    std::map<std::string, N> reduce_subst;  // Reduce text -> its result leaves
+
+    // A lowerer over the same function, with no row binding yet.
+    Lowerer sibling() const {
+        Lowerer L;
+        L.func = func;
+        L.vectorize = vectorize;
+        L.tbl = tbl;
+        L.input_ix = input_ix;
+        L.plan.func_name = plan.func_name;
+        L.plan.inputs = plan.inputs;
+        L.plan.tbl = plan.tbl;
+        L.plan.vectorize = vectorize;
+        return L;
+    }
 
     N rw(const N& e) {
         auto m = memo.find(e.get());
@@ -198,6 +213,11 @@ struct Lowerer {
                 if (t < 0) throw Unsupported("length of a computed vector");
                 return leaf("$len." + std::to_string(t) + "." + std::to_string(e->idx),
                             types::IDX_T);
+            }
+            case K::Reduce: {
+                auto r = reduce_subst.find(to_text(e));
+                if (r != reduce_subst.end()) return r->second;
+                throw Unsupported("reduction inside a row expression: " + to_text(e));
             }
             default:
                 throw Unsupported("node kind inside a row expression: " + to_text(e));
@@ -572,6 +592,212 @@ struct Lowerer {
         return oc;
     }
 
+    // ------------------------------------------------------- group join
+
+    // Reductions reachable from an expression through the symbol table.
+    void find_reduces(const N& e, std::vector<N>& out, std::set<std::string>& seen) {
+        if (!e) return;
+        if (e->kind == K::Sym) {
+            if (!leaf_name(e).empty() || input_index(e) >= 0) return;
+            auto t = tbl.find(e->name);
+            if (t != tbl.end()) find_reduces(t->second, out, seen);
+            return;
+        }
+        if (e->kind == K::Reduce) {
+            if (seen.insert(to_text(e)).second) out.push_back(e);
+            return;
+        }
+        for (auto& a : e->args) find_reduces(a, out, seen);
+    }
+
+    bool is_group_join(const N& op) {
+        std::vector<N> reds;
+        std::set<std::string> seen;
+        for (size_t i = 1; i < op->args.size(); i++) find_reduces(op->args[i], reds, seen);
+        return !reds.empty();
+    }
+
+    static bool is_sym_named(const N& e, const std::string& name) {
+        return e->kind == K::Sym && e->name == name;
+    }
+
+    void lower_group_join(const N& op, const std::string& name) {
+        if (op->iters.size() != 1) throw Unsupported("group join with more than one iterator");
+        std::vector<N> reds;
+        std::set<std::string> seen;
+        for (size_t i = 1; i < op->args.size(); i++) find_reduces(op->args[i], reds, seen);
+        if (reds.size() != 1)
+            throw Unsupported("a group join needs exactly one inner reduction");
+        const N red = reds[0];
+        const N t = op->iters[0];
+        auto gj = std::make_shared<GroupJoinPlan>();
+
+        // Build side: the outer Op's iteration space (VecSpace over B plus
+        // its predicate), selecting (key, build row).
+        Lowerer Lb = sibling();
+        Source sb = Lb.bind_op(mk_op(op->iters, op->args[0], {}));
+        if (Lb.plan.source != Plan::TABLE_ROWS)
+            throw Unsupported("group join whose outer Op is not over a table");
+        const int B = Lb.driver;
+        if (plan.inputs[B]->type.dtypes[0] != types::I64)
+            throw Unsupported("group key must be an int64 column 0");
+        if (!Lb.plan.probes.empty())
+            throw Unsupported("probes in the group join's build predicate");
+        OpComp bo;
+        bo.sym_name = name + ".build";
+        bo.pred = sb.pred;
+        bo.values = {Lb.col_leaf(B, 0), mk_cast(types::I64, leaf("$row", types::IDX_T))};
+        bo.types = {types::I64, types::I64};
+        bo.vec_type = Type(BT::VECTOR, {types::I64, types::I64}, 1);
+        Lb.plan.ops = {bo};
+        Lb.plan.func_name = func->name + "_build";
+        Lb.plan.output_type = bo.vec_type;
+        gj->build = B;
+
+        // Probe side: the inner reduction's Op without the key equality.
+        N rv = resolve_vec(red->args[0]);
+        if (rv->kind != K::Op || rv->iters.size() != 1)
+            throw Unsupported("inner reduction of a group join must be over a one-iterator Op");
+        std::vector<N> cj;
+        conjuncts(resolve_bool_sym(rv->args[0]), cj);
+        N key_expr, rest;
+        for (auto& c0 : cj) {
+            N c = resolve_bool_sym(c0);
+            if (!key_expr && c->kind == K::Nary && c->op == MathOp::EQ && c->args.size() == 2) {
+                const bool l = is_sym_named(c->args[0], t->name);
+                const bool r = is_sym_named(c->args[1], t->name);
+                if (l != r) {
+                    key_expr = l ? c->args[1] : c->args[0];
+                    continue;
+                }
+            }
+            rest = rest ? mk_nary(MathOp::AND, {rest, c}) : c;
+        }
+        if (!key_expr)
+            throw Unsupported("inner reduction has no equality with the outer iterator");
+        std::vector<N> outs(rv->args.begin() + 1, rv->args.end());
+        Lowerer Lp = sibling();
+        Source sp;
+        N key;
+        try {
+            sp = Lp.bind_op(mk_op(rv->iters, rest ? rest : true_c(), outs));
+            key = Lp.rw(key_expr);
+        } catch (const IRError& e) {
+            throw Unsupported(std::string("inner reduction depends on the outer row beyond the "
+                                          "key equality: ") + e.what());
+        }
+        if (Lp.plan.source != Plan::TABLE_ROWS)
+            throw Unsupported("group join whose inner Op is not over a table");
+        if (key->type != types::I64) key = mk_cast(types::I64, key);
+        const int gjix = (int)Lp.plan.probes.size();
+        if (gjix >= 4) throw Unsupported("more than 4 probed tables");
+        Probe gp;
+        gp.table = B;
+        gp.key = key;
+        gp.guard = sp.pred;
+        gp.external = true;
+        Lp.plan.probes.push_back(gp);
+        const N rank = leaf("$prow." + std::to_string(gjix), types::I64);
+
+        // Fields of the inner reduction.
+        std::vector<Field> fields;
+        std::vector<int> path;
+        N stree = Lp.state_tree(red->type, path, fields);
+        Lp.subst[red->args[2]->name] = stree;
+        Lp.subst[red->args[3]->name] = sp.row_value;
+        Lp.memo.clear();
+        N accr;
+        try {
+            accr = Lp.rw(red->args[4]);
+        } catch (const IRError& e) {
+            throw Unsupported(std::string("group join accumulator depends on the outer row: ") +
+                              e.what());
+        }
+        std::vector<N> flat;
+        flatten(accr, flat);
+        if (flat.size() != fields.size()) throw IRError("accumulator shape");
+        std::vector<N> inits;
+        flatten(fold_host(red->args[1]), inits);
+        OpComp po;
+        po.sym_name = name + ".probe";
+        po.pred = mk_and(sp.pred, mk_nary(MathOp::GTE, {rank, mk_const(types::I64, 0)}));
+        po.values = {mk_cast(types::I32, rank)};
+        po.types = {types::I32};
+        for (size_t f = 0; f < fields.size(); f++) {
+            Dec d = Lp.decompose(flat[f], "$s." + std::to_string(f));
+            GField g;
+            g.type = fields[f].type;
+            g.path = fields[f].path;
+            g.init = inits.at(f);
+            g.updated = d.upd;
+            if (d.upd) {
+                g.op = d.op;
+                if (mentions_state(d.contrib) || (d.pred && mentions_state(d.pred)))
+                    throw Unsupported("state field depends on another field");
+                if (d.contrib->kind == K::Const) {
+                    g.const_contrib = d.contrib;
+                } else {
+                    g.pay = (int)po.values.size() - 1;
+                    po.values.push_back(d.contrib);
+                    po.types.push_back(g.type);
+                }
+                if (d.pred) {
+                    g.pred_pay = (int)po.values.size() - 1;
+                    po.values.push_back(d.pred);
+                    po.types.push_back(types::BOOL_T);
+                }
+            }
+            gj->fields.push_back(g);
+        }
+        if (po.values.size() > 16) throw Unsupported("more than 15 group join payload columns");
+        for (size_t j = 1; j < po.types.size(); j++) gj->pay_types.push_back(po.types[j]);
+        std::vector<Type> vts;
+        for (auto& ty : po.types) vts.push_back(ty);
+        po.vec_type = Type(BT::VECTOR, vts, 1);
+        Lp.plan.ops = {po};
+        Lp.plan.func_name = func->name + "_probe";
+        Lp.plan.output_type = po.vec_type;
+        gj->probe = Lp.driver;
+        gj->state_type = red->type;
+
+        // Finalize: the outer Op's outputs per group.
+        Lowerer Lf = sibling();
+        Lf.set_driver(B);
+        Lf.iter_sym = t;
+        Lf.subst[t->name] = leaf("$key", types::I64);
+        std::vector<N> accl;
+        for (size_t f = 0; f < fields.size(); f++)
+            accl.push_back(leaf("$acc." + std::to_string(f), fields[f].type));
+        size_t k = 0;
+        std::function<N(const Type&)> nest = [&](const Type& ty) -> N {
+            if (ty.is_struct()) {
+                std::vector<N> v;
+                for (auto& d : ty.dtypes) v.push_back(nest(d));
+                return mk_new(v);
+            }
+            return accl.at(k++);
+        };
+        Lf.reduce_subst[to_text(red)] = nest(red->type);
+        gj->out_values.push_back(leaf("$key", types::I64));
+        gj->out_types.push_back(types::I64);
+        for (size_t i = 1; i < op->args.size(); i++) {
+            N v = Lf.rw(op->args[i]);
+            if (!v->type.is_primitive()) throw Unsupported("non-primitive group join output");
+            gj->out_values.push_back(v);
+            gj->out_types.push_back(v->type);
+        }
+        Lf.plan.func_name = func->name + "_final";
+        gj->build_plan = Lb.plan;
+        gj->probe_plan = Lp.plan;
+        gj->vec_type = op->type;
+        gj->sym_name = name;
+        plan.gj = gj;
+        plan.driver = gj->probe;
+        plan.source = Plan::TABLE_ROWS;
+        plan.start = mk_const(types::IDX_T, 0);
+        plan.end = leaf("$len." + std::to_string(gj->probe) + ".0", types::IDX_T);
+    }
+
     Plan run() {
         plan.func_name = func->name;
         for (size_t i = 1; i < func->args.size(); i++) {
@@ -603,6 +829,14 @@ struct Lowerer {
         };
         walk(out);
         if (comps.empty()) throw Unsupported("function output has no Reduce or Op");
+        if (comps.size() == 1 && comps[0].second->kind == K::Op &&
+            is_group_join(comps[0].second)) {
+            if (!(out->kind == K::Sym && out->name == comps[0].first))
+                throw Unsupported("a group join must be the function's output");
+            lower_group_join(comps[0].second, comps[0].first);
+            plan.output = out;
+            return plan;
+        }
         for (auto& [name, node] : comps) {
             if (node->kind == K::Reduce) {
                 plan.reduces.push_back(lower_reduce(node, name));
@@ -656,6 +890,24 @@ std::string Plan::describe() const {
                 o << " (not updated)";
             o << "\n";
         }
+    }
+    if (gj) {
+        o << "  group join: build table input " << gj->build << ", probe table input "
+          << gj->probe << "\n";
+        o << "  build:\n" << gj->build_plan.describe();
+        o << "  probe:\n" << gj->probe_plan.describe();
+        for (size_t f = 0; f < gj->fields.size(); f++) {
+            auto& g = gj->fields[f];
+            o << "    group field " << f << " " << g.type.str();
+            if (g.updated)
+                o << " " << aggop_name(g.op) << " of "
+                  << (g.const_contrib ? to_text(g.const_contrib) : "payload " + std::to_string(g.pay))
+                  << (g.pred_pay >= 0 ? " if payload " + std::to_string(g.pred_pay) : "");
+            else
+                o << " (not updated)";
+            o << "\n";
+        }
+        for (auto& v : gj->out_values) o << "    output " << to_text(v) << "\n";
     }
     for (auto& op : ops) {
         o << "  select " << op.sym_name << " -> " << op.vec_type.str() << " if "

@@ -12,6 +12,7 @@
 // A field that does not decompose makes the query RF_ERR_UNSUPPORTED.
 #pragma once
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -62,7 +63,11 @@ struct OpComp {
 struct Probe {
     int table;  // func input index of the probed table
     N key;      // int64 row expression giving the key
+    N guard;    // probe only rows where this holds (null: every row)
+    bool external = false;  // uses an index supplied at run time (group join)
 };
+
+struct GroupJoinPlan;
 
 struct Plan {
     enum Source { TABLE_ROWS, RANGE } source = TABLE_ROWS;
@@ -78,8 +83,44 @@ struct Plan {
     Type output_type;
     std::vector<std::pair<N, N>> tbl;  // the func's symbol table
     bool vectorize = false;
+    std::shared_ptr<GroupJoinPlan> gj;  // set: the function is a group join
 
     std::string describe() const;
+};
+
+// A hash join with a grouped aggregate, written in reffine as an Op over the
+// build table B whose outputs reduce the rows of the probe table P whose key
+// equals the Op's iterator:
+//   _op([t], _in(t, B) & bpred(B[t]),
+//       [f(t, B[t], _red(_op([i], _in(i, P) & (key(P[i]) == t) & ppred(P[i]), outs), init, acc))])
+// The reference runs the inner reduction once per build row
+// (src/pass/loopgen.cpp:113-269). The GPU plan: select the qualifying build
+// keys and rows, sort them by key (ranks), index key -> rank, scan P once
+// selecting (rank, per-field contributions) of joined rows, stable-sort those
+// by rank and fold each group in ascending probe-row order from the
+// reduction's init -- the inner loop's fold order.
+struct GField {
+    Type type;
+    AggOp op = AggOp::ADD;
+    bool updated = false;
+    N init;            // host expression
+    N const_contrib;   // constant contribution (no payload column)
+    int pay = -1;      // payload column of the contribution
+    int pred_pay = -1; // payload column of the field's own predicate
+    std::vector<int> path;
+};
+
+struct GroupJoinPlan {
+    int build = -1, probe = -1;  // func input indexes of B and P
+    Plan build_plan;             // Op over B: (key i64, build row i64)
+    Plan probe_plan;             // Op over P: (rank i32, payload...); probe 0 is the group index
+    std::vector<GField> fields;  // flattened state of the inner reduction
+    Type state_type;
+    std::vector<Type> pay_types; // payload columns after the rank
+    std::vector<N> out_values;   // $key, then the outer Op outputs over $key, $acc.F, B's row
+    std::vector<Type> out_types;
+    Type vec_type;
+    std::string sym_name;
 };
 
 // Lowers `func` (a parsed IR Func) into a plan; throws Unsupported for

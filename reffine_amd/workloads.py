@@ -9,9 +9,9 @@ id, so its output column 0 is the int32 selection vector.
 """
 from __future__ import annotations
 
-from .reffiner import (_add, _cast, _eq, _f64, _f64_t, _func, _gte, _i8, _i8_t, _i32, _i32_t, _i64, _i64_t, _sub,
-                       _idx, _in, _len, _lt, _lte, _mul, _new, _op, _red, _sel, _subvec, _sym,
-                       _vec_t, _get)
+from .reffiner import (_add, _cast, _eq, _f64, _f64_t, _func, _gt, _gte, _i8, _i8_t, _i32, _i32_t, _i64,
+                       _i64_t, _sub, _idx, _in, _len, _lt, _lte, _mul, _new, _op, _red, _sel, _subvec,
+                       _sym, _vec_t, _get)
 
 
 def scan_agg_c1(lo: int = 365, hi: int = 730, vmax: float = 0.5):
@@ -128,3 +128,47 @@ def group_agg_c3(dmax: int = 2526):
 def c3_bytes_per_row() -> int:
     """f1 (1) + f2 (1) + qty (8) + price (8) + disc (8) + tax (8) + d (4)."""
     return 38
+
+
+# Hash join + high-cardinality grouped aggregate.
+C4_BUILD_TYPE = _vec_t(1, _i64_t, _i32_t, _i8_t)   # bk, bd, bf
+C4_PROBE_TYPE = _vec_t(1, _i64_t, _f64_t, _i64_t)  # pk, pv, pq
+
+
+def group_join_c4(bf_eq: int = 1, pq_gt: int = 10):
+    """SUM(pv*pq) f64 and COUNT i64 per build key bk with bf == bf_eq, over
+    the probe rows with pk == bk and pq > pq_gt; one output row per
+    qualifying build key, in key order.
+
+    Written as reffine's nested form: an Op over the build table whose
+    outputs reduce an Op over the probe table correlated by `i == t`
+    (nested Ops as in test/src/test_nested.cpp:9-25, joins through `_in`
+    as in test/src/test_join.cpp:9-44, the reduction as in
+    test/src/test_reduce.cpp:61-122). Output columns: bk, SUM, COUNT.
+    """
+    B = _sym("B", C4_BUILD_TYPE)
+    P = _sym("P", C4_PROBE_TYPE)
+    t = _sym("t", _i64_t)
+    i = _sym("i", _i64_t)
+    pe = P[i]
+    rows = _op([i], _in(i, P) & _eq(i, t) & _gt(pe[1], _i64(pq_gt)),
+               [_mul(pe[0], _cast(_f64_t, pe[1]))])
+
+    def acc(s, r):
+        return _new([_add(_get(s, 0), r[1]), _add(_get(s, 1), _i64(1))])
+
+    red = _red(rows, lambda: _new([_f64(0), _i64(0)]), acc)
+    grp = _sym("grp", red)
+    be = B[t]
+    out = _op([t], _in(t, B) & _eq(be[1], _i8(bf_eq)), [_get(grp, 0), _get(grp, 1)])
+    os_ = _sym("out", out)
+    fn = _func("group_join_c4", os_, [B, P])
+    fn.tbl[grp] = red
+    fn.tbl[os_] = out
+    return fn
+
+
+def c4_bytes(n_build: int, n_probe: int, groups: int) -> int:
+    """Algorithmic bytes: build bk + bf (9 B/row), probe pk + pv + pq
+    (24 B/row), output bk + SUM + COUNT (24 B/group)."""
+    return 9 * n_build + 24 * n_probe + 24 * groups

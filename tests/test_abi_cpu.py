@@ -79,3 +79,15 @@ def test_range_bounds_are_extracted():
     import ref_queries as rq
     s = runtime.compile_check(rq.gen_table_op())
     assert "scan range rows [(cast x (sym lb" in s and "(cast x (sym ub" in s
+
+
+def test_group_join_lowers_and_compiles():
+    """The nested Op/Reduce form lowers to the group-join plan: a build
+    selection, a probe scan with a guarded hash probe, and a per-group fold."""
+    text = runtime.compile_check(workloads.group_join_c4())
+    assert "group join: build table input 0, probe table input 1" in text
+    assert "group field 0 f64 add of payload 0" in text
+    assert "group field 1 i64 add of (const i64 1)" in text
+    assert text.count("--- kernel ---") == 3
+    assert "? rf_probe(" in text  # probed only where the probe predicate holds
+    assert runtime.compile_check(workloads.group_join_c4(), nullable=True)

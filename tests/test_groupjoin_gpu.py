@@ -1,0 +1,134 @@
+"""Hash join + grouped aggregate on the GPU (the nested reffine form of
+reffine_amd.workloads.group_join_c4) against the oracle.
+
+Group keys and COUNTs are bit-exact. Each group's FP64 sum is a left fold
+in ascending probe-row order from 0.0 -- the order of the reference's inner
+loop -- so it is compared bit-exactly with np.add.at (sequential), and
+within 1e-12 relative of the correctly rounded sum (math.fsum).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import brute
+from reffine_amd import DeviceTable, ReffineError, compile_op, datagen, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _upload(b, p, pvalid=None, bvalid=None):
+    B = DeviceTable.from_columns([b["bk"], b["bd"], b["bf"]], dim=1, validity=bvalid)
+    P = DeviceTable.from_columns([p["pk"], p["pv"], p["pq"]], dim=1, validity=pvalid)
+    return B, P
+
+
+def _cols(out):
+    return out.column(0), out.column(1), out.column(2)
+
+
+def _check(out, b, p, bf_eq=1, pq_gt=10, fsum_groups=2000):
+    keys, sums, counts, ranks, vals = brute.group_join_c4(b, p, bf_eq, pq_gt)
+    k, s, c = _cols(out)
+    assert np.array_equal(k, keys)
+    assert np.array_equal(c, counts)
+    assert np.array_equal(s.view(np.int64), sums.view(np.int64)), "fold order differs"
+    ng = min(len(keys), fsum_groups)
+    if ng:
+        sel = ranks < ng
+        want = brute.group_fsums(ng, ranks[sel], vals[sel])
+        err = max(brute.rel_err(float(a), float(w)) for a, w in zip(s[:ng], want))
+        assert err <= 1e-12
+
+
+def test_group_join_matches_nested_loops(device):
+    b, p = datagen.join_tables(300, 6000, seed=5)
+    B, P = _upload(b, p)
+    q = compile_op(workloads.group_join_c4())
+    out = q(B, P)
+    want = brute.group_join_nested(b, p)
+    k, s, c = _cols(out)
+    assert [(int(x), float(y), int(z)) for x, y, z in zip(k, s, c)] == want
+
+
+@pytest.mark.parametrize("nb,np_", [(1, 10), (1000, 1), (5000, 100_000), (100_000, 2_000_000)])
+def test_group_join_sizes(device, nb, np_):
+    b, p = datagen.join_tables(nb, np_, seed=nb + np_)
+    B, P = _upload(b, p)
+    out = compile_op(workloads.group_join_c4())(B, P)
+    _check(out, b, p)
+
+
+def test_group_join_full_size(device):
+    """1.5M build rows, 60M probe rows (about 300K groups, 9.6M joined)."""
+    b, p = datagen.join_tables(1_500_000, 60_000_000, seed=11)
+    B, P = _upload(b, p)
+    q = compile_op(workloads.group_join_c4())
+    out = q(B, P)
+    _check(out, b, p)
+    out2 = q(B, P)  # a second run reuses the workspace
+    assert np.array_equal(out2.column(1).view(np.int64), out.column(1).view(np.int64))
+
+
+def test_group_join_constants_and_empty(device):
+    b, p = datagen.join_tables(2000, 50_000, seed=3)
+    B, P = _upload(b, p)
+    # no qualifying build key
+    out = compile_op(workloads.group_join_c4(bf_eq=9))(B, P)
+    assert out.nrows == 0
+    # no qualifying probe row: every group has COUNT 0 and SUM 0.0
+    out = compile_op(workloads.group_join_c4(pq_gt=100))(B, P)
+    k, s, c = _cols(out)
+    assert len(k) == int((b["bf"] == 1).sum()) and not c.any() and not s.any()
+    for bf_eq, pq_gt in ((0, 1), (4, 49)):
+        _check(compile_op(workloads.group_join_c4(bf_eq, pq_gt))(B, P), b, p, bf_eq, pq_gt)
+
+
+def test_group_join_keys_not_dense(device):
+    """Negative, sparse and extreme keys (the index holds any int64)."""
+    rng = np.random.default_rng(4)
+    nb = 3000
+    keys = np.unique(rng.integers(-2**62, 2**62, size=nb, dtype=np.int64))
+    keys[0] = np.iinfo(np.int64).min
+    keys[-1] = np.iinfo(np.int64).max
+    keys = rng.permutation(keys)
+    nb = len(keys)
+    b = {"bk": keys, "bd": np.zeros(nb, np.int32), "bf": rng.integers(0, 2, nb).astype(np.int8)}
+    n = 200_000
+    pk = keys[rng.integers(0, nb, n)]
+    pk[::7] = rng.integers(-2**62, 2**62, size=len(pk[::7]), dtype=np.int64)  # mostly misses
+    p = {"pk": pk, "pv": rng.random(n), "pq": rng.integers(1, 51, n).astype(np.int64)}
+    B, P = _upload(b, p)
+    _check(compile_op(workloads.group_join_c4())(B, P), b, p)
+
+
+def test_group_join_nulls(device):
+    """Null build keys are not iterated; null probe keys join nothing
+    (the validity of the iterator column, iter_space.cpp:60-66)."""
+    b, p = datagen.join_tables(4000, 100_000, seed=8)
+    rng = np.random.default_rng(1)
+    bv = rng.random(4000) > 0.1
+    pv = rng.random(100_000) > 0.2
+    B, P = _upload(b, p, pvalid=[pv, None, None], bvalid=[bv, None, None])
+    out = compile_op(workloads.group_join_c4())(B, P)
+    b2 = {k: v[bv] for k, v in b.items()}
+    p2 = {k: v[pv] for k, v in p.items()}
+    _check(out, b2, p2)
+
+
+def test_group_join_duplicate_build_keys_rejected(device):
+    b, p = datagen.join_tables(1000, 10_000, seed=2)
+    b["bk"] = b["bk"].copy()
+    b["bf"] = np.ones(1000, np.int8)
+    b["bk"][10] = b["bk"][20]
+    B, P = _upload(b, p)
+    with pytest.raises(ReffineError, match="more than once"):
+        compile_op(workloads.group_join_c4())(B, P)
+
+
+def test_group_join_timing(device):
+    b, p = datagen.join_tables(50_000, 1_000_000, seed=9)
+    B, P = _upload(b, p)
+    q = compile_op(workloads.group_join_c4())
+    total, scan = q.time_kernels(B, P, iters=3)
+    assert total > 0 and 0 < scan <= total

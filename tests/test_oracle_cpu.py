@@ -114,3 +114,30 @@ def test_datagen_is_a_pure_function_of_seed_col_row():
     g = datagen.group_table(100_000)
     f1 = np.bincount(g["f1"]) / 100_000
     assert abs(f1[0] - 0.25) < 0.01 and abs(f1[2] - 0.5) < 0.01
+
+
+def test_group_join_oracle_against_nested_loops():
+    """The brute-force group join (sorted keys, searchsorted, sequential
+    np.add.at) equals the reference form's nested loops bit for bit,
+    including groups without joined rows and unsorted build keys."""
+    for seed, nb, n in ((1, 40, 3000), (2, 200, 5000), (3, 1, 100)):
+        b, p = datagen.join_tables(nb, n, seed=seed)
+        keys, sums, counts, _, _ = brute.group_join_c4(b, p)
+        want = brute.group_join_nested(b, p)
+        got = [(int(k), float(s), int(c)) for k, s, c in zip(keys, sums, counts)]
+        assert got == want
+    # a planted mistake (pairwise instead of sequential order) is caught
+    b, p = datagen.join_tables(50, 20000, seed=4)
+    keys, sums, counts, ranks, vals = brute.group_join_c4(b, p)
+    pair = np.array([math.fsum([0.0]) if not (ranks == g).any() else
+                     np.sum(vals[ranks == g]) for g in range(len(keys))])
+    assert not np.array_equal(pair.view(np.int64), sums.view(np.int64))
+    assert [(int(k), float(s), int(c)) for k, s, c in zip(keys, sums, counts)] == \
+        brute.group_join_nested(b, p)
+
+
+def test_group_fsums_are_correctly_rounded():
+    ranks = np.array([0, 1, 0, 1, 0], dtype=np.int64)
+    vals = np.array([1e16, 1.0, 1.0, 2.0, -1e16])
+    s = brute.group_fsums(3, ranks, vals)
+    assert list(s) == [1.0, 3.0, 0.0]
