@@ -47,3 +47,48 @@ def time_c2(rows: int, seed: int = 2, reps: int = 3) -> dict:
             "sample": f"{rows} rows of the same 5-column layout, median of {reps} runs; {what}; "
                       f"host {_host()} ({os.cpu_count()} logical CPUs)",
             "rows_per_s": round(rows / s)}
+
+
+def _median_time(run, reps: int) -> float:
+    run()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times))
+
+
+def time_c3(rows: int, seed: int = 7, reps: int = 3) -> dict:
+    """Times the grouped aggregate (TPC-H Q1 idiom) with the CPU JIT
+    restatement on `rows` rows of the same 8-column layout."""
+    from reffine_amd import datagen, workloads
+    from . import cjit
+    c = datagen.group_table(rows, seed)
+    fn = cjit.compile_op(workloads.group_agg_c3())
+    inp = {"cols": [c["rowid"], c["f1"], c["f2"], c["qty"], c["price"], c["disc"], c["tax"],
+                    c["d"]]}
+    s = _median_time(lambda: fn(inp), reps)
+    return {"value": round(38.0 * rows / s / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"{rows} rows of the same 8-column layout, median of {reps} runs; oracle/cjit: "
+                      f"CEmitter-style C, clang -O3, single thread; host {_host()} "
+                      f"({os.cpu_count()} logical CPUs)",
+            "rows_per_s": round(rows / s)}
+
+
+def time_c4(n_build: int, n_probe: int, seed: int = 11, reps: int = 3) -> dict:
+    """Times the hash join + grouped aggregate with the numpy restatement
+    (sort the build keys, binary-search each probe key, sequential per-group
+    fold) on a sample; the reference's own CPU form is a nested loop over
+    build x probe rows, far slower at these sizes."""
+    from reffine_amd import datagen
+    from . import brute
+    b, p = datagen.join_tables(n_build, n_probe, seed)
+    s = _median_time(lambda: brute.group_join_c4(b, p), reps)
+    keys = int((b["bf"] == 1).sum())
+    nbytes = 9 * n_build + 24 * n_probe + 24 * keys
+    return {"value": round(nbytes / s / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"build {n_build} / probe {n_probe} rows of the same layout, median of {reps} "
+                      f"runs; oracle/brute: numpy sort + searchsorted + sequential add.at, single "
+                      f"thread; host {_host()} ({os.cpu_count()} logical CPUs)",
+            "rows_per_s": round(n_probe / s)}

@@ -878,6 +878,7 @@ GroupJoinState::~GroupJoinState() {
     dev_free(hkeys);
     dev_free(hrows);
     dev_free(hist);
+    dev_free(filter);
     dev_free(rank2);
     for (void* p : pay2) dev_free(p);
     dev_free(seg);
@@ -1043,11 +1044,19 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
     dev_fill_u64(small + 3, (uint64_t)INT64_MAX, 1);
     RF_HIP(aot_index_build(skeys, nullptr, K, G.hkeys, G.hrows, cap - 1, small + 3, s));
     RF_HIP(aot_adjacent_dups(skeys, K, small + 2, s));
+    // Bitmap filter: about 32 bits per key, at most 1 MiB (cache-resident).
+    uint64_t fbits = 1024;
+    while (fbits < 32 * (uint64_t)K && fbits < ((uint64_t)1 << 23)) fbits <<= 1;
+    grow(G.filter, G.filter_cap, fbits / 8);
+    dev_fill_u64(G.filter, 0, (int64_t)(fbits / 64));
+    RF_HIP(aot_filter_build(skeys, K, G.filter, fbits - 1, s));
     RfProbeDev h{};
     h.keys = (const int64_t*)G.hkeys;
     h.rows = (const int64_t*)G.hrows;
     h.mask = cap - 1;
     h.min_key_row = (kmin == RF_EMPTY_KEY) ? 0 : -1;  // the smallest key has rank 0
+    h.filter = (const uint32_t*)G.filter;
+    h.fmask = fbits - 1;
     for (size_t j = 0; j < gp.probe_plan.probes.size(); j++)
         if (gp.probe_plan.probes[j].external) pq->probe_override[j] = h;
 

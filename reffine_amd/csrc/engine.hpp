@@ -58,6 +58,8 @@ hipError_t aot_radix_sort(void* const keys[2], void* const pay[2][8], const int3
 hipError_t aot_minmax_i64(const int64_t* keys, const int64_t* n_dev, int64_t n_max, void* out,
                           hipStream_t s);
 hipError_t aot_adjacent_dups(const int64_t* keys, int64_t n, void* flag, hipStream_t s);
+hipError_t aot_filter_build(const int64_t* keys, int64_t n, void* filter, uint64_t fmask,
+                           hipStream_t s);
 hipError_t aot_seg_bounds(const uint32_t* rank, int64_t n, int64_t* lo, int64_t* hi,
                           hipStream_t s);
 
@@ -191,6 +193,8 @@ struct GroupJoinState {
     void* hkeys = nullptr;                   // key -> rank index
     void* hrows = nullptr;
     size_t hcap = 0;
+    void* filter = nullptr;                  // bitmap filter of the index keys
+    size_t filter_cap = 0;
     void* hist = nullptr;
     size_t hist_cap = 0;
     void* rank2 = nullptr;                   // sort ping-pong buffers (joined rows)

@@ -39,6 +39,7 @@ struct Gen {
     KernelSource& ks;
     std::map<std::string, int> param_ix;
     std::map<std::pair<int, int>, int> col_ix;
+    std::set<int> carried;  // probes the row code computes; emit reuses them
 
     Gen(const Plan& p, const std::vector<std::vector<bool>>& nl, KernelSource& k)
         : plan(p), nullable(nl), ks(k) {}
@@ -114,6 +115,8 @@ struct Emitter {
 
     std::string probe(int j) {
         std::string h = "h" + std::to_string(j);
+        if (mode == Mode::EMIT && g.carried.count(j))
+            return "cy.p" + std::to_string(j) + "[s][k]";
         if (!probes_done.count(j)) {
             const Probe& pr = g.plan.probes[j];
             std::string k = expr(pr.key);
@@ -125,6 +128,7 @@ struct Emitter {
                 os << indent << "const int64_t " << h << " = rf_probe(L.probe[" << j
                    << "], (int64_t)" << k << ");\n";
             }
+            if (mode == Mode::ROWS) os << indent << "cy.p" << j << "[s][k] = " << h << ";\n";
             probes_done.insert(j);
         }
         return h;
@@ -337,6 +341,17 @@ Tuning default_tuning(const Plan& plan) {
     const int budget = t.pipeline ? 640 : 320;  // load bytes per lane per tile
     t.sub = 4;
     if (row_bytes > 0) t.sub = std::max(1, std::min(16, budget / (t.items * row_bytes)));
+    // At most 32 rows per lane per wave-tile: narrow columns would otherwise
+    // unpack into more registers than a wave has.
+    t.sub = std::min(t.sub, std::max(1, 32 / t.items));
+    if (!plan.probes.empty()) {
+        // Hash probes are dependent random reads: latency, not bandwidth,
+        // bounds the scan. Few rows per lane, one register set and many
+        // resident waves keep more probes in flight.
+        t.pipeline = 0;
+        t.sub = std::max(1, 8 / t.items);
+        t.waves_per_cu = 16;
+    }
     return t;
 }
 
@@ -379,6 +394,7 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
     }
     std::string sel_pred;
     if (ks.has_sel) sel_pred = em.expr(plan.ops[0].pred);
+    g.carried = em.probes_done;
 
     std::ostringstream o;
     o << "// Generated row code for reffine function '" << plan.func_name << "'.\n";
@@ -390,6 +406,10 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
     o << "    struct Acc {";
     for (size_t i = 0; i < ks.acc.size(); i++) o << " " << cty(ks.acc[i].type) << " a" << i << ";";
     if (ks.acc.empty()) o << " int unused;";
+    o << " };\n";
+    o << "    struct Carry {";
+    for (int j : g.carried) o << " int64_t p" << j << "[SUB][ITEMS];";
+    if (g.carried.empty()) o << " int unused;";
     o << " };\n";
     o << "    static __device__ __forceinline__ void acc_zero(Acc& a) {";
     for (size_t i = 0; i < ks.acc.size(); i++)
@@ -424,7 +444,8 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
     }
     // rows(): one sub-step with bounds (edge tiles).
     o << "    template <bool FULL>\n"
-      << "    static __device__ __forceinline__ uint32_t rows(Acc& a, const RfScanArgs& L, int64_t r) {\n";
+      << "    static __device__ __forceinline__ uint32_t rows(Acc& a, const RfScanArgs& L, int64_t r, "
+         "Carry& cy, int s) {\n";
     for (size_t s = 0; s < ks.cols.size(); s++) {
         const ColSlot& c = ks.cols[s];
         if (!c.vector) continue;
@@ -436,7 +457,7 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
             o << "        const uint32_t vb" << s << " = rf_valid_bits<ITEMS>(L.valid[" << s
               << "], r);\n";
     }
-    o << "        return row_block<FULL>(a, L, r" << vargs << ");\n    }\n";
+    o << "        return row_block<FULL>(a, L, r, cy, s" << vargs << ");\n    }\n";
     // Tile: the registers of one interior wave-tile; load() issues every
     // sub-step's loads, compute() evaluates the rows (the skeleton overlaps
     // the next tile's load() with this tile's compute()).
@@ -474,16 +495,16 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
         if (c.nullable) targs_s += ", T.vb" + std::to_string(s) + "[s]";
     }
     o << "    static __device__ __forceinline__ void compute(Acc& a, const Tile& T, "
-         "const RfScanArgs& L, int64_t row0, uint32_t lane, uint32_t* bits) {\n"
+         "const RfScanArgs& L, int64_t row0, uint32_t lane, uint32_t* bits, Carry& cy) {\n"
       << "        const int64_t rb = row0 + (int64_t)lane * ITEMS;\n        (void)T;\n"
       << "#pragma unroll\n        for (int s = 0; s < SUB; s++) {\n"
       << "            const int64_t r = rb + (int64_t)s * 64 * ITEMS;\n"
-      << "            bits[s] = row_block<true>(a, L, r" << targs_s << ");\n        }\n    }\n";
+      << "            bits[s] = row_block<true>(a, L, r, cy, s" << targs_s << ");\n        }\n    }\n";
     // row_block(): the query's row code over ITEMS loaded rows.
     o << "    template <bool FULL>\n"
       << "    static __device__ __forceinline__ uint32_t row_block(Acc& a, const RfScanArgs& L, "
-         "int64_t r" << vparams << ") {\n";
-    o << "        uint32_t sel = 0;\n";
+         "int64_t r, Carry& cy, int s" << vparams << ") {\n";
+    o << "        uint32_t sel = 0;\n        (void)cy; (void)s;\n";
     o << "#pragma unroll\n        for (int k = 0; k < ITEMS; k++) {\n";
     o << "            const int64_t row = r + k;\n";
     o << "            const bool inb = FULL || (row >= L.lo && row < L.end);\n";
@@ -500,7 +521,8 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
 
     // ---- emit(): selection outputs of one qualifying row, into staging;
     // move(): one staged row to its output position.
-    o << "    static __device__ __forceinline__ void emit(const RfScanArgs& L, int64_t row, int64_t pos) {\n";
+    o << "    static __device__ __forceinline__ void emit(const RfScanArgs& L, int64_t row, int64_t pos, "
+         "const Carry& cy, int s, int k) {\n        (void)cy; (void)s; (void)k;\n";
     std::ostringstream mv;
     if (ks.has_sel) {
         std::ostringstream eb;
@@ -562,7 +584,7 @@ KernelSource gen_group_fold(const GroupJoinPlan& gj, const std::vector<std::vect
     for (size_t f = 0; f < gj.fields.size(); f++)
         if (gj.fields[f].updated && gj.fields[f].const_contrib) cv[f] = ec.expr(gj.fields[f].const_contrib);
     o << cst.str();
-    o << "        for (int64_t i = i0; i < i1; i++) {\n";
+    o << "#pragma unroll 4\n        for (int64_t i = i0; i < i1; i++) {\n";
     for (size_t f = 0; f < gj.fields.size(); f++) {
         const GField& gf = gj.fields[f];
         if (!gf.updated) continue;

@@ -36,6 +36,8 @@ struct RfProbeDev {
     const int64_t* rows;   // slot row (minimum row holding the key)
     uint64_t mask;         // capacity - 1 (capacity a power of two)
     int64_t min_key_row;   // row of key == RF_EMPTY_KEY, -1 if none
+    const uint32_t* filter;  // optional bitmap of (hash >> 32) & fmask over the keys
+    uint64_t fmask;          // filter bits - 1 (a power of two)
 };
 
 struct RfScanArgs {

@@ -102,7 +102,14 @@ __device__ __forceinline__ uint64_t rf_hash64(uint64_t k) {
 
 __device__ __forceinline__ int64_t rf_probe(const RfProbeDev& h, int64_t key) {
     if (key == RF_EMPTY_KEY) return h.min_key_row;
-    uint64_t s = rf_hash64((uint64_t)key) & h.mask;
+    const uint64_t hk = rf_hash64((uint64_t)key);
+    if (h.filter) {
+        // Keys absent from the filter are absent from the table: one read
+        // of a small (cache-resident) bitmap instead of a slot read.
+        const uint64_t fb = (hk >> 32) & h.fmask;
+        if (!((h.filter[fb >> 5] >> (fb & 31)) & 1u)) return -1;
+    }
+    uint64_t s = hk & h.mask;
     for (;;) {
         int64_t k = h.keys[s];
         if (k == key) return h.rows[s];
@@ -240,16 +247,17 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
         Acc& a = Q::DET ? a_tile : run;
         if constexpr (Q::DET) Q::acc_zero(a);
         uint32_t bits[SUB];
+        typename Q::Carry cy;  // per-row values the selection's emit reuses
         if (interior) {
-            Q::compute(a, T, L, row0, lane, bits);
+            Q::compute(a, T, L, row0, lane, bits, cy);
         } else {
 #pragma unroll
             for (int s = 0; s < SUB; s++) {
                 const int64_t r = row0 + ((int64_t)s * 64 + lane) * ITEMS;
                 if (r >= L.lo && r + ITEMS <= L.end)
-                    bits[s] = Q::template rows<true>(a, L, r);
+                    bits[s] = Q::template rows<true>(a, L, r, cy, s);
                 else if (r + ITEMS > L.lo && r < L.end)
-                    bits[s] = Q::template rows<false>(a, L, r);
+                    bits[s] = Q::template rows<false>(a, L, r, cy, s);
                 else
                     bits[s] = 0;
             }
@@ -276,12 +284,12 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
                 if (b) {
                     const int64_t r = row0 + ((int64_t)s * 64 + lane) * ITEMS;
                     int64_t pos = wave_base + cursor + below;
-                    uint32_t bb = b;
-                    while (bb) {
-                        const int k = __builtin_ctz(bb);
-                        bb &= bb - 1;
-                        Q::emit(L, r + k, pos);
-                        pos++;
+#pragma unroll
+                    for (int k = 0; k < ITEMS; k++) {
+                        if ((b >> k) & 1u) {
+                            Q::emit(L, r + k, pos, cy, s, k);
+                            pos++;
+                        }
                     }
                 }
                 cursor += total;

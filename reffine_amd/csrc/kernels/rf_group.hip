@@ -120,8 +120,60 @@ extern "C" __global__ __launch_bounds__(RF_SORT_BLOCK) void rf_sort_scatter_u64(
     rf_sort_scatter<uint64_t>(P);
 }
 
-// Exclusive scan of the m histogram entries in place (one block of 1024
-// threads; each thread owns a contiguous range).
+// Exclusive scan of m histogram entries in three launches: per-chunk sums,
+// a one-block scan of the chunk sums, per-chunk scans with the offset.
+#define RF_SCAN_CHUNK 4096
+
+__device__ __forceinline__ uint32_t rf_block_sum_u32(uint32_t v, uint32_t* s) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += s[w];
+    return t;
+}
+
+extern "C" __global__ __launch_bounds__(256) void rf_scan_chunks(const uint32_t* __restrict__ h,
+                                                                 int64_t m, uint32_t* part) {
+    __shared__ uint32_t s[4];
+    const int64_t a = (int64_t)blockIdx.x * RF_SCAN_CHUNK;
+    const int64_t b = a + RF_SCAN_CHUNK < m ? a + RF_SCAN_CHUNK : m;
+    uint32_t v = 0;
+    for (int64_t i = a + threadIdx.x; i < b; i += 256) v += h[i];
+    v = rf_block_sum_u32(v, s);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+extern "C" __global__ __launch_bounds__(256) void rf_scan_apply(uint32_t* h, int64_t m,
+                                                                const uint32_t* __restrict__ part) {
+    __shared__ uint32_t s[256];
+    const int t = threadIdx.x;
+    const int64_t a = (int64_t)blockIdx.x * RF_SCAN_CHUNK + (int64_t)t * 16;
+    uint32_t v[16];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        v[k] = (a + k < m) ? h[a + k] : 0u;
+        sum += v[k];
+    }
+    s[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t x = t >= off ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += x;
+        __syncthreads();
+    }
+    uint32_t run = part[blockIdx.x] + s[t] - sum;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        if (a + k < m) h[a + k] = run;
+        run += v[k];
+    }
+}
+
+// Exclusive scan in place by one block of 1024 threads, each owning a
+// contiguous range (the chunk sums of the scan above).
 extern "C" __global__ __launch_bounds__(1024) void rf_sort_scan(uint32_t* hist, int64_t m) {
     __shared__ uint32_t s[1024];
     const int t = threadIdx.x;
@@ -148,10 +200,12 @@ extern "C" __global__ __launch_bounds__(1024) void rf_sort_scan(uint32_t* hist, 
 
 // Minimum and maximum of n int64 keys (n read from device memory), as
 // sign-flipped unsigned values: out[0] = min ^ 2^63, out[1] = max ^ 2^63.
-// out must hold {~0, 0} before the launch.
+// out must hold {~0, 0} before the launch. One atomic pair per block (a
+// launch of at most 256 blocks): atomics on one address serialise.
 extern "C" __global__ __launch_bounds__(256) void rf_minmax_i64(const int64_t* __restrict__ keys,
                                                                 const int64_t* __restrict__ n_dev,
                                                                 unsigned long long* out) {
+    __shared__ unsigned long long smn[4], smx[4];
     const int64_t n = *n_dev;
     unsigned long long mn = ~0ull, mx = 0ull;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -166,8 +220,28 @@ extern "C" __global__ __launch_bounds__(256) void rf_minmax_i64(const int64_t* _
         mx = b > mx ? b : mx;
     }
     if ((threadIdx.x & 63) == 0) {
+        smn[threadIdx.x >> 6] = mn;
+        smx[threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; w++) {
+            mn = smn[w] < mn ? smn[w] : mn;
+            mx = smx[w] > mx ? smx[w] : mx;
+        }
         atomicMin(&out[0], mn);
         atomicMax(&out[1], mx);
+    }
+}
+
+// Bitmap filter of a key set: bit (hash >> 32) & fmask of every key (the
+// slot index uses the low bits of the same hash).
+extern "C" __global__ void rf_filter_build(const int64_t* __restrict__ keys, int64_t n,
+                                           uint32_t* filter, uint64_t fmask) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t fb = (rf_hash64((uint64_t)keys[i]) >> 32) & fmask;
+        atomicOr(&filter[fb >> 5], 1u << (fb & 31));
     }
 }
 
@@ -202,9 +276,12 @@ static int grid_cap(int64_t n, int block, int cap) {
     return (int)g;
 }
 
+// Histogram entries a sort of n keys needs (digit counts + chunk sums).
 int64_t aot_sort_hist_entries(int64_t n) {
-    const int64_t nb = (n + RF_SORT_TILE - 1) / RF_SORT_TILE;
-    return 256 * (nb < 1 ? 1 : nb);
+    int64_t nb = (n + RF_SORT_TILE - 1) / RF_SORT_TILE;
+    if (nb < 1) nb = 1;
+    const int64_t m = 256 * nb;
+    return m + (m + RF_SCAN_CHUNK - 1) / RF_SCAN_CHUNK;
 }
 
 // Stable LSD radix sort of n keys (4 or 8 bytes) by the low `bits` bits of
@@ -238,7 +315,12 @@ hipError_t aot_radix_sort(void* const keys[2], void* const pay[2][RF_SORT_MAXPAY
             hipLaunchKernelGGL(rf_sort_hist_u32, g, b, 0, s, P);
         else
             hipLaunchKernelGGL(rf_sort_hist_u64, g, b, 0, s, P);
-        hipLaunchKernelGGL(rf_sort_scan, dim3(1), dim3(1024), 0, s, hist, 256 * P.nblocks);
+        const int64_t m = 256 * P.nblocks;
+        const int64_t nch = (m + RF_SCAN_CHUNK - 1) / RF_SCAN_CHUNK;
+        uint32_t* part = hist + m;  // the caller sizes hist for m + chunks entries
+        hipLaunchKernelGGL(rf_scan_chunks, dim3((unsigned)nch), dim3(256), 0, s, hist, m, part);
+        hipLaunchKernelGGL(rf_sort_scan, dim3(1), dim3(1024), 0, s, part, nch);
+        hipLaunchKernelGGL(rf_scan_apply, dim3((unsigned)nch), dim3(256), 0, s, hist, m, part);
         if (key_bytes == 4)
             hipLaunchKernelGGL(rf_sort_scatter_u32, g, b, 0, s, P);
         else
@@ -253,8 +335,16 @@ hipError_t aot_radix_sort(void* const keys[2], void* const pay[2][RF_SORT_MAXPAY
 
 hipError_t aot_minmax_i64(const int64_t* keys, const int64_t* n_dev, int64_t n_max, void* out,
                           hipStream_t s) {
-    hipLaunchKernelGGL(rf_minmax_i64, dim3(grid_cap(n_max, 256, 1024)), dim3(256), 0, s, keys,
+    hipLaunchKernelGGL(rf_minmax_i64, dim3(grid_cap(n_max, 256 * 16, 256)), dim3(256), 0, s, keys,
                        n_dev, (unsigned long long*)out);
+    return hipGetLastError();
+}
+
+hipError_t aot_filter_build(const int64_t* keys, int64_t n, void* filter, uint64_t fmask,
+                           hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rf_filter_build, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, n,
+                       (uint32_t*)filter, fmask);
     return hipGetLastError();
 }
 

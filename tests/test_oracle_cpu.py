@@ -141,3 +141,20 @@ def test_group_fsums_are_correctly_rounded():
     vals = np.array([1e16, 1.0, 1.0, 2.0, -1e16])
     s = brute.group_fsums(3, ranks, vals)
     assert list(s) == [1.0, 3.0, 0.0]
+
+
+def test_cjit_grouped_aggregate_matches_brute():
+    """The CPU JIT restatement of the Q1-idiom grouped aggregate (used as
+    the grouped workload's CPU baseline) agrees with the brute force."""
+    c = datagen.group_table(40_000, 3)
+    fn = cjit.compile_op(workloads.group_agg_c3())
+    (st,) = fn({"cols": [c["rowid"], c["f1"], c["f2"], c["qty"], c["price"], c["disc"],
+                         c["tax"], c["d"]]})
+    want = brute.group_c3(c)
+    flat = list(st)
+    assert len(flat) == len(want)
+    for g, w in zip(flat, want):
+        if isinstance(w, int):
+            assert g == w
+        else:
+            assert brute.rel_err(g, w) <= 1e-12
