@@ -216,6 +216,78 @@ int rf_query_time(rf_query* q, const rf_arg* args, int64_t n_args,
 
 int rf_query_free(rf_query* q);
 
+/* ------------------------------------------------------ batched / served */
+
+/* A batch runner: a round of work units (one compiled query over its
+ * arguments each) run together -- one persistent work-queue launch per
+ * distinct query kernel over all (unit, wave-tile) items, then per-unit
+ * completion and selection placement. Each unit's result equals the
+ * deterministic-mode run of the same query alone (RF_COMPILE_DETERMINISTIC
+ * queries give identical bits in both). `max_units` units per round;
+ * `arena_bytes` of device memory for the units' partials and selection
+ * buffers (admission by footprint: rf_batch_add returns RF_ERR_BUSY when a
+ * unit does not fit); `use_graph` != 0 captures a round into a hipGraph once
+ * per set of query kernels and replays it. No reference counterpart (the
+ * reference runs one query per call, benchmark/main.cpp:76-82). */
+typedef struct rf_batch rf_batch;
+int rf_batch_create(int32_t max_units, int64_t arena_bytes, int32_t use_graph,
+                    rf_batch** out);
+/* Arena bytes one unit of `q` over `args` needs. */
+int rf_batch_footprint(rf_query* q, const rf_arg* args, int64_t n_args,
+                       int64_t* bytes);
+/* Adds a unit (the arguments are copied; the tables must stay alive until
+ * the round is reset). RF_ERR_BUSY: the round is full. */
+int rf_batch_add(rf_batch* b, rf_query* q, const rf_arg* args, int64_t n_args,
+                 int32_t* unit);
+/* Launches the round asynchronously on the runner's own stream. */
+int rf_batch_launch(rf_batch* b);
+/* *done = 1 once the launched round has finished (no wait). */
+int rf_batch_poll(rf_batch* b, int32_t* done);
+int rf_batch_wait(rf_batch* b);
+/* Output of a unit of a finished round, as rf_query_run gives it; a
+ * selection table is a view of the runner's arena, valid until
+ * rf_batch_reset (free it with rf_table_free). */
+int rf_batch_result(rf_batch* b, int32_t unit, void* output_addr,
+                    rf_table** out_table);
+/* Waits for the round if needed and starts an empty one. */
+int rf_batch_reset(rf_batch* b);
+int rf_batch_free(rf_batch* b);
+
+/* One request of the serving loop. Inputs: the compiled query, its
+ * arguments, the arrival time (microseconds from the start of the run,
+ * ascending over the array), where to write the reduction output
+ * (output_addr, may be NULL) and the selection's first column (sel_out,
+ * sel_cap bytes, may be NULL). Outputs: out_rows (selection rows),
+ * done_us (completion time) and status (RF_OK or an error code). */
+typedef struct rf_request {
+    rf_query* q;
+    const rf_arg* args;
+    int64_t n_args;
+    double arrival_us;
+    void* output_addr;
+    void* sel_out;
+    int64_t sel_cap;
+    int64_t out_rows;
+    double done_us;
+    int32_t status;
+    int32_t pad;
+} rf_request;
+
+/* Serves `n` requests as they arrive (a real-time loop against the arrival
+ * times) with continuous batching over two batch runners on two streams:
+ * requests that arrive while a round is on the GPU are admitted into the
+ * other runner as units and arena memory allow, and it launches at once, so
+ * rounds overlap on the device. Latency of request i = done_us - arrival_us.
+ * `wall_us` receives the time from start to the last completion. No
+ * reference counterpart. */
+int rf_serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes,
+             int32_t use_graph, double* wall_us);
+
+/* The plain counterpart of rf_serve for the same requests: each one, once
+ * it has arrived, runs alone through rf_query_run (its own launches and
+ * host synchronisation), in arrival order. */
+int rf_serve_plain(rf_request* reqs, int64_t n, double* wall_us);
+
 /* Times a plain 16-bytes-per-lane device copy of `bytes` (read + write)
  * over `iters` launches; the measured HBM ceiling the scan kernels are
  * compared with, and the reference size for calibrating byte counters. No

@@ -291,6 +291,89 @@ int rf_query_time(rf_query* q, const rf_arg* args, int64_t n_args, int32_t iters
     });
 }
 
+int rf_batch_create(int32_t max_units, int64_t arena_bytes, int32_t use_graph, rf_batch** out) {
+    return guard([&] {
+        need(out != nullptr, "NULL argument");
+        *out = rf::batch_create(max_units, arena_bytes, use_graph != 0);
+    });
+}
+
+int rf_batch_footprint(rf_query* q, const rf_arg* args, int64_t n_args, int64_t* bytes) {
+    return guard([&] {
+        need(q && bytes && (args || n_args == 0), "NULL argument");
+        *bytes = rf::batch_footprint(q, args, n_args);
+    });
+}
+
+int rf_batch_add(rf_batch* b, rf_query* q, const rf_arg* args, int64_t n_args, int32_t* unit) {
+    int busy = 0;
+    int st = guard([&] {
+        need(b && q && unit && (args || n_args == 0), "NULL argument");
+        if (!rf::batch_add(b, q, args, n_args, unit)) busy = 1;
+    });
+    if (st == RF_OK && busy) {
+        g_last_error = "round full: no unit slot or arena memory left";
+        return RF_ERR_BUSY;
+    }
+    return st;
+}
+
+int rf_batch_launch(rf_batch* b) {
+    return guard([&] {
+        need(b != nullptr, "NULL argument");
+        rf::batch_launch(b);
+    });
+}
+
+int rf_batch_poll(rf_batch* b, int32_t* done) {
+    return guard([&] {
+        need(b && done, "NULL argument");
+        *done = rf::batch_poll(b) ? 1 : 0;
+    });
+}
+
+int rf_batch_wait(rf_batch* b) {
+    return guard([&] {
+        need(b != nullptr, "NULL argument");
+        rf::batch_wait(b);
+    });
+}
+
+int rf_batch_result(rf_batch* b, int32_t unit, void* output_addr, rf_table** out_table) {
+    return guard([&] {
+        need(b != nullptr, "NULL argument");
+        rf::batch_result(b, unit, output_addr, out_table);
+    });
+}
+
+int rf_batch_reset(rf_batch* b) {
+    return guard([&] {
+        need(b != nullptr, "NULL argument");
+        rf::batch_reset(b);
+    });
+}
+
+int rf_batch_free(rf_batch* b) {
+    return guard([&] { delete b; });
+}
+
+int rf_serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes, int32_t use_graph,
+             double* wall_us) {
+    return guard([&] {
+        need(reqs || n == 0, "NULL argument");
+        const double w = rf::serve(reqs, n, max_units, arena_bytes, use_graph != 0);
+        if (wall_us) *wall_us = w;
+    });
+}
+
+int rf_serve_plain(rf_request* reqs, int64_t n, double* wall_us) {
+    return guard([&] {
+        need(reqs || n == 0, "NULL argument");
+        const double w = rf::serve_plain(reqs, n);
+        if (wall_us) *wall_us = w;
+    });
+}
+
 int rf_calib_copy(int64_t bytes, int32_t iters, double* ms_per_copy) {
     return guard([&] {
         need(ms_per_copy && bytes >= 16 && iters > 0, "bad argument");

@@ -281,7 +281,15 @@ Module& jit_module(const KernelSource& ks) {
     auto m = std::make_unique<Module>();
     RF_HIP(hipModuleLoadData(&m->mod, code.data()));
     RF_HIP(hipModuleGetFunction(&m->fn, m->mod, ks.entry.c_str()));
-    if (ks.entry == "rf_q") RF_HIP(hipModuleGetFunction(&m->gather, m->mod, "rf_q_gather"));
+    if (ks.entry == "rf_q") {
+        RF_HIP(hipModuleGetFunction(&m->gather, m->mod, "rf_q_gather"));
+        RF_HIP(hipModuleGetFunction(&m->bscan, m->mod, "rf_qb"));
+        RF_HIP(hipModuleGetFunction(&m->bdone, m->mod, "rf_qb_done"));
+        RF_HIP(hipModuleGetFunction(&m->bgather, m->mod, "rf_qb_gather"));
+        int nb = 0;
+        RF_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, m->bscan, ks.tune.block, 0));
+        m->bscan_blocks_per_cu = std::max(1, nb);
+    }
     int nb = 0;
     RF_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, m->fn, ks.tune.block, 0));
     m->max_blocks_per_cu = std::max(1, nb);
@@ -668,6 +676,28 @@ static void check_args(const Plan& plan, const rf_arg* args, int64_t n_args) {
     }
 }
 
+// Output column descriptor (Arrow format, storage type) of `count` rows.
+static Column out_column(const Type& ty, int64_t count) {
+    Column col;
+    col.type = ty;
+    switch (ty.bt) {
+        case BT::BOOL: case BT::INT8: col.format = "c"; break;
+        case BT::UINT8: col.format = "C"; break;
+        case BT::INT16: col.format = "s"; break;
+        case BT::UINT16: col.format = "S"; break;
+        case BT::INT32: col.format = "i"; break;
+        case BT::UINT32: col.format = "I"; break;
+        case BT::INT64: case BT::IDX: col.format = "l"; col.type = types::I64; break;
+        case BT::UINT64: col.format = "L"; break;
+        case BT::FLOAT32: col.format = "f"; break;
+        case BT::FLOAT64: col.format = "g"; break;
+        default: throw IRError("output column type");
+    }
+    if (ty.bt == BT::BOOL) col.type = types::I8;
+    col.length = col.logical_length = count;
+    return col;
+}
+
 // A device table of `count` rows with fresh columns of `types` (one arena).
 static rf_table* alloc_table(const std::vector<Type>& types, int64_t count, int64_t dim) {
     auto t = std::make_unique<rf_table>();
@@ -681,30 +711,14 @@ static rf_table* alloc_table(const std::vector<Type>& types, int64_t count, int6
     char* arena = (char*)dev_alloc(std::max<size_t>(total, 256));
     t->owned.push_back(arena);
     for (size_t j = 0; j < types.size(); j++) {
-        Column col;
-        col.type = types[j];
-        switch (col.type.bt) {
-            case BT::BOOL: case BT::INT8: col.format = "c"; break;
-            case BT::UINT8: col.format = "C"; break;
-            case BT::INT16: col.format = "s"; break;
-            case BT::UINT16: col.format = "S"; break;
-            case BT::INT32: col.format = "i"; break;
-            case BT::UINT32: col.format = "I"; break;
-            case BT::INT64: case BT::IDX: col.format = "l"; col.type = types::I64; break;
-            case BT::UINT64: col.format = "L"; break;
-            case BT::FLOAT32: col.format = "f"; break;
-            case BT::FLOAT64: col.format = "g"; break;
-            default: throw IRError("output column type");
-        }
-        if (types[j].bt == BT::BOOL) col.type = types::I8;
-        col.length = col.logical_length = count;
+        Column col = out_column(types[j], count);
         col.values = arena + offs[j];
         t->cols.push_back(col);
     }
     return t.release();
 }
 
-Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/) {
+Launch prepare_args(rf_query* q, const rf_arg* args, int64_t n_args) {
     const Plan& plan = q->plan;
     check_args(plan, args, n_args);
     Launch l;
@@ -752,6 +766,15 @@ Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/
         A.probe[j] = probe_of(t);
     }
     fill_params(ks, c, A.p);
+    return l;
+}
+
+Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/) {
+    Launch l = prepare_args(q, args, n_args);
+    const KernelSource& ks = l.spec->ks;
+    RfScanArgs& A = l.args;
+    const int64_t tile = (int64_t)ks.tune.sub * 64 * ks.tune.items;
+    const int64_t lo = l.lo, end = l.end;
     // Workspace.
     Device& d = device();
     const Module& m = *l.spec->mod;
@@ -1156,16 +1179,10 @@ void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_add
             RF_HIP(hipMemcpyAsync(stage + 4096, q->out_count, 8, hipMemcpyDeviceToHost, d.stream));
     }
     RF_HIP(hipStreamSynchronize(d.stream));
-    const Plan& plan = q->plan;
-    EvalCtx c{&plan, args, n_args, {}, {}};
-    for (auto& kv : plan.tbl) c.tbl[kv.first->name] = kv.second;
-    for (size_t r = 0; r < plan.reduces.size(); r++)
-        c.comps[plan.reduces[r].sym_name] = finish_reduce(plan, ks, r, stage, l.empty, c);
     rf_table* sel = nullptr;
-    if (!plan.ops.empty()) {
+    if (!q->plan.ops.empty()) {
         int64_t count = l.empty ? 0 : *(int64_t*)(stage + 4096);
-        const OpComp& oc = plan.ops[0];
-        std::unique_ptr<rf_table> t(alloc_table(ks.out_types, count, oc.n_iters));
+        std::unique_ptr<rf_table> t(alloc_table(ks.out_types, count, q->plan.ops[0].n_iters));
         for (size_t j = 0; j < ks.out_types.size(); j++)
             if (count > 0)
                 RF_HIP(hipMemcpyAsync(t->cols[j].values, q->outs[j],
@@ -1173,10 +1190,23 @@ void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_add
                                       hipMemcpyDeviceToDevice, d.stream));
         RF_HIP(hipStreamSynchronize(d.stream));
         sel = t.release();
+    }
+    finish_output(q, ks, args, n_args, stage, l.empty, sel, output_addr, out_table);
+}
+
+void finish_output(rf_query* q, const KernelSource& ks, const rf_arg* args, int64_t n_args,
+                   const char* acc, bool empty, rf_table* sel, void* output_addr,
+                   rf_table** out_table) {
+    const Plan& plan = q->plan;
+    EvalCtx c{&plan, args, n_args, {}, {}};
+    for (auto& kv : plan.tbl) c.tbl[kv.first->name] = kv.second;
+    for (size_t r = 0; r < plan.reduces.size(); r++)
+        c.comps[plan.reduces[r].sym_name] = finish_reduce(plan, ks, r, acc, empty, c);
+    if (!plan.ops.empty()) {
         HostVal hv;
-        hv.type = oc.vec_type;
+        hv.type = plan.ops[0].vec_type;
         hv.table = sel;
-        c.comps[oc.sym_name] = hv;
+        c.comps[plan.ops[0].sym_name] = hv;
     }
     // Func output: evaluated over the component results (a struct output
     // that holds an Op result carries the table handle in its field).
@@ -1192,6 +1222,18 @@ void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_add
     }
     if (output_addr) write_value(out, plan.output_type, (char*)output_addr);
     if (out_table) *out_table = sel;
+}
+
+rf_table* table_view(const std::vector<Type>& types, const std::vector<void*>& values,
+                     int64_t count, int64_t dim) {
+    auto t = std::make_unique<rf_table>();
+    t->dim = dim;
+    for (size_t j = 0; j < types.size(); j++) {
+        Column col = out_column(types[j], count);
+        col.values = values[j];
+        t->cols.push_back(col);
+    }
+    return t.release();
 }
 
 void time_query(rf_query* q, const rf_arg* args, int64_t n_args, int iters, double* ms_all,

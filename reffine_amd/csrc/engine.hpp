@@ -107,7 +107,11 @@ struct Module {
     hipModule_t mod = nullptr;
     hipFunction_t fn = nullptr;      // the entry: rf_q (scan) or rf_gj_fold
     hipFunction_t gather = nullptr;  // rf_q_gather: selection placement (scan modules)
+    hipFunction_t bscan = nullptr;   // rf_qb: batched persistent scan
+    hipFunction_t bdone = nullptr;   // rf_qb_done: per-unit completion
+    hipFunction_t bgather = nullptr; // rf_qb_gather: per-unit selection placement
     int max_blocks_per_cu = 1;
+    int bscan_blocks_per_cu = 1;
 };
 
 // Compiles (or fetches from the cache) the specialised scan kernel.
@@ -169,6 +173,18 @@ struct Launch {
 
 // Prepares arguments and workspace for one run (no launch).
 Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* ok);
+// Arguments of one run (columns, probes, constants, rows, tiles) without
+// the query's workspace pointers.
+Launch prepare_args(rf_query* q, const rf_arg* args, int64_t n_args);
+// Host-side finish of a run: the function output from the reductions'
+// device result bytes `acc` and the selection table `sel`.
+void finish_output(rf_query* q, const KernelSource& ks, const rf_arg* args, int64_t n_args,
+                   const char* acc, bool empty, rf_table* sel, void* output_addr,
+                   rf_table** out_table);
+// A table over borrowed device columns (freed with table_free, which then
+// releases only the host descriptor).
+rf_table* table_view(const std::vector<Type>& types, const std::vector<void*>& values,
+                     int64_t count, int64_t dim);
 void launch(rf_query* q, Launch& l, hipStream_t s, bool scan_only);
 // Launch + download + host-side finish; returns the output.
 void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_addr,
@@ -204,6 +220,19 @@ struct GroupJoinState {
     size_t seg_cap = 0;
     ~GroupJoinState();
 };
+
+// Batched execution (engine_batch.cpp).
+rf_batch* batch_create(int cap, int64_t arena_bytes, bool use_graph);
+int64_t batch_footprint(rf_query* q, const rf_arg* args, int64_t n_args);
+bool batch_add(rf_batch* b, rf_query* q, const rf_arg* args, int64_t n_args, int32_t* unit);
+void batch_launch(rf_batch* b);
+bool batch_poll(rf_batch* b);
+void batch_wait(rf_batch* b);
+void batch_result(rf_batch* b, int32_t unit, void* output_addr, rf_table** out_table);
+void batch_reset(rf_batch* b);
+int batch_units(const rf_batch* b);
+double serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes, bool use_graph);
+double serve_plain(rf_request* reqs, int64_t n);
 
 // Creates the sub-queries of a group join and compiles its all-valid
 // kernels; `probe_tune` (optional) sets the probe scan's launch shape.

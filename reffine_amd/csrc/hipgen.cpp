@@ -550,6 +550,12 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
       << ") void rf_q(RfScanArgs L) { rf_scan<Q>(L); }\n";
     o << "extern \"C\" __global__ __launch_bounds__(256) void rf_q_gather(RfScanArgs L) { "
          "rf_gather<Q>(L); }\n";
+    o << "extern \"C\" __global__ __launch_bounds__(" << tune.block
+      << ") void rf_qb(RfBatchArgs B) { rf_batch_scan<Q>(B); }\n";
+    o << "extern \"C\" __global__ __launch_bounds__(" << tune.block
+      << ") void rf_qb_done(RfBatchArgs B) { rf_batch_done<Q>(B); }\n";
+    o << "extern \"C\" __global__ __launch_bounds__(256) void rf_qb_gather(RfBatchArgs B) { "
+         "rf_batch_gather<Q>(B); }\n";
     ks.code = o.str();
     return ks;
 }

@@ -76,3 +76,20 @@ struct RfFoldArgs {
     void* out[RF_MAXOUT];         // output columns, one row per group
     union RfParam p[RF_MAXPARAMS];
 };
+
+// Batched execution: a persistent launch over (work unit, wave-tile) items.
+struct RfBatchHdr {
+    int64_t nunits;      // work units of this launch
+    int64_t nitems;      // their total wave-tiles
+    int64_t unit_base;   // first unit in the round's work array
+    int64_t start_base;  // first entry in the round's item_start array
+};
+
+// Pointers are fixed per runner (a captured launch replays unchanged); the
+// header, written per round, says where this launch's units are.
+struct RfBatchArgs {
+    const struct RfBatchHdr* hdr;
+    const struct RfScanArgs* work;     // the round's units
+    const int64_t* item_start;         // per launch: nunits + 1 cumulative tile counts
+    unsigned long long* queue;         // item counter, zero before the launch
+};

@@ -206,6 +206,67 @@ __device__ __forceinline__ uint32_t rf_lanes_below(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// Evaluates one wave-tile starting at row0: folds its rows into `a`, writes
+// the selection bits of each sub-step and the per-row values the selection's
+// emit reuses. Interior tiles use the registers loaded into `T`; edge tiles
+// load with bounds checks.
+template <class Q>
+__device__ __forceinline__ void rf_tile_rows(const RfScanArgs& L, const typename Q::Tile& T,
+                                             int64_t row0, bool interior, uint32_t lane,
+                                             typename Q::Acc& a, uint32_t* bits,
+                                             typename Q::Carry& cy) {
+    constexpr int ITEMS = Q::ITEMS, SUB = Q::SUB;
+    if (interior) {
+        Q::compute(a, T, L, row0, lane, bits, cy);
+    } else {
+#pragma unroll
+        for (int s = 0; s < SUB; s++) {
+            const int64_t r = row0 + ((int64_t)s * 64 + lane) * ITEMS;
+            if (r >= L.lo && r + ITEMS <= L.end)
+                bits[s] = Q::template rows<true>(a, L, r, cy, s);
+            else if (r + ITEMS > L.lo && r < L.end)
+                bits[s] = Q::template rows<false>(a, L, r, cy, s);
+            else
+                bits[s] = 0;
+        }
+    }
+}
+
+// Appends a tile's qualifying rows, in row order, to the staging columns at
+// positions base + cursor, ...; advances the wave-uniform `cursor`. Rows of
+// a sub-step in ascending order are (lane, item): the position of (lane, k)
+// is the number of qualifying rows in lower lanes (one ballot per item) plus
+// those of items < k in the lane.
+template <class Q>
+__device__ __forceinline__ void rf_tile_emit(const RfScanArgs& L, int64_t row0, uint32_t lane,
+                                             const uint32_t* bits, const typename Q::Carry& cy,
+                                             int64_t base, int64_t& cursor) {
+    constexpr int ITEMS = Q::ITEMS, SUB = Q::SUB;
+#pragma unroll
+    for (int s = 0; s < SUB; s++) {
+        const uint32_t b = bits[s];
+        uint32_t below = 0, total = 0;
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const uint64_t m = __ballot((b >> k) & 1u);
+            below += rf_lanes_below(m);
+            total += (uint32_t)__builtin_popcountll(m);
+        }
+        if (b) {
+            const int64_t r = row0 + ((int64_t)s * 64 + lane) * ITEMS;
+            int64_t pos = base + cursor + below;
+#pragma unroll
+            for (int k = 0; k < ITEMS; k++) {
+                if ((b >> k) & 1u) {
+                    Q::emit(L, r + k, pos, cy, s, k);
+                    pos++;
+                }
+            }
+        }
+        cursor += total;
+    }
+}
+
 // Scan kernel. Wave w (of W = gridDim.x * BLOCK/64) owns wave-tiles
 // [w*per, min((w+1)*per, ntiles)) with per = ceil(ntiles / W).
 //   * reductions fold into registers (per wave-tile partials in the
@@ -248,53 +309,12 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
         if constexpr (Q::DET) Q::acc_zero(a);
         uint32_t bits[SUB];
         typename Q::Carry cy;  // per-row values the selection's emit reuses
-        if (interior) {
-            Q::compute(a, T, L, row0, lane, bits, cy);
-        } else {
-#pragma unroll
-            for (int s = 0; s < SUB; s++) {
-                const int64_t r = row0 + ((int64_t)s * 64 + lane) * ITEMS;
-                if (r >= L.lo && r + ITEMS <= L.end)
-                    bits[s] = Q::template rows<true>(a, L, r, cy, s);
-                else if (r + ITEMS > L.lo && r < L.end)
-                    bits[s] = Q::template rows<false>(a, L, r, cy, s);
-                else
-                    bits[s] = 0;
-            }
-        }
+        rf_tile_rows<Q>(L, T, row0, interior, lane, a, bits, cy);
         if constexpr (Q::NACC > 0 && Q::DET) {
             rf_wave_reduce<Q>(a);
             if (lane == 0) reinterpret_cast<Acc*>(L.partials)[tile] = a;
         }
-        if constexpr (Q::HAS_SEL) {
-            // Rows of a sub-step in ascending order are (lane, item): the
-            // position of (lane, k) is the number of qualifying rows in lower
-            // lanes (one ballot per item) plus those of items < k in the lane.
-            const int64_t wave_base = gw * L.stage_rows;
-#pragma unroll
-            for (int s = 0; s < SUB; s++) {
-                const uint32_t b = bits[s];
-                uint32_t below = 0, total = 0;
-#pragma unroll
-                for (int k = 0; k < ITEMS; k++) {
-                    const uint64_t m = __ballot((b >> k) & 1u);
-                    below += rf_lanes_below(m);
-                    total += (uint32_t)__builtin_popcountll(m);
-                }
-                if (b) {
-                    const int64_t r = row0 + ((int64_t)s * 64 + lane) * ITEMS;
-                    int64_t pos = wave_base + cursor + below;
-#pragma unroll
-                    for (int k = 0; k < ITEMS; k++) {
-                        if ((b >> k) & 1u) {
-                            Q::emit(L, r + k, pos, cy, s, k);
-                            pos++;
-                        }
-                    }
-                }
-                cursor += total;
-            }
-        }
+        if constexpr (Q::HAS_SEL) rf_tile_emit<Q>(L, row0, lane, bits, cy, gw * L.stage_rows, cursor);
     };
     auto interior = [&](int64_t tile) {
         const int64_t row0 = L.start + tile * TILE;
@@ -384,4 +404,112 @@ __device__ __forceinline__ void rf_gather(const RfScanArgs& L) {
         const int64_t src = w * L.stage_rows, dst = L.wave_dst[w], cnt = L.wave_cnt[w];
         for (int64_t i = threadIdx.x; i < cnt; i += blockDim.x) Q::move(L, src + i, dst + i);
     }
+}
+
+// ------------------------------------------------------ batched execution
+//
+// Many work units (one query over one record batch each, an RfScanArgs per
+// unit in device memory) run in one persistent launch: waves take
+// (unit, wave-tile) items from one atomic counter. Every item is processed
+// as the deterministic scan processes a tile: its fold from the identity,
+// wave-combined into partials[tile]; its qualifying rows staged at
+// tile * TILE with the count in wave_cnt[tile] (so stage_rows = TILE and
+// nwaves = ntiles for a unit). rf_batch_done then combines each unit's
+// partials in the deterministic scan's fixed order and turns the tile
+// counts into offsets; rf_gather places the rows. A unit therefore gets
+// the same bits as the deterministic scan of the same query alone.
+
+// Wave-uniform item counter: the wave's lane 0 takes the next ticket.
+__device__ __forceinline__ int64_t rf_next_item(unsigned long long* queue) {
+    unsigned long long t = 0;
+    if (__lane_id() == 0) t = atomicAdd(queue, 1ull);
+    return rf_scalar64((int64_t)__shfl(t, 0));
+}
+
+template <class Q>
+__device__ __forceinline__ void rf_batch_scan(const RfBatchArgs& B) {
+    typedef typename Q::Acc Acc;
+    constexpr int ITEMS = Q::ITEMS, SUB = Q::SUB;
+    constexpr int64_t TILE = (int64_t)SUB * 64 * ITEMS;
+    const uint32_t lane = __lane_id();
+    const int64_t nunits = B.hdr->nunits, nitems = B.hdr->nitems;
+    const RfScanArgs* work = B.work + B.hdr->unit_base;
+    const int64_t* item_start = B.item_start + B.hdr->start_base;
+    int64_t item = rf_next_item(B.queue);
+    while (item < nitems) {
+        // Take the next ticket before working on this one, so its latency
+        // overlaps the tile.
+        const int64_t next = rf_next_item(B.queue);
+        // Unit of the item: the last u with item_start[u] <= item.
+        int64_t lo = 0, hi = nunits - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (item_start[mid] <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        lo = rf_scalar64(lo);
+        const RfScanArgs& L = work[lo];
+        const int64_t tile = item - item_start[lo];
+        const int64_t row0 = L.start + tile * TILE;
+        const bool interior = row0 >= L.lo && row0 + TILE <= L.end;
+        typename Q::Tile T;
+        if (interior) Q::load(T, L, row0, lane);
+        Acc a;
+        Q::acc_zero(a);
+        uint32_t bits[SUB];
+        typename Q::Carry cy;
+        rf_tile_rows<Q>(L, T, row0, interior, lane, a, bits, cy);
+        if constexpr (Q::NACC > 0) {
+            rf_wave_reduce<Q>(a);
+            if (lane == 0) reinterpret_cast<Acc*>(L.partials)[tile] = a;
+        }
+        if constexpr (Q::HAS_SEL) {
+            int64_t cursor = 0;
+            rf_tile_emit<Q>(L, row0, lane, bits, cy, tile * TILE, cursor);
+            if (lane == 0) L.wave_cnt[tile] = cursor;
+        }
+        item = next;
+    }
+}
+
+// One block per unit: the deterministic scan's completion for that unit.
+template <class Q>
+__device__ __forceinline__ void rf_batch_done(const RfBatchArgs& B) {
+    typedef typename Q::Acc Acc;
+    constexpr int BLOCK = Q::BLOCK;
+    __shared__ Acc s_waves[BLOCK / 64];
+    __shared__ uint64_t s_scan[BLOCK / 64];
+    const int64_t u = blockIdx.x;
+    if (u >= B.hdr->nunits) return;
+    const RfScanArgs& L = B.work[B.hdr->unit_base + u];
+    if constexpr (Q::NACC > 0) {
+        const Acc* parts = reinterpret_cast<const Acc*>(L.partials);
+        Acc a;
+        Q::acc_zero(a);
+        for (int64_t i = threadIdx.x; i < L.ntiles; i += BLOCK) Q::acc_combine(a, parts[i]);
+        rf_block_reduce<Q>(a, s_waves);
+        if (threadIdx.x == 0) *reinterpret_cast<Acc*>(L.result) = a;
+    }
+    if constexpr (Q::HAS_SEL) {
+        const int64_t n = L.nwaves;
+        const int64_t pt = (n + BLOCK - 1) / BLOCK;
+        const int64_t lo = (int64_t)threadIdx.x * pt;
+        uint64_t mine = 0;
+        for (int64_t i = lo; i < lo + pt && i < n; i++) mine += (uint64_t)L.wave_cnt[i];
+        uint64_t total;
+        uint64_t off = rf_block_excl_scan<BLOCK>(mine, s_scan, total);
+        for (int64_t i = lo; i < lo + pt && i < n; i++) {
+            L.wave_dst[i] = (int64_t)off;
+            off += (uint64_t)L.wave_cnt[i];
+        }
+        if (threadIdx.x == 0) *L.out_count = (int64_t)total;
+    }
+}
+
+// Blocks (x, unit): the selection placement of every unit.
+template <class Q>
+__device__ __forceinline__ void rf_batch_gather(const RfBatchArgs& B) {
+    const int64_t u = blockIdx.y;
+    if (u >= B.hdr->nunits) return;
+    rf_gather<Q>(B.work[B.hdr->unit_base + u]);
 }

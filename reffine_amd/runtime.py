@@ -121,6 +121,23 @@ _SIGS = [
     ("rf_query_free", ctypes.c_int, [ctypes.c_void_p]),
     ("rf_calib_copy", ctypes.c_int, [ctypes.c_int64, ctypes.c_int32,
                                      ctypes.POINTER(ctypes.c_double)]),
+    ("rf_batch_create", ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
+                                       ctypes.POINTER(ctypes.c_void_p)]),
+    ("rf_batch_footprint", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RfArg), ctypes.c_int64,
+                                          ctypes.POINTER(ctypes.c_int64)]),
+    ("rf_batch_add", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(RfArg),
+                                    ctypes.c_int64, ctypes.POINTER(ctypes.c_int32)]),
+    ("rf_batch_launch", ctypes.c_int, [ctypes.c_void_p]),
+    ("rf_batch_poll", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    ("rf_batch_wait", ctypes.c_int, [ctypes.c_void_p]),
+    ("rf_batch_result", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                       ctypes.POINTER(ctypes.c_void_p)]),
+    ("rf_batch_reset", ctypes.c_int, [ctypes.c_void_p]),
+    ("rf_batch_free", ctypes.c_int, [ctypes.c_void_p]),
+    ("rf_serve", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64,
+                                ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
+    ("rf_serve_plain", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64,
+                                      ctypes.POINTER(ctypes.c_double)]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]
@@ -300,6 +317,25 @@ class DeviceTable:
         p = ctypes.c_void_p()
         _ck(lib().rf_table_column(self.handle, c, ctypes.byref(p), None, None, None, 0))
         return p.value or 0
+
+    def view(self, dim: int) -> "DeviceTable":
+        """The same device columns as a table of another iterator dimension
+        (e.g. dim 2 for a `_subvec` fold and dim 1 for an `_op` over the
+        rows). Borrows the buffers: this table must outlive the view."""
+        n = self.ncols
+        fmts = (ctypes.c_char_p * n)(*[f.encode() for f in self.formats])
+        vals = (ctypes.c_void_p * n)()
+        valid = (ctypes.c_void_p * n)()
+        for c in range(n):
+            v, b = ctypes.c_void_p(), ctypes.c_void_p()
+            _ck(lib().rf_table_column(self.handle, c, ctypes.byref(v), ctypes.byref(b), None, None, 0))
+            vals[c], valid[c] = v.value, b.value
+        out = ctypes.c_void_p()
+        _ck(lib().rf_table_wrap_device(dim, n, self.nrows, fmts, vals, valid, ctypes.byref(out)))
+        t = DeviceTable(out.value)
+        t.dim = dim
+        t._base = self
+        return t
 
     def build_index(self) -> "DeviceTable":
         _ck(lib().rf_table_build_index(self.handle))

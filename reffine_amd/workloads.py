@@ -90,21 +90,23 @@ C3_GROUPS = [(a, b) for a in range(3) for b in range(2)]
 C3_FIELDS = ["sum_qty", "sum_price", "sum_disc_price", "sum_charge", "sum_disc", "count"]
 
 
-def group_agg_c3(dmax: int = 2526):
+def group_agg_c3(dmax: int | None = 2526):
     """Low-cardinality grouped aggregate in the TPC-H Q1 idiom
     (benchmark/include/tpch.h:412-478): one unrolled
     `_sel(f1 == a & f2 == b & d <= dmax, s + x, s)` per (group, field).
     State: for each of the 6 groups (f1, f2) in order (0,0), (0,1), ...,
     (2,1): SUM(qty) i64, SUM(price), SUM(price*(1-disc)),
     SUM(price*(1-disc)*(1+tax)), SUM(disc) f64, COUNT i64. AVG(qty),
-    AVG(price), AVG(disc) are the sums over COUNT."""
+    AVG(price), AVG(disc) are the sums over COUNT. With dmax=None the bound
+    is a scalar i32 input of the function (a prepared query)."""
     T = _sym("T", C3_TYPE)
+    dsym = _sym("dmax", _i32_t) if dmax is None else None
 
     def acc(s, v):
         f1, f2, qty, price, disc, tax, d = (_get(v, i) for i in range(7))
         disc_price = _mul(price, _sub(_f64(1), disc))
         charge = _mul(disc_price, _add(_f64(1), tax))
-        keep = _lte(d, _i32(dmax))
+        keep = _lte(d, dsym if dsym is not None else _i32(dmax))
         vals = [qty, price, disc_price, charge, disc, _i64(1)]
         out = []
         for g, (a, b) in enumerate(C3_GROUPS):
@@ -120,7 +122,7 @@ def group_agg_c3(dmax: int = 2526):
 
     red = _red(_subvec(T, _idx(0), _len(T, 1)), init, acc)
     rs = _sym("q1", red)
-    fn = _func("group_c3", rs, [T])
+    fn = _func("group_c3", rs, [T] if dsym is None else [T, dsym])
     fn.tbl[rs] = red
     return fn
 
@@ -172,3 +174,74 @@ def c4_bytes(n_build: int, n_probe: int, groups: int) -> int:
     """Algorithmic bytes: build bk + bf (9 B/row), probe pk + pv + pq
     (24 B/row), output bk + SUM + COUNT (24 B/group)."""
     return 9 * n_build + 24 * n_probe + 24 * groups
+
+
+# The grouped layout iterated by row id (an Op over `_in(t, T)` needs a
+# one-dimensional vector; the same device columns serve both views).
+C3_DIM1_TYPE = _vec_t(1, _i64_t, _i8_t, _i8_t, _i64_t, _f64_t, _f64_t, _f64_t, _i32_t)
+
+
+def q6_template():
+    """Scan-filter-aggregate + selection over the grouped layout (the
+    TPC-H Q6 shape of the 60M-row scan): WHERE d0 <= d < d0+365 AND
+    dlo <= disc <= dhi AND qty < qmax -> COUNT, SUM(qty) i64,
+    SUM(price*disc) f64 and the selected row ids. The predicate constants
+    are scalar inputs (d0 i32, dlo f64, dhi f64, qmax i64), so one compiled
+    query serves every constant choice."""
+    T = _sym("T", C3_DIM1_TYPE)
+    d0 = _sym("d0", _i32_t)
+    dlo = _sym("dlo", _f64_t)
+    dhi = _sym("dhi", _f64_t)
+    qmax = _sym("qmax", _i64_t)
+
+    def pred(t):
+        e = T[t]
+        qty, disc, d = e[2], e[4], e[6]
+        return (_in(t, T) & _gte(d, d0) & _lt(d, _add(d0, _i32(365))) & _gte(disc, dlo) &
+                _lte(disc, dhi) & _lt(qty, qmax)), e
+
+    t = _sym("t", _i64_t)
+    p, e = pred(t)
+    rows = _op([t], p, [e[2], e[3], e[4]])  # iterator, qty, price, disc
+
+    def acc(s, r):
+        return _new([_add(_get(s, 0), _i64(1)), _add(_get(s, 1), r[1]),
+                     _add(_get(s, 2), _mul(r[2], r[3]))])
+
+    red = _red(rows, lambda: _new([_i64(0), _i64(0), _f64(0)]), acc)
+    rs = _sym("agg", red)
+    t2 = _sym("t", _i64_t)
+    p2, _ = pred(t2)
+    sel = _op([t2], p2, [])
+    ss = _sym("sel", sel)
+    out = _new([rs, ss])
+    os_ = _sym("out", out)
+    fn = _func("q6_batch", os_, [T, d0, dlo, dhi, qmax])
+    fn.tbl[rs] = red
+    fn.tbl[ss] = sel
+    fn.tbl[os_] = out
+    return fn
+
+
+def c5_mix(n_queries: int, n_batches: int, seed: int = 17, templates=("q6", "q1")):
+    """The request mix: per query a template (uniform), a target record
+    batch (uniform) and seeded predicate constants with at most 6 decimals.
+    Returns a list of (template, batch, params)."""
+    from . import datagen
+    import numpy as np
+    tpl = datagen.randint(seed, 1, n_queries, 0, len(templates), np.int64)
+    bat = datagen.randint(seed, 2, n_queries, 0, n_batches, np.int64)
+    d0 = datagen.randint(seed, 3, n_queries, 0, 2192, np.int64)
+    lo = datagen.randint(seed, 4, n_queries, 0, 9, np.int64)
+    qm = datagen.randint(seed, 5, n_queries, 10, 41, np.int64)
+    dm = datagen.randint(seed, 6, n_queries, 2000, 2556, np.int64)
+    out = []
+    for i in range(n_queries):
+        name = templates[int(tpl[i])]
+        if name == "q6":
+            params = (int(d0[i]), round(int(lo[i]) * 0.01, 2), round(int(lo[i]) * 0.01 + 0.02, 2),
+                      int(qm[i]))
+        else:
+            params = (int(dm[i]),)
+        out.append((name, int(bat[i]), params))
+    return out
