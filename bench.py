@@ -62,10 +62,12 @@ def main() -> int:
     ap.add_argument("--cpu-rows", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--workload", choices=["c2", "c3", "c4"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
                     help="c2 (default): the 60M-row scan-filter-aggregate + selection; c3: the "
                          "60M-row grouped aggregate; c4: the 1.5M x 60M hash join + grouped "
-                         "aggregate")
+                         "aggregate; c5: 4096 queries over 64 record batches, batched/served "
+                         "against plain per-query launches")
+    ap.add_argument("--queries", type=int, default=4096, help="c5: number of queries")
     ap.add_argument("--counter-pass", action="store_true",
                     help="untimed: run the query 3 times and a 1 GiB calibration copy 3 times "
                          "(the program profiled by the FETCH_SIZE / WRITE_SIZE passes)")
@@ -83,6 +85,8 @@ def main() -> int:
         return bench_c3(args)
     if args.workload == "c4":
         return bench_c4(args)
+    if args.workload == "c5":
+        return bench_c5(args)
     n = args.rows
     log(f"generating {n} rows")
     c = datagen.scan_table(n, args.seed, with_q=True, rowid_dtype=np.int32)
@@ -254,6 +258,84 @@ def bench_c4(args) -> int:
         from oracle import cpu_baseline as cb
         out_line["cpu_baseline"] = cb.time_c4(nb, min(args.cpu_rows, 10_000_000))
     print(json.dumps(out_line), flush=True)
+    return 0 if ok else 1
+
+
+def c5_setup(n_queries: int, seed: int = 5):
+    """64 record batches (log-normal row counts) in the grouped layout, two
+    prepared query templates and the request mix."""
+    from reffine_amd import DeviceTable, compile_op, datagen, workloads
+    sizes = datagen.lognormal_sizes(64, seed)
+    tables = []
+    for b, n in enumerate(sizes):
+        c = datagen.group_table(int(n), 1000 + b)
+        t1 = DeviceTable.from_columns([c["rowid"], c["f1"], c["f2"], c["qty"], c["price"],
+                                       c["disc"], c["tax"], c["d"]], dim=1)
+        tables.append((t1, t1.view(2)))
+    queries = {"q6": compile_op(workloads.q6_template(), deterministic=True),
+               "q1": compile_op(workloads.group_agg_c3(None), deterministic=True)}
+    mix = workloads.c5_mix(n_queries, len(tables), seed=17)
+    work = []
+    nbytes = 0
+    for name, b, params in mix:
+        t1, t2 = tables[b]
+        work.append((queries[name], (t1 if name == "q6" else t2,) + params))
+        nbytes += (20 if name == "q6" else 38) * int(sizes[b])
+    return sizes, tables, queries, work, nbytes
+
+
+def bench_c5(args) -> int:
+    """Batched/served form against plain per-query launches: closed-loop
+    throughput of both, then latency at a fixed Poisson arrival rate."""
+    from reffine_amd import batch, datagen
+    log("building 64 record batches and the query mix")
+    sizes, tables, queries, work, nbytes = c5_setup(args.queries)
+    reqs0 = [(q, inp, 0.0) for q, inp in work]
+    log("plain, closed loop")
+    batch.serve_plain(reqs0[:64])  # warm
+    plain = batch.serve_plain(reqs0)
+    log("batched, closed loop")
+    batch.serve(reqs0[:64])  # warm (captures the round graph)
+    served = batch.serve(reqs0)
+    same = (np.array_equal(plain.out_rows, served.out_rows) and all(
+        batch.reduction_bytes(q, a) == batch.reduction_bytes(q, b)
+        for (q, _), a, b in zip(work, plain.outputs, served.outputs)))
+    ok = bool(same and (plain.status == 0).all() and (served.status == 0).all())
+    qps_plain, qps_batched = plain.qps, served.qps
+    rate = 0.5 * qps_batched
+    arrivals = datagen.poisson_arrivals(len(work), rate, seed=29)
+    log(f"served at {rate:.0f} queries/s (Poisson)")
+    reqs = [(q, inp, t) for (q, inp), t in zip(work, arrivals)]
+    at_rate = batch.serve(reqs)
+    lat = at_rate.latency_us
+    plain_rate = min(0.5 * qps_plain, rate)
+    arrivals_p = datagen.poisson_arrivals(len(work), plain_rate, seed=29)
+    log(f"plain at {plain_rate:.0f} queries/s (Poisson)")
+    plain_at = batch.serve_plain([(q, inp, t) for (q, inp), t in zip(work, arrivals_p)])
+    lat_p = plain_at.latency_us
+    out = {
+        "metric": METRIC, "value": round(qps_batched, 1), "unit": "queries/s", "n_gpus": 1,
+        "steps": len(work), "warmup": 64, "ms_per_step": round(served.wall_us / 1e3 / len(work), 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+int64",
+        "data": "synthetic (seeded SplitMix64 record batches, resident in HBM)",
+        "config": {"workload": f"{len(work)} queries, uniform over a Q6-shaped scan-filter-aggregate "
+                               "+ selection and a Q1-shaped 6-group aggregate, over 64 record "
+                               "batches of log-normal row counts", "rows_total": int(sizes.sum()),
+                   "rows_median": int(np.median(sizes)), "rows_min": int(sizes.min()),
+                   "rows_max": int(sizes.max()), "identical_results": ok},
+        "batched": {"qps": round(qps_batched, 1), "wall_s": round(served.wall_us * 1e-6, 4),
+                    "gbs_scanned": round(nbytes / (served.wall_us * 1e-6) / 1e9, 1)},
+        "plain": {"qps": round(qps_plain, 1), "wall_s": round(plain.wall_us * 1e-6, 4),
+                  "gbs_scanned": round(nbytes / (plain.wall_us * 1e-6) / 1e9, 1)},
+        "speedup_vs_plain": round(qps_batched / qps_plain, 2),
+        "served_at_rate": {"offered_qps": round(rate, 1), "achieved_qps": round(at_rate.qps, 1),
+                           "p50_ms": round(float(np.percentile(lat, 50)) / 1e3, 4),
+                           "p99_ms": round(float(np.percentile(lat, 99)) / 1e3, 4)},
+        "plain_at_rate": {"offered_qps": round(plain_rate, 1), "achieved_qps": round(plain_at.qps, 1),
+                          "p50_ms": round(float(np.percentile(lat_p, 50)) / 1e3, 4),
+                          "p99_ms": round(float(np.percentile(lat_p, 99)) / 1e3, 4)},
+    }
+    print(json.dumps(out), flush=True)
     return 0 if ok else 1
 
 

@@ -165,7 +165,7 @@ static void compile_impl(const char* ir_text, uint32_t flags, const int32_t* tun
     auto q = std::make_unique<rf_query>();
     q->plan = rf::lower_func(f, false);
     q->det = (flags & RF_COMPILE_DETERMINISTIC) != 0;
-    q->tune = rf::default_tuning(q->plan);
+    q->tune = rf::default_tuning(q->plan, q->det);
     if (tuning) {
         rf::Tuning t{tuning[0], tuning[1], tuning[2], tuning[3] > 0 ? tuning[3] : 16,
                      tuning[4] != 0 ? 1 : 0};
@@ -214,7 +214,7 @@ int rf_compile_check(const char* ir_text, uint32_t flags, char* buf, size_t cap)
         need(ir_text != nullptr, "NULL argument");
         rf::N f = rf::parse_func(ir_text);
         rf::Plan plan = rf::lower_func(f, false);
-        rf::Tuning tune = rf::default_tuning(plan);
+        rf::Tuning tune = rf::default_tuning(plan, (flags & RF_COMPILE_DETERMINISTIC) != 0);
         std::vector<std::vector<bool>> nl(plan.inputs.size());
         for (size_t i = 0; i < plan.inputs.size(); i++)
             nl[i].assign(plan.inputs[i]->type.dtypes.size(), (flags & 2u) != 0);

@@ -304,6 +304,53 @@ static void collect_driver_cols(const N& e, const Plan& plan, std::map<int, int>
     for (auto& a : e->args) collect_driver_cols(a, plan, cols);
 }
 
+static void driver_row_shape(const Plan& plan, int& wmax, int& row_bytes, int& nacc) {
+    std::map<int, int> cols;  // driver column -> width, loaded per row
+    for (auto& r : plan.reduces)
+        for (auto& f : r.fields)
+            if (f.updated) {
+                collect_driver_cols(f.pred, plan, cols);
+                collect_driver_cols(f.contrib, plan, cols);
+            }
+    for (auto& o : plan.ops) collect_driver_cols(o.pred, plan, cols);
+    for (auto& p : plan.probes) {
+        collect_driver_cols(p.key, plan, cols);
+        collect_driver_cols(p.guard, plan, cols);
+    }
+    wmax = row_bytes = nacc = 0;
+    for (auto& kv : cols) {
+        wmax = std::max(wmax, kv.second);
+        row_bytes += kv.second;
+    }
+    for (auto& r : plan.reduces)
+        for (auto& f : r.fields) nacc += f.updated ? 1 : 0;
+}
+
+// Sub-steps one set of loads may hold: about 640 (pipelined) or 320 bytes
+// per lane, at most 32 rows per lane.
+int load_cap(const Plan& plan, int items, bool pipelined) {
+    int wmax, row_bytes, nacc;
+    driver_row_shape(plan, wmax, row_bytes, nacc);
+    const int budget = pipelined ? 640 : 320;
+    int sub = 4;
+    if (row_bytes > 0) sub = std::max(1, std::min(16, budget / (items * row_bytes)));
+    sub = std::min(sub, std::max(1, 32 / items));
+    if (!plan.probes.empty()) sub = std::max(1, 8 / items);
+    return sub;
+}
+
+Tuning default_tuning(const Plan& plan, bool det) {
+    Tuning t = default_tuning(plan);
+    if (det) {
+        // The deterministic mode combines each wave-tile across lanes; tiles
+        // of several load groups (at least 4096 rows) amortise that combine.
+        const int rows = t.sub * 64 * t.items;
+        const int groups = std::max(1, (4096 + rows - 1) / rows);
+        t.sub *= groups;
+    }
+    return t;
+}
+
 Tuning default_tuning(const Plan& plan) {
     std::map<int, int> cols;  // driver column -> width, loaded per row
     for (auto& r : plan.reduces)
@@ -364,6 +411,13 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
     if (tune.block % 64 != 0 || tune.block > 1024) throw IRError("block size");
     if (16 % tune.items != 0) throw IRError("ITEMS must divide 16");
     ks.has_sel = !plan.ops.empty();
+    // Load group: the largest divisor of SUB within the register budget of
+    // one set of loads (a wave-tile of SUB sub-steps is evaluated in SUB/LSUB
+    // groups; the rows each lane folds, and their order, do not change).
+    const int lcap = load_cap(plan, tune.items, tune.pipeline != 0);
+    int lsub = 1;
+    for (int d = 1; d <= tune.sub; d++)
+        if (tune.sub % d == 0 && d <= lcap) lsub = d;
 
     // Accumulator layout: one field per updated state field.
     for (size_t c = 0; c < plan.reduces.size(); c++)
@@ -400,7 +454,7 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
     o << "// Generated row code for reffine function '" << plan.func_name << "'.\n";
     o << "struct Q {\n";
     o << "    enum { BLOCK = " << tune.block << ", ITEMS = " << tune.items
-      << ", SUB = " << tune.sub << ", NACC = " << ks.acc.size()
+      << ", SUB = " << tune.sub << ", LSUB = " << lsub << ", NACC = " << ks.acc.size()
       << ", HAS_SEL = " << (ks.has_sel ? 1 : 0) << ", DET = " << (det ? 1 : 0)
       << ", PIPE = " << (tune.pipeline ? 1 : 0) << " };\n";
     o << "    struct Acc {";
@@ -408,7 +462,7 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
     if (ks.acc.empty()) o << " int unused;";
     o << " };\n";
     o << "    struct Carry {";
-    for (int j : g.carried) o << " int64_t p" << j << "[SUB][ITEMS];";
+    for (int j : g.carried) o << " int64_t p" << j << "[LSUB][ITEMS];";
     if (g.carried.empty()) o << " int unused;";
     o << " };\n";
     o << "    static __device__ __forceinline__ void acc_zero(Acc& a) {";
@@ -467,15 +521,15 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
         const ColSlot& c = ks.cols[s];
         if (!c.vector) continue;
         any_vec = true;
-        o << " RfVec<" << cty(c.type) << ", ITEMS> c" << s << "[SUB];";
-        if (c.nullable) o << " uint32_t vb" << s << "[SUB];";
+        o << " RfVec<" << cty(c.type) << ", ITEMS> c" << s << "[LSUB];";
+        if (c.nullable) o << " uint32_t vb" << s << "[LSUB];";
     }
     if (!any_vec) o << " int unused;";
     o << " };\n";
     o << "    static __device__ __forceinline__ void load(Tile& T, const RfScanArgs& L, "
          "int64_t row0, uint32_t lane) {\n"
       << "        const int64_t rb = row0 + (int64_t)lane * ITEMS;\n        (void)rb; (void)T;\n"
-      << "#pragma unroll\n        for (int s = 0; s < SUB; s++) {\n"
+      << "#pragma unroll\n        for (int s = 0; s < LSUB; s++) {\n"
       << "            const int64_t r = rb + (int64_t)s * 64 * ITEMS;\n            (void)r;\n";
     for (size_t s = 0; s < ks.cols.size(); s++) {
         const ColSlot& c = ks.cols[s];
@@ -497,7 +551,7 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
     o << "    static __device__ __forceinline__ void compute(Acc& a, const Tile& T, "
          "const RfScanArgs& L, int64_t row0, uint32_t lane, uint32_t* bits, Carry& cy) {\n"
       << "        const int64_t rb = row0 + (int64_t)lane * ITEMS;\n        (void)T;\n"
-      << "#pragma unroll\n        for (int s = 0; s < SUB; s++) {\n"
+      << "#pragma unroll\n        for (int s = 0; s < LSUB; s++) {\n"
       << "            const int64_t r = rb + (int64_t)s * 64 * ITEMS;\n"
       << "            bits[s] = row_block<true>(a, L, r, cy, s" << targs_s << ");\n        }\n    }\n";
     // row_block(): the query's row code over ITEMS loaded rows.

@@ -70,6 +70,10 @@ KernelSource gen_group_fold(const GroupJoinPlan& gj,
 
 // Picks ITEMS from the narrowest driver column the row code loads.
 Tuning default_tuning(const Plan& plan);
+// The same, for the deterministic mode when `det` (larger wave-tiles).
+Tuning default_tuning(const Plan& plan, bool det);
+// Sub-steps one set of loads may hold in registers.
+int load_cap(const Plan& plan, int items, bool pipelined);
 
 // Identity of a fold as a C literal of `t`.
 std::string fold_identity(AggOp op, const Type& t);

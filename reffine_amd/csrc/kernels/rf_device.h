@@ -206,21 +206,28 @@ __device__ __forceinline__ uint32_t rf_lanes_below(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Evaluates one wave-tile starting at row0: folds its rows into `a`, writes
-// the selection bits of each sub-step and the per-row values the selection's
-// emit reuses. Interior tiles use the registers loaded into `T`; edge tiles
-// load with bounds checks.
+// A wave-tile (the fold unit of the deterministic order: SUB sub-steps of
+// 64 lanes x ITEMS rows) is evaluated in load groups of LSUB sub-steps (the
+// registers one load holds); in sub-step s of a group starting at row0,
+// lane l owns rows row0 + (s*64 + l)*ITEMS + [0, ITEMS). A tile of NG
+// groups therefore gives each lane the same rows, in the same ascending
+// order, as one group of SUB sub-steps would.
+
+// Evaluates one load group starting at row0: folds its rows into `a`,
+// writes the selection bits of each sub-step and the per-row values the
+// selection's emit reuses. Interior groups use the registers loaded into
+// `T`; edge groups load with bounds checks.
 template <class Q>
 __device__ __forceinline__ void rf_tile_rows(const RfScanArgs& L, const typename Q::Tile& T,
                                              int64_t row0, bool interior, uint32_t lane,
                                              typename Q::Acc& a, uint32_t* bits,
                                              typename Q::Carry& cy) {
-    constexpr int ITEMS = Q::ITEMS, SUB = Q::SUB;
+    constexpr int ITEMS = Q::ITEMS, LSUB = Q::LSUB;
     if (interior) {
         Q::compute(a, T, L, row0, lane, bits, cy);
     } else {
 #pragma unroll
-        for (int s = 0; s < SUB; s++) {
+        for (int s = 0; s < LSUB; s++) {
             const int64_t r = row0 + ((int64_t)s * 64 + lane) * ITEMS;
             if (r >= L.lo && r + ITEMS <= L.end)
                 bits[s] = Q::template rows<true>(a, L, r, cy, s);
@@ -232,18 +239,18 @@ __device__ __forceinline__ void rf_tile_rows(const RfScanArgs& L, const typename
     }
 }
 
-// Appends a tile's qualifying rows, in row order, to the staging columns at
-// positions base + cursor, ...; advances the wave-uniform `cursor`. Rows of
-// a sub-step in ascending order are (lane, item): the position of (lane, k)
-// is the number of qualifying rows in lower lanes (one ballot per item) plus
-// those of items < k in the lane.
+// Appends a load group's qualifying rows, in row order, to the staging
+// columns at positions base + cursor, ...; advances the wave-uniform
+// `cursor`. Rows of a sub-step in ascending order are (lane, item): the
+// position of (lane, k) is the number of qualifying rows in lower lanes (one
+// ballot per item) plus those of items < k in the lane.
 template <class Q>
 __device__ __forceinline__ void rf_tile_emit(const RfScanArgs& L, int64_t row0, uint32_t lane,
                                              const uint32_t* bits, const typename Q::Carry& cy,
                                              int64_t base, int64_t& cursor) {
-    constexpr int ITEMS = Q::ITEMS, SUB = Q::SUB;
+    constexpr int ITEMS = Q::ITEMS, LSUB = Q::LSUB;
 #pragma unroll
-    for (int s = 0; s < SUB; s++) {
+    for (int s = 0; s < LSUB; s++) {
         const uint32_t b = bits[s];
         uint32_t below = 0, total = 0;
 #pragma unroll
@@ -268,7 +275,8 @@ __device__ __forceinline__ void rf_tile_emit(const RfScanArgs& L, int64_t row0, 
 }
 
 // Scan kernel. Wave w (of W = gridDim.x * BLOCK/64) owns wave-tiles
-// [w*per, min((w+1)*per, ntiles)) with per = ceil(ntiles / W).
+// [w*per, min((w+1)*per, ntiles)) with per = ceil(ntiles / W), processed as
+// a sequence of load groups.
 //   * reductions fold into registers (per wave-tile partials in the
 //     deterministic mode);
 //   * a selection is compacted per wave with ballot/popcount: the wave
@@ -280,9 +288,10 @@ __device__ __forceinline__ void rf_tile_emit(const RfScanArgs& L, int64_t row0, 
 template <class Q>
 __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
     typedef typename Q::Acc Acc;
-    constexpr int BLOCK = Q::BLOCK, ITEMS = Q::ITEMS, SUB = Q::SUB;
+    constexpr int BLOCK = Q::BLOCK, ITEMS = Q::ITEMS, LSUB = Q::LSUB;
+    constexpr int NG = Q::SUB / Q::LSUB;  // load groups per wave-tile
     constexpr int WAVES = BLOCK / 64;
-    constexpr int64_t TILE = (int64_t)SUB * 64 * ITEMS;
+    constexpr int64_t GROUP = (int64_t)LSUB * 64 * ITEMS;
     __shared__ Acc s_waves[WAVES];
     __shared__ uint64_t s_scan[WAVES];
     __shared__ uint32_t s_last;
@@ -293,57 +302,63 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
     const int64_t per = (L.ntiles + nw - 1) / nw;
     const int64_t t_begin = rf_scalar64(gw * per);
     const int64_t t_end = rf_scalar64(t_begin + per < L.ntiles ? t_begin + per : L.ntiles);
+    const int64_t g_begin = t_begin * NG, g_end = t_end * NG;  // the wave's load groups
 
     Acc run;
     Q::acc_zero(run);
+    Acc a_tile;  // deterministic mode: the current tile's fold
+    Q::acc_zero(a_tile);
     int64_t cursor = 0;  // rows this wave has staged (wave-uniform)
 
-    // One wave-tile: rows from the loaded registers `T` (interior tiles) or
-    // with bounds checks (the edge tiles), then its fold and compaction.
-    auto process = [&](const typename Q::Tile& T, int64_t tile, bool interior) {
-        const int64_t row0 = L.start + tile * TILE;
-        // The deterministic mode folds each tile from the identity; otherwise
-        // rows fold straight into the wave's running state.
-        Acc a_tile;
+    // One load group: rows from the loaded registers `T` (interior groups)
+    // or with bounds checks (edge groups), then its fold and compaction. In
+    // the deterministic mode each tile folds from the identity and its last
+    // group writes the tile's partial; otherwise rows fold straight into
+    // the wave's running state.
+    auto process = [&](const typename Q::Tile& T, int64_t grp, bool interior) {
+        const int64_t row0 = L.start + grp * GROUP;
         Acc& a = Q::DET ? a_tile : run;
-        if constexpr (Q::DET) Q::acc_zero(a);
-        uint32_t bits[SUB];
+        uint32_t bits[LSUB];
         typename Q::Carry cy;  // per-row values the selection's emit reuses
         rf_tile_rows<Q>(L, T, row0, interior, lane, a, bits, cy);
-        if constexpr (Q::NACC > 0 && Q::DET) {
-            rf_wave_reduce<Q>(a);
-            if (lane == 0) reinterpret_cast<Acc*>(L.partials)[tile] = a;
-        }
         if constexpr (Q::HAS_SEL) rf_tile_emit<Q>(L, row0, lane, bits, cy, gw * L.stage_rows, cursor);
+        if constexpr (Q::NACC > 0 && Q::DET) {
+            if (grp % NG == NG - 1) {
+                Acc t = a_tile;
+                rf_wave_reduce<Q>(t);
+                if (lane == 0) reinterpret_cast<Acc*>(L.partials)[grp / NG] = t;
+                Q::acc_zero(a_tile);
+            }
+        }
     };
-    auto interior = [&](int64_t tile) {
-        const int64_t row0 = L.start + tile * TILE;
-        return row0 >= L.lo && row0 + TILE <= L.end;
+    auto interior = [&](int64_t grp) {
+        const int64_t row0 = L.start + grp * GROUP;
+        return row0 >= L.lo && row0 + GROUP <= L.end;
     };
 
     if constexpr (Q::PIPE) {
-        // Software pipeline over the wave's tiles, two register sets: the
-        // next tile's loads are in flight while the current tile is
+        // Software pipeline over the wave's load groups, two register sets:
+        // the next group's loads are in flight while the current group is
         // evaluated.
         typename Q::Tile TA, TB;
-        if (t_begin < t_end && interior(t_begin)) Q::load(TA, L, L.start + t_begin * TILE, lane);
-        for (int64_t tile = t_begin; tile < t_end; tile += 2) {
-            const bool in0 = interior(tile);
-            const bool has1 = tile + 1 < t_end;
-            const bool in1 = has1 && interior(tile + 1);
-            if (in1) Q::load(TB, L, L.start + (tile + 1) * TILE, lane);
-            process(TA, tile, in0);
+        if (g_begin < g_end && interior(g_begin)) Q::load(TA, L, L.start + g_begin * GROUP, lane);
+        for (int64_t grp = g_begin; grp < g_end; grp += 2) {
+            const bool in0 = interior(grp);
+            const bool has1 = grp + 1 < g_end;
+            const bool in1 = has1 && interior(grp + 1);
+            if (in1) Q::load(TB, L, L.start + (grp + 1) * GROUP, lane);
+            process(TA, grp, in0);
             if (!has1) break;
-            if (tile + 2 < t_end && interior(tile + 2))
-                Q::load(TA, L, L.start + (tile + 2) * TILE, lane);
-            process(TB, tile + 1, in1);
+            if (grp + 2 < g_end && interior(grp + 2))
+                Q::load(TA, L, L.start + (grp + 2) * GROUP, lane);
+            process(TB, grp + 1, in1);
         }
     } else {
-        for (int64_t tile = t_begin; tile < t_end; tile++) {
+        for (int64_t grp = g_begin; grp < g_end; grp++) {
             typename Q::Tile T;
-            const bool in = interior(tile);
-            if (in) Q::load(T, L, L.start + tile * TILE, lane);
-            process(T, tile, in);
+            const bool in = interior(grp);
+            if (in) Q::load(T, L, L.start + grp * GROUP, lane);
+            process(T, grp, in);
         }
     }
 
@@ -429,7 +444,9 @@ __device__ __forceinline__ int64_t rf_next_item(unsigned long long* queue) {
 template <class Q>
 __device__ __forceinline__ void rf_batch_scan(const RfBatchArgs& B) {
     typedef typename Q::Acc Acc;
-    constexpr int ITEMS = Q::ITEMS, SUB = Q::SUB;
+    constexpr int ITEMS = Q::ITEMS, SUB = Q::SUB, LSUB = Q::LSUB;
+    constexpr int NG = SUB / LSUB;
+    constexpr int64_t GROUP = (int64_t)LSUB * 64 * ITEMS;
     constexpr int64_t TILE = (int64_t)SUB * 64 * ITEMS;
     const uint32_t lane = __lane_id();
     const int64_t nunits = B.hdr->nunits, nitems = B.hdr->nitems;
@@ -450,22 +467,25 @@ __device__ __forceinline__ void rf_batch_scan(const RfBatchArgs& B) {
         lo = rf_scalar64(lo);
         const RfScanArgs& L = work[lo];
         const int64_t tile = item - item_start[lo];
-        const int64_t row0 = L.start + tile * TILE;
-        const bool interior = row0 >= L.lo && row0 + TILE <= L.end;
-        typename Q::Tile T;
-        if (interior) Q::load(T, L, row0, lane);
         Acc a;
         Q::acc_zero(a);
-        uint32_t bits[SUB];
-        typename Q::Carry cy;
-        rf_tile_rows<Q>(L, T, row0, interior, lane, a, bits, cy);
+        int64_t cursor = 0;
+#pragma unroll 1
+        for (int g = 0; g < NG; g++) {
+            const int64_t row0 = L.start + tile * TILE + g * GROUP;
+            const bool interior = row0 >= L.lo && row0 + GROUP <= L.end;
+            typename Q::Tile T;
+            if (interior) Q::load(T, L, row0, lane);
+            uint32_t bits[LSUB];
+            typename Q::Carry cy;
+            rf_tile_rows<Q>(L, T, row0, interior, lane, a, bits, cy);
+            if constexpr (Q::HAS_SEL) rf_tile_emit<Q>(L, row0, lane, bits, cy, tile * TILE, cursor);
+        }
         if constexpr (Q::NACC > 0) {
             rf_wave_reduce<Q>(a);
             if (lane == 0) reinterpret_cast<Acc*>(L.partials)[tile] = a;
         }
         if constexpr (Q::HAS_SEL) {
-            int64_t cursor = 0;
-            rf_tile_emit<Q>(L, row0, lane, bits, cy, tile * TILE, cursor);
             if (lane == 0) L.wave_cnt[tile] = cursor;
         }
         item = next;
