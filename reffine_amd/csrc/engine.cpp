@@ -1,6 +1,9 @@
 // HIP runtime layer (see engine.hpp).
 #include "engine.hpp"
 
+#include <map>
+#include <unordered_map>
+
 #include <hip/hiprtc.h>
 
 #include <algorithm>
@@ -46,21 +49,81 @@ Device& init_device(int id) {
     return g_dev;
 }
 
+// Device memory. Freed blocks up to 512 MiB are kept (8 GiB in total at
+// most) and handed out again for the same size class: hipFree synchronises
+// the device and hipMalloc costs tens of microseconds, and a repeated query
+// allocates and frees its result table on every run. A block is freed only
+// once nothing on the device uses it (callers free results after their run
+// has finished).
+namespace {
+std::mutex g_mem_mu;
+std::unordered_map<void*, size_t> g_live;  // block -> size class
+std::multimap<size_t, void*> g_cache;      // free blocks by size class
+size_t g_cached = 0;
+constexpr size_t kCacheBlockMax = size_t(512) << 20;
+constexpr size_t kCacheMax = size_t(8) << 30;
+
+size_t size_class(size_t b) {
+    if (b <= 256) return 256;
+    if (b <= (size_t(1) << 20)) {
+        size_t c = 256;
+        while (c < b) c <<= 1;
+        return c;
+    }
+    const size_t mb = size_t(1) << 20;
+    return (b + mb - 1) / mb * mb;
+}
+
+void release_cache_locked() {
+    for (auto& kv : g_cache) (void)hipFree(kv.second);
+    g_cache.clear();
+    g_cached = 0;
+}
+}  // namespace
+
 void* dev_alloc(size_t bytes) {
-    if (bytes == 0) bytes = 256;
-    bytes = (bytes + 255) & ~size_t(255);
+    const size_t sz = size_class(bytes);
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    auto it = g_cache.find(sz);
+    if (it != g_cache.end()) {
+        void* p = it->second;
+        g_cache.erase(it);
+        g_cached -= sz;
+        g_live[p] = sz;
+        return p;
+    }
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
+    hipError_t e = hipMalloc(&p, sz);
+    if (e != hipSuccess && !g_cache.empty()) {
+        (void)hipGetLastError();
+        release_cache_locked();
+        e = hipMalloc(&p, sz);
+    }
     if (e != hipSuccess) {
         (void)hipGetLastError();
         throw NoMem("device allocation of " + std::to_string(bytes) + " bytes failed: " +
                     hipGetErrorString(e));
     }
+    g_live[p] = sz;
     return p;
 }
 
 void dev_free(void* p) {
-    if (p) (void)hipFree(p);
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    auto it = g_live.find(p);
+    if (it == g_live.end()) {
+        (void)hipFree(p);
+        return;
+    }
+    const size_t sz = it->second;
+    g_live.erase(it);
+    if (sz <= kCacheBlockMax && g_cached + sz <= kCacheMax) {
+        g_cache.emplace(sz, p);
+        g_cached += sz;
+    } else {
+        (void)hipFree(p);
+    }
 }
 
 void dev_fill_u64(void* p, uint64_t v, int64_t n) {
@@ -652,7 +715,11 @@ static void fill_params(const KernelSource& ks, EvalCtx& c, RfParam* out) {
         const ParamSlot& p = ks.params[i];
         HostVal v;
         if (p.kind == ParamSlot::CONST) {
-            v = eval(mk_const(p.type, p.value), c);
+            // A constant's value as host evaluation gives it (K::Const).
+            if (p.type.is_float()) out[i].f = p.value;
+            else if (p.type.is_unsigned()) out[i].i = wrap_int(p.type, (int64_t)(uint64_t)p.value);
+            else out[i].i = wrap_int(p.type, (int64_t)p.value);
+            continue;
         } else if (p.kind == ParamSlot::ARG) {
             v = eval_leaf(mk_sym("$arg." + std::to_string(p.arg), p.type), c);
         } else {
