@@ -25,8 +25,6 @@
 
 namespace rf {
 
-hipError_t aot_fill_u64(void* p, uint64_t v, int64_t n, hipStream_t s);
-
 static constexpr int kMaxGroups = 8;     // distinct query kernels per round
 static constexpr size_t kSlot = 512;     // result bytes per unit (Acc, count at the end)
 
@@ -42,9 +40,9 @@ struct rf_batch {
     // Device regions and their pinned host mirrors.
     char* arena = nullptr;
     size_t arena_bytes = 0, arena_used = 0;
-    char* d_ctl = nullptr;   // hdr[kMaxGroups] | queue[kMaxGroups] | start[cap + kMaxGroups] | work[cap]
+    char* d_ctl = nullptr;   // hdr[kMaxGroups] | start[cap + kMaxGroups] | work[cap]
     char* h_ctl = nullptr;
-    size_t ctl_bytes = 0, off_queue = 0, off_start = 0, off_work = 0;
+    size_t ctl_bytes = 0, off_start = 0, off_work = 0;
     char* d_res = nullptr;   // [cap][kSlot]
     char* h_res = nullptr;
     struct Unit {
@@ -85,8 +83,7 @@ rf_batch* batch_create(int cap, int64_t arena_bytes, bool use_graph) {
     RF_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
     b->arena_bytes = (size_t)arena_bytes;
     b->arena = (char*)dev_alloc(b->arena_bytes);
-    b->off_queue = align256(sizeof(RfBatchHdr) * kMaxGroups);
-    b->off_start = b->off_queue + align256(8 * kMaxGroups);
+    b->off_start = align256(sizeof(RfBatchHdr) * kMaxGroups);
     b->off_work = b->off_start + align256(8 * (size_t)(cap + kMaxGroups));
     b->ctl_bytes = b->off_work + sizeof(RfScanArgs) * (size_t)cap;
     b->d_ctl = (char*)dev_alloc(b->ctl_bytes);
@@ -201,13 +198,11 @@ static void enqueue_round(rf_batch* b, const std::vector<Group>& gs, bool full_c
     size_t bytes = b->ctl_bytes;
     if (!full_copy) bytes = b->off_work + sizeof(RfScanArgs) * b->units.size();
     RF_HIP(hipMemcpyAsync(b->d_ctl, b->h_ctl, bytes, hipMemcpyHostToDevice, s));
-    RF_HIP(aot_fill_u64(b->d_ctl + b->off_queue, 0, kMaxGroups, s));
     for (size_t g = 0; g < gs.size(); g++) {
         RfBatchArgs B;
         B.hdr = (const RfBatchHdr*)b->d_ctl + g;
         B.work = (const RfScanArgs*)(b->d_ctl + b->off_work);
         B.item_start = (const int64_t*)(b->d_ctl + b->off_start);
-        B.queue = (unsigned long long*)(b->d_ctl + b->off_queue) + g;
         void* params[] = {&B};
         const Group& G = gs[g];
         const int grid = d.num_cus * G.mod->bscan_blocks_per_cu;
@@ -218,14 +213,13 @@ static void enqueue_round(rf_batch* b, const std::vector<Group>& gs, bool full_c
         B.hdr = (const RfBatchHdr*)b->d_ctl + g;
         B.work = (const RfScanArgs*)(b->d_ctl + b->off_work);
         B.item_start = (const int64_t*)(b->d_ctl + b->off_start);
-        B.queue = (unsigned long long*)(b->d_ctl + b->off_queue) + g;
         void* params[] = {&B};
         const Group& G = gs[g];
         const int units = full_copy ? b->cap : (int)G.units.size();
         RF_HIP(hipModuleLaunchKernel(G.mod->bdone, units, 1, 1, G.block, 1, 1, 0, s, params, nullptr));
         if (G.has_sel)
-            RF_HIP(hipModuleLaunchKernel(G.mod->bgather, 16, units, 1, 256, 1, 1, 0, s, params,
-                                         nullptr));
+            RF_HIP(hipModuleLaunchKernel(G.mod->bgather, d.num_cus * 4, 1, 1, 256, 1, 1, 0, s,
+                                         params, nullptr));
     }
     const size_t rb = full_copy ? kSlot * (size_t)b->cap : kSlot * b->units.size();
     RF_HIP(hipMemcpyAsync(b->h_res, b->d_res, rb, hipMemcpyDeviceToHost, s));

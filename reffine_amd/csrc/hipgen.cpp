@@ -144,8 +144,8 @@ struct Emitter {
             int s = g.col_slot(t, c, mode == Mode::ROWS);
             const std::string T = cty(g.ks.cols[s].type);
             if (mode == Mode::ROWS) return "c" + std::to_string(s) + ".v[k]";
-            if (mode == Mode::FINAL) return "((const " + T + "*)L.col[" + std::to_string(s) + "])[brow]";
-            return "((const " + T + "*)L.col[" + std::to_string(s) + "])[row]";
+            if (mode == Mode::FINAL) return "rf_g((const " + T + "*)L.col[" + std::to_string(s) + "])[brow]";
+            return "rf_g((const " + T + "*)L.col[" + std::to_string(s) + "])[row]";
         }
         if (kind == "$valid") {
             int t = std::stoi(parts[1]), c = std::stoi(parts[2]);
@@ -162,7 +162,7 @@ struct Emitter {
             int s = g.col_slot(t, c, false);
             std::string h = probe(j);
             const std::string T = cty(g.ks.cols[s].type);
-            return tmp("(" + h + " >= 0 ? ((const " + T + "*)L.col[" + std::to_string(s) +
+            return tmp("(" + h + " >= 0 ? rf_g((const " + T + "*)L.col[" + std::to_string(s) +
                        "])[" + h + "] : (" + T + ")0)");
         }
         if (kind == "$found") {
@@ -587,10 +587,10 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
         o << eb.str();
         for (size_t j = 0; j < vals.size(); j++) {
             const std::string T = cty(plan.ops[0].types[j]), J = std::to_string(j);
-            o << "        ((" << T << "*)L.stage[" << J << "])[pos] = (" << T << ")" << vals[j]
+            o << "        rf_gw((" << T << "*)L.stage[" << J << "])[pos] = (" << T << ")" << vals[j]
               << ";\n";
-            mv << "        ((" << T << "*)L.out[" << J << "])[dst] = ((const " << T << "*)L.stage["
-               << J << "])[src];\n";
+            mv << "        rf_gw((" << T << "*)L.out[" << J << "])[dst] = rf_g((const " << T
+               << "*)L.stage[" << J << "])[src];\n";
             ks.out_types.push_back(plan.ops[0].types[j]);
         }
     } else {

@@ -91,5 +91,4 @@ struct RfBatchArgs {
     const struct RfBatchHdr* hdr;
     const struct RfScanArgs* work;     // the round's units
     const int64_t* item_start;         // per launch: nunits + 1 cumulative tile counts
-    unsigned long long* queue;         // item counter, zero before the launch
 };

@@ -31,6 +31,19 @@
 
 // ------------------------------------------------------------ loads
 
+// Every buffer the kernels touch is global memory. Pointers read from
+// memory (a batch's per-unit arguments) lose that address space, which would
+// make their accesses flat ones; these casts restore it.
+#define RF_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const RF_GLOBAL T* rf_g(const T* p) {
+    return (const RF_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ RF_GLOBAL T* rf_gw(T* p) {
+    return (RF_GLOBAL T*)p;
+}
+
 template <typename T, int N>
 struct RfVec {
     T v[N];
@@ -45,21 +58,21 @@ __device__ __forceinline__ RfVec<T, N> rf_ldv(const T* __restrict__ p) {
     constexpr int BYTES = N * (int)sizeof(T);
     if constexpr (BYTES % 16 == 0) {
         typedef unsigned int u4 __attribute__((ext_vector_type(4)));
-        const u4* q = reinterpret_cast<const u4*>(p);
+        const RF_GLOBAL u4* q = reinterpret_cast<const RF_GLOBAL u4*>(rf_g(p));
         u4 tmp[BYTES / 16];
 #pragma unroll
         for (int i = 0; i < BYTES / 16; i++) tmp[i] = __builtin_nontemporal_load(q + i);
         __builtin_memcpy(&r, tmp, BYTES);
     } else if constexpr (BYTES % 8 == 0) {
         typedef unsigned int u2 __attribute__((ext_vector_type(2)));
-        const u2* q = reinterpret_cast<const u2*>(p);
+        const RF_GLOBAL u2* q = reinterpret_cast<const RF_GLOBAL u2*>(rf_g(p));
         u2 tmp[BYTES / 8];
 #pragma unroll
         for (int i = 0; i < BYTES / 8; i++) tmp[i] = __builtin_nontemporal_load(q + i);
         __builtin_memcpy(&r, tmp, BYTES);
     } else {
 #pragma unroll
-        for (int i = 0; i < N; i++) r.v[i] = p[i];
+        for (int i = 0; i < N; i++) r.v[i] = rf_g(p)[i];
     }
     return r;
 }
@@ -69,7 +82,7 @@ __device__ __forceinline__ RfVec<T, N> rf_ldv_tail(const T* __restrict__ p,
                                                    int64_t r, int64_t end) {
     RfVec<T, N> x;
 #pragma unroll
-    for (int i = 0; i < N; i++) x.v[i] = (r + i < end) ? p[r + i] : T(0);
+    for (int i = 0; i < N; i++) x.v[i] = (r + i < end) ? rf_g(p)[r + i] : T(0);
     return x;
 }
 
@@ -79,13 +92,13 @@ __device__ __forceinline__ RfVec<T, N> rf_ldv_tail(const T* __restrict__ p,
 template <int N>
 __device__ __forceinline__ uint32_t rf_valid_bits(const uint16_t* __restrict__ bm,
                                                   int64_t r) {
-    uint32_t w = bm[r >> 4];
+    uint32_t w = rf_g(bm)[r >> 4];
     return (w >> (uint32_t)(r & 15)) & ((1u << N) - 1u);
 }
 
 __device__ __forceinline__ bool rf_valid_bit(const uint16_t* __restrict__ bm,
                                              int64_t i) {
-    return (bm[i >> 4] >> (uint32_t)(i & 15)) & 1u;
+    return (rf_g(bm)[i >> 4] >> (uint32_t)(i & 15)) & 1u;
 }
 
 // ------------------------------------------------------------ probes
@@ -107,12 +120,12 @@ __device__ __forceinline__ int64_t rf_probe(const RfProbeDev& h, int64_t key) {
         // Keys absent from the filter are absent from the table: one read
         // of a small (cache-resident) bitmap instead of a slot read.
         const uint64_t fb = (hk >> 32) & h.fmask;
-        if (!((h.filter[fb >> 5] >> (fb & 31)) & 1u)) return -1;
+        if (!((rf_g(h.filter)[fb >> 5] >> (fb & 31)) & 1u)) return -1;
     }
     uint64_t s = hk & h.mask;
     for (;;) {
-        int64_t k = h.keys[s];
-        if (k == key) return h.rows[s];
+        int64_t k = rf_g(h.keys)[s];
+        if (k == key) return rf_g(h.rows)[s];
         if (k == RF_EMPTY_KEY) return -1;
         s = (s + 1) & h.mask;
     }
@@ -326,7 +339,7 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
             if (grp % NG == NG - 1) {
                 Acc t = a_tile;
                 rf_wave_reduce<Q>(t);
-                if (lane == 0) reinterpret_cast<Acc*>(L.partials)[grp / NG] = t;
+                if (lane == 0) rf_gw(reinterpret_cast<Acc*>(L.partials))[grp / NG] = t;
                 Q::acc_zero(a_tile);
             }
         }
@@ -416,7 +429,7 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
 template <class Q>
 __device__ __forceinline__ void rf_gather(const RfScanArgs& L) {
     for (int64_t w = blockIdx.x; w < L.nwaves; w += gridDim.x) {
-        const int64_t src = w * L.stage_rows, dst = L.wave_dst[w], cnt = L.wave_cnt[w];
+        const int64_t src = w * L.stage_rows, dst = rf_g(L.wave_dst)[w], cnt = rf_g(L.wave_cnt)[w];
         for (int64_t i = threadIdx.x; i < cnt; i += blockDim.x) Q::move(L, src + i, dst + i);
     }
 }
@@ -424,8 +437,9 @@ __device__ __forceinline__ void rf_gather(const RfScanArgs& L) {
 // ------------------------------------------------------ batched execution
 //
 // Many work units (one query over one record batch each, an RfScanArgs per
-// unit in device memory) run in one persistent launch: waves take
-// (unit, wave-tile) items from one atomic counter. Every item is processed
+// unit in device memory) run in one persistent launch over the list of
+// (unit, wave-tile) items; each wave takes a contiguous range of the list.
+// Every item is processed
 // as the deterministic scan processes a tile: its fold from the identity,
 // wave-combined into partials[tile]; its qualifying rows staged at
 // tile * TILE with the count in wave_cnt[tile] (so stage_rows = TILE and
@@ -434,39 +448,39 @@ __device__ __forceinline__ void rf_gather(const RfScanArgs& L) {
 // counts into offsets; rf_gather places the rows. A unit therefore gets
 // the same bits as the deterministic scan of the same query alone.
 
-// Wave-uniform item counter: the wave's lane 0 takes the next ticket.
-__device__ __forceinline__ int64_t rf_next_item(unsigned long long* queue) {
-    unsigned long long t = 0;
-    if (__lane_id() == 0) t = atomicAdd(queue, 1ull);
-    return rf_scalar64((int64_t)__shfl(t, 0));
-}
-
 template <class Q>
 __device__ __forceinline__ void rf_batch_scan(const RfBatchArgs& B) {
     typedef typename Q::Acc Acc;
     constexpr int ITEMS = Q::ITEMS, SUB = Q::SUB, LSUB = Q::LSUB;
     constexpr int NG = SUB / LSUB;
+    constexpr int WAVES = Q::BLOCK / 64;
     constexpr int64_t GROUP = (int64_t)LSUB * 64 * ITEMS;
     constexpr int64_t TILE = (int64_t)SUB * 64 * ITEMS;
     const uint32_t lane = __lane_id();
     const int64_t nunits = B.hdr->nunits, nitems = B.hdr->nitems;
     const RfScanArgs* work = B.work + B.hdr->unit_base;
     const int64_t* item_start = B.item_start + B.hdr->start_base;
-    int64_t item = rf_next_item(B.queue);
-    while (item < nitems) {
-        // Take the next ticket before working on this one, so its latency
-        // overlaps the tile.
-        const int64_t next = rf_next_item(B.queue);
-        // Unit of the item: the last u with item_start[u] <= item.
-        int64_t lo = 0, hi = nunits - 1;
-        while (lo < hi) {
-            const int64_t mid = (lo + hi + 1) >> 1;
-            if (item_start[mid] <= item) lo = mid;
-            else hi = mid - 1;
-        }
-        lo = rf_scalar64(lo);
-        const RfScanArgs& L = work[lo];
-        const int64_t tile = item - item_start[lo];
+    // Items are the units' wave-tiles, concatenated; wave w takes the
+    // contiguous range [w*per, (w+1)*per) (one counter shared by every wave
+    // would serialise on its address).
+    const int64_t gw = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+    const int64_t nw = (int64_t)gridDim.x * WAVES;
+    const int64_t per = (nitems + nw - 1) / nw;
+    int64_t item = rf_scalar64(gw * per);
+    const int64_t item_end = rf_scalar64(item + per < nitems ? item + per : nitems);
+    if (item >= item_end) return;
+    // Unit of the first item: the last u with item_start[u] <= item.
+    int64_t u = 0, hi = nunits - 1;
+    while (u < hi) {
+        const int64_t mid = (u + hi + 1) >> 1;
+        if (item_start[mid] <= item) u = mid;
+        else hi = mid - 1;
+    }
+    u = rf_scalar64(u);
+    for (; item < item_end; item++) {
+        while (item_start[u + 1] <= item) u++;
+        const RfScanArgs& L = work[u];
+        const int64_t tile = item - item_start[u];
         Acc a;
         Q::acc_zero(a);
         int64_t cursor = 0;
@@ -483,12 +497,11 @@ __device__ __forceinline__ void rf_batch_scan(const RfBatchArgs& B) {
         }
         if constexpr (Q::NACC > 0) {
             rf_wave_reduce<Q>(a);
-            if (lane == 0) reinterpret_cast<Acc*>(L.partials)[tile] = a;
+            if (lane == 0) rf_gw(reinterpret_cast<Acc*>(L.partials))[tile] = a;
         }
         if constexpr (Q::HAS_SEL) {
-            if (lane == 0) L.wave_cnt[tile] = cursor;
+            if (lane == 0) rf_gw(L.wave_cnt)[tile] = cursor;
         }
-        item = next;
     }
 }
 
@@ -503,33 +516,63 @@ __device__ __forceinline__ void rf_batch_done(const RfBatchArgs& B) {
     if (u >= B.hdr->nunits) return;
     const RfScanArgs& L = B.work[B.hdr->unit_base + u];
     if constexpr (Q::NACC > 0) {
-        const Acc* parts = reinterpret_cast<const Acc*>(L.partials);
+        const RF_GLOBAL Acc* parts = rf_g(reinterpret_cast<const Acc*>(L.partials));
         Acc a;
         Q::acc_zero(a);
-        for (int64_t i = threadIdx.x; i < L.ntiles; i += BLOCK) Q::acc_combine(a, parts[i]);
+        for (int64_t i = threadIdx.x; i < L.ntiles; i += BLOCK) {
+            const Acc p = parts[i];
+            Q::acc_combine(a, p);
+        }
         rf_block_reduce<Q>(a, s_waves);
-        if (threadIdx.x == 0) *reinterpret_cast<Acc*>(L.result) = a;
+        if (threadIdx.x == 0) *rf_gw(reinterpret_cast<Acc*>(L.result)) = a;
     }
     if constexpr (Q::HAS_SEL) {
         const int64_t n = L.nwaves;
         const int64_t pt = (n + BLOCK - 1) / BLOCK;
         const int64_t lo = (int64_t)threadIdx.x * pt;
+        const RF_GLOBAL int64_t* cnt = rf_g(L.wave_cnt);
+        RF_GLOBAL int64_t* dst = rf_gw(L.wave_dst);
         uint64_t mine = 0;
-        for (int64_t i = lo; i < lo + pt && i < n; i++) mine += (uint64_t)L.wave_cnt[i];
+        for (int64_t i = lo; i < lo + pt && i < n; i++) mine += (uint64_t)cnt[i];
         uint64_t total;
         uint64_t off = rf_block_excl_scan<BLOCK>(mine, s_scan, total);
         for (int64_t i = lo; i < lo + pt && i < n; i++) {
-            L.wave_dst[i] = (int64_t)off;
-            off += (uint64_t)L.wave_cnt[i];
+            dst[i] = (int64_t)off;
+            off += (uint64_t)cnt[i];
         }
-        if (threadIdx.x == 0) *L.out_count = (int64_t)total;
+        if (threadIdx.x == 0) *rf_gw(L.out_count) = (int64_t)total;
     }
 }
 
-// Blocks (x, unit): the selection placement of every unit.
+// Selection placement of every unit: a persistent launch over the same
+// (unit, wave-tile) item list as the scan, one wave per tile's staged rows,
+// each wave a contiguous range of items.
 template <class Q>
 __device__ __forceinline__ void rf_batch_gather(const RfBatchArgs& B) {
-    const int64_t u = blockIdx.y;
-    if (u >= B.hdr->nunits) return;
-    rf_gather<Q>(B.work[B.hdr->unit_base + u]);
+    constexpr int64_t TILE = (int64_t)Q::SUB * 64 * Q::ITEMS;
+    const uint32_t lane = __lane_id();
+    const int64_t nunits = B.hdr->nunits, nitems = B.hdr->nitems;
+    const RfScanArgs* work = B.work + B.hdr->unit_base;
+    const int64_t* item_start = B.item_start + B.hdr->start_base;
+    const int64_t gw = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / 64);
+    const int64_t per = (nitems + nw - 1) / nw;
+    int64_t item = rf_scalar64(gw * per);
+    const int64_t item_end = rf_scalar64(item + per < nitems ? item + per : nitems);
+    if (item >= item_end) return;
+    int64_t u = 0, hi = nunits - 1;
+    while (u < hi) {
+        const int64_t mid = (u + hi + 1) >> 1;
+        if (item_start[mid] <= item) u = mid;
+        else hi = mid - 1;
+    }
+    u = rf_scalar64(u);
+    for (; item < item_end; item++) {
+        while (item_start[u + 1] <= item) u++;
+        const RfScanArgs& L = work[u];
+        const int64_t tile = item - item_start[u];
+        const int64_t src = tile * TILE;
+        const int64_t dst = rf_g(L.wave_dst)[tile], cnt = rf_g(L.wave_cnt)[tile];
+        for (int64_t i = lane; i < cnt; i += 64) Q::move(L, src + i, dst + i);
+    }
 }
