@@ -219,9 +219,9 @@ int rf_query_free(rf_query* q);
 /* ------------------------------------------------------ batched / served */
 
 /* A batch runner: a round of work units (one compiled query over its
- * arguments each) run together -- one persistent work-queue launch per
- * distinct query kernel over all (unit, wave-tile) items, then per-unit
- * completion and selection placement. Each unit's result equals the
+ * arguments each) run together -- per distinct query kernel, one persistent
+ * launch over the list of all its units' (unit, wave-tile) items (each wave
+ * a contiguous range), then per-unit completion and selection placement. Each unit's result equals the
  * deterministic-mode run of the same query alone (RF_COMPILE_DETERMINISTIC
  * queries give identical bits in both). `max_units` units per round;
  * `arena_bytes` of device memory for the units' partials and selection

@@ -29,7 +29,7 @@ EMBED = {"kRfAbiSrc": "kernels/rf_abi.h", "kRfDeviceSrc": "kernels/rf_device.h"}
 COMMON = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
           f"-I{CSRC}", f"-I{BUILD}", f"-I{os.path.join(HERE, '..', 'include')}",
           f"-I{os.path.join(ROCM, 'include')}"]
-# gfx950 device flags: no packed FP32 (see design notes), no FMA contraction
+# gfx950 device flags: packed FP32 instructions disabled, no FMA contraction
 # so FP64 arithmetic rounds exactly as the C text says.
 DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-ffp-contract=off", "-munsafe-fp-atomics",
                 "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

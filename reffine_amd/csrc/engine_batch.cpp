@@ -2,9 +2,9 @@
 //
 // A batch runner collects work units -- one compiled query over its
 // arguments (one record batch) each -- into a round and runs the round with
-// one persistent launch per distinct query kernel (rf_qb, see
-// kernels/rf_device.h), one completion launch per kernel and one placement
-// launch for selections. Everything a launch reads lives in buffers that are
+// one persistent launch per distinct query kernel over the list of its
+// units' wave-tiles (rf_qb, see kernels/rf_device.h), one completion launch
+// per kernel and one placement launch for selections. Everything a launch reads lives in buffers that are
 // fixed per runner (a control region rewritten each round, a result region,
 // a bump arena for the units' partials and selection buffers), so a round can
 // be captured into a hipGraph once per set of kernels and replayed.
