@@ -169,9 +169,10 @@ int rf_compile(const char* ir_text, uint32_t flags, rf_query** out);
 
 /* rf_compile with an explicit launch shape `tuning` = {BLOCK (64..1024,
  * multiple of 64), ITEMS (1, 2, 4, 8 or 16), SUB (1..64), WAVES_PER_CU
- * (1..64, 0 = 16), PIPELINE (0/1: overlap each wave's next tile loads with
- * the current tile)}: the autotuner's entry point. A wave-tile is
- * SUB*64*ITEMS rows. */
+ * (1..64, 0 = 16), PIPELINE (0/1: overlap each wave's next load group with
+ * the current one), LSUB (sub-steps per load group, dividing SUB; 0 = from
+ * the register budget)}: the autotuner's entry point. A wave-tile is
+ * SUB*64*ITEMS rows, loaded LSUB*64*ITEMS rows at a time. */
 int rf_compile_tuned(const char* ir_text, uint32_t flags,
                      const int32_t* tuning, rf_query** out);
 
@@ -188,12 +189,12 @@ int rf_compile_check(const char* ir_text, uint32_t flags, char* buf,
 int rf_query_output_info(const rf_query* q, int64_t* out_bytes,
                          int32_t* out_is_table);
 
-/* Launch shape of the query's scan kernel, written to tuning[0..4]:
+/* Launch shape of the query's scan kernel, written to tuning[0..5]:
  * threads per block, rows per lane per sub-step (ITEMS), sub-steps per
- * wave-tile (SUB), resident waves per CU and the pipelining flag. In the
- * deterministic mode
- * ITEMS and SUB fix the reduction order (wave-tiles of SUB*64*ITEMS rows,
- * the completion combine over BLOCK threads). No reference counterpart. */
+ * wave-tile (SUB), resident waves per CU, the pipelining flag and LSUB
+ * (0 = from the register budget). In the deterministic mode ITEMS and SUB
+ * fix the reduction order (wave-tiles of SUB*64*ITEMS rows, the completion
+ * combine over BLOCK threads). No reference counterpart. */
 int rf_query_tuning(const rf_query* q, int32_t* tuning);
 
 /* Human-readable lowering summary (kernel kind, columns, fields). */

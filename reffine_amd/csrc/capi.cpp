@@ -168,7 +168,9 @@ static void compile_impl(const char* ir_text, uint32_t flags, const int32_t* tun
     q->tune = rf::default_tuning(q->plan, q->det);
     if (tuning) {
         rf::Tuning t{tuning[0], tuning[1], tuning[2], tuning[3] > 0 ? tuning[3] : 16,
-                     tuning[4] != 0 ? 1 : 0};
+                     tuning[4] != 0 ? 1 : 0, tuning[5] > 0 ? tuning[5] : 0};
+        need(t.lsub == 0 || (t.lsub <= t.sub && t.sub % t.lsub == 0),
+             "lsub must divide sub");
         need(t.block >= 64 && t.block <= 1024 && t.block % 64 == 0, "block must be 64..1024, x64");
         need(t.items == 1 || t.items == 2 || t.items == 4 || t.items == 8 || t.items == 16,
              "items must be 1, 2, 4, 8 or 16");
@@ -255,6 +257,7 @@ int rf_query_tuning(const rf_query* q, int32_t* tuning) {
         tuning[2] = q->tune.sub;
         tuning[3] = q->tune.waves_per_cu;
         tuning[4] = q->tune.pipeline;
+        tuning[5] = q->tune.lsub;
     });
 }
 

@@ -880,22 +880,18 @@ Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/
             q->wave_arrays = dev_alloc(q->waves_cap * 2 * 8);
         }
         if (stage_total > q->stage_rows_cap || rows > q->out_rows_cap ||
-            q->outs.size() != ks.out_types.size()) {
+            q->outs.size() != ks.out_types.size() || q->stages.size() != ks.stage_types.size()) {
             for (void* p : q->outs) dev_free(p);
             for (void* p : q->stages) dev_free(p);
             q->outs.clear();
             q->stages.clear();
-            for (auto& t : ks.out_types) {
-                q->outs.push_back(dev_alloc(rows * t.width()));
-                q->stages.push_back(dev_alloc(stage_total * t.width()));
-            }
+            for (auto& t : ks.out_types) q->outs.push_back(dev_alloc(rows * t.width()));
+            for (auto& t : ks.stage_types) q->stages.push_back(dev_alloc(stage_total * t.width()));
             q->out_rows_cap = rows;
             q->stage_rows_cap = stage_total;
         }
-        for (size_t j = 0; j < q->outs.size(); j++) {
-            A.out[j] = q->outs[j];
-            A.stage[j] = q->stages[j];
-        }
+        for (size_t j = 0; j < q->outs.size(); j++) A.out[j] = q->outs[j];
+        for (size_t j = 0; j < q->stages.size(); j++) A.stage[j] = q->stages[j];
         A.stage_rows = stage_rows;
         A.nwaves = nw;
         A.wave_cnt = (int64_t*)q->wave_arrays;

@@ -114,8 +114,8 @@ int64_t batch_footprint(rf_query* q, const rf_arg* args, int64_t n_args) {
     if (ks.has_sel) {
         const int64_t rows = std::max<int64_t>(l.end - l.lo, 1);
         f += 2 * align256((size_t)nt * 8);
-        for (auto& t : ks.out_types)
-            f += align256((size_t)(nt * tile) * t.width()) + align256((size_t)rows * t.width());
+        for (auto& t : ks.stage_types) f += align256((size_t)(nt * tile) * t.width());
+        for (auto& t : ks.out_types) f += align256((size_t)rows * t.width());
     }
     return (int64_t)f;
 }
@@ -150,11 +150,13 @@ bool batch_add(rf_batch* b, rf_query* q, const rf_arg* args, int64_t n_args, int
         if (!A.wave_cnt || !A.wave_dst) return false;
         A.stage_rows = tile;
         A.nwaves = A.ntiles;
+        for (size_t j = 0; j < ks.stage_types.size(); j++) {
+            A.stage[j] = arena_take(b, (size_t)(nt * tile) * ks.stage_types[j].width(), used);
+            if (!A.stage[j]) return false;
+        }
         for (size_t j = 0; j < ks.out_types.size(); j++) {
-            const size_t w = ks.out_types[j].width();
-            A.stage[j] = arena_take(b, (size_t)(nt * tile) * w, used);
-            A.out[j] = arena_take(b, (size_t)rows * w, used);
-            if (!A.stage[j] || !A.out[j]) return false;
+            A.out[j] = arena_take(b, (size_t)rows * ks.out_types[j].width(), used);
+            if (!A.out[j]) return false;
             u.outs.push_back(A.out[j]);
         }
     }

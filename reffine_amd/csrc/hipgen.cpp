@@ -28,10 +28,11 @@ std::vector<std::string> split(const std::string& s, char d) {
     return out;
 }
 
-// ROWS: the scan's row code over vector-loaded rows; EMIT: one qualifying
-// row re-read by index; FINAL: one group of a group join (build-table
-// columns at the group's build row).
-enum class Mode { ROWS, EMIT, FINAL };
+// ROWS: the scan's row code over vector-loaded rows; MOVE: the outputs of
+// one qualifying row, read by index where the placement kernel moves the
+// row (probe results the row code computed come from staging); FINAL: one
+// group of a group join (build-table columns at the group's build row).
+enum class Mode { ROWS, MOVE, FINAL };
 
 struct Gen {
     const Plan& plan;
@@ -113,10 +114,14 @@ struct Emitter {
         return v;
     }
 
+    std::set<int> staged;  // MOVE: carried probe results read from staging
+
     std::string probe(int j) {
         std::string h = "h" + std::to_string(j);
-        if (mode == Mode::EMIT && g.carried.count(j))
-            return "cy.p" + std::to_string(j) + "[s][k]";
+        if (mode == Mode::MOVE && g.carried.count(j)) {
+            staged.insert(j);
+            return "sc" + std::to_string(j);
+        }
         if (!probes_done.count(j)) {
             const Probe& pr = g.plan.probes[j];
             std::string k = expr(pr.key);
@@ -145,7 +150,7 @@ struct Emitter {
             const std::string T = cty(g.ks.cols[s].type);
             if (mode == Mode::ROWS) return "c" + std::to_string(s) + ".v[k]";
             if (mode == Mode::FINAL) return "rf_g((const " + T + "*)L.col[" + std::to_string(s) + "])[brow]";
-            return "rf_g((const " + T + "*)L.col[" + std::to_string(s) + "])[row]";
+            return "rf_g((const " + T + "*)L.col[" + std::to_string(s) + "])[row]";  // MOVE
         }
         if (kind == "$valid") {
             int t = std::stoi(parts[1]), c = std::stoi(parts[2]);
@@ -418,6 +423,10 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
     int lsub = 1;
     for (int d = 1; d <= tune.sub; d++)
         if (tune.sub % d == 0 && d <= lcap) lsub = d;
+    if (tune.lsub > 0) {
+        if (tune.sub % tune.lsub != 0) throw IRError("LSUB must divide SUB");
+        lsub = tune.lsub;
+    }
 
     // Accumulator layout: one field per updated state field.
     for (size_t c = 0; c < plan.reduces.size(); c++)
@@ -573,31 +582,44 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
     if (ks.has_sel) o << "            sel |= (uint32_t)(inb && " << sel_pred << ") << k;\n";
     o << "        }\n        return sel;\n    }\n";
 
-    // ---- emit(): selection outputs of one qualifying row, into staging;
-    // move(): one staged row to its output position.
-    o << "    static __device__ __forceinline__ void emit(const RfScanArgs& L, int64_t row, int64_t pos, "
-         "const Carry& cy, int s, int k) {\n        (void)cy; (void)s; (void)k;\n";
+    // ---- emit(): stages a qualifying row's index (and the probe results its
+    // row code computed) -- no loads in the scan; move(): the placement
+    // kernel reads the row's outputs by index and writes them at their
+    // output position (late materialisation: the scan never waits on a
+    // selected row's reads).
     std::ostringstream mv;
+    std::ostringstream em_body;
     if (ks.has_sel) {
-        std::ostringstream eb;
-        Emitter ee(g, Mode::EMIT, eb, "        ");
+        std::ostringstream mb;
+        Emitter me(g, Mode::MOVE, mb, "        ");
         std::vector<std::string> vals;
-        for (auto& v : plan.ops[0].values) vals.push_back(ee.expr(v));
+        for (auto& v : plan.ops[0].values) vals.push_back(me.expr(v));
         if (vals.size() > 16) throw Unsupported("more than 16 output columns");
-        o << eb.str();
+        ks.stage_types.push_back(types::I64);  // row index
+        em_body << "        rf_gw((int64_t*)L.stage[0])[pos] = row;\n";
+        mv << "        const int64_t row = rf_g((const int64_t*)L.stage[0])[src];\n";
+        for (int j : me.staged) {
+            const std::string S = std::to_string(ks.stage_types.size());
+            em_body << "        rf_gw((int64_t*)L.stage[" << S << "])[pos] = cy.p" << j << "[s][k];\n";
+            mv << "        const int64_t sc" << j << " = rf_g((const int64_t*)L.stage[" << S
+               << "])[src];\n";
+            ks.stage_types.push_back(types::I64);
+        }
+        if (ks.stage_types.size() > 16) throw Unsupported("more than 16 staged values");
+        mv << mb.str();
         for (size_t j = 0; j < vals.size(); j++) {
             const std::string T = cty(plan.ops[0].types[j]), J = std::to_string(j);
-            o << "        rf_gw((" << T << "*)L.stage[" << J << "])[pos] = (" << T << ")" << vals[j]
-              << ";\n";
-            mv << "        rf_gw((" << T << "*)L.out[" << J << "])[dst] = rf_g((const " << T
-               << "*)L.stage[" << J << "])[src];\n";
+            mv << "        rf_gw((" << T << "*)L.out[" << J << "])[dst] = (" << T << ")" << vals[j]
+               << ";\n";
             ks.out_types.push_back(plan.ops[0].types[j]);
         }
     } else {
-        o << "        (void)L; (void)row; (void)pos;\n";
+        em_body << "        (void)L; (void)row; (void)pos;\n";
         mv << "        (void)L; (void)src; (void)dst;\n";
     }
-    o << "    }\n";
+    o << "    static __device__ __forceinline__ void emit(const RfScanArgs& L, int64_t row, int64_t pos, "
+         "const Carry& cy, int s, int k) {\n        (void)cy; (void)s; (void)k;\n"
+      << em_body.str() << "    }\n";
     o << "    static __device__ __forceinline__ void move(const RfScanArgs& L, int64_t src, int64_t dst) {\n"
       << mv.str() << "    }\n};\n\n";
     o << "extern \"C\" __global__ __launch_bounds__(" << tune.block

@@ -41,7 +41,8 @@ struct Tuning {
     int items = 4;
     int sub = 4;
     int waves_per_cu = 16;  // launch width: resident waves aimed at per CU
-    int pipeline = 0;       // 1: overlap the next wave-tile's loads with this one's rows
+    int pipeline = 0;       // 1: overlap the next load group's loads with this one's rows
+    int lsub = 0;           // sub-steps per load group (divides sub); 0: from the load budget
 };
 
 struct KernelSource {
@@ -54,6 +55,7 @@ struct KernelSource {
     int acc_bytes = 0;
     bool has_sel = false;
     std::vector<Type> out_types;   // selection output columns
+    std::vector<Type> stage_types; // staged per selected row: row index, carried probe results
     Tuning tune;
     bool det = false;
 };

@@ -436,9 +436,98 @@ struct Lowerer {
         return e;
     }
 
+    // Bounds lb <= it < ub of iterator `it` among the conjuncts (removed from
+    // `cj`); false if it is not bounded on both sides.
+    bool take_bounds(const N& it, std::vector<N>& cj, N& lb, N& ub) {
+        std::vector<N> rest;
+        lb = ub = nullptr;
+        auto one = mk_const(it->type, 1);
+        for (auto& c0 : cj) {
+            N c = resolve_bool_sym(c0);
+            if (c->kind == K::Nary && c->args.size() == 2 &&
+                (c->op == MathOp::GTE || c->op == MathOp::GT || c->op == MathOp::LT ||
+                 c->op == MathOp::LTE)) {
+                bool l_it = c->args[0]->kind == K::Sym && c->args[0]->name == it->name;
+                bool r_it = c->args[1]->kind == K::Sym && c->args[1]->name == it->name;
+                if (l_it && !r_it) {
+                    N b = c->args[1];
+                    if (c->op == MathOp::GTE && !lb) { lb = b; continue; }
+                    if (c->op == MathOp::GT && !lb) { lb = mk_nary(MathOp::ADD, {b, one}); continue; }
+                    if (c->op == MathOp::LT && !ub) { ub = b; continue; }
+                    if (c->op == MathOp::LTE && !ub) { ub = mk_nary(MathOp::ADD, {b, one}); continue; }
+                } else if (r_it && !l_it) {
+                    N b = c->args[0];
+                    if (c->op == MathOp::LTE && !lb) { lb = b; continue; }
+                    if (c->op == MathOp::LT && !lb) { lb = mk_nary(MathOp::ADD, {b, one}); continue; }
+                    if (c->op == MathOp::GT && !ub) { ub = b; continue; }
+                    if (c->op == MathOp::GTE && !ub) { ub = mk_nary(MathOp::ADD, {b, one}); continue; }
+                }
+            }
+            rest.push_back(c);
+        }
+        if (!lb || !ub) return false;
+        cj = rest;
+        return true;
+    }
+
+    static bool mentions_sym(const N& e, const std::string& name) {
+        return mentions(e, [&](const N& n) { return n->kind == K::Sym && n->name == name; });
+    }
+
+    // Two iterators bounded by ranges (NestedSpace of two bounded spaces,
+    // test/src/test_nested.cpp:9-25): one range scan over the product,
+    // row r = (a - lb_a) * n_b + (b - lb_b), a outer and b inner as the
+    // nested loops emit them.
+    Source bind_nested(const N& op) {
+        const N a = op->iters[0], b = op->iters[1];
+        for (auto& it : op->iters)
+            if (!it->type.is_int() && !it->type.is_idx())
+                throw Unsupported("nested Op over a non-integer iterator");
+        std::vector<N> cj;
+        conjuncts(resolve_bool_sym(op->args[0]), cj);
+        for (auto& c0 : cj)
+            if (resolve_bool_sym(c0)->kind == K::In)
+                throw Unsupported("two-iterator Op over a table");
+        N lba, uba, lbb, ubb;
+        if (!take_bounds(a, cj, lba, uba) || !take_bounds(b, cj, lbb, ubb))
+            throw Unsupported("Op iterators are not bounded on both sides");
+        for (auto& e : {lba, uba, lbb, ubb})
+            if (mentions_sym(e, a->name) || mentions_sym(e, b->name))
+                throw Unsupported("an iterator bound depends on another iterator");
+        plan.source = Plan::RANGE;
+        const N zero = mk_const(types::IDX_T, 0);
+        N na = mk_nary(MathOp::MAX, {mk_nary(MathOp::SUB, {mk_cast(types::IDX_T, uba),
+                                                           mk_cast(types::IDX_T, lba)}), zero});
+        N nb = mk_nary(MathOp::MAX, {mk_nary(MathOp::SUB, {mk_cast(types::IDX_T, ubb),
+                                                           mk_cast(types::IDX_T, lbb)}), zero});
+        plan.start = zero;
+        plan.end = mk_nary(MathOp::MUL, {na, nb});
+        // Row-side expressions: bounds are host values (constants or scalar
+        // inputs), rewritten to leaves.
+        const N row = leaf("$row", types::IDX_T);
+        const N nbr = rw(nb);
+        const N ia = mk_nary(MathOp::ADD, {mk_cast(types::IDX_T, rw(lba)),
+                                           mk_nary(MathOp::DIV, {row, nbr})});
+        const N ib = mk_nary(MathOp::ADD, {mk_cast(types::IDX_T, rw(lbb)),
+                                           mk_nary(MathOp::MOD, {row, nbr})});
+        subst[a->name] = mk_cast(a->type, ia);
+        subst[b->name] = mk_cast(b->type, ib);
+        iter_sym = nullptr;
+        Source s;
+        N pred;
+        for (auto& c : cj) pred = mk_and(pred, rw(c));
+        s.pred = pred ? pred : true_c();
+        std::vector<N> vals;
+        for (auto& it : op->iters) vals.push_back(rw(it));
+        for (size_t i = 1; i < op->args.size(); i++) vals.push_back(rw(op->args[i]));
+        s.row_value = mk_new(vals);
+        return s;
+    }
+
     Source bind_op(const N& op) {
+        if (op->iters.size() == 2) return bind_nested(op);
         if (op->iters.size() != 1)
-            throw Unsupported("Op with more than one iterator");
+            throw Unsupported("Op with more than two iterators");
         iter_sym = op->iters[0];
         std::vector<N> cj;
         conjuncts(resolve_bool_sym(op->args[0]), cj);

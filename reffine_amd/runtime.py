@@ -408,7 +408,7 @@ class Query:
         if tuning is None:
             _ck(lib().rf_compile(self.text.encode(), flags, ctypes.byref(h)))
         else:
-            tv = (ctypes.c_int32 * 5)(*(list(tuning) + [0] * (5 - len(tuning))))
+            tv = (ctypes.c_int32 * 6)(*(list(tuning) + [0] * (6 - len(tuning))))
             _ck(lib().rf_compile_tuned(self.text.encode(), flags, tv, ctypes.byref(h)))
         self.handle = h
         nb = ctypes.c_int64()
@@ -465,10 +465,11 @@ class Query:
                                 ctypes.byref(scan)))
         return ms.value, scan.value
 
-    def tuning(self) -> tuple[int, int, int, int, int]:
-        """(BLOCK, ITEMS, SUB, WAVES_PER_CU, PIPELINE) of the scan kernel;
-        with deterministic=True BLOCK, ITEMS and SUB fix the reduction order."""
-        t = (ctypes.c_int32 * 5)()
+    def tuning(self) -> tuple[int, int, int, int, int, int]:
+        """(BLOCK, ITEMS, SUB, WAVES_PER_CU, PIPELINE, LSUB) of the scan
+        kernel; with deterministic=True BLOCK, ITEMS and SUB fix the
+        reduction order."""
+        t = (ctypes.c_int32 * 6)()
         _ck(lib().rf_query_tuning(self.handle, t))
         return tuple(t)
 

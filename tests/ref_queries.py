@@ -110,3 +110,16 @@ def join_op():
     fn.tbl[c_s] = ce
     fn.tbl[op_sym] = op
     return fn
+
+
+def nested_op(a_ub=10, b_ub=10):
+    """test/src/test_nested.cpp:9-25: a two-iterator Op over
+    0 <= a < a_ub, 0 <= b < b_ub with output a + b."""
+    a = _sym("a", _i64_t)
+    b = _sym("b", _i64_t)
+    op = _op([a, b], _gte(a, _i64(0)) & _lt(a, _i64(a_ub)) & _gte(b, _i64(0)) &
+             _lt(b, _i64(b_ub)), [_add(a, b)])
+    os_ = _sym("op", op)
+    fn = _func("foo", os_, [])
+    fn.tbl[os_] = op
+    return fn

@@ -91,3 +91,9 @@ def test_group_join_lowers_and_compiles():
     assert text.count("--- kernel ---") == 3
     assert "? rf_probe(" in text  # probed only where the probe predicate holds
     assert runtime.compile_check(workloads.group_join_c4(), nullable=True)
+
+
+def test_nested_two_iterator_op_compiles():
+    import ref_queries as rq
+    text = runtime.compile_check(rq.nested_op())
+    assert "scan range rows [(const x 0), (mul" in text

@@ -55,3 +55,16 @@ def test_empty_range_and_single_row(device):
     assert gen(9, 3).nrows == 0
     t = gen(-3, -2)
     np.testing.assert_array_equal(t.column(0), [-3])
+
+
+@pytest.mark.parametrize("a_ub,b_ub", [(10, 10), (3, 7), (1, 1), (0, 5)])
+def test_nested_golden(device, a_ub, b_ub):
+    """test/src/test_nested.cpp:42-53: row idx holds (idx / b_ub, idx % b_ub,
+    their sum), a outer and b inner."""
+    out = compile_op(rq.nested_op(a_ub, b_ub))()
+    n = a_ub * b_ub
+    assert out.nrows == n
+    idx = np.arange(n)
+    assert np.array_equal(out.column(0), idx // max(b_ub, 1))
+    assert np.array_equal(out.column(1), idx % max(b_ub, 1))
+    assert np.array_equal(out.column(2), idx // max(b_ub, 1) + idx % max(b_ub, 1))
