@@ -355,16 +355,15 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
         // evaluated.
         typename Q::Tile TA, TB;
         if (g_begin < g_end && interior(g_begin)) Q::load(TA, L, L.start + g_begin * GROUP, lane);
-        for (int64_t grp = g_begin; grp < g_end; grp += 2) {
-            const bool in0 = interior(grp);
-            const bool has1 = grp + 1 < g_end;
-            const bool in1 = has1 && interior(grp + 1);
+        bool in0 = g_begin < g_end && interior(g_begin);
+        for (int64_t grp = g_begin; grp < g_end; grp++) {
+            const bool in1 = grp + 1 < g_end && interior(grp + 1);
             if (in1) Q::load(TB, L, L.start + (grp + 1) * GROUP, lane);
             process(TA, grp, in0);
-            if (!has1) break;
-            if (grp + 2 < g_end && interior(grp + 2))
-                Q::load(TA, L, L.start + (grp + 2) * GROUP, lane);
-            process(TB, grp + 1, in1);
+            // One copy of the row code per iteration (the instruction cache
+            // holds it); the next group's registers move into place.
+            TA = TB;
+            in0 = in1;
         }
     } else {
         for (int64_t grp = g_begin; grp < g_end; grp++) {
