@@ -189,7 +189,8 @@ static void compile_impl(const char* ir_text, uint32_t flags, const int32_t* tun
     for (size_t i = 0; i < q->plan.inputs.size(); i++)
         nl[i].assign(q->plan.inputs[i]->type.dtypes.size(), false);
     rf::Spec sp;
-    sp.ks = rf::gen_scan(q->plan, nl, q->tune, q->det);
+    sp.ks = q->plan.seg ? rf::gen_seg_fold(*q->plan.seg, q->plan, nl)
+                        : rf::gen_scan(q->plan, nl, q->tune, q->det);
     sp.mod = &rf::jit_module(sp.ks);
     std::string sig;
     for (size_t i = 0; i < q->plan.inputs.size(); i++) {
@@ -223,6 +224,8 @@ int rf_compile_check(const char* ir_text, uint32_t flags, char* buf, size_t cap)
         std::vector<rf::KernelSource> kss;
         if (plan.gj)
             kss = rf::group_join_sources(plan, nl);
+        else if (plan.seg)
+            kss.push_back(rf::gen_seg_fold(*plan.seg, plan, nl));
         else
             kss.push_back(rf::gen_scan(plan, nl, tune, (flags & RF_COMPILE_DETERMINISTIC) != 0));
         size_t bytes = 0;

@@ -683,7 +683,8 @@ static Spec& get_spec(rf_query* q, const rf_arg* args, int64_t n_args) {
         for (auto& c : args[i].u.table->cols) nl[i].push_back(c.validity != nullptr);
     }
     Spec sp;
-    sp.ks = gen_scan(q->plan, nl, q->tune, q->det);
+    sp.ks = q->plan.seg ? gen_seg_fold(*q->plan.seg, q->plan, nl)
+                        : gen_scan(q->plan, nl, q->tune, q->det);
     sp.mod = &jit_module(sp.ks);
     auto r = q->specs.emplace(sig, std::move(sp));
     return r.first->second;
@@ -1220,9 +1221,51 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
     return out.release();
 }
 
+// Runs a run reduction (SegReducePlan): one output row per run.
+static rf_table* run_seg_reduce(rf_query* q, const rf_arg* args, int64_t n_args) {
+    Device& d = device();
+    check_args(q->plan, args, n_args);
+    const SegReducePlan& sp = *q->plan.seg;
+    Spec& spec = get_spec(q, args, n_args);
+    const rf_table* t = args[sp.table].u.table;
+    const Column& c0 = t->cols.at(0);
+    if (c0.enc != Enc::RUNEND) throw ArrowError("column 0 is not run-end encoded");
+    if (c0.validity) throw Unsupported("null run values");
+    const int64_t runs = c0.length;
+    std::unique_ptr<rf_table> out(alloc_table(sp.out_types, runs, 1));
+    if (runs == 0) return out.release();
+    RfSegArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.runs = runs;
+    A.rows = c0.logical_length;
+    A.run_ends = (const int32_t*)c0.run_ends;
+    A.keys = c0.values;
+    for (size_t s = 0; s < spec.ks.cols.size(); s++) {
+        const ColSlot& cs = spec.ks.cols[s];
+        const Column& col = args[cs.table].u.table->cols.at(cs.col);
+        A.col[s] = col.values;
+        A.valid[s] = (const uint16_t*)col.validity;
+    }
+    for (size_t j = 0; j < out->cols.size(); j++) A.out[j] = out->cols[j].values;
+    EvalCtx c{&q->plan, args, n_args, {}, {}};
+    fill_params(spec.ks, c, A.p);
+    void* params[] = {&A};
+    const int g = (int)std::min<int64_t>((runs + 255) / 256, 16384);
+    RF_HIP(hipModuleLaunchKernel(spec.mod->fn, g, 1, 1, 256, 1, 1, 0, d.stream, params, nullptr));
+    RF_HIP(hipStreamSynchronize(d.stream));
+    return out.release();
+}
+
 void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_addr,
                rf_table** out_table) {
     std::lock_guard<std::mutex> lk(q->mu);
+    if (q->plan.seg) {
+        if (!out_table) throw IRError("an Op output needs out_table");
+        rf_table* t = run_seg_reduce(q, args, n_args);
+        *out_table = t;
+        if (output_addr) *(void**)output_addr = t;
+        return;
+    }
     if (q->gj) {
         if (!out_table) throw IRError("an Op output needs out_table");
         rf_table* t = run_group_join(q, args, n_args, nullptr);
@@ -1331,6 +1374,21 @@ void time_query(rf_query* q, const rf_arg* args, int64_t n_args, int iters, doub
         (void)hipEventDestroy(tm.scan1);
         *ms_all = a / iters;
         *ms_scan = scan / iters;
+        return;
+    }
+    if (q->plan.seg) {
+        if (iters <= 0) return;
+        hipEvent_t e[2];
+        for (auto& x : e) RF_HIP(hipEventCreate(&x));
+        table_free(run_seg_reduce(q, args, n_args));
+        RF_HIP(hipEventRecord(e[0], d.stream));
+        for (int i = 0; i < iters; i++) table_free(run_seg_reduce(q, args, n_args));
+        RF_HIP(hipEventRecord(e[1], d.stream));
+        RF_HIP(hipEventSynchronize(e[1]));
+        float a = 0;
+        RF_HIP(hipEventElapsedTime(&a, e[0], e[1]));
+        for (auto& x : e) (void)hipEventDestroy(x);
+        *ms_all = *ms_scan = a / iters;
         return;
     }
     Launch l = prepare(q, args, n_args, nullptr);

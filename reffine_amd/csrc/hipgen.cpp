@@ -179,8 +179,8 @@ struct Emitter {
             if (mode == Mode::FINAL) throw IRError("row index in a group join output");
             return "row";
         }
-        if (kind == "$key" && mode == Mode::FINAL) return "key";
-        if (kind == "$acc" && mode == Mode::FINAL) return "s" + parts[1];
+        if (kind == "$key") return "key";
+        if (kind == "$acc") return "s" + parts[1];
         if (kind == "$arg") {
             ParamSlot p{ParamSlot::ARG, e->type, 0, std::stoi(parts[1]), 0};
             return g.param(p, name + ":" + e->type.str());
@@ -632,6 +632,66 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
       << ") void rf_qb_done(RfBatchArgs B) { rf_batch_done<Q>(B); }\n";
     o << "extern \"C\" __global__ __launch_bounds__(256) void rf_qb_gather(RfBatchArgs B) { "
          "rf_batch_gather<Q>(B); }\n";
+    ks.code = o.str();
+    return ks;
+}
+
+KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
+                          const std::vector<std::vector<bool>>& nullable) {
+    KernelSource ks;
+    ks.entry = "rf_seg_fold";
+    ks.tune.block = 256;
+    Gen g(plan, nullable, ks);
+    std::ostringstream o;
+    o << "// Generated run reduction for '" << sp.sym_name << "'.\n";
+    o << "extern \"C\" __global__ __launch_bounds__(256) void rf_seg_fold(RfSegArgs L) {\n";
+    o << "    for (int64_t run = blockIdx.x * (int64_t)256 + threadIdx.x; run < L.runs;\n"
+         "         run += (int64_t)gridDim.x * 256) {\n";
+    o << "        const int64_t lo = run > 0 ? (int64_t)L.run_ends[run - 1] : 0;\n";
+    o << "        const int64_t hi0 = (int64_t)L.run_ends[run];\n";
+    o << "        const int64_t hi = hi0 < L.rows ? hi0 : L.rows;\n";
+    std::ostringstream init;
+    Emitter ei(g, Mode::FINAL, init, "        ");
+    std::vector<std::string> iv;
+    for (auto& f : sp.fields) iv.push_back(ei.expr(f.init));
+    o << init.str();
+    for (size_t f = 0; f < sp.fields.size(); f++) {
+        const std::string T = cty(sp.fields[f].type);
+        o << "        " << T << " s" << f << " = (" << T << ")" << iv[f] << ";\n";
+    }
+    // The run's rows in ascending order: the reference's SubVector loop.
+    std::ostringstream body;
+    Emitter eb(g, Mode::MOVE, body, "            ");
+    eb.ntmp = ei.ntmp;
+    std::vector<std::string> pr(sp.fields.size()), ct(sp.fields.size());
+    for (size_t f = 0; f < sp.fields.size(); f++) {
+        if (!sp.fields[f].updated) continue;
+        pr[f] = eb.expr(sp.fields[f].pred);
+        ct[f] = eb.expr(sp.fields[f].contrib);
+    }
+    o << "        for (int64_t row = lo; row < hi; row++) {\n" << body.str();
+    for (size_t f = 0; f < sp.fields.size(); f++) {
+        if (!sp.fields[f].updated) continue;
+        const std::string S = "s" + std::to_string(f);
+        o << "            " << S << " = " << pr[f] << " ? "
+          << combine_expr(sp.fields[f].op, sp.fields[f].type, S, ct[f]) << " : " << S << ";\n";
+    }
+    o << "        }\n";
+    o << "        const " << cty(sp.key_type) << " key = ((const " << cty(sp.key_type)
+      << "*)L.keys)[run];\n        (void)key;\n";
+    std::ostringstream ob;
+    Emitter eo(g, Mode::FINAL, ob, "        ");
+    eo.ntmp = eb.ntmp;
+    std::vector<std::string> ov;
+    for (auto& v : sp.out_values) ov.push_back(eo.expr(v));
+    if (ov.size() > 16) throw Unsupported("more than 16 run reduction outputs");
+    o << ob.str();
+    for (size_t j = 0; j < ov.size(); j++) {
+        const std::string T = cty(sp.out_types[j]);
+        o << "        ((" << T << "*)L.out[" << j << "])[run] = (" << T << ")" << ov[j] << ";\n";
+        ks.out_types.push_back(sp.out_types[j]);
+    }
+    o << "    }\n}\n";
     ks.code = o.str();
     return ks;
 }

@@ -70,6 +70,10 @@ KernelSource gen_scan(const Plan& plan,
 KernelSource gen_group_fold(const GroupJoinPlan& gj,
                             const std::vector<std::vector<bool>>& nullable);
 
+// Per-run fold and output kernel (rf_seg_fold) of a run reduction.
+KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
+                          const std::vector<std::vector<bool>>& nullable);
+
 // Picks ITEMS from the narrowest driver column the row code loads.
 Tuning default_tuning(const Plan& plan);
 // The same, for the deterministic mode when `det` (larger wave-tiles).

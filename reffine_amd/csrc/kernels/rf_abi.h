@@ -92,3 +92,16 @@ struct RfBatchArgs {
     const struct RfScanArgs* work;     // the round's units
     const int64_t* item_start;         // per launch: nunits + 1 cumulative tile counts
 };
+
+// One run reduction (generated rf_seg_fold): run r covers rows
+// [run_ends[r-1], run_ends[r]) (from 0 for r = 0), clamped to `rows`.
+struct RfSegArgs {
+    int64_t runs;
+    int64_t rows;
+    const int32_t* run_ends;
+    const void* keys;             // run values
+    const void* col[RF_MAXCOLS];  // columns the row code and outputs read
+    const uint16_t* valid[RF_MAXCOLS];
+    void* out[RF_MAXOUT];         // output columns, one row per run
+    union RfParam p[RF_MAXPARAMS];
+};

@@ -887,6 +887,120 @@ struct Lowerer {
         plan.end = leaf("$len." + std::to_string(gj->probe) + ".0", types::IDX_T);
     }
 
+    // ------------------------------------------------------- run segments
+
+    // The reduction of an Op output whose vector is T[t], T a table with a
+    // run-end encoded column 0 and t the Op's iterator; null otherwise.
+    N seg_reduce_of(const N& op) {
+        if (op->iters.size() != 1) return nullptr;
+        std::vector<N> reds;
+        std::set<std::string> seen;
+        for (size_t i = 1; i < op->args.size(); i++) find_reduces(op->args[i], reds, seen);
+        if (reds.size() != 1) return nullptr;
+        N v = resolve_vec(reds[0]->args[0]);
+        if (v->kind != K::Element) return nullptr;
+        N vec = resolve_vec(v->args[0]);
+        int t = input_index(vec);
+        if (t < 0) return nullptr;
+        const Type& vt = plan.inputs[t]->type;
+        if (vt.dim < 2 || vt.enc.empty() || vt.enc[0] != Enc::RUNEND) return nullptr;
+        if (!(v->args[1]->kind == K::Sym && v->args[1]->name == op->iters[0]->name)) return nullptr;
+        return reds[0];
+    }
+
+    void lower_seg_reduce(const N& op, const N& red, const std::string& name) {
+        const N t = op->iters[0];
+        N ev = resolve_vec(red->args[0]);
+        const int T = input_index(resolve_vec(ev->args[0]));
+        const Type& vt = plan.inputs[T]->type;
+        // The Op's space: exactly In(t, T) (VecSpace over the run values).
+        std::vector<N> cj;
+        conjuncts(resolve_bool_sym(op->args[0]), cj);
+        bool has_in = false;
+        for (auto& c0 : cj) {
+            N c = resolve_bool_sym(c0);
+            if (c->kind == K::In && is_sym_named(c->args[0], t->name) &&
+                input_index(resolve_vec(c->args[1])) == T) {
+                has_in = true;
+                continue;
+            }
+            throw Unsupported("run reduction with a predicate beyond In(t, T)");
+        }
+        if (!has_in) throw Unsupported("run reduction without In(t, T)");
+        if (vt.dtypes.size() < 2) throw Unsupported("run-end table without value columns");
+        for (size_t c = 1; c < vt.dtypes.size(); c++)
+            if (c < vt.enc.size() && vt.enc[c] == Enc::RUNEND)
+                throw Unsupported("run-end encoded value column");
+        auto sp = std::make_shared<SegReducePlan>();
+        sp->table = T;
+        sp->key_type = vt.dtypes[0];
+        // Fields: the fold over the run's rows (SubVector rows: columns 1..).
+        Lowerer Lr = sibling();
+        Lr.set_driver(T);
+        std::vector<int> path;
+        N stree = Lr.state_tree(red->type, path, sp->fields);
+        Lr.subst[red->args[2]->name] = stree;
+        Lr.subst[red->args[3]->name] = Lr.row_struct(T, 1);
+        Lr.memo.clear();
+        N accr;
+        try {
+            accr = Lr.rw(red->args[4]);
+        } catch (const IRError& e) {
+            throw Unsupported(std::string("run reduction depends on the run beyond its rows: ") +
+                              e.what());
+        }
+        std::vector<N> flat;
+        flatten(accr, flat);
+        if (flat.size() != sp->fields.size()) throw IRError("accumulator shape");
+        std::vector<N> inits;
+        flatten(fold_host(red->args[1]), inits);
+        for (size_t f = 0; f < sp->fields.size(); f++) {
+            Dec d = Lr.decompose(flat[f], "$s." + std::to_string(f));
+            Field& fl = sp->fields[f];
+            fl.init = inits.at(f);
+            fl.updated = d.upd;
+            if (d.upd) {
+                fl.op = d.op;
+                fl.pred = mk_and(Lr.valid_leaf(T, 1), d.pred);
+                fl.contrib = d.contrib;
+                if (mentions_state(fl.contrib) || mentions_state(fl.pred))
+                    throw Unsupported("state field depends on another field");
+            }
+        }
+        // Outputs per run.
+        Lowerer Lf = sibling();
+        Lf.subst[t->name] = leaf("$key", t->type);
+        std::vector<N> accl;
+        for (size_t f = 0; f < sp->fields.size(); f++)
+            accl.push_back(leaf("$acc." + std::to_string(f), sp->fields[f].type));
+        size_t k = 0;
+        std::function<N(const Type&)> nest = [&](const Type& ty) -> N {
+            if (ty.is_struct()) {
+                std::vector<N> v;
+                for (auto& d : ty.dtypes) v.push_back(nest(d));
+                return mk_new(v);
+            }
+            return accl.at(k++);
+        };
+        Lf.reduce_subst[to_text(red)] = nest(red->type);
+        sp->out_values.push_back(leaf("$key", t->type));
+        sp->out_types.push_back(t->type);
+        for (size_t i = 1; i < op->args.size(); i++) {
+            N v = Lf.rw(op->args[i]);
+            if (!v->type.is_primitive()) throw Unsupported("non-primitive run reduction output");
+            sp->out_values.push_back(v);
+            sp->out_types.push_back(v->type);
+        }
+        sp->state_type = red->type;
+        sp->vec_type = op->type;
+        sp->sym_name = name;
+        plan.seg = sp;
+        plan.driver = T;
+        plan.source = Plan::TABLE_ROWS;
+        plan.start = mk_const(types::IDX_T, 0);
+        plan.end = leaf("$len." + std::to_string(T) + ".0", types::IDX_T);
+    }
+
     Plan run() {
         plan.func_name = func->name;
         for (size_t i = 1; i < func->args.size(); i++) {
@@ -918,6 +1032,15 @@ struct Lowerer {
         };
         walk(out);
         if (comps.empty()) throw Unsupported("function output has no Reduce or Op");
+        if (comps.size() == 1 && comps[0].second->kind == K::Op) {
+            if (N red = seg_reduce_of(comps[0].second)) {
+                if (!(out->kind == K::Sym && out->name == comps[0].first))
+                    throw Unsupported("a run reduction must be the function's output");
+                lower_seg_reduce(comps[0].second, red, comps[0].first);
+                plan.output = out;
+                return plan;
+            }
+        }
         if (comps.size() == 1 && comps[0].second->kind == K::Op &&
             is_group_join(comps[0].second)) {
             if (!(out->kind == K::Sym && out->name == comps[0].first))
@@ -979,6 +1102,20 @@ std::string Plan::describe() const {
                 o << " (not updated)";
             o << "\n";
         }
+    }
+    if (seg) {
+        o << "  run reduction over the run-end column 0 of table input " << seg->table << "\n";
+        for (size_t f = 0; f < seg->fields.size(); f++) {
+            auto& fl = seg->fields[f];
+            o << "    run field " << f << " " << fl.type.str();
+            if (fl.updated)
+                o << " " << aggop_name(fl.op) << " if " << to_text(fl.pred) << " of "
+                  << to_text(fl.contrib);
+            else
+                o << " (not updated)";
+            o << "\n";
+        }
+        for (auto& v : seg->out_values) o << "    output " << to_text(v) << "\n";
     }
     if (gj) {
         o << "  group join: build table input " << gj->build << ", probe table input "

@@ -68,6 +68,7 @@ struct Probe {
 };
 
 struct GroupJoinPlan;
+struct SegReducePlan;
 
 struct Plan {
     enum Source { TABLE_ROWS, RANGE } source = TABLE_ROWS;
@@ -84,6 +85,7 @@ struct Plan {
     std::vector<std::pair<N, N>> tbl;  // the func's symbol table
     bool vectorize = false;
     std::shared_ptr<GroupJoinPlan> gj;  // set: the function is a group join
+    std::shared_ptr<SegReducePlan> seg; // set: a reduction per run of a run-end column
 
     std::string describe() const;
 };
@@ -136,6 +138,28 @@ struct HostVal {
     int64_t i = 0;
     std::vector<HostVal> fields;  // struct
     void* table = nullptr;        // vector-typed output (device table)
+};
+
+}  // namespace rf
+
+namespace rf {
+
+// An Op over the runs of the run-end encoded column 0 of a 2-dimensional
+// table whose outputs reduce each run's rows:
+//   _op([t], _in(t, T), [f(t, _red(T[t], init, acc))])
+// (test/src/test_multidim.cpp:9-30). T[t] is the sub-vector of the run's
+// rows (src/pass/loopgen.cpp:34-38); the reduction folds them in row order
+// with the SubVector body condition, the validity of column 1
+// (loopgen.cpp:221-242). One output row per run, in run order.
+struct SegReducePlan {
+    int table = -1;             // func input index of T
+    std::vector<Field> fields;  // row predicates/contributions over $col.T.C, $valid.T.C
+    Type state_type;
+    Type key_type;              // type of the run values (the iterator)
+    std::vector<N> out_values;  // $key, then the Op outputs over $key and $acc.F
+    std::vector<Type> out_types;
+    Type vec_type;
+    std::string sym_name;
 };
 
 }  // namespace rf

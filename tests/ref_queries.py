@@ -123,3 +123,33 @@ def nested_op(a_ub=10, b_ub=10):
     fn = _func("foo", os_, [])
     fn.tbl[os_] = op
     return fn
+
+
+RUNEND_TYPE = _vec_t(2, _i64_t, _i64_t, _i64_t, encodings=["runend", "flat", "flat"])
+
+
+def multidim_op():
+    """test/src/test_multidim.cpp:9-30: over the runs t of the run-end
+    encoded column x of a 2-dimensional table (x, y, d), the sum of column d
+    (`_get(v, 1)` of a row (y, d)) over the run's rows."""
+    vec = _sym("vec_in", RUNEND_TYPE)
+    t = _sym("t_sym", _i64_t)
+    red = _red(vec[t], lambda: _i64(0), lambda s, v: _add(s, _get(v, 1)))
+    rs = _sym("sum", red)
+    op = _op([t], _in(t, vec), [rs])
+    os_ = _sym("op", op)
+    fn = _func("red", os_, [vec])
+    fn.tbl[os_] = op
+    fn.tbl[rs] = red
+    return fn
+
+
+def runend_table():
+    """The reference's runend fixture (benchmark/lib/runend.py:5-7) as a
+    pyarrow record batch with x run-end encoded (int32 run ends)."""
+    import pyarrow as pa
+    import pyarrow.compute as pc
+    d = np.load(os.path.join(GOLDEN, "runend.npz"))
+    x = pc.run_end_encode(pa.array(d["x"]))
+    return pa.RecordBatch.from_arrays([x, pa.array(d["y"]), pa.array(d["d"])],
+                                      names=["x", "y", "d"])

@@ -97,3 +97,10 @@ def test_nested_two_iterator_op_compiles():
     import ref_queries as rq
     text = runtime.compile_check(rq.nested_op())
     assert "scan range rows [(const x 0), (mul" in text
+
+
+def test_run_reduction_compiles():
+    import ref_queries as rq
+    text = runtime.compile_check(rq.multidim_op())
+    assert "run reduction over the run-end column 0 of table input 0" in text
+    assert "rf_seg_fold" in text

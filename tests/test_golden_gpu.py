@@ -68,3 +68,59 @@ def test_nested_golden(device, a_ub, b_ub):
     assert np.array_equal(out.column(0), idx // max(b_ub, 1))
     assert np.array_equal(out.column(1), idx % max(b_ub, 1))
     assert np.array_equal(out.column(2), idx // max(b_ub, 1) + idx % max(b_ub, 1))
+
+
+def test_multidim_golden(device):
+    """test/src/test_multidim.cpp:32-54: one output row per run of the
+    run-end encoded x, holding the run value and the sum of d over its rows."""
+    e = rq.expected()["multidim_op_test"]
+    t = DeviceTable.from_arrow(rq.runend_table(), dim=2)
+    out = compile_op(rq.multidim_op())(t)
+    assert out.nrows == len(e["keys"])
+    assert list(out.column(0)) == e["keys"]
+    assert list(out.column(1)) == e["sums"]
+
+
+def test_run_reduction_random(device):
+    """Random runs (lengths 1..200), nulls in the summed column's
+    predicate column, int64 and float64 folds: counts and int sums exact,
+    the f64 sum bit-exact against the sequential fold of each run."""
+    import pyarrow as pa
+    import pyarrow.compute as pc
+    from reffine_amd.reffiner import (_add, _cast, _f64, _f64_t, _func, _get, _i64, _i64_t, _in,
+                                      _new, _op, _red, _sym, _vec_t)
+    rng = np.random.default_rng(12)
+    lens = rng.integers(1, 200, size=800)
+    keys = np.cumsum(rng.integers(1, 5, size=len(lens))).astype(np.int64)
+    x = np.repeat(keys, lens)
+    n = len(x)
+    y = rng.integers(-1000, 1000, size=n).astype(np.int64)
+    v = rng.random(n)
+    valid_y = rng.random(n) > 0.1
+    batch = pa.RecordBatch.from_arrays(
+        [pc.run_end_encode(pa.array(x)), pa.array(y, mask=~valid_y), pa.array(v)],
+        names=["x", "y", "v"])
+    T = _sym("T", _vec_t(2, _i64_t, _i64_t, _f64_t, encodings=["runend", "flat", "flat"]))
+    t = _sym("t", _i64_t)
+    red = _red(T[t], lambda: _new([_i64(0), _i64(0), _f64(0)]),
+               lambda s, r: _new([_add(_get(s, 0), _i64(1)), _add(_get(s, 1), _get(r, 0)),
+                                  _add(_get(s, 2), _get(r, 1))]))
+    rs = _sym("agg", red)
+    op = _op([t], _in(t, T), [_get(rs, 0), _get(rs, 1), _get(rs, 2)])
+    os_ = _sym("op", op)
+    fn = _func("runs", os_, [T])
+    fn.tbl[os_] = op
+    fn.tbl[rs] = red
+    out = compile_op(fn)(DeviceTable.from_arrow(batch, dim=2))
+    ends = np.cumsum(lens)
+    starts = ends - lens
+    assert np.array_equal(out.column(0), keys)
+    cnt, sy, sv = out.column(1), out.column(2), out.column(3)
+    for r in range(len(lens)):
+        m = valid_y[starts[r]:ends[r]]  # SubVector rows qualify by column 1's validity
+        assert cnt[r] == m.sum()
+        assert sy[r] == y[starts[r]:ends[r]][m].sum()
+        acc = 0.0
+        for val in v[starts[r]:ends[r]][m]:
+            acc = acc + float(val)
+        assert np.float64(sv[r]).view(np.int64) == np.float64(acc).view(np.int64)
