@@ -279,15 +279,26 @@ def c5_setup(n_queries: int, seed: int = 5):
         t1 = DeviceTable.from_columns([c["rowid"], c["f1"], c["f2"], c["qty"], c["price"],
                                        c["disc"], c["tax"], c["d"]], dim=1)
         tables.append((t1, t1.view(2)))
+    dim = datagen.dim_table()
+    D = DeviceTable.from_columns([dim["dk"], dim["dflag"]], dim=1)
     queries = {"q6": compile_op(workloads.q6_template(), deterministic=True),
-               "q1": compile_op(workloads.group_agg_c3(None), deterministic=True)}
+               "q1": compile_op(workloads.group_agg_c3(None), deterministic=True),
+               "join": compile_op(workloads.dim_join_template(), deterministic=True)}
     mix = workloads.c5_mix(n_queries, len(tables), seed=17)
     work = []
     nbytes = 0
     for name, b, params in mix:
         t1, t2 = tables[b]
-        work.append((queries[name], (t1 if name == "q6" else t2,) + params))
-        nbytes += (20 if name == "q6" else 38) * int(sizes[b])
+        if name == "q6":
+            work.append((queries[name], (t1,) + params))
+            nbytes += 20 * int(sizes[b])
+        elif name == "q1":
+            work.append((queries[name], (t2,) + params))
+            nbytes += 38 * int(sizes[b])
+        else:
+            work.append((queries[name], (D, t1) + params))
+            nbytes += 28 * int(sizes[b]) + 9 * len(dim["dk"])
+    tables.append((D, None))
     return sizes, tables, queries, work, nbytes
 
 
@@ -344,8 +355,10 @@ def c5_measure(n_queries: int) -> dict:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+int64",
         "data": "synthetic (seeded SplitMix64 record batches, resident in HBM)",
         "config": {"workload": f"{len(work)} queries, uniform over a Q6-shaped scan-filter-aggregate "
-                               "+ selection and a Q1-shaped 6-group aggregate, over 64 record "
-                               "batches of log-normal row counts", "rows_total": int(sizes.sum()),
+                               "+ selection, a Q1-shaped 6-group aggregate and a hash join with a "
+                               "2556-row date dimension grouped by day (group joins run beside "
+                               "the batched rounds), over 64 record batches of log-normal row "
+                               "counts", "rows_total": int(sizes.sum()),
                    "rows_median": int(np.median(sizes)), "rows_min": int(sizes.min()),
                    "rows_max": int(sizes.max()), "identical_results": ok},
         "batched": {"qps": round(qps_batched, 1), "wall_s": round(served.wall_us * 1e-6, 4),

@@ -120,3 +120,21 @@ def group_join_nested(build, probe, bf_eq=1, pq_gt=10):
                 s1 = s1 + 1
         out.append((t, s0, s1))
     return sorted(out)
+
+
+def dim_join(dim, c, f, qmax):
+    """The date-dimension join: for each day dk with dflag == f (ascending),
+    SUM(price*(1-disc)) folded from 0.0 in row order and COUNT over the rows
+    with d == dk and qty < qmax."""
+    keys = np.sort(np.asarray(dim["dk"])[np.asarray(dim["dflag"]) == f])
+    m = np.asarray(c["qty"]) < qmax
+    d = np.asarray(c["d"]).astype(np.int64)[m]
+    vals = (np.asarray(c["price"]) * (1.0 - np.asarray(c["disc"])))[m]
+    if len(keys) == 0:
+        return keys, np.zeros(0), np.zeros(0, dtype=np.int64)
+    pos = np.searchsorted(keys, d)
+    ok = (pos < len(keys)) & (keys[np.minimum(pos, len(keys) - 1)] == d)
+    sums = np.zeros(len(keys))
+    np.add.at(sums, pos[ok], vals[ok])
+    counts = np.bincount(pos[ok], minlength=len(keys)).astype(np.int64)
+    return keys, sums, counts

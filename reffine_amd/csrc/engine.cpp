@@ -1,6 +1,7 @@
 // HIP runtime layer (see engine.hpp).
 #include "engine.hpp"
 
+#include <atomic>
 #include <map>
 #include <unordered_map>
 
@@ -144,6 +145,9 @@ static bool format_type(const std::string& f, Type& t) {
 }
 
 }  // namespace rf
+
+static std::atomic<uint64_t> g_table_uid{1};
+rf_table::rf_table() : uid(g_table_uid.fetch_add(1)) {}
 
 rf::Type rf_table::type() const {
     std::vector<rf::Type> d;
@@ -1070,13 +1074,38 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
     if (!G.small) G.small = dev_alloc(256);
     uint64_t* small = (uint64_t*)G.small;  // [0..1] key range, [2] duplicate flag, [3] scratch
 
+    // The build side (steps 1-3) depends only on the build table and the
+    // build plan's scalar arguments; it stays resident between runs.
+    std::string bkey = std::to_string(args[gp.build].u.table->uid);
+    {
+        const Launch lbk = prepare_args(bq, args, n_args);
+        const KernelSource& bks = lbk.spec->ks;
+        for (size_t i = 0; i < bks.params.size(); i++)
+            if (bks.params[i].kind != ParamSlot::CONST) {
+                bkey += "|";
+                bkey += std::to_string(lbk.args.p[i].i);
+            }
+    }
+    const bool cached = (bkey == G.build_key) && G.groups > 0;
+    char* hs = nullptr;
+    int64_t K = 0, kmin = 0;
+    const int64_t* skeys = nullptr;
+    const int64_t* srows = nullptr;
+    const Spec& fs = fold_spec(q, args, n_args);
+    if (cached) {
+        K = G.groups;
+        kmin = G.kmin;
+        skeys = (const int64_t*)(G.which ? G.keys2 : bq->outs[0]);
+        srows = (const int64_t*)(G.which ? G.rows2 : bq->outs[1]);
+        hs = (char*)bq->host_stage;  // allocated by the run that built the cache
+    } else {
+    G.build_key.clear();
     // 1. Build selection: (key, build row) of the qualifying build rows.
     Launch lb = prepare(bq, args, n_args, nullptr);
-    char* hs = (char*)bq->host_stage;  // pinned, allocated by prepare
+    hs = (char*)bq->host_stage;  // pinned, allocated by prepare
     dev_fill_u64(small, ~0ull, 1);
     dev_fill_u64(small + 1, 0, 3);
     launch(bq, lb, s, false);
-    int64_t K = 0;
     uint64_t mm[2] = {~0ull, 0};
     if (!lb.empty) {
         const int64_t nb = args[gp.build].u.table->rows();
@@ -1088,10 +1117,9 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
         K = *(int64_t*)hs;
         std::memcpy(mm, hs + 8, 16);
     }
-    const Spec& fs = fold_spec(q, args, n_args);
     if (K == 0) return alloc_table(gp.out_types, 0, 1);
     if (K >= ((int64_t)1 << 31)) throw Unsupported("more than 2^31 groups");
-    const int64_t kmin = (int64_t)(mm[0] ^ 0x8000000000000000ull);
+    kmin = (int64_t)(mm[0] ^ 0x8000000000000000ull);
     const int64_t kmax = (int64_t)(mm[1] ^ 0x8000000000000000ull);
 
     // 2. Sort the build keys (ranks = positions in key order).
@@ -1111,8 +1139,9 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
     int which = 0;
     RF_HIP(aot_radix_sort(kb, pb, pw8, 1, 8, K, bits_of((uint64_t)kmax - (uint64_t)kmin),
                           (uint64_t)kmin, (uint32_t*)G.hist, &which, s));
-    const int64_t* skeys = (const int64_t*)kb[which];
-    const int64_t* srows = (const int64_t*)pb[which][0];
+    skeys = (const int64_t*)kb[which];
+    srows = (const int64_t*)pb[which][0];
+    G.which = which;
 
     // 3. Index key -> rank (capacity >= table_scale * K, a power of two).
     uint64_t cap = 16;
@@ -1137,13 +1166,19 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
     grow(G.filter, G.filter_cap, fbits / 8);
     dev_fill_u64(G.filter, 0, (int64_t)(fbits / 64));
     RF_HIP(aot_filter_build(skeys, K, G.filter, fbits - 1, s));
+    G.index_cap = cap;
+    G.filter_bits = fbits;
+    G.groups = K;
+    G.kmin = kmin;
+    G.build_key = bkey;
+    }
     RfProbeDev h{};
     h.keys = (const int64_t*)G.hkeys;
     h.rows = (const int64_t*)G.hrows;
-    h.mask = cap - 1;
+    h.mask = G.index_cap - 1;
     h.min_key_row = (kmin == RF_EMPTY_KEY) ? 0 : -1;  // the smallest key has rank 0
     h.filter = (const uint32_t*)G.filter;
-    h.fmask = fbits - 1;
+    h.fmask = G.filter_bits - 1;
     for (size_t j = 0; j < gp.probe_plan.probes.size(); j++)
         if (gp.probe_plan.probes[j].external) pq->probe_override[j] = h;
 

@@ -92,6 +92,8 @@ struct HashIndex {
 }  // namespace rf
 
 struct rf_table {
+    rf_table();
+    uint64_t uid;    // unique per table object (a freed table's address may be reused)
     int64_t dim = 1;
     std::vector<rf::Column> cols;
     std::vector<void*> owned;  // device allocations freed with the table
@@ -218,6 +220,13 @@ struct GroupJoinState {
     size_t join_cap = 0;
     void* seg = nullptr;                     // lo | hi per group
     size_t seg_cap = 0;
+    // Build side kept resident: valid for build_key (the build table's uid
+    // and the build plan's scalar arguments).
+    std::string build_key;
+    int64_t groups = 0;
+    int64_t kmin = 0;
+    int which = 0;                            // sort buffer holding the sorted keys/rows
+    uint64_t index_cap = 0, filter_bits = 0;
     ~GroupJoinState();
 };
 

@@ -16,9 +16,13 @@
 // while one round is on the GPU fill the other, which launches as soon as it
 // has work, so rounds overlap on the device (continuous batching).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <map>
+#include <mutex>
 #include <thread>
 
 #include "engine.hpp"
@@ -328,9 +332,35 @@ static double now_us(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// One request through the plain path (rf_query_run); fills its outputs and
+// status (not its completion time).
+static void run_plain_request(rf_request& rq) {
+    rf_table* sel = nullptr;
+    rq.status = 0;
+    rq.out_rows = 0;
+    try {
+        run_query(rq.q, rq.args, rq.n_args, rq.output_addr, &sel);
+        rq.out_rows = sel ? sel->rows() : 0;
+        if (sel && rq.sel_out && rq.out_rows > 0 && !sel->cols.empty()) {
+            const Column& c0 = sel->cols[0];
+            const size_t nb = std::min<size_t>((size_t)rq.out_rows * c0.type.width(),
+                                               (size_t)rq.sel_cap);
+            RF_HIP(hipMemcpy(rq.sel_out, c0.values, nb, hipMemcpyDeviceToHost));
+        }
+    } catch (const Unsupported&) {
+        rq.status = RF_ERR_UNSUPPORTED;
+    } catch (const std::exception&) {
+        rq.status = RF_ERR_INVALID;
+    }
+    if (sel) table_free(sel);
+}
+
 // Runs `n` requests (arrival times in us from the start, ascending) through
 // two runners with continuous batching; fills each request's completion
-// time, status and outputs. Returns the wall time in us.
+// time, status and outputs. Requests a runner cannot take (group joins) run
+// on a worker thread through the plain path on the library stream, so they
+// overlap with the rounds on the runners' streams. Returns the wall time in
+// us.
 double serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes, bool use_graph) {
     for (int64_t i = 1; i < n; i++)
         if (reqs[i].arrival_us < reqs[i - 1].arrival_us)
@@ -342,7 +372,42 @@ double serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes
     int64_t next = 0, finished = 0;
     int filling = 0;
     const auto t0 = std::chrono::steady_clock::now();
-    while (finished < n) {
+    // The worker for requests that run alone.
+    std::mutex wmu;
+    std::condition_variable wcv;
+    std::deque<int64_t> wq;
+    bool wstop = false;
+    std::atomic<int64_t> wdone{0};
+    std::thread worker([&] {
+        for (;;) {
+            int64_t i;
+            {
+                std::unique_lock<std::mutex> lk(wmu);
+                wcv.wait(lk, [&] { return wstop || !wq.empty(); });
+                if (wq.empty()) return;
+                i = wq.front();
+                wq.pop_front();
+            }
+            run_plain_request(reqs[i]);
+            reqs[i].done_us = now_us(t0);
+            wdone.fetch_add(1);
+        }
+    });
+    struct Joiner {
+        std::thread& t;
+        std::mutex& m;
+        std::condition_variable& cv;
+        bool& stop;
+        ~Joiner() {
+            {
+                std::lock_guard<std::mutex> lk(m);
+                stop = true;
+            }
+            cv.notify_all();
+            if (t.joinable()) t.join();
+        }
+    } joiner{worker, wmu, wcv, wstop};
+    while (finished + wdone.load() < n) {
         bool progress = false;
         // Collect finished rounds.
         for (int k = 0; k < 2; k++) {
@@ -379,6 +444,16 @@ double serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes
         if (!f->launched) {
             while (next < n && reqs[next].arrival_us <= t) {
                 rf_request& rq = reqs[next];
+                if (rq.q && rq.q->gj) {
+                    {
+                        std::lock_guard<std::mutex> lk(wmu);
+                        wq.push_back(next);
+                    }
+                    wcv.notify_one();
+                    next++;
+                    progress = true;
+                    continue;
+                }
                 int32_t unit = -1;
                 bool added = false;
                 int status = 0;
@@ -422,24 +497,7 @@ double serve_plain(rf_request* reqs, int64_t n) {
     for (int64_t i = 0; i < n; i++) {
         rf_request& rq = reqs[i];
         while (now_us(t0) < rq.arrival_us) std::this_thread::yield();
-        rf_table* sel = nullptr;
-        rq.status = 0;
-        rq.out_rows = 0;
-        try {
-            run_query(rq.q, rq.args, rq.n_args, rq.output_addr, &sel);
-            rq.out_rows = sel ? sel->rows() : 0;
-            if (sel && rq.sel_out && rq.out_rows > 0 && !sel->cols.empty()) {
-                const Column& c0 = sel->cols[0];
-                const size_t nb = std::min<size_t>((size_t)rq.out_rows * c0.type.width(),
-                                                   (size_t)rq.sel_cap);
-                RF_HIP(hipMemcpy(rq.sel_out, c0.values, nb, hipMemcpyDeviceToHost));
-            }
-        } catch (const Unsupported&) {
-            rq.status = RF_ERR_UNSUPPORTED;
-        } catch (const std::exception&) {
-            rq.status = RF_ERR_INVALID;
-        }
-        if (sel) table_free(sel);
+        run_plain_request(rq);
         rq.done_us = now_us(t0);
     }
     return now_us(t0);

@@ -104,3 +104,8 @@ def poisson_arrivals(n: int, qps: float, seed: int = 23) -> np.ndarray:
     """Arrival times in microseconds of a seeded Poisson process."""
     gaps = -np.log1p(-uniform(seed, 1, n)) / qps * 1e6
     return np.cumsum(gaps) - gaps[0]
+
+
+def dim_table(seed: int = 31, days: int = 2556) -> dict:
+    """Date dimension: dk = 0..days-1, dflag uniform in [0, 5)."""
+    return {"dk": np.arange(days, dtype=np.int64), "dflag": randint(seed, 1, days, 0, 5, np.int8)}

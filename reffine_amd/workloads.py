@@ -223,7 +223,7 @@ def q6_template():
     return fn
 
 
-def c5_mix(n_queries: int, n_batches: int, seed: int = 17, templates=("q6", "q1")):
+def c5_mix(n_queries: int, n_batches: int, seed: int = 17, templates=("q6", "q1", "join")):
     """The request mix: per query a template (uniform), a target record
     batch (uniform) and seeded predicate constants with at most 6 decimals.
     Returns a list of (template, batch, params)."""
@@ -235,13 +235,50 @@ def c5_mix(n_queries: int, n_batches: int, seed: int = 17, templates=("q6", "q1"
     lo = datagen.randint(seed, 4, n_queries, 0, 9, np.int64)
     qm = datagen.randint(seed, 5, n_queries, 10, 41, np.int64)
     dm = datagen.randint(seed, 6, n_queries, 2000, 2556, np.int64)
+    fl = datagen.randint(seed, 7, n_queries, 0, 5, np.int64)
     out = []
     for i in range(n_queries):
         name = templates[int(tpl[i])]
         if name == "q6":
             params = (int(d0[i]), round(int(lo[i]) * 0.01, 2), round(int(lo[i]) * 0.01 + 0.02, 2),
                       int(qm[i]))
-        else:
+        elif name == "q1":
             params = (int(dm[i]),)
+        else:
+            params = (int(fl[i]), int(qm[i]))
         out.append((name, int(bat[i]), params))
     return out
+
+
+# Date dimension joined with the grouped layout's d column.
+C5_DIM_TYPE = _vec_t(1, _i64_t, _i8_t)  # dk (day number), dflag
+
+
+def dim_join_template():
+    """Hash join of a record batch (grouped layout) with the date dimension
+    on d = dk, for the days with dflag == f: per day, SUM(price*(1-disc)) f64
+    and COUNT i64 over the rows with qty < qmax; one row per qualifying day,
+    in key order. The nested form of group_join_c4 with the build and probe
+    constants as scalar inputs (f i8, qmax i64)."""
+    D = _sym("D", C5_DIM_TYPE)
+    T = _sym("T", C3_DIM1_TYPE)
+    f = _sym("f", _i8_t)
+    qmax = _sym("qmax", _i64_t)
+    t = _sym("t", _i64_t)
+    i = _sym("i", _i64_t)
+    e = T[i]
+    rows = _op([i], _in(i, T) & _eq(_cast(_i64_t, e[6]), t) & _lt(e[2], qmax),
+               [_mul(e[3], _sub(_f64(1), e[4]))])
+
+    def acc(s, r):
+        return _new([_add(_get(s, 0), r[1]), _add(_get(s, 1), _i64(1))])
+
+    red = _red(rows, lambda: _new([_f64(0), _i64(0)]), acc)
+    grp = _sym("grp", red)
+    de = D[t]
+    out = _op([t], _in(t, D) & _eq(de[0], f), [_get(grp, 0), _get(grp, 1)])
+    os_ = _sym("out", out)
+    fn = _func("dim_join", os_, [D, T, f, qmax])
+    fn.tbl[grp] = red
+    fn.tbl[os_] = out
+    return fn

@@ -31,14 +31,27 @@ def batches(device):
 @pytest.fixture(scope="module")
 def queries(device):
     return {"q6": compile_op(workloads.q6_template(), deterministic=True),
-            "q1": compile_op(workloads.group_agg_c3(None), deterministic=True)}
+            "q1": compile_op(workloads.group_agg_c3(None), deterministic=True),
+            "join": compile_op(workloads.dim_join_template(), deterministic=True)}
 
 
-def _work(queries, batches, mix):
+@pytest.fixture(scope="module")
+def dim(device):
+    d = datagen.dim_table()
+    return d, DeviceTable.from_columns([d["dk"], d["dflag"]], dim=1)
+
+
+BATCHED = ("q6", "q1")  # templates a batch runner takes
+
+
+def _work(queries, batches, mix, D=None):
     work = []
     for name, b, params in mix:
         t1, t2 = batches[b]
-        work.append((queries[name], (t1 if name == "q6" else t2,) + params))
+        if name == "join":
+            work.append((queries[name], (D, t1) + params))
+        else:
+            work.append((queries[name], (t1 if name == "q6" else t2,) + params))
     return work
 
 
@@ -71,7 +84,7 @@ def _cmp(plain, batched):
 
 @pytest.mark.parametrize("graph", [False, True])
 def test_batch_matches_plain(queries, batches, graph):
-    mix = workloads.c5_mix(60, len(SIZES), seed=3)
+    mix = workloads.c5_mix(60, len(SIZES), seed=3, templates=BATCHED)
     work = _work(queries, batches, mix)
     plain = _plain(work)
     b = Batch(max_units=64, arena_bytes=1 << 30, use_graph=graph)
@@ -85,7 +98,7 @@ def test_batch_matches_plain(queries, batches, graph):
 def test_batch_admission_by_footprint(queries, batches):
     """A small arena admits a few units per round; every unit still runs and
     matches the plain result."""
-    mix = workloads.c5_mix(24, len(SIZES), seed=4)
+    mix = workloads.c5_mix(24, len(SIZES), seed=4, templates=BATCHED)
     work = _work(queries, batches, mix)
     big = max(Batch(4, 1 << 20).footprint(q, *inp) for q, inp in work)
     b = Batch(max_units=64, arena_bytes=big + (big // 2), use_graph=False)
@@ -105,9 +118,10 @@ def test_batch_same_query_many_units(queries, batches):
     _cmp(_plain(work), Batch(max_units=32, use_graph=True).run(work))
 
 
-def test_serve_matches_plain_and_reports_latency(queries, batches):
+def test_serve_matches_plain_and_reports_latency(queries, batches, dim):
     mix = workloads.c5_mix(400, len(SIZES), seed=5)
-    work = _work(queries, batches, mix)
+    assert {m[0] for m in mix} == {"q6", "q1", "join"}
+    work = _work(queries, batches, mix, dim[1])
     arrivals = datagen.poisson_arrivals(len(work), qps=20_000.0, seed=6)
     reqs = [(q, inp, t) for (q, inp), t in zip(work, arrivals)]
     caps = [int(inp[0].nrows) for _, inp, _ in reqs]
@@ -131,3 +145,26 @@ def test_serve_rejects_what_cannot_run(queries, batches):
     s = serve(reqs, max_units=8, arena_bytes=1 << 20, use_graph=False)
     assert s.status[0] == 0 and s.status[2] == 0
     assert s.status[1] != 0
+
+
+def test_group_join_in_a_batch_is_rejected(queries, batches, dim):
+    from reffine_amd import ReffineError
+    b = Batch(max_units=4)
+    with pytest.raises(ReffineError, match="group joins do not run in a batch"):
+        b.add(queries["join"], dim[1], batches[4][0], 1, 20)
+
+
+def test_dim_join_against_oracle_and_build_cache(queries, batches, dim):
+    """The join template over several batches and flags (the build side is
+    kept while the build table and flag repeat) against the oracle."""
+    from oracle import brute
+    d, D = dim
+    q = queries["join"]
+    for i, f in ((5, 1), (6, 1), (6, 3), (5, 3), (3, 3)):
+        n = SIZES[i]
+        c = datagen.group_table(n, 100 + i)
+        out = q(D, batches[i][0], f, 30)
+        keys, sums, counts = brute.dim_join(d, c, f, 30)
+        assert np.array_equal(out.column(0), keys)
+        assert np.array_equal(out.column(2), counts)
+        assert np.array_equal(out.column(1).view(np.int64), sums.view(np.int64))

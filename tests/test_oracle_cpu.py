@@ -158,3 +158,20 @@ def test_cjit_grouped_aggregate_matches_brute():
             assert g == w
         else:
             assert brute.rel_err(g, w) <= 1e-12
+
+
+def test_dim_join_oracle_against_loops():
+    c = datagen.group_table(3000, 9)
+    d = datagen.dim_table()
+    keys, sums, counts = brute.dim_join(d, c, 2, 25)
+    want = {}
+    for k in d["dk"][d["dflag"] == 2]:
+        want[int(k)] = [0.0, 0]
+    for i in range(3000):
+        k = int(c["d"][i])
+        if c["qty"][i] < 25 and k in want:
+            want[k][0] = want[k][0] + float(c["price"][i]) * (1.0 - float(c["disc"][i]))
+            want[k][1] += 1
+    assert list(keys) == sorted(want)
+    assert [want[k][1] for k in sorted(want)] == list(counts)
+    assert [want[k][0] for k in sorted(want)] == list(sums)
