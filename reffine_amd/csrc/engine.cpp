@@ -348,6 +348,7 @@ Module& jit_module(const KernelSource& ks) {
     auto m = std::make_unique<Module>();
     RF_HIP(hipModuleLoadData(&m->mod, code.data()));
     RF_HIP(hipModuleGetFunction(&m->fn, m->mod, ks.entry.c_str()));
+    if (ks.entry == "rf_gj_fold") RF_HIP(hipModuleGetFunction(&m->fold_b, m->mod, "rf_gj_fold_b"));
     if (ks.entry == "rf_q") {
         RF_HIP(hipModuleGetFunction(&m->gather, m->mod, "rf_q_gather"));
         RF_HIP(hipModuleGetFunction(&m->bscan, m->mod, "rf_qb"));
@@ -736,7 +737,7 @@ static void fill_params(const KernelSource& ks, EvalCtx& c, RfParam* out) {
     }
 }
 
-static void check_args(const Plan& plan, const rf_arg* args, int64_t n_args) {
+void check_args(const Plan& plan, const rf_arg* args, int64_t n_args) {
     if (n_args != (int64_t)plan.inputs.size())
         throw IRError("function '" + plan.func_name + "' takes " +
                       std::to_string(plan.inputs.size()) + " arguments, got " +
@@ -771,7 +772,7 @@ static Column out_column(const Type& ty, int64_t count) {
 }
 
 // A device table of `count` rows with fresh columns of `types` (one arena).
-static rf_table* alloc_table(const std::vector<Type>& types, int64_t count, int64_t dim) {
+rf_table* alloc_table(const std::vector<Type>& types, int64_t count, int64_t dim) {
     auto t = std::make_unique<rf_table>();
     t->dim = dim;
     size_t total = 0;
@@ -966,13 +967,29 @@ GroupJoinState::~GroupJoinState() {
     dev_free(small);
     dev_free(keys2);
     dev_free(rows2);
-    dev_free(hkeys);
-    dev_free(hrows);
     dev_free(hist);
-    dev_free(filter);
     dev_free(rank2);
     for (void* p : pay2) dev_free(p);
     dev_free(seg);
+}
+
+GjBuild::~GjBuild() {
+    dev_free(keys);
+    dev_free(rows);
+    dev_free(hkeys);
+    dev_free(hrows);
+    dev_free(filter);
+}
+
+RfProbeDev GjBuild::probe() const {
+    RfProbeDev h{};
+    h.keys = (const int64_t*)hkeys;
+    h.rows = (const int64_t*)hrows;
+    h.mask = index_cap - 1;
+    h.min_key_row = (kmin == RF_EMPTY_KEY) ? 0 : -1;  // the smallest key has rank 0
+    h.filter = (const uint32_t*)filter;
+    h.fmask = filter_bits - 1;
+    return h;
 }
 
 static void grow(void*& p, size_t& cap, size_t need) {
@@ -1033,7 +1050,7 @@ std::vector<KernelSource> group_join_sources(const Plan& plan,
             gen_group_fold(gp, nullable)};
 }
 
-static Spec& fold_spec(rf_query* q, const rf_arg* args, int64_t n_args) {
+Spec& gj_fold_spec(rf_query* q, const rf_arg* args, int64_t n_args) {
     std::string sig = null_signature(q->plan, args, n_args);
     auto& specs = q->gj->fold_specs;
     auto it = specs.find(sig);
@@ -1053,60 +1070,32 @@ struct GjTiming {
     hipEvent_t scan0 = nullptr, scan1 = nullptr;  // around the probe scan kernel
 };
 
-static int bits_of(uint64_t range) {
+int bits_of(uint64_t range) {
     int b = 0;
     while (b < 64 && (range >> b) != 0) b++;
     return b;
 }
 
-// Runs a group join (see GroupJoinPlan). Three host synchronisations: after
-// the build selection (group count and key range), after the probe scan
-// (joined row count) and at the end.
-static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
-                                GjTiming* tm) {
+// Builds the resident build side (two host synchronisations: the group
+// count and key range, then the duplicate-key check).
+static std::shared_ptr<GjBuild> make_build(rf_query* q, const rf_arg* args, int64_t n_args) {
     Device& d = device();
     hipStream_t s = d.stream;
     GroupJoinState& G = *q->gj;
     const GroupJoinPlan& gp = *q->plan.gj;
-    check_args(q->plan, args, n_args);
     rf_query* bq = G.build_q.get();
-    rf_query* pq = G.probe_q.get();
     if (!G.small) G.small = dev_alloc(256);
     uint64_t* small = (uint64_t*)G.small;  // [0..1] key range, [2] duplicate flag, [3] scratch
+    auto B = std::make_shared<GjBuild>();
 
-    // The build side (steps 1-3) depends only on the build table and the
-    // build plan's scalar arguments; it stays resident between runs.
-    std::string bkey = std::to_string(args[gp.build].u.table->uid);
-    {
-        const Launch lbk = prepare_args(bq, args, n_args);
-        const KernelSource& bks = lbk.spec->ks;
-        for (size_t i = 0; i < bks.params.size(); i++)
-            if (bks.params[i].kind != ParamSlot::CONST) {
-                bkey += "|";
-                bkey += std::to_string(lbk.args.p[i].i);
-            }
-    }
-    const bool cached = (bkey == G.build_key) && G.groups > 0;
-    char* hs = nullptr;
-    int64_t K = 0, kmin = 0;
-    const int64_t* skeys = nullptr;
-    const int64_t* srows = nullptr;
-    const Spec& fs = fold_spec(q, args, n_args);
-    if (cached) {
-        K = G.groups;
-        kmin = G.kmin;
-        skeys = (const int64_t*)(G.which ? G.keys2 : bq->outs[0]);
-        srows = (const int64_t*)(G.which ? G.rows2 : bq->outs[1]);
-        hs = (char*)bq->host_stage;  // allocated by the run that built the cache
-    } else {
-    G.build_key.clear();
     // 1. Build selection: (key, build row) of the qualifying build rows.
     Launch lb = prepare(bq, args, n_args, nullptr);
-    hs = (char*)bq->host_stage;  // pinned, allocated by prepare
+    char* hs = (char*)bq->host_stage;  // pinned, allocated by prepare
     dev_fill_u64(small, ~0ull, 1);
     dev_fill_u64(small + 1, 0, 3);
     launch(bq, lb, s, false);
     uint64_t mm[2] = {~0ull, 0};
+    int64_t K = 0;
     if (!lb.empty) {
         const int64_t nb = args[gp.build].u.table->rows();
         RF_HIP(aot_minmax_i64((const int64_t*)bq->outs[0], (const int64_t*)bq->out_count, nb, small,
@@ -1117,9 +1106,9 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
         K = *(int64_t*)hs;
         std::memcpy(mm, hs + 8, 16);
     }
-    if (K == 0) return alloc_table(gp.out_types, 0, 1);
+    if (K == 0) return B;
     if (K >= ((int64_t)1 << 31)) throw Unsupported("more than 2^31 groups");
-    kmin = (int64_t)(mm[0] ^ 0x8000000000000000ull);
+    const int64_t kmin = (int64_t)(mm[0] ^ 0x8000000000000000ull);
     const int64_t kmax = (int64_t)(mm[1] ^ 0x8000000000000000ull);
 
     // 2. Sort the build keys (ranks = positions in key order).
@@ -1139,62 +1128,113 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
     int which = 0;
     RF_HIP(aot_radix_sort(kb, pb, pw8, 1, 8, K, bits_of((uint64_t)kmax - (uint64_t)kmin),
                           (uint64_t)kmin, (uint32_t*)G.hist, &which, s));
-    skeys = (const int64_t*)kb[which];
-    srows = (const int64_t*)pb[which][0];
-    G.which = which;
+    B->keys = dev_alloc((size_t)K * 8);
+    B->rows = dev_alloc((size_t)K * 8);
+    RF_HIP(hipMemcpyAsync(B->keys, kb[which], (size_t)K * 8, hipMemcpyDeviceToDevice, s));
+    RF_HIP(hipMemcpyAsync(B->rows, pb[which][0], (size_t)K * 8, hipMemcpyDeviceToDevice, s));
+    const int64_t* skeys = (const int64_t*)B->keys;
 
     // 3. Index key -> rank (capacity >= table_scale * K, a power of two).
     uint64_t cap = 16;
     while (cap < (uint64_t)G.table_scale * (uint64_t)K) cap <<= 1;
-    if (cap * 8 > G.hcap || !G.hkeys) {
-        dev_free(G.hkeys);
-        dev_free(G.hrows);
-        G.hkeys = nullptr;
-        G.hrows = nullptr;
-        G.hkeys = dev_alloc(cap * 8);
-        G.hrows = dev_alloc(cap * 8);
-        G.hcap = cap * 8;
-    }
-    dev_fill_u64(G.hkeys, (uint64_t)RF_EMPTY_KEY, (int64_t)cap);
-    dev_fill_u64(G.hrows, (uint64_t)INT64_MAX, (int64_t)cap);
+    B->hkeys = dev_alloc(cap * 8);
+    B->hrows = dev_alloc(cap * 8);
+    dev_fill_u64(B->hkeys, (uint64_t)RF_EMPTY_KEY, (int64_t)cap);
+    dev_fill_u64(B->hrows, (uint64_t)INT64_MAX, (int64_t)cap);
     dev_fill_u64(small + 3, (uint64_t)INT64_MAX, 1);
-    RF_HIP(aot_index_build(skeys, nullptr, K, G.hkeys, G.hrows, cap - 1, small + 3, s));
+    RF_HIP(aot_index_build(skeys, nullptr, K, B->hkeys, B->hrows, cap - 1, small + 3, s));
     RF_HIP(aot_adjacent_dups(skeys, K, small + 2, s));
     // Bitmap filter: about 32 bits per key, at most 1 MiB (cache-resident).
     uint64_t fbits = 1024;
     while (fbits < 32 * (uint64_t)K && fbits < ((uint64_t)1 << 23)) fbits <<= 1;
-    grow(G.filter, G.filter_cap, fbits / 8);
-    dev_fill_u64(G.filter, 0, (int64_t)(fbits / 64));
-    RF_HIP(aot_filter_build(skeys, K, G.filter, fbits - 1, s));
-    G.index_cap = cap;
-    G.filter_bits = fbits;
-    G.groups = K;
-    G.kmin = kmin;
-    G.build_key = bkey;
+    B->filter = dev_alloc(fbits / 8);
+    dev_fill_u64(B->filter, 0, (int64_t)(fbits / 64));
+    RF_HIP(aot_filter_build(skeys, K, B->filter, fbits - 1, s));
+    RF_HIP(hipMemcpyAsync(hs, small + 2, 8, hipMemcpyDeviceToHost, s));
+    RF_HIP(hipStreamSynchronize(s));
+    if (*(uint64_t*)hs != 0)
+        throw IRError("group join: the build table holds a qualifying key more than once "
+                      "(each group key must occur in one build row)");
+    B->groups = K;
+    B->kmin = kmin;
+    B->index_cap = cap;
+    B->filter_bits = fbits;
+    return B;
+}
+
+std::shared_ptr<GjBuild> gj_build(rf_query* q, const rf_arg* args, int64_t n_args) {
+    GroupJoinState& G = *q->gj;
+    const GroupJoinPlan& gp = *q->plan.gj;
+    // The build side depends only on the build table and the build plan's
+    // scalar arguments.
+    std::string bkey = std::to_string(args[gp.build].u.table->uid);
+    {
+        const Launch lbk = prepare_args(G.build_q.get(), args, n_args);
+        const KernelSource& bks = lbk.spec->ks;
+        for (size_t i = 0; i < bks.params.size(); i++)
+            if (bks.params[i].kind != ParamSlot::CONST) {
+                bkey += "|";
+                bkey += std::to_string(lbk.args.p[i].i);
+            }
     }
-    RfProbeDev h{};
-    h.keys = (const int64_t*)G.hkeys;
-    h.rows = (const int64_t*)G.hrows;
-    h.mask = G.index_cap - 1;
-    h.min_key_row = (kmin == RF_EMPTY_KEY) ? 0 : -1;  // the smallest key has rank 0
-    h.filter = (const uint32_t*)G.filter;
-    h.fmask = G.filter_bits - 1;
+    auto it = G.builds.find(bkey);
+    if (it != G.builds.end()) return it->second;
+    std::shared_ptr<GjBuild> B = make_build(q, args, n_args);
+    if (G.build_order.size() >= GroupJoinState::kBuilds) {
+        G.builds.erase(G.build_order.front());
+        G.build_order.erase(G.build_order.begin());
+    }
+    G.builds.emplace(bkey, B);
+    G.build_order.push_back(bkey);
+    return B;
+}
+
+void gj_fold_args(rf_query* q, const Spec& fs, const GjBuild& b, const rf_arg* args,
+                  int64_t n_args, RfFoldArgs& A) {
+    std::memset(&A, 0, sizeof A);
+    A.groups = b.groups;
+    A.keys = (const int64_t*)b.keys;
+    A.brow = (const int64_t*)b.rows;
+    for (size_t c = 0; c < fs.ks.cols.size(); c++) {
+        const ColSlot& cs = fs.ks.cols[c];
+        const Column& col = args[cs.table].u.table->cols.at(cs.col);
+        A.col[c] = col.values;
+        A.valid[c] = (const uint16_t*)col.validity;
+    }
+    EvalCtx c{&q->plan, args, n_args, {}, {}};
+    fill_params(fs.ks, c, A.p);
+}
+
+// Runs a group join (see GroupJoinPlan): the build side from the cache (or
+// built), then the probe scan, a stable sort of the joined rows by group
+// rank, segment bounds and the per-group fold. Two host synchronisations
+// once the build side is resident: the joined row count and the end.
+static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
+                                GjTiming* tm) {
+    Device& d = device();
+    hipStream_t s = d.stream;
+    GroupJoinState& G = *q->gj;
+    const GroupJoinPlan& gp = *q->plan.gj;
+    check_args(q->plan, args, n_args);
+    rf_query* pq = G.probe_q.get();
+    const std::shared_ptr<GjBuild> B = gj_build(q, args, n_args);
+    const int64_t K = B->groups;
+    if (K == 0) return alloc_table(gp.out_types, 0, 1);
+    const Spec& fs = gj_fold_spec(q, args, n_args);
+    const RfProbeDev h = B->probe();
     for (size_t j = 0; j < gp.probe_plan.probes.size(); j++)
         if (gp.probe_plan.probes[j].external) pq->probe_override[j] = h;
 
     // 4. Probe scan: (rank, payload) of joined probe rows, in row order.
     Launch lp = prepare(pq, args, n_args, nullptr);
+    char* hs = (char*)pq->host_stage;
     if (tm) RF_HIP(hipEventRecord(tm->scan0, s));
     launch(pq, lp, s, true);
     if (tm) RF_HIP(hipEventRecord(tm->scan1, s));
     launch_gather(lp, s);
     RF_HIP(hipMemcpyAsync(hs, pq->out_count, 8, hipMemcpyDeviceToHost, s));
-    RF_HIP(hipMemcpyAsync(hs + 8, small + 2, 8, hipMemcpyDeviceToHost, s));
     RF_HIP(hipStreamSynchronize(s));
     const int64_t J = lp.empty ? 0 : *(int64_t*)hs;
-    if (*(uint64_t*)(hs + 8) != 0)
-        throw IRError("group join: the build table holds a qualifying key more than once "
-                      "(each group key must occur in one build row)");
 
     // 5. Stable sort of the joined rows by rank.
     const int np = (int)gp.pay_types.size();
@@ -1233,22 +1273,11 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
     // 7. Fold each group and evaluate the outputs.
     std::unique_ptr<rf_table> out(alloc_table(gp.out_types, K, 1));
     RfFoldArgs A;
-    std::memset(&A, 0, sizeof A);
-    A.groups = K;
-    A.keys = skeys;
-    A.brow = srows;
+    gj_fold_args(q, fs, *B, args, n_args, A);
     A.seg_lo = lo;
     A.seg_hi = hi;
     for (int j = 0; j < np; j++) A.pay[j] = pj[wj][j];
-    for (size_t c = 0; c < fs.ks.cols.size(); c++) {
-        const ColSlot& cs = fs.ks.cols[c];
-        const Column& col = args[cs.table].u.table->cols.at(cs.col);
-        A.col[c] = col.values;
-        A.valid[c] = (const uint16_t*)col.validity;
-    }
     for (size_t j = 0; j < out->cols.size(); j++) A.out[j] = out->cols[j].values;
-    EvalCtx c{&q->plan, args, n_args, {}, {}};
-    fill_params(fs.ks, c, A.p);
     void* params[] = {&A};
     const int g = (int)std::min<int64_t>((K + 255) / 256, 16384);
     RF_HIP(hipModuleLaunchKernel(fs.mod->fn, g, 1, 1, 256, 1, 1, 0, s, params, nullptr));

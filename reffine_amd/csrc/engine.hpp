@@ -112,6 +112,7 @@ struct Module {
     hipFunction_t bscan = nullptr;   // rf_qb: batched persistent scan
     hipFunction_t bdone = nullptr;   // rf_qb_done: per-unit completion
     hipFunction_t bgather = nullptr; // rf_qb_gather: per-unit selection placement
+    hipFunction_t fold_b = nullptr;  // rf_gj_fold_b: folds of several runs (fold modules)
     int max_blocks_per_cu = 1;
     int bscan_blocks_per_cu = 1;
 };
@@ -183,6 +184,10 @@ Launch prepare_args(rf_query* q, const rf_arg* args, int64_t n_args);
 void finish_output(rf_query* q, const KernelSource& ks, const rf_arg* args, int64_t n_args,
                    const char* acc, bool empty, rf_table* sel, void* output_addr,
                    rf_table** out_table);
+// Checks argument kinds and table schemas against the plan's inputs.
+void check_args(const Plan& plan, const rf_arg* args, int64_t n_args);
+// A new device table of `count` rows (owned columns, no validity).
+rf_table* alloc_table(const std::vector<Type>& types, int64_t count, int64_t dim);
 // A table over borrowed device columns (freed with table_free, which then
 // releases only the host descriptor).
 rf_table* table_view(const std::vector<Type>& types, const std::vector<void*>& values,
@@ -200,6 +205,24 @@ void c_layout(const Type& t, size_t& size, size_t& align);
 
 // Group join: sub-queries for the build and probe scans, the per-group
 // fold module and a grow-only workspace.
+// The resident build side of a group join for one build table and one set
+// of build-plan scalar arguments: the qualifying keys sorted (a key's rank
+// is its position), their build rows, the key -> rank index and its bitmap
+// filter. Shared (reference-counted) between plain runs and batch rounds.
+struct GjBuild {
+    int64_t groups = 0;     // qualifying build keys (0: the join is empty)
+    int64_t kmin = 0;
+    void* keys = nullptr;   // i64 [groups], ascending
+    void* rows = nullptr;   // i64 [groups], build row of each key
+    void* hkeys = nullptr;  // index key -> rank
+    void* hrows = nullptr;
+    uint64_t index_cap = 0;
+    void* filter = nullptr;
+    uint64_t filter_bits = 0;
+    RfProbeDev probe() const;  // the probe plan's index for the external probes
+    ~GjBuild();
+};
+
 struct GroupJoinState {
     std::unique_ptr<rf_query> build_q, probe_q;
     std::map<std::string, Spec> fold_specs;  // by nullability signature
@@ -208,11 +231,6 @@ struct GroupJoinState {
     void* keys2 = nullptr;                   // sort ping-pong buffers (build side)
     void* rows2 = nullptr;
     size_t keys_cap = 0;
-    void* hkeys = nullptr;                   // key -> rank index
-    void* hrows = nullptr;
-    size_t hcap = 0;
-    void* filter = nullptr;                  // bitmap filter of the index keys
-    size_t filter_cap = 0;
     void* hist = nullptr;
     size_t hist_cap = 0;
     void* rank2 = nullptr;                   // sort ping-pong buffers (joined rows)
@@ -220,15 +238,26 @@ struct GroupJoinState {
     size_t join_cap = 0;
     void* seg = nullptr;                     // lo | hi per group
     size_t seg_cap = 0;
-    // Build side kept resident: valid for build_key (the build table's uid
-    // and the build plan's scalar arguments).
-    std::string build_key;
-    int64_t groups = 0;
-    int64_t kmin = 0;
-    int which = 0;                            // sort buffer holding the sorted keys/rows
-    uint64_t index_cap = 0, filter_bits = 0;
+    // Build sides kept resident, by build key (the build table's uid and the
+    // build plan's scalar arguments); the oldest is dropped beyond kBuilds.
+    static constexpr size_t kBuilds = 16;
+    std::map<std::string, std::shared_ptr<GjBuild>> builds;
+    std::vector<std::string> build_order;
     ~GroupJoinState();
 };
+
+// The build side of group join `q` for these arguments: from the cache, or
+// built on the library stream (synchronous). Caller holds q->mu.
+std::shared_ptr<GjBuild> gj_build(rf_query* q, const rf_arg* args, int64_t n_args);
+// The fold module of group join `q` for the arguments' nullability.
+Spec& gj_fold_spec(rf_query* q, const rf_arg* args, int64_t n_args);
+// Fold arguments of one group join run, apart from the joined rows (pay,
+// seg_lo, seg_hi) and the output columns.
+void gj_fold_args(rf_query* q, const Spec& fs, const GjBuild& b, const rf_arg* args,
+                  int64_t n_args, RfFoldArgs& A);
+int bits_of(uint64_t range);
+hipError_t aot_gj_concat(const RfCatSeg* segs, int32_t nseg, int64_t max_rows,
+                         const RfCatArgs& A, hipStream_t s);
 
 // Batched execution (engine_batch.cpp).
 rf_batch* batch_create(int cap, int64_t arena_bytes, bool use_graph);

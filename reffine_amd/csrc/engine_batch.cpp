@@ -4,25 +4,30 @@
 // arguments (one record batch) each -- into a round and runs the round with
 // one persistent launch per distinct query kernel over the list of its
 // units' wave-tiles (rf_qb, see kernels/rf_device.h), one completion launch
-// per kernel and one placement launch for selections. Everything a launch reads lives in buffers that are
-// fixed per runner (a control region rewritten each round, a result region,
-// a bump arena for the units' partials and selection buffers), so a round can
-// be captured into a hipGraph once per set of kernels and replayed.
-// Admission is by device-memory footprint: a unit that does not fit the
-// arena (or a full unit table) is refused with RF_ERR_BUSY until the round
-// is recycled.
+// per kernel and one placement launch for selections. Everything a launch
+// reads lives in buffers that are fixed per runner (a control region
+// rewritten each round, a result region, a bump arena for the units'
+// partials and selection buffers), so a round can be captured into a
+// hipGraph once per set of kernels and replayed. Admission is by
+// device-memory footprint: a unit that does not fit the arena (or a full
+// unit table) is refused with RF_ERR_BUSY until the round is recycled.
+//
+// A group join unit runs its probe scan as an ordinary unit of the round
+// (against the resident build side of its build table, shared with plain
+// runs); once the round's scans are done, one post-phase per group join
+// query concatenates all its units' joined rows (ranks offset by each
+// unit's first group), sorts them once with a stable radix sort, finds the
+// segment bounds and folds every unit's groups in one launch. A stable sort
+// keeps each unit's rows in row order within a group, so every group folds
+// in the same order as the unit's plain run.
 //
 // The serving loop keeps two runners on two streams: requests that arrive
 // while one round is on the GPU fill the other, which launches as soon as it
 // has work, so rounds overlap on the device (continuous batching).
 #include <algorithm>
-#include <atomic>
 #include <chrono>
-#include <condition_variable>
 #include <cstring>
-#include <deque>
 #include <map>
-#include <mutex>
 #include <thread>
 
 #include "engine.hpp"
@@ -34,13 +39,31 @@ static constexpr size_t kSlot = 512;     // result bytes per unit (Acc, count at
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// Grow-only device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    void* need(size_t bytes) {
+        if (bytes > cap || !p) {
+            dev_free(p);
+            p = nullptr;
+            cap = 0;
+            p = dev_alloc(std::max<size_t>(bytes, 256));
+            cap = bytes;
+        }
+        return p;
+    }
+    ~DevBuf() { dev_free(p); }
+};
+
 }  // namespace rf
 
 struct rf_batch {
     int cap = 0;
     bool use_graph = false;
     hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+    hipEvent_t done = nullptr;       // the round's scans (and results copied back)
+    hipEvent_t post_done = nullptr;  // the group join post-phase
     // Device regions and their pinned host mirrors.
     char* arena = nullptr;
     size_t arena_bytes = 0, arena_used = 0;
@@ -55,22 +78,39 @@ struct rf_batch {
         rf::Launch l;
         std::vector<void*> outs;
         size_t footprint = 0;
+        // Group join: the resident build side, the fold module and the
+        // output columns (one row per qualifying build key).
+        std::shared_ptr<rf::GjBuild> build;
+        const rf::Spec* fold = nullptr;
+        std::vector<void*> gj_outs;
+        bool no_scan = false;  // a group join without qualifying build keys
     };
     std::vector<Unit> units;
+    int64_t join_groups = 0;  // groups of the round's group join units
     bool launched = false;
+    bool post_pending = false;   // scans launched, post-phase not yet
+    bool post_launched = false;
     std::map<std::vector<rf::Module*>, hipGraphExec_t> graphs;
+    // Post-phase buffers (grow-only): concatenated ranks and payloads
+    // (ping-pong), sort histogram, segment bounds, run descriptors.
+    rf::DevBuf rank[2], pay[2][8], hist, seg, d_post;
+    char* h_post = nullptr;
+    size_t h_post_cap = 0;
     ~rf_batch();
 };
 
 rf_batch::~rf_batch() {
     if (launched && done) (void)hipEventSynchronize(done);
+    if (post_launched && post_done) (void)hipEventSynchronize(post_done);
     for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
     rf::dev_free(arena);
     rf::dev_free(d_ctl);
     rf::dev_free(d_res);
     if (h_ctl) (void)hipHostFree(h_ctl);
     if (h_res) (void)hipHostFree(h_res);
+    if (h_post) (void)hipHostFree(h_post);
     if (done) (void)hipEventDestroy(done);
+    if (post_done) (void)hipEventDestroy(post_done);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -85,6 +125,7 @@ rf_batch* batch_create(int cap, int64_t arena_bytes, bool use_graph) {
     b->use_graph = use_graph;
     RF_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     RF_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+    RF_HIP(hipEventCreateWithFlags(&b->post_done, hipEventDisableTiming));
     b->arena_bytes = (size_t)arena_bytes;
     b->arena = (char*)dev_alloc(b->arena_bytes);
     b->off_start = align256(sizeof(RfBatchHdr) * kMaxGroups);
@@ -106,33 +147,71 @@ static void* arena_take(rf_batch* b, size_t bytes, size_t& used) {
     return p;
 }
 
-// Device bytes a unit of `q` over `args` takes from a runner's arena.
-int64_t batch_footprint(rf_query* q, const rf_arg* args, int64_t n_args) {
-    if (q->gj) throw Unsupported("group joins do not run in a batch");
-    Launch l = prepare_args(q, args, n_args);
-    const KernelSource& ks = l.spec->ks;
-    const int64_t tile = (int64_t)ks.tune.sub * 64 * ks.tune.items;
-    const int64_t nt = std::max<int64_t>(l.args.ntiles, 1);
+// What a unit launches: the query's own scan, or for a group join the probe
+// scan against the resident build side (built now if not cached).
+static void plan_unit(rf_batch::Unit& u, rf_query* q, const rf_arg* args, int64_t n_args) {
+    u.q = q;
+    u.args.assign(args, args + n_args);
+    if (q->plan.seg) throw Unsupported("run reductions do not run in a batch");
+    if (!q->gj) {
+        u.l = prepare_args(q, u.args.data(), n_args);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(q->mu);
+    check_args(q->plan, u.args.data(), n_args);
+    u.build = gj_build(q, u.args.data(), n_args);
+    u.fold = &gj_fold_spec(q, u.args.data(), n_args);
+    if (u.build->groups == 0) {
+        u.no_scan = true;
+        return;
+    }
+    rf_query* pq = q->gj->probe_q.get();
+    u.l = prepare_args(pq, u.args.data(), n_args);
+    const RfProbeDev h = u.build->probe();
+    const auto& probes = q->plan.gj->probe_plan.probes;
+    for (size_t j = 0; j < probes.size(); j++)
+        if (probes[j].external) u.l.args.probe[j] = h;
+}
+
+static size_t unit_footprint(const rf_batch::Unit& u) {
     size_t f = 0;
+    if (u.build) {
+        if (u.no_scan) return 0;
+        for (auto& t : u.q->plan.gj->out_types) f += align256((size_t)u.build->groups * t.width());
+    }
+    const KernelSource& ks = u.l.spec->ks;
+    const int64_t tile = (int64_t)ks.tune.sub * 64 * ks.tune.items;
+    const int64_t nt = std::max<int64_t>(u.l.args.ntiles, 1);
     if (ks.acc_bytes) f += align256((size_t)nt * ks.acc_bytes);
     if (ks.has_sel) {
-        const int64_t rows = std::max<int64_t>(l.end - l.lo, 1);
+        const int64_t rows = std::max<int64_t>(u.l.end - u.l.lo, 1);
         f += 2 * align256((size_t)nt * 8);
         for (auto& t : ks.stage_types) f += align256((size_t)(nt * tile) * t.width());
         for (auto& t : ks.out_types) f += align256((size_t)rows * t.width());
     }
-    return (int64_t)f;
+    return f;
+}
+
+// Device bytes a unit of `q` over `args` takes from a runner's arena.
+int64_t batch_footprint(rf_query* q, const rf_arg* args, int64_t n_args) {
+    rf_batch::Unit u;
+    plan_unit(u, q, args, n_args);
+    return (int64_t)unit_footprint(u);
 }
 
 // Adds a unit; returns false (nothing changed) when the round is full.
 bool batch_add(rf_batch* b, rf_query* q, const rf_arg* args, int64_t n_args, int32_t* unit) {
     if (b->launched) throw IRError("the round is launched: wait for it and reset the batch");
-    if (q->gj) throw Unsupported("group joins do not run in a batch");
     if ((int)b->units.size() >= b->cap) return false;
     rf_batch::Unit u;
-    u.q = q;
-    u.args.assign(args, args + n_args);
-    u.l = prepare_args(q, u.args.data(), n_args);
+    plan_unit(u, q, args, n_args);
+    const int idx = (int)b->units.size();
+    if (u.no_scan) {
+        b->units.push_back(std::move(u));
+        *unit = idx;
+        return true;
+    }
+    if (u.build && b->join_groups + u.build->groups >= ((int64_t)1 << 31)) return false;
     const KernelSource& ks = u.l.spec->ks;
     if ((size_t)ks.acc_bytes > kSlot - 8)
         throw Unsupported("reduction state too large for a batch result slot");
@@ -140,7 +219,6 @@ bool batch_add(rf_batch* b, rf_query* q, const rf_arg* args, int64_t n_args, int
     const int64_t tile = (int64_t)ks.tune.sub * 64 * ks.tune.items;
     const int64_t nt = std::max<int64_t>(A.ntiles, 1);
     size_t used = b->arena_used;
-    const int idx = (int)b->units.size();
     char* res = b->d_res + kSlot * (size_t)idx;
     A.result = res;
     A.out_count = (int64_t*)(res + kSlot - 8);
@@ -164,6 +242,14 @@ bool batch_add(rf_batch* b, rf_query* q, const rf_arg* args, int64_t n_args, int
             u.outs.push_back(A.out[j]);
         }
     }
+    if (u.build) {
+        for (auto& t : q->plan.gj->out_types) {
+            void* p = arena_take(b, (size_t)u.build->groups * t.width(), used);
+            if (!p) return false;
+            u.gj_outs.push_back(p);
+        }
+        b->join_groups += u.build->groups;
+    }
     u.footprint = used - b->arena_used;
     b->arena_used = used;
     b->units.push_back(std::move(u));
@@ -184,6 +270,7 @@ static std::vector<Group> groups_of(rf_batch* b) {
     std::map<Module*, Group> m;
     for (size_t i = 0; i < b->units.size(); i++) {
         const auto& u = b->units[i];
+        if (u.no_scan) continue;
         Module* mod = u.l.spec->mod;
         auto it = m.find(mod);
         if (it == m.end())
@@ -233,12 +320,15 @@ static void enqueue_round(rf_batch* b, const std::vector<Group>& gs, bool full_c
 
 void batch_launch(rf_batch* b) {
     if (b->launched) throw IRError("the round is already launched");
-    if (b->units.empty()) {
+    b->post_pending = b->post_launched = false;
+    for (auto& u : b->units)
+        if (u.build && !u.no_scan) b->post_pending = true;
+    std::vector<Group> gs = groups_of(b);
+    if (gs.empty()) {
         b->launched = true;
         RF_HIP(hipEventRecord(b->done, b->stream));
         return;
     }
-    std::vector<Group> gs = groups_of(b);
     // Host mirror of the control region: per group its header and item
     // starts; the units' arguments in group order.
     RfBatchHdr* hdr = (RfBatchHdr*)b->h_ctl;
@@ -286,9 +376,156 @@ void batch_launch(rf_batch* b) {
     b->launched = true;
 }
 
+// The group join post-phase of a finished round (joined row counts are in
+// the host copy of the result slots). Per fold module (one group join query
+// and nullability): concatenate the units' joined rows, one stable radix
+// sort by (unit's first group + rank), segment bounds, one fold launch over
+// every unit's groups.
+static void launch_post(rf_batch* b) {
+    hipStream_t s = b->stream;
+    std::map<const Spec*, std::vector<int>> by_fold;
+    for (size_t i = 0; i < b->units.size(); i++) {
+        const auto& u = b->units[i];
+        if (u.build && !u.no_scan) by_fold[u.fold].push_back((int)i);
+    }
+    // Descriptors per fold module: segs[n] | folds[n] | gstart[n + 1].
+    size_t bytes = 0;
+    for (auto& kv : by_fold) {
+        const size_t n = kv.second.size();
+        bytes += align256(n * sizeof(RfCatSeg)) + align256(n * sizeof(RfFoldArgs)) +
+                 align256((n + 1) * 8);
+    }
+    if (bytes > b->h_post_cap) {
+        if (b->h_post) RF_HIP(hipHostFree(b->h_post));
+        b->h_post = nullptr;
+        b->h_post_cap = 0;
+        RF_HIP(hipHostMalloc((void**)&b->h_post, bytes, hipHostMallocDefault));
+        b->h_post_cap = bytes;
+    }
+    char* dpost = (char*)b->d_post.need(bytes);
+    // Joined rows and groups per fold module; the shared sort and segment
+    // buffers are sized for the largest before anything is enqueued (a
+    // buffer reallocated between two modules' launches would be freed while
+    // the first module's launches still use it).
+    std::vector<int64_t> rows_of, groups_of_fold;
+    int64_t max_rows = 0, max_groups = 0;
+    int max_np = 0;
+    for (auto& kv : by_fold) {
+        int64_t jt = 0, gt = 0;
+        for (int ui : kv.second) {
+            const rf_batch::Unit& u = b->units[ui];
+            jt += u.l.empty ? 0 : *(const int64_t*)(b->h_res + kSlot * (size_t)ui + kSlot - 8);
+            gt += u.build->groups;
+        }
+        rows_of.push_back(jt);
+        groups_of_fold.push_back(gt);
+        max_rows = std::max(max_rows, jt);
+        max_groups = std::max(max_groups, gt);
+        max_np = std::max(max_np, (int)b->units[kv.second[0]].q->plan.gj->pay_types.size());
+    }
+    if (max_rows > 0) {
+        for (int x = 0; x < 2; x++) {
+            b->rank[x].need((size_t)max_rows * 4);
+            for (int j = 0; j < max_np; j++) b->pay[x][j].need((size_t)max_rows * 8);
+        }
+        b->hist.need((size_t)aot_sort_hist_entries(max_rows) * 4);
+    }
+    b->seg.need((size_t)max_groups * 16);
+    size_t off = 0, fi = 0;
+    for (auto& kv : by_fold) {
+        const std::vector<int>& us = kv.second;
+        const rf_batch::Unit& u0 = b->units[us[0]];
+        const GroupJoinPlan& gp = *u0.q->plan.gj;
+        const int np = (int)gp.pay_types.size();
+        const size_t n = us.size();
+        const size_t o_seg = off, o_fold = o_seg + align256(n * sizeof(RfCatSeg)),
+                     o_gs = o_fold + align256(n * sizeof(RfFoldArgs));
+        off = o_gs + align256((n + 1) * 8);
+        RfCatSeg* segs = (RfCatSeg*)(b->h_post + o_seg);
+        RfFoldArgs* folds = (RfFoldArgs*)(b->h_post + o_fold);
+        int64_t* gstart = (int64_t*)(b->h_post + o_gs);
+        const int64_t jt = rows_of[fi], gt = groups_of_fold[fi];
+        fi++;
+        int64_t dst = 0, g0 = 0, maxj = 0;
+        for (size_t k = 0; k < n; k++) {
+            const rf_batch::Unit& u = b->units[us[k]];
+            const int64_t J =
+                u.l.empty ? 0 : *(const int64_t*)(b->h_res + kSlot * (size_t)us[k] + kSlot - 8);
+            std::memset(&segs[k], 0, sizeof(RfCatSeg));
+            segs[k].rank = (const uint32_t*)u.outs[0];
+            for (int j = 0; j < np; j++) segs[k].pay[j] = u.outs[1 + j];
+            segs[k].n = J;
+            segs[k].dst = dst;
+            segs[k].base = (uint32_t)g0;
+            gstart[k] = g0;
+            dst += J;
+            g0 += u.build->groups;
+            maxj = std::max(maxj, J);
+        }
+        gstart[n] = g0;
+        void* kb[2] = {b->rank[0].p, b->rank[1].p};
+        void* pb[2][8] = {};
+        int32_t pw[8] = {};
+        for (int j = 0; j < np; j++) {
+            pw[j] = gp.pay_types[j].width();
+            pb[0][j] = b->pay[0][j].p;
+            pb[1][j] = b->pay[1][j].p;
+        }
+        int64_t* lo = (int64_t*)b->seg.p;
+        int64_t* hi = lo + gt;
+        int w = 0;
+        if (jt > 0) {
+            RF_HIP(hipMemcpyAsync(dpost + o_seg, b->h_post + o_seg, n * sizeof(RfCatSeg),
+                                  hipMemcpyHostToDevice, s));
+            RfCatArgs CA;
+            std::memset(&CA, 0, sizeof CA);
+            CA.rank = (uint32_t*)kb[0];
+            for (int j = 0; j < np; j++) {
+                CA.pay[j] = pb[0][j];
+                CA.width[j] = pw[j];
+            }
+            CA.np = np;
+            RF_HIP(aot_gj_concat((const RfCatSeg*)(dpost + o_seg), (int32_t)n, maxj, CA, s));
+            if (jt > 1)
+                RF_HIP(aot_radix_sort(kb, pb, pw, np, 4, jt, bits_of((uint64_t)(gt - 1)), 0,
+                                      (uint32_t*)b->hist.p, &w, s));
+        }
+        RF_HIP(aot_fill_u64(lo, 0, 2 * gt, s));
+        if (jt > 0) RF_HIP(aot_seg_bounds((const uint32_t*)kb[w], jt, lo, hi, s));
+        for (size_t k = 0; k < n; k++) {
+            const rf_batch::Unit& u = b->units[us[k]];
+            RfFoldArgs& A = folds[k];
+            gj_fold_args(u.q, *u.fold, *u.build, u.args.data(), (int64_t)u.args.size(), A);
+            A.seg_lo = lo + gstart[k];
+            A.seg_hi = hi + gstart[k];
+            for (int j = 0; j < np; j++) A.pay[j] = pb[w][j];
+            for (size_t j = 0; j < u.gj_outs.size(); j++) A.out[j] = u.gj_outs[j];
+        }
+        RF_HIP(hipMemcpyAsync(dpost + o_fold, b->h_post + o_fold, o_gs + (n + 1) * 8 - o_fold,
+                              hipMemcpyHostToDevice, s));
+        RfFoldBatch FB;
+        FB.list = (const RfFoldArgs*)(dpost + o_fold);
+        FB.gstart = (const int64_t*)(dpost + o_gs);
+        FB.n = (int64_t)n;
+        void* params[] = {&FB};
+        const int g = (int)std::min<int64_t>((gt + 255) / 256, 16384);
+        RF_HIP(hipModuleLaunchKernel(kv.first->mod->fold_b, g, 1, 1, 256, 1, 1, 0, s, params,
+                                     nullptr));
+    }
+    RF_HIP(hipEventRecord(b->post_done, s));
+    b->post_pending = false;
+    b->post_launched = true;
+}
+
 bool batch_poll(rf_batch* b) {
     if (!b->launched) return false;
-    hipError_t e = hipEventQuery(b->done);
+    if (b->post_pending) {
+        hipError_t e = hipEventQuery(b->done);
+        if (e == hipErrorNotReady) return false;
+        RF_HIP(e);
+        launch_post(b);
+    }
+    hipError_t e = hipEventQuery(b->post_launched ? b->post_done : b->done);
     if (e == hipErrorNotReady) return false;
     RF_HIP(e);
     return true;
@@ -297,14 +534,27 @@ bool batch_poll(rf_batch* b) {
 void batch_wait(rf_batch* b) {
     if (!b->launched) throw IRError("no round launched");
     RF_HIP(hipEventSynchronize(b->done));
+    if (b->post_pending) launch_post(b);
+    if (b->post_launched) RF_HIP(hipEventSynchronize(b->post_done));
 }
 
-// Output of a unit after the round finished. A selection result is a view
-// of the runner's arena: valid until the runner is reset.
+// Output of a unit after the round finished. A selection result (and a
+// group join's table) is a view of the runner's arena: valid until the
+// runner is reset.
 void batch_result(rf_batch* b, int32_t unit, void* output_addr, rf_table** out_table) {
     if (!b->launched) throw IRError("no round launched");
+    if (b->post_pending) throw IRError("the round has not finished");
     if (unit < 0 || unit >= (int32_t)b->units.size()) throw IRError("no such unit");
     auto& u = b->units[unit];
+    if (u.build) {
+        if (!out_table) throw IRError("an Op output needs out_table");
+        const GroupJoinPlan& gp = *u.q->plan.gj;
+        rf_table* t = u.no_scan ? alloc_table(gp.out_types, 0, 1)
+                                : table_view(gp.out_types, u.gj_outs, u.build->groups, 1);
+        *out_table = t;
+        if (output_addr) *(void**)output_addr = t;
+        return;
+    }
     const KernelSource& ks = u.l.spec->ks;
     const char* res = b->h_res + kSlot * (size_t)unit;
     rf_table* sel = nullptr;
@@ -318,10 +568,15 @@ void batch_result(rf_batch* b, int32_t unit, void* output_addr, rf_table** out_t
 }
 
 void batch_reset(rf_batch* b) {
-    if (b->launched) RF_HIP(hipEventSynchronize(b->done));
+    if (b->launched) {
+        RF_HIP(hipEventSynchronize(b->done));
+        if (b->post_launched) RF_HIP(hipEventSynchronize(b->post_done));
+    }
     b->launched = false;
+    b->post_pending = b->post_launched = false;
     b->units.clear();
     b->arena_used = 0;
+    b->join_groups = 0;
 }
 
 int batch_units(const rf_batch* b) { return (int)b->units.size(); }
@@ -332,6 +587,18 @@ static double now_us(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Copies the first column of a request's selection (or group join table)
+// to its host buffer, up to its capacity.
+static void copy_first_column(rf_request& rq, const rf_table* sel) {
+    rq.out_rows = sel ? sel->rows() : 0;
+    if (sel && rq.sel_out && rq.out_rows > 0 && !sel->cols.empty()) {
+        const Column& c0 = sel->cols[0];
+        const size_t nb =
+            std::min<size_t>((size_t)rq.out_rows * c0.type.width(), (size_t)rq.sel_cap);
+        RF_HIP(hipMemcpy(rq.sel_out, c0.values, nb, hipMemcpyDeviceToHost));
+    }
+}
+
 // One request through the plain path (rf_query_run); fills its outputs and
 // status (not its completion time).
 static void run_plain_request(rf_request& rq) {
@@ -340,13 +607,7 @@ static void run_plain_request(rf_request& rq) {
     rq.out_rows = 0;
     try {
         run_query(rq.q, rq.args, rq.n_args, rq.output_addr, &sel);
-        rq.out_rows = sel ? sel->rows() : 0;
-        if (sel && rq.sel_out && rq.out_rows > 0 && !sel->cols.empty()) {
-            const Column& c0 = sel->cols[0];
-            const size_t nb = std::min<size_t>((size_t)rq.out_rows * c0.type.width(),
-                                               (size_t)rq.sel_cap);
-            RF_HIP(hipMemcpy(rq.sel_out, c0.values, nb, hipMemcpyDeviceToHost));
-        }
+        copy_first_column(rq, sel);
     } catch (const Unsupported&) {
         rq.status = RF_ERR_UNSUPPORTED;
     } catch (const std::exception&) {
@@ -357,10 +618,7 @@ static void run_plain_request(rf_request& rq) {
 
 // Runs `n` requests (arrival times in us from the start, ascending) through
 // two runners with continuous batching; fills each request's completion
-// time, status and outputs. Requests a runner cannot take (group joins) run
-// on a worker thread through the plain path on the library stream, so they
-// overlap with the rounds on the runners' streams. Returns the wall time in
-// us.
+// time, status and outputs. Returns the wall time in us.
 double serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes, bool use_graph) {
     for (int64_t i = 1; i < n; i++)
         if (reqs[i].arrival_us < reqs[i - 1].arrival_us)
@@ -372,42 +630,7 @@ double serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes
     int64_t next = 0, finished = 0;
     int filling = 0;
     const auto t0 = std::chrono::steady_clock::now();
-    // The worker for requests that run alone.
-    std::mutex wmu;
-    std::condition_variable wcv;
-    std::deque<int64_t> wq;
-    bool wstop = false;
-    std::atomic<int64_t> wdone{0};
-    std::thread worker([&] {
-        for (;;) {
-            int64_t i;
-            {
-                std::unique_lock<std::mutex> lk(wmu);
-                wcv.wait(lk, [&] { return wstop || !wq.empty(); });
-                if (wq.empty()) return;
-                i = wq.front();
-                wq.pop_front();
-            }
-            run_plain_request(reqs[i]);
-            reqs[i].done_us = now_us(t0);
-            wdone.fetch_add(1);
-        }
-    });
-    struct Joiner {
-        std::thread& t;
-        std::mutex& m;
-        std::condition_variable& cv;
-        bool& stop;
-        ~Joiner() {
-            {
-                std::lock_guard<std::mutex> lk(m);
-                stop = true;
-            }
-            cv.notify_all();
-            if (t.joinable()) t.join();
-        }
-    } joiner{worker, wmu, wcv, wstop};
-    while (finished + wdone.load() < n) {
+    while (finished < n) {
         bool progress = false;
         // Collect finished rounds.
         for (int k = 0; k < 2; k++) {
@@ -420,13 +643,7 @@ double serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes
                 rq.status = 0;
                 try {
                     batch_result(b, (int32_t)i, rq.output_addr, &sel);
-                    rq.out_rows = sel ? sel->rows() : 0;
-                    if (sel && rq.sel_out && rq.out_rows > 0 && !sel->cols.empty()) {
-                        const Column& c0 = sel->cols[0];
-                        const size_t nb = std::min<size_t>((size_t)rq.out_rows * c0.type.width(),
-                                                           (size_t)rq.sel_cap);
-                        RF_HIP(hipMemcpy(rq.sel_out, c0.values, nb, hipMemcpyDeviceToHost));
-                    }
+                    copy_first_column(rq, sel);
                 } catch (...) {
                     rq.status = RF_ERR_INVALID;
                 }
@@ -444,16 +661,6 @@ double serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes
         if (!f->launched) {
             while (next < n && reqs[next].arrival_us <= t) {
                 rf_request& rq = reqs[next];
-                if (rq.q && rq.q->gj) {
-                    {
-                        std::lock_guard<std::mutex> lk(wmu);
-                        wq.push_back(next);
-                    }
-                    wcv.notify_one();
-                    next++;
-                    progress = true;
-                    continue;
-                }
                 int32_t unit = -1;
                 bool added = false;
                 int status = 0;

@@ -704,9 +704,8 @@ KernelSource gen_group_fold(const GroupJoinPlan& gj, const std::vector<std::vect
     Gen g(gj.build_plan, nullable, ks);
     std::ostringstream o;
     o << "// Generated group fold and outputs for group join '" << gj.sym_name << "'.\n";
-    o << "extern \"C\" __global__ __launch_bounds__(256) void rf_gj_fold(RfFoldArgs L) {\n";
-    o << "    for (int64_t grp = blockIdx.x * (int64_t)256 + threadIdx.x; grp < L.groups;\n"
-         "         grp += (int64_t)gridDim.x * 256) {\n";
+    o << "__device__ __forceinline__ void rf_gj_fold_one(const RfFoldArgs& L, const int64_t grp) {\n";
+    o << "    {\n";
     o << "        const int64_t i0 = L.seg_lo[grp], i1 = L.seg_hi[grp];\n";
     // Field state, initialised from the reduction's init (host values).
     std::ostringstream init;
@@ -758,6 +757,26 @@ KernelSource gen_group_fold(const GroupJoinPlan& gj, const std::vector<std::vect
         ks.out_types.push_back(gj.out_types[j]);
     }
     o << "    }\n}\n";
+    // One run: a thread per group.
+    o << "extern \"C\" __global__ __launch_bounds__(256) void rf_gj_fold(RfFoldArgs L) {\n"
+         "    for (int64_t grp = blockIdx.x * (int64_t)256 + threadIdx.x; grp < L.groups;\n"
+         "         grp += (int64_t)gridDim.x * 256)\n"
+         "        rf_gj_fold_one(L, grp);\n"
+         "}\n";
+    // Several runs: a thread per group of the launch; its run by binary
+    // search over the cumulative group counts.
+    o << "extern \"C\" __global__ __launch_bounds__(256) void rf_gj_fold_b(RfFoldBatch B) {\n"
+         "    const int64_t total = B.gstart[B.n];\n"
+         "    for (int64_t g = blockIdx.x * (int64_t)256 + threadIdx.x; g < total;\n"
+         "         g += (int64_t)gridDim.x * 256) {\n"
+         "        int64_t lo = 0, hi = B.n - 1;\n"
+         "        while (lo < hi) {\n"
+         "            const int64_t mid = (lo + hi + 1) >> 1;\n"
+         "            if (B.gstart[mid] <= g) lo = mid; else hi = mid - 1;\n"
+         "        }\n"
+         "        rf_gj_fold_one(B.list[lo], g - B.gstart[lo]);\n"
+         "    }\n"
+         "}\n";
     ks.code = o.str();
     return ks;
 }

@@ -77,6 +77,34 @@ struct RfFoldArgs {
     union RfParam p[RF_MAXPARAMS];
 };
 
+// Folds of several group join runs in one launch (rf_gj_fold_b): run u
+// owns groups [gstart[u], gstart[u+1]) of the launch, its own arguments in
+// list[u] (group indexes in list[u] are local to the run).
+struct RfFoldBatch {
+    const struct RfFoldArgs* list;
+    const int64_t* gstart;  // n + 1 cumulative group counts
+    int64_t n;
+};
+
+// Concatenation of several runs' joined rows (rank + payload columns) into
+// one array; run u's ranks are offset by its first group so one stable sort
+// orders every run's groups at once.
+struct RfCatSeg {
+    const uint32_t* rank;
+    const void* pay[8];
+    int64_t n;        // joined rows of the run
+    int64_t dst;      // first row in the concatenation
+    uint32_t base;    // first group of the run
+    uint32_t pad;
+};
+
+struct RfCatArgs {
+    uint32_t* rank;   // concatenated ranks
+    void* pay[8];     // concatenated payload columns
+    int32_t width[8]; // payload widths in bytes (1, 2, 4, 8)
+    int32_t np;
+};
+
 // Batched execution: a persistent launch over (work unit, wave-tile) items.
 struct RfBatchHdr {
     int64_t nunits;      // work units of this launch

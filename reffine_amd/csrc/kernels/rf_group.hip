@@ -265,6 +265,33 @@ extern "C" __global__ void rf_seg_bounds(const uint32_t* __restrict__ rank, int6
     }
 }
 
+// Concatenation of several runs' joined rows: blockIdx.y strides over the
+// runs, blockIdx.x over a run's rows. Ranks get the run's first group added.
+template <typename T>
+__device__ __forceinline__ void rf_cat_copy(void* dst, const void* src, int64_t d, int64_t i) {
+    ((T*)dst)[d] = ((const T*)src)[i];
+}
+
+extern "C" __global__ __launch_bounds__(256) void rf_gj_concat(const RfCatSeg* __restrict__ segs,
+                                                               int64_t nseg, RfCatArgs A) {
+    for (int64_t u = blockIdx.y; u < nseg; u += gridDim.y) {
+        const RfCatSeg& S = segs[u];
+        for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < S.n;
+             i += (int64_t)gridDim.x * 256) {
+            const int64_t d = S.dst + i;
+            A.rank[d] = S.rank[i] + S.base;
+            for (int j = 0; j < A.np; j++) {
+                switch (A.width[j]) {
+                    case 8: rf_cat_copy<uint64_t>(A.pay[j], S.pay[j], d, i); break;
+                    case 4: rf_cat_copy<uint32_t>(A.pay[j], S.pay[j], d, i); break;
+                    case 2: rf_cat_copy<uint16_t>(A.pay[j], S.pay[j], d, i); break;
+                    default: rf_cat_copy<uint8_t>(A.pay[j], S.pay[j], d, i); break;
+                }
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------- host launchers
 
 namespace rf {
@@ -360,6 +387,16 @@ hipError_t aot_seg_bounds(const uint32_t* rank, int64_t n, int64_t* lo, int64_t*
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(rf_seg_bounds, dim3(grid_cap(n, 256, 16384)), dim3(256), 0, s, rank, n,
                        lo, hi);
+    return hipGetLastError();
+}
+
+// segs: device array of nseg runs; max_rows: the largest run.
+hipError_t aot_gj_concat(const RfCatSeg* segs, int32_t nseg, int64_t max_rows,
+                         const RfCatArgs& A, hipStream_t s) {
+    if (nseg <= 0 || max_rows <= 0) return hipSuccess;
+    const int gx = grid_cap(max_rows, 1024, 256);
+    const int gy = nseg < 65535 ? nseg : 65535;
+    hipLaunchKernelGGL(rf_gj_concat, dim3(gx, gy), dim3(256), 0, s, segs, (int64_t)nseg, A);
     return hipGetLastError();
 }
 

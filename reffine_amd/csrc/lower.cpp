@@ -838,7 +838,7 @@ struct Lowerer {
             }
             gj->fields.push_back(g);
         }
-        if (po.values.size() > 16) throw Unsupported("more than 15 group join payload columns");
+        if (po.values.size() > 9) throw Unsupported("more than 8 group join payload columns");
         for (size_t j = 1; j < po.types.size(); j++) gj->pay_types.push_back(po.types[j]);
         std::vector<Type> vts;
         for (auto& ty : po.types) vts.push_back(ty);

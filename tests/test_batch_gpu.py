@@ -147,11 +147,19 @@ def test_serve_rejects_what_cannot_run(queries, batches):
     assert s.status[1] != 0
 
 
-def test_group_join_in_a_batch_is_rejected(queries, batches, dim):
-    from reffine_amd import ReffineError
-    b = Batch(max_units=4)
-    with pytest.raises(ReffineError, match="group joins do not run in a batch"):
-        b.add(queries["join"], dim[1], batches[4][0], 1, 20)
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_group_joins_match_plain(queries, batches, dim, graph):
+    """Group join units (several build flags, a flag no day has, empty and
+    large probe batches) in rounds with the other templates: each unit's
+    table equals its plain run bit for bit."""
+    mix = workloads.c5_mix(48, len(SIZES), seed=8)
+    work = _work(queries, batches, mix, dim[1])
+    work += [(queries["join"], (dim[1], batches[b][0], f, 30)) for b, f in
+             [(0, 1), (6, 2), (5, 7), (2, 0), (6, 7)]]
+    plain = _plain(work)
+    b = Batch(max_units=24, arena_bytes=1 << 30, use_graph=graph)
+    for rep in range(2):
+        _cmp(plain, b.run(work))
 
 
 def test_dim_join_against_oracle_and_build_cache(queries, batches, dim):
