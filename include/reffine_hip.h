@@ -222,8 +222,11 @@ int rf_query_free(rf_query* q);
 /* A batch runner: a round of work units (one compiled query over its
  * arguments each) run together -- per distinct query kernel, one persistent
  * launch over the list of all its units' (unit, wave-tile) items (each wave
- * a contiguous range), then per-unit completion and selection placement. Each unit's result equals the
- * deterministic-mode run of the same query alone (RF_COMPILE_DETERMINISTIC
+ * a contiguous range), then per-unit completion and selection placement.
+ * A group join unit runs its probe scan in the round; after the round one
+ * sort and one fold launch serve all its group join units. Each unit's
+ * result equals the deterministic-mode run of the same query alone
+ * (RF_COMPILE_DETERMINISTIC
  * queries give identical bits in both). `max_units` units per round;
  * `arena_bytes` of device memory for the units' partials and selection
  * buffers (admission by footprint: rf_batch_add returns RF_ERR_BUSY when a
@@ -242,11 +245,12 @@ int rf_batch_add(rf_batch* b, rf_query* q, const rf_arg* args, int64_t n_args,
                  int32_t* unit);
 /* Launches the round asynchronously on the runner's own stream. */
 int rf_batch_launch(rf_batch* b);
-/* *done = 1 once the launched round has finished (no wait). */
+/* *done = 1 once the launched round has finished (no wait; launches the
+ * group join sort and fold once the round's scans are done). */
 int rf_batch_poll(rf_batch* b, int32_t* done);
 int rf_batch_wait(rf_batch* b);
 /* Output of a unit of a finished round, as rf_query_run gives it; a
- * selection table is a view of the runner's arena, valid until
+ * selection (or group join) table is a view of the runner's arena, valid until
  * rf_batch_reset (free it with rf_table_free). */
 int rf_batch_result(rf_batch* b, int32_t unit, void* output_addr,
                     rf_table** out_table);
