@@ -897,6 +897,7 @@ Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/
             q->stage_rows_cap = stage_total;
         }
         for (size_t j = 0; j < q->outs.size(); j++) A.out[j] = q->outs[j];
+        A.out_cap = (int64_t)q->out_rows_cap;
         for (size_t j = 0; j < q->stages.size(); j++) A.stage[j] = q->stages[j];
         A.stage_rows = stage_rows;
         A.nwaves = nw;
@@ -1340,6 +1341,19 @@ void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_add
     Device& d = device();
     Launch l = prepare(q, args, n_args, nullptr);
     const KernelSource& ks = l.spec->ks;
+    const int64_t dim = q->plan.ops.empty() ? 1 : q->plan.ops[0].n_iters;
+    // A selection is placed straight into the returned table, sized from
+    // the previous run's row count (twice it, plus 4096 rows); rows that do
+    // not fit are placed again into a table of the exact size. One host
+    // synchronisation in the common case.
+    std::unique_ptr<rf_table> sel_t;
+    int64_t sel_cap = 0;
+    if (ks.has_sel && !l.empty) {
+        sel_cap = std::min<int64_t>(std::max<int64_t>(l.end - l.lo, 1), 2 * q->sel_hint + 4096);
+        sel_t.reset(alloc_table(ks.out_types, sel_cap, dim));
+        for (size_t j = 0; j < ks.out_types.size(); j++) l.args.out[j] = sel_t->cols[j].values;
+        l.args.out_cap = sel_cap;
+    }
     launch(q, l, d.stream, false);
     char* stage = (char*)q->host_stage;
     if (!l.empty) {
@@ -1351,15 +1365,19 @@ void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_add
     RF_HIP(hipStreamSynchronize(d.stream));
     rf_table* sel = nullptr;
     if (!q->plan.ops.empty()) {
-        int64_t count = l.empty ? 0 : *(int64_t*)(stage + 4096);
-        std::unique_ptr<rf_table> t(alloc_table(ks.out_types, count, q->plan.ops[0].n_iters));
-        for (size_t j = 0; j < ks.out_types.size(); j++)
-            if (count > 0)
-                RF_HIP(hipMemcpyAsync(t->cols[j].values, q->outs[j],
-                                      (size_t)count * ks.out_types[j].width(),
-                                      hipMemcpyDeviceToDevice, d.stream));
-        RF_HIP(hipStreamSynchronize(d.stream));
-        sel = t.release();
+        const int64_t count = l.empty ? 0 : *(int64_t*)(stage + 4096);
+        if (!sel_t || count > sel_cap) {
+            sel_t.reset(alloc_table(ks.out_types, count, dim));
+            if (count > 0) {
+                for (size_t j = 0; j < ks.out_types.size(); j++) l.args.out[j] = sel_t->cols[j].values;
+                l.args.out_cap = count;
+                launch_gather(l, d.stream);
+                RF_HIP(hipStreamSynchronize(d.stream));
+            }
+        }
+        for (auto& c : sel_t->cols) c.length = c.logical_length = count;
+        q->sel_hint = count;
+        sel = sel_t.release();
     }
     finish_output(q, ks, args, n_args, stage, l.empty, sel, output_addr, out_table);
 }

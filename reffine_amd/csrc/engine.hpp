@@ -152,6 +152,7 @@ struct rf_query {
     std::vector<void*> stages;    // per-block staging of the same columns
     size_t out_rows_cap = 0;
     size_t stage_rows_cap = 0;
+    int64_t sel_hint = 0;         // selection rows of the last plain run
     void* host_stage = nullptr;  // pinned: result Acc + count
     std::mutex mu;
     ~rf_query();

@@ -232,6 +232,7 @@ bool batch_add(rf_batch* b, rf_query* q, const rf_arg* args, int64_t n_args, int
         if (!A.wave_cnt || !A.wave_dst) return false;
         A.stage_rows = tile;
         A.nwaves = A.ntiles;
+        A.out_cap = rows;
         for (size_t j = 0; j < ks.stage_types.size(); j++) {
             A.stage[j] = arena_take(b, (size_t)(nt * tile) * ks.stage_types[j].width(), used);
             if (!A.stage[j]) return false;

@@ -59,6 +59,7 @@ struct RfScanArgs {
     void* stage[RF_MAXOUT];
     void* out[RF_MAXOUT];
     int64_t* out_count;
+    int64_t out_cap;      // rows the out columns hold (placement stops there)
     union RfParam p[RF_MAXPARAMS];
 };
 

@@ -424,12 +424,16 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
     }
 }
 
-// Moves each wave's staged rows to its output offset.
+// Moves each wave's staged rows to its output offset. Rows at offsets
+// beyond out_cap are not placed (the host sized the output from an
+// estimate and places them again into a larger table).
 template <class Q>
 __device__ __forceinline__ void rf_gather(const RfScanArgs& L) {
     for (int64_t w = blockIdx.x; w < L.nwaves; w += gridDim.x) {
         const int64_t src = w * L.stage_rows, dst = rf_g(L.wave_dst)[w], cnt = rf_g(L.wave_cnt)[w];
-        for (int64_t i = threadIdx.x; i < cnt; i += blockDim.x) Q::move(L, src + i, dst + i);
+        const int64_t room = L.out_cap - dst;
+        const int64_t n = cnt < room ? cnt : room;
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) Q::move(L, src + i, dst + i);
     }
 }
 
