@@ -200,6 +200,12 @@ int rf_query_tuning(const rf_query* q, int32_t* tuning);
 /* Human-readable lowering summary (kernel kind, columns, fields). */
 int rf_query_describe(const rf_query* q, char* buf, size_t cap);
 
+/* Drops the resident build sides a group join query keeps (per build table
+ * and build scalars); the next run builds again. A no-op for other queries.
+ * No reference counterpart (the reference evaluates the nested loop from
+ * scratch on every call, src/pass/loopgen.cpp:113-269). */
+int rf_query_clear_cache(rf_query* q);
+
 /* Replaces the call of the generated entry point `void name(OutT*, ...)`:
  * runs the query on device tables and synchronises. `output_addr` receives
  * the reduction struct (host memory, rf_query_output_info bytes); for Op

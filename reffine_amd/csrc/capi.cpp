@@ -264,6 +264,13 @@ int rf_query_tuning(const rf_query* q, int32_t* tuning) {
     });
 }
 
+int rf_query_clear_cache(rf_query* q) {
+    return guard([&] {
+        need(q, "NULL argument");
+        rf::gj_clear_cache(q);
+    });
+}
+
 int rf_query_describe(const rf_query* q, char* buf, size_t cap) {
     return guard([&] {
         need(q && buf && cap, "NULL argument");

@@ -220,12 +220,19 @@ struct GjBuild {
     uint64_t index_cap = 0;
     void* filter = nullptr;
     uint64_t filter_bits = 0;
+    void* dense = nullptr;  // direct map key - kmin -> rank (dense key ranges;
+    uint64_t dense_len = 0; // then no index or filter)
     RfProbeDev probe() const;  // the probe plan's index for the external probes
     ~GjBuild();
 };
 
 struct GroupJoinState {
     std::unique_ptr<rf_query> build_q, probe_q;
+    // The probe scan against a hash index (sparse build keys) prefers
+    // another launch shape than against a direct map: compiled on first use
+    // unless the caller fixed the tuning.
+    std::unique_ptr<rf_query> probe_q_hash;
+    bool probe_tuned = false;
     std::map<std::string, Spec> fold_specs;  // by nullability signature
     int table_scale = 2;                     // index capacity >= scale * groups
     void* small = nullptr;                   // key range, duplicate flag, scratch
@@ -250,6 +257,10 @@ struct GroupJoinState {
 // The build side of group join `q` for these arguments: from the cache, or
 // built on the library stream (synchronous). Caller holds q->mu.
 std::shared_ptr<GjBuild> gj_build(rf_query* q, const rf_arg* args, int64_t n_args);
+// The probe sub-query for a build side (direct map or hash index).
+rf_query* gj_probe_query(rf_query* q, const GjBuild& b);
+// Drops the cached build sides (in-flight batch rounds keep theirs).
+void gj_clear_cache(rf_query* q);
 // The fold module of group join `q` for the arguments' nullability.
 Spec& gj_fold_spec(rf_query* q, const rf_arg* args, int64_t n_args);
 // Fold arguments of one group join run, apart from the joined rows (pay,
@@ -257,6 +268,8 @@ Spec& gj_fold_spec(rf_query* q, const rf_arg* args, int64_t n_args);
 void gj_fold_args(rf_query* q, const Spec& fs, const GjBuild& b, const rf_arg* args,
                   int64_t n_args, RfFoldArgs& A);
 int bits_of(uint64_t range);
+hipError_t aot_dense_build(const int64_t* keys, int64_t n, int64_t base, void* dense,
+                           hipStream_t s);
 hipError_t aot_gj_concat(const RfCatSeg* segs, int32_t nseg, int64_t max_rows,
                          const RfCatArgs& A, hipStream_t s);
 

@@ -165,7 +165,7 @@ static void plan_unit(rf_batch::Unit& u, rf_query* q, const rf_arg* args, int64_
         u.no_scan = true;
         return;
     }
-    rf_query* pq = q->gj->probe_q.get();
+    rf_query* pq = gj_probe_query(q, *u.build);
     u.l = prepare_args(pq, u.args.data(), n_args);
     const RfProbeDev h = u.build->probe();
     const auto& probes = q->plan.gj->probe_plan.probes;

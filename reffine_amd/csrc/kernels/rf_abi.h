@@ -38,6 +38,12 @@ struct RfProbeDev {
     int64_t min_key_row;   // row of key == RF_EMPTY_KEY, -1 if none
     const uint32_t* filter;  // optional bitmap of (hash >> 32) & fmask over the keys
     uint64_t fmask;          // filter bits - 1 (a power of two)
+    // Optional direct map for a dense key range: row of key k is
+    // dense[k - dense_base] for k in [dense_base, dense_base + dense_len)
+    // (-1: none); keys outside the range have no row. Replaces the slots.
+    const int32_t* dense;
+    int64_t dense_base;
+    uint64_t dense_len;
 };
 
 struct RfScanArgs {

@@ -107,6 +107,146 @@ __device__ __forceinline__ void rf_sort_scatter(const RfSortPass& P) {
     }
 }
 
+template <typename T>
+__device__ __forceinline__ void rf_copy_elem(void* dst, int64_t di, const void* src, int64_t si) {
+    ((T*)dst)[di] = ((const T*)src)[si];
+}
+
+__device__ __forceinline__ void rf_copy_w(int w, void* dst, int64_t di, const void* src, int64_t si) {
+    switch (w) {
+        case 1: rf_copy_elem<uint8_t>(dst, di, src, si); break;
+        case 2: rf_copy_elem<uint16_t>(dst, di, src, si); break;
+        case 4: rf_copy_elem<uint32_t>(dst, di, src, si); break;
+        default: rf_copy_elem<uint64_t>(dst, di, src, si); break;
+    }
+}
+
+// One payload column of a wave's elements into its LDS plane at their local
+// positions (all of this lane's loads in flight together).
+template <typename T>
+__device__ __forceinline__ void rf_place_plane(void* plane, const uint32_t (&lpos)[RF_SORT_ITEMS],
+                                               const void* src, int64_t wbase, int64_t e1) {
+    const int lane = threadIdx.x & 63;
+    T v[RF_SORT_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RF_SORT_ITEMS; r++) {
+        const int64_t e = wbase + r * 64 + lane;
+        v[r] = e < e1 ? ((const T*)src)[e] : (T)0;
+    }
+#pragma unroll
+    for (int r = 0; r < RF_SORT_ITEMS; r++)
+        if (wbase + r * 64 + lane < e1) ((T*)plane)[lpos[r]] = v[r];
+}
+
+// The same stable scatter, staged through LDS. Each wave ranks its own
+// contiguous quarter of the tile against per-wave digit counters (ballots
+// find the lanes with the same digit; the wave's lowest such lane advances
+// the counter, read by all of them first -- a wave's LDS operations are in
+// order, so no barrier is needed inside the loop); a prefix over the waves
+// per digit and the tile's digit starts then give every element its place
+// in the tile in digit order. The tile is placed in LDS and written out in
+// LDS order, so consecutive threads write consecutive positions of a
+// digit's output range. The tile's count of digit d is the difference of
+// consecutive entries of the scanned (digit-major) histogram. Dynamic LDS:
+// TILE keys, then one plane of TILE values per payload.
+template <typename K>
+__device__ __forceinline__ void rf_sort_scatter_lds(const RfSortPass& P) {
+    constexpr int WAVES = RF_SORT_BLOCK / 64;
+    constexpr int64_t WTILE = RF_SORT_TILE / WAVES;  // elements per wave
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t goff[256], lstart[256];
+    __shared__ uint32_t cnt[WAVES][256];
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const int64_t e0 = (int64_t)blockIdx.x * RF_SORT_TILE;
+    const int64_t e1 = e0 + RF_SORT_TILE < P.n ? e0 + RF_SORT_TILE : P.n;
+    const int64_t m = 256 * P.nblocks;
+    const int64_t idx = (int64_t)t * P.nblocks + blockIdx.x;
+    const uint32_t g = P.hist[idx];
+    const uint32_t c = (idx + 1 < m ? P.hist[idx + 1] : (uint32_t)P.n) - g;
+    goff[t] = g;
+    lstart[t] = c;
+#pragma unroll
+    for (int v = 0; v < WAVES; v++) cnt[v][t] = 0;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t x = t >= off ? lstart[t - off] : 0u;
+        __syncthreads();
+        lstart[t] += x;
+        __syncthreads();
+    }
+    lstart[t] -= c;  // exclusive
+    // This wave's elements: [wbase, wbase + WTILE), RF_SORT_ITEMS rounds of 64.
+    const int64_t wbase = e0 + (int64_t)w * WTILE;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    K kr[RF_SORT_ITEMS];
+    uint32_t lpos[RF_SORT_ITEMS];
+    uint32_t dig[RF_SORT_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RF_SORT_ITEMS; r++) {
+        const int64_t e = wbase + r * 64 + lane;
+        kr[r] = e < e1 ? ((const K*)P.kin)[e] : (K)0;
+    }
+#pragma unroll
+    for (int r = 0; r < RF_SORT_ITEMS; r++) {
+        const bool ok = wbase + r * 64 + lane < e1;
+        const uint32_t d = ok ? (uint32_t)(((uint64_t)kr[r] - P.bias) >> P.shift) & 255u : 0u;
+        uint64_t peers = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool on = (d >> b) & 1u;
+            const uint64_t mk = __ballot(on);
+            peers &= on ? mk : ~mk;
+        }
+        const uint32_t below = (uint32_t)__popcll(peers & lt);
+        const uint32_t before = cnt[w][d];
+        if (ok && below == 0) cnt[w][d] = before + (uint32_t)__popcll(peers);
+        lpos[r] = before + below;  // rank among the wave's elements of digit d
+        dig[r] = d;
+    }
+    __syncthreads();
+    {
+        // Per digit: exclusive prefix over the waves, from the tile start.
+        uint32_t run = lstart[t];
+#pragma unroll
+        for (int v = 0; v < WAVES; v++) {
+            const uint32_t x = cnt[v][t];
+            cnt[v][t] = run;
+            run += x;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RF_SORT_ITEMS; r++) lpos[r] += cnt[w][dig[r]];
+    K* sk = (K*)lds;
+    unsigned char* planes = lds + (size_t)RF_SORT_TILE * sizeof(K);
+#pragma unroll
+    for (int r = 0; r < RF_SORT_ITEMS; r++)
+        if (wbase + r * 64 + lane < e1) sk[lpos[r]] = kr[r];
+    size_t poff = 0;
+    for (int j = 0; j < P.np; j++) {
+        void* plane = planes + poff;
+        switch (P.pw[j]) {
+            case 1: rf_place_plane<uint8_t>(plane, lpos, P.pin[j], wbase, e1); break;
+            case 2: rf_place_plane<uint16_t>(plane, lpos, P.pin[j], wbase, e1); break;
+            case 4: rf_place_plane<uint32_t>(plane, lpos, P.pin[j], wbase, e1); break;
+            default: rf_place_plane<uint64_t>(plane, lpos, P.pin[j], wbase, e1); break;
+        }
+        poff += (size_t)RF_SORT_TILE * P.pw[j];
+    }
+    __syncthreads();
+    const int tn = (int)(e1 - e0);
+    for (int i = t; i < tn; i += RF_SORT_BLOCK) {
+        const K key = sk[i];
+        const uint32_t d = (uint32_t)(((uint64_t)key - P.bias) >> P.shift) & 255u;
+        const int64_t pos = (int64_t)goff[d] + (i - (int64_t)lstart[d]);
+        ((K*)P.kout)[pos] = key;
+        size_t off = 0;
+        for (int j = 0; j < P.np; j++) {
+            rf_copy_w(P.pw[j], P.pout[j], pos, planes + off, i);
+            off += (size_t)RF_SORT_TILE * P.pw[j];
+        }
+    }
+}
 extern "C" __global__ __launch_bounds__(RF_SORT_BLOCK) void rf_sort_hist_u32(RfSortPass P) {
     rf_sort_hist<uint32_t>(P);
 }
@@ -118,6 +258,12 @@ extern "C" __global__ __launch_bounds__(RF_SORT_BLOCK) void rf_sort_scatter_u32(
 }
 extern "C" __global__ __launch_bounds__(RF_SORT_BLOCK) void rf_sort_scatter_u64(RfSortPass P) {
     rf_sort_scatter<uint64_t>(P);
+}
+extern "C" __global__ __launch_bounds__(RF_SORT_BLOCK) void rf_sort_scatter_lds_u32(RfSortPass P) {
+    rf_sort_scatter_lds<uint32_t>(P);
+}
+extern "C" __global__ __launch_bounds__(RF_SORT_BLOCK) void rf_sort_scatter_lds_u64(RfSortPass P) {
+    rf_sort_scatter_lds<uint64_t>(P);
 }
 
 // Exclusive scan of m histogram entries in three launches: per-chunk sums,
@@ -265,6 +411,14 @@ extern "C" __global__ void rf_seg_bounds(const uint32_t* __restrict__ rank, int6
     }
 }
 
+// Direct map of sorted distinct keys: dense[keys[i] - base] = i (the rank).
+extern "C" __global__ void rf_dense_build(const int64_t* __restrict__ keys, int64_t n, int64_t base,
+                                          int32_t* dense) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dense[(uint64_t)keys[i] - (uint64_t)base] = (int32_t)i;
+}
+
 // Concatenation of several runs' joined rows: blockIdx.y strides over the
 // runs, blockIdx.x over a run's rows. Ranks get the run's first group added.
 template <typename T>
@@ -327,7 +481,14 @@ hipError_t aot_radix_sort(void* const keys[2], void* const pay[2][RF_SORT_MAXPAY
     P.hist = hist;
     P.np = np;
     P.bias = bias;
-    for (int j = 0; j < np; j++) P.pw[j] = pw[j];
+    size_t elem = (size_t)key_bytes;
+    for (int j = 0; j < np; j++) {
+        P.pw[j] = pw[j];
+        elem += (size_t)pw[j];
+    }
+    // Staged scatter when a tile's elements fit 48 KiB of LDS.
+    const size_t lds_bytes = (size_t)RF_SORT_TILE * elem;
+    const bool staged = lds_bytes <= 49152;
     int cur = 0;
     for (int shift = 0; shift < bits; shift += 8) {
         P.kin = keys[cur];
@@ -348,7 +509,11 @@ hipError_t aot_radix_sort(void* const keys[2], void* const pay[2][RF_SORT_MAXPAY
         hipLaunchKernelGGL(rf_scan_chunks, dim3((unsigned)nch), dim3(256), 0, s, hist, m, part);
         hipLaunchKernelGGL(rf_sort_scan, dim3(1), dim3(1024), 0, s, part, nch);
         hipLaunchKernelGGL(rf_scan_apply, dim3((unsigned)nch), dim3(256), 0, s, hist, m, part);
-        if (key_bytes == 4)
+        if (staged && key_bytes == 4)
+            hipLaunchKernelGGL(rf_sort_scatter_lds_u32, g, b, lds_bytes, s, P);
+        else if (staged)
+            hipLaunchKernelGGL(rf_sort_scatter_lds_u64, g, b, lds_bytes, s, P);
+        else if (key_bytes == 4)
             hipLaunchKernelGGL(rf_sort_scatter_u32, g, b, 0, s, P);
         else
             hipLaunchKernelGGL(rf_sort_scatter_u64, g, b, 0, s, P);
@@ -387,6 +552,14 @@ hipError_t aot_seg_bounds(const uint32_t* rank, int64_t n, int64_t* lo, int64_t*
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(rf_seg_bounds, dim3(grid_cap(n, 256, 16384)), dim3(256), 0, s, rank, n,
                        lo, hi);
+    return hipGetLastError();
+}
+
+hipError_t aot_dense_build(const int64_t* keys, int64_t n, int64_t base, void* dense,
+                           hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rf_dense_build, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, n,
+                       base, (int32_t*)dense);
     return hipGetLastError();
 }
 

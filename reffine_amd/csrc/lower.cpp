@@ -844,6 +844,7 @@ struct Lowerer {
         for (auto& ty : po.types) vts.push_back(ty);
         po.vec_type = Type(BT::VECTOR, vts, 1);
         Lp.plan.ops = {po};
+        Lp.plan.early_sel = true;  // joined rows are a large share of the probe rows
         Lp.plan.func_name = func->name + "_probe";
         Lp.plan.output_type = po.vec_type;
         gj->probe = Lp.driver;

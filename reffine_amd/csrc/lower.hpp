@@ -84,6 +84,10 @@ struct Plan {
     Type output_type;
     std::vector<std::pair<N, N>> tbl;  // the func's symbol table
     bool vectorize = false;
+    // Selection outputs are evaluated in the scan from the loaded rows and
+    // staged (early materialisation) instead of read again by row index in
+    // the placement kernel; for selections that keep a large share of rows.
+    bool early_sel = false;
     std::shared_ptr<GroupJoinPlan> gj;  // set: the function is a group join
     std::shared_ptr<SegReducePlan> seg; // set: a reduction per run of a run-end column
 

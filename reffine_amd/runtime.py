@@ -113,6 +113,7 @@ _SIGS = [
                                             ctypes.POINTER(ctypes.c_int32)]),
     ("rf_query_describe", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]),
     ("rf_query_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    ("rf_query_clear_cache", ctypes.c_int, [ctypes.c_void_p]),
     ("rf_query_run", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RfArg), ctypes.c_int64,
                                     ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     ("rf_query_time", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RfArg), ctypes.c_int64,
@@ -472,6 +473,11 @@ class Query:
         t = (ctypes.c_int32 * 6)()
         _ck(lib().rf_query_tuning(self.handle, t))
         return tuple(t)
+
+    def clear_cache(self):
+        """Drops a group join's resident build sides (the next run builds
+        again)."""
+        _ck(lib().rf_query_clear_cache(self.handle))
 
     def describe(self) -> str:
         buf = ctypes.create_string_buffer(1 << 20)

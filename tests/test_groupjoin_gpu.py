@@ -132,3 +132,21 @@ def test_group_join_timing(device):
     q = compile_op(workloads.group_join_c4())
     total, scan = q.time_kernels(B, P, iters=3)
     assert total > 0 and 0 < scan <= total
+
+
+def test_group_join_build_cache_is_keyed_and_clearable(device):
+    """The resident build side is reused only for the same build table: runs
+    alternating between two build tables, and runs after the cache is
+    cleared, all give the oracle's answer."""
+    b1, p = datagen.join_tables(5000, 200_000, seed=41)
+    b2, _ = datagen.join_tables(5000, 10, seed=42)
+    B1, P = _upload(b1, p)
+    B2 = DeviceTable.from_columns([b2["bk"], b2["bd"], b2["bf"]], dim=1)
+    q = compile_op(workloads.group_join_c4())
+    for B, b in ((B1, b1), (B2, b2), (B1, b1)):
+        _check(q(B, P), b, p)
+    q.clear_cache()
+    _check(q(B2, P), b2, p)
+    q.clear_cache()
+    q.clear_cache()
+    _check(q(B1, P), b1, p)
