@@ -980,13 +980,15 @@ GjBuild::~GjBuild() {
     dev_free(hkeys);
     dev_free(hrows);
     dev_free(filter);
-    dev_free(dense);
+    dev_free(dense_bits);
+    dev_free(dense_cum);
 }
 
 RfProbeDev GjBuild::probe() const {
     RfProbeDev h{};
-    if (dense) {
-        h.dense = (const int32_t*)dense;
+    if (dense_bits) {
+        h.dense_bits = (const uint64_t*)dense_bits;
+        h.dense_cum = (const uint32_t*)dense_cum;
         h.dense_base = kmin;
         h.dense_len = dense_len;
         return h;
@@ -1143,17 +1145,20 @@ static std::shared_ptr<GjBuild> make_build(rf_query* q, const rf_arg* args, int6
     RF_HIP(hipMemcpyAsync(B->rows, pb[which][0], (size_t)K * 8, hipMemcpyDeviceToDevice, s));
     const int64_t* skeys = (const int64_t*)B->keys;
 
-    // 3. Key -> rank: a direct map when the key range is dense (at most 16
-    // slots per key plus 64K, so the map stays about as small as an index),
-    // else the open-addressing index and its bitmap filter.
+    // 3. Key -> rank: a rank map (a bit per key of the range and a count per
+    // 64 bits: 1.5 bits per slot) when the key range is dense -- at most 64
+    // slots per key plus 2^20, so it stays smaller than an index and fits
+    // the caches -- else the open-addressing index and its bitmap filter.
     const uint64_t range = (uint64_t)kmax - (uint64_t)kmin + 1;
     RF_HIP(aot_adjacent_dups(skeys, K, small + 2, s));
-    if (kmin != RF_EMPTY_KEY && range != 0 && range <= 16 * (uint64_t)K + 65536 &&
-        range < ((uint64_t)1 << 31)) {
-        B->dense = dev_alloc(((range + 1) / 2) * 8);
+    if (kmin != RF_EMPTY_KEY && range != 0 && range <= 64 * (uint64_t)K + ((uint64_t)1 << 20) &&
+        range < ((uint64_t)1 << 40)) {
+        const int64_t words = (int64_t)((range + 63) / 64);
+        B->dense_bits = dev_alloc((size_t)words * 8);
+        B->dense_cum = dev_alloc((size_t)aot_rank_cum_entries(words) * 4);
         B->dense_len = range;
-        dev_fill_u64(B->dense, ~0ull, (int64_t)((range + 1) / 2));
-        RF_HIP(aot_dense_build(skeys, K, kmin, B->dense, s));
+        dev_fill_u64(B->dense_bits, 0, words);
+        RF_HIP(aot_rank_build(skeys, K, kmin, B->dense_bits, words, B->dense_cum, s));
         RF_HIP(hipMemcpyAsync(hs, small + 2, 8, hipMemcpyDeviceToHost, s));
         RF_HIP(hipStreamSynchronize(s));
         if (*(uint64_t*)hs != 0)
@@ -1218,7 +1223,7 @@ std::shared_ptr<GjBuild> gj_build(rf_query* q, const rf_arg* args, int64_t n_arg
 
 rf_query* gj_probe_query(rf_query* q, const GjBuild& b) {
     GroupJoinState& G = *q->gj;
-    if (b.dense || G.probe_tuned) return G.probe_q.get();
+    if (b.dense_bits || G.probe_tuned) return G.probe_q.get();
     if (!G.probe_q_hash) {
         // Dependent probes of a hash index: latency bound; more resident
         // waves keep more in flight (measured on the 60M-row probe with

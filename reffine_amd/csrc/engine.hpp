@@ -220,8 +220,11 @@ struct GjBuild {
     uint64_t index_cap = 0;
     void* filter = nullptr;
     uint64_t filter_bits = 0;
-    void* dense = nullptr;  // direct map key - kmin -> rank (dense key ranges;
-    uint64_t dense_len = 0; // then no index or filter)
+    // Rank map over [kmin, kmin + dense_len) for dense key ranges (then no
+    // index or filter): key bits and the words' first ranks.
+    void* dense_bits = nullptr;
+    void* dense_cum = nullptr;
+    uint64_t dense_len = 0;
     RfProbeDev probe() const;  // the probe plan's index for the external probes
     ~GjBuild();
 };
@@ -268,8 +271,9 @@ Spec& gj_fold_spec(rf_query* q, const rf_arg* args, int64_t n_args);
 void gj_fold_args(rf_query* q, const Spec& fs, const GjBuild& b, const rf_arg* args,
                   int64_t n_args, RfFoldArgs& A);
 int bits_of(uint64_t range);
-hipError_t aot_dense_build(const int64_t* keys, int64_t n, int64_t base, void* dense,
-                           hipStream_t s);
+int64_t aot_rank_cum_entries(int64_t words);
+hipError_t aot_rank_build(const int64_t* keys, int64_t n, int64_t base, void* bits, int64_t words,
+                          void* cum, hipStream_t s);
 hipError_t aot_gj_concat(const RfCatSeg* segs, int32_t nseg, int64_t max_rows,
                          const RfCatArgs& A, hipStream_t s);
 

@@ -38,10 +38,13 @@ struct RfProbeDev {
     int64_t min_key_row;   // row of key == RF_EMPTY_KEY, -1 if none
     const uint32_t* filter;  // optional bitmap of (hash >> 32) & fmask over the keys
     uint64_t fmask;          // filter bits - 1 (a power of two)
-    // Optional direct map for a dense key range: row of key k is
-    // dense[k - dense_base] for k in [dense_base, dense_base + dense_len)
-    // (-1: none); keys outside the range have no row. Replaces the slots.
-    const int32_t* dense;
+    // Optional rank map for a dense key range (replaces the slots): bit
+    // k - dense_base of dense_bits marks key k, for k in [dense_base,
+    // dense_base + dense_len); its row is the number of marked keys below
+    // it: dense_cum[word] (marked keys in earlier 64-bit words) plus the
+    // marked bits below it in its word. Keys outside the range have no row.
+    const uint64_t* dense_bits;
+    const uint32_t* dense_cum;
     int64_t dense_base;
     uint64_t dense_len;
 };

@@ -114,10 +114,15 @@ __device__ __forceinline__ uint64_t rf_hash64(uint64_t k) {
 }
 
 __device__ __forceinline__ int64_t rf_probe(const RfProbeDev& h, int64_t key) {
-    if (h.dense) {
-        // One read: the offset wraps to >= dense_len for keys below the base.
+    if (h.dense_bits) {
+        // Rank of the key among the marked keys (a small, cache-resident
+        // structure); the offset wraps to >= dense_len for keys below the base.
         const uint64_t off = (uint64_t)key - (uint64_t)h.dense_base;
-        return off < h.dense_len ? (int64_t)rf_g(h.dense)[off] : -1;
+        if (off >= h.dense_len) return -1;
+        const uint64_t wd = rf_g(h.dense_bits)[off >> 6];
+        const uint64_t bit = 1ull << (off & 63);
+        if (!(wd & bit)) return -1;
+        return (int64_t)rf_g(h.dense_cum)[off >> 6] + (int64_t)__popcll(wd & (bit - 1));
     }
     if (key == RF_EMPTY_KEY) return h.min_key_row;
     const uint64_t hk = rf_hash64((uint64_t)key);

@@ -411,12 +411,22 @@ extern "C" __global__ void rf_seg_bounds(const uint32_t* __restrict__ rank, int6
     }
 }
 
-// Direct map of sorted distinct keys: dense[keys[i] - base] = i (the rank).
-extern "C" __global__ void rf_dense_build(const int64_t* __restrict__ keys, int64_t n, int64_t base,
-                                          int32_t* dense) {
+// Rank map of sorted distinct keys: mark bit (key - base); then the popcount
+// of each 64-bit word (scanned afterwards into the words' first ranks).
+extern "C" __global__ void rf_rank_bits(const int64_t* __restrict__ keys, int64_t n, int64_t base,
+                                        unsigned long long* bits) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t off = (uint64_t)keys[i] - (uint64_t)base;
+        atomicOr(&bits[off >> 6], 1ull << (off & 63));
+    }
+}
+
+extern "C" __global__ void rf_word_popc(const uint64_t* __restrict__ bits, int64_t words,
+                                        uint32_t* cnt) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < words;
          i += (int64_t)gridDim.x * blockDim.x)
-        dense[(uint64_t)keys[i] - (uint64_t)base] = (int32_t)i;
+        cnt[i] = (uint32_t)__popcll(bits[i]);
 }
 
 // Concatenation of several runs' joined rows: blockIdx.y strides over the
@@ -555,11 +565,26 @@ hipError_t aot_seg_bounds(const uint32_t* rank, int64_t n, int64_t* lo, int64_t*
     return hipGetLastError();
 }
 
-hipError_t aot_dense_build(const int64_t* keys, int64_t n, int64_t base, void* dense,
-                           hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(rf_dense_build, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, n,
-                       base, (int32_t*)dense);
+// Entries the rank map's cumulative counts need for `words` words (the
+// counts and the scan's chunk sums).
+int64_t aot_rank_cum_entries(int64_t words) {
+    return words + (words + RF_SCAN_CHUNK - 1) / RF_SCAN_CHUNK + 1;
+}
+
+// bits: `words` zeroed words; cum: aot_rank_cum_entries(words) entries.
+hipError_t aot_rank_build(const int64_t* keys, int64_t n, int64_t base, void* bits, int64_t words,
+                          void* cum, hipStream_t s) {
+    if (n <= 0 || words <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rf_rank_bits, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, n, base,
+                       (unsigned long long*)bits);
+    hipLaunchKernelGGL(rf_word_popc, dim3(grid_cap(words, 256, 4096)), dim3(256), 0, s,
+                       (const uint64_t*)bits, words, (uint32_t*)cum);
+    uint32_t* c = (uint32_t*)cum;
+    const int64_t nch = (words + RF_SCAN_CHUNK - 1) / RF_SCAN_CHUNK;
+    uint32_t* part = c + words;
+    hipLaunchKernelGGL(rf_scan_chunks, dim3((unsigned)nch), dim3(256), 0, s, c, words, part);
+    hipLaunchKernelGGL(rf_sort_scan, dim3(1), dim3(1024), 0, s, part, nch);
+    hipLaunchKernelGGL(rf_scan_apply, dim3((unsigned)nch), dim3(256), 0, s, c, words, part);
     return hipGetLastError();
 }
 
