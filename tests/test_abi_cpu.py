@@ -104,3 +104,19 @@ def test_run_reduction_compiles():
     text = runtime.compile_check(rq.multidim_op())
     assert "run reduction over the run-end column 0 of table input 0" in text
     assert "rf_seg_fold" in text
+
+
+def test_union_space_is_rejected():
+    """An Op over _in(t, A) | _in(t, B) (the reference's UnionSpace merge
+    loop) is refused as unsupported at compile time, not lowered wrongly."""
+    from reffine_amd.reffiner import _func, _get, _i64_t, _in, _op, _sub, _sym, _vec_t
+    t = _sym("t", _i64_t)
+    A = _sym("left", _vec_t(1, _i64_t, _i64_t))
+    B = _sym("right", _vec_t(1, _i64_t, _i64_t))
+    op = _op([t], _in(t, A) | _in(t, B), [_sub(_get(A[t], 0), _get(B[t], 0))])
+    o = _sym("op", op)
+    fn = _func("ojoin", o, [A, B])
+    fn.tbl[o] = op
+    with pytest.raises(runtime.ReffineError) as e:
+        runtime.compile_check(fn)
+    assert e.value.status == 2, e.value  # RF_ERR_UNSUPPORTED
