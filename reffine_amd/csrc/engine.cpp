@@ -348,7 +348,11 @@ Module& jit_module(const KernelSource& ks) {
     auto m = std::make_unique<Module>();
     RF_HIP(hipModuleLoadData(&m->mod, code.data()));
     RF_HIP(hipModuleGetFunction(&m->fn, m->mod, ks.entry.c_str()));
-    if (ks.entry == "rf_gj_fold") RF_HIP(hipModuleGetFunction(&m->fold_b, m->mod, "rf_gj_fold_b"));
+    if (ks.entry == "rf_gj_fold") {
+        RF_HIP(hipModuleGetFunction(&m->fold_b, m->mod, "rf_gj_fold_b"));
+        RF_HIP(hipModuleGetFunction(&m->fold_w, m->mod, "rf_gj_fold_w"));
+        RF_HIP(hipModuleGetFunction(&m->fold_wb, m->mod, "rf_gj_fold_wb"));
+    }
     if (ks.entry == "rf_q") {
         RF_HIP(hipModuleGetFunction(&m->gather, m->mod, "rf_q_gather"));
         RF_HIP(hipModuleGetFunction(&m->bscan, m->mod, "rf_qb"));
@@ -1334,8 +1338,13 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
     for (int j = 0; j < np; j++) A.pay[j] = pj[wj][j];
     for (size_t j = 0; j < out->cols.size(); j++) A.out[j] = out->cols[j].values;
     void* params[] = {&A};
-    const int g = (int)std::min<int64_t>((K + 255) / 256, 16384);
-    RF_HIP(hipModuleLaunchKernel(fs.mod->fn, g, 1, 1, 256, 1, 1, 0, s, params, nullptr));
+    if (J >= kWaveFoldRows * K) {  // large groups: a wave per group
+        const int g = (int)std::min<int64_t>((K + 3) / 4, 65536);
+        RF_HIP(hipModuleLaunchKernel(fs.mod->fold_w, g, 1, 1, 256, 1, 1, 0, s, params, nullptr));
+    } else {
+        const int g = (int)std::min<int64_t>((K + 255) / 256, 16384);
+        RF_HIP(hipModuleLaunchKernel(fs.mod->fn, g, 1, 1, 256, 1, 1, 0, s, params, nullptr));
+    }
     RF_HIP(hipStreamSynchronize(s));
     return out.release();
 }

@@ -113,6 +113,8 @@ struct Module {
     hipFunction_t bdone = nullptr;   // rf_qb_done: per-unit completion
     hipFunction_t bgather = nullptr; // rf_qb_gather: per-unit selection placement
     hipFunction_t fold_b = nullptr;  // rf_gj_fold_b: folds of several runs (fold modules)
+    hipFunction_t fold_w = nullptr;  // rf_gj_fold_w / _wb: a wave per group (large groups)
+    hipFunction_t fold_wb = nullptr;
     int max_blocks_per_cu = 1;
     int bscan_blocks_per_cu = 1;
 };
@@ -271,6 +273,8 @@ Spec& gj_fold_spec(rf_query* q, const rf_arg* args, int64_t n_args);
 void gj_fold_args(rf_query* q, const Spec& fs, const GjBuild& b, const rf_arg* args,
                   int64_t n_args, RfFoldArgs& A);
 int bits_of(uint64_t range);
+// Joined rows per group (on average) from which a wave folds each group.
+constexpr int64_t kWaveFoldRows = 64;
 int64_t aot_rank_cum_entries(int64_t words);
 hipError_t aot_rank_build(const int64_t* keys, int64_t n, int64_t base, void* bits, int64_t words,
                           void* cum, hipStream_t s);

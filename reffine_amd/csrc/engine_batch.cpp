@@ -509,9 +509,15 @@ static void launch_post(rf_batch* b) {
         FB.gstart = (const int64_t*)(dpost + o_gs);
         FB.n = (int64_t)n;
         void* params[] = {&FB};
-        const int g = (int)std::min<int64_t>((gt + 255) / 256, 16384);
-        RF_HIP(hipModuleLaunchKernel(kv.first->mod->fold_b, g, 1, 1, 256, 1, 1, 0, s, params,
-                                     nullptr));
+        if (jt >= kWaveFoldRows * gt) {  // large groups: a wave per group
+            const int g = (int)std::min<int64_t>((gt + 3) / 4, 65536);
+            RF_HIP(hipModuleLaunchKernel(kv.first->mod->fold_wb, g, 1, 1, 256, 1, 1, 0, s, params,
+                                         nullptr));
+        } else {
+            const int g = (int)std::min<int64_t>((gt + 255) / 256, 16384);
+            RF_HIP(hipModuleLaunchKernel(kv.first->mod->fold_b, g, 1, 1, 256, 1, 1, 0, s, params,
+                                         nullptr));
+        }
     }
     RF_HIP(hipEventRecord(b->post_done, s));
     b->post_pending = false;

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <map>
 #include <set>
+#include <functional>
 #include <sstream>
 
 namespace rf {
@@ -729,79 +730,128 @@ KernelSource gen_group_fold(const GroupJoinPlan& gj, const std::vector<std::vect
     ks.tune.block = 256;
     // Build-table columns are read by index at the group's build row.
     Gen g(gj.build_plan, nullable, ks);
+    if (gj.out_values.size() > 16) throw Unsupported("more than 16 group join output columns");
+    for (auto& t : gj.out_types) ks.out_types.push_back(t);
+    // The fold of one group, in two forms with the same arithmetic in the
+    // same order (the group's joined rows ascending, from the init):
+    //  * rf_gj_fold_one: one thread folds its group;
+    //  * rf_gj_fold_wave: the 64 lanes of a wave load the group's rows 64 at
+    //    a time (coalesced) and every lane folds the chunk in row order from
+    //    lane broadcasts -- for large groups, where a lone thread would wait
+    //    on one load after another.
+    auto fold_fn = [&](bool wave) {
+        std::ostringstream o;
+        o << "__device__ __forceinline__ void " << (wave ? "rf_gj_fold_wave" : "rf_gj_fold_one")
+          << "(const RfFoldArgs& L, const int64_t grp) {\n    {\n";
+        o << "        const int64_t i0 = L.seg_lo[grp], i1 = L.seg_hi[grp];\n";
+        std::ostringstream init;
+        Emitter ei(g, Mode::FINAL, init, "        ");
+        std::vector<std::string> iv;
+        for (auto& f : gj.fields) iv.push_back(ei.expr(f.init));
+        o << init.str();
+        for (size_t f = 0; f < gj.fields.size(); f++) {
+            const std::string T = cty(gj.fields[f].type);
+            o << "        " << T << " s" << f << " = (" << T << ")" << iv[f] << ";\n";
+        }
+        std::ostringstream cst;
+        Emitter ec(g, Mode::FINAL, cst, "        ");
+        ec.ntmp = ei.ntmp;  // one scope: temporaries stay distinct
+        std::vector<std::string> cv(gj.fields.size());
+        for (size_t f = 0; f < gj.fields.size(); f++)
+            if (gj.fields[f].updated && gj.fields[f].const_contrib)
+                cv[f] = ec.expr(gj.fields[f].const_contrib);
+        o << cst.str();
+        // Per-field update from the row's payload values `val(j)`.
+        auto updates = [&](const std::string& ind, const std::function<std::string(int)>& val) {
+            for (size_t f = 0; f < gj.fields.size(); f++) {
+                const GField& gf = gj.fields[f];
+                if (!gf.updated) continue;
+                const std::string T = cty(gf.type), S = "s" + std::to_string(f);
+                std::string v = gf.const_contrib ? "(" + T + ")" + cv[f] : val(gf.pay);
+                std::string upd = combine_expr(gf.op, gf.type, S, v);
+                if (gf.pred_pay >= 0)
+                    o << ind << S << " = " << val(gf.pred_pay) << " ? " << upd << " : " << S << ";\n";
+                else
+                    o << ind << S << " = " << upd << ";\n";
+            }
+        };
+        if (!wave) {
+            o << "#pragma unroll 4\n        for (int64_t i = i0; i < i1; i++) {\n";
+            updates("            ", [&](int j) {
+                return "((const " + cty(gj.pay_types[j]) + "*)L.pay[" + std::to_string(j) + "])[i]";
+            });
+            o << "        }\n";
+        } else {
+            std::set<int> used;
+            for (auto& gf : gj.fields) {
+                if (!gf.updated) continue;
+                if (!gf.const_contrib) used.insert(gf.pay);
+                if (gf.pred_pay >= 0) used.insert(gf.pred_pay);
+            }
+            o << "        const int lane = (int)(threadIdx.x & 63);\n";
+            o << "        for (int64_t c0 = i0; c0 < i1; c0 += 64) {\n";
+            o << "            const int64_t i = c0 + lane;\n";
+            for (int j : used) {
+                const std::string T = cty(gj.pay_types[j]), J = std::to_string(j);
+                o << "            const " << T << " v" << J << " = i < i1 ? ((const " << T
+                  << "*)L.pay[" << J << "])[i] : (" << T << ")0;\n";
+            }
+            o << "            const int n = (int)(i1 - c0 < 64 ? i1 - c0 : 64);\n";
+            o << "            for (int k = 0; k < n; k++) {\n";
+            updates("                ", [&](int j) { return "rf_readlane(v" + std::to_string(j) + ", k)"; });
+            o << "            }\n        }\n";
+        }
+        o << "        const int64_t key = L.keys[grp];\n        const int64_t brow = L.brow[grp];\n"
+             "        (void)key; (void)brow;\n";
+        std::ostringstream body;
+        Emitter em(g, Mode::FINAL, body, "        ");
+        em.ntmp = ec.ntmp;
+        std::vector<std::string> ov;
+        for (auto& v : gj.out_values) ov.push_back(em.expr(v));
+        o << body.str();
+        if (wave) o << "        if (lane != 0) return;\n";
+        for (size_t j = 0; j < ov.size(); j++) {
+            const std::string T = cty(gj.out_types[j]);
+            o << "        ((" << T << "*)L.out[" << j << "])[grp] = (" << T << ")" << ov[j] << ";\n";
+        }
+        o << "    }\n}\n";
+        return o.str();
+    };
     std::ostringstream o;
     o << "// Generated group fold and outputs for group join '" << gj.sym_name << "'.\n";
-    o << "__device__ __forceinline__ void rf_gj_fold_one(const RfFoldArgs& L, const int64_t grp) {\n";
-    o << "    {\n";
-    o << "        const int64_t i0 = L.seg_lo[grp], i1 = L.seg_hi[grp];\n";
-    // Field state, initialised from the reduction's init (host values).
-    std::ostringstream init;
-    Emitter ei(g, Mode::FINAL, init, "        ");
-    std::vector<std::string> iv;
-    for (auto& f : gj.fields) iv.push_back(ei.expr(f.init));
-    o << init.str();
-    for (size_t f = 0; f < gj.fields.size(); f++) {
-        const std::string T = cty(gj.fields[f].type);
-        o << "        " << T << " s" << f << " = (" << T << ")" << iv[f] << ";\n";
-    }
-    // Sequential fold of the group's joined rows in ascending probe-row order.
-    std::ostringstream cst;
-    Emitter ec(g, Mode::FINAL, cst, "        ");
-    ec.ntmp = ei.ntmp;  // one scope: temporaries stay distinct
-    std::vector<std::string> cv(gj.fields.size());
-    for (size_t f = 0; f < gj.fields.size(); f++)
-        if (gj.fields[f].updated && gj.fields[f].const_contrib) cv[f] = ec.expr(gj.fields[f].const_contrib);
-    o << cst.str();
-    o << "#pragma unroll 4\n        for (int64_t i = i0; i < i1; i++) {\n";
-    for (size_t f = 0; f < gj.fields.size(); f++) {
-        const GField& gf = gj.fields[f];
-        if (!gf.updated) continue;
-        const std::string T = cty(gf.type), S = "s" + std::to_string(f);
-        std::string v = gf.const_contrib
-                            ? "(" + T + ")" + cv[f]
-                            : "((const " + cty(gj.pay_types[gf.pay]) + "*)L.pay[" +
-                                  std::to_string(gf.pay) + "])[i]";
-        std::string upd = combine_expr(gf.op, gf.type, S, v);
-        if (gf.pred_pay >= 0)
-            o << "            " << S << " = ((const bool*)L.pay[" << gf.pred_pay << "])[i] ? "
-              << upd << " : " << S << ";\n";
-        else
-            o << "            " << S << " = " << upd << ";\n";
-    }
-    o << "        }\n";
-    o << "        const int64_t key = L.keys[grp];\n        const int64_t brow = L.brow[grp];\n"
-         "        (void)key; (void)brow;\n";
-    std::ostringstream body;
-    Emitter em(g, Mode::FINAL, body, "        ");
-    em.ntmp = ec.ntmp;
-    std::vector<std::string> ov;
-    for (auto& v : gj.out_values) ov.push_back(em.expr(v));
-    if (ov.size() > 16) throw Unsupported("more than 16 group join output columns");
-    o << body.str();
-    for (size_t j = 0; j < ov.size(); j++) {
-        const std::string T = cty(gj.out_types[j]);
-        o << "        ((" << T << "*)L.out[" << j << "])[grp] = (" << T << ")" << ov[j] << ";\n";
-        ks.out_types.push_back(gj.out_types[j]);
-    }
-    o << "    }\n}\n";
-    // One run: a thread per group.
+    o << fold_fn(false) << fold_fn(true);
+    // A run's groups (a thread, or a wave, per group) and several runs' groups
+    // in one launch (the run by binary search over cumulative group counts).
+    const char* find_run =
+        "        int64_t lo = 0, hi = B.n - 1;\n"
+        "        while (lo < hi) {\n"
+        "            const int64_t mid = (lo + hi + 1) >> 1;\n"
+        "            if (B.gstart[mid] <= g) lo = mid; else hi = mid - 1;\n"
+        "        }\n";
     o << "extern \"C\" __global__ __launch_bounds__(256) void rf_gj_fold(RfFoldArgs L) {\n"
          "    for (int64_t grp = blockIdx.x * (int64_t)256 + threadIdx.x; grp < L.groups;\n"
          "         grp += (int64_t)gridDim.x * 256)\n"
          "        rf_gj_fold_one(L, grp);\n"
          "}\n";
-    // Several runs: a thread per group of the launch; its run by binary
-    // search over the cumulative group counts.
+    o << "extern \"C\" __global__ __launch_bounds__(256) void rf_gj_fold_w(RfFoldArgs L) {\n"
+         "    for (int64_t grp = blockIdx.x * (int64_t)4 + (threadIdx.x >> 6); grp < L.groups;\n"
+         "         grp += (int64_t)gridDim.x * 4)\n"
+         "        rf_gj_fold_wave(L, grp);\n"
+         "}\n";
     o << "extern \"C\" __global__ __launch_bounds__(256) void rf_gj_fold_b(RfFoldBatch B) {\n"
          "    const int64_t total = B.gstart[B.n];\n"
          "    for (int64_t g = blockIdx.x * (int64_t)256 + threadIdx.x; g < total;\n"
          "         g += (int64_t)gridDim.x * 256) {\n"
-         "        int64_t lo = 0, hi = B.n - 1;\n"
-         "        while (lo < hi) {\n"
-         "            const int64_t mid = (lo + hi + 1) >> 1;\n"
-         "            if (B.gstart[mid] <= g) lo = mid; else hi = mid - 1;\n"
-         "        }\n"
-         "        rf_gj_fold_one(B.list[lo], g - B.gstart[lo]);\n"
+      << find_run
+      << "        rf_gj_fold_one(B.list[lo], g - B.gstart[lo]);\n"
+         "    }\n"
+         "}\n";
+    o << "extern \"C\" __global__ __launch_bounds__(256) void rf_gj_fold_wb(RfFoldBatch B) {\n"
+         "    const int64_t total = B.gstart[B.n];\n"
+         "    for (int64_t g = blockIdx.x * (int64_t)4 + (threadIdx.x >> 6); g < total;\n"
+         "         g += (int64_t)gridDim.x * 4) {\n"
+      << find_run
+      << "        rf_gj_fold_wave(B.list[lo], g - B.gstart[lo]);\n"
          "    }\n"
          "}\n";
     ks.code = o.str();

@@ -163,6 +163,21 @@ __device__ __forceinline__ T rf_shfl_xor(T v, int o) {
     }
 }
 
+// Lane k's value (k wave-uniform), for every lane.
+template <typename T>
+__device__ __forceinline__ T rf_readlane(T v, int k) {
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, k);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), k);
+        return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+    } else if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), k));
+    } else {
+        return (T)__builtin_amdgcn_readlane((int)v, k);
+    }
+}
+
 __device__ __forceinline__ uint32_t rf_lane() { return __lane_id(); }
 
 // ---------------------------------------------------- block reductions

@@ -51,7 +51,9 @@ def test_group_join_matches_nested_loops(device):
     assert [(int(x), float(y), int(z)) for x, y, z in zip(k, s, c)] == want
 
 
-@pytest.mark.parametrize("nb,np_", [(1, 10), (1000, 1), (5000, 100_000), (100_000, 2_000_000)])
+# (50, 2M): ~10 groups of ~32K joined rows each -- folded a wave per group.
+@pytest.mark.parametrize("nb,np_", [(1, 10), (1000, 1), (5000, 100_000), (100_000, 2_000_000),
+                                    (50, 2_000_000)])
 def test_group_join_sizes(device, nb, np_):
     b, p = datagen.join_tables(nb, np_, seed=nb + np_)
     B, P = _upload(b, p)
