@@ -114,8 +114,11 @@ int rf_sync(void);
  * (src/utils/utils.cpp:11-34, include/reffine/arrow/table.h:24-60): value
  * buffers and validity bitmaps are copied to 256-byte aligned device
  * memory. `dim` is the table's iterator dimension (reffine's
- * ArrowTable::dim, include/reffine/arrow/abi.h:56-69). Children with
- * offset != 0 are rejected with RF_ERR_ARROW. */
+ * ArrowTable::dim, include/reffine/arrow/abi.h:56-69). Offsets are
+ * honoured (the reference's accessors ignore them, vinstr.h:25-111): a
+ * sliced batch -- offset on the struct or on its children -- uploads only
+ * its rows; validity bits at any bit offset; a run-end column keeps the
+ * runs covering the slice, cut to it. */
 int rf_table_upload(const struct ArrowSchema* schema,
                     const struct ArrowArray* array, int64_t dim,
                     rf_table** out);

@@ -8,6 +8,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <list>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -161,13 +162,38 @@ rf::Type rf_table::type() const {
 
 namespace rf {
 
+// Bits [bit0, bit0 + n) of an Arrow validity bitmap (LSB first) as a bitmap
+// starting at bit 0: the source itself when bit0 is a whole byte, else a
+// shifted host copy kept in `keep`.
+static const void* bitmap_window(const void* bm, int64_t bit0, int64_t n,
+                                 std::list<std::vector<uint8_t>>& keep) {
+    if (!bm || n <= 0) return bm;
+    const uint8_t* in = (const uint8_t*)bm;
+    if (bit0 % 8 == 0) return in + bit0 / 8;
+    const int64_t k = bit0 >> 3, last = (bit0 + n - 1) >> 3, nbytes = (n + 7) / 8;
+    const int sh = (int)(bit0 & 7);
+    keep.emplace_back((size_t)nbytes, (uint8_t)0);
+    uint8_t* out = keep.back().data();
+    for (int64_t j = 0; j < nbytes; j++) {
+        const uint32_t lo = (uint32_t)in[k + j] >> sh;
+        const uint32_t hi = (k + j + 1 <= last) ? (uint32_t)in[k + j + 1] << (8 - sh) : 0u;
+        out[j] = (uint8_t)(lo | hi);
+    }
+    return out;
+}
+
+// Copies a record batch (Arrow C data interface, struct array) to HBM. Array
+// offsets are honoured (a sliced batch uploads only its rows): a column's
+// rows are [parent offset + column offset, + batch length); a run-end column
+// keeps the runs that cover its window, with run ends cut to the window and
+// made relative to its start.
 rf_table* table_upload(const ArrowSchema* schema, const ArrowArray* array, int64_t dim) {
     if (!schema || !array) throw ArrowError("null schema or array");
     if (!schema->format || std::string(schema->format) != "+s")
         throw ArrowError("record batch must be a struct array (format \"+s\")");
     if (schema->n_children != array->n_children)
         throw ArrowError("schema/array child count mismatch");
-    if (array->offset != 0) throw ArrowError("struct array offset != 0 is not supported");
+    if (array->offset < 0 || array->length < 0) throw ArrowError("negative offset or length");
     if (dim < 1 || dim > schema->n_children) throw ArrowError("bad dim");
     Device& d = device();
     auto t = std::make_unique<rf_table>();
@@ -179,22 +205,27 @@ rf_table* table_upload(const ArrowSchema* schema, const ArrowArray* array, int64
         void** dst;
     };
     std::vector<Piece> pieces;
+    std::list<std::vector<uint8_t>> keep;  // host copies alive until the copies finish
     size_t total = 0;
     auto add = [&](const void* src, size_t bytes, void** dst) {
         pieces.push_back({src, bytes, total, dst});
         total += (bytes + 255) & ~size_t(255);
         if (bytes == 0) total += 256;
     };
+    const int64_t rows = array->length;
     t->cols.resize(schema->n_children);
     for (int64_t i = 0; i < schema->n_children; i++) {
         const ArrowSchema* cs = schema->children[i];
         const ArrowArray* ca = array->children[i];
         Column& c = t->cols[i];
         c.format = cs->format ? cs->format : "";
+        const std::string col = "column " + std::to_string(i);
+        const int64_t eo = array->offset + ca->offset;  // the column's first row
+        if (ca->offset < 0 || ca->length < array->offset + rows)
+            throw ArrowError(col + " is shorter than the batch");
         const ArrowArray* va = ca;
         const ArrowSchema* vs = cs;
-        if (ca->offset != 0)
-            throw ArrowError("column " + std::to_string(i) + " has offset != 0");
+        int64_t v0 = eo, nv = rows;  // values window
         if (c.format == "+r") {
             if (ca->n_children != 2 || cs->n_children != 2)
                 throw ArrowError("run-end column needs run_ends and values children");
@@ -203,22 +234,45 @@ rf_table* table_upload(const ArrowSchema* schema, const ArrowArray* array, int64
             const ArrowArray* re = ca->children[0];
             va = ca->children[1];
             vs = cs->children[1];
-            if (re->offset != 0 || va->offset != 0)
-                throw ArrowError("run-end children with offset != 0");
             c.enc = Enc::RUNEND;
-            c.logical_length = ca->length;
-            add(re->buffers[1], (size_t)re->length * 4, &c.run_ends);
+            c.logical_length = rows;
+            const int32_t* ends = (const int32_t*)re->buffers[1] + re->offset;
+            const int64_t nr = re->length;
+            // Runs covering logical rows [eo, eo + rows): r0 is the first run
+            // ending after eo, r1 the first ending at or after eo + rows.
+            const int32_t* p0 = std::upper_bound(ends, ends + nr, (int32_t)eo);
+            const int64_t r0 = p0 - ends;
+            const int64_t r1 = std::lower_bound(ends, ends + nr, (int32_t)(eo + rows)) - ends;
+            const int64_t nruns = rows == 0 ? 0 : std::min<int64_t>(r1, nr - 1) - r0 + 1;
+            if (rows > 0 && (r0 >= nr || r1 >= nr))
+                throw ArrowError(col + ": run ends do not cover the batch");
+            if (eo == 0 && nruns == nr && (nr == 0 || ends[nr - 1] == rows)) {
+                add(ends, (size_t)nr * 4, &c.run_ends);
+            } else {
+                keep.emplace_back((size_t)std::max<int64_t>(nruns, 0) * 4, (uint8_t)0);
+                int32_t* cut = (int32_t*)keep.back().data();
+                for (int64_t k = 0; k < nruns; k++)
+                    cut[k] = (int32_t)(std::min<int64_t>(ends[r0 + k], eo + rows) - eo);
+                add(cut, (size_t)nruns * 4, &c.run_ends);
+            }
+            v0 = va->offset + r0;
+            nv = nruns;
+            if (va->offset < 0 || va->length < r0 + nruns)
+                throw ArrowError(col + ": fewer run values than runs");
         }
         if (!format_type(vs->format ? vs->format : "", c.type))
             throw ArrowError("unsupported Arrow format '" + std::string(vs->format) + "'");
-        c.length = va->length;
+        c.length = nv;
         if (c.enc == Enc::FLAT) c.logical_length = c.length;
-        c.null_count = va->null_count;
         if (va->n_buffers < 2) throw ArrowError("primitive array without a values buffer");
-        add(va->buffers[1], (size_t)c.length * c.type.width(), &c.values);
+        const size_t w = (size_t)c.type.width();
+        add((const char*)va->buffers[1] + (size_t)v0 * w, (size_t)nv * w, &c.values);
+        // A window of a column with nulls may hold none; the bitmap goes up
+        // either way (null_count -1 means not computed).
+        c.null_count = (v0 == 0 && nv == va->length) ? va->null_count : -1;
         if (va->buffers[0] && va->null_count != 0) {
-            size_t bytes = ((size_t)c.length + 7) / 8;
-            add(va->buffers[0], bytes, &c.validity);
+            const void* bm = bitmap_window(va->buffers[0], v0, nv, keep);
+            add(bm, ((size_t)nv + 7) / 8, &c.validity);
         }
     }
     char* arena = (char*)dev_alloc(total);
