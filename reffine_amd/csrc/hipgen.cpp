@@ -586,11 +586,33 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
     o << "            const bool inb = FULL || (row >= L.lo && row < L.end);\n";
     o << "            (void)row;\n";
     o << body.str();
-    for (size_t i = 0; i < ks.acc.size(); i++) {
-        std::string ai = "a.a" + std::to_string(i);
-        o << "            " << ai << " = (inb && " << acc_pred[i] << ") ? "
-          << combine_expr(ks.acc[i].op, ks.acc[i].type, ai, acc_contrib[i]) << " : " << ai
-          << ";\n";
+    // Accumulators sharing a predicate update under one branch: lanes whose
+    // row fails it are masked off instead of selecting each accumulator's
+    // old value (two 32-bit selects per 64-bit accumulator).
+    {
+        std::vector<bool> done(ks.acc.size(), false);
+        for (size_t i = 0; i < ks.acc.size(); i++) {
+            if (done[i]) continue;
+            std::vector<size_t> same;
+            for (size_t j = i; j < ks.acc.size(); j++)
+                if (!done[j] && acc_pred[j] == acc_pred[i]) same.push_back(j);
+            if (same.size() == 1) {
+                std::string ai = "a.a" + std::to_string(i);
+                o << "            " << ai << " = (inb && " << acc_pred[i] << ") ? "
+                  << combine_expr(ks.acc[i].op, ks.acc[i].type, ai, acc_contrib[i]) << " : " << ai
+                  << ";\n";
+                done[i] = true;
+                continue;
+            }
+            o << "            if (inb && " << acc_pred[i] << ") {\n";
+            for (size_t j : same) {
+                std::string aj = "a.a" + std::to_string(j);
+                o << "                " << aj << " = "
+                  << combine_expr(ks.acc[j].op, ks.acc[j].type, aj, acc_contrib[j]) << ";\n";
+                done[j] = true;
+            }
+            o << "            }\n";
+        }
     }
     if (ks.has_sel) o << "            sel |= (uint32_t)(inb && " << sel_pred << ") << k;\n";
     for (size_t j = 0; j < early_vals.size(); j++) {
