@@ -349,9 +349,16 @@ Tuning default_tuning(const Plan& plan, bool det) {
     Tuning t = default_tuning(plan);
     if (det) {
         // The deterministic mode combines each wave-tile across lanes; tiles
-        // of several load groups (at least 4096 rows) amortise that combine.
+        // of several load groups (at least 4096 rows, 8192 with more than 16
+        // accumulators: the combine costs per accumulator) amortise that
+        // combine (measured on the 60M-row grouped scan, 36 accumulators:
+        // 0.417 ms at 8192 rows per tile, 0.460 ms at 4096).
+        int nacc = 0;
+        for (auto& r : plan.reduces)
+            for (auto& f : r.fields) nacc += f.updated ? 1 : 0;
+        const int want = nacc > 16 ? 8192 : 4096;
         const int rows = t.sub * 64 * t.items;
-        const int groups = std::max(1, (4096 + rows - 1) / rows);
+        const int groups = std::max(1, (want + rows - 1) / rows);
         t.sub *= groups;
     }
     return t;
@@ -501,6 +508,11 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
       << "        Acc b;";
     for (size_t i = 0; i < ks.acc.size(); i++)
         o << " b.a" << i << " = rf_shfl_xor(a.a" << i << ", o);";
+    o << "\n        acc_combine(a, b);\n    }\n";    o << "    template <int CTRL>\n"
+      << "    static __device__ __forceinline__ void acc_dpp(Acc& a) {\n"
+      << "        Acc b;";
+    for (size_t i = 0; i < ks.acc.size(); i++)
+        o << " b.a" << i << " = rf_dpp<CTRL>(a.a" << i << ");";
     o << "\n        acc_combine(a, b);\n    }\n";
 
     // Vector-loaded columns become parameters of row_block.

@@ -163,6 +163,24 @@ __device__ __forceinline__ T rf_shfl_xor(T v, int o) {
     }
 }
 
+// The value of the lane that DPP control CTRL selects (inside a 16-lane row).
+template <int CTRL, typename T>
+__device__ __forceinline__ T rf_dpp(T v) {
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint32_t lo =
+            (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, true);
+        const uint32_t hi =
+            (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, true);
+        return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+    } else if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(
+            T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+    } else {
+        return (T)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true);
+    }
+}
+
 // Lane k's value (k wave-uniform), for every lane.
 template <typename T>
 __device__ __forceinline__ T rf_readlane(T v, int k) {
@@ -184,8 +202,18 @@ __device__ __forceinline__ uint32_t rf_lane() { return __lane_id(); }
 
 template <class Q>
 __device__ __forceinline__ void rf_wave_reduce(typename Q::Acc& a) {
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) Q::acc_shfl_xor(a, o);
+    // The xor butterfly o = 1, 2, 4, ..., 32. The first four levels move
+    // data with DPP inside 16-lane rows (no LDS round trip): xor 1 and 2 as
+    // quad permutations; then, with every lane of a quad holding the quad's
+    // value, the half-row mirror pairs quads as xor 4 does, and the row
+    // mirror pairs half-rows as xor 8 does -- for a commutative combine the
+    // lanes hold exactly the butterfly's values. xor 16 and 32 cross rows.
+    Q::template acc_dpp<0xB1>(a);   // quad_perm(1,0,3,2): xor 1
+    Q::template acc_dpp<0x4E>(a);   // quad_perm(2,3,0,1): xor 2
+    Q::template acc_dpp<0x141>(a);  // row_half_mirror
+    Q::template acc_dpp<0x140>(a);  // row_mirror
+    Q::acc_shfl_xor(a, 16);
+    Q::acc_shfl_xor(a, 32);
 }
 
 // Reduces `a` over the block in the fixed order; valid in thread 0.
