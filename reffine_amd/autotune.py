@@ -1,6 +1,7 @@
 """Launch-shape autotuner for lowered queries.
 
-Sweeps (BLOCK, ITEMS, SUB) for a query on its real input tables, times each
+Sweeps launch shapes (BLOCK, ITEMS, SUB, WAVES_PER_CU, PIPELINE, LSUB) for a
+query on its real input tables, times each
 candidate's kernel with HIP events (Query.time) and keeps the fastest. The
 choice is cached in memory per (function IR shape, table row counts).
 Deterministic queries are tuned like the others; their reduction order then
@@ -12,18 +13,22 @@ import re
 
 from .runtime import Query, ReffineError, compile_op
 
-_cache: dict[tuple, tuple[int, int, int]] = {}
+_cache: dict[tuple, tuple[int, ...]] = {}
 
 
-def candidates(has_selection: bool) -> list[tuple[int, int, int, int, int]]:
-    """Launch shapes to try (BLOCK, ITEMS, SUB, WAVES_PER_CU, PIPELINE):
-    wave-tiles of SUB*64*ITEMS rows, up to 32 rows per lane per tile."""
+def candidates(has_selection: bool) -> list[tuple[int, int, int, int, int, int]]:
+    """Launch shapes to try (BLOCK, ITEMS, SUB, WAVES_PER_CU, PIPELINE, LSUB):
+    wave-tiles of SUB*64*ITEMS rows, up to 32 rows per lane per tile, loaded
+    LSUB sub-steps at a time (0: as many as the register budget allows)."""
     out = []
     for items in (2, 4):
-        for sub in (1, 2, 4, 8):
+        for sub in (2, 4, 8, 16):
+            if items * sub > 32:
+                continue
             for wpc in (8, 16):
                 for pipe in (0, 1):
-                    out.append((256, items, sub, wpc, pipe))
+                    for lsub in ((0, 4) if sub > 4 else (0,)):
+                        out.append((256, items, sub, wpc, pipe, lsub))
     return out
 
 
@@ -35,7 +40,7 @@ def _shape_key(q: Query, inputs) -> tuple:
 
 
 def autotune(func, inputs, deterministic: bool = False, iters: int = 10,
-             cands: list[tuple[int, int, int]] | None = None, verbose: bool = False):
+             cands: list[tuple[int, ...]] | None = None, verbose: bool = False):
     """Returns (best Query, {tuning: ms}) for `func` on `inputs`."""
     base = compile_op(func, deterministic=deterministic)
     key = _shape_key(base, inputs)

@@ -1100,7 +1100,11 @@ void setup_group_join(rf_query* q, const Tuning* probe_tune) {
     const GroupJoinPlan& gp = *q->plan.gj;
     auto G = std::make_unique<GroupJoinState>();
     G->build_q = sub_query(gp.build_plan, default_tuning(gp.build_plan));
-    G->probe_q = sub_query(gp.probe_plan, probe_tune ? *probe_tune : default_tuning(gp.probe_plan));
+    // The probe scan against a rank map (dense build keys); the one against
+    // a hash index is compiled on first use (gj_probe_query).
+    Plan dp = gp.probe_plan;
+    dp.dense_probes = true;
+    G->probe_q = sub_query(dp, probe_tune ? *probe_tune : default_tuning(dp));
     G->probe_q->probe_override.resize(gp.probe_plan.probes.size());
     G->probe_tuned = probe_tune != nullptr;
     Spec fs;
@@ -1114,7 +1118,10 @@ void setup_group_join(rf_query* q, const Tuning* probe_tune) {
 std::vector<KernelSource> group_join_sources(const Plan& plan,
                                              const std::vector<std::vector<bool>>& nullable) {
     const GroupJoinPlan& gp = *plan.gj;
+    Plan dp = gp.probe_plan;
+    dp.dense_probes = true;
     return {gen_scan(gp.build_plan, nullable, default_tuning(gp.build_plan), false),
+            gen_scan(dp, nullable, default_tuning(dp), false),
             gen_scan(gp.probe_plan, nullable, default_tuning(gp.probe_plan), false),
             gen_group_fold(gp, nullable)};
 }
@@ -1279,16 +1286,18 @@ std::shared_ptr<GjBuild> gj_build(rf_query* q, const rf_arg* args, int64_t n_arg
     return B;
 }
 
+// The kernel that reads a rank map is only ever paired with a build side
+// that has one (its probe has no hash-index path).
 rf_query* gj_probe_query(rf_query* q, const GjBuild& b) {
     GroupJoinState& G = *q->gj;
-    if (b.dense_bits || G.probe_tuned) return G.probe_q.get();
+    if (b.dense_bits) return G.probe_q.get();
     if (!G.probe_q_hash) {
         // Dependent probes of a hash index: latency bound; more resident
         // waves keep more in flight (measured on the 60M-row probe with
         // sparse keys: 0.82 ms at 16 waves per CU, 0.99 ms at 8).
         const Plan& pp = q->plan.gj->probe_plan;
-        Tuning t = default_tuning(pp);
-        t.waves_per_cu = 16;
+        Tuning t = G.probe_tuned ? G.probe_q->tune : default_tuning(pp);
+        if (!G.probe_tuned) t.waves_per_cu = 16;
         G.probe_q_hash = sub_query(pp, t);
         G.probe_q_hash->probe_override.resize(pp.probes.size());
     }

@@ -126,7 +126,11 @@ struct Emitter {
         if (!probes_done.count(j)) {
             const Probe& pr = g.plan.probes[j];
             std::string k = expr(pr.key);
-            if (pr.guard) {
+            if (pr.external && g.plan.dense_probes) {
+                const std::string gd = pr.guard ? expr(pr.guard) : std::string("true");
+                os << indent << "const int64_t " << h << " = rf_probe_dense(L.probe[" << j
+                   << "], (int64_t)" << k << ", " << gd << ");\n";
+            } else if (pr.guard) {
                 std::string gd = expr(pr.guard);
                 os << indent << "const int64_t " << h << " = (" << gd << ") ? rf_probe(L.probe["
                    << j << "], (int64_t)" << k << ") : (int64_t)-1;\n";
@@ -412,6 +416,13 @@ Tuning default_tuning(const Plan& plan) {
         t.pipeline = 0;
         t.sub = std::max(1, 8 / t.items);
         t.waves_per_cu = 8;
+        if (plan.dense_probes) {
+            // Rank-map probes are cache hits: the pipelined shape with twice
+            // the rows per tile is the fastest measured (0.480 ms against
+            // 0.492 ms on the join workload's probe).
+            t.pipeline = 1;
+            t.sub *= 2;
+        }
     }
     return t;
 }

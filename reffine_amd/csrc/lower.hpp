@@ -88,6 +88,10 @@ struct Plan {
     // staged (early materialisation) instead of read again by row index in
     // the placement kernel; for selections that keep a large share of rows.
     bool early_sel = false;
+    // External probes read a rank map (a group join's dense build keys):
+    // the kernel calls the branch-free rank lookup; it must only be given
+    // rank maps.
+    bool dense_probes = false;
     std::shared_ptr<GroupJoinPlan> gj;  // set: the function is a group join
     std::shared_ptr<SegReducePlan> seg; // set: a reduction per run of a run-end column
 

@@ -495,7 +495,7 @@ class Query:
 def compile_op(func: Func, vectorize: bool = False, deterministic: bool = False,
                tuning: tuple | None = None) -> Query:
     """compile_op<T>(func, vectorize) (include/reffine/utils/utils.h:47-54).
-    `tuning` = (BLOCK, ITEMS, SUB[, WAVES_PER_CU[, PIPELINE]]) overrides the
+    `tuning` = (BLOCK, ITEMS, SUB[, WAVES_PER_CU[, PIPELINE[, LSUB]]]) overrides the
     default launch shape."""
     return Query(func, vectorize, deterministic, tuning)
 

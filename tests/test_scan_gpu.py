@@ -90,3 +90,23 @@ def test_c2_with_nulls(device):
     assert cnt == int(m.sum()) and sq == int(c["q"][m].sum())
     assert brute.rel_err(svq, brute.fsum(c["v"][m] * c["q"][m])) <= FP_TOL
     np.testing.assert_array_equal(sel.column(0), c["rowid"][m])
+
+
+def test_autotune_keeps_results(device):
+    """The autotuner's choice (a few candidates on a 2M-row C2 table)
+    computes the same count, integer sum and selection as the default shape,
+    and is reused from its cache for the same shape of inputs."""
+    from reffine_amd import autotune
+    n = 2_000_000
+    c = datagen.scan_table(n, 31, with_q=True, rowid_dtype=np.int32)
+    t = DeviceTable.from_columns([c["rowid"], c["k"], c["v"], c["d"], c["q"]], dim=1)
+    fn = workloads.scan_select_c2()
+    cands = [(256, 2, 4, 16, 0, 0), (256, 2, 16, 8, 1, 0), (256, 4, 8, 8, 1, 4)]
+    best, times = autotune.autotune(fn, [t], cands=cands)
+    assert set(times) == set(cands) and all(ms > 0 for ms in times.values())
+    (cnt, sq, _), sel = best(t)
+    (cnt0, sq0, _), sel0 = compile_op(fn)(t)
+    assert (cnt, sq) == (cnt0, sq0)
+    assert np.array_equal(sel.column(0), sel0.column(0))
+    again, t2 = autotune.autotune(fn, [t])
+    assert t2 == {} and again.tuning() == best.tuning()
