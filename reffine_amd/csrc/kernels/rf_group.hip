@@ -51,6 +51,25 @@ __device__ __forceinline__ void rf_sort_hist(const RfSortPass& P) {
     P.hist[(int64_t)t * P.nblocks + blockIdx.x] = h[t];
 }
 
+// The same histogram with 2^RB digits (RB = 10: two passes where 8-bit
+// digits need three).
+template <typename K, int RB>
+__device__ __forceinline__ void rf_sort_hist_w(const RfSortPass& P) {
+    constexpr int R = 1 << RB;
+    __shared__ uint32_t h[R];
+    const int t = threadIdx.x;
+    for (int d = t; d < R; d += RF_SORT_BLOCK) h[d] = 0;
+    __syncthreads();
+    const int64_t e0 = (int64_t)blockIdx.x * RF_SORT_TILE;
+    const int64_t e1 = e0 + RF_SORT_TILE < P.n ? e0 + RF_SORT_TILE : P.n;
+    for (int64_t e = e0 + t; e < e1; e += RF_SORT_BLOCK) {
+        const uint64_t k = (uint64_t)((const K*)P.kin)[e];
+        atomicAdd(&h[(uint32_t)((k - P.bias) >> P.shift) & (uint32_t)(R - 1)], 1u);
+    }
+    __syncthreads();
+    for (int d = t; d < R; d += RF_SORT_BLOCK) P.hist[(int64_t)d * P.nblocks + blockIdx.x] = h[d];
+}
+
 // Stable scatter of one tile. Element e = tile0 + round*256 + thread; its
 // destination is the digit's running offset for this block, plus the
 // elements of the same digit in lower waves of the round, plus those in
@@ -149,33 +168,54 @@ __device__ __forceinline__ void rf_place_plane(void* plane, const uint32_t (&lpo
 // digit's output range. The tile's count of digit d is the difference of
 // consecutive entries of the scanned (digit-major) histogram. Dynamic LDS:
 // TILE keys, then one plane of TILE values per payload.
-template <typename K>
+template <typename K, int RB>
 __device__ __forceinline__ void rf_sort_scatter_lds(const RfSortPass& P) {
+    constexpr int R = 1 << RB;          // digits
+    constexpr int DT = R / RF_SORT_BLOCK;  // digits per thread
     constexpr int WAVES = RF_SORT_BLOCK / 64;
     constexpr int64_t WTILE = RF_SORT_TILE / WAVES;  // elements per wave
     extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ uint32_t goff[256], lstart[256];
-    __shared__ uint32_t cnt[WAVES][256];
+    __shared__ uint32_t goff[R];
+    __shared__ uint16_t lstart[R];      // <= RF_SORT_TILE
+    __shared__ uint16_t cnt[WAVES][R];  // <= WTILE
+    __shared__ uint32_t s_scan[RF_SORT_BLOCK];
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     const int64_t e0 = (int64_t)blockIdx.x * RF_SORT_TILE;
     const int64_t e1 = e0 + RF_SORT_TILE < P.n ? e0 + RF_SORT_TILE : P.n;
-    const int64_t m = 256 * P.nblocks;
-    const int64_t idx = (int64_t)t * P.nblocks + blockIdx.x;
-    const uint32_t g = P.hist[idx];
-    const uint32_t c = (idx + 1 < m ? P.hist[idx + 1] : (uint32_t)P.n) - g;
-    goff[t] = g;
-    lstart[t] = c;
+    const int64_t m = (int64_t)R * P.nblocks;
+    // Thread t owns digits [t*DT, (t+1)*DT): their global offsets, their
+    // counts in this tile (differences of the scanned histogram) and their
+    // local starts (an exclusive scan over the digits).
+    uint32_t c[DT];
+    uint32_t csum = 0;
 #pragma unroll
-    for (int v = 0; v < WAVES; v++) cnt[v][t] = 0;
+    for (int j = 0; j < DT; j++) {
+        const int d = t * DT + j;
+        const int64_t idx = (int64_t)d * P.nblocks + blockIdx.x;
+        const uint32_t g = P.hist[idx];
+        c[j] = (idx + 1 < m ? P.hist[idx + 1] : (uint32_t)P.n) - g;
+        goff[d] = g;
+        csum += c[j];
+#pragma unroll
+        for (int v = 0; v < WAVES; v++) cnt[v][d] = 0;
+    }
+    s_scan[t] = csum;
     __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        const uint32_t x = t >= off ? lstart[t - off] : 0u;
+    for (int off = 1; off < RF_SORT_BLOCK; off <<= 1) {
+        const uint32_t x = t >= off ? s_scan[t - off] : 0u;
         __syncthreads();
-        lstart[t] += x;
+        s_scan[t] += x;
         __syncthreads();
     }
-    lstart[t] -= c;  // exclusive
-    // This wave's elements: [wbase, wbase + WTILE), RF_SORT_ITEMS rounds of 64.
+    {
+        uint32_t run = s_scan[t] - csum;
+#pragma unroll
+        for (int j = 0; j < DT; j++) {
+            lstart[t * DT + j] = (uint16_t)run;
+            run += c[j];
+        }
+    }
+    // Ranking: each wave over its contiguous quarter of the tile (see above).
     const int64_t wbase = e0 + (int64_t)w * WTILE;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     K kr[RF_SORT_ITEMS];
@@ -189,28 +229,31 @@ __device__ __forceinline__ void rf_sort_scatter_lds(const RfSortPass& P) {
 #pragma unroll
     for (int r = 0; r < RF_SORT_ITEMS; r++) {
         const bool ok = wbase + r * 64 + lane < e1;
-        const uint32_t d = ok ? (uint32_t)(((uint64_t)kr[r] - P.bias) >> P.shift) & 255u : 0u;
+        const uint32_t d =
+            ok ? (uint32_t)(((uint64_t)kr[r] - P.bias) >> P.shift) & (uint32_t)(R - 1) : 0u;
         uint64_t peers = __ballot(ok);
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
+        for (int b = 0; b < RB; b++) {
             const bool on = (d >> b) & 1u;
             const uint64_t mk = __ballot(on);
             peers &= on ? mk : ~mk;
         }
         const uint32_t below = (uint32_t)__popcll(peers & lt);
         const uint32_t before = cnt[w][d];
-        if (ok && below == 0) cnt[w][d] = before + (uint32_t)__popcll(peers);
+        if (ok && below == 0) cnt[w][d] = (uint16_t)(before + (uint32_t)__popcll(peers));
         lpos[r] = before + below;  // rank among the wave's elements of digit d
         dig[r] = d;
     }
     __syncthreads();
-    {
+#pragma unroll
+    for (int j = 0; j < DT; j++) {
         // Per digit: exclusive prefix over the waves, from the tile start.
-        uint32_t run = lstart[t];
+        const int d = t * DT + j;
+        uint32_t run = lstart[d];
 #pragma unroll
         for (int v = 0; v < WAVES; v++) {
-            const uint32_t x = cnt[v][t];
-            cnt[v][t] = run;
+            const uint32_t x = cnt[v][d];
+            cnt[v][d] = (uint16_t)run;
             run += x;
         }
     }
@@ -237,7 +280,7 @@ __device__ __forceinline__ void rf_sort_scatter_lds(const RfSortPass& P) {
     const int tn = (int)(e1 - e0);
     for (int i = t; i < tn; i += RF_SORT_BLOCK) {
         const K key = sk[i];
-        const uint32_t d = (uint32_t)(((uint64_t)key - P.bias) >> P.shift) & 255u;
+        const uint32_t d = (uint32_t)(((uint64_t)key - P.bias) >> P.shift) & (uint32_t)(R - 1);
         const int64_t pos = (int64_t)goff[d] + (i - (int64_t)lstart[d]);
         ((K*)P.kout)[pos] = key;
         size_t off = 0;
@@ -260,10 +303,22 @@ extern "C" __global__ __launch_bounds__(RF_SORT_BLOCK) void rf_sort_scatter_u64(
     rf_sort_scatter<uint64_t>(P);
 }
 extern "C" __global__ __launch_bounds__(RF_SORT_BLOCK) void rf_sort_scatter_lds_u32(RfSortPass P) {
-    rf_sort_scatter_lds<uint32_t>(P);
+    rf_sort_scatter_lds<uint32_t, 8>(P);
 }
 extern "C" __global__ __launch_bounds__(RF_SORT_BLOCK) void rf_sort_scatter_lds_u64(RfSortPass P) {
-    rf_sort_scatter_lds<uint64_t>(P);
+    rf_sort_scatter_lds<uint64_t, 8>(P);
+}
+extern "C" __global__ __launch_bounds__(RF_SORT_BLOCK) void rf_sort_hist10_u32(RfSortPass P) {
+    rf_sort_hist_w<uint32_t, 10>(P);
+}
+extern "C" __global__ __launch_bounds__(RF_SORT_BLOCK) void rf_sort_hist10_u64(RfSortPass P) {
+    rf_sort_hist_w<uint64_t, 10>(P);
+}
+extern "C" __global__ __launch_bounds__(RF_SORT_BLOCK) void rf_sort_scatter10_u32(RfSortPass P) {
+    rf_sort_scatter_lds<uint32_t, 10>(P);
+}
+extern "C" __global__ __launch_bounds__(RF_SORT_BLOCK) void rf_sort_scatter10_u64(RfSortPass P) {
+    rf_sort_scatter_lds<uint64_t, 10>(P);
 }
 
 // Exclusive scan of m histogram entries in three launches: per-chunk sums,
@@ -471,7 +526,7 @@ static int grid_cap(int64_t n, int block, int cap) {
 int64_t aot_sort_hist_entries(int64_t n) {
     int64_t nb = (n + RF_SORT_TILE - 1) / RF_SORT_TILE;
     if (nb < 1) nb = 1;
-    const int64_t m = 256 * nb;
+    const int64_t m = 1024 * nb;  // room for 10-bit digits
     return m + (m + RF_SCAN_CHUNK - 1) / RF_SCAN_CHUNK;
 }
 
@@ -496,11 +551,14 @@ hipError_t aot_radix_sort(void* const keys[2], void* const pay[2][RF_SORT_MAXPAY
         P.pw[j] = pw[j];
         elem += (size_t)pw[j];
     }
-    // Staged scatter when a tile's elements fit 48 KiB of LDS.
+    // Staged scatter when a tile's elements fit 48 KiB of LDS; then 10-bit
+    // digits when they save a pass (17-20 bits: two passes instead of three).
     const size_t lds_bytes = (size_t)RF_SORT_TILE * elem;
     const bool staged = lds_bytes <= 49152;
+    const bool wide = staged && (bits + 9) / 10 < (bits + 7) / 8;
+    const int step = wide ? 10 : 8;
     int cur = 0;
-    for (int shift = 0; shift < bits; shift += 8) {
+    for (int shift = 0; shift < bits; shift += step) {
         P.kin = keys[cur];
         P.kout = keys[cur ^ 1];
         for (int j = 0; j < np; j++) {
@@ -509,17 +567,25 @@ hipError_t aot_radix_sort(void* const keys[2], void* const pay[2][RF_SORT_MAXPAY
         }
         P.shift = shift;
         const dim3 g((unsigned)P.nblocks), b(RF_SORT_BLOCK);
-        if (key_bytes == 4)
+        if (wide && key_bytes == 4)
+            hipLaunchKernelGGL(rf_sort_hist10_u32, g, b, 0, s, P);
+        else if (wide)
+            hipLaunchKernelGGL(rf_sort_hist10_u64, g, b, 0, s, P);
+        else if (key_bytes == 4)
             hipLaunchKernelGGL(rf_sort_hist_u32, g, b, 0, s, P);
         else
             hipLaunchKernelGGL(rf_sort_hist_u64, g, b, 0, s, P);
-        const int64_t m = 256 * P.nblocks;
+        const int64_t m = (int64_t)(1 << step) * P.nblocks;
         const int64_t nch = (m + RF_SCAN_CHUNK - 1) / RF_SCAN_CHUNK;
         uint32_t* part = hist + m;  // the caller sizes hist for m + chunks entries
         hipLaunchKernelGGL(rf_scan_chunks, dim3((unsigned)nch), dim3(256), 0, s, hist, m, part);
         hipLaunchKernelGGL(rf_sort_scan, dim3(1), dim3(1024), 0, s, part, nch);
         hipLaunchKernelGGL(rf_scan_apply, dim3((unsigned)nch), dim3(256), 0, s, hist, m, part);
-        if (staged && key_bytes == 4)
+        if (wide && key_bytes == 4)
+            hipLaunchKernelGGL(rf_sort_scatter10_u32, g, b, lds_bytes, s, P);
+        else if (wide)
+            hipLaunchKernelGGL(rf_sort_scatter10_u64, g, b, lds_bytes, s, P);
+        else if (staged && key_bytes == 4)
             hipLaunchKernelGGL(rf_sort_scatter_lds_u32, g, b, lds_bytes, s, P);
         else if (staged)
             hipLaunchKernelGGL(rf_sort_scatter_lds_u64, g, b, lds_bytes, s, P);
