@@ -292,8 +292,10 @@ typedef struct rf_request {
  * requests that arrive while a round is on the GPU are admitted into the
  * other runner as units and arena memory allow, and it launches at once, so
  * rounds overlap on the device. Latency of request i = done_us - arrival_us.
- * `wall_us` receives the time from start to the last completion. No
- * reference counterpart. */
+ * `wall_us` receives the time from start to the last completion. The two
+ * runners (arena, buffers, captured graphs) are kept for the next call with
+ * the same max_units, arena_bytes and use_graph (at most four runners in
+ * all, held until the process ends). No reference counterpart. */
 int rf_serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes,
              int32_t use_graph, double* wall_us);
 

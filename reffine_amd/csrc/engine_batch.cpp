@@ -28,6 +28,7 @@
 #include <chrono>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <thread>
 
 #include "engine.hpp"
@@ -623,6 +624,45 @@ static void run_plain_request(rf_request& rq) {
     if (sel) table_free(sel);
 }
 
+// Runners kept between serving calls (their arenas, result regions,
+// post-phase buffers and captured graphs), by (units, arena, graph): a
+// serving loop started again does not allocate or capture again. At most
+// kPoolRunners are kept; the rest are freed when returned.
+static constexpr size_t kPoolRunners = 4;
+static std::mutex g_pool_mu;
+// Never destroyed: runners hold HIP objects, which must not be released
+// from static destructors after the runtime has shut down.
+static auto& g_pool = *new std::vector<std::unique_ptr<rf_batch>>();
+
+static std::unique_ptr<rf_batch> runner_take(int32_t units, int64_t arena, bool graph) {
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        for (size_t i = 0; i < g_pool.size(); i++) {
+            rf_batch* b = g_pool[i].get();
+            if (b->cap == units && (int64_t)b->arena_bytes == arena && b->use_graph == graph) {
+                std::unique_ptr<rf_batch> r = std::move(g_pool[i]);
+                g_pool.erase(g_pool.begin() + (std::ptrdiff_t)i);
+                return r;
+            }
+        }
+    }
+    return std::unique_ptr<rf_batch>(batch_create(units, arena, graph));
+}
+
+struct RunnerLease {
+    std::unique_ptr<rf_batch> b;
+    ~RunnerLease() {
+        if (!b) return;
+        try {
+            batch_reset(b.get());
+        } catch (...) {
+            return;  // a runner in a bad state is freed, not pooled
+        }
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        if (g_pool.size() < kPoolRunners) g_pool.push_back(std::move(b));
+    }
+};
+
 // Runs `n` requests (arrival times in us from the start, ascending) through
 // two runners with continuous batching; fills each request's completion
 // time, status and outputs. Returns the wall time in us.
@@ -630,9 +670,11 @@ double serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes
     for (int64_t i = 1; i < n; i++)
         if (reqs[i].arrival_us < reqs[i - 1].arrival_us)
             throw IRError("requests must be ordered by arrival time");
-    std::unique_ptr<rf_batch> r0(batch_create(max_units, arena_bytes, use_graph));
-    std::unique_ptr<rf_batch> r1(batch_create(max_units, arena_bytes, use_graph));
-    rf_batch* runners[2] = {r0.get(), r1.get()};
+    if (max_units < 1 || max_units > 65536) throw IRError("units per round must be 1..65536");
+    if (arena_bytes < 4096) throw IRError("arena must hold at least 4 KiB");
+    RunnerLease r0{runner_take(max_units, arena_bytes, use_graph)};
+    RunnerLease r1{runner_take(max_units, arena_bytes, use_graph)};
+    rf_batch* runners[2] = {r0.b.get(), r1.b.get()};
     std::vector<std::vector<int64_t>> members(2);  // request index per unit
     int64_t next = 0, finished = 0;
     int filling = 0;
