@@ -55,6 +55,25 @@ struct ArrowArray {
 };
 #endif /* ARROW_C_DATA_INTERFACE */
 
+/* Arrow C Device data interface (arrow/c/abi.h): an ArrowArray with the
+ * device that holds its buffers. */
+#ifndef ARROW_C_DEVICE_DATA_INTERFACE
+#define ARROW_C_DEVICE_DATA_INTERFACE
+typedef int32_t ArrowDeviceType;
+#define ARROW_DEVICE_CPU 1
+#define ARROW_DEVICE_CUDA 2
+#define ARROW_DEVICE_CUDA_HOST 3
+#define ARROW_DEVICE_ROCM 10
+#define ARROW_DEVICE_ROCM_HOST 11
+struct ArrowDeviceArray {
+    struct ArrowArray array;
+    int64_t device_id;
+    ArrowDeviceType device_type;
+    void* sync_event;
+    int64_t reserved[3];
+};
+#endif /* ARROW_C_DEVICE_DATA_INTERFACE */
+
 enum rf_status {
     RF_OK = 0,
     RF_ERR_INVALID = 1,     /* bad argument, malformed IR, type error      */
@@ -122,6 +141,28 @@ int rf_sync(void);
 int rf_table_upload(const struct ArrowSchema* schema,
                     const struct ArrowArray* array, int64_t dim,
                     rf_table** out);
+
+/* Imports a record batch given through the Arrow C Device data interface
+ * (struct array "+s"). The array is moved in (its release callback runs
+ * here after a copy or on error, or when the table is freed). ROCm memory on
+ * this library's device is borrowed without a copy when every column is flat
+ * with 16-byte aligned values and validity on a 16-bit word boundary (the
+ * kernels' load granules), else copied; host memory (CPU, pinned) is
+ * uploaded as rf_table_upload does. A ROCm array's sync_event (hipEvent_t*)
+ * is waited on before use. The schema is read, not released. Replaces the
+ * host-only ArrowTable exchange (include/reffine/arrow/abi.h:56-69,
+ * src/utils/utils.cpp:11-34). */
+int rf_table_import_device(const struct ArrowSchema* schema, struct ArrowDeviceArray* array,
+                           int64_t dim, rf_table** out);
+
+/* Exports a device table as an Arrow C Device array (device_type ROCm, no
+ * copy, sync_event NULL: the data is complete). The exported structs share
+ * the table's device buffers and keep them alive after rf_table_free until
+ * their release callbacks run; the caller owns both structs. A table over
+ * borrowed buffers (rf_table_wrap_device, batch results) exports views that
+ * must not outlive the buffers' owner. */
+int rf_table_export_device(const rf_table* t, struct ArrowSchema* schema,
+                           struct ArrowDeviceArray* array);
 
 /* Creates a device table over column buffers that are already in device
  * memory (borrowed; the caller keeps them alive). `formats[i]` is the Arrow

@@ -22,7 +22,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
-HOST_SOURCES = ["ir.cpp", "lower.cpp", "hipgen.cpp", "engine.cpp", "engine_batch.cpp", "capi.cpp"]
+HOST_SOURCES = ["ir.cpp", "lower.cpp", "hipgen.cpp", "engine.cpp", "engine_batch.cpp",
+                "arrow_device.cpp", "capi.cpp"]
 DEVICE_SOURCES = ["kernels/rf_aot.hip", "kernels/rf_group.hip"]
 EMBED = {"kRfAbiSrc": "kernels/rf_abi.h", "kRfDeviceSrc": "kernels/rf_device.h"}
 

@@ -80,6 +80,19 @@ int rf_table_upload(const struct ArrowSchema* schema, const struct ArrowArray* a
     });
 }
 
+int rf_table_import_device(const struct ArrowSchema* schema, struct ArrowDeviceArray* array,
+                           int64_t dim, rf_table** out) {
+    return guard([&] {
+        need(out != nullptr, "out is NULL");
+        *out = rf::table_import_device(schema, array, dim);
+    });
+}
+
+int rf_table_export_device(const rf_table* t, struct ArrowSchema* schema,
+                           struct ArrowDeviceArray* array) {
+    return guard([&] { rf::table_export_device(t, schema, array); });
+}
+
 int rf_table_wrap_device(int64_t dim, int64_t n_cols, int64_t n_rows,
                          const char* const* formats, void* const* values,
                          void* const* validity, rf_table** out) {

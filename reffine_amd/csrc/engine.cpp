@@ -145,10 +145,29 @@ static bool format_type(const std::string& f, Type& t) {
     return true;
 }
 
+bool format_type_of(const std::string& f, Type& t) { return format_type(f, t); }
+
+std::string arrow_format_of(const Type& t) {
+    switch (t.bt) {
+        case BT::BOOL: case BT::INT8: return "c";
+        case BT::UINT8: return "C";
+        case BT::INT16: return "s";
+        case BT::UINT16: return "S";
+        case BT::INT32: return "i";
+        case BT::UINT32: return "I";
+        case BT::INT64: case BT::IDX: return "l";
+        case BT::UINT64: return "L";
+        case BT::FLOAT32: return "f";
+        case BT::FLOAT64: return "g";
+        default: throw ArrowError("no Arrow format for type " + t.str());
+    }
+}
+
 }  // namespace rf
 
 static std::atomic<uint64_t> g_table_uid{1};
-rf_table::rf_table() : uid(g_table_uid.fetch_add(1)) {}
+rf_table::rf_table()
+    : uid(g_table_uid.fetch_add(1)), owned(std::make_shared<rf::OwnedBuffers>()) {}
 
 rf::Type rf_table::type() const {
     std::vector<rf::Type> d;
@@ -276,7 +295,7 @@ rf_table* table_upload(const ArrowSchema* schema, const ArrowArray* array, int64
         }
     }
     char* arena = (char*)dev_alloc(total);
-    t->owned.push_back(arena);
+    t->owned->p.push_back(arena);
     for (auto& p : pieces) {
         *p.dst = arena + p.off;
         if (p.bytes && p.src)
@@ -325,9 +344,9 @@ void table_build_index(rf_table* t) {
     if (t->has_index) return;
     const uint64_t cap = next_pow2(std::max<uint64_t>(16, 2 * (uint64_t)c.length));
     void* keys = dev_alloc(cap * 8);
-    t->owned.push_back(keys);
+    t->owned->p.push_back(keys);
     void* rows = dev_alloc(cap * 8);
-    t->owned.push_back(rows);
+    t->owned->p.push_back(rows);
     void* mk = dev_alloc(8);
     dev_fill_u64(keys, (uint64_t)RF_EMPTY_KEY, (int64_t)cap);
     dev_fill_u64(rows, (uint64_t)INT64_MAX, (int64_t)cap);
@@ -768,10 +787,12 @@ rf_query::~rf_query() {
 
 namespace rf {
 
+OwnedBuffers::~OwnedBuffers() {
+    for (void* x : p) dev_free(x);
+}
+
 void table_free(rf_table* t) {
-    if (!t) return;
-    for (void* p : t->owned) dev_free(p);
-    delete t;
+    delete t;  // its buffers go with the last holder
 }
 
 static void fill_params(const KernelSource& ks, EvalCtx& c, RfParam* out) {
@@ -840,7 +861,7 @@ rf_table* alloc_table(const std::vector<Type>& types, int64_t count, int64_t dim
         total += ((size_t)std::max<int64_t>(count, 1) * ty.width() + 255) & ~size_t(255);
     }
     char* arena = (char*)dev_alloc(std::max<size_t>(total, 256));
-    t->owned.push_back(arena);
+    t->owned->p.push_back(arena);
     for (size_t j = 0; j < types.size(); j++) {
         Column col = out_column(types[j], count);
         col.values = arena + offs[j];

@@ -82,6 +82,13 @@ struct Column {
     int64_t logical_length = 0;
 };
 
+// Device allocations of a table, freed when the last holder drops them (the
+// table, and any Arrow device array exported from it).
+struct OwnedBuffers {
+    std::vector<void*> p;
+    ~OwnedBuffers();
+};
+
 struct HashIndex {
     void* keys = nullptr;
     void* rows = nullptr;
@@ -96,7 +103,8 @@ struct rf_table {
     uint64_t uid;    // unique per table object (a freed table's address may be reused)
     int64_t dim = 1;
     std::vector<rf::Column> cols;
-    std::vector<void*> owned;  // device allocations freed with the table
+    std::shared_ptr<rf::OwnedBuffers> owned;  // device allocations (shared with exports)
+    std::vector<std::shared_ptr<void>> keep;   // borrowed inputs kept alive (imports)
     rf::HashIndex index;
     bool has_index = false;
     rf::Type type() const;
@@ -163,6 +171,11 @@ struct rf_query {
 namespace rf {
 
 rf_table* table_upload(const ArrowSchema* schema, const ArrowArray* array, int64_t dim);
+// Arrow C Device data interface (arrow_device.cpp).
+rf_table* table_import_device(const ArrowSchema* schema, ArrowDeviceArray* arr, int64_t dim);
+void table_export_device(const rf_table* t, ArrowSchema* schema, ArrowDeviceArray* arr);
+bool format_type_of(const std::string& f, Type& t);
+std::string arrow_format_of(const Type& t);
 rf_table* table_wrap(int64_t dim, int64_t n_cols, int64_t n_rows, const char* const* formats,
                      void* const* values, void* const* validity);
 void table_build_index(rf_table* t);

@@ -57,6 +57,24 @@ ArrowArray._fields_ = [
     ("private_data", ctypes.c_void_p)]
 
 
+class ArrowDeviceArray(ctypes.Structure):
+    """Arrow C Device data interface: an ArrowArray and the device holding it."""
+    _fields_ = [("array", ArrowArray), ("device_id", ctypes.c_int64),
+                ("device_type", ctypes.c_int32), ("sync_event", ctypes.c_void_p),
+                ("reserved", ctypes.c_int64 * 3)]
+
+
+ARROW_DEVICE_CPU = 1
+ARROW_DEVICE_ROCM = 10
+
+
+def _release(struct_ptr, typ):
+    """Calls an Arrow C struct's release callback (if it has one)."""
+    rel = struct_ptr.contents.release
+    if rel:
+        ctypes.CFUNCTYPE(None, ctypes.POINTER(typ))(rel)(struct_ptr)
+
+
 class _ArgU(ctypes.Union):
     _fields_ = [("table", ctypes.c_void_p), ("i64", ctypes.c_int64), ("f64", ctypes.c_double)]
 
@@ -83,6 +101,11 @@ _SIGS = [
     ("rf_sync", ctypes.c_int, []),
     ("rf_table_upload", ctypes.c_int, [ctypes.POINTER(ArrowSchema), ctypes.POINTER(ArrowArray),
                                        ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]),
+    ("rf_table_import_device", ctypes.c_int, [ctypes.POINTER(ArrowSchema),
+                                              ctypes.POINTER(ArrowDeviceArray), ctypes.c_int64,
+                                              ctypes.POINTER(ctypes.c_void_p)]),
+    ("rf_table_export_device", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ArrowSchema),
+                                              ctypes.POINTER(ArrowDeviceArray)]),
     ("rf_table_wrap_device", ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                             ctypes.POINTER(ctypes.c_char_p),
                                             ctypes.POINTER(ctypes.c_void_p),
@@ -295,6 +318,38 @@ class DeviceTable:
         t.dim = dim
         return t
 
+    @classmethod
+    def from_arrow_device(cls, schema: "ArrowSchema", array: "ArrowDeviceArray",
+                          dim: int = 1) -> "DeviceTable":
+        """Imports an Arrow C Device array (moved in: its release callback
+        runs here or when the table is freed; the schema stays the caller's).
+        ROCm memory is borrowed, host memory uploaded."""
+        out = ctypes.c_void_p()
+        _ck(lib().rf_table_import_device(ctypes.byref(schema), ctypes.byref(array), dim,
+                                         ctypes.byref(out)))
+        t = cls(out.value)
+        t.dim = dim
+        return t
+
+    @classmethod
+    def from_arrow_c_device(cls, batch, dim: int = 1) -> "DeviceTable":
+        """Uploads a pyarrow RecordBatch through the Arrow C Device data
+        interface (pyarrow exports host batches as device type CPU)."""
+        s = ArrowSchema()
+        a = ArrowDeviceArray()
+        batch._export_to_c_device(ctypes.addressof(a), ctypes.addressof(s))
+        try:
+            return cls.from_arrow_device(s, a, dim)
+        finally:
+            _release(ctypes.pointer(s), ArrowSchema)
+
+    def export_device(self) -> "DeviceExport":
+        """This table as an Arrow C Device array (ROCm, zero copy)."""
+        e = DeviceExport()
+        _ck(lib().rf_table_export_device(self.handle, ctypes.byref(e.schema),
+                                         ctypes.byref(e.array)))
+        return e
+
     @property
     def dtype(self) -> DataType:
         """The reffine vector type (ArrowTable2::get_data_type, table.h:64-80)."""
@@ -393,6 +448,30 @@ def _decode(buf: bytes, t: DataType, off: int = 0):
         return DeviceTable(h) if h else None
     v = np.frombuffer(buf, dtype=_NP_OF[t.btype], count=1, offset=off)[0]
     return v.item()
+
+
+class DeviceExport:
+    """An exported (schema, ArrowDeviceArray) pair; releases both when
+    collected unless moved (import_as_table)."""
+
+    def __init__(self):
+        self.schema = ArrowSchema()
+        self.array = ArrowDeviceArray()
+
+    def import_as_table(self, dim: int = 1) -> "DeviceTable":
+        """Moves the array into a new device table that borrows its buffers."""
+        return DeviceTable.from_arrow_device(self.schema, self.array, dim)
+
+    def __del__(self):
+        try:
+            if self.array.array.release:
+                ctypes.CFUNCTYPE(None, ctypes.POINTER(ArrowArray))(self.array.array.release)(
+                    ctypes.byref(self.array.array))
+            if self.schema.release:
+                ctypes.CFUNCTYPE(None, ctypes.POINTER(ArrowSchema))(self.schema.release)(
+                    ctypes.byref(self.schema))
+        except Exception:
+            pass
 
 
 class Query:

@@ -131,3 +131,93 @@ def test_sliced_run_end_column(device, off, ln):
         for val in vs[starts[r]:ends[r]][m]:
             acc = acc + float(val)
         assert np.float64(sv[r]).view(np.int64) == np.float64(acc).view(np.int64)
+
+
+def _table_with_nulls(n=5000, seed=11):
+    rng = np.random.default_rng(seed)
+    cols = [np.arange(n, dtype=np.int64), rng.integers(-9, 9, n).astype(np.int64), rng.random(n),
+            rng.integers(0, 100, n).astype(np.int32)]
+    valid = [None, None, rng.random(n) > 0.3, None]
+    return cols, valid
+
+
+def test_device_export_import_round_trip(device):
+    """A table exported as an Arrow C Device array (ROCm, zero copy) and
+    imported again borrows the same buffers, keeps them alive after the
+    original is freed, and gives the same columns, validity and query
+    results."""
+    from reffine_amd import runtime, workloads
+    cols, valid = _table_with_nulls()
+    t = DeviceTable.from_columns(cols, dim=2, validity=valid)
+    e = t.export_device()
+    assert e.array.device_type == runtime.ARROW_DEVICE_ROCM and e.array.device_id == 0
+    assert e.schema.format == b"+s" and e.schema.n_children == 4
+    assert e.array.array.length == len(cols[0]) and e.array.sync_event is None
+    assert e.schema.children[1].contents.format == b"l"
+    assert e.schema.children[2].contents.format == b"g"
+    ptr = t.device_ptr(2)
+    q = compile_op(workloads.scan_agg_c1(0, 60, 0.9))
+    want = q(t)
+    u = e.import_as_table(dim=2)
+    assert u.device_ptr(2) == ptr  # borrowed, not copied
+    t.free()
+    del t
+    for c in range(4):
+        assert np.array_equal(u.column(c), cols[c]) if valid[c] is None else \
+            np.array_equal(u.column(c)[valid[c]], cols[c][valid[c]])
+    assert np.array_equal(u.validity(2), valid[2])
+    got = q(u)
+    assert got[0] == want[0]
+    assert np.float64(got[1]).view(np.int64) == np.float64(want[1]).view(np.int64)
+
+
+@pytest.mark.parametrize("off", [16, 3])
+def test_device_import_of_a_slice(device, off):
+    """A device array whose struct offset keeps the kernels' load granules
+    (16 rows of 8 bytes, 16 validity bits) is borrowed; any other offset is
+    imported by a copy. Both hold exactly the slice's rows."""
+    cols, valid = _table_with_nulls(seed=12)
+    n = len(cols[0])
+    t = DeviceTable.from_columns(cols, dim=1, validity=valid)
+    e = t.export_device()
+    e.array.array.offset = off
+    e.array.array.length = n - off - 5
+    u = e.import_as_table()
+    ln = n - off - 5
+    assert u.nrows == ln
+    s = slice(off, off + ln)
+    if off % 16 == 0:
+        assert u.device_ptr(1) == t.device_ptr(1) + off * 8
+    else:
+        assert u.device_ptr(1) != t.device_ptr(1) + off * 8
+    assert np.array_equal(u.column(1), cols[1][s])
+    assert np.array_equal(u.column(3), cols[3][s])
+    assert np.array_equal(u.validity(2), valid[2][s])
+    m = valid[2][s]
+    assert np.array_equal(u.column(2)[m], cols[2][s][m])
+
+
+def test_host_batch_through_the_device_interface(device):
+    """pyarrow exports a host batch as an Arrow C Device array of device
+    type CPU; it is uploaded like rf_table_upload, slices included."""
+    batch, cols, valid = _batch(n=3000, seed=5)
+    for off, ln in ((0, 3000), (37, 1000)):
+        t = DeviceTable.from_arrow_c_device(batch.slice(off, ln))
+        assert t.nrows == ln
+        assert np.array_equal(t.column(0), cols["a"][off:off + ln])
+        assert np.array_equal(t.validity(1), valid["b"][off:off + ln])
+
+
+def test_run_end_table_exports_and_imports(device):
+    """A run-end column exports as a "+r" child (run ends + values) and
+    imports again (by a copy: run ends are not borrowed) with the same
+    run reduction result."""
+    import ref_queries as rq
+    t = DeviceTable.from_arrow(rq.runend_table(), dim=2)
+    e = t.export_device()
+    assert e.schema.children[0].contents.format == b"+r"
+    u = e.import_as_table(dim=2)
+    q = compile_op(rq.multidim_op())
+    a, b = q(t), q(u)
+    for c in range(a.ncols):
+        assert np.array_equal(a.column(c), b.column(c))
