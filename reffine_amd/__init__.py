@@ -7,4 +7,4 @@ select / hash-probe queries.
 """
 from .reffiner import *  # noqa: F401,F403
 from .runtime import (DeviceTable, Query, ReffineError, compile_check, compile_op,  # noqa: F401
-                      device_info, init)
+                      device_info, init, load_arrow_file)

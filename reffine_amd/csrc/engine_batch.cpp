@@ -154,11 +154,13 @@ static void plan_unit(rf_batch::Unit& u, rf_query* q, const rf_arg* args, int64_
     u.q = q;
     u.args.assign(args, args + n_args);
     if (q->plan.seg) throw Unsupported("run reductions do not run in a batch");
+    // The query's spec cache (and a group join's build cache) may be used by
+    // a plain run on another thread.
+    std::lock_guard<std::mutex> lk(q->mu);
     if (!q->gj) {
         u.l = prepare_args(q, u.args.data(), n_args);
         return;
     }
-    std::lock_guard<std::mutex> lk(q->mu);
     check_args(q->plan, u.args.data(), n_args);
     u.build = gj_build(q, u.args.data(), n_args);
     u.fold = &gj_fold_spec(q, u.args.data(), n_args);

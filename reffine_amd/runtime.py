@@ -592,3 +592,25 @@ def compile_check(func: Func, deterministic: bool = False, nullable: bool = Fals
     flags = (RF_COMPILE_DETERMINISTIC if deterministic else 0) | (2 if nullable else 0)
     _ck(lib().rf_compile_check(func.ir().encode(), flags, buf, len(buf)))
     return buf.value.decode()
+
+
+def load_arrow_file(path: str, dim: int = 1) -> DeviceTable:
+    """Reads an Arrow IPC file (file or stream format) and uploads it as one
+    device table -- the counterpart of the reference's load_arrow_file
+    (src/utils/utils.cpp:11-34), which reads the same files into a host
+    ArrowTable2. Record batches of the file are combined into one."""
+    import pyarrow as pa
+    import pyarrow.ipc as ipc
+    try:
+        with ipc.open_file(path) as r:
+            table = r.read_all()
+    except pa.ArrowInvalid:
+        with ipc.open_stream(path) as r:
+            table = r.read_all()
+    batches = table.combine_chunks().to_batches()
+    if not batches:
+        batch = pa.RecordBatch.from_arrays([pa.array([], type=f.type) for f in table.schema],
+                                           schema=table.schema)
+    else:
+        batch = batches[0]
+    return DeviceTable.from_arrow(batch, dim)

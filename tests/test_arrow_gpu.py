@@ -221,3 +221,22 @@ def test_run_end_table_exports_and_imports(device):
     a, b = q(t), q(u)
     for c in range(a.ncols):
         assert np.array_equal(a.column(c), b.column(c))
+
+
+@pytest.mark.parametrize("fmt", ["file", "stream"])
+def test_load_arrow_file(device, tmp_path, fmt):
+    """An Arrow IPC file (several record batches, nulls) loads into one
+    device table holding all its rows."""
+    import pyarrow as pa
+    import pyarrow.ipc as ipc
+    from reffine_amd import load_arrow_file
+    batch, cols, valid = _batch(n=4000, seed=8)
+    path = str(tmp_path / f"t.{fmt}")
+    opener = ipc.new_file if fmt == "file" else ipc.new_stream
+    with opener(path, batch.schema) as w:
+        for off in (0, 1000, 2500):
+            w.write_batch(batch.slice(off, {0: 1000, 1000: 1500, 2500: 1500}[off]))
+    t = load_arrow_file(path)
+    assert t.nrows == 4000
+    assert np.array_equal(t.column(0), cols["a"])
+    assert np.array_equal(t.validity(3), valid["d"])
