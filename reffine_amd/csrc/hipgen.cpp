@@ -888,15 +888,15 @@ KernelSource gen_group_fold(const GroupJoinPlan& gj, const std::vector<std::vect
          "    for (int64_t g = blockIdx.x * (int64_t)256 + threadIdx.x; g < total;\n"
          "         g += (int64_t)gridDim.x * 256) {\n"
       << find_run
-      << "        rf_gj_fold_one(B.list[lo], g - B.gstart[lo]);\n"
+      << "        rf_gj_fold_one(*(const RfFoldArgs*)(B.list + lo), g - B.gstart[lo]);\n"
          "    }\n"
          "}\n";
     o << "extern \"C\" __global__ __launch_bounds__(256) void rf_gj_fold_wb(RfFoldBatch B) {\n"
          "    const int64_t total = B.gstart[B.n];\n"
-         "    for (int64_t g = blockIdx.x * (int64_t)4 + (threadIdx.x >> 6); g < total;\n"
+         "    for (int64_t g = rf_scalar64(blockIdx.x * (int64_t)4 + (threadIdx.x >> 6)); g < total;\n"
          "         g += (int64_t)gridDim.x * 4) {\n"
       << find_run
-      << "        rf_gj_fold_wave(B.list[lo], g - B.gstart[lo]);\n"
+      << "        rf_gj_fold_wave(*(const RfFoldArgs*)(B.list + rf_scalar64(lo)), g - B.gstart[lo]);\n"
          "    }\n"
          "}\n";
     ks.code = o.str();

@@ -87,11 +87,21 @@ struct RfFoldArgs {
     union RfParam p[RF_MAXPARAMS];
 };
 
+// Device code reads a batch's unit arguments through the constant address
+// space (no lane writes them during a launch), so uniform reads become
+// scalar loads that later global stores do not force to repeat. The host
+// sees a plain pointer of the same size.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define RF_UNIT_AS __attribute__((address_space(4)))
+#else
+#define RF_UNIT_AS
+#endif
+
 // Folds of several group join runs in one launch (rf_gj_fold_b): run u
 // owns groups [gstart[u], gstart[u+1]) of the launch, its own arguments in
 // list[u] (group indexes in list[u] are local to the run).
 struct RfFoldBatch {
-    const struct RfFoldArgs* list;
+    const RF_UNIT_AS struct RfFoldArgs* list;
     const int64_t* gstart;  // n + 1 cumulative group counts
     int64_t n;
 };
@@ -127,7 +137,7 @@ struct RfBatchHdr {
 // header, written per round, says where this launch's units are.
 struct RfBatchArgs {
     const struct RfBatchHdr* hdr;
-    const struct RfScanArgs* work;     // the round's units
+    const RF_UNIT_AS struct RfScanArgs* work;  // the round's units
     const int64_t* item_start;         // per launch: nunits + 1 cumulative tile counts
 };
 

@@ -529,7 +529,7 @@ __device__ __forceinline__ void rf_batch_scan(const RfBatchArgs& B) {
     constexpr int64_t TILE = (int64_t)SUB * 64 * ITEMS;
     const uint32_t lane = __lane_id();
     const int64_t nunits = B.hdr->nunits, nitems = B.hdr->nitems;
-    const RfScanArgs* work = B.work + B.hdr->unit_base;
+    const RfScanArgs* work = (const RfScanArgs*)(B.work + B.hdr->unit_base);
     const int64_t* item_start = B.item_start + B.hdr->start_base;
     // Items are the units' wave-tiles, concatenated; wave w takes the
     // contiguous range [w*per, (w+1)*per) (one counter shared by every wave
@@ -585,7 +585,7 @@ __device__ __forceinline__ void rf_batch_done(const RfBatchArgs& B) {
     __shared__ uint64_t s_scan[BLOCK / 64];
     const int64_t u = blockIdx.x;
     if (u >= B.hdr->nunits) return;
-    const RfScanArgs& L = B.work[B.hdr->unit_base + u];
+    const RfScanArgs& L = *(const RfScanArgs*)(B.work + B.hdr->unit_base + u);
     if constexpr (Q::NACC > 0) {
         const RF_GLOBAL Acc* parts = rf_g(reinterpret_cast<const Acc*>(L.partials));
         Acc a;
@@ -623,7 +623,7 @@ __device__ __forceinline__ void rf_batch_gather(const RfBatchArgs& B) {
     constexpr int64_t TILE = (int64_t)Q::SUB * 64 * Q::ITEMS;
     const uint32_t lane = __lane_id();
     const int64_t nunits = B.hdr->nunits, nitems = B.hdr->nitems;
-    const RfScanArgs* work = B.work + B.hdr->unit_base;
+    const RfScanArgs* work = (const RfScanArgs*)(B.work + B.hdr->unit_base);
     const int64_t* item_start = B.item_start + B.hdr->start_base;
     const int64_t gw = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x / 64);
