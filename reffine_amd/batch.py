@@ -15,7 +15,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .runtime import DeviceTable, Query, ReffineError, _ck, _decode, lib
+from .runtime import DeviceTable, Query, ReffineError, _ck, lib
 
 RF_ERR_BUSY = 6
 
@@ -38,7 +38,7 @@ def _materialize(q: Query, raw: bytes, tbl: int | None):
         cols = [t.column(c) for c in range(t.ncols)]
         t.free()
         return cols
-    val = _decode(raw, q.out_type)
+    val = q.decode(raw)
 
     def conv(v):
         if isinstance(v, tuple):

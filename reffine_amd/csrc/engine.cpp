@@ -1501,14 +1501,16 @@ void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_add
         for (size_t j = 0; j < ks.out_types.size(); j++) l.args.out[j] = sel_t->cols[j].values;
         l.args.out_cap = sel_cap;
     }
-    launch(q, l, d.stream, false);
+    // The completing block writes the reduction result and the selection
+    // count straight into the query's pinned host stage (mapped into the
+    // device's address space): no copy kernel follows the scan, and the
+    // values are the host's once the stream is synchronised.
     char* stage = (char*)q->host_stage;
-    if (!l.empty) {
-        if (ks.acc_bytes)
-            RF_HIP(hipMemcpyAsync(stage, q->result, ks.acc_bytes, hipMemcpyDeviceToHost, d.stream));
-        if (ks.has_sel)
-            RF_HIP(hipMemcpyAsync(stage + 4096, q->out_count, 8, hipMemcpyDeviceToHost, d.stream));
-    }
+    if (!q->host_stage_dev) RF_HIP(hipHostGetDevicePointer(&q->host_stage_dev, q->host_stage, 0));
+    if ((size_t)ks.acc_bytes > 4096) throw Unsupported("reduction state larger than 4 KiB");
+    l.args.result = q->host_stage_dev;
+    l.args.out_count = (int64_t*)((char*)q->host_stage_dev + 4096);
+    launch(q, l, d.stream, false);
     RF_HIP(hipStreamSynchronize(d.stream));
     rf_table* sel = nullptr;
     if (!q->plan.ops.empty()) {

@@ -163,7 +163,8 @@ struct rf_query {
     size_t out_rows_cap = 0;
     size_t stage_rows_cap = 0;
     int64_t sel_hint = 0;         // selection rows of the last plain run
-    void* host_stage = nullptr;  // pinned: result Acc + count
+    void* host_stage = nullptr;  // pinned: result Acc [0, 4096) + count at 4096
+    void* host_stage_dev = nullptr;  // the same memory as the device addresses it
     std::mutex mu;
     ~rf_query();
 };

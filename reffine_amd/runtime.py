@@ -18,6 +18,7 @@ falls back to a CPU path.
 from __future__ import annotations
 
 import ctypes
+import struct
 import os
 from typing import Sequence
 
@@ -434,20 +435,56 @@ _NP_OF = {"b": np.bool_, "i8": np.int8, "u8": np.uint8, "i16": np.int16, "u16": 
           "f32": np.float32, "f64": np.float64, "x": np.int64}
 
 
+_SC = {"b": "?", "i8": "b", "u8": "B", "i16": "h", "u16": "H", "i32": "i", "u32": "I",
+       "f32": "f", "i64": "q", "u64": "Q", "f64": "d", "x": "q"}
+
+
+class _Decoder:
+    """Reads an output struct with one struct.unpack_from (fields at their C
+    offsets, padding skipped) and nests the values as the type does."""
+
+    def __init__(self, t: DataType):
+        fmt, leaves, self.vec = ["<"], [], []
+        pos = 0
+
+        def walk(tp, off):
+            nonlocal pos
+            if tp.is_struct():
+                out = []
+                for d in tp.dtypes:
+                    sz, al = _layout(d)
+                    off = (off + al - 1) // al * al
+                    out.append(walk(d, off))
+                    off += sz
+                return tuple(out)
+            if off > pos:
+                fmt.append(f"{off - pos}x")
+            sz, _ = _layout(tp)
+            fmt.append("Q" if tp.is_vector() else _SC[tp.btype])
+            pos = off + sz
+            leaves.append(tp.is_vector())
+            return len(leaves) - 1
+
+        self.tpl = walk(t, 0)
+        self.vec = leaves
+        self.st = struct.Struct("".join(fmt))
+
+    def __call__(self, buf: bytes):
+        vals = self.st.unpack_from(buf)
+
+        def build(x):
+            if isinstance(x, tuple):
+                return tuple(build(y) for y in x)
+            v = vals[x]
+            if self.vec[x]:
+                return DeviceTable(v) if v else None
+            return v
+        return build(self.tpl)
+
+
 def _decode(buf: bytes, t: DataType, off: int = 0):
-    if t.is_struct():
-        vals = []
-        for d in t.dtypes:
-            s, a = _layout(d)
-            off = (off + a - 1) // a * a
-            vals.append(_decode(buf, d, off))
-            off += s
-        return tuple(vals)
-    if t.is_vector():
-        h = int.from_bytes(buf[off:off + 8], "little")
-        return DeviceTable(h) if h else None
-    v = np.frombuffer(buf, dtype=_NP_OF[t.btype], count=1, offset=off)[0]
-    return v.item()
+    """Decodes an output struct at offset `off` of `buf`."""
+    return _Decoder(t)(buf[off:])
 
 
 class DeviceExport:
@@ -520,7 +557,7 @@ class Query:
         buf = ctypes.create_string_buffer(max(8, self.out_bytes))
         tbl = ctypes.c_void_p()
         # A struct output carries its Op tables inside the struct (decoded
-        # by _decode); only a vector output comes back through out_table.
+        # by Query.decode); only a vector output comes back through out_table.
         _ck(lib().rf_query_run(self.handle, arr, len(inputs), buf,
                                ctypes.byref(tbl) if self.out_is_table else None))
         return buf.raw[:self.out_bytes], (DeviceTable(tbl.value) if tbl.value else None)
@@ -529,8 +566,14 @@ class Query:
         raw, tbl = self.run_raw(*inputs)
         if self.out_is_table:
             return tbl
-        val = _decode(raw, self.out_type)
-        return val
+        return self.decode(raw)
+
+    def decode(self, raw: bytes):
+        """The function output from the bytes of its output struct."""
+        dec = getattr(self, "_decoder", None)
+        if dec is None:
+            dec = self._decoder = _Decoder(self.out_type)
+        return dec(raw)
 
     def time(self, *inputs, iters: int = 20) -> float:
         """Mean device ms of a run (all of the query's kernels)."""
