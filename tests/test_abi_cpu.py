@@ -121,3 +121,36 @@ def test_union_space_is_rejected():
     with pytest.raises(runtime.ReffineError) as e:
         runtime.compile_check(fn)
     assert e.value.status == 2, e.value  # RF_ERR_UNSUPPORTED
+
+
+def test_output_struct_decoder_matches_c_layout():
+    """The precompiled output decoder reads each field at its C offset
+    (natural alignment, declaration order: ScalarPass,
+    src/pass/scalarpass.cpp:103-227) -- checked against a field-by-field
+    numpy read on random bytes for the workloads' nested output structs."""
+    import numpy as np
+    from reffine_amd.runtime import _NP_OF, _Decoder, _layout
+    from reffine_amd import workloads as wl
+
+    def slow(buf, t, off=0):
+        if t.is_struct():
+            out = []
+            for d in t.dtypes:
+                s, a = _layout(d)
+                off = (off + a - 1) // a * a
+                out.append(slow(buf, d, off))
+                off += s
+            return tuple(out)
+        if t.is_vector():
+            return int.from_bytes(buf[off:off + 8], "little")
+        return np.frombuffer(buf, dtype=_NP_OF[t.btype], count=1, offset=off)[0].item()
+
+    rng = np.random.default_rng(4)
+    for fn in (wl.group_agg_c3(), wl.scan_select_c2(), wl.scan_agg_c1(), wl.q6_template()):
+        t = fn.output.type
+        size, _ = _layout(t)
+        dec = _Decoder(t)
+        dec.vec = [False] * len(dec.vec)  # table handles compared as integers
+        for _ in range(10):
+            buf = rng.integers(0, 256, size, dtype=np.uint8).tobytes()
+            assert repr(dec(buf)) == repr(slow(buf, t))
