@@ -87,5 +87,20 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+C_ABI_TEST = os.path.join(HERE, "..", "tests", "c_abi")
+
+
+def build_c_abi_test() -> str:
+    """Compiles the C host program of tests/c_abi against the library (gcc,
+    no Python, no HIP headers: only include/reffine_hip.h)."""
+    out_dir = os.path.join(C_ABI_TEST, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, "rf_c_scan")
+    _run(["gcc", "-O2", "-std=c11", "-Wall", f"-I{os.path.join(HERE, '..', 'include')}",
+          os.path.join(C_ABI_TEST, "rf_c_scan.c"), "-o", exe, f"-L{BUILD}", "-lreffine_hip",
+          f"-Wl,-rpath,{BUILD}", "-lm"])
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))

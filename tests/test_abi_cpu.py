@@ -154,3 +154,15 @@ def test_output_struct_decoder_matches_c_layout():
         for _ in range(10):
             buf = rng.integers(0, 256, size, dtype=np.uint8).tobytes()
             assert repr(dec(buf)) == repr(slow(buf, t))
+
+
+def test_c_host_program_compiles_against_the_header():
+    """tests/c_abi/rf_c_scan.c, a C host of the ABI, compiles against
+    include/reffine_hip.h alone (C11, warnings as errors)."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run(["gcc", "-fsyntax-only", "-std=c11", "-Wall", "-Werror",
+                        "-I", os.path.join(root, "include"),
+                        os.path.join(root, "tests", "c_abi", "rf_c_scan.c")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
