@@ -1208,7 +1208,41 @@ static std::shared_ptr<GjBuild> make_build(rf_query* q, const rf_arg* args, int6
     const int64_t kmin = (int64_t)(mm[0] ^ 0x8000000000000000ull);
     const int64_t kmax = (int64_t)(mm[1] ^ 0x8000000000000000ull);
 
-    // 2. Sort the build keys (ranks = positions in key order).
+    const uint64_t range = (uint64_t)kmax - (uint64_t)kmin + 1;
+    const char* dup_msg =
+        "group join: the build table holds a qualifying key more than once "
+        "(each group key must occur in one build row)";
+
+    // 2-3 with dense keys: a rank map (a bit per key of the range and a
+    // count per 64 bits: 1.5 bits per slot) when the key range is dense --
+    // at most 64 slots per key plus 2^20, so it stays smaller than an index
+    // and fits the caches. It is built from the unsorted keys; a key's rank
+    // (marked keys below it) is its position in key order, so the keys and
+    // rows are placed by rank with no sort. Fewer marked keys than rows means
+    // a duplicate key.
+    if (kmin != RF_EMPTY_KEY && range != 0 && range <= 64 * (uint64_t)K + ((uint64_t)1 << 20) &&
+        range < ((uint64_t)1 << 40)) {
+        const int64_t words = (int64_t)((range + 63) / 64);
+        B->dense_bits = dev_alloc((size_t)words * 8);
+        B->dense_cum = dev_alloc((size_t)aot_rank_cum_entries(words) * 4);
+        B->dense_len = range;
+        B->keys = dev_alloc((size_t)K * 8);
+        B->rows = dev_alloc((size_t)K * 8);
+        dev_fill_u64(B->dense_bits, 0, words);
+        RF_HIP(aot_rank_build((const int64_t*)bq->outs[0], K, kmin, B->dense_bits, words,
+                              B->dense_cum, s));
+        RF_HIP(aot_rank_scatter((const int64_t*)bq->outs[0], (const int64_t*)bq->outs[1], K, kmin,
+                                B->dense_bits, words, B->dense_cum, (int64_t*)B->keys,
+                                (int64_t*)B->rows, small + 2, s));
+        RF_HIP(hipMemcpyAsync(hs, small + 2, 8, hipMemcpyDeviceToHost, s));
+        RF_HIP(hipStreamSynchronize(s));
+        if (*(uint64_t*)hs != (uint64_t)K) throw IRError(dup_msg);
+        B->groups = K;
+        B->kmin = kmin;
+        return B;
+    }
+
+    // 2. Sparse keys: sort the build keys (ranks = positions in key order).
     if ((size_t)K * 8 > G.keys_cap || !G.keys2) {
         dev_free(G.keys2);
         dev_free(G.rows2);
@@ -1231,29 +1265,8 @@ static std::shared_ptr<GjBuild> make_build(rf_query* q, const rf_arg* args, int6
     RF_HIP(hipMemcpyAsync(B->rows, pb[which][0], (size_t)K * 8, hipMemcpyDeviceToDevice, s));
     const int64_t* skeys = (const int64_t*)B->keys;
 
-    // 3. Key -> rank: a rank map (a bit per key of the range and a count per
-    // 64 bits: 1.5 bits per slot) when the key range is dense -- at most 64
-    // slots per key plus 2^20, so it stays smaller than an index and fits
-    // the caches -- else the open-addressing index and its bitmap filter.
-    const uint64_t range = (uint64_t)kmax - (uint64_t)kmin + 1;
+    // 3. Key -> rank: the open-addressing index and its bitmap filter.
     RF_HIP(aot_adjacent_dups(skeys, K, small + 2, s));
-    if (kmin != RF_EMPTY_KEY && range != 0 && range <= 64 * (uint64_t)K + ((uint64_t)1 << 20) &&
-        range < ((uint64_t)1 << 40)) {
-        const int64_t words = (int64_t)((range + 63) / 64);
-        B->dense_bits = dev_alloc((size_t)words * 8);
-        B->dense_cum = dev_alloc((size_t)aot_rank_cum_entries(words) * 4);
-        B->dense_len = range;
-        dev_fill_u64(B->dense_bits, 0, words);
-        RF_HIP(aot_rank_build(skeys, K, kmin, B->dense_bits, words, B->dense_cum, s));
-        RF_HIP(hipMemcpyAsync(hs, small + 2, 8, hipMemcpyDeviceToHost, s));
-        RF_HIP(hipStreamSynchronize(s));
-        if (*(uint64_t*)hs != 0)
-            throw IRError("group join: the build table holds a qualifying key more than once "
-                          "(each group key must occur in one build row)");
-        B->groups = K;
-        B->kmin = kmin;
-        return B;
-    }
     uint64_t cap = 16;
     while (cap < (uint64_t)G.table_scale * (uint64_t)K) cap <<= 1;
     B->hkeys = dev_alloc(cap * 8);
@@ -1270,9 +1283,7 @@ static std::shared_ptr<GjBuild> make_build(rf_query* q, const rf_arg* args, int6
     RF_HIP(aot_filter_build(skeys, K, B->filter, fbits - 1, s));
     RF_HIP(hipMemcpyAsync(hs, small + 2, 8, hipMemcpyDeviceToHost, s));
     RF_HIP(hipStreamSynchronize(s));
-    if (*(uint64_t*)hs != 0)
-        throw IRError("group join: the build table holds a qualifying key more than once "
-                      "(each group key must occur in one build row)");
+    if (*(uint64_t*)hs != 0) throw IRError(dup_msg);
     B->groups = K;
     B->kmin = kmin;
     B->index_cap = cap;

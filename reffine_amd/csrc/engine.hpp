@@ -290,6 +290,9 @@ int bits_of(uint64_t range);
 // Joined rows per group (on average) from which a wave folds each group.
 constexpr int64_t kWaveFoldRows = 64;
 int64_t aot_rank_cum_entries(int64_t words);
+hipError_t aot_rank_scatter(const int64_t* keys, const int64_t* rows, int64_t n, int64_t base,
+                            const void* bits, int64_t words, const void* cum, int64_t* keys_out,
+                            int64_t* rows_out, void* total, hipStream_t s);
 hipError_t aot_rank_build(const int64_t* keys, int64_t n, int64_t base, void* bits, int64_t words,
                           void* cum, hipStream_t s);
 hipError_t aot_gj_concat(const RfCatSeg* segs, int32_t nseg, int64_t max_rows,

@@ -477,6 +477,34 @@ extern "C" __global__ void rf_rank_bits(const int64_t* __restrict__ keys, int64_
     }
 }
 
+// Build rows in key order without a sort: each distinct key's rank is its
+// position in the rank map (marked keys below it), so (key, row) goes to
+// slot rank. Keys must be distinct (checked by the caller through the count
+// of marked keys).
+extern "C" __global__ void rf_rank_scatter(const int64_t* __restrict__ keys,
+                                           const int64_t* __restrict__ rows, int64_t n, int64_t base,
+                                           const uint64_t* __restrict__ bits,
+                                           const uint32_t* __restrict__ cum, int64_t* keys_out,
+                                           int64_t* rows_out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = keys[i];
+        const uint64_t off = (uint64_t)k - (uint64_t)base;
+        const uint64_t w = bits[off >> 6];
+        const int64_t r = (int64_t)cum[off >> 6] + (int64_t)__popcll(w & ((1ull << (off & 63)) - 1));
+        keys_out[r] = k;
+        rows_out[r] = rows[i];
+    }
+}
+
+// The number of marked keys: the last word's first rank plus its popcount.
+extern "C" __global__ void rf_rank_total(const uint64_t* __restrict__ bits,
+                                         const uint32_t* __restrict__ cum, int64_t words,
+                                         uint64_t* out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *out = (uint64_t)cum[words - 1] + (uint64_t)__popcll(bits[words - 1]);
+}
+
 extern "C" __global__ void rf_word_popc(const uint64_t* __restrict__ bits, int64_t words,
                                         uint32_t* cnt) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < words;
@@ -628,6 +656,19 @@ hipError_t aot_seg_bounds(const uint32_t* rank, int64_t n, int64_t* lo, int64_t*
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(rf_seg_bounds, dim3(grid_cap(n, 256, 16384)), dim3(256), 0, s, rank, n,
                        lo, hi);
+    return hipGetLastError();
+}
+
+// Keys and rows in key order from unsorted distinct keys and a built rank
+// map, and the number of marked keys into *total.
+hipError_t aot_rank_scatter(const int64_t* keys, const int64_t* rows, int64_t n, int64_t base,
+                            const void* bits, int64_t words, const void* cum, int64_t* keys_out,
+                            int64_t* rows_out, void* total, hipStream_t s) {
+    if (n <= 0 || words <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rf_rank_scatter, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, rows, n,
+                       base, (const uint64_t*)bits, (const uint32_t*)cum, keys_out, rows_out);
+    hipLaunchKernelGGL(rf_rank_total, dim3(1), dim3(64), 0, s, (const uint64_t*)bits,
+                       (const uint32_t*)cum, words, (uint64_t*)total);
     return hipGetLastError();
 }
 
