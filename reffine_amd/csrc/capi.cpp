@@ -211,6 +211,23 @@ static void compile_impl(const char* ir_text, uint32_t flags, const int32_t* tun
         sig += std::string(q->plan.inputs[i]->type.dtypes.size(), '0') + "|";
     }
     q->specs.emplace(sig, std::move(sp));
+    if (q->plan.runs) {
+        // The run stage's fold over T with its own (run-end) type.
+        const rf::RunStagePlan& rs = *q->plan.runs;
+        std::vector<std::vector<bool>> rnl = nl;
+        rnl[rs.table].assign(rs.input->type.dtypes.size(), false);
+        rf::Spec rsp;
+        rsp.ks = rf::gen_seg_fold(rs.seg, q->plan, rnl);
+        rsp.mod = &rf::jit_module(rsp.ks);
+        std::string rsig;
+        for (size_t i = 0; i < q->plan.inputs.size(); i++) {
+            if (!q->plan.inputs[i]->type.is_vector()) continue;
+            const size_t nc = (int)i == rs.table ? rs.input->type.dtypes.size()
+                                                 : q->plan.inputs[i]->type.dtypes.size();
+            rsig += std::string(nc, '0') + "|";
+        }
+        q->run_specs.emplace(rsig, std::move(rsp));
+    }
     *out = q.release();
 }
 
@@ -241,6 +258,11 @@ int rf_compile_check(const char* ir_text, uint32_t flags, char* buf, size_t cap)
             kss.push_back(rf::gen_seg_fold(*plan.seg, plan, nl));
         else
             kss.push_back(rf::gen_scan(plan, nl, tune, (flags & RF_COMPILE_DETERMINISTIC) != 0));
+        if (plan.runs) {
+            std::vector<std::vector<bool>> rnl = nl;
+            rnl[plan.runs->table].assign(plan.runs->input->type.dtypes.size(), (flags & 2u) != 0);
+            kss.insert(kss.begin(), rf::gen_seg_fold(plan.runs->seg, plan, rnl));
+        }
         size_t bytes = 0;
         std::string text;
         for (auto& ks : kss) {

@@ -737,14 +737,15 @@ void c_layout(const Type& t, size_t& size, size_t& align) {
 
 // ----------------------------------------------------------- execution
 
-static std::string null_signature(const Plan& plan, const rf_arg* args, int64_t n_args) {
+static std::string null_signature(const std::vector<N>& inputs, const rf_arg* args,
+                                  int64_t n_args) {
     std::string s;
-    for (size_t i = 0; i < plan.inputs.size(); i++) {
-        if (!plan.inputs[i]->type.is_vector()) continue;
+    for (size_t i = 0; i < inputs.size(); i++) {
+        if (!inputs[i]->type.is_vector()) continue;
         if ((int64_t)i >= n_args || args[i].kind != 0 || !args[i].u.table)
             throw IRError("argument " + std::to_string(i) + " must be a table");
         const rf_table* t = args[i].u.table;
-        const Type& want = plan.inputs[i]->type;
+        const Type& want = inputs[i]->type;
         Type have = t->type();
         if (have != want)
             throw IRError("argument " + std::to_string(i) + " has type " + have.str() +
@@ -755,8 +756,37 @@ static std::string null_signature(const Plan& plan, const rf_arg* args, int64_t 
     return s;
 }
 
+static std::vector<std::vector<bool>> nullable_of(size_t n_inputs, const rf_arg* args,
+                                                  const std::vector<N>& inputs) {
+    std::vector<std::vector<bool>> nl(n_inputs);
+    for (size_t i = 0; i < n_inputs; i++) {
+        if (!inputs[i]->type.is_vector()) continue;
+        for (auto& c : args[i].u.table->cols) nl[i].push_back(c.validity != nullptr);
+    }
+    return nl;
+}
+
+// The run stage's inputs: the function's, with T's own run-end type.
+static std::vector<N> run_stage_inputs(const Plan& plan) {
+    std::vector<N> in = plan.inputs;
+    in[plan.runs->table] = plan.runs->input;
+    return in;
+}
+
+// The run-reduction kernel of a run stage, by the inputs' nullability.
+static Spec& get_run_spec(rf_query* q, const rf_arg* args, int64_t n_args) {
+    const std::vector<N> inputs = run_stage_inputs(q->plan);
+    std::string sig = null_signature(inputs, args, n_args);
+    auto it = q->run_specs.find(sig);
+    if (it != q->run_specs.end()) return it->second;
+    Spec sp;
+    sp.ks = gen_seg_fold(q->plan.runs->seg, q->plan, nullable_of(inputs.size(), args, inputs));
+    sp.mod = &jit_module(sp.ks);
+    return q->run_specs.emplace(sig, std::move(sp)).first->second;
+}
+
 static Spec& get_spec(rf_query* q, const rf_arg* args, int64_t n_args) {
-    std::string sig = null_signature(q->plan, args, n_args);
+    std::string sig = null_signature(q->plan.inputs, args, n_args);
     auto it = q->specs.find(sig);
     if (it != q->specs.end()) return it->second;
     std::vector<std::vector<bool>> nl(q->plan.inputs.size());
@@ -1148,7 +1178,7 @@ std::vector<KernelSource> group_join_sources(const Plan& plan,
 }
 
 Spec& gj_fold_spec(rf_query* q, const rf_arg* args, int64_t n_args) {
-    std::string sig = null_signature(q->plan, args, n_args);
+    std::string sig = null_signature(q->plan.inputs, args, n_args);
     auto& specs = q->gj->fold_specs;
     auto it = specs.find(sig);
     if (it != specs.end()) return it->second;
@@ -1444,44 +1474,86 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
     return out.release();
 }
 
-// Runs a run reduction (SegReducePlan): one output row per run.
-static rf_table* run_seg_reduce(rf_query* q, const rf_arg* args, int64_t n_args) {
-    Device& d = device();
-    check_args(q->plan, args, n_args);
-    const SegReducePlan& sp = *q->plan.seg;
-    Spec& spec = get_spec(q, args, n_args);
-    const rf_table* t = args[sp.table].u.table;
-    const Column& c0 = t->cols.at(0);
-    if (c0.enc != Enc::RUNEND) throw ArrowError("column 0 is not run-end encoded");
-    if (c0.validity) throw Unsupported("null run values");
+// Launches a run reduction (SegReducePlan) on `s`: one row of `out` per run
+// of T's run-end column 0.
+static void launch_seg(rf_query* q, const SegReducePlan& sp, Spec& spec, const rf_arg* args,
+                       int64_t n_args, rf_table* out, hipStream_t s) {
+    const Column& c0 = args[sp.table].u.table->cols.at(0);
     const int64_t runs = c0.length;
-    std::unique_ptr<rf_table> out(alloc_table(sp.out_types, runs, 1));
-    if (runs == 0) return out.release();
+    if (runs == 0) return;
     RfSegArgs A;
     std::memset(&A, 0, sizeof A);
     A.runs = runs;
     A.rows = c0.logical_length;
     A.run_ends = (const int32_t*)c0.run_ends;
     A.keys = c0.values;
-    for (size_t s = 0; s < spec.ks.cols.size(); s++) {
-        const ColSlot& cs = spec.ks.cols[s];
+    for (size_t k = 0; k < spec.ks.cols.size(); k++) {
+        const ColSlot& cs = spec.ks.cols[k];
         const Column& col = args[cs.table].u.table->cols.at(cs.col);
-        A.col[s] = col.values;
-        A.valid[s] = (const uint16_t*)col.validity;
+        A.col[k] = col.values;
+        A.valid[k] = (const uint16_t*)col.validity;
+    }
+    for (size_t j = 0; j < sp.sub.probes.size(); j++) {
+        if ((int)j >= RF_MAXPROBES) throw Unsupported("more than 4 probed tables");
+        rf_table* pt = args[sp.sub.probes[j].table].u.table;
+        if (!pt->has_index) table_build_index(pt);
+        A.probe[j] = probe_of(pt);
     }
     for (size_t j = 0; j < out->cols.size(); j++) A.out[j] = out->cols[j].values;
     EvalCtx c{&q->plan, args, n_args, {}, {}};
     fill_params(spec.ks, c, A.p);
     void* params[] = {&A};
     const int g = (int)std::min<int64_t>((runs + 255) / 256, 16384);
-    RF_HIP(hipModuleLaunchKernel(spec.mod->fn, g, 1, 1, 256, 1, 1, 0, d.stream, params, nullptr));
+    RF_HIP(hipModuleLaunchKernel(spec.mod->fn, g, 1, 1, 256, 1, 1, 0, s, params, nullptr));
+}
+
+// A new table of one row per run of T (no launch).
+static rf_table* alloc_seg_table(const SegReducePlan& sp, const rf_arg* args) {
+    const Column& c0 = args[sp.table].u.table->cols.at(0);
+    if (c0.enc != Enc::RUNEND) throw ArrowError("column 0 is not run-end encoded");
+    if (c0.validity) throw Unsupported("null run values");
+    return alloc_table(sp.out_types, c0.length, 1);
+}
+
+// Runs a run reduction (SegReducePlan): one output row per run.
+static rf_table* run_seg_reduce(rf_query* q, const rf_arg* args, int64_t n_args) {
+    Device& d = device();
+    check_args(q->plan, args, n_args);
+    Spec& spec = get_spec(q, args, n_args);
+    std::unique_ptr<rf_table> out(alloc_seg_table(*q->plan.seg, args));
+    launch_seg(q, *q->plan.seg, spec, args, n_args, out.get(), d.stream);
     RF_HIP(hipStreamSynchronize(d.stream));
+    return out.release();
+}
+
+// The runs table of a run stage (key, every reduction's fields per run),
+// launched on the library stream (the scan of it follows on that stream).
+static rf_table* run_stage_table(rf_query* q, const rf_arg* args, int64_t n_args) {
+    check_args(q->plan, args, n_args);
+    Spec& spec = get_run_spec(q, args, n_args);
+    const SegReducePlan& sp = q->plan.runs->seg;
+    std::unique_ptr<rf_table> out(alloc_seg_table(sp, args));
+    // The runs table is internal: bool fields keep their type (one byte
+    // each, 0 or 1) where returned tables store them as int8.
+    for (size_t j = 0; j < out->cols.size(); j++)
+        if (sp.out_types[j].bt == BT::BOOL) out->cols[j].type = types::BOOL_T;
+    launch_seg(q, sp, spec, args, n_args, out.get(), device().stream);
     return out.release();
 }
 
 void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_addr,
                rf_table** out_table) {
     std::lock_guard<std::mutex> lk(q->mu);
+    // A run stage first folds T's runs; the plan then scans that table in
+    // T's place (same stream, so no synchronisation in between).
+    std::unique_ptr<rf_table> runs;
+    std::vector<rf_arg> staged;
+    if (q->plan.runs) {
+        runs.reset(run_stage_table(q, args, n_args));
+        staged.assign(args, args + n_args);
+        staged[q->plan.runs->table].u.table = runs.get();
+        args = staged.data();
+    }
     if (q->plan.seg) {
         if (!out_table) throw IRError("an Op output needs out_table");
         rf_table* t = run_seg_reduce(q, args, n_args);
@@ -1618,6 +1690,19 @@ void time_query(rf_query* q, const rf_arg* args, int64_t n_args, int iters, doub
         *ms_scan = scan / iters;
         return;
     }
+    std::unique_ptr<rf_table> runs;
+    std::vector<rf_arg> staged;
+    Spec* run_spec = nullptr;
+    const rf_arg* user_args = args;
+    if (q->plan.runs) {
+        // Timed end to end: the runs table's fold, then the scan of it.
+        runs.reset(run_stage_table(q, args, n_args));
+        RF_HIP(hipStreamSynchronize(d.stream));
+        run_spec = &get_run_spec(q, args, n_args);
+        staged.assign(args, args + n_args);
+        staged[q->plan.runs->table].u.table = runs.get();
+        args = staged.data();
+    }
     if (q->plan.seg) {
         if (iters <= 0) return;
         hipEvent_t e[2];
@@ -1639,7 +1724,11 @@ void time_query(rf_query* q, const rf_arg* args, int64_t n_args, int iters, doub
     for (auto& x : e) RF_HIP(hipEventCreate(&x));
     launch(q, l, d.stream, false);  // warm
     RF_HIP(hipEventRecord(e[0], d.stream));
-    for (int i = 0; i < iters; i++) launch(q, l, d.stream, false);
+    for (int i = 0; i < iters; i++) {
+        if (run_spec)
+            launch_seg(q, q->plan.runs->seg, *run_spec, user_args, n_args, runs.get(), d.stream);
+        launch(q, l, d.stream, false);
+    }
     RF_HIP(hipEventRecord(e[1], d.stream));
     for (int i = 0; i < iters; i++) launch(q, l, d.stream, true);
     RF_HIP(hipEventRecord(e[2], d.stream));

@@ -150,6 +150,7 @@ struct rf_query {
     bool det = false;
     rf::Tuning tune;
     std::map<std::string, rf::Spec> specs;  // by nullability signature
+    std::map<std::string, rf::Spec> run_specs;  // a run stage's fold, likewise
     // Workspace (grow-only, reused across runs).
     void* partials = nullptr;
     size_t partials_bytes = 0;

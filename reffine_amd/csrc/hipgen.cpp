@@ -711,10 +711,12 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
 
 KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
                           const std::vector<std::vector<bool>>& nullable) {
+    (void)plan;
     KernelSource ks;
     ks.entry = "rf_seg_fold";
     ks.tune.block = 256;
-    Gen g(plan, nullable, ks);
+    if (sp.sub.probes.size() > 4) throw Unsupported("more than 4 probed tables");
+    Gen g(sp.sub, nullable, ks);
     std::ostringstream o;
     o << "// Generated run reduction for '" << sp.sym_name << "'.\n";
     o << "extern \"C\" __global__ __launch_bounds__(256) void rf_seg_fold(RfSegArgs L) {\n";
@@ -723,8 +725,12 @@ KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
     o << "        const int64_t lo = run > 0 ? (int64_t)L.run_ends[run - 1] : 0;\n";
     o << "        const int64_t hi0 = (int64_t)L.run_ends[run];\n";
     o << "        const int64_t hi = hi0 < L.rows ? hi0 : L.rows;\n";
+    o << "        const " << cty(sp.key_type) << " key = ((const " << cty(sp.key_type)
+      << "*)L.keys)[run];\n        (void)key;\n";
     std::ostringstream init;
     Emitter ei(g, Mode::FINAL, init, "        ");
+    // Tables read at the run's key: probed once per run, before its rows.
+    for (size_t j = 0; j < sp.sub.probes.size(); j++) ei.probe((int)j);
     std::vector<std::string> iv;
     for (auto& f : sp.fields) iv.push_back(ei.expr(f.init));
     o << init.str();
@@ -736,6 +742,7 @@ KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
     std::ostringstream body;
     Emitter eb(g, Mode::MOVE, body, "            ");
     eb.ntmp = ei.ntmp;
+    eb.probes_done = ei.probes_done;
     std::vector<std::string> pr(sp.fields.size()), ct(sp.fields.size());
     for (size_t f = 0; f < sp.fields.size(); f++) {
         if (!sp.fields[f].updated) continue;
@@ -750,11 +757,10 @@ KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
           << combine_expr(sp.fields[f].op, sp.fields[f].type, S, ct[f]) << " : " << S << ";\n";
     }
     o << "        }\n";
-    o << "        const " << cty(sp.key_type) << " key = ((const " << cty(sp.key_type)
-      << "*)L.keys)[run];\n        (void)key;\n";
     std::ostringstream ob;
     Emitter eo(g, Mode::FINAL, ob, "        ");
     eo.ntmp = eb.ntmp;
+    eo.probes_done = ei.probes_done;
     std::vector<std::string> ov;
     for (auto& v : sp.out_values) ov.push_back(eo.expr(v));
     if (ov.size() > 16) throw Unsupported("more than 16 run reduction outputs");

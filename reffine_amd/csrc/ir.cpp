@@ -237,6 +237,25 @@ N mk_len(N vec, int64_t col) {
     return n;
 }
 
+N mk_locate(N vec, N iter) {
+    check(vec->type.is_vector(), "Locate in non-vector");
+    check(vec->type.dim <= 2, "Locate in a vector of more than 2 dimensions");
+    check(iter->type == vec->type.iterty(), "Locate iterator type mismatch");
+    auto n = std::make_shared<Node>(K::Locate, types::IDX_T);
+    n->args = {vec, iter};
+    return n;
+}
+
+N mk_readdata(N vec, N idx, int64_t col) {
+    check(vec->type.is_vector(), "ReadData of non-vector");
+    check(col >= 0 && col < (int64_t)vec->type.dtypes.size(), "ReadData column out of range");
+    check(idx->type.is_idx() || idx->type.is_int(), "ReadData index must be an integer");
+    auto n = std::make_shared<Node>(K::ReadData, vec->type.dtypes[col]);
+    n->args = {vec, idx};
+    n->idx = col;
+    return n;
+}
+
 N mk_op(std::vector<N> iters, N pred, std::vector<N> outputs) {
     std::vector<Type> d;
     std::vector<Enc> e;
@@ -438,6 +457,8 @@ struct Builder {
         if (h == "in") return mk_in(arg(1), arg(2));
         if (h == "subvec") return mk_subvec(arg(1), arg(2), arg(3));
         if (h == "len") return mk_len(arg(1), integer(e.items.at(2)));
+        if (h == "locate") return mk_locate(arg(1), arg(2));
+        if (h == "readdata") return mk_readdata(arg(1), arg(2), integer(e.items.at(3)));
         if (h == "op") {
             check(e.items.size() == 4, "(op (ITERS) PRED (OUTPUTS))");
             std::vector<N> iters, outs;
@@ -528,6 +549,11 @@ std::string to_text(const N& n) {
                    " " + to_text(n->args[2]) + ")";
         case K::Length:
             return "(len " + to_text(n->args[0]) + " " + std::to_string(n->idx) + ")";
+        case K::Locate:
+            return "(locate " + to_text(n->args[0]) + " " + to_text(n->args[1]) + ")";
+        case K::ReadData:
+            return "(readdata " + to_text(n->args[0]) + " " + to_text(n->args[1]) + " " +
+                   std::to_string(n->idx) + ")";
         case K::Op: {
             std::string s = "(op (";
             for (size_t i = 0; i < n->iters.size(); i++)

@@ -88,7 +88,7 @@ enum class MathOp {
 
 enum class K {
     Sym, Const, Cast, Select, Get, New, Nary, Element, In, SubVector,
-    Length, Op, Reduce, Func
+    Length, Op, Reduce, Func, Locate, ReadData
 };
 
 struct Node;
@@ -122,6 +122,11 @@ N mk_element(N vec, N iter);
 N mk_in(N iter, N vec);
 N mk_subvec(N vec, N start, N end);
 N mk_len(N vec, int64_t col);
+// include/reffine/ir/op_to_loop.h:106-117: the row of `iter` in vec's index
+// (idx, -1 when absent).
+N mk_locate(N vec, N iter);
+// include/reffine/ir/op_to_loop.h:27-41: column `col` of vec at row `idx`.
+N mk_readdata(N vec, N idx, int64_t col);
 N mk_op(std::vector<N> iters, N pred, std::vector<N> outputs);
 N mk_reduce(N vec, N init, N state_sym, N val_sym, N acc);
 N mk_func(const std::string& name, N output, std::vector<N> inputs,

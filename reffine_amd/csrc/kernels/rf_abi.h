@@ -151,5 +151,6 @@ struct RfSegArgs {
     const void* col[RF_MAXCOLS];  // columns the row code and outputs read
     const uint16_t* valid[RF_MAXCOLS];
     void* out[RF_MAXOUT];         // output columns, one row per run
+    struct RfProbeDev probe[RF_MAXPROBES];  // tables read at the run's key
     union RfParam p[RF_MAXPARAMS];
 };

@@ -67,6 +67,9 @@ struct Lowerer {
     N iter_sym;         // Op iterator symbol (VecSpace / range form)
     int driver = -1;
     int driver_dim = 1;
+    // An In(t, T) over this input drives the scan when present (the runs
+    // table of a run stage); otherwise the first In does.
+    int prefer_driver = -1;
     std::map<std::string, int> probe_of;  // probe key text -> probe index
 
     int input_index(const N& sym) {
@@ -213,6 +216,39 @@ struct Lowerer {
                 if (t < 0) throw Unsupported("length of a computed vector");
                 return leaf("$len." + std::to_string(t) + "." + std::to_string(e->idx),
                             types::IDX_T);
+            }
+            case K::Locate: {
+                // vector_locate (include/reffine/vinstr/vinstr.h:137-146): a
+                // probe of the table's index; -1 when the key is absent.
+                N vec = resolve_vec(e->args[0]);
+                int t = input_index(vec);
+                if (t < 0) throw Unsupported("Locate in a computed vector");
+                int j = probe_for(t, rw(e->args[1]));
+                return leaf("$prow." + std::to_string(j), types::IDX_T);
+            }
+            case K::ReadData: {
+                // ReadData(T, i, c): column c (storage index) at row i
+                // (src/pass/readwritepass.cpp:16-24); here i must be a row
+                // found by Locate in the same table, or the scanned row.
+                N vec = resolve_vec(e->args[0]);
+                int t = input_index(vec);
+                if (t < 0) throw Unsupported("ReadData of a computed vector");
+                const Type& vt = plan.inputs[t]->type;
+                const int c = (int)e->idx;
+                if (c < (int)vt.enc.size() && vt.enc[c] == Enc::RUNEND)
+                    throw Unsupported("ReadData of a run-end encoded column");
+                N ix = rw(e->args[1]);
+                const std::string ln = leaf_name(ix);
+                if (ln.rfind("$prow.", 0) == 0) {
+                    int j = std::stoi(ln.substr(6));
+                    if (plan.probes.at(j).table != t)
+                        throw Unsupported("ReadData at a row located in another table");
+                    return leaf("$pcol." + std::to_string(j) + "." + std::to_string(c),
+                                vt.dtypes[c]);
+                }
+                if (ln == "$row" && t == driver && plan.source == Plan::TABLE_ROWS)
+                    return col_leaf(t, c);
+                throw Unsupported("ReadData at a computed row index");
             }
             case K::Reduce: {
                 auto r = reduce_subst.find(to_text(e));
@@ -541,10 +577,8 @@ struct Lowerer {
                 c->args[0]->name == iter_sym->name) {
                 N vec = resolve_vec(c->args[1]);
                 int t = input_index(vec);
-                if (t >= 0) {
-                    drv = t;
-                    break;
-                }
+                if (t >= 0 && (drv < 0 || t == prefer_driver)) drv = t;
+                if (drv >= 0 && (prefer_driver < 0 || drv == prefer_driver)) break;
             }
         }
         Source s;
@@ -890,108 +924,125 @@ struct Lowerer {
 
     // ------------------------------------------------------- run segments
 
-    // The reduction of an Op output whose vector is T[t], T a table with a
-    // run-end encoded column 0 and t the Op's iterator; null otherwise.
+    // T if `red` reduces T[t] (T a table input whose column 0 is run-end
+    // encoded, t a symbol: the run of t's rows, loopgen.cpp:34-38); -1
+    // otherwise.
+    int run_table_of(const N& red, N* it = nullptr) {
+        if (red->kind != K::Reduce) return -1;
+        N v = resolve_vec(red->args[0]);
+        if (v->kind != K::Element || v->args[1]->kind != K::Sym) return -1;
+        int t = input_index(resolve_vec(v->args[0]));
+        if (t < 0) return -1;
+        const Type& vt = plan.inputs[t]->type;
+        if (vt.dim < 2 || vt.enc.empty() || vt.enc[0] != Enc::RUNEND) return -1;
+        if (it) *it = v->args[1];
+        return t;
+    }
+
+    static void check_run_table(const Type& vt) {
+        if (vt.dtypes.size() < 2) throw Unsupported("run-end table without value columns");
+        for (size_t c = 1; c < vt.dtypes.size(); c++)
+            if (c < vt.enc.size() && vt.enc[c] == Enc::RUNEND)
+                throw Unsupported("run-end encoded value column");
+    }
+
+    // The reduction of an Op output whose vector is T[t], with t the Op's
+    // iterator and the Op's space exactly In(t, T); null otherwise.
     N seg_reduce_of(const N& op) {
         if (op->iters.size() != 1) return nullptr;
         std::vector<N> reds;
         std::set<std::string> seen;
         for (size_t i = 1; i < op->args.size(); i++) find_reduces(op->args[i], reds, seen);
         if (reds.size() != 1) return nullptr;
-        N v = resolve_vec(reds[0]->args[0]);
-        if (v->kind != K::Element) return nullptr;
-        N vec = resolve_vec(v->args[0]);
-        int t = input_index(vec);
-        if (t < 0) return nullptr;
-        const Type& vt = plan.inputs[t]->type;
-        if (vt.dim < 2 || vt.enc.empty() || vt.enc[0] != Enc::RUNEND) return nullptr;
-        if (!(v->args[1]->kind == K::Sym && v->args[1]->name == op->iters[0]->name)) return nullptr;
-        return reds[0];
-    }
-
-    void lower_seg_reduce(const N& op, const N& red, const std::string& name) {
-        const N t = op->iters[0];
-        N ev = resolve_vec(red->args[0]);
-        const int T = input_index(resolve_vec(ev->args[0]));
-        const Type& vt = plan.inputs[T]->type;
-        // The Op's space: exactly In(t, T) (VecSpace over the run values).
+        N it;
+        const int T = run_table_of(reds[0], &it);
+        if (T < 0 || it->name != op->iters[0]->name) return nullptr;
         std::vector<N> cj;
         conjuncts(resolve_bool_sym(op->args[0]), cj);
-        bool has_in = false;
         for (auto& c0 : cj) {
             N c = resolve_bool_sym(c0);
-            if (c->kind == K::In && is_sym_named(c->args[0], t->name) &&
-                input_index(resolve_vec(c->args[1])) == T) {
-                has_in = true;
-                continue;
-            }
-            throw Unsupported("run reduction with a predicate beyond In(t, T)");
+            if (!(c->kind == K::In && is_sym_named(c->args[0], it->name) &&
+                  input_index(resolve_vec(c->args[1])) == T))
+                return nullptr;
         }
-        if (!has_in) throw Unsupported("run reduction without In(t, T)");
-        if (vt.dtypes.size() < 2) throw Unsupported("run-end table without value columns");
-        for (size_t c = 1; c < vt.dtypes.size(); c++)
-            if (c < vt.enc.size() && vt.enc[c] == Enc::RUNEND)
-                throw Unsupported("run-end encoded value column");
-        auto sp = std::make_shared<SegReducePlan>();
-        sp->table = T;
-        sp->key_type = vt.dtypes[0];
-        // Fields: the fold over the run's rows (SubVector rows: columns 1..).
-        Lowerer Lr = sibling();
-        Lr.set_driver(T);
+        return cj.empty() ? nullptr : reds[0];
+    }
+
+    // Appends the fields of one run reduction to `fields`: the fold over
+    // the run's rows (columns 1.. of T) under the SubVector body condition,
+    // the validity of column 1 (src/ir/op_to_loop.h:11-25 via
+    // loopgen.cpp:34-38). The run's key may appear as $key (subst).
+    void lower_run_fields(const N& red, int T, std::vector<Field>& fields) {
+        const size_t off = fields.size();
         std::vector<int> path;
-        N stree = Lr.state_tree(red->type, path, sp->fields);
-        Lr.subst[red->args[2]->name] = stree;
-        Lr.subst[red->args[3]->name] = Lr.row_struct(T, 1);
-        Lr.memo.clear();
+        N stree = state_tree(red->type, path, fields);
+        subst[red->args[2]->name] = stree;
+        subst[red->args[3]->name] = row_struct(T, 1);
+        memo.clear();
         N accr;
         try {
-            accr = Lr.rw(red->args[4]);
+            accr = rw(red->args[4]);
         } catch (const IRError& e) {
             throw Unsupported(std::string("run reduction depends on the run beyond its rows: ") +
                               e.what());
         }
         std::vector<N> flat;
         flatten(accr, flat);
-        if (flat.size() != sp->fields.size()) throw IRError("accumulator shape");
+        if (flat.size() != fields.size() - off) throw IRError("accumulator shape");
         std::vector<N> inits;
         flatten(fold_host(red->args[1]), inits);
-        for (size_t f = 0; f < sp->fields.size(); f++) {
-            Dec d = Lr.decompose(flat[f], "$s." + std::to_string(f));
-            Field& fl = sp->fields[f];
+        for (size_t f = 0; f < flat.size(); f++) {
+            Dec d = decompose(flat[f], "$s." + std::to_string(off + f));
+            Field& fl = fields[off + f];
             fl.init = inits.at(f);
             fl.updated = d.upd;
             if (d.upd) {
                 fl.op = d.op;
-                fl.pred = mk_and(Lr.valid_leaf(T, 1), d.pred);
+                fl.pred = mk_and(valid_leaf(T, 1), d.pred);
                 fl.contrib = d.contrib;
                 if (mentions_state(fl.contrib) || mentions_state(fl.pred))
                     throw Unsupported("state field depends on another field");
             }
         }
-        // Outputs per run.
-        Lowerer Lf = sibling();
-        Lf.subst[t->name] = leaf("$key", t->type);
-        std::vector<N> accl;
-        for (size_t f = 0; f < sp->fields.size(); f++)
-            accl.push_back(leaf("$acc." + std::to_string(f), sp->fields[f].type));
+        memo.clear();
+    }
+
+    // A reduction's state rebuilt from one leaf per flattened field.
+    static N nest_fields(const Type& ty, const std::function<N(size_t)>& leaf_of, size_t& k) {
+        if (ty.is_struct()) {
+            std::vector<N> v;
+            for (auto& d : ty.dtypes) v.push_back(nest_fields(d, leaf_of, k));
+            return mk_new(v);
+        }
+        return leaf_of(k++);
+    }
+
+    void lower_seg_reduce(const N& op, const N& red, const std::string& name) {
+        const N t = op->iters[0];
+        const int T = run_table_of(red);
+        const Type& vt = plan.inputs[T]->type;
+        check_run_table(vt);
+        auto sp = std::make_shared<SegReducePlan>();
+        sp->table = T;
+        sp->key_type = vt.dtypes[0];
+        Lowerer Lr = sibling();
+        Lr.set_driver(T);
+        Lr.subst[t->name] = leaf("$key", t->type);
+        Lr.lower_run_fields(red, T, sp->fields);
+        // Outputs per run over the key and the fold's fields.
         size_t k = 0;
-        std::function<N(const Type&)> nest = [&](const Type& ty) -> N {
-            if (ty.is_struct()) {
-                std::vector<N> v;
-                for (auto& d : ty.dtypes) v.push_back(nest(d));
-                return mk_new(v);
-            }
-            return accl.at(k++);
-        };
-        Lf.reduce_subst[to_text(red)] = nest(red->type);
+        Lr.reduce_subst[to_text(red)] = nest_fields(
+            red->type,
+            [&](size_t f) { return leaf("$acc." + std::to_string(f), sp->fields[f].type); }, k);
         sp->out_values.push_back(leaf("$key", t->type));
         sp->out_types.push_back(t->type);
         for (size_t i = 1; i < op->args.size(); i++) {
-            N v = Lf.rw(op->args[i]);
+            N v = Lr.rw(op->args[i]);
             if (!v->type.is_primitive()) throw Unsupported("non-primitive run reduction output");
             sp->out_values.push_back(v);
             sp->out_types.push_back(v->type);
         }
+        sp->sub = std::move(Lr.plan);
         sp->state_type = red->type;
         sp->vec_type = op->type;
         sp->sym_name = name;
@@ -1000,6 +1051,112 @@ struct Lowerer {
         plan.source = Plan::TABLE_ROWS;
         plan.start = mk_const(types::IDX_T, 0);
         plan.end = leaf("$len." + std::to_string(T) + ".0", types::IDX_T);
+    }
+
+    // A run stage: reductions over T[t] (T's column 0 run-end encoded)
+    // anywhere in the function, t bound by an Op whose space has In(t, T) --
+    // the shape of TPC-H Q3/Q4/Q12/Q18 over lineitem (benchmark/include/
+    // tpch.h:8-158, 528-723), where the reference merges T's runs with
+    // another table (InterSpace, iter_space.cpp:318-364) and folds each
+    // matching run. Here a first kernel folds every run of T into a runs
+    // table (the run's key, then the fields of every such reduction; a
+    // field may read other tables at the run's key), and the function is
+    // lowered as usual with that table in T's place: In(t, T) scans it and
+    // each reduction reads its fields at the scanned row.
+    bool lower_run_stage(const N& out) {
+        std::vector<N> reds;
+        std::set<std::string> texts;
+        int T = -1;
+        N it;
+        std::set<const Node*> seen;
+        std::set<std::string> syms;
+        std::function<void(const N&)> find = [&](const N& e) {
+            if (!e || !seen.insert(e.get()).second) return;
+            N i;
+            const int t = run_table_of(e, &i);
+            if (t >= 0) {
+                if (T >= 0 && (t != T || i->name != it->name))
+                    throw Unsupported("run reductions over different tables or iterators");
+                T = t;
+                it = i;
+                if (texts.insert(to_text(e)).second) reds.push_back(e);
+                return;
+            }
+            if (e->kind == K::Sym) {
+                if (input_index(e) >= 0 || !leaf_name(e).empty()) return;
+                if (!syms.insert(e->name).second) return;
+                auto d = tbl.find(e->name);
+                if (d != tbl.end()) find(d->second);
+                return;
+            }
+            for (auto& a : e->args) find(a);
+        };
+        find(out);
+        if (T < 0) return false;
+        // T may appear only as In(t, T) and inside its run reductions.
+        bool other_use = false;
+        seen.clear();
+        syms.clear();
+        std::function<void(const N&)> uses = [&](const N& e) {
+            if (!e || !seen.insert(e.get()).second) return;
+            if (run_table_of(e) >= 0) return;
+            if (e->kind == K::In && input_index(resolve_vec(e->args[1])) == T) {
+                if (!is_sym_named(e->args[0], it->name)) other_use = true;
+                return;
+            }
+            if (e->kind == K::Sym) {
+                const int ix = input_index(e);
+                if (ix == T) other_use = true;
+                if (ix >= 0 || !leaf_name(e).empty()) return;
+                if (!syms.insert(e->name).second) return;
+                auto d = tbl.find(e->name);
+                if (d != tbl.end()) uses(d->second);
+                return;
+            }
+            for (auto& a : e->args) uses(a);
+        };
+        uses(out);
+        if (other_use)
+            throw Unsupported("run-end table used beyond In(t, T) and reductions over its runs");
+        const Type& vt = plan.inputs[T]->type;
+        check_run_table(vt);
+        auto rs = std::make_shared<RunStagePlan>();
+        rs->table = T;
+        rs->input = plan.inputs[T];
+        SegReducePlan& sp = rs->seg;
+        sp.table = T;
+        sp.key_type = vt.dtypes[0];
+        Lowerer Lr = sibling();
+        Lr.set_driver(T);
+        Lr.subst[it->name] = leaf("$key", it->type);
+        std::vector<size_t> offs;
+        for (auto& r : reds) {
+            offs.push_back(sp.fields.size());
+            Lr.lower_run_fields(r, T, sp.fields);
+        }
+        if (sp.fields.size() + 1 > 16) throw Unsupported("more than 15 run reduction fields");
+        sp.out_values.push_back(leaf("$key", it->type));
+        sp.out_types.push_back(vt.dtypes[0]);
+        for (size_t f = 0; f < sp.fields.size(); f++) {
+            const Type& ft = sp.fields[f].type;
+            if (ft.is_idx()) throw Unsupported("idx-typed field of a run reduction");
+            sp.out_values.push_back(leaf("$acc." + std::to_string(f), ft));
+            sp.out_types.push_back(ft);
+        }
+        sp.sub = std::move(Lr.plan);
+        sp.sym_name = it->name;
+        sp.vec_type = Type(BT::VECTOR, sp.out_types, 1,
+                           std::vector<Enc>(sp.out_types.size(), Enc::FLAT));
+        // The rest of the function scans the runs table in T's place.
+        plan.inputs[T] = mk_sym(rs->input->name, sp.vec_type);
+        prefer_driver = T;
+        for (size_t r = 0; r < reds.size(); r++) {
+            size_t k = offs[r];
+            reduce_subst[to_text(reds[r])] = nest_fields(
+                reds[r]->type, [&](size_t f) { return col_leaf(T, 1 + (int)f); }, k);
+        }
+        plan.runs = rs;
+        return true;
     }
 
     Plan run() {
@@ -1042,7 +1199,8 @@ struct Lowerer {
                 return plan;
             }
         }
-        if (comps.size() == 1 && comps[0].second->kind == K::Op &&
+        const bool staged = lower_run_stage(out);
+        if (!staged && comps.size() == 1 && comps[0].second->kind == K::Op &&
             is_group_join(comps[0].second)) {
             if (!(out->kind == K::Sym && out->name == comps[0].first))
                 throw Unsupported("a group join must be the function's output");
@@ -1058,6 +1216,8 @@ struct Lowerer {
                 plan.ops.push_back(lower_op(node, name));
             }
         }
+        if (staged && plan.driver != plan.runs->table)
+            throw Unsupported("reductions over runs outside an Op over In(t, T)");
         plan.output = out;
         return plan;
     }
@@ -1104,10 +1264,13 @@ std::string Plan::describe() const {
             o << "\n";
         }
     }
-    if (seg) {
-        o << "  run reduction over the run-end column 0 of table input " << seg->table << "\n";
-        for (size_t f = 0; f < seg->fields.size(); f++) {
-            auto& fl = seg->fields[f];
+    auto seg_text = [&](const SegReducePlan& sp) {
+        o << "  run reduction over the run-end column 0 of table input " << sp.table << "\n";
+        for (size_t j = 0; j < sp.sub.probes.size(); j++)
+            o << "    probe " << j << ": table input " << sp.sub.probes[j].table << " at key "
+              << to_text(sp.sub.probes[j].key) << "\n";
+        for (size_t f = 0; f < sp.fields.size(); f++) {
+            auto& fl = sp.fields[f];
             o << "    run field " << f << " " << fl.type.str();
             if (fl.updated)
                 o << " " << aggop_name(fl.op) << " if " << to_text(fl.pred) << " of "
@@ -1116,7 +1279,13 @@ std::string Plan::describe() const {
                 o << " (not updated)";
             o << "\n";
         }
-        for (auto& v : seg->out_values) o << "    output " << to_text(v) << "\n";
+        for (auto& v : sp.out_values) o << "    output " << to_text(v) << "\n";
+    };
+    if (seg) seg_text(*seg);
+    if (runs) {
+        o << "  run stage: the runs table " << runs->seg.vec_type.str() << " takes table input "
+          << runs->table << "'s place\n";
+        seg_text(runs->seg);
     }
     if (gj) {
         o << "  group join: build table input " << gj->build << ", probe table input "

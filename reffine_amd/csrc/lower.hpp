@@ -69,6 +69,7 @@ struct Probe {
 
 struct GroupJoinPlan;
 struct SegReducePlan;
+struct RunStagePlan;
 
 struct Plan {
     enum Source { TABLE_ROWS, RANGE } source = TABLE_ROWS;
@@ -94,6 +95,7 @@ struct Plan {
     bool dense_probes = false;
     std::shared_ptr<GroupJoinPlan> gj;  // set: the function is a group join
     std::shared_ptr<SegReducePlan> seg; // set: a reduction per run of a run-end column
+    std::shared_ptr<RunStagePlan> runs; // set: a runs table is built first (T's place)
 
     std::string describe() const;
 };
@@ -161,6 +163,7 @@ namespace rf {
 // (loopgen.cpp:221-242). One output row per run, in run order.
 struct SegReducePlan {
     int table = -1;             // func input index of T
+    Plan sub;                   // the row code's plan: driver T, probes keyed by $key
     std::vector<Field> fields;  // row predicates/contributions over $col.T.C, $valid.T.C
     Type state_type;
     Type key_type;              // type of the run values (the iterator)
@@ -168,6 +171,16 @@ struct SegReducePlan {
     std::vector<Type> out_types;
     Type vec_type;
     std::string sym_name;
+};
+
+// The first stage of a function with reductions over the runs of a
+// run-end table T used by an Op over In(t, T): folds every run into a runs
+// table (key, then every reduction's fields) that the function's plan
+// scans in T's place.
+struct RunStagePlan {
+    int table = -1;       // func input index of T
+    N input;              // T's symbol with its own (run-end) type
+    SegReducePlan seg;    // out_values: $key, then $acc.F for every field
 };
 
 }  // namespace rf

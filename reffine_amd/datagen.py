@@ -109,3 +109,79 @@ def poisson_arrivals(n: int, qps: float, seed: int = 23) -> np.ndarray:
 def dim_table(seed: int = 31, days: int = 2556) -> dict:
     """Date dimension: dk = 0..days-1, dflag uniform in [0, 5)."""
     return {"dk": np.arange(days, dtype=np.int64), "dflag": randint(seed, 1, days, 0, 5, np.int8)}
+
+
+_DAY = 86_400
+_EPOCH_1992 = 694_224_000  # 1992-01-01 in seconds
+
+
+def tpch_tables(n_orders: int, seed: int = 53) -> dict:
+    """TPC-H-shaped lineitem / orders / customer columns (numeric codes in
+    place of strings, dates in epoch seconds), laid out as reffine's TPC-H
+    programs read them (benchmark/include/tpch.h): lineitem sorted by
+    orderkey with 1..7 lines per order; orders with sparse keys
+    (8 of every 32, as dbgen); customers n_orders/10.
+
+    So that every join path is exercised: 1 order in 16 is missing from the
+    orders table (its lines stay), 1 in 13 has no lines (orders only), and
+    1 in 11 names a customer key past the customer table (Locate misses).
+    """
+    i = np.arange(n_orders, dtype=np.int64)
+    okey = (i // 8) * 32 + (i % 8) + 1
+    n_cust = max(1, n_orders // 10)
+    odate = _EPOCH_1992 + randint(seed, 1, n_orders, 0, 2406, np.int64) * _DAY
+    ocust = randint(seed, 2, n_orders, 1, n_cust + 1, np.int64)
+    ocust = np.where(randint(seed, 3, n_orders, 0, 11, np.int64) == 0, n_cust + 1 + i, ocust)
+    in_orders = randint(seed, 4, n_orders, 0, 16, np.int64) != 0
+    nlines = randint(seed, 5, n_orders, 1, 8, np.int64)
+    nlines = np.where(randint(seed, 6, n_orders, 0, 13, np.int64) == 0, 0, nlines)
+    nl = int(nlines.sum())
+    lo = np.repeat(np.arange(n_orders), nlines)          # order of each line
+    line = np.arange(nl, dtype=np.int64) - np.repeat(np.cumsum(nlines) - nlines, nlines)
+    qty = randint(seed, 7, nl, 1, 51, np.int64).astype(np.float64)
+    pprice = randint(seed, 8, nl, 90_000, 200_001, np.int64).astype(np.float64) / 100.0
+    ship = odate[lo] + randint(seed, 9, nl, 1, 122, np.int64) * _DAY
+    lineitem = {
+        "l_orderkey": okey[lo],
+        "l_partkey": randint(seed, 10, nl, 1, 200_001, np.int64),
+        "l_suppkey": randint(seed, 11, nl, 1, 10_001, np.int64),
+        "l_linenumber": line + 1,
+        "l_quantity": qty,
+        "l_extendedprice": np.round(qty * pprice, 2),
+        "l_discount": randint(seed, 12, nl, 0, 11, np.int64).astype(np.float64) / 100.0,
+        "l_tax": randint(seed, 13, nl, 0, 9, np.int64).astype(np.float64) / 100.0,
+        "l_returnflag": randint(seed, 14, nl, 0, 3, np.int8),
+        "l_linestatus": randint(seed, 15, nl, 0, 2, np.int8),
+        "l_shipdate": ship,
+        "l_commitdate": odate[lo] + randint(seed, 16, nl, 30, 91, np.int64) * _DAY,
+        "l_receiptdate": ship + randint(seed, 17, nl, 1, 31, np.int64) * _DAY,
+        "l_shipinstruct": randint(seed, 18, nl, 0, 4, np.int8),
+        "l_shipmode": randint(seed, 19, nl, 0, 7, np.int8),
+        "l_comment": randint(seed, 20, nl, 0, 1 << 40, np.int64),
+    }
+    keep = in_orders
+    orders = {
+        "o_orderkey": okey[keep],
+        "o_custkey": ocust[keep],
+        "o_orderstatus": randint(seed, 21, n_orders, 0, 3, np.int8)[keep],
+        "o_totalprice": (randint(seed, 22, n_orders, 100_000, 50_000_001, np.int64)
+                         .astype(np.float64) / 100.0)[keep],
+        "o_orderdate": odate[keep],
+        "o_orderpriority": randint(seed, 23, n_orders, 0, 5, np.int8)[keep],
+        "o_clerk": randint(seed, 24, n_orders, 1, 1001, np.int64)[keep],
+        "o_shippriority": np.zeros(int(keep.sum()), dtype=np.int32),
+        "o_comment": randint(seed, 25, n_orders, 0, 1 << 40, np.int64)[keep],
+    }
+    ci = np.arange(n_cust, dtype=np.int64)
+    customer = {
+        "c_custkey": ci + 1,
+        "c_name": randint(seed, 26, n_cust, 0, 1 << 40, np.int64),
+        "c_address": randint(seed, 27, n_cust, 0, 1 << 40, np.int64),
+        "c_nationkey": randint(seed, 28, n_cust, 0, 25, np.int64),
+        "c_phone": randint(seed, 29, n_cust, 0, 1 << 40, np.int64),
+        "c_acctbal": (randint(seed, 30, n_cust, -99_999, 1_000_000, np.int64)
+                      .astype(np.float64) / 100.0),
+        "c_mktsegment": randint(seed, 31, n_cust, 0, 5, np.int8),
+        "c_comment": randint(seed, 32, n_cust, 0, 1 << 40, np.int64),
+    }
+    return {"lineitem": lineitem, "orders": orders, "customer": customer}

@@ -323,6 +323,36 @@ class Length(Expr):
         return f"(len {self.vec.ir()} {self.col})"
 
 
+class Locate(Expr):
+    """include/reffine/ir/op_to_loop.h:106-117: the row of `it` in vec's index
+    (vector_locate, include/reffine/vinstr/vinstr.h:137-146), -1 if absent."""
+
+    def __init__(self, vec: Expr, it: Expr):
+        _check(vec.type.is_vector(), "Locate in non-vector")
+        _check(vec.type.dim <= 2, "Locate in a vector of more than 2 dimensions")
+        _check(it.type == vec.type.iterty(), "Locate iterator type mismatch")
+        super().__init__(_idx_t)
+        self.vec, self.it = vec, it
+
+    def ir(self):
+        return f"(locate {self.vec.ir()} {self.it.ir()})"
+
+
+class ReadData(Expr):
+    """include/reffine/ir/op_to_loop.h:27-41: column `col` (storage index,
+    column 0 included) of vec at row `idx`."""
+
+    def __init__(self, vec: Expr, idx: Expr, col: int):
+        _check(vec.type.is_vector(), "ReadData of non-vector")
+        _check(0 <= col < len(vec.type.dtypes), "ReadData column out of range")
+        _check(idx.type.is_idx() or idx.type.is_int(), "ReadData index must be an integer")
+        super().__init__(vec.type.dtypes[col])
+        self.vec, self.idx, self.col = vec, idx, col
+
+    def ir(self):
+        return f"(readdata {self.vec.ir()} {self.idx.ir()} {self.col})"
+
+
 class Op(Expr):
     """include/reffine/ir/op.h:13-53."""
 
@@ -415,6 +445,8 @@ def _elem(vec, it) -> Element: return Element(vec, it)
 def _in(it, vec) -> In: return In(it, vec)
 def _subvec(vec, start, end) -> SubVector: return SubVector(vec, start, end)
 def _len(vec, col: int) -> Length: return Length(vec, col)
+def _locate(vec, it) -> Locate: return Locate(vec, it)
+def _readdata(vec, idx, col: int) -> ReadData: return ReadData(vec, idx, col)
 def _op(iters, pred, outputs) -> Op: return Op(iters, pred, outputs)
 def _red(vec, init, acc) -> Reduce: return Reduce(vec, init, acc)
 def _sym(name: str, t) -> Sym: return Sym(name, t)

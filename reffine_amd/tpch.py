@@ -1,0 +1,203 @@
+"""TPC-H queries over a run-end encoded lineitem, as reffine writes them.
+
+reffine's TPC-H programs (benchmark/include/tpch.h) load lineitem as a
+2-dimensional table whose column 0, l_orderkey, is run-end encoded: the
+lines of an order form one run. Q3, Q4, Q12 and Q18 are Ops over the
+orderkeys present in both lineitem and orders (`_in(k, lineitem) &
+_in(k, orders)`, a merge of the two key sets) whose predicates and outputs
+fold the order's lines `_red(lineitem[k], ...)`. The builders below keep
+those expressions, column numbers and constants; the tables are the ones
+`reffine_amd.datagen.tpch_tables` makes (numeric codes for string columns,
+dates in epoch seconds).
+
+Column numbers: `_get(lineitem[k]-row, c)` is lineitem column c + 1 (the
+run's rows are columns 1..), `_get(orders[k], c)` is orders column c + 1,
+`_readdata(customer, i, c)` is customer column c itself.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .reffiner import (_add, _and, _eq, _f64, _f64_t, _false, _func, _get, _gt, _gte, _i8,
+                       _i8_t, _i32, _i32_t, _i64, _i64_t, _idx, _in, _locate, _lt, _mul, _new,
+                       _not, _op, _or, _readdata, _red, _sel, _sub, _sym, _vec_t)
+
+LINEITEM_COLS = ["l_orderkey", "l_partkey", "l_suppkey", "l_linenumber", "l_quantity",
+                 "l_extendedprice", "l_discount", "l_tax", "l_returnflag", "l_linestatus",
+                 "l_shipdate", "l_commitdate", "l_receiptdate", "l_shipinstruct", "l_shipmode",
+                 "l_comment"]
+ORDERS_COLS = ["o_orderkey", "o_custkey", "o_orderstatus", "o_totalprice", "o_orderdate",
+               "o_orderpriority", "o_clerk", "o_shippriority", "o_comment"]
+CUSTOMER_COLS = ["c_custkey", "c_name", "c_address", "c_nationkey", "c_phone", "c_acctbal",
+                 "c_mktsegment", "c_comment"]
+
+LINEITEM_TYPE = _vec_t(2, _i64_t, _i64_t, _i64_t, _i64_t, _f64_t, _f64_t, _f64_t, _f64_t,
+                       _i8_t, _i8_t, _i64_t, _i64_t, _i64_t, _i8_t, _i8_t, _i64_t,
+                       encodings=["runend"] + ["flat"] * 15)
+ORDERS_TYPE = _vec_t(1, _i64_t, _i64_t, _i8_t, _f64_t, _i64_t, _i8_t, _i64_t, _i32_t, _i64_t)
+CUSTOMER_TYPE = _vec_t(1, _i64_t, _i64_t, _i64_t, _i64_t, _i64_t, _f64_t, _i8_t, _i64_t)
+
+
+def q3(segment: int = 1, date: int = 795484800):
+    """tpch.h:30-73 (TPCHQuery3::build_op): per order in lineitem and
+    orders, the revenue of its lines shipped after `date`; orders placed
+    before `date` by a customer of market segment `segment` with positive
+    revenue. Output (orderkey, revenue f64)."""
+    lineitem = _sym("lineitem", LINEITEM_TYPE)
+    orders = _sym("orders", ORDERS_TYPE)
+    customer = _sym("customer", CUSTOMER_TYPE)
+    orderkey = _sym("orderkey", _i64_t)
+
+    def acc(s, v):
+        new_s = _add(s, _mul(_get(v, 4), _sub(_f64(1), _get(v, 5))))
+        return _sel(_gt(_get(v, 9), _i64(date)), new_s, s)
+
+    red = _red(lineitem[orderkey], lambda: _f64(0), acc)
+    red_sym = _sym("red", red)
+    c_idx = _locate(customer, _get(orders[orderkey], 0))
+    c_idx_sym = _sym("c_idx", c_idx)
+    filt = (_gte(c_idx_sym, _idx(0)) & _lt(_get(orders[orderkey], 3), _i64(date))
+            & _eq(_readdata(customer, c_idx_sym, 6), _i8(segment)) & _gt(red_sym, _f64(0)))
+    filter_sym = _sym("filter", filt)
+    pred = _in(orderkey, lineitem) & _in(orderkey, orders) & filter_sym
+    op = _op([orderkey], pred, [red_sym])
+    op_sym = _sym("op", op)
+    fn = _func("tpchquery3", op_sym, [lineitem, orders, customer])
+    fn.tbl[c_idx_sym] = c_idx
+    fn.tbl[filter_sym] = filt
+    fn.tbl[op_sym] = op
+    fn.tbl[red_sym] = red
+    return fn
+
+
+def q4(start_date: int = 700000000, end_date: int = 900000000):
+    """tpch.h:98-148 (TPCHQuery4::build_op): orders placed in [start_date,
+    end_date) with at least one line received after its commit date,
+    counted by order priority 0..4 (five int32 counts)."""
+    lineitem = _sym("lineitem", LINEITEM_TYPE)
+    orders = _sym("orders", ORDERS_TYPE)
+    orderkey = _sym("orderkey", _i64_t)
+    exists = _red(lineitem[orderkey], lambda: _false(),
+                  lambda s, v: _or(_lt(_get(v, 10), _get(v, 11)), s))
+    exists_sym = _sym("exists", exists)
+    filt = (_gte(_get(orders[orderkey], 3), _i64(start_date))
+            & _lt(_get(orders[orderkey], 3), _i64(end_date)) & exists_sym)
+    filter_sym = _sym("filter", filt)
+    pred = _in(orderkey, orders) & _in(orderkey, lineitem) & filter_sym
+
+    def acc(s, val):
+        v = _get(val, 1)
+        return _new([_sel(_eq(v, _i8(p)), _add(_get(s, p), _i32(1)), _get(s, p))
+                     for p in range(5)])
+
+    red = _red(_op([orderkey], pred, [_get(orders[orderkey], 4)]),
+               lambda: _new([_i32(0)] * 5), acc)
+    red_sym = _sym("red", red)
+    fn = _func("tpchquery4", red_sym, [lineitem, orders])
+    fn.tbl[exists_sym] = exists
+    fn.tbl[filter_sym] = filt
+    fn.tbl[red_sym] = red
+    return fn
+
+
+def q12():
+    """tpch.h:545-604 (TPCHQuery12::build_op): per order in both tables,
+    the number of its lines committed before receipt and shipped before
+    commit, counted as high-priority (order priority 0 or 1) or low.
+    Output (orderkey, high i32, low i32)."""
+    lineitem = _sym("lineitem", LINEITEM_TYPE)
+    orders = _sym("orders", ORDERS_TYPE)
+    orderkey = _sym("orderkey", _i64_t)
+    oelem = orders[orderkey]
+    oelem_sym = _sym("oelem", oelem)
+    prio = _get(oelem_sym, 4)
+    prio_sym = _sym("prio", prio)
+    cond = _or(_eq(prio_sym, _i8(0)), _eq(prio_sym, _i8(1)))
+    cond_sym = _sym("cond", cond)
+    high = _sel(cond_sym, _i32(1), _i32(0))
+    low = _sel(_not(cond_sym), _i32(1), _i32(0))
+    hc_sym = _sym("hc", high)
+    lc_sym = _sym("lc", low)
+
+    def acc(s, v):
+        hc, lc = _get(s, 0), _get(s, 1)
+        comm, ship, rec = _get(v, 10), _get(v, 9), _get(v, 11)
+        f = _lt(comm, rec) & _lt(ship, comm)
+        return _sel(f, _new([_add(hc_sym, hc), _add(lc_sym, lc)]), s)
+
+    red = _red(lineitem[orderkey], lambda: _new([_i32(0), _i32(0)]), acc)
+    red_sym = _sym("red", red)
+    op = _op([orderkey], _in(orderkey, lineitem) & _in(orderkey, orders),
+             [_get(red_sym, 0), _get(red_sym, 1)])
+    op_sym = _sym("op", op)
+    fn = _func("tpchquery12", op_sym, [lineitem, orders])
+    fn.tbl[op_sym] = op
+    fn.tbl[lc_sym] = low
+    fn.tbl[hc_sym] = high
+    fn.tbl[cond_sym] = cond
+    fn.tbl[prio_sym] = prio
+    fn.tbl[oelem_sym] = oelem
+    fn.tbl[red_sym] = red
+    return fn
+
+
+def q18(quantity: int = 300):
+    """tpch.h:684-715 (TPCHQuery18::build_op): orders in both tables whose
+    lines' total quantity exceeds `quantity`. Output (orderkey, total f64)."""
+    lineitem = _sym("lineitem", LINEITEM_TYPE)
+    orders = _sym("orders", ORDERS_TYPE)
+    orderkey = _sym("orderkey", _i64_t)
+    red = _red(lineitem[orderkey], lambda: _f64(0), lambda s, v: _add(s, _get(v, 3)))
+    red_sym = _sym("red", red)
+    filt = _gt(red_sym, _f64(quantity))
+    filter_sym = _sym("filter", filt)
+    pred = _in(orderkey, lineitem) & _in(orderkey, orders) & filter_sym
+    op = _op([orderkey], pred, [red_sym])
+    op_sym = _sym("op", op)
+    fn = _func("tpchquery18", op_sym, [lineitem, orders])
+    fn.tbl[filter_sym] = filt
+    fn.tbl[op_sym] = op
+    fn.tbl[red_sym] = red
+    return fn
+
+
+QUERIES = {"q3": q3, "q4": q4, "q12": q12, "q18": q18}
+# Tables each query reads, in argument order.
+ARGS = {"q3": ("lineitem", "orders", "customer"), "q4": ("lineitem", "orders"),
+        "q12": ("lineitem", "orders"), "q18": ("lineitem", "orders")}
+
+
+def record_batches(tables: dict) -> dict:
+    """pyarrow record batches of tpch_tables' columns: lineitem with
+    l_orderkey run-end encoded (int32 run ends), the others flat."""
+    import pyarrow as pa
+    import pyarrow.compute as pc
+    li = tables["lineitem"]
+    arrays = [pc.run_end_encode(pa.array(li["l_orderkey"]), run_end_type=pa.int32())]
+    arrays += [pa.array(li[c]) for c in LINEITEM_COLS[1:]]
+    out = {"lineitem": pa.RecordBatch.from_arrays(arrays, names=LINEITEM_COLS)}
+    for name, cols in (("orders", ORDERS_COLS), ("customer", CUSTOMER_COLS)):
+        out[name] = pa.RecordBatch.from_arrays([pa.array(tables[name][c]) for c in cols],
+                                               names=cols)
+    return out
+
+
+def algorithmic_bytes(query: str, tables: dict, out_rows: int = 0) -> int:
+    """Bytes the query must read and write: lineitem's run values (8 B) and
+    run ends (4 B) per order, its referenced columns per line, the orders /
+    customer columns read at matched keys (counted once per table row), and
+    the output rows."""
+    li, od = tables["lineitem"], tables["orders"]
+    n_runs = int(np.count_nonzero(np.diff(li["l_orderkey"])) + (len(li["l_orderkey"]) > 0))
+    n_li, n_o = len(li["l_orderkey"]), len(od["o_orderkey"])
+    n_c = len(tables["customer"]["c_custkey"])
+    runs = 12 * n_runs
+    if query == "q3":
+        return runs + 24 * n_li + 24 * n_o + 9 * n_c + 16 * out_rows
+    if query == "q4":
+        return runs + 16 * n_li + 17 * n_o
+    if query == "q12":
+        return runs + 24 * n_li + 9 * n_o + 16 * out_rows
+    if query == "q18":
+        return runs + 8 * n_li + 8 * n_o + 16 * out_rows
+    raise KeyError(query)

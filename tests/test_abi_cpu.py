@@ -166,3 +166,39 @@ def test_c_host_program_compiles_against_the_header():
                         os.path.join(root, "tests", "c_abi", "rf_c_scan.c")],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.parametrize("name", ["q3", "q4", "q12", "q18"])
+def test_tpch_run_stage_queries_compile(name):
+    """TPC-H Q3/Q4/Q12/Q18 over the run-end lineitem (benchmark/include/
+    tpch.h) lower to a run stage -- a fold per lineitem run, then a scan of
+    the runs table with the orders (and customer) probes -- and compile."""
+    from reffine_amd import tpch
+    text = runtime.compile_check(tpch.QUERIES[name]())
+    assert "run stage: the runs table" in text
+    assert text.count("--- kernel ---") == 2
+    assert "probe 0: table input 1" in text
+    if name == "q12":  # the fold reads orders at the run's key
+        assert "probe 0: table input 1 at key (sym $key i64)" in text
+    if name == "q3":  # _locate / _readdata into customer
+        assert "probe 1: table input 2 key (sym $pcol.0.1 i64)" in text
+        assert "(sym $pcol.1.6 i8)" in text
+
+
+def test_run_table_used_beyond_its_runs_is_rejected():
+    """A run-end table read other than through In(t, T) and T[t]
+    reductions has no runs-table form."""
+    from reffine_amd import tpch
+    from reffine_amd.reffiner import _f64, _func, _get, _gt, _in, _len, _op, _red, _sym, _i64_t, _lt, _cast, _idx
+    lineitem = _sym("lineitem", tpch.LINEITEM_TYPE)
+    orders = _sym("orders", tpch.ORDERS_TYPE)
+    k = _sym("k", _i64_t)
+    red = _red(lineitem[k], lambda: _f64(0), lambda s, v: s + _get(v, 3))
+    rs = _sym("red", red)
+    op = _op([k], _in(k, lineitem) & _in(k, orders) & _lt(_idx(0), _len(lineitem, 1)), [rs])
+    os_ = _sym("op", op)
+    fn = _func("bad", os_, [lineitem, orders])
+    fn.tbl[os_] = op
+    fn.tbl[rs] = red
+    with pytest.raises(runtime.ReffineError):
+        runtime.compile_check(fn)

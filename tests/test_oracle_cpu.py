@@ -175,3 +175,74 @@ def test_dim_join_oracle_against_loops():
     assert list(keys) == sorted(want)
     assert [want[k][1] for k in sorted(want)] == list(counts)
     assert [want[k][0] for k in sorted(want)] == list(sums)
+
+
+def _tpch_frames(t):
+    import pandas as pd
+    return (pd.DataFrame(t["lineitem"]), pd.DataFrame(t["orders"]),
+            pd.DataFrame(t["customer"]))
+
+
+def test_tpch_oracle_against_pandas():
+    """oracle.tpch (row loops over a merge of the two key sets) against an
+    independent group-by / join formulation of the same four queries in
+    pandas: the same keys, exact counts and integer-valued sums, revenue
+    within 1e-12 (pandas sums in its own order)."""
+    from oracle import tpch as ot
+    t = datagen.tpch_tables(6000, seed=5)
+    li, od, cu = _tpch_frames(t)
+    both = od[od.o_orderkey.isin(li.l_orderkey)]
+    # Q18 (quantity 150 so that some orders qualify)
+    tot = li.groupby("l_orderkey").l_quantity.sum()
+    m = both.assign(tot=both.o_orderkey.map(tot))
+    m = m[m.tot > 150].sort_values("o_orderkey")
+    k, v = ot.q18(t, quantity=150)
+    np.testing.assert_array_equal(k, m.o_orderkey.to_numpy())
+    np.testing.assert_array_equal(v, m.tot.to_numpy())
+    assert len(k) > 20
+    # Q3
+    date, seg = 795484800, 1
+    rev = (li.l_extendedprice * (1.0 - li.l_discount)).where(li.l_shipdate > date, 0.0)
+    r = rev.groupby(li.l_orderkey).sum()
+    m = both.assign(rev=both.o_orderkey.map(r))
+    m = m.merge(cu[["c_custkey", "c_mktsegment"]], left_on="o_custkey", right_on="c_custkey",
+                how="inner")
+    m = m[(m.o_orderdate < date) & (m.c_mktsegment == seg) & (m.rev > 0)]
+    m = m.sort_values("o_orderkey")
+    k, v = ot.q3(t, segment=seg, date=date)
+    np.testing.assert_array_equal(k, m.o_orderkey.to_numpy())
+    np.testing.assert_allclose(v, m.rev.to_numpy(), rtol=1e-12)
+    assert len(k) > 10
+    # Q4
+    ex = (li.l_commitdate < li.l_receiptdate).groupby(li.l_orderkey).any()
+    m = both[(both.o_orderdate >= 700000000) & (both.o_orderdate < 900000000)
+             & both.o_orderkey.map(ex)]
+    want = [int((m.o_orderpriority == p).sum()) for p in range(5)]
+    assert ot.q4(t) == want
+    # Q12
+    k, h, lo = ot.q12(t)
+    wk, wh, wl = _pandas_q12(t)
+    np.testing.assert_array_equal(k, wk)
+    np.testing.assert_array_equal(h, wh)
+    np.testing.assert_array_equal(lo, wl)
+
+
+def _pandas_q12(t):
+    li, od, _ = _tpch_frames(t)
+    both = od[od.o_orderkey.isin(li.l_orderkey)]
+    ok = ((li.l_commitdate < li.l_receiptdate) & (li.l_shipdate < li.l_commitdate))
+    n = ok.groupby(li.l_orderkey).sum()
+    m = both.sort_values("o_orderkey")
+    cnt = m.o_orderkey.map(n).to_numpy()
+    high = np.isin(m.o_orderpriority.to_numpy(), [0, 1])
+    return m.o_orderkey.to_numpy(), np.where(high, cnt, 0), np.where(high, 0, cnt)
+
+
+def test_tpch_oracle_catches_a_planted_mistake():
+    """The pandas formulation disagrees with an oracle that reads the ship
+    date where Q12 reads the commit date (the mistake planted by running
+    the oracle on lineitem columns swapped accordingly)."""
+    from oracle import tpch as ot
+    t = datagen.tpch_tables(3000, seed=9)
+    bad = {**t, "lineitem": {**t["lineitem"], "l_commitdate": t["lineitem"]["l_shipdate"]}}
+    assert not np.array_equal(ot.q12(bad)[1], _pandas_q12(t)[1])
