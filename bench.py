@@ -62,11 +62,14 @@ def main() -> int:
     ap.add_argument("--cpu-rows", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "tpch"], default="c2",
                     help="c2 (default): the 60M-row scan-filter-aggregate + selection; c3: the "
                          "60M-row grouped aggregate; c4: the 1.5M x 60M hash join + grouped "
                          "aggregate; c5: 4096 queries over 64 record batches, batched/served "
-                         "against plain per-query launches")
+                         "against plain per-query launches; tpch: TPC-H Q3/Q4/Q12/Q18 over a "
+                         "run-end encoded lineitem")
+    ap.add_argument("--orders", type=int, default=15_000_000,
+                    help="tpch: orders (lineitem has about 3.75 lines per order)")
     ap.add_argument("--queries", type=int, default=4096, help="c5: number of queries")
     ap.add_argument("--no-batched", action="store_true",
                     help="c2: leave out the batched/served measurement (4096 queries over 64 "
@@ -90,6 +93,8 @@ def main() -> int:
         return bench_c4(args)
     if args.workload == "c5":
         return bench_c5(args)
+    if args.workload == "tpch":
+        return bench_tpch(args)
     n = args.rows
     log(f"generating {n} rows")
     c = datagen.scan_table(n, args.seed, with_q=True, rowid_dtype=np.int32)
@@ -272,6 +277,90 @@ def bench_c4(args) -> int:
         out_line["cpu_baseline"] = cb.time_c4(nb, min(args.cpu_rows, 10_000_000))
     print(json.dumps(out_line), flush=True)
     return 0 if ok else 1
+
+
+def _run_fold_bytes(name: str, t: dict) -> int:
+    """Algorithmic bytes of a query's run fold: run values + run ends
+    (12 B per run), the lineitem columns the fold reads, the orders columns
+    it reads at the run's key, and the runs table it writes (key + fields)."""
+    li = t["lineitem"]
+    n_li = len(li["l_orderkey"])
+    n_runs = int(np.count_nonzero(np.diff(li["l_orderkey"]))) + (n_li > 0)
+    per_line = {"q3": 24, "q4": 16, "q12": 24, "q18": 8}[name]
+    per_run_out = {"q3": 16, "q4": 9, "q12": 16, "q18": 16}[name]
+    per_run_probe = {"q12": 1}.get(name, 0)
+    return 12 * n_runs + per_line * n_li + (per_run_out + per_run_probe) * n_runs
+
+
+def bench_tpch(args) -> int:
+    """TPC-H Q3, Q4, Q12 and Q18 as reffine writes them (benchmark/include/
+    tpch.h) over a run-end encoded lineitem: each query is a fold per
+    lineitem run into a runs table, then a scan of that table probing orders
+    (and customer for Q3)."""
+    from oracle import tpch as ot
+    from reffine_amd import DeviceTable, compile_op, datagen, tpch
+    n = args.orders
+    log(f"generating {n} orders")
+    t = datagen.tpch_tables(n, seed=53)
+    n_li = len(t["lineitem"]["l_orderkey"])
+    log(f"uploading lineitem ({n_li} rows), orders, customer")
+    rb = tpch.record_batches(t)
+    dev = {"lineitem": DeviceTable.from_arrow(rb["lineitem"], dim=2),
+           "orders": DeviceTable.from_arrow(rb["orders"]).build_index(),
+           "customer": DeviceTable.from_arrow(rb["customer"]).build_index()}
+    del rb
+    per = {}
+    tot_bytes, tot_ms, ok_all = 0, 0.0, True
+    fold_best = None
+    for name in ("q3", "q4", "q12", "q18"):
+        q = compile_op(tpch.QUERIES[name]())
+        a = [dev[x] for x in tpch.ARGS[name]]
+        log(f"{name}: timing")
+        out, ms = _timed(lambda: q(*a), args.steps, args.warmup)
+        want = getattr(ot, name + "_np")(t)
+        if name == "q4":
+            ok, rows = list(out) == want, 0
+        else:
+            rows = out.nrows
+            ok = rows == len(want[0]) and all(
+                np.array_equal(out.column(j).view(np.uint8), np.asarray(w).view(np.uint8))
+                for j, w in enumerate(want))
+        run_ms, scan_ms = q.time_kernels(*a, iters=max(args.steps, 5))
+        alg = tpch.algorithmic_bytes(name, t, rows)
+        fold_ms = run_ms - scan_ms
+        fold_gbs = _run_fold_bytes(name, t) / (fold_ms * 1e-3) / 1e9
+        per[name] = {"ms": round(ms, 4), "device_ms": round(run_ms, 4),
+                     "run_fold_ms": round(fold_ms, 4), "runs_scan_ms": round(scan_ms, 4),
+                     "gbs": round(alg / (ms * 1e-3) / 1e9, 2), "run_fold_gbs": round(fold_gbs, 2),
+                     "rows_out": rows, "verified": bool(ok)}
+        log(f"{name}: {per[name]}")
+        tot_bytes += alg
+        tot_ms += ms
+        ok_all = ok_all and ok
+        if fold_best is None or fold_ms > fold_best[1]:
+            fold_best = (name, fold_ms, fold_gbs)
+    cfg = {"workload": f"TPC-H Q3, Q4, Q12, Q18 (benchmark/include/tpch.h) over lineitem "
+                       f"({n_li} rows, l_orderkey run-end encoded) and orders ({n} orders); "
+                       "value: algorithmic bytes of the four / their summed wall time",
+           "orders": n, "lineitem_rows": n_li, "queries": per, "verified": bool(ok_all)}
+    line = _line(cfg, tot_bytes / (tot_ms * 1e-3) / 1e9, tot_ms / 4, args, fold_best[2],
+                 f"rf_seg_fold ({fold_best[0]}: fold of every lineitem run into the runs table)",
+                 fold_best[1], per[fold_best[0]]["device_ms"])
+    if not args.no_cpu:
+        import time as _t
+        sample = min(n, 1_500_000)
+        ts = datagen.tpch_tables(sample, seed=53) if sample < n else t
+        t0 = _t.perf_counter()
+        for name in ("q3", "q4", "q12", "q18"):
+            getattr(ot, name + "_np")(ts)
+        cpu_s = _t.perf_counter() - t0
+        nb = sum(tpch.algorithmic_bytes(x, ts, 0) for x in ("q3", "q4", "q12", "q18"))
+        line["cpu_baseline"] = {"value": round(nb / cpu_s / 1e9, 3), "unit": "GB/s", "cores": 1,
+                                "kind": "port",
+                                "sample": f"the four queries over {sample} orders, numpy "
+                                          "restatement folding runs position by position"}
+    print(json.dumps(line), flush=True)
+    return 0 if ok_all else 1
 
 
 def c5_setup(n_queries: int, seed: int = 5):

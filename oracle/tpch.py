@@ -113,3 +113,93 @@ def q18(t: dict, quantity: int = 300):
             keys.append(k)
             vals.append(s)
     return np.array(keys, dtype=np.int64), np.array(vals, dtype=np.float64)
+
+
+# ---------------------------------------------------------------------------
+# The same four queries vectorised over runs (numpy), for tables too large
+# for the row loops above. Each run is still folded in row order: position
+# j of every run at once, j = 0, 1, ... (a sequential fold per run).
+
+def _runs_np(keys: np.ndarray):
+    n = len(keys)
+    if n == 0:
+        z = np.zeros(0, dtype=np.int64)
+        return z, z, z
+    cut = np.flatnonzero(np.diff(keys)) + 1
+    lo = np.concatenate([[0], cut]).astype(np.int64)
+    hi = np.concatenate([cut, [n]]).astype(np.int64)
+    return keys[lo], lo, hi
+
+
+def _fold_np(lo, hi, vals, op, init):
+    """Per run, init op vals[lo] op vals[lo+1] ... in row order."""
+    s = np.full(len(lo), init, dtype=np.asarray(vals).dtype if init is None else type(init))
+    ln = hi - lo
+    for j in range(int(ln.max()) if len(ln) else 0):
+        m = ln > j
+        s[m] = op(s[m], vals[lo[m] + j])
+    return s
+
+
+def _match_np(li, od):
+    """Runs of lineitem whose key is in orders, ascending: (run keys, run
+    lo, run hi, orders row)."""
+    rk, lo, hi = _runs_np(li["l_orderkey"])
+    ok = od["o_orderkey"]
+    order = np.argsort(ok, kind="stable")
+    pos = np.searchsorted(ok[order], rk)
+    pos = np.minimum(pos, max(len(ok) - 1, 0))
+    hit = (len(ok) > 0) & (ok[order][pos] == rk) if len(ok) else np.zeros(len(rk), bool)
+    orow = order[pos]
+    sel = np.flatnonzero(hit)
+    srt = sel[np.argsort(rk[sel], kind="stable")]
+    return rk[srt], lo[srt], hi[srt], orow[srt]
+
+
+def q18_np(t: dict, quantity: int = 300):
+    k, lo, hi, _ = _match_np(t["lineitem"], t["orders"])
+    s = _fold_np(lo, hi, t["lineitem"]["l_quantity"], np.add, 0.0)
+    m = s > float(quantity)
+    return k[m], s[m]
+
+
+def q3_np(t: dict, segment: int = 1, date: int = 795484800):
+    li, od, cu = t["lineitem"], t["orders"], t["customer"]
+    k, lo, hi, o = _match_np(li, od)
+    rev = li["l_extendedprice"] * (1.0 - li["l_discount"])
+    s = np.zeros(len(lo))
+    ln = hi - lo
+    for j in range(int(ln.max()) if len(ln) else 0):
+        m = ln > j
+        r = lo[m] + j
+        s[m] = np.where(li["l_shipdate"][r] > date, s[m] + rev[r], s[m])
+    ck = cu["c_custkey"]
+    corder = np.argsort(ck, kind="stable")
+    cust = od["o_custkey"][o]
+    cp = np.minimum(np.searchsorted(ck[corder], cust), len(ck) - 1)
+    found = ck[corder][cp] == cust
+    seg = cu["c_mktsegment"][corder[cp]]
+    m = found & (od["o_orderdate"][o] < date) & (seg == segment) & (s > 0.0)
+    return k[m], s[m]
+
+
+def q4_np(t: dict, start_date: int = 700000000, end_date: int = 900000000):
+    li, od = t["lineitem"], t["orders"]
+    k, lo, hi, o = _match_np(li, od)
+    late = li["l_commitdate"] < li["l_receiptdate"]
+    ex = _fold_np(lo, hi, late, np.logical_or, False)
+    d = od["o_orderdate"][o]
+    m = (d >= start_date) & (d < end_date) & ex
+    p = od["o_orderpriority"][o][m]
+    return [int((p == j).sum()) for j in range(5)]
+
+
+def q12_np(t: dict):
+    li, od = t["lineitem"], t["orders"]
+    k, lo, hi, o = _match_np(li, od)
+    ok = ((li["l_commitdate"] < li["l_receiptdate"]) &
+          (li["l_shipdate"] < li["l_commitdate"])).astype(np.int32)
+    n = _fold_np(lo, hi, ok, np.add, np.int32(0))
+    p = od["o_orderpriority"][o]
+    high = (p == 0) | (p == 1)
+    return k, np.where(high, n, 0).astype(np.int32), np.where(high, 0, n).astype(np.int32)

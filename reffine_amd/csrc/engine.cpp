@@ -342,6 +342,50 @@ void table_build_index(rf_table* t) {
                          "(include/reffine/arrow/table.h:107-138)");
     Device& d = device();
     if (t->has_index) return;
+    // Keys dense in their range (all valid, at most 32 key slots per row
+    // plus 2^20): a rank map -- 1.5 bits per slot, small enough to stay in
+    // the caches, where a probe of the hash slots is a random HBM read.
+    if (!c.validity && c.length > 0 && c.length < ((int64_t)1 << 31)) {
+        uint64_t* small = (uint64_t*)dev_alloc(32);
+        const uint64_t init[4] = {~0ull, 0ull, (uint64_t)c.length, 0ull};
+        RF_HIP(hipMemcpyAsync(small, init, 32, hipMemcpyHostToDevice, d.stream));
+        RF_HIP(aot_minmax_i64((const int64_t*)c.values, (const int64_t*)(small + 2), c.length,
+                              small, d.stream));
+        RF_HIP(aot_not_ascending((const int64_t*)c.values, c.length, small + 3, d.stream));
+        uint64_t mm[4];
+        RF_HIP(hipMemcpyAsync(mm, small, 32, hipMemcpyDeviceToHost, d.stream));
+        RF_HIP(hipStreamSynchronize(d.stream));
+        dev_free(small);
+        const int64_t kmin = (int64_t)(mm[0] ^ 0x8000000000000000ull);
+        const int64_t kmax = (int64_t)(mm[1] ^ 0x8000000000000000ull);
+        const uint64_t range = (uint64_t)kmax - (uint64_t)kmin + 1;
+        if (range != 0 && range <= 32 * (uint64_t)c.length + ((uint64_t)1 << 20)) {
+            const int64_t words = (int64_t)((range + 63) / 64);
+            void* bits = dev_alloc((size_t)words * 8);
+            t->owned->p.push_back(bits);
+            void* cum = dev_alloc((size_t)aot_rank_cum_entries(words) * 4);
+            t->owned->p.push_back(cum);
+            dev_fill_u64(bits, 0, words);
+            RF_HIP(aot_rank_build((const int64_t*)c.values, c.length, kmin, bits, words, cum,
+                                  d.stream));
+            void* rows = nullptr;
+            if (mm[3] != 0) {  // not strictly ascending: the first row of each key
+                rows = dev_alloc((size_t)c.length * 8);
+                t->owned->p.push_back(rows);
+                dev_fill_u64(rows, (uint64_t)INT64_MAX, c.length);
+                RF_HIP(aot_rank_rows((const int64_t*)c.values, c.length, kmin, bits, cum,
+                                     (int64_t*)rows, d.stream));
+            }
+            RF_HIP(hipStreamSynchronize(d.stream));
+            t->index.dense_bits = bits;
+            t->index.dense_cum = cum;
+            t->index.dense_rows = rows;
+            t->index.dense_base = kmin;
+            t->index.dense_len = range;
+            t->has_index = true;
+            return;
+        }
+    }
     const uint64_t cap = next_pow2(std::max<uint64_t>(16, 2 * (uint64_t)c.length));
     void* keys = dev_alloc(cap * 8);
     t->owned->p.push_back(keys);
@@ -370,6 +414,11 @@ RfProbeDev probe_of(const rf_table* t) {
     h.rows = (const int64_t*)t->index.rows;
     h.mask = t->index.mask;
     h.min_key_row = t->index.min_key_row;
+    h.dense_bits = (const uint64_t*)t->index.dense_bits;
+    h.dense_cum = (const uint32_t*)t->index.dense_cum;
+    h.dense_base = t->index.dense_base;
+    h.dense_len = t->index.dense_len;
+    h.dense_rows = (const int64_t*)t->index.dense_rows;
     return h;
 }
 

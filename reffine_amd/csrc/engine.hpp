@@ -58,6 +58,9 @@ hipError_t aot_radix_sort(void* const keys[2], void* const pay[2][8], const int3
 hipError_t aot_minmax_i64(const int64_t* keys, const int64_t* n_dev, int64_t n_max, void* out,
                           hipStream_t s);
 hipError_t aot_adjacent_dups(const int64_t* keys, int64_t n, void* flag, hipStream_t s);
+hipError_t aot_not_ascending(const int64_t* keys, int64_t n, void* flag, hipStream_t s);
+hipError_t aot_rank_rows(const int64_t* keys, int64_t n, int64_t base, const void* bits,
+                         const void* cum, int64_t* rows, hipStream_t s);
 hipError_t aot_filter_build(const int64_t* keys, int64_t n, void* filter, uint64_t fmask,
                            hipStream_t s);
 hipError_t aot_seg_bounds(const uint32_t* rank, int64_t n, int64_t* lo, int64_t* hi,
@@ -94,6 +97,15 @@ struct HashIndex {
     void* rows = nullptr;
     uint64_t mask = 0;
     int64_t min_key_row = -1;
+    // Rank map in place of the slots (keys dense in their range): a bit per
+    // key of [dense_base, dense_base + dense_len), the marked keys before
+    // each 64-bit word, and the first row of each rank (null: rank == row,
+    // the keys being strictly ascending).
+    void* dense_bits = nullptr;
+    void* dense_cum = nullptr;
+    void* dense_rows = nullptr;
+    int64_t dense_base = 0;
+    uint64_t dense_len = 0;
 };
 
 }  // namespace rf

@@ -47,6 +47,7 @@ struct RfProbeDev {
     const uint32_t* dense_cum;
     int64_t dense_base;
     uint64_t dense_len;
+    const int64_t* dense_rows;  // row of each rank; null: the rank is the row
 };
 
 struct RfScanArgs {

@@ -122,7 +122,8 @@ __device__ __forceinline__ int64_t rf_probe(const RfProbeDev& h, int64_t key) {
         const uint64_t wd = rf_g(h.dense_bits)[off >> 6];
         const uint64_t bit = 1ull << (off & 63);
         if (!(wd & bit)) return -1;
-        return (int64_t)rf_g(h.dense_cum)[off >> 6] + (int64_t)__popcll(wd & (bit - 1));
+        const int64_t r = (int64_t)rf_g(h.dense_cum)[off >> 6] + (int64_t)__popcll(wd & (bit - 1));
+        return h.dense_rows ? rf_g(h.dense_rows)[r] : r;
     }
     if (key == RF_EMPTY_KEY) return h.min_key_row;
     const uint64_t hk = rf_hash64((uint64_t)key);

@@ -477,6 +477,29 @@ extern "C" __global__ void rf_rank_bits(const int64_t* __restrict__ keys, int64_
     }
 }
 
+// Sets *flag unless the keys are strictly ascending.
+extern "C" __global__ void rf_not_ascending(const int64_t* __restrict__ keys, int64_t n,
+                                            unsigned int* flag) {
+    for (int64_t i = 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (keys[i] <= keys[i - 1]) atomicOr(flag, 1u);
+}
+
+// The first row of each distinct key at its rank in a built rank map:
+// rows[rank] = min row holding the key (rows must hold INT64_MAX before).
+extern "C" __global__ void rf_rank_rows(const int64_t* __restrict__ keys, int64_t n, int64_t base,
+                                        const uint64_t* __restrict__ bits,
+                                        const uint32_t* __restrict__ cum,
+                                        unsigned long long* rows) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t off = (uint64_t)keys[i] - (uint64_t)base;
+        const uint64_t wd = bits[off >> 6];
+        const uint64_t r = (uint64_t)cum[off >> 6] + (uint64_t)__popcll(wd & ((1ull << (off & 63)) - 1));
+        atomicMin(&rows[r], (unsigned long long)i);
+    }
+}
+
 // Build rows in key order without a sort: each distinct key's rank is its
 // position in the rank map (marked keys below it), so (key, row) goes to
 // slot rank. Keys must be distinct (checked by the caller through the count
@@ -692,6 +715,21 @@ hipError_t aot_rank_build(const int64_t* keys, int64_t n, int64_t base, void* bi
     hipLaunchKernelGGL(rf_scan_chunks, dim3((unsigned)nch), dim3(256), 0, s, c, words, part);
     hipLaunchKernelGGL(rf_sort_scan, dim3(1), dim3(1024), 0, s, part, nch);
     hipLaunchKernelGGL(rf_scan_apply, dim3((unsigned)nch), dim3(256), 0, s, c, words, part);
+    return hipGetLastError();
+}
+
+hipError_t aot_not_ascending(const int64_t* keys, int64_t n, void* flag, hipStream_t s) {
+    if (n <= 1) return hipSuccess;
+    hipLaunchKernelGGL(rf_not_ascending, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, n,
+                       (unsigned int*)flag);
+    return hipGetLastError();
+}
+
+hipError_t aot_rank_rows(const int64_t* keys, int64_t n, int64_t base, const void* bits,
+                         const void* cum, int64_t* rows, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rf_rank_rows, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, n, base,
+                       (const uint64_t*)bits, (const uint32_t*)cum, (unsigned long long*)rows);
     return hipGetLastError();
 }
 

@@ -95,3 +95,54 @@ def test_small_tables(device):
     np.testing.assert_array_equal(out.column(0), k)
     np.testing.assert_array_equal(out.column(1), v)
     assert list(compile_op(tpch.q4())(*_args(dev, "q4"))) == ot.q4(t)
+
+
+def _locate_fn(T_type, P_type):
+    from reffine_amd.reffiner import _func, _get, _in, _locate, _op, _sym, _i64_t
+    T = _sym("T", T_type)
+    P = _sym("P", P_type)
+    t = _sym("t", _i64_t)
+    op = _op([t], _in(t, P), [_locate(T, _get(P[t], 0))])
+    os_ = _sym("op", op)
+    fn = _func("locate", os_, [T, P])
+    fn.tbl[os_] = op
+    return fn
+
+
+@pytest.mark.parametrize("form", ["ascending", "shuffled_dups", "sparse", "nulls"])
+def test_locate_index_forms(device, form):
+    """_locate (vector_locate, include/reffine/vinstr/vinstr.h:137-146) over
+    each form of the table index: the rank map (keys dense in their range,
+    strictly ascending or not, with repeated keys: the first row) and the
+    hash slots (sparse keys, or null keys, which are never found)."""
+    import pyarrow as pa
+    from reffine_amd.reffiner import _i64_t, _vec_t
+    rng = np.random.default_rng(7)
+    n = 50_000
+    valid = None
+    if form == "ascending":
+        keys = np.arange(n, dtype=np.int64) * 3 + 1000
+    elif form == "shuffled_dups":
+        keys = rng.integers(-500, n // 2, n).astype(np.int64)
+    elif form == "sparse":
+        keys = rng.integers(-2**62, 2**62, n).astype(np.int64)
+        keys[:100] = keys[100:200]
+    else:
+        keys = rng.integers(0, n, n).astype(np.int64)
+        valid = rng.random(n) > 0.2
+    first = {}
+    for r, k in enumerate(keys.tolist()):
+        if (valid is None or valid[r]) and k not in first:
+            first[k] = r
+    probe = np.concatenate([keys[rng.integers(0, n, 4000)], rng.integers(-10**6, 10**6, 4000),
+                            keys[:50]]).astype(np.int64)
+    T = DeviceTable.from_arrow(pa.RecordBatch.from_arrays(
+        [pa.array(keys, mask=None if valid is None else ~valid), pa.array(np.zeros(n))],
+        names=["k", "x"]))
+    T.build_index()
+    P = DeviceTable.from_arrow(pa.RecordBatch.from_arrays(
+        [pa.array(np.arange(len(probe), dtype=np.int64)), pa.array(probe)], names=["i", "k"]))
+    from reffine_amd.reffiner import _f64_t
+    out = compile_op(_locate_fn(_vec_t(1, _i64_t, _f64_t), _vec_t(1, _i64_t, _i64_t)))(T, P)
+    want = np.array([first.get(k, -1) for k in probe.tolist()], dtype=np.int64)
+    np.testing.assert_array_equal(out.column(1), want)
