@@ -42,6 +42,10 @@ struct Gen {
     std::map<std::string, int> param_ix;
     std::map<std::pair<int, int>, int> col_ix;
     std::set<int> carried;  // probes the row code computes; emit reuses them
+    // Run folds: scanned-table columns read from the block's LDS copy
+    // (sm<slot>[row - blo]) instead of global memory; the slots so read.
+    bool lds_rows = false;
+    std::set<int> lds_slots;
 
     Gen(const Plan& p, const std::vector<std::vector<bool>>& nl, KernelSource& k)
         : plan(p), nullable(nl), ks(k) {}
@@ -155,6 +159,10 @@ struct Emitter {
             const std::string T = cty(g.ks.cols[s].type);
             if (mode == Mode::ROWS) return "c" + std::to_string(s) + ".v[k]";
             if (mode == Mode::FINAL) return "rf_g((const " + T + "*)L.col[" + std::to_string(s) + "])[brow]";
+            if (g.lds_rows) {
+                g.lds_slots.insert(s);
+                return "sm" + std::to_string(s) + "[row - blo]";
+            }
             return "rf_g((const " + T + "*)L.col[" + std::to_string(s) + "])[row]";  // MOVE
         }
         if (kind == "$valid") {
@@ -717,60 +725,140 @@ KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
     ks.tune.block = 256;
     if (sp.sub.probes.size() > 4) throw Unsupported("more than 4 probed tables");
     Gen g(sp.sub, nullable, ks);
-    std::ostringstream o;
-    o << "// Generated run reduction for '" << sp.sym_name << "'.\n";
-    o << "extern \"C\" __global__ __launch_bounds__(256) void rf_seg_fold(RfSegArgs L) {\n";
-    o << "    for (int64_t run = blockIdx.x * (int64_t)256 + threadIdx.x; run < L.runs;\n"
-         "         run += (int64_t)gridDim.x * 256) {\n";
-    o << "        const int64_t lo = run > 0 ? (int64_t)L.run_ends[run - 1] : 0;\n";
-    o << "        const int64_t hi0 = (int64_t)L.run_ends[run];\n";
-    o << "        const int64_t hi = hi0 < L.rows ? hi0 : L.rows;\n";
-    o << "        const " << cty(sp.key_type) << " key = ((const " << cty(sp.key_type)
-      << "*)L.keys)[run];\n        (void)key;\n";
+    // A block folds 256 consecutive runs, a thread per run, each run's rows
+    // in row order (the reference's SubVector loop). The rows the block's
+    // runs cover are first copied to LDS with coalesced loads (a thread's
+    // own run would otherwise read a few rows at a stride of the neighbouring
+    // runs' lengths); blocks whose runs span more rows than the LDS copy
+    // holds read global memory.
     std::ostringstream init;
     Emitter ei(g, Mode::FINAL, init, "        ");
     // Tables read at the run's key: probed once per run, before its rows.
     for (size_t j = 0; j < sp.sub.probes.size(); j++) ei.probe((int)j);
+    // So are the probed tables' columns the fold reads (per run, not per row).
+    std::function<void(const N&)> pcols = [&](const N& e) {
+        if (!e) return;
+        if (e->kind == K::Sym && leaf_name(e).rfind("$pcol.", 0) == 0) {
+            ei.expr(e);
+            return;
+        }
+        for (auto& a : e->args) pcols(a);
+    };
+    for (auto& f : sp.fields) {
+        if (!f.updated) continue;
+        pcols(f.pred);
+        pcols(f.contrib);
+    }
     std::vector<std::string> iv;
     for (auto& f : sp.fields) iv.push_back(ei.expr(f.init));
-    o << init.str();
-    for (size_t f = 0; f < sp.fields.size(); f++) {
-        const std::string T = cty(sp.fields[f].type);
-        o << "        " << T << " s" << f << " = (" << T << ")" << iv[f] << ";\n";
-    }
-    // The run's rows in ascending order: the reference's SubVector loop.
-    std::ostringstream body;
-    Emitter eb(g, Mode::MOVE, body, "            ");
-    eb.ntmp = ei.ntmp;
-    eb.probes_done = ei.probes_done;
-    std::vector<std::string> pr(sp.fields.size()), ct(sp.fields.size());
-    for (size_t f = 0; f < sp.fields.size(); f++) {
-        if (!sp.fields[f].updated) continue;
-        pr[f] = eb.expr(sp.fields[f].pred);
-        ct[f] = eb.expr(sp.fields[f].contrib);
-    }
-    o << "        for (int64_t row = lo; row < hi; row++) {\n" << body.str();
-    for (size_t f = 0; f < sp.fields.size(); f++) {
-        if (!sp.fields[f].updated) continue;
-        const std::string S = "s" + std::to_string(f);
-        o << "            " << S << " = " << pr[f] << " ? "
-          << combine_expr(sp.fields[f].op, sp.fields[f].type, S, ct[f]) << " : " << S << ";\n";
-    }
-    o << "        }\n";
+    auto body = [&](bool lds, std::ostringstream& out) {
+        std::ostringstream b;
+        g.lds_rows = lds;
+        Emitter eb(g, Mode::MOVE, b, "                ");
+        eb.ntmp = ei.ntmp;
+        eb.probes_done = ei.probes_done;
+        eb.memo = ei.memo;
+        std::vector<std::string> pr(sp.fields.size()), ct(sp.fields.size());
+        for (size_t f = 0; f < sp.fields.size(); f++) {
+            if (!sp.fields[f].updated) continue;
+            pr[f] = eb.expr(sp.fields[f].pred);
+            ct[f] = eb.expr(sp.fields[f].contrib);
+        }
+        g.lds_rows = false;
+        out << "            for (int64_t row = lo; row < hi; row++) {\n" << b.str();
+        for (size_t f = 0; f < sp.fields.size(); f++) {
+            if (!sp.fields[f].updated) continue;
+            const std::string S = "s" + std::to_string(f);
+            out << "                " << S << " = " << pr[f] << " ? "
+                << combine_expr(sp.fields[f].op, sp.fields[f].type, S, ct[f]) << " : " << S
+                << ";\n";
+        }
+        out << "            }\n";
+        return (int)eb.ntmp;
+    };
+    std::ostringstream loop_lds, loop_glb;
+    int nt = body(true, loop_lds);
+    nt = std::max(nt, body(false, loop_glb));
     std::ostringstream ob;
-    Emitter eo(g, Mode::FINAL, ob, "        ");
-    eo.ntmp = eb.ntmp;
+    Emitter eo(g, Mode::FINAL, ob, "            ");
+    eo.ntmp = nt;
     eo.probes_done = ei.probes_done;
     std::vector<std::string> ov;
     for (auto& v : sp.out_values) ov.push_back(eo.expr(v));
     if (ov.size() > 16) throw Unsupported("more than 16 run reduction outputs");
+    // LDS rows per block: 6 per run (TPC-H orders hold 1..7 lines, 4 on
+    // average), at most 40 KiB over the staged columns (4 blocks per CU).
+    int row_bytes = 0;
+    for (int sl : g.lds_slots) row_bytes += ks.cols[sl].type.width();
+    int cap = 0;
+    if (row_bytes > 0) cap = std::min(1536, (40960 / row_bytes) / 256 * 256);
+    const bool staged = cap >= 256;
+
+    std::ostringstream o;
+    o << "// Generated run reduction for '" << sp.sym_name << "'.\n";
+    o << "extern \"C\" __global__ __launch_bounds__(256) void rf_seg_fold(RfSegArgs L) {\n";
+    if (staged) {
+        o << "    constexpr int64_t CAP = " << cap << ";\n";
+        for (int sl : g.lds_slots)
+            o << "    __shared__ " << cty(ks.cols[sl].type) << " sm" << sl << "[CAP];\n";
+    }
+    o << "    for (int64_t base = blockIdx.x * (int64_t)256; base < L.runs;\n"
+         "         base += (int64_t)gridDim.x * 256) {\n";
+    o << "        const int64_t run = base + threadIdx.x;\n";
+    // The run's key, its probes and the fold's init come first, so that
+    // their loads overlap the copy below.
+    o << "        const int64_t rr = run < L.runs ? run : L.runs - 1;\n";
+    o << "        const " << cty(sp.key_type) << " key = ((const " << cty(sp.key_type)
+      << "*)L.keys)[rr];\n        (void)key;\n";
+    o << init.str();
+    if (staged) {
+        o << "        const int64_t last = (base + 256 < L.runs ? base + 256 : L.runs) - 1;\n";
+        o << "        int64_t blo = base > 0 ? (int64_t)L.run_ends[base - 1] : 0;\n";
+        o << "        int64_t bhi = (int64_t)L.run_ends[last];\n";
+        o << "        blo = blo < L.rows ? blo : L.rows;\n";
+        o << "        bhi = bhi < L.rows ? bhi : L.rows;\n";
+        o << "        bhi = bhi > blo ? bhi : blo;\n";
+        o << "        const bool in_lds = bhi - blo <= CAP;\n";
+        o << "        __syncthreads();  // the previous runs' LDS reads are done\n";
+        o << "        if (in_lds) {\n";
+        o << "            for (int64_t i = threadIdx.x; i < bhi - blo; i += 256) {\n";
+        for (int sl : g.lds_slots) {
+            const std::string T = cty(ks.cols[sl].type);
+            o << "                sm" << sl << "[i] = rf_g((const " << T << "*)L.col[" << sl
+              << "])[blo + i];\n";
+        }
+        o << "            }\n";
+        o << "        }\n";
+        o << "        __syncthreads();\n";
+    }
+    o << "        if (run < L.runs) {\n";
+    o << "            int64_t lo = run > 0 ? (int64_t)L.run_ends[run - 1] : 0;\n";
+    o << "            const int64_t hi0 = (int64_t)L.run_ends[run];\n";
+    o << "            int64_t hi = hi0 < L.rows ? hi0 : L.rows;\n";
+    for (size_t f = 0; f < sp.fields.size(); f++) {
+        const std::string T = cty(sp.fields[f].type);
+        o << "            " << T << " s" << f << " = (" << T << ")" << iv[f] << ";\n";
+    }
+    if (staged) {
+        // Rows outside the block's copy are never read (run ends that do
+        // not ascend leave such runs short, not out of bounds).
+        o << "            if (in_lds) {\n";
+        o << "                lo = lo > blo ? lo : blo;\n";
+        o << "                hi = hi < bhi ? hi : bhi;\n";
+        o << loop_lds.str();
+        o << "            } else {\n";
+        o << loop_glb.str();
+        o << "            }\n";
+    } else {
+        o << loop_glb.str();
+    }
     o << ob.str();
     for (size_t j = 0; j < ov.size(); j++) {
         const std::string T = cty(sp.out_types[j]);
-        o << "        ((" << T << "*)L.out[" << j << "])[run] = (" << T << ")" << ov[j] << ";\n";
+        o << "            ((" << T << "*)L.out[" << j << "])[run] = (" << T << ")" << ov[j] << ";\n";
         ks.out_types.push_back(sp.out_types[j]);
     }
-    o << "    }\n}\n";
+    o << "        }\n    }\n}\n";
     ks.code = o.str();
     return ks;
 }
