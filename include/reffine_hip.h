@@ -186,9 +186,12 @@ int rf_table_download_column(const rf_table* t, int64_t col, void* values_out,
                              size_t values_cap, uint8_t* validity_out,
                              size_t validity_cap);
 
-/* Builds the device hash index over int64 column 0 (key -> first valid
- * row). Replaces ArrowTable2::build_index / build_vector_index
- * (include/reffine/arrow/table.h:107-138, src/vinstr/external.cpp:11-16). */
+/* Builds the device index over int64 column 0 (key -> first valid row): a
+ * rank map (a bit per key of the range, counts per 64 keys, first rows) when
+ * the column has no nulls and its keys span at most 32 values per row plus
+ * 2^20, open addressing otherwise. Replaces ArrowTable2::build_index /
+ * build_vector_index (include/reffine/arrow/table.h:107-138,
+ * src/vinstr/external.cpp:11-16). */
 int rf_table_build_index(rf_table* t);
 
 /* Looks up `n` keys in the index built above; writes the row or -1
@@ -261,7 +264,9 @@ int rf_query_run(rf_query* q, const rf_arg* args, int64_t n_args,
 /* Times `iters` back-to-back runs of the query's kernels with HIP events on
  * the library stream (no host work): `ms_per_run` is the mean device time of
  * a whole run (scan + selection gather), `ms_scan_kernel` (optional) that of
- * the scan kernel alone. */
+ * the kernel that streams the input table alone: the scan kernel, or for a
+ * function with reductions over the runs of a run-end table, the run fold
+ * (whose runs table the scan then reads). */
 int rf_query_time(rf_query* q, const rf_arg* args, int64_t n_args,
                   int32_t iters, double* ms_per_run, double* ms_scan_kernel);
 

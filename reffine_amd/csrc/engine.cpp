@@ -1779,7 +1779,12 @@ void time_query(rf_query* q, const rf_arg* args, int64_t n_args, int iters, doub
         launch(q, l, d.stream, false);
     }
     RF_HIP(hipEventRecord(e[1], d.stream));
-    for (int i = 0; i < iters; i++) launch(q, l, d.stream, true);
+    for (int i = 0; i < iters; i++) {
+        if (run_spec)  // a run stage: the fold streams the table
+            launch_seg(q, q->plan.runs->seg, *run_spec, user_args, n_args, runs.get(), d.stream);
+        else
+            launch(q, l, d.stream, true);
+    }
     RF_HIP(hipEventRecord(e[2], d.stream));
     RF_HIP(hipEventSynchronize(e[2]));
     float a = 0, b = 0;

@@ -580,7 +580,8 @@ class Query:
         return self.time_kernels(*inputs, iters=iters)[0]
 
     def time_kernels(self, *inputs, iters: int = 20) -> tuple[float, float]:
-        """(mean device ms of a run, mean ms of the scan kernel alone), from
+        """(mean device ms of a run, mean ms of the kernel that streams the
+        input table alone: the scan kernel, or a run stage's run fold), from
         HIP events on the library stream."""
         arr = self._args(inputs)
         ms, scan = ctypes.c_double(), ctypes.c_double()
