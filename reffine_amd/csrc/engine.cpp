@@ -376,7 +376,11 @@ void table_build_index(rf_table* t) {
                 RF_HIP(aot_rank_rows((const int64_t*)c.values, c.length, kmin, bits, cum,
                                      (int64_t*)rows, d.stream));
             }
+            void* pack = dev_alloc((size_t)words * 16);
+            t->owned->p.push_back(pack);
+            RF_HIP(aot_rank_pack(bits, cum, words, pack, d.stream));
             RF_HIP(hipStreamSynchronize(d.stream));
+            t->index.dense_pack = pack;
             t->index.dense_bits = bits;
             t->index.dense_cum = cum;
             t->index.dense_rows = rows;
@@ -419,6 +423,7 @@ RfProbeDev probe_of(const rf_table* t) {
     h.dense_base = t->index.dense_base;
     h.dense_len = t->index.dense_len;
     h.dense_rows = (const int64_t*)t->index.dense_rows;
+    h.dense_pack = (const uint64_t*)t->index.dense_pack;
     return h;
 }
 
@@ -1140,6 +1145,7 @@ GjBuild::~GjBuild() {
     dev_free(filter);
     dev_free(dense_bits);
     dev_free(dense_cum);
+    dev_free(dense_pack);
 }
 
 RfProbeDev GjBuild::probe() const {
@@ -1147,6 +1153,7 @@ RfProbeDev GjBuild::probe() const {
     if (dense_bits) {
         h.dense_bits = (const uint64_t*)dense_bits;
         h.dense_cum = (const uint32_t*)dense_cum;
+        h.dense_pack = (const uint64_t*)dense_pack;
         h.dense_base = kmin;
         h.dense_len = dense_len;
         return h;
@@ -1313,6 +1320,8 @@ static std::shared_ptr<GjBuild> make_build(rf_query* q, const rf_arg* args, int6
         RF_HIP(aot_rank_scatter((const int64_t*)bq->outs[0], (const int64_t*)bq->outs[1], K, kmin,
                                 B->dense_bits, words, B->dense_cum, (int64_t*)B->keys,
                                 (int64_t*)B->rows, small + 2, s));
+        B->dense_pack = dev_alloc((size_t)words * 16);
+        RF_HIP(aot_rank_pack(B->dense_bits, B->dense_cum, words, B->dense_pack, s));
         RF_HIP(hipMemcpyAsync(hs, small + 2, 8, hipMemcpyDeviceToHost, s));
         RF_HIP(hipStreamSynchronize(s));
         if (*(uint64_t*)hs != (uint64_t)K) throw IRError(dup_msg);

@@ -59,6 +59,9 @@ hipError_t aot_minmax_i64(const int64_t* keys, const int64_t* n_dev, int64_t n_m
                           hipStream_t s);
 hipError_t aot_adjacent_dups(const int64_t* keys, int64_t n, void* flag, hipStream_t s);
 hipError_t aot_not_ascending(const int64_t* keys, int64_t n, void* flag, hipStream_t s);
+// words x 16 bytes: word w's bits, then its count (RfProbeDev::dense_pack).
+hipError_t aot_rank_pack(const void* bits, const void* cum, int64_t words, void* pack,
+                         hipStream_t s);
 hipError_t aot_rank_rows(const int64_t* keys, int64_t n, int64_t base, const void* bits,
                          const void* cum, int64_t* rows, hipStream_t s);
 hipError_t aot_filter_build(const int64_t* keys, int64_t n, void* filter, uint64_t fmask,
@@ -104,6 +107,7 @@ struct HashIndex {
     void* dense_bits = nullptr;
     void* dense_cum = nullptr;
     void* dense_rows = nullptr;
+    void* dense_pack = nullptr;  // bits and counts interleaved, for probes
     int64_t dense_base = 0;
     uint64_t dense_len = 0;
 };
@@ -253,6 +257,7 @@ struct GjBuild {
     // index or filter): key bits and the words' first ranks.
     void* dense_bits = nullptr;
     void* dense_cum = nullptr;
+    void* dense_pack = nullptr;  // bits and counts interleaved, for probes
     uint64_t dense_len = 0;
     RfProbeDev probe() const;  // the probe plan's index for the external probes
     ~GjBuild();

@@ -48,6 +48,10 @@ struct RfProbeDev {
     int64_t dense_base;
     uint64_t dense_len;
     const int64_t* dense_rows;  // row of each rank; null: the rank is the row
+    // The same map packed for probes: 16 bytes per 64-bit word, the word's
+    // key bits then (low 32 bits) the marked keys before it -- one load per
+    // probe. Probes use it when it is set.
+    const uint64_t* dense_pack;
 };
 
 struct RfScanArgs {

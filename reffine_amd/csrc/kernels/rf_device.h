@@ -113,7 +113,27 @@ __device__ __forceinline__ uint64_t rf_hash64(uint64_t k) {
     return k;
 }
 
+// The bits and count of a rank map's word w in one 16-byte load.
+__device__ __forceinline__ void rf_rank_word(const RfProbeDev& h, uint64_t w, uint64_t& bits,
+                                             uint32_t& below) {
+    typedef unsigned long long u2 __attribute__((ext_vector_type(2)));
+    const u2 v = rf_g(reinterpret_cast<const u2*>(h.dense_pack))[w];
+    bits = v.x;
+    below = (uint32_t)v.y;
+}
+
 __device__ __forceinline__ int64_t rf_probe(const RfProbeDev& h, int64_t key) {
+    if (h.dense_pack) {
+        const uint64_t off = (uint64_t)key - (uint64_t)h.dense_base;
+        if (off >= h.dense_len) return -1;
+        uint64_t wd;
+        uint32_t below;
+        rf_rank_word(h, off >> 6, wd, below);
+        const uint64_t bit = 1ull << (off & 63);
+        if (!(wd & bit)) return -1;
+        const int64_t r = (int64_t)below + (int64_t)__popcll(wd & (bit - 1));
+        return h.dense_rows ? rf_g(h.dense_rows)[r] : r;
+    }
     if (h.dense_bits) {
         // Rank of the key among the marked keys (a small, cache-resident
         // structure); the offset wraps to >= dense_len for keys below the base.
@@ -151,6 +171,14 @@ __device__ __forceinline__ int64_t rf_probe_dense(const RfProbeDev& h, int64_t k
     if (!guard) return -1;
     const uint64_t off = (uint64_t)key - (uint64_t)h.dense_base;
     if (off >= h.dense_len) return -1;
+    if (h.dense_pack) {
+        uint64_t wd;
+        uint32_t below;
+        rf_rank_word(h, off >> 6, wd, below);
+        const uint64_t bit = 1ull << (off & 63);
+        if (!(wd & bit)) return -1;
+        return (int64_t)below + (int64_t)__popcll(wd & (bit - 1));
+    }
     const uint64_t wd = rf_g(h.dense_bits)[off >> 6];
     const uint64_t bit = 1ull << (off & 63);
     if (!(wd & bit)) return -1;

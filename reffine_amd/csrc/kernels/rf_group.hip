@@ -477,6 +477,17 @@ extern "C" __global__ void rf_rank_bits(const int64_t* __restrict__ keys, int64_
     }
 }
 
+// The probe form of a rank map: word w's bits and count side by side.
+extern "C" __global__ void rf_rank_pack(const uint64_t* __restrict__ bits,
+                                        const uint32_t* __restrict__ cum, int64_t words,
+                                        unsigned long long* pack) {
+    for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < words;
+         w += (int64_t)gridDim.x * blockDim.x) {
+        pack[2 * w] = bits[w];
+        pack[2 * w + 1] = cum[w];
+    }
+}
+
 // Sets *flag unless the keys are strictly ascending.
 extern "C" __global__ void rf_not_ascending(const int64_t* __restrict__ keys, int64_t n,
                                             unsigned int* flag) {
@@ -715,6 +726,15 @@ hipError_t aot_rank_build(const int64_t* keys, int64_t n, int64_t base, void* bi
     hipLaunchKernelGGL(rf_scan_chunks, dim3((unsigned)nch), dim3(256), 0, s, c, words, part);
     hipLaunchKernelGGL(rf_sort_scan, dim3(1), dim3(1024), 0, s, part, nch);
     hipLaunchKernelGGL(rf_scan_apply, dim3((unsigned)nch), dim3(256), 0, s, c, words, part);
+    return hipGetLastError();
+}
+
+hipError_t aot_rank_pack(const void* bits, const void* cum, int64_t words, void* pack,
+                         hipStream_t s) {
+    if (words <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rf_rank_pack, dim3(grid_cap(words, 256, 4096)), dim3(256), 0, s,
+                       (const uint64_t*)bits, (const uint32_t*)cum, words,
+                       (unsigned long long*)pack);
     return hipGetLastError();
 }
 
