@@ -56,8 +56,8 @@ def cpu_baseline(sample_rows: int) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--rows", type=int, default=60_000_000)
     ap.add_argument("--cpu-rows", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
