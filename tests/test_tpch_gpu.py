@@ -146,3 +146,91 @@ def test_locate_index_forms(device, form):
     out = compile_op(_locate_fn(_vec_t(1, _i64_t, _f64_t), _vec_t(1, _i64_t, _i64_t)))(T, P)
     want = np.array([first.get(k, -1) for k in probe.tolist()], dtype=np.int64)
     np.testing.assert_array_equal(out.column(1), want)
+
+
+def test_run_stage_nulls_and_a_second_table(device):
+    """A run stage over random runs with nulls in column 1 (the SubVector
+    body condition: null rows are not folded), merged with a second table
+    that lacks some keys; the runs' sums filtered and output with a column
+    of the second table. Expected values from the sequential fold of each
+    run."""
+    import pyarrow as pa
+    import pyarrow.compute as pc
+    from reffine_amd.reffiner import (_add, _f64, _f64_t, _func, _get, _gt, _i64, _i64_t, _in,
+                                      _new, _op, _red, _sym, _vec_t)
+    rng = np.random.default_rng(31)
+    lens = rng.integers(1, 40, size=3000)
+    keys = np.cumsum(rng.integers(1, 4, size=len(lens))).astype(np.int64)
+    x = np.repeat(keys, lens)
+    n = len(x)
+    y = rng.integers(-50, 50, size=n).astype(np.int64)
+    v = rng.random(n)
+    valid_y = rng.random(n) > 0.1
+    T = DeviceTable.from_arrow(pa.RecordBatch.from_arrays(
+        [pc.run_end_encode(pa.array(x)), pa.array(y, mask=~valid_y), pa.array(v)],
+        names=["x", "y", "v"]), dim=2)
+    ukeys = keys[rng.random(len(keys)) > 0.3]
+    w = rng.integers(0, 1000, size=len(ukeys)).astype(np.int64)
+    U = DeviceTable.from_columns([ukeys, w], dim=1).build_index()
+    Ts = _sym("T", _vec_t(2, _i64_t, _i64_t, _f64_t, encodings=["runend", "flat", "flat"]))
+    Us = _sym("U", _vec_t(1, _i64_t, _i64_t))
+    t = _sym("t", _i64_t)
+    red = _red(Ts[t], lambda: _new([_i64(0), _f64(0)]),
+               lambda s, r: _new([_add(_get(s, 0), _get(r, 0)), _add(_get(s, 1), _get(r, 1))]))
+    rs = _sym("agg", red)
+    op = _op([t], _in(t, Ts) & _in(t, Us) & _gt(_get(rs, 0), _i64(0)),
+             [_get(rs, 0), _get(rs, 1), _get(Us[t], 0)])
+    os_ = _sym("op", op)
+    fn = _func("runs2", os_, [Ts, Us])
+    fn.tbl[os_] = op
+    fn.tbl[rs] = red
+    out = compile_op(fn)(T, U)
+    wmap = dict(zip(ukeys.tolist(), w.tolist()))
+    ends = np.cumsum(lens)
+    starts = ends - lens
+    ek, es, ev, ew = [], [], [], []
+    for r in range(len(lens)):
+        k = int(keys[r])
+        if k not in wmap:
+            continue
+        sy, sv = 0, 0.0
+        for i in range(starts[r], ends[r]):
+            if not valid_y[i]:
+                continue  # the SubVector loop skips the row
+            sy += int(y[i])
+            sv = sv + float(v[i])
+        if sy > 0:
+            ek.append(k); es.append(sy); ev.append(sv); ew.append(wmap[k])
+    assert len(ek) > 200
+    np.testing.assert_array_equal(out.column(0), ek)
+    np.testing.assert_array_equal(out.column(1), es)
+    np.testing.assert_array_equal(out.column(2).view(np.uint64), np.array(ev).view(np.uint64))
+    np.testing.assert_array_equal(out.column(3), ew)
+
+
+def test_run_stage_deterministic_and_not_batched(device, tables):
+    """Deterministic compilation gives the same results; a batch refuses
+    a run stage with an unsupported-status error."""
+    from reffine_amd.batch import Batch
+    from reffine_amd.runtime import ReffineError
+    t, dev = tables
+    assert list(compile_op(tpch.q4(), deterministic=True)(*_args(dev, "q4"))) == ot.q4(t)
+    out = compile_op(tpch.q18(150), deterministic=True)(*_args(dev, "q18"))
+    k, v = ot.q18(t, 150)
+    np.testing.assert_array_equal(out.column(0), k)
+    np.testing.assert_array_equal(out.column(1), v)
+    q = compile_op(tpch.q18(150), deterministic=True)
+    b = Batch()
+    with pytest.raises(ReffineError):
+        b.add(q, *_args(dev, "q18"))
+
+
+def test_run_stage_empty_lineitem(device, tables):
+    """No lineitem rows: no runs, empty outputs and zero counts."""
+    t0 = datagen.tpch_tables(5, seed=2)
+    t0["lineitem"] = {c: a[:0] for c, a in t0["lineitem"].items()}
+    rb = tpch.record_batches(t0)
+    li = DeviceTable.from_arrow(rb["lineitem"], dim=2)
+    _, dev = tables
+    assert compile_op(tpch.q18(0))(li, dev["orders"]).nrows == 0
+    assert list(compile_op(tpch.q4())(li, dev["orders"])) == [0, 0, 0, 0, 0]
