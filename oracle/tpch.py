@@ -203,3 +203,39 @@ def q12_np(t: dict):
     p = od["o_orderpriority"][o]
     high = (p == 0) | (p == 1)
     return k, np.where(high, n, 0).astype(np.int32), np.where(high, 0, n).astype(np.int32)
+
+
+def example(t: dict, nation_key: int = 0, size: int = 15):
+    """tpch.h:606-682. Per partsupp run (a part's suppliers, in storage
+    order), the availqty of rows whose supplier's nation is nation_key,
+    folded in row order as int32; parts of size > `size`. Returns
+    (partkeys, quantities)."""
+    ps, part, sup = t["partsupp"], t["part"], t["supplier"]
+    prow = {int(k): i for i, k in enumerate(part["p_partkey"].tolist())}
+    srow = {int(k): i for i, k in enumerate(sup["s_suppkey"].tolist())}
+    nation = sup["s_nationkey"].tolist()
+    skey, qty = ps["ps_suppkey"].tolist(), ps["ps_availqty"].tolist()
+    psize = part["p_size"].tolist()
+    keys, vals = [], []
+    for k, lo, hi in _runs(ps["ps_partkey"]):
+        v = np.int32(0)
+        for r in range(lo, hi):
+            if int(nation[srow[skey[r]]]) == nation_key:
+                v = np.int32(v + np.int32(qty[r]))
+        if int(psize[prow[k]]) > size:
+            keys.append(k)
+            vals.append(int(v))
+    return np.array(keys, dtype=np.int64), np.array(vals, dtype=np.int32)
+
+
+def example_np(t: dict, nation_key: int = 0, size: int = 15):
+    ps, part, sup = t["partsupp"], t["part"], t["supplier"]
+    rk, lo, hi = _runs_np(ps["ps_partkey"])
+    sorder = np.argsort(sup["s_suppkey"], kind="stable")
+    si = sorder[np.searchsorted(sup["s_suppkey"][sorder], ps["ps_suppkey"])]
+    contrib = np.where(sup["s_nationkey"][si] == nation_key, ps["ps_availqty"], 0).astype(np.int32)
+    v = _fold_np(lo, hi, contrib, np.add, np.int32(0))
+    porder = np.argsort(part["p_partkey"], kind="stable")
+    pi = porder[np.searchsorted(part["p_partkey"][porder], rk)]
+    m = part["p_size"][pi] > size
+    return rk[m], v[m]

@@ -733,13 +733,33 @@ KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
     // holds read global memory.
     std::ostringstream init;
     Emitter ei(g, Mode::FINAL, init, "        ");
-    // Tables read at the run's key: probed once per run, before its rows.
-    for (size_t j = 0; j < sp.sub.probes.size(); j++) ei.probe((int)j);
-    // So are the probed tables' columns the fold reads (per run, not per row).
+    // Tables read at the run's key: probed once per run, before its rows
+    // (probes keyed by a row's columns stay in the row loop).
+    auto row_free = [](const N& e) {
+        std::function<bool(const N&)> f = [&](const N& x) -> bool {
+            if (!x) return true;
+            const std::string ln = leaf_name(x);
+            if (ln.rfind("$col", 0) == 0 || ln.rfind("$valid", 0) == 0 || ln == "$row")
+                return false;
+            for (auto& a : x->args)
+                if (!f(a)) return false;
+            return true;
+        };
+        return f(e);
+    };
+    std::set<int> per_run;
+    for (size_t j = 0; j < sp.sub.probes.size(); j++) {
+        const Probe& pr = sp.sub.probes[j];
+        if (row_free(pr.key) && row_free(pr.guard)) {
+            ei.probe((int)j);
+            per_run.insert((int)j);
+        }
+    }
+    // So are those tables' columns the fold reads (per run, not per row).
     std::function<void(const N&)> pcols = [&](const N& e) {
         if (!e) return;
         if (e->kind == K::Sym && leaf_name(e).rfind("$pcol.", 0) == 0) {
-            ei.expr(e);
+            if (per_run.count(std::stoi(split(leaf_name(e), '.')[1]))) ei.expr(e);
             return;
         }
         for (auto& a : e->args) pcols(a);

@@ -185,3 +185,49 @@ def tpch_tables(n_orders: int, seed: int = 53) -> dict:
         "c_comment": randint(seed, 32, n_cust, 0, 1 << 40, np.int64),
     }
     return {"lineitem": lineitem, "orders": orders, "customer": customer}
+
+
+def tpch_parts(n_parts: int, seed: int = 59) -> dict:
+    """TPC-H-shaped part / supplier / partsupp columns as reffine's
+    TPCHQueryExample reads them (benchmark/include/tpch.h:606-682): partsupp
+    sorted by partkey with 4 suppliers per part (its partkey run-end
+    encoded), suppliers n_parts/20; every partsupp key exists in part and
+    supplier."""
+    n_supp = max(1, n_parts // 20)
+    pk = np.arange(1, n_parts + 1, dtype=np.int64)
+    ps_part = np.repeat(pk, 4)
+    j = np.tile(np.arange(4, dtype=np.int64), n_parts)
+    # dbgen's supplier spread for a part: (p + j * (S/4 + (p-1)/S)) mod S + 1
+    ps_supp = (ps_part + j * (n_supp // 4 + (ps_part - 1) // n_supp)) % n_supp + 1
+    n_ps = len(ps_part)
+    part = {
+        "p_partkey": pk,
+        "p_name": randint(seed, 1, n_parts, 0, 1 << 40, np.int64),
+        "p_mfgr": randint(seed, 2, n_parts, 1, 6, np.int64),
+        "p_brand": randint(seed, 3, n_parts, 1, 26, np.int64),
+        "p_type": randint(seed, 4, n_parts, 0, 150, np.int64),
+        "p_size": randint(seed, 5, n_parts, 1, 51, np.int32),
+        "p_container": randint(seed, 6, n_parts, 0, 40, np.int64),
+        "p_retailprice": (90_000 + (pk % 20_001) + 100 * (pk % 1000)).astype(np.float64) / 100.0,
+        "p_comment": randint(seed, 7, n_parts, 0, 1 << 40, np.int64),
+    }
+    sk = np.arange(1, n_supp + 1, dtype=np.int64)
+    supplier = {
+        "s_suppkey": sk,
+        "s_name": randint(seed, 8, n_supp, 0, 1 << 40, np.int64),
+        "s_address": randint(seed, 9, n_supp, 0, 1 << 40, np.int64),
+        "s_nationkey": randint(seed, 10, n_supp, 0, 25, np.int64),
+        "s_phone": randint(seed, 11, n_supp, 0, 1 << 40, np.int64),
+        "s_acctbal": (randint(seed, 12, n_supp, -99_999, 1_000_000, np.int64)
+                      .astype(np.float64) / 100.0),
+        "s_comment": randint(seed, 13, n_supp, 0, 1 << 40, np.int64),
+    }
+    partsupp = {
+        "ps_partkey": ps_part,
+        "ps_suppkey": ps_supp,
+        "ps_availqty": randint(seed, 14, n_ps, 1, 10_000, np.int32),
+        "ps_supplycost": (randint(seed, 15, n_ps, 100, 100_001, np.int64)
+                          .astype(np.float64) / 100.0),
+        "ps_comment": randint(seed, 16, n_ps, 0, 1 << 40, np.int64),
+    }
+    return {"part": part, "supplier": supplier, "partsupp": partsupp}

@@ -161,10 +161,49 @@ def q18(quantity: int = 300):
     return fn
 
 
-QUERIES = {"q3": q3, "q4": q4, "q12": q12, "q18": q18}
+PART_COLS = ["p_partkey", "p_name", "p_mfgr", "p_brand", "p_type", "p_size", "p_container",
+             "p_retailprice", "p_comment"]
+SUPPLIER_COLS = ["s_suppkey", "s_name", "s_address", "s_nationkey", "s_phone", "s_acctbal",
+                 "s_comment"]
+PARTSUPP_COLS = ["ps_partkey", "ps_suppkey", "ps_availqty", "ps_supplycost", "ps_comment"]
+PART_TYPE = _vec_t(1, _i64_t, _i64_t, _i64_t, _i64_t, _i64_t, _i32_t, _i64_t, _f64_t, _i64_t)
+SUPPLIER_TYPE = _vec_t(1, _i64_t, _i64_t, _i64_t, _i64_t, _i64_t, _f64_t, _i64_t)
+PARTSUPP_TYPE = _vec_t(2, _i64_t, _i64_t, _i32_t, _f64_t, _i64_t,
+                       encodings=["runend", "flat", "flat", "flat", "flat"])
+
+
+def example(nation_key: int = 0, size: int = 15):
+    """tpch.h:606-682 (TPCHQueryExample::build_op): per part larger than
+    `size`, the available quantity of its partsupp rows whose supplier is in
+    nation `nation_key` (a probe of supplier by each row's supplier key
+    inside the fold of the part's run). Output (partkey, quantity i32)."""
+    part = _sym("part", PART_TYPE)
+    supplier = _sym("supplier", SUPPLIER_TYPE)
+    partsupp = _sym("partsupp", PARTSUPP_TYPE)
+    partkey = _sym("partkey", _i64_t)
+
+    def acc(s, v):
+        nkey = _get(supplier[_get(v, 0)], 2)
+        return _sel(_eq(nkey, _i64(nation_key)), _add(s, _get(v, 1)), s)
+
+    value = _red(partsupp[partkey], lambda: _i32(0), acc)
+    value_sym = _sym("value", value)
+    filt = _gt(_get(part[partkey], 4), _i32(size))
+    filter_sym = _sym("filter", filt)
+    op = _op([partkey], _in(partkey, partsupp) & filter_sym, [value_sym])
+    op_sym = _sym("op", op)
+    fn = _func("tpchexample", op_sym, [part, supplier, partsupp])
+    fn.tbl[op_sym] = op
+    fn.tbl[value_sym] = value
+    fn.tbl[filter_sym] = filt
+    return fn
+
+
+QUERIES = {"q3": q3, "q4": q4, "q12": q12, "q18": q18, "example": example}
 # Tables each query reads, in argument order.
 ARGS = {"q3": ("lineitem", "orders", "customer"), "q4": ("lineitem", "orders"),
-        "q12": ("lineitem", "orders"), "q18": ("lineitem", "orders")}
+        "q12": ("lineitem", "orders"), "q18": ("lineitem", "orders"),
+        "example": ("part", "supplier", "partsupp")}
 
 
 def record_batches(tables: dict) -> dict:
@@ -177,6 +216,21 @@ def record_batches(tables: dict) -> dict:
     arrays += [pa.array(li[c]) for c in LINEITEM_COLS[1:]]
     out = {"lineitem": pa.RecordBatch.from_arrays(arrays, names=LINEITEM_COLS)}
     for name, cols in (("orders", ORDERS_COLS), ("customer", CUSTOMER_COLS)):
+        out[name] = pa.RecordBatch.from_arrays([pa.array(tables[name][c]) for c in cols],
+                                               names=cols)
+    return out
+
+
+def part_batches(tables: dict) -> dict:
+    """pyarrow record batches of tpch_parts' columns: partsupp with
+    ps_partkey run-end encoded (int32 run ends), part and supplier flat."""
+    import pyarrow as pa
+    import pyarrow.compute as pc
+    ps = tables["partsupp"]
+    arrays = [pc.run_end_encode(pa.array(ps["ps_partkey"]), run_end_type=pa.int32())]
+    arrays += [pa.array(ps[c]) for c in PARTSUPP_COLS[1:]]
+    out = {"partsupp": pa.RecordBatch.from_arrays(arrays, names=PARTSUPP_COLS)}
+    for name, cols in (("part", PART_COLS), ("supplier", SUPPLIER_COLS)):
         out[name] = pa.RecordBatch.from_arrays([pa.array(tables[name][c]) for c in cols],
                                                names=cols)
     return out
@@ -201,3 +255,13 @@ def algorithmic_bytes(query: str, tables: dict, out_rows: int = 0) -> int:
     if query == "q18":
         return runs + 8 * n_li + 8 * n_o + 16 * out_rows
     raise KeyError(query)
+
+
+def example_bytes(tables: dict, out_rows: int = 0) -> int:
+    """Algorithmic bytes of the Example query: partsupp run values + ends
+    (12 B per part), suppkey + availqty per partsupp row (12 B), supplier
+    nationkey (8 B per supplier), part size (4 B per part), output rows."""
+    n_ps = len(tables["partsupp"]["ps_partkey"])
+    n_p = len(tables["part"]["p_partkey"])
+    n_s = len(tables["supplier"]["s_suppkey"])
+    return 12 * n_p + 12 * n_ps + 8 * n_s + 4 * n_p + 12 * out_rows

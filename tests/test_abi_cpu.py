@@ -168,7 +168,7 @@ def test_c_host_program_compiles_against_the_header():
     assert r.returncode == 0, r.stderr
 
 
-@pytest.mark.parametrize("name", ["q3", "q4", "q12", "q18"])
+@pytest.mark.parametrize("name", ["q3", "q4", "q12", "q18", "example"])
 def test_tpch_run_stage_queries_compile(name):
     """TPC-H Q3/Q4/Q12/Q18 over the run-end lineitem (benchmark/include/
     tpch.h) lower to a run stage -- a fold per lineitem run, then a scan of
@@ -177,9 +177,12 @@ def test_tpch_run_stage_queries_compile(name):
     text = runtime.compile_check(tpch.QUERIES[name]())
     assert "run stage: the runs table" in text
     assert text.count("--- kernel ---") == 2
-    assert "probe 0: table input 1" in text
+    assert "probe 0: table input" in text
     if name == "q12":  # the fold reads orders at the run's key
         assert "probe 0: table input 1 at key (sym $key i64)" in text
+    if name == "example":  # supplier probed by each partsupp row inside the fold
+        assert "probe 0: table input 1 at key (sym $col.2.1 i64)" in text
+        assert "rf_probe(L.probe[0], (int64_t)sm1[row - blo])" in text
     if name == "q3":  # _locate / _readdata into customer
         assert "probe 1: table input 2 key (sym $pcol.0.1 i64)" in text
         assert "(sym $pcol.1.6 i8)" in text

@@ -258,3 +258,25 @@ def test_tpch_vectorised_oracle_matches_the_row_loops():
         for x, y in zip(loop(t, *a), vec(t, *a)):
             np.testing.assert_array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
     assert ot.q4(t) == ot.q4_np(t)
+
+
+def test_tpch_example_oracle_against_pandas():
+    """oracle.tpch.example (row loops) against a pandas merge + group-by, and
+    its numpy form against the loops."""
+    import pandas as pd
+    from oracle import tpch as ot
+    t = datagen.tpch_parts(4000, seed=8)
+    ps, part, sup = (pd.DataFrame(t["partsupp"]), pd.DataFrame(t["part"]),
+                     pd.DataFrame(t["supplier"]))
+    for nation, size in ((0, 15), (7, 30)):
+        m = ps.merge(sup[["s_suppkey", "s_nationkey"]], left_on="ps_suppkey", right_on="s_suppkey")
+        m["q"] = np.where(m.s_nationkey == nation, m.ps_availqty, 0)
+        tot = m.groupby("ps_partkey").q.sum()
+        pp = part[part.p_size > size].sort_values("p_partkey")
+        pp = pp[pp.p_partkey.isin(tot.index)]
+        k, v = ot.example(t, nation, size)
+        np.testing.assert_array_equal(k, pp.p_partkey.to_numpy())
+        np.testing.assert_array_equal(v, tot.loc[pp.p_partkey].to_numpy())
+        kn, vn = ot.example_np(t, nation, size)
+        np.testing.assert_array_equal(k, kn)
+        np.testing.assert_array_equal(v, vn)

@@ -234,3 +234,19 @@ def test_run_stage_empty_lineitem(device, tables):
     _, dev = tables
     assert compile_op(tpch.q18(0))(li, dev["orders"]).nrows == 0
     assert list(compile_op(tpch.q4())(li, dev["orders"])) == [0, 0, 0, 0, 0]
+
+
+@pytest.mark.parametrize("nation,size", [(0, 15), (3, 40), (24, 0)])
+def test_example(device, nation, size):
+    """TPCHQueryExample (tpch.h:606-682): the fold over a part's partsupp
+    run probes supplier by each row's supplier key."""
+    t = datagen.tpch_parts(20000, seed=4)
+    rb = tpch.part_batches(t)
+    dev = {"part": DeviceTable.from_arrow(rb["part"]).build_index(),
+           "supplier": DeviceTable.from_arrow(rb["supplier"]).build_index(),
+           "partsupp": DeviceTable.from_arrow(rb["partsupp"], dim=2)}
+    out = compile_op(tpch.example(nation, size))(*[dev[a] for a in tpch.ARGS["example"]])
+    k, v = ot.example(t, nation, size)
+    assert len(k) > 1000 and v.sum() > 0
+    np.testing.assert_array_equal(out.column(0), k)
+    np.testing.assert_array_equal(out.column(1), v)
