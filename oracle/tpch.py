@@ -239,3 +239,74 @@ def example_np(t: dict, nation_key: int = 0, size: int = 15):
     pi = porder[np.searchsorted(part["p_partkey"][porder], rk)]
     m = part["p_size"][pi] > size
     return rk[m], v[m]
+
+
+# ---------------------------------------------------------------------------
+# The SubVector folds of tpch.h: Q1, Q6 and TPC-DS Q9. Counts exact; float64
+# sums correctly rounded (math.fsum), the value the GPU's tree order is
+# checked against within 1e-12.
+
+def _fsum(x) -> float:
+    import math
+    return math.fsum(np.asarray(x, dtype=np.float64).tolist()) if len(x) else 0.0
+
+
+def q1(t: dict):
+    """tpch.h:398-486: rows [0, number of lineitem runs) (_len of the
+    run-end column 0), three l_returnflag groups of SUM(qty), SUM(price),
+    SUM(price*(1-disc)), SUM(price*(1-disc)*(1+tax)), COUNT."""
+    li = t["lineitem"]
+    n = len(_runs_np(li["l_orderkey"])[0])
+    rf = li["l_returnflag"][:n]
+    qty, ep = li["l_quantity"][:n], li["l_extendedprice"][:n]
+    disc, tax = li["l_discount"][:n], li["l_tax"][:n]
+    disc_price = ep * (1.0 - disc)
+    charge = disc_price * (1.0 + tax)
+    out = []
+    for g in range(3):
+        m = rf == g
+        out += [_fsum(qty[m]), _fsum(ep[m]), _fsum(disc_price[m]), _fsum(charge[m]),
+                float(m.sum())]
+    return out
+
+
+def q6(t: dict, start: int = 820454400, end: int = 852076800, disc: float = 0.05,
+       quant: float = 24.5):
+    """tpch.h:160-216: SUM(price * discount) over the qualifying lines."""
+    li = t["lineitem"]
+    d, q = li["l_discount"], li["l_quantity"]
+    m = ((li["l_shipdate"] >= start) & (li["l_shipdate"] < end) & (d >= disc - 0.01) &
+         (d <= disc + 0.01) & (q < quant))
+    return _fsum(li["l_extendedprice"][m] * d[m])
+
+
+def tpcds_q9(ss: dict):
+    """tpch.h:218-313: five quantity buckets of (COUNT, SUM(ext_tax),
+    SUM(net_paid_inc_tax))."""
+    q = ss["ss_quantity"]
+    out = []
+    for b in range(5):
+        m = (q >= 20 * b + 1) & (q <= 20 * b + 20)
+        out.append((int(m.sum()), _fsum(ss["ss_ext_tax"][m]), _fsum(ss["ss_net_paid_inc_tax"][m])))
+    return out
+
+
+def q11(t: dict, nation_key: int = 0):
+    """tpch.h:315-396 (the function's output, op): per partsupp run, the
+    sum of availqty * supplycost over rows whose supplier is in
+    nation_key, folded in row order; parts with a positive sum."""
+    ps, sup = t["partsupp"], t["supplier"]
+    srow = {int(k): i for i, k in enumerate(sup["s_suppkey"].tolist())}
+    nation = sup["s_nationkey"].tolist()
+    skey, qty = ps["ps_suppkey"].tolist(), ps["ps_availqty"].tolist()
+    cost = ps["ps_supplycost"].tolist()
+    keys, vals = [], []
+    for k, lo, hi in _runs(ps["ps_partkey"]):
+        s = 0.0
+        for r in range(lo, hi):
+            if int(nation[srow[skey[r]]]) == nation_key:
+                s = s + float(qty[r]) * cost[r]
+        if s > 0.0:
+            keys.append(k)
+            vals.append(s)
+    return np.array(keys, dtype=np.int64), np.array(vals, dtype=np.float64)

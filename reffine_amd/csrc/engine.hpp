@@ -124,7 +124,9 @@ struct rf_table {
     rf::HashIndex index;
     bool has_index = false;
     rf::Type type() const;
-    int64_t rows() const { return cols.empty() ? 0 : cols[0].length; }
+    // Rows of the table: the logical length (a run-end column 0 holds fewer
+    // runs than rows).
+    int64_t rows() const { return cols.empty() ? 0 : cols[0].logical_length; }
 };
 
 namespace rf {

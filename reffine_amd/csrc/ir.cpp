@@ -256,6 +256,14 @@ N mk_readdata(N vec, N idx, int64_t col) {
     return n;
 }
 
+N mk_initval(std::vector<N> inits, N val) {
+    for (auto& i : inits) check(i->kind == K::Sym, "InitVal initialises symbols");
+    auto n = std::make_shared<Node>(K::InitVal, val->type);
+    n->iters = std::move(inits);
+    n->args = {val};
+    return n;
+}
+
 N mk_op(std::vector<N> iters, N pred, std::vector<N> outputs) {
     std::vector<Type> d;
     std::vector<Enc> e;
@@ -458,6 +466,12 @@ struct Builder {
         if (h == "subvec") return mk_subvec(arg(1), arg(2), arg(3));
         if (h == "len") return mk_len(arg(1), integer(e.items.at(2)));
         if (h == "locate") return mk_locate(arg(1), arg(2));
+        if (h == "initval") {
+            check(e.items.size() == 3, "(initval (SYMS) VAL)");
+            std::vector<N> inits;
+            for (auto& i : e.items[1].items) inits.push_back(sym(i));
+            return mk_initval(inits, expr(e.items[2]));
+        }
         if (h == "readdata") return mk_readdata(arg(1), arg(2), integer(e.items.at(3)));
         if (h == "op") {
             check(e.items.size() == 4, "(op (ITERS) PRED (OUTPUTS))");
@@ -551,6 +565,12 @@ std::string to_text(const N& n) {
             return "(len " + to_text(n->args[0]) + " " + std::to_string(n->idx) + ")";
         case K::Locate:
             return "(locate " + to_text(n->args[0]) + " " + to_text(n->args[1]) + ")";
+        case K::InitVal: {
+            std::string s = "(initval (";
+            for (size_t i = 0; i < n->iters.size(); i++)
+                s += (i ? " " : "") + to_text(n->iters[i]);
+            return s + ") " + to_text(n->args[0]) + ")";
+        }
         case K::ReadData:
             return "(readdata " + to_text(n->args[0]) + " " + to_text(n->args[1]) + " " +
                    std::to_string(n->idx) + ")";

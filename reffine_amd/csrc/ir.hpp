@@ -88,7 +88,7 @@ enum class MathOp {
 
 enum class K {
     Sym, Const, Cast, Select, Get, New, Nary, Element, In, SubVector,
-    Length, Op, Reduce, Func, Locate, ReadData
+    Length, Op, Reduce, Func, Locate, ReadData, InitVal
 };
 
 struct Node;
@@ -127,6 +127,9 @@ N mk_len(N vec, int64_t col);
 N mk_locate(N vec, N iter);
 // include/reffine/ir/op_to_loop.h:27-41: column `col` of vec at row `idx`.
 N mk_readdata(N vec, N idx, int64_t col);
+// include/reffine/ir/expr.h:16-26: `val`, with the symbols `inits` computed
+// first (iters holds them).
+N mk_initval(std::vector<N> inits, N val);
 N mk_op(std::vector<N> iters, N pred, std::vector<N> outputs);
 N mk_reduce(N vec, N init, N state_sym, N val_sym, N acc);
 N mk_func(const std::string& name, N output, std::vector<N> inputs,

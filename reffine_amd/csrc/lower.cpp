@@ -217,6 +217,9 @@ struct Lowerer {
                 return leaf("$len." + std::to_string(t) + "." + std::to_string(e->idx),
                             types::IDX_T);
             }
+            case K::InitVal:
+                // The initialised symbols are lowered where they are used.
+                return rw(e->args[0]);
             case K::Locate: {
                 // vector_locate (include/reffine/vinstr/vinstr.h:137-146): a
                 // probe of the table's index; -1 when the key is absent.

@@ -150,7 +150,7 @@ def tpch_tables(n_orders: int, seed: int = 53) -> dict:
         "l_extendedprice": np.round(qty * pprice, 2),
         "l_discount": randint(seed, 12, nl, 0, 11, np.int64).astype(np.float64) / 100.0,
         "l_tax": randint(seed, 13, nl, 0, 9, np.int64).astype(np.float64) / 100.0,
-        "l_returnflag": randint(seed, 14, nl, 0, 3, np.int8),
+        "l_returnflag": randint(seed, 14, nl, 0, 3, np.int32),
         "l_linestatus": randint(seed, 15, nl, 0, 2, np.int8),
         "l_shipdate": ship,
         "l_commitdate": odate[lo] + randint(seed, 16, nl, 30, 91, np.int64) * _DAY,
@@ -231,3 +231,25 @@ def tpch_parts(n_parts: int, seed: int = 59) -> dict:
         "ps_comment": randint(seed, 16, n_ps, 0, 1 << 40, np.int64),
     }
     return {"part": part, "supplier": supplier, "partsupp": partsupp}
+
+
+def store_sales(n: int, seed: int = 61) -> dict:
+    """TPC-DS-shaped store_sales columns as reffine's TPCDSQuery9 reads them
+    (benchmark/include/tpch.h:218-313): 23 columns in TPC-DS order, keys as
+    int64, ss_quantity int32 in [1, 100], amounts float64 in cents."""
+    names = ["ss_sold_date_sk", "ss_sold_time_sk", "ss_item_sk", "ss_customer_sk", "ss_cdemo_sk",
+             "ss_hdemo_sk", "ss_addr_sk", "ss_store_sk", "ss_promo_sk", "ss_ticket_number",
+             "ss_quantity", "ss_wholesale_cost", "ss_list_price", "ss_sales_price",
+             "ss_ext_discount_amt", "ss_ext_sales_price", "ss_ext_wholesale_cost",
+             "ss_ext_list_price", "ss_ext_tax", "ss_coupon_amt", "ss_net_paid",
+             "ss_net_paid_inc_tax", "ss_net_profit"]
+    out = {}
+    for c, name in enumerate(names):
+        if c < 10:
+            out[name] = randint(seed, c + 1, n, 0, 1 << 30, np.int64)
+        elif c == 10:
+            out[name] = randint(seed, c + 1, n, 1, 101, np.int32)
+        else:
+            out[name] = randint(seed, c + 1, n, 0, 2_000_000, np.int64).astype(np.float64) / 100.0
+    out["ss_sold_date_sk"] = np.sort(out["ss_sold_date_sk"])
+    return out

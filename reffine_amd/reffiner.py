@@ -353,6 +353,20 @@ class ReadData(Expr):
         return f"(readdata {self.vec.ir()} {self.idx.ir()} {self.col})"
 
 
+class InitVal(Expr):
+    """include/reffine/ir/expr.h:16-26: `val`, with the symbols `inits`
+    computed first."""
+
+    def __init__(self, inits: Sequence[Sym], val: Expr):
+        inits = list(inits)
+        _check(all(isinstance(i, Sym) for i in inits), "InitVal initialises symbols")
+        super().__init__(val.type)
+        self.inits, self.val = inits, val
+
+    def ir(self):
+        return "(initval (" + " ".join(i.ir() for i in self.inits) + ") " + self.val.ir() + ")"
+
+
 class Op(Expr):
     """include/reffine/ir/op.h:13-53."""
 
@@ -446,6 +460,7 @@ def _in(it, vec) -> In: return In(it, vec)
 def _subvec(vec, start, end) -> SubVector: return SubVector(vec, start, end)
 def _len(vec, col: int) -> Length: return Length(vec, col)
 def _locate(vec, it) -> Locate: return Locate(vec, it)
+def _initval(inits, val) -> InitVal: return InitVal(inits, val)
 def _readdata(vec, idx, col: int) -> ReadData: return ReadData(vec, idx, col)
 def _op(iters, pred, outputs) -> Op: return Op(iters, pred, outputs)
 def _red(vec, init, acc) -> Reduce: return Reduce(vec, init, acc)

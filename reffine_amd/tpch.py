@@ -18,9 +18,10 @@ from __future__ import annotations
 
 import numpy as np
 
-from .reffiner import (_add, _and, _eq, _f64, _f64_t, _false, _func, _get, _gt, _gte, _i8,
-                       _i8_t, _i32, _i32_t, _i64, _i64_t, _idx, _in, _locate, _lt, _mul, _new,
-                       _not, _op, _or, _readdata, _red, _sel, _sub, _sym, _vec_t)
+from .reffiner import (_add, _and, _cast, _eq, _f64, _f64_t, _false, _func, _get, _gt, _gte,
+                       _i8, _i8_t, _i32, _i32_t, _i64, _i64_t, _idx, _in, _initval, _len,
+                       _locate, _lt, _lte, _mul, _new, _not, _op, _or, _readdata, _red, _sel,
+                       _sub, _subvec, _sym, _vec_t)
 
 LINEITEM_COLS = ["l_orderkey", "l_partkey", "l_suppkey", "l_linenumber", "l_quantity",
                  "l_extendedprice", "l_discount", "l_tax", "l_returnflag", "l_linestatus",
@@ -32,7 +33,7 @@ CUSTOMER_COLS = ["c_custkey", "c_name", "c_address", "c_nationkey", "c_phone", "
                  "c_mktsegment", "c_comment"]
 
 LINEITEM_TYPE = _vec_t(2, _i64_t, _i64_t, _i64_t, _i64_t, _f64_t, _f64_t, _f64_t, _f64_t,
-                       _i8_t, _i8_t, _i64_t, _i64_t, _i64_t, _i8_t, _i8_t, _i64_t,
+                       _i32_t, _i8_t, _i64_t, _i64_t, _i64_t, _i8_t, _i8_t, _i64_t,
                        encodings=["runend"] + ["flat"] * 15)
 ORDERS_TYPE = _vec_t(1, _i64_t, _i64_t, _i8_t, _f64_t, _i64_t, _i8_t, _i64_t, _i32_t, _i64_t)
 CUSTOMER_TYPE = _vec_t(1, _i64_t, _i64_t, _i64_t, _i64_t, _i64_t, _f64_t, _i8_t, _i64_t)
@@ -199,11 +200,133 @@ def example(nation_key: int = 0, size: int = 15):
     return fn
 
 
-QUERIES = {"q3": q3, "q4": q4, "q12": q12, "q18": q18, "example": example}
+def q1():
+    """tpch.h:398-486 (TPCHQuery1::build_op), as written: a fold of lineitem
+    rows [0, _len(lineitem, 0)) -- for the run-end column 0 that length is
+    its number of runs (src/pass/readwritepass.cpp, Length of a run-end
+    child), so the reference folds that many leading rows -- into three
+    groups by l_returnflag of (SUM(qty), SUM(price), SUM(disc_price),
+    SUM(charge), COUNT) as 15 float64 fields."""
+    lineitem = _sym("lineitem", LINEITEM_TYPE)
+
+    def acc(s, v):
+        rf, qty, eprice, disc, tax = _get(v, 7), _get(v, 3), _get(v, 4), _get(v, 5), _get(v, 6)
+        base = eprice
+        disc_price = _mul(eprice, _sub(_f64(1), disc))
+        charge = _mul(disc_price, _add(_f64(1), tax))
+        out = []
+        for g in range(3):
+            for j, x in enumerate((qty, base, disc_price, charge, _f64(1))):
+                f = _get(s, 5 * g + j)
+                out.append(_sel(_eq(rf, _i32(g)), _add(f, x), f))
+        return _new(out)
+
+    red = _red(_subvec(lineitem, _idx(0), _len(lineitem, 0)), lambda: _new([_f64(0)] * 15), acc)
+    red_sym = _sym("red", red)
+    fn = _func("tpchquery1", red_sym, [lineitem])
+    fn.tbl[red_sym] = red
+    return fn
+
+
+def q6(start: int = 820454400, end: int = 852076800, disc: float = 0.05, quant: float = 24.5):
+    """tpch.h:160-216 (TPCHQuery6::build_op): SUM(price * discount) over the
+    lines shipped in [start, end) with discount within 0.01 of `disc` and
+    quantity below `quant`. The reference passes 0.05f and 24.5f; its C
+    text prints constants with 6 decimals, so 0.05 is what its loop uses."""
+    vec_in_sym = _sym("lineitem", LINEITEM_TYPE)
+
+    def acc(s, v):
+        date_pred = _gte(_get(v, 9), _i64(start)) & _lt(_get(v, 9), _i64(end))
+        discount_pred = (_gte(_get(v, 5), _sub(_f64(disc), _f64(0.01))) &
+                         _lte(_get(v, 5), _add(_f64(disc), _f64(0.01))))
+        pred = date_pred & discount_pred & _lt(_get(v, 3), _f64(quant))
+        return _sel(pred, _add(s, _mul(_get(v, 4), _get(v, 5))), s)
+
+    red = _red(_subvec(vec_in_sym, _idx(0), _len(vec_in_sym, 1)), lambda: _f64(0), acc)
+    red_sym = _sym("red", red)
+    fn = _func("tpchquery6", red_sym, [vec_in_sym])
+    fn.tbl[red_sym] = red
+    return fn
+
+
+STORE_SALES_TYPE = _vec_t(2, *([_i64_t] * 10 + [_i32_t] + [_f64_t] * 12))
+
+
+def tpcds_q9():
+    """tpch.h:218-313 (TPCDSQuery9::build_op): five ss_quantity buckets
+    (1-20, ..., 81-100) of (COUNT int32, SUM(ss_ext_tax), SUM(ss_net_paid_inc_tax))."""
+    vec_in_sym = _sym("store_sales", STORE_SALES_TYPE)
+
+    def acc(s, v):
+        q, tax, paid = _get(v, 9), _get(v, 17), _get(v, 20)
+        out = []
+        for b in range(5):
+            bk = _get(s, b)
+            out.append(_sel(_gte(q, _i32(20 * b + 1)) & _lte(q, _i32(20 * b + 20)),
+                            _new([_add(_get(bk, 0), _i32(1)), _add(_get(bk, 1), tax),
+                                  _add(_get(bk, 2), paid)]), bk))
+        return _new(out)
+
+    red = _red(_subvec(vec_in_sym, _idx(0), _len(vec_in_sym, 1)),
+               lambda: _new([_new([_i32(0), _f64(0), _f64(0)]) for _ in range(5)]), acc)
+    red_sym = _sym("red", red)
+    fn = _func("tpcdsquery9", red_sym, [vec_in_sym])
+    fn.tbl[red_sym] = red
+    return fn
+
+
+def q11(nation_key: int = 0, fraction: float = 0.0001):
+    """tpch.h:315-396 (TPCHQuery11::build_op), as written: the function's
+    output is the per-part value over its partsupp run (availqty * supplycost
+    of the rows whose supplier is in `nation_key`), for values above 0. The
+    threshold and the second Op are in the symbol table but not reachable
+    from the output, so they are parsed and not run (as in the reference,
+    whose loop generation starts from the output)."""
+    supplier = _sym("supplier", SUPPLIER_TYPE)
+    partsupp = _sym("partsupp", PARTSUPP_TYPE)
+    partkey = _sym("partkey", _i64_t)
+
+    def acc(s, v):
+        nkey = _get(supplier[_get(v, 0)], 2)
+        cost = _get(v, 2)
+        qty = _cast(_f64_t, _get(v, 1))
+        return _sel(_eq(nkey, _i64(nation_key)), _add(s, _mul(qty, cost)), s)
+
+    value = _red(partsupp[partkey], lambda: _f64(0), acc)
+    value_sym = _sym("value", value)
+    filt = _gt(value_sym, _f64(0))
+    filter_sym = _sym("filter", filt)
+    op = _op([partkey], _in(partkey, partsupp) & filter_sym, [value_sym])
+    op_sym = _sym("op", op)
+    threshold = _initval([op_sym], _red(op_sym, lambda: _f64(0),
+                                        lambda s, v: _add(s, _get(v, 1))))
+    threshold_sym = _sym("threshold", threshold)
+    partkey2 = _sym("partkey2", _i64_t)
+    val = _get(op_sym[partkey2], 0)
+    val_sym = _sym("val", val)
+    filter2 = _gt(val_sym, _mul(threshold_sym, _f64(fraction)))
+    filter2_sym = _sym("filter2", filter2)
+    op2 = _initval([threshold_sym], _op([partkey2], _in(partkey2, op_sym) & filter2_sym,
+                                        [val_sym]))
+    op2_sym = _sym("op2", op2)
+    fn = _func("tpchquery11", op_sym, [supplier, partsupp])
+    fn.tbl[op_sym] = op
+    fn.tbl[value_sym] = value
+    fn.tbl[filter_sym] = filt
+    fn.tbl[threshold_sym] = threshold
+    fn.tbl[op2_sym] = op2
+    fn.tbl[filter2_sym] = filter2
+    fn.tbl[val_sym] = val
+    return fn
+
+
+QUERIES = {"q3": q3, "q4": q4, "q12": q12, "q18": q18, "example": example, "q1": q1,
+           "q6": q6, "q11": q11}
 # Tables each query reads, in argument order.
 ARGS = {"q3": ("lineitem", "orders", "customer"), "q4": ("lineitem", "orders"),
         "q12": ("lineitem", "orders"), "q18": ("lineitem", "orders"),
-        "example": ("part", "supplier", "partsupp")}
+        "example": ("part", "supplier", "partsupp"), "q1": ("lineitem",), "q6": ("lineitem",),
+        "q11": ("supplier", "partsupp")}
 
 
 def record_batches(tables: dict) -> dict:
@@ -219,6 +342,12 @@ def record_batches(tables: dict) -> dict:
         out[name] = pa.RecordBatch.from_arrays([pa.array(tables[name][c]) for c in cols],
                                                names=cols)
     return out
+
+
+def store_sales_batch(cols: dict):
+    """A pyarrow record batch of datagen.store_sales' columns (flat)."""
+    import pyarrow as pa
+    return pa.RecordBatch.from_arrays([pa.array(a) for a in cols.values()], names=list(cols))
 
 
 def part_batches(tables: dict) -> dict:

@@ -280,3 +280,42 @@ def test_tpch_example_oracle_against_pandas():
         kn, vn = ot.example_np(t, nation, size)
         np.testing.assert_array_equal(k, kn)
         np.testing.assert_array_equal(v, vn)
+
+
+def test_tpch_fold_oracles_against_loops():
+    """oracle.tpch's Q1 / Q6 / TPC-DS Q9 (numpy masks + fsum) against plain
+    row loops with math.fsum on small tables, and Q11 (row loops) against a
+    pandas merge + group-by."""
+    import math
+    import pandas as pd
+    from oracle import tpch as ot
+    t = datagen.tpch_tables(1500, seed=13)
+    li = t["lineitem"]
+    n = len(np.unique(li["l_orderkey"]))  # runs of the sorted key column
+    acc = [[] for _ in range(15)]
+    for r in range(n):
+        g = int(li["l_returnflag"][r])
+        ep, disc, tax = li["l_extendedprice"][r], li["l_discount"][r], li["l_tax"][r]
+        dp = ep * (1.0 - disc)
+        for j, x in enumerate((li["l_quantity"][r], ep, dp, dp * (1.0 + tax), 1.0)):
+            acc[5 * g + j].append(x)
+    assert ot.q1(t) == [math.fsum(a) for a in acc]
+    s6 = [li["l_extendedprice"][r] * li["l_discount"][r] for r in range(len(li["l_orderkey"]))
+          if 820454400 <= li["l_shipdate"][r] < 852076800 and 0.04 <= li["l_discount"][r] <= 0.06
+          and li["l_quantity"][r] < 24.5]
+    assert s6 and ot.q6(t) == math.fsum(s6)
+    ss = datagen.store_sales(3000, seed=3)
+    for b, (c, a, p) in enumerate(ot.tpcds_q9(ss)):
+        rows = [r for r in range(3000) if 20 * b + 1 <= ss["ss_quantity"][r] <= 20 * b + 20]
+        assert c == len(rows)
+        assert a == math.fsum(ss["ss_ext_tax"][r] for r in rows)
+        assert p == math.fsum(ss["ss_net_paid_inc_tax"][r] for r in rows)
+    tp = datagen.tpch_parts(3000, seed=2)
+    ps, sup = pd.DataFrame(tp["partsupp"]), pd.DataFrame(tp["supplier"])
+    m = ps.merge(sup[["s_suppkey", "s_nationkey"]], left_on="ps_suppkey", right_on="s_suppkey")
+    m["v"] = np.where(m.s_nationkey == 4, m.ps_availqty.astype(np.float64) * m.ps_supplycost, 0.0)
+    tot = m.groupby("ps_partkey").v.sum()
+    tot = tot[tot > 0]
+    k, v = ot.q11(tp, 4)
+    np.testing.assert_array_equal(k, tot.index.to_numpy())
+    np.testing.assert_allclose(v, tot.to_numpy(), rtol=1e-12)

@@ -250,3 +250,52 @@ def test_example(device, nation, size):
     assert len(k) > 1000 and v.sum() > 0
     np.testing.assert_array_equal(out.column(0), k)
     np.testing.assert_array_equal(out.column(1), v)
+
+
+def _close(got, want, rel=1e-12):
+    return abs(got - want) <= rel * max(abs(want), 1e-300)
+
+
+def test_q1_and_q6(device, tables):
+    """Q1 and Q6 (tpch.h:160-216, 398-486) as written, over the same
+    lineitem: Q1 folds the first (number of runs) rows, the reference's
+    _len of its run-end column 0."""
+    t, dev = tables
+    got = compile_op(tpch.q1())(dev["lineitem"])
+    want = ot.q1(t)
+    flat = [x for x in (got if isinstance(got, (list, tuple)) else [got])]
+    assert len(flat) == 15
+    for g, w in zip(flat, want):
+        assert _close(g, w), (g, w)
+    assert sum(want[4::5]) == len(np.unique(t["lineitem"]["l_orderkey"]))
+    got6 = compile_op(tpch.q6())(dev["lineitem"])
+    assert _close(got6, ot.q6(t))
+    assert ot.q6(t) > 0
+
+
+def test_tpcds_q9(device):
+    ss = datagen.store_sales(300_000, seed=7)
+    T = DeviceTable.from_arrow(tpch.store_sales_batch(ss), dim=2)
+    got = compile_op(tpch.tpcds_q9())(T)
+    want = ot.tpcds_q9(ss)
+    flat = []
+    for x in got:
+        flat += list(x) if isinstance(x, (list, tuple)) else [x]
+    got = [flat[3 * b:3 * b + 3] for b in range(5)]
+    for (c, a, b), (wc, wa, wb) in zip(got, want):
+        assert c == wc and _close(a, wa) and _close(b, wb)
+
+
+def test_q11(device):
+    """Q11 as written: its symbol table holds _initval nodes and a second
+    Op that the output does not reach."""
+    t = datagen.tpch_parts(20000, seed=6)
+    rb = tpch.part_batches(t)
+    sup = DeviceTable.from_arrow(rb["supplier"]).build_index()
+    ps = DeviceTable.from_arrow(rb["partsupp"], dim=2)
+    for nation in (0, 5):
+        out = compile_op(tpch.q11(nation))(sup, ps)
+        k, v = ot.q11(t, nation)
+        assert len(k) > 100
+        np.testing.assert_array_equal(out.column(0), k)
+        np.testing.assert_array_equal(out.column(1).view(np.uint64), v.view(np.uint64))
