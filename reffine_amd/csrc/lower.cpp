@@ -573,6 +573,25 @@ struct Lowerer {
         // The first In(t, T) over an input table drives the scan (VecSpace,
         // iter_space.cpp:55-91); further In's are hash semi-joins
         // (InterSpace, iter_space.cpp:318-364).
+        // _in(t, A) | _in(t, B) is the reference's UnionSpace outer merge
+        // (iter_space.cpp:216-316), whose reads past the shorter vector have
+        // no defined result: refused by name.
+        for (auto& c0 : cj) {
+            N c = resolve_bool_sym(c0);
+            if (c->kind == K::Nary && c->op == MathOp::OR) {
+                bool has_in = false;
+                std::function<void(const N&)> f = [&](const N& x) {
+                    N y = resolve_bool_sym(x);
+                    if (y->kind == K::In) has_in = true;
+                    if (y->kind == K::Nary && y->op == MathOp::OR)
+                        for (auto& a : y->args) f(a);
+                };
+                f(c);
+                if (has_in)
+                    throw Unsupported("an Op over _in(t, A) | _in(t, B) (UnionSpace, the outer "
+                                      "merge join) is not supported");
+            }
+        }
         int drv = -1;
         for (auto& c0 : cj) {
             N c = resolve_bool_sym(c0);

@@ -7,8 +7,9 @@ import os
 
 import numpy as np
 
-from reffine_amd.reffiner import (_add, _func, _get, _gt, _gte, _i64, _i64_t, _idx, _in, _len,
-                                  _lt, _lte, _new, _op, _red, _subvec, _sym, _vec_t)
+from reffine_amd.reffiner import (_add, _eq, _func, _get, _gt, _gte, _i64, _i64_t, _idx, _in,
+                                  _len, _lt, _lte, _mod, _new, _op, _red, _sub, _subvec, _sym,
+                                  _vec_t)
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 STUDENT_COLS = ["id", "hours_studied", "attendance", "sleep_hours", "previous_scores",
@@ -153,3 +154,51 @@ def runend_table():
     x = pc.run_end_encode(pa.array(d["x"]))
     return pa.RecordBatch.from_arrays([x, pa.array(d["y"]), pa.array(d["d"])],
                                       names=["x", "y", "d"])
+
+
+# benchmark/include/micro.h (MicroBench): the table is (key i64, value i64),
+# loaded with dim 1 for left/right and dim 2 for `in`.
+MICRO_T = _vec_t(1, _i64_t, _i64_t)
+MICRO_T2 = _vec_t(2, _i64_t, _i64_t)
+
+
+def micro_select():
+    """micro.h:43-60: the even keys (an Op with no outputs)."""
+    vec_in = _sym("vec_in", MICRO_T)
+    t = _sym("t", _i64_t)
+    cond = _eq(_mod(t, _i64(2)), _i64(0))
+    cs = _sym("cond", cond)
+    op = _op([t], _in(t, vec_in) & cs, [])
+    os_ = _sym("op", op)
+    fn = _func("selectbench", os_, [vec_in])
+    fn.tbl[os_] = op
+    fn.tbl[cs] = cond
+    return fn
+
+
+def micro_join(outer: bool = False):
+    """micro.h:62-102: keys of both tables (inner) or of either (outer,
+    a UnionSpace) with the difference of their values."""
+    left = _sym("left", MICRO_T)
+    right = _sym("right", MICRO_T)
+    t = _sym("t", _i64_t)
+    diff = _sub(_get(left[t], 0), _get(right[t], 0))
+    ds = _sym("diff", diff)
+    sp = (_in(t, left) | _in(t, right)) if outer else (_in(t, left) & _in(t, right))
+    op = _op([t], sp, [ds])
+    os_ = _sym("op", op)
+    fn = _func("ojoinbench" if outer else "ijoinbench", os_, [left, right])
+    fn.tbl[os_] = op
+    fn.tbl[ds] = diff
+    return fn
+
+
+def micro_sum():
+    """micro.h:104-115: SUM of the value column over a dim-2 view."""
+    vec_in = _sym("vec_in", MICRO_T2)
+    red = _red(_subvec(vec_in, _idx(0), _len(vec_in, 1)), lambda: _i64(0),
+               lambda s, v: _add(_get(v, 0), s))
+    rs = _sym("red", red)
+    fn = _func("sumbench", rs, [vec_in])
+    fn.tbl[rs] = red
+    return fn

@@ -121,6 +121,7 @@ def test_union_space_is_rejected():
     with pytest.raises(runtime.ReffineError) as e:
         runtime.compile_check(fn)
     assert e.value.status == 2, e.value  # RF_ERR_UNSUPPORTED
+    assert "UnionSpace" in str(e.value)
 
 
 def test_output_struct_decoder_matches_c_layout():

@@ -124,3 +124,27 @@ def test_run_reduction_random(device):
         for val in v[starts[r]:ends[r]][m]:
             acc = acc + float(val)
         assert np.float64(sv[r]).view(np.int64) == np.float64(acc).view(np.int64)
+
+
+def test_micro_benchmark_programs(device):
+    """benchmark/include/micro.h: select (even keys), inner join (keys of
+    both tables, value difference), sum; the outer join (UnionSpace) is
+    refused with an unsupported-status error."""
+    from reffine_amd.runtime import ReffineError
+    rng = np.random.default_rng(3)
+    lk = np.unique(rng.integers(0, 400_000, 150_000)).astype(np.int64)
+    rk = np.unique(rng.integers(0, 400_000, 150_000)).astype(np.int64)
+    lv = rng.integers(-10**9, 10**9, len(lk)).astype(np.int64)
+    rv = rng.integers(-10**9, 10**9, len(rk)).astype(np.int64)
+    L = DeviceTable.from_columns([lk, lv], dim=1).build_index()
+    R = DeviceTable.from_columns([rk, rv], dim=1).build_index()
+    sel = compile_op(rq.micro_select())(L)
+    np.testing.assert_array_equal(sel.column(0), lk[lk % 2 == 0])
+    inner = compile_op(rq.micro_join())(L, R)
+    both, li, ri = np.intersect1d(lk, rk, return_indices=True)
+    np.testing.assert_array_equal(inner.column(0), both)
+    np.testing.assert_array_equal(inner.column(1), lv[li] - rv[ri])
+    L2 = DeviceTable.from_columns([lk, lv], dim=2)
+    assert compile_op(rq.micro_sum())(L2) == int(lv.sum())
+    with pytest.raises(ReffineError):
+        compile_op(rq.micro_join(outer=True))
