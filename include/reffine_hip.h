@@ -186,7 +186,8 @@ int rf_table_download_column(const rf_table* t, int64_t col, void* values_out,
                              size_t values_cap, uint8_t* validity_out,
                              size_t validity_cap);
 
-/* Builds the device index over int64 column 0 (key -> first valid row): a
+/* Builds the device index over int64 column 0 (key -> first valid row; for
+ * a run-end column 0, key -> run, as the reference indexes the run values): a
  * rank map (a bit per key of the range, counts per 64 keys, first rows) when
  * the column has no nulls and its keys span at most 32 values per row plus
  * 2^20, open addressing otherwise. Replaces ArrowTable2::build_index /

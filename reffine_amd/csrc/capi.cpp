@@ -2,6 +2,7 @@
 // catches exceptions and turns them into a status code plus the thread's
 // last-error text; nothing crosses the ABI as an exception or exit().
 #include <cstring>
+#include <functional>
 #include <string>
 
 #include "engine.hpp"
@@ -170,13 +171,17 @@ int rf_table_free(rf_table* t) {
     return guard([&] { rf::table_free(t); });
 }
 
-static void compile_impl(const char* ir_text, uint32_t flags, const int32_t* tuning,
-                         rf_query** out) {
-    need(ir_text && out, "NULL argument");
-    rf::device();
-    rf::N f = rf::parse_func(ir_text);
+static std::unique_ptr<rf_query> make_query(const rf::N& f0, uint32_t flags,
+                                            const int32_t* tuning) {
+    std::vector<rf::Prelude> pre;
+    rf::N f = rf::split_materialized(f0, pre);
     auto q = std::make_unique<rf_query>();
     q->plan = rf::lower_func(f, false);
+    q->n_user_args = (int64_t)f0->args.size() - 1;
+    for (auto& p : pre) {
+        q->pre.push_back(make_query(p.func, flags, nullptr));
+        q->pre_index.push_back(p.index);
+    }
     q->det = (flags & RF_COMPILE_DETERMINISTIC) != 0;
     q->tune = rf::default_tuning(q->plan, q->det);
     if (tuning) {
@@ -193,8 +198,7 @@ static void compile_impl(const char* ir_text, uint32_t flags, const int32_t* tun
     }
     if (q->plan.gj) {
         rf::setup_group_join(q.get(), tuning ? &q->tune : nullptr);
-        *out = q.release();
-        return;
+        return q;
     }
     // Compile the all-valid specialisation now so compile errors surface
     // here, as compile_op's do.
@@ -228,7 +232,14 @@ static void compile_impl(const char* ir_text, uint32_t flags, const int32_t* tun
         }
         q->run_specs.emplace(rsig, std::move(rsp));
     }
-    *out = q.release();
+    return q;
+}
+
+static void compile_impl(const char* ir_text, uint32_t flags, const int32_t* tuning,
+                         rf_query** out) {
+    need(ir_text && out, "NULL argument");
+    rf::device();
+    *out = make_query(rf::parse_func(ir_text), flags, tuning).release();
 }
 
 int rf_compile(const char* ir_text, uint32_t flags, rf_query** out) {
@@ -245,24 +256,40 @@ int rf_compile_tuned(const char* ir_text, uint32_t flags, const int32_t* tuning,
 int rf_compile_check(const char* ir_text, uint32_t flags, char* buf, size_t cap) {
     return guard([&] {
         need(ir_text != nullptr, "NULL argument");
-        rf::N f = rf::parse_func(ir_text);
-        rf::Plan plan = rf::lower_func(f, false);
-        rf::Tuning tune = rf::default_tuning(plan, (flags & RF_COMPILE_DETERMINISTIC) != 0);
-        std::vector<std::vector<bool>> nl(plan.inputs.size());
-        for (size_t i = 0; i < plan.inputs.size(); i++)
-            nl[i].assign(plan.inputs[i]->type.dtypes.size(), (flags & 2u) != 0);
+        std::string desc;
         std::vector<rf::KernelSource> kss;
-        if (plan.gj)
-            kss = rf::group_join_sources(plan, nl);
-        else if (plan.seg)
-            kss.push_back(rf::gen_seg_fold(*plan.seg, plan, nl));
-        else
-            kss.push_back(rf::gen_scan(plan, nl, tune, (flags & RF_COMPILE_DETERMINISTIC) != 0));
-        if (plan.runs) {
-            std::vector<std::vector<bool>> rnl = nl;
-            rnl[plan.runs->table].assign(plan.runs->input->type.dtypes.size(), (flags & 2u) != 0);
-            kss.insert(kss.begin(), rf::gen_seg_fold(plan.runs->seg, plan, rnl));
-        }
+        // Preludes (computed vectors read as tables) first, then the function.
+        std::function<void(const rf::N&)> add = [&](const rf::N& f0) {
+            std::vector<rf::Prelude> pre;
+            rf::N f = rf::split_materialized(f0, pre);
+            for (auto& p : pre) {
+                add(p.func);
+                desc += std::string("  materialized as an input") +
+                        (p.index ? " with an index" : "") + ": " + rf::to_text(p.func->args[0]) +
+                        "\n";
+            }
+            rf::Plan plan = rf::lower_func(f, false);
+            rf::Tuning tune = rf::default_tuning(plan, (flags & RF_COMPILE_DETERMINISTIC) != 0);
+            std::vector<std::vector<bool>> nl(plan.inputs.size());
+            for (size_t i = 0; i < plan.inputs.size(); i++)
+                nl[i].assign(plan.inputs[i]->type.dtypes.size(), (flags & 2u) != 0);
+            if (plan.runs) {
+                std::vector<std::vector<bool>> rnl = nl;
+                rnl[plan.runs->table].assign(plan.runs->input->type.dtypes.size(),
+                                             (flags & 2u) != 0);
+                kss.push_back(rf::gen_seg_fold(plan.runs->seg, plan, rnl));
+            }
+            if (plan.gj) {
+                for (auto& ks : rf::group_join_sources(plan, nl)) kss.push_back(ks);
+            } else if (plan.seg) {
+                kss.push_back(rf::gen_seg_fold(*plan.seg, plan, nl));
+            } else {
+                kss.push_back(
+                    rf::gen_scan(plan, nl, tune, (flags & RF_COMPILE_DETERMINISTIC) != 0));
+            }
+            desc += plan.describe();
+        };
+        add(rf::parse_func(ir_text));
         size_t bytes = 0;
         std::string text;
         for (auto& ks : kss) {
@@ -270,7 +297,7 @@ int rf_compile_check(const char* ir_text, uint32_t flags, char* buf, size_t cap)
             text += "\n--- kernel ---\n" + ks.code;
         }
         if (buf && cap) {
-            std::string s = plan.describe() + "code object bytes: " + std::to_string(bytes) + text;
+            std::string s = desc + "code object bytes: " + std::to_string(bytes) + text;
             std::strncpy(buf, s.c_str(), cap - 1);
             buf[cap - 1] = 0;
         }

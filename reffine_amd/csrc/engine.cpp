@@ -336,10 +336,12 @@ static uint64_t next_pow2(uint64_t x) {
 
 void table_build_index(rf_table* t) {
     if (t->cols.empty()) throw ArrowError("empty table");
+    // Over the values of column 0: its rows, or for a run-end column its run
+    // values (key -> run), as ArrowTable2::build_index indexes
+    // (include/reffine/arrow/table.h:107-138).
     const Column& c = t->cols[0];
-    if (c.type != types::I64 || c.enc != Enc::FLAT)
-        throw ArrowError("index needs a flat int64 column 0 "
-                         "(include/reffine/arrow/table.h:107-138)");
+    if (c.type != types::I64)
+        throw ArrowError("index needs an int64 column 0 (include/reffine/arrow/table.h:107-138)");
     Device& d = device();
     if (t->has_index) return;
     // Keys dense in their range (all valid, at most 32 key slots per row
@@ -1599,8 +1601,35 @@ static rf_table* run_stage_table(rf_query* q, const rf_arg* args, int64_t n_args
     return out.release();
 }
 
+// Runs a query's preludes on the caller's arguments: `full` gets those
+// arguments and then each prelude's result table (owned by `mats`).
+static void run_preludes(rf_query* q, const rf_arg* args, int64_t n_args,
+                         std::vector<std::unique_ptr<rf_table>>& mats, std::vector<rf_arg>& full) {
+    if (n_args != q->n_user_args)
+        throw IRError("function '" + q->plan.func_name + "' takes " +
+                      std::to_string(q->n_user_args) + " arguments, got " + std::to_string(n_args));
+    full.assign(args, args + n_args);
+    for (size_t i = 0; i < q->pre.size(); i++) {
+        rf_table* t = nullptr;
+        run_query(q->pre[i].get(), args, n_args, nullptr, &t);
+        mats.emplace_back(t);
+        if (q->pre_index[i]) table_build_index(t);
+        rf_arg a{};
+        a.kind = 0;
+        a.u.table = t;
+        full.push_back(a);
+    }
+}
+
 void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_addr,
                rf_table** out_table) {
+    std::vector<std::unique_ptr<rf_table>> mats;
+    std::vector<rf_arg> full;
+    if (!q->pre.empty()) {
+        run_preludes(q, args, n_args, mats, full);
+        args = full.data();
+        n_args = (int64_t)full.size();
+    }
     std::lock_guard<std::mutex> lk(q->mu);
     // A run stage first folds T's runs; the plan then scans that table in
     // T's place (same stream, so no synchronisation in between).
@@ -1716,9 +1745,32 @@ rf_table* table_view(const std::vector<Type>& types, const std::vector<void*>& v
 
 void time_query(rf_query* q, const rf_arg* args, int64_t n_args, int iters, double* ms_all,
                 double* ms_scan) {
-    std::lock_guard<std::mutex> lk(q->mu);
     Device& d = device();
     *ms_all = *ms_scan = 0.0;
+    if (!q->pre.empty()) {
+        // Whole runs, preludes included (each run synchronises).
+        if (iters <= 0) return;
+        hipEvent_t e[2];
+        for (auto& x : e) RF_HIP(hipEventCreate(&x));
+        std::vector<char> out(4096);
+        rf_table* t = nullptr;
+        run_query(q, args, n_args, out.data(), &t);
+        if (t) table_free(t);
+        RF_HIP(hipEventRecord(e[0], d.stream));
+        for (int i = 0; i < iters; i++) {
+            t = nullptr;
+            run_query(q, args, n_args, out.data(), &t);
+            if (t) table_free(t);
+        }
+        RF_HIP(hipEventRecord(e[1], d.stream));
+        RF_HIP(hipEventSynchronize(e[1]));
+        float a = 0;
+        RF_HIP(hipEventElapsedTime(&a, e[0], e[1]));
+        for (auto& x : e) (void)hipEventDestroy(x);
+        *ms_all = *ms_scan = a / iters;
+        return;
+    }
+    std::lock_guard<std::mutex> lk(q->mu);
     if (q->gj) {
         // Whole group joins (host synchronisations included) and the probe
         // scan kernel of each.

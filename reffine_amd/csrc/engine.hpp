@@ -169,6 +169,11 @@ struct rf_query {
     rf::Tuning tune;
     std::map<std::string, rf::Spec> specs;  // by nullability signature
     std::map<std::string, rf::Spec> run_specs;  // a run stage's fold, likewise
+    // Computed vectors read as tables: run first (on the caller's arguments),
+    // their results appended to the arguments (indexed where pre_index).
+    std::vector<std::unique_ptr<rf_query>> pre;
+    std::vector<bool> pre_index;
+    int64_t n_user_args = -1;
     // Workspace (grow-only, reused across runs).
     void* partials = nullptr;
     size_t partials_bytes = 0;

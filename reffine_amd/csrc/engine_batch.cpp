@@ -154,6 +154,7 @@ static void plan_unit(rf_batch::Unit& u, rf_query* q, const rf_arg* args, int64_
     u.q = q;
     u.args.assign(args, args + n_args);
     if (q->plan.seg || q->plan.runs) throw Unsupported("run reductions do not run in a batch");
+    if (!q->pre.empty()) throw Unsupported("computed vectors read as tables do not run in a batch");
     // The query's spec cache (and a group join's build cache) may be used by
     // a plain run on another thread.
     std::lock_guard<std::mutex> lk(q->mu);

@@ -264,6 +264,13 @@ N mk_initval(std::vector<N> inits, N val) {
     return n;
 }
 
+N mk_buildidx(N vec) {
+    check(vec->type.is_vector(), "BuildIndex of non-vector");
+    auto n = std::make_shared<Node>(K::BuildIdx, vec->type);
+    n->args = {vec};
+    return n;
+}
+
 N mk_op(std::vector<N> iters, N pred, std::vector<N> outputs) {
     std::vector<Type> d;
     std::vector<Enc> e;
@@ -466,6 +473,7 @@ struct Builder {
         if (h == "subvec") return mk_subvec(arg(1), arg(2), arg(3));
         if (h == "len") return mk_len(arg(1), integer(e.items.at(2)));
         if (h == "locate") return mk_locate(arg(1), arg(2));
+        if (h == "buildidx") return mk_buildidx(arg(1));
         if (h == "initval") {
             check(e.items.size() == 3, "(initval (SYMS) VAL)");
             std::vector<N> inits;
@@ -565,6 +573,8 @@ std::string to_text(const N& n) {
             return "(len " + to_text(n->args[0]) + " " + std::to_string(n->idx) + ")";
         case K::Locate:
             return "(locate " + to_text(n->args[0]) + " " + to_text(n->args[1]) + ")";
+        case K::BuildIdx:
+            return "(buildidx " + to_text(n->args[0]) + ")";
         case K::InitVal: {
             std::string s = "(initval (";
             for (size_t i = 0; i < n->iters.size(); i++)

@@ -88,7 +88,7 @@ enum class MathOp {
 
 enum class K {
     Sym, Const, Cast, Select, Get, New, Nary, Element, In, SubVector,
-    Length, Op, Reduce, Func, Locate, ReadData, InitVal
+    Length, Op, Reduce, Func, Locate, ReadData, InitVal, BuildIdx
 };
 
 struct Node;
@@ -130,6 +130,8 @@ N mk_readdata(N vec, N idx, int64_t col);
 // include/reffine/ir/expr.h:16-26: `val`, with the symbols `inits` computed
 // first (iters holds them).
 N mk_initval(std::vector<N> inits, N val);
+// include/reffine/ir/op_to_loop.h:184-190: vec with an index built over it.
+N mk_buildidx(N vec);
 N mk_op(std::vector<N> iters, N pred, std::vector<N> outputs);
 N mk_reduce(N vec, N init, N state_sym, N val_sym, N acc);
 N mk_func(const std::string& name, N output, std::vector<N> inputs,

@@ -220,6 +220,8 @@ struct Lowerer {
             case K::InitVal:
                 // The initialised symbols are lowered where they are used.
                 return rw(e->args[0]);
+            case K::BuildIdx:
+                throw Unsupported("an index over a computed vector outside the symbol table");
             case K::Locate: {
                 // vector_locate (include/reffine/vinstr/vinstr.h:137-146): a
                 // probe of the table's index; -1 when the key is absent.
@@ -1199,8 +1201,10 @@ struct Lowerer {
             if (e->kind == K::Sym) {
                 auto t = tbl.find(e->name);
                 if (t == tbl.end()) return;
-                if (t->second->kind == K::Reduce || t->second->kind == K::Op) {
-                    if (seen.insert(e->name).second) comps.push_back({e->name, t->second});
+                N d = t->second;
+                while (d->kind == K::InitVal) d = d->args[0];  // its inits are lowered where used
+                if (d->kind == K::Reduce || d->kind == K::Op) {
+                    if (seen.insert(e->name).second) comps.push_back({e->name, d});
                     return;
                 }
                 walk(t->second);
@@ -1244,6 +1248,89 @@ struct Lowerer {
         return plan;
     }
 };
+
+static N strip_initval(N d) {
+    while (d && d->kind == K::InitVal) d = d->args[0];
+    return d;
+}
+
+N split_materialized(const N& func, std::vector<Prelude>& pre) {
+    if (!func || func->kind != K::Func) throw IRError("expected a Func");
+    std::map<std::string, N> tbl, tsym;
+    for (auto& kv : func->tbl) {
+        tbl[kv.first->name] = kv.second;
+        tsym[kv.first->name] = kv.first;
+    }
+    std::set<std::string> inputs;
+    for (size_t i = 1; i < func->args.size(); i++) inputs.insert(func->args[i]->name);
+    std::vector<std::string> cands;
+    std::set<std::string> cand_set;
+    std::set<const Node*> seen;
+    // A vector operand naming a computed vector (Op or BuildIndex).
+    auto computed = [&](const N& v) -> bool {
+        if (v->kind != K::Sym || inputs.count(v->name) || !tbl.count(v->name)) return false;
+        N d = strip_initval(tbl[v->name]);
+        if (d->kind != K::Op && d->kind != K::BuildIdx) return false;
+        if (cand_set.insert(v->name).second) cands.push_back(v->name);
+        return true;
+    };
+    std::function<void(const N&)> visit = [&](const N& e) {
+        if (!e || !seen.insert(e.get()).second) return;
+        switch (e->kind) {
+            case K::Element:
+            case K::SubVector:
+            case K::ReadData:
+            case K::Locate:
+            case K::Length:
+                if (computed(e->args[0])) {
+                    for (size_t i = 1; i < e->args.size(); i++) visit(e->args[i]);
+                    return;
+                }
+                break;
+            case K::In:
+                if (computed(e->args[1])) {
+                    visit(e->args[0]);
+                    return;
+                }
+                break;
+            case K::Sym: {
+                if (inputs.count(e->name)) return;
+                auto t = tbl.find(e->name);
+                if (t != tbl.end()) visit(t->second);
+                return;
+            }
+            default:
+                break;
+        }
+        for (auto& a : e->args) visit(a);
+    };
+    visit(func->args[0]);
+    if (cands.empty()) return func;
+    std::vector<N> in(func->args.begin() + 1, func->args.end());
+    std::vector<std::pair<N, N>> main_tbl;
+    for (auto& kv : func->tbl)
+        if (!cand_set.count(kv.first->name)) main_tbl.push_back(kv);
+    for (auto& c : cands) {
+        N d = strip_initval(tbl[c]);
+        Prelude p;
+        N target = tsym[c];
+        std::vector<std::pair<N, N>> sub_tbl = func->tbl;
+        if (d->kind == K::BuildIdx) {
+            p.index = true;
+            N v = d->args[0];
+            if (v->kind == K::Sym && tbl.count(v->name)) {
+                target = v;
+            } else {
+                target = mk_sym(c + ".vec", v->type);
+                sub_tbl.push_back({target, v});
+            }
+        }
+        p.func = mk_func(func->name + "." + c, target, in, sub_tbl);
+        pre.push_back(p);
+        in.push_back(mk_sym(c, tsym[c]->type));
+    }
+    return mk_func(func->name, func->args[0], in, main_tbl);
+}
 
 Plan lower_func(const N& func, bool vectorize) {
     if (!func || func->kind != K::Func) throw IRError("expected a Func");

@@ -135,6 +135,20 @@ struct GroupJoinPlan {
     std::string sym_name;
 };
 
+// A computed vector the function reads as a table (an Op, or an Op with an
+// index built over it, named in the symbol table and used through Element,
+// In, SubVector, ReadData, Locate or Length): run first as a function of its
+// own (same inputs), its result passed to the function as one more input.
+struct Prelude {
+    N func;            // the sub-function (output: the Op)
+    bool index = false; // build_index on its result (BuildIndex)
+};
+
+// Splits the computed vectors out of `func`: returns the function with one
+// more input per prelude (appended in order) and fills `pre`; returns
+// `func` itself when it reads none.
+N split_materialized(const N& func, std::vector<Prelude>& pre);
+
 // Lowers `func` (a parsed IR Func) into a plan; throws Unsupported for
 // shapes the kernels do not cover and IRError for ill-formed IR.
 Plan lower_func(const N& func, bool vectorize);

@@ -253,3 +253,24 @@ def store_sales(n: int, seed: int = 61) -> dict:
             out[name] = randint(seed, c + 1, n, 0, 2_000_000, np.int64).astype(np.float64) / 100.0
     out["ss_sold_date_sk"] = np.sort(out["ss_sold_date_sk"])
     return out
+
+
+def graph_tables(n_nodes: int, n_edges: int, seed: int = 67) -> dict:
+    """A directed multigraph as reffine's PageRank reads it (benchmark/
+    include/misc.h:212-291): edges (src, dst, count) sorted by (src, dst)
+    and rev_edges (dst, src, count) sorted by (dst, src), each pair once
+    with its multiplicity; pr (id, rank float32 = 1/n) for every node.
+    About 1 node in 8 has no outgoing edge."""
+    src = randint(seed, 1, n_edges, 0, n_nodes, np.int64)
+    dst = randint(seed, 2, n_edges, 0, n_nodes, np.int64)
+    keep = (src % 8) != 3  # nodes with no outgoing edges
+    src, dst = src[keep], dst[keep]
+    pair = np.unique(src * n_nodes + dst, return_counts=True)
+    s, d, c = pair[0] // n_nodes, pair[0] % n_nodes, pair[1].astype(np.int64)
+    r = np.lexsort((s, d))
+    return {
+        "edges": {"src": s, "dst": d, "count": c},
+        "rev_edges": {"dst": d[r], "src": s[r], "count": c[r]},
+        "pr": {"id": np.arange(n_nodes, dtype=np.int64),
+               "rank": np.full(n_nodes, 1.0 / n_nodes, dtype=np.float32)},
+    }

@@ -367,6 +367,19 @@ class InitVal(Expr):
         return "(initval (" + " ".join(i.ir() for i in self.inits) + ") " + self.val.ir() + ")"
 
 
+class BuildIdx(Expr):
+    """include/reffine/ir/op_to_loop.h:184-190: `vec` with an index over its
+    iterator column (build_vector_index)."""
+
+    def __init__(self, vec: Expr):
+        _check(vec.type.is_vector(), "BuildIndex of non-vector")
+        super().__init__(vec.type)
+        self.vec = vec
+
+    def ir(self):
+        return f"(buildidx {self.vec.ir()})"
+
+
 class Op(Expr):
     """include/reffine/ir/op.h:13-53."""
 
@@ -461,6 +474,7 @@ def _subvec(vec, start, end) -> SubVector: return SubVector(vec, start, end)
 def _len(vec, col: int) -> Length: return Length(vec, col)
 def _locate(vec, it) -> Locate: return Locate(vec, it)
 def _initval(inits, val) -> InitVal: return InitVal(inits, val)
+def _buildidx(vec) -> BuildIdx: return BuildIdx(vec)
 def _readdata(vec, idx, col: int) -> ReadData: return ReadData(vec, idx, col)
 def _op(iters, pred, outputs) -> Op: return Op(iters, pred, outputs)
 def _red(vec, init, acc) -> Reduce: return Reduce(vec, init, acc)

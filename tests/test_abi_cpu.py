@@ -206,3 +206,14 @@ def test_run_table_used_beyond_its_runs_is_rejected():
     fn.tbl[rs] = red
     with pytest.raises(runtime.ReffineError):
         runtime.compile_check(fn)
+
+
+def test_pagerank_materializes_the_indexed_op():
+    """The PageRank step (misc.h:212-291): the _buildidx'd Op is split out as
+    a function of its own (a run stage over edges) whose result is an
+    indexed input of the rest (a run fold over rev_edges probing it)."""
+    from reffine_amd import graph
+    text = runtime.compile_check(graph.pagerank(1000))
+    assert "materialized as an input with an index" in text
+    assert text.count("--- kernel ---") == 3
+    assert "probe 0: table input 3 at key (sym $col.1.1 i64)" in text

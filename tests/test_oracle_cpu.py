@@ -319,3 +319,26 @@ def test_tpch_fold_oracles_against_loops():
     k, v = ot.q11(tp, 4)
     np.testing.assert_array_equal(k, tot.index.to_numpy())
     np.testing.assert_allclose(v, tot.to_numpy(), rtol=1e-12)
+
+
+def test_pagerank_oracle_against_sparse_matrix():
+    """oracle.graph.pagerank against the matrix form of one PageRank step:
+    new = alpha * A^T (pr / outdeg) + (1 - alpha) / n over the nodes with
+    in-edges (float64, within float32 rounding), and a planted mistake (the
+    edge multiplicity dropped) that the check catches."""
+    import scipy.sparse as sp
+    from oracle import graph as og
+    n = 800
+    t = datagen.graph_tables(n, 6000, seed=9)
+    e = t["edges"]
+    A = sp.coo_matrix((e["count"].astype(np.float64), (e["src"], e["dst"])), shape=(n, n)).tocsr()
+    deg = np.asarray(A.sum(axis=1)).ravel()
+    pr = t["pr"]["rank"].astype(np.float64)
+    contrib = np.divide(pr, deg, out=np.zeros(n), where=deg > 0)
+    want = 0.85 * (A.T @ contrib) + 0.15 / n
+    k, v = og.pagerank(t, n)
+    np.testing.assert_allclose(v, want[k], rtol=1e-5)
+    assert set(k.tolist()) == set(np.flatnonzero(np.asarray(A.sum(axis=0)).ravel() > 0).tolist())
+    bad = {**t, "rev_edges": {**t["rev_edges"], "count": np.ones_like(t["rev_edges"]["count"])}}
+    if (t["rev_edges"]["count"] > 1).any():
+        assert not np.allclose(og.pagerank(bad, n)[1], want[k], rtol=1e-5)
