@@ -44,3 +44,34 @@ def pagerank(t: dict, n: int, alpha: float = 0.85):
         keys.append(k)
         vals.append(np.float32(np.float32(a * s) + c0))
     return np.array(keys, dtype=np.int64), np.array(vals, dtype=np.float32)
+
+
+def nbody(b: dict, n: int, g: float = 1.0, dt: float = 0.01):
+    """Restates reffine's Nbody step (benchmark/include/misc.h:77-202):
+    the pair table in the nested loops' order (a, then b; a == b skipped,
+    its rows compacted), each force one IEEE operation at a time as the C
+    text writes it; per body idx the sequential sum of pair rows
+    [idx*n, (idx+1)*n) that exist; then the updated velocities and
+    positions. Returns the 8 output columns (id, M, X, Y, Z, VX, VY, VZ)."""
+    m, x, y, z = b["m"], b["x"], b["y"], b["z"]
+    a_i, b_i = np.divmod(np.arange(n * n, dtype=np.int64), n)
+    keep = a_i != b_i
+    a_i, b_i = a_i[keep], b_i[keep]
+    dx, dy, dz = x[a_i] - x[b_i], y[a_i] - y[b_i], z[a_i] - z[b_i]
+    dist2 = (dx * dx + dy * dy) + dz * dz
+    dist = np.sqrt(dist2)
+    f = (g * (m[a_i] * m[b_i])) / dist2
+    pf = [(f * dx) / dist, (f * dy) / dist, (f * dz) / dist]
+    rows = len(a_i)
+    force = np.zeros((3, n))
+    for idx in range(n):
+        lo, hi = idx * n, min((idx + 1) * n, rows)
+        for c in range(3):
+            s = 0.0
+            for r in range(lo, hi):
+                s = s + float(pf[c][r])
+            force[c, idx] = s
+    out_v = [b["vx"] + (force[0] / m) * dt, b["vy"] + (force[1] / m) * dt,
+             b["vz"] + (force[2] / m) * dt]
+    out_p = [x + out_v[0] * dt, y + out_v[1] * dt, z + out_v[2] * dt]
+    return [np.arange(n, dtype=np.int64), m.copy(), *out_p, *out_v]

@@ -801,9 +801,15 @@ static std::string null_signature(const std::vector<N>& inputs, const rf_arg* ar
         if ((int64_t)i >= n_args || args[i].kind != 0 || !args[i].u.table)
             throw IRError("argument " + std::to_string(i) + " must be a table");
         const rf_table* t = args[i].u.table;
+        // idx columns are stored as int64 (a computed table's iterators).
+        std::function<Type(Type)> norm = [&](Type x) {
+            if (x.bt == BT::IDX) x.bt = BT::INT64;
+            for (auto& d : x.dtypes) d = norm(d);
+            return x;
+        };
         const Type& want = inputs[i]->type;
         Type have = t->type();
-        if (have != want)
+        if (norm(have) != norm(want))
             throw IRError("argument " + std::to_string(i) + " has type " + have.str() +
                           ", the function expects " + want.str());
         for (auto& c : t->cols) s += c.validity ? '1' : '0';
@@ -1539,14 +1545,22 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
 static void launch_seg(rf_query* q, const SegReducePlan& sp, Spec& spec, const rf_arg* args,
                        int64_t n_args, rf_table* out, hipStream_t s) {
     const Column& c0 = args[sp.table].u.table->cols.at(0);
-    const int64_t runs = c0.length;
-    if (runs == 0) return;
     RfSegArgs A;
     std::memset(&A, 0, sizeof A);
-    A.runs = runs;
-    A.rows = c0.logical_length;
-    A.run_ends = (const int32_t*)c0.run_ends;
-    A.keys = c0.values;
+    if (sp.ranged) {
+        // One fold per key in [lb, ub): out holds that many rows.
+        EvalCtx c{&q->plan, args, n_args, {}, {}};
+        A.runs = out->rows();
+        A.key0 = eval_idx(sp.lb, c);
+        A.rows = args[sp.table].u.table->rows();
+    } else {
+        A.runs = c0.length;
+        A.rows = c0.logical_length;
+        A.run_ends = (const int32_t*)c0.run_ends;
+        A.keys = c0.values;
+    }
+    const int64_t runs = A.runs;
+    if (runs == 0) return;
     for (size_t k = 0; k < spec.ks.cols.size(); k++) {
         const ColSlot& cs = spec.ks.cols[k];
         const Column& col = args[cs.table].u.table->cols.at(cs.col);
@@ -1567,8 +1581,15 @@ static void launch_seg(rf_query* q, const SegReducePlan& sp, Spec& spec, const r
     RF_HIP(hipModuleLaunchKernel(spec.mod->fn, g, 1, 1, 256, 1, 1, 0, s, params, nullptr));
 }
 
-// A new table of one row per run of T (no launch).
-static rf_table* alloc_seg_table(const SegReducePlan& sp, const rf_arg* args) {
+// A new table of one row per run of T, or per key of a ranged fold (no
+// launch).
+static rf_table* alloc_seg_table(const SegReducePlan& sp, const rf_arg* args,
+                                 rf_query* q = nullptr, int64_t n_args = 0) {
+    if (sp.ranged) {
+        EvalCtx c{&q->plan, args, n_args, {}, {}};
+        const int64_t lb = eval_idx(sp.lb, c), ub = eval_idx(sp.ub, c);
+        return alloc_table(sp.out_types, ub > lb ? ub - lb : 0, 1);
+    }
     const Column& c0 = args[sp.table].u.table->cols.at(0);
     if (c0.enc != Enc::RUNEND) throw ArrowError("column 0 is not run-end encoded");
     if (c0.validity) throw Unsupported("null run values");
@@ -1580,7 +1601,7 @@ static rf_table* run_seg_reduce(rf_query* q, const rf_arg* args, int64_t n_args)
     Device& d = device();
     check_args(q->plan, args, n_args);
     Spec& spec = get_spec(q, args, n_args);
-    std::unique_ptr<rf_table> out(alloc_seg_table(*q->plan.seg, args));
+    std::unique_ptr<rf_table> out(alloc_seg_table(*q->plan.seg, args, q, n_args));
     launch_seg(q, *q->plan.seg, spec, args, n_args, out.get(), d.stream);
     RF_HIP(hipStreamSynchronize(d.stream));
     return out.release();

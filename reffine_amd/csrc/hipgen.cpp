@@ -229,6 +229,21 @@ struct Emitter {
             case K::Nary:
                 r = tmp(nary(e));
                 break;
+            case K::ReadData: {
+                // Column c of table t at a computed row; 0 outside the table.
+                const std::string tn = leaf_name(e->args[0]);
+                if (tn.rfind("$tab.", 0) != 0) throw IRError("ReadData of " + tn);
+                const int t = std::stoi(tn.substr(5)), c = (int)e->idx;
+                const int s = g.col_slot(t, c, false);
+                const std::string T = cty(g.ks.cols[s].type);
+                const std::string i = "(int64_t)" + expr(e->args[1]);
+                ParamSlot p{ParamSlot::LEN, types::IDX_T, 0, t, c};
+                const std::string len =
+                    g.param(p, "$len." + std::to_string(t) + "." + std::to_string(c));
+                r = tmp("(" + i + " >= 0 && " + i + " < " + len + ") ? rf_g((const " + T +
+                        "*)L.col[" + std::to_string(s) + "])[" + i + "] : (" + T + ")0");
+                break;
+            }
             default:
                 throw Unsupported("node in row code: " + key);
         }
@@ -812,7 +827,16 @@ KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
     for (int sl : g.lds_slots) row_bytes += ks.cols[sl].type.width();
     int cap = 0;
     if (row_bytes > 0) cap = std::min(1536, (40960 / row_bytes) / 256 * 256);
-    const bool staged = cap >= 256;
+    const bool staged = cap >= 256 && !sp.ranged;
+    std::string lo_e, hi_e;
+    std::ostringstream bnd;
+    if (sp.ranged) {
+        // A ranged fold's rows [lo, hi): expressions of its key.
+        Emitter er(g, Mode::FINAL, bnd, "            ");
+        er.ntmp = nt + 1000;
+        lo_e = er.expr(sp.lo);
+        hi_e = er.expr(sp.hi);
+    }
 
     std::ostringstream o;
     o << "// Generated run reduction for '" << sp.sym_name << "'.\n";
@@ -828,8 +852,11 @@ KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
     // The run's key, its probes and the fold's init come first, so that
     // their loads overlap the copy below.
     o << "        const int64_t rr = run < L.runs ? run : L.runs - 1;\n";
-    o << "        const " << cty(sp.key_type) << " key = ((const " << cty(sp.key_type)
-      << "*)L.keys)[rr];\n        (void)key;\n";
+    if (sp.ranged)
+        o << "        const int64_t key = L.key0 + rr;\n        (void)key;\n";
+    else
+        o << "        const " << cty(sp.key_type) << " key = ((const " << cty(sp.key_type)
+          << "*)L.keys)[rr];\n        (void)key;\n";
     o << init.str();
     if (staged) {
         o << "        const int64_t last = (base + 256 < L.runs ? base + 256 : L.runs) - 1;\n";
@@ -852,9 +879,17 @@ KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
         o << "        __syncthreads();\n";
     }
     o << "        if (run < L.runs) {\n";
-    o << "            int64_t lo = run > 0 ? (int64_t)L.run_ends[run - 1] : 0;\n";
-    o << "            const int64_t hi0 = (int64_t)L.run_ends[run];\n";
-    o << "            int64_t hi = hi0 < L.rows ? hi0 : L.rows;\n";
+    if (sp.ranged) {
+        o << bnd.str();
+        o << "            int64_t lo = (int64_t)" << lo_e << ";\n";
+        o << "            int64_t hi = (int64_t)" << hi_e << ";\n";
+        o << "            lo = lo < 0 ? 0 : lo;\n";
+        o << "            hi = hi < L.rows ? hi : L.rows;\n";
+    } else {
+        o << "            int64_t lo = run > 0 ? (int64_t)L.run_ends[run - 1] : 0;\n";
+        o << "            const int64_t hi0 = (int64_t)L.run_ends[run];\n";
+        o << "            int64_t hi = hi0 < L.rows ? hi0 : L.rows;\n";
+    }
     for (size_t f = 0; f < sp.fields.size(); f++) {
         const std::string T = cty(sp.fields[f].type);
         o << "            " << T << " s" << f << " = (" << T << ")" << iv[f] << ";\n";

@@ -151,6 +151,7 @@ struct RfBatchArgs {
 struct RfSegArgs {
     int64_t runs;
     int64_t rows;
+    int64_t key0;                 // ranged folds: the key of fold 0
     const int32_t* run_ends;
     const void* keys;             // run values
     const void* col[RF_MAXCOLS];  // columns the row code and outputs read

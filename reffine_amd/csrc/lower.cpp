@@ -253,7 +253,10 @@ struct Lowerer {
                 }
                 if (ln == "$row" && t == driver && plan.source == Plan::TABLE_ROWS)
                     return col_leaf(t, c);
-                throw Unsupported("ReadData at a computed row index");
+                // Any other row index: an indexed read of the column (rows
+                // outside the table read 0; the reference reads them
+                // unchecked).
+                return mk_readdata(leaf("$tab." + std::to_string(t), vt), ix, c);
             }
             case K::Reduce: {
                 auto r = reduce_subst.find(to_text(e));
@@ -1077,6 +1080,92 @@ struct Lowerer {
         plan.end = leaf("$len." + std::to_string(T) + ".0", types::IDX_T);
     }
 
+    // A ranged fold: an Op over one range iterator (lb <= i < ub and nothing
+    // else) whose outputs reduce _subvec(T, lo(i), hi(i)) of an input table
+    // T, every reduction over the same rows -- misc.h:104-139's per-body
+    // force sums over the pair table. One fold per i of rows [lo, hi), in
+    // row order (the SubVector loop, loopgen.cpp:221-242).
+    bool lower_range_fold(const N& op, const std::string& name) {
+        if (op->iters.size() != 1) return false;
+        std::vector<N> reds;
+        std::set<std::string> seen;
+        for (size_t i = 1; i < op->args.size(); i++) find_reduces(op->args[i], reds, seen);
+        if (reds.empty()) return false;
+        int T = -1;
+        N lo, hi;
+        for (auto& r : reds) {
+            N v = resolve_vec(r->args[0]);
+            if (v->kind != K::SubVector) return false;
+            const int t = input_index(resolve_vec(v->args[0]));
+            if (t < 0) return false;
+            if (T >= 0 && (t != T || to_text(v->args[1]) != to_text(lo) ||
+                           to_text(v->args[2]) != to_text(hi)))
+                throw Unsupported("reductions over different ranges in one ranged fold");
+            T = t;
+            lo = v->args[1];
+            hi = v->args[2];
+        }
+        for (auto& c0 : [&] { std::vector<N> cj; conjuncts(resolve_bool_sym(op->args[0]), cj); return cj; }())
+            if (resolve_bool_sym(c0)->kind == K::In) return false;
+        const N it = op->iters[0];
+        const Type& vt = plan.inputs[T]->type;
+        if (vt.dtypes.size() < 2) throw Unsupported("SubVector of a one-column vector");
+        auto sp = std::make_shared<SegReducePlan>();
+        sp->ranged = true;
+        sp->table = T;
+        sp->key_type = types::I64;
+        Lowerer Lr = sibling();
+        Lr.set_driver(T);
+        Lr.subst[it->name] = mk_cast(it->type, leaf("$key", types::I64));
+        std::vector<size_t> offs;
+        for (auto& r : reds) {
+            offs.push_back(sp->fields.size());
+            Lr.lower_run_fields(r, T, sp->fields);
+        }
+        Lr.memo.clear();
+        sp->lo = Lr.rw(lo);
+        sp->hi = Lr.rw(hi);
+        // Bounds and outputs: the Op bound as a range scan, its reductions
+        // read as the fold's fields, its iterator ($row) as the fold's key.
+        Lowerer Lb = sibling();
+        for (size_t r = 0; r < reds.size(); r++) {
+            size_t k = offs[r];
+            Lb.reduce_subst[to_text(reds[r])] = nest_fields(
+                reds[r]->type,
+                [&](size_t f) { return leaf("$acc." + std::to_string(f), sp->fields[f].type); },
+                k);
+        }
+        Source src = Lb.bind_op(op);
+        if (Lb.plan.source != Plan::RANGE || !Lb.plan.probes.empty())
+            throw Unsupported("ranged fold over a table scan or with probes");
+        if (!(src.pred->kind == K::Const && src.pred->val != 0))
+            throw Unsupported("ranged fold with a predicate beyond the iterator's bounds");
+        sp->lb = Lb.plan.start;
+        sp->ub = Lb.plan.end;
+        std::function<N(const N&)> key_of = [&](const N& e) -> N {
+            if (is_leaf(e, "$row")) return mk_cast(e->type, leaf("$key", types::I64));
+            if (e->args.empty()) return e;
+            auto c = std::make_shared<Node>(*e);
+            for (auto& a : c->args) a = key_of(a);
+            return c;
+        };
+        for (auto& v : src.row_value->args) {
+            if (!v->type.is_primitive()) throw Unsupported("non-primitive ranged fold output");
+            sp->out_values.push_back(key_of(v));
+            sp->out_types.push_back(v->type.is_idx() ? types::I64 : v->type);
+        }
+        sp->sub = std::move(Lr.plan);
+        sp->state_type = reds[0]->type;
+        sp->vec_type = op->type;
+        sp->sym_name = name;
+        plan.seg = sp;
+        plan.driver = T;
+        plan.source = Plan::TABLE_ROWS;
+        plan.start = mk_const(types::IDX_T, 0);
+        plan.end = leaf("$len." + std::to_string(T) + ".1", types::IDX_T);
+        return true;
+    }
+
     // A run stage: reductions over T[t] (T's column 0 run-end encoded)
     // anywhere in the function, t bound by an Op whose space has In(t, T) --
     // the shape of TPC-H Q3/Q4/Q12/Q18 over lineitem (benchmark/include/
@@ -1224,6 +1313,12 @@ struct Lowerer {
                 plan.output = out;
                 return plan;
             }
+        }
+        if (comps.size() == 1 && comps[0].second->kind == K::Op &&
+            out->kind == K::Sym && out->name == comps[0].first &&
+            lower_range_fold(comps[0].second, comps[0].first)) {
+            plan.output = out;
+            return plan;
         }
         const bool staged = lower_run_stage(out);
         if (!staged && comps.size() == 1 && comps[0].second->kind == K::Op &&
@@ -1374,7 +1469,12 @@ std::string Plan::describe() const {
         }
     }
     auto seg_text = [&](const SegReducePlan& sp) {
-        o << "  run reduction over the run-end column 0 of table input " << sp.table << "\n";
+        if (sp.ranged)
+            o << "  ranged fold: for $key in [" << to_text(sp.lb) << ", " << to_text(sp.ub)
+              << "), rows [" << to_text(sp.lo) << ", " << to_text(sp.hi) << ") of table input "
+              << sp.table << "\n";
+        else
+            o << "  run reduction over the run-end column 0 of table input " << sp.table << "\n";
         for (size_t j = 0; j < sp.sub.probes.size(); j++)
             o << "    probe " << j << ": table input " << sp.sub.probes[j].table << " at key "
               << to_text(sp.sub.probes[j].key) << "\n";

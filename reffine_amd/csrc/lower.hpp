@@ -178,6 +178,11 @@ namespace rf {
 struct SegReducePlan {
     int table = -1;             // func input index of T
     Plan sub;                   // the row code's plan: driver T, probes keyed by $key
+    // Ranged form: one fold per value $key of a range iterator lb <= $key <
+    // ub (host expressions), over rows [lo, hi) of T (expressions of $key)
+    // instead of T's runs.
+    bool ranged = false;
+    N lb, ub, lo, hi;
     std::vector<Field> fields;  // row predicates/contributions over $col.T.C, $valid.T.C
     Type state_type;
     Type key_type;              // type of the run values (the iterator)

@@ -274,3 +274,13 @@ def graph_tables(n_nodes: int, n_edges: int, seed: int = 67) -> dict:
         "pr": {"id": np.arange(n_nodes, dtype=np.int64),
                "rank": np.full(n_nodes, 1.0 / n_nodes, dtype=np.float32)},
     }
+
+
+def bodies(n: int, seed: int = 71) -> dict:
+    """Bodies as reffine's Nbody reads them (misc.h:63-210): id = row, mass
+    in [1, 2), positions in [-1, 1)^3 (distinct with probability one),
+    velocities in [-0.1, 0.1)^3."""
+    u = lambda c: uniform(seed, c, n)
+    return {"id": np.arange(n, dtype=np.int64), "m": 1.0 + u(1), "x": 2 * u(2) - 1,
+            "y": 2 * u(3) - 1, "z": 2 * u(4) - 1, "vx": 0.2 * u(5) - 0.1,
+            "vy": 0.2 * u(6) - 0.1, "vz": 0.2 * u(7) - 0.1}

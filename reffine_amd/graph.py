@@ -11,7 +11,7 @@ first, as a function of its own, and passed to the rest as one more table
 """
 from __future__ import annotations
 
-from .reffiner import (_add, _buildidx, _cast, _div, _f32, _f32_t, _func, _get, _i64, _i64_t,
+from .reffiner import (_add, _buildidx, _cast, _div, _f32, _f32_t, _f64_t, _func, _get, _i64, _i64_t,
                        _in, _initval, _mul, _op, _red, _sym, _vec_t)
 
 EDGES_TYPE = _vec_t(2, _i64_t, _i64_t, _i64_t, encodings=["runend", "flat", "flat"])
@@ -71,3 +71,81 @@ def record_batches(tables: dict) -> dict:
     out["pr"] = pa.RecordBatch.from_arrays([pa.array(tables["pr"]["id"]),
                                             pa.array(tables["pr"]["rank"])], names=["id", "rank"])
     return out
+
+
+BODIES_TYPE = _vec_t(1, _i64_t, _f64_t, _f64_t, _f64_t, _f64_t, _f64_t, _f64_t, _f64_t)
+
+
+def nbody(n: int, g: float = 1.0, dt: float = 0.01):
+    """misc.h:77-202 (Nbody::build_op), as written: the pairwise forces of
+    n bodies as an Op over two ranges (id_a != id_b), then per body the sums
+    of the pair table's rows [idx*n, (idx+1)*n) (the reference's windows:
+    the pair table skips id_a == id_b, so a window runs into the next body's
+    pairs, and the last one past the table's end, where the reference reads
+    unchecked and this build folds only the rows that exist), then the
+    updated positions and velocities. bodies: (id, m, x, y, z, vx, vy, vz)."""
+    from .reffiner import _eq, _gte, _idx, _idx_t, _lt, _not, _readdata, _sqrt, _sub, _subvec, _f64
+    bodies = _sym("bodies", BODIES_TYPE)
+    id_a = _sym("id_a", _idx_t)
+    id_b = _sym("id_b", _idx_t)
+    cond = _not(_eq(id_a, id_b))
+    cond_sym = _sym("cond", cond)
+    dx = _sub(_readdata(bodies, id_a, 2), _readdata(bodies, id_b, 2))
+    dx_sym = _sym("dx", dx)
+    dy = _sub(_readdata(bodies, id_a, 3), _readdata(bodies, id_b, 3))
+    dy_sym = _sym("dy", dy)
+    dz = _sub(_readdata(bodies, id_a, 4), _readdata(bodies, id_b, 4))
+    dz_sym = _sym("dz", dz)
+    dist2 = _add(_add(_mul(dx_sym, dx_sym), _mul(dy_sym, dy_sym)), _mul(dz_sym, dz_sym))
+    dist2_sym = _sym("dist2", dist2)
+    dist = _sqrt(dist2_sym)
+    dist_sym = _sym("dist", dist)
+    f = _div(_mul(_f64(g), _mul(_readdata(bodies, id_a, 1), _readdata(bodies, id_b, 1))),
+             dist2_sym)
+    f_sym = _sym("f", f)
+    fx = _div(_mul(f_sym, dx_sym), dist_sym)
+    fx_sym = _sym("fx", fx)
+    fy = _div(_mul(f_sym, dy_sym), dist_sym)
+    fy_sym = _sym("fy", fy)
+    fz = _div(_mul(f_sym, dz_sym), dist_sym)
+    fz_sym = _sym("fz", fz)
+    op = _op([id_a, id_b], cond_sym & _gte(id_a, _idx(0)) & _lt(id_a, _idx(n)) &
+             _gte(id_b, _idx(0)) & _lt(id_b, _idx(n)), [fx_sym, fy_sym, fz_sym])
+    op_sym = _sym("pairs", op)
+    idx = _sym("idx", _idx_t)
+
+    def window_sum(col):
+        return _red(_subvec(op_sym, _mul(idx, _idx(n)), _mul(_add(idx, _idx(1)), _idx(n))),
+                    lambda: _f64(0), lambda s, v: _add(s, _get(v, col)))
+
+    sfx, sfy, sfz = window_sum(1), window_sum(2), window_sum(3)
+    sfx_sym, sfy_sym, sfz_sym = _sym("sfx", sfx), _sym("sfy", sfy), _sym("sfz", sfz)
+    op2 = _op([idx], _gte(idx, _idx(0)) & _lt(idx, _idx(n)), [sfx_sym, sfy_sym, sfz_sym])
+    op2_2 = _initval([op_sym], op2)
+    op2_sym = _sym("force", op2_2)
+    id_ = _sym("ID", _idx_t)
+    new_m_sym = _sym("M", _readdata(bodies, id_, 1))
+    old_m = _readdata(bodies, id_, 1)
+    vx = _add(_readdata(bodies, id_, 5),
+              _mul(_div(_readdata(op2_sym, id_, 1), new_m_sym), _f64(dt)))
+    vy = _add(_readdata(bodies, id_, 6),
+              _mul(_div(_readdata(op2_sym, id_, 2), new_m_sym), _f64(dt)))
+    vz = _add(_readdata(bodies, id_, 7),
+              _mul(_div(_readdata(op2_sym, id_, 3), new_m_sym), _f64(dt)))
+    vx_sym, vy_sym, vz_sym = _sym("VX", vx), _sym("VY", vy), _sym("VZ", vz)
+    x = _add(_readdata(bodies, id_, 2), _mul(vx_sym, _f64(dt)))
+    y = _add(_readdata(bodies, id_, 3), _mul(vy_sym, _f64(dt)))
+    z = _add(_readdata(bodies, id_, 4), _mul(vz_sym, _f64(dt)))
+    x_sym, y_sym, z_sym = _sym("X", x), _sym("Y", y), _sym("Z", z)
+    op3 = _op([id_], _gte(id_, _idx(0)) & _lt(id_, _idx(n)),
+              [new_m_sym, x_sym, y_sym, z_sym, vx_sym, vy_sym, vz_sym])
+    op3_2 = _initval([op2_sym], op3)
+    op3_sym = _sym("new_bodies", op3_2)
+    fn = _func("nbodies", op3_sym, [bodies])
+    for s_, e_ in ((new_m_sym, old_m), (vx_sym, vx), (vy_sym, vy), (vz_sym, vz), (x_sym, x),
+                   (y_sym, y), (z_sym, z), (cond_sym, cond), (dx_sym, dx), (dy_sym, dy),
+                   (dz_sym, dz), (dist2_sym, dist2), (dist_sym, dist), (f_sym, f), (fx_sym, fx),
+                   (fy_sym, fy), (fz_sym, fz), (op_sym, op), (op2_sym, op2_2), (sfx_sym, sfx),
+                   (sfy_sym, sfy), (sfz_sym, sfz), (op3_sym, op3_2)):
+        fn.tbl[s_] = e_
+    return fn

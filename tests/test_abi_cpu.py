@@ -217,3 +217,14 @@ def test_pagerank_materializes_the_indexed_op():
     assert "materialized as an input with an index" in text
     assert text.count("--- kernel ---") == 3
     assert "probe 0: table input 3 at key (sym $col.1.1 i64)" in text
+
+
+def test_nbody_splits_into_three_plans():
+    """Nbody (misc.h:63-210): the pair table and the force table are
+    materialized (nested), the force sums are a ranged fold over the pair
+    table's windows, the last Op reads both tables at computed rows."""
+    from reffine_amd import graph
+    text = runtime.compile_check(graph.nbody(64))
+    assert text.count("materialized as an input") == 2
+    assert "ranged fold: for $key in" in text
+    assert "readdata" in text

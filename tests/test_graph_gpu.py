@@ -28,3 +28,21 @@ def test_pagerank_step(device, n, m):
     again = q(E, R, P)
     np.testing.assert_array_equal(again.column(1).view(np.uint32), v.view(np.uint32))
     assert q.time(E, R, P, iters=2) > 0
+
+
+@pytest.mark.parametrize("n", [64, 256])
+def test_nbody_step(device, n):
+    """reffine's Nbody step (misc.h:63-210) as written: the pair table (an
+    Op over two ranges reading bodies at computed rows) and the per-body
+    window sums over it are computed first, the last Op reads them at its
+    iterator. Float64 results bit for bit against oracle.graph.nbody."""
+    b = datagen.bodies(n, seed=3)
+    T = DeviceTable.from_columns([b[c] for c in ("id", "m", "x", "y", "z", "vx", "vy", "vz")],
+                                 dim=1)
+    out = compile_op(graph.nbody(n))(T)
+    want = og.nbody(b, n)
+    assert out.nrows == n
+    for c in range(8):
+        got = out.column(c)
+        np.testing.assert_array_equal(got.view(np.uint64), np.asarray(want[c]).view(np.uint64),
+                                      err_msg=f"column {c}")
