@@ -172,14 +172,17 @@ int rf_table_free(rf_table* t) {
 }
 
 static std::unique_ptr<rf_query> make_query(const rf::N& f0, uint32_t flags,
-                                            const int32_t* tuning) {
+                                            const int32_t* tuning, int depth = 0) {
+    // Computed vectors nest (Nbody: the force table reads the pair table);
+    // a symbol table that refers back to itself would not end.
+    need(depth < 8, "computed vectors nested more than 8 deep");
     std::vector<rf::Prelude> pre;
     rf::N f = rf::split_materialized(f0, pre);
     auto q = std::make_unique<rf_query>();
     q->plan = rf::lower_func(f, false);
     q->n_user_args = (int64_t)f0->args.size() - 1;
     for (auto& p : pre) {
-        q->pre.push_back(make_query(p.func, flags, nullptr));
+        q->pre.push_back(make_query(p.func, flags, nullptr, depth + 1));
         q->pre_index.push_back(p.index);
     }
     q->det = (flags & RF_COMPILE_DETERMINISTIC) != 0;
@@ -259,11 +262,12 @@ int rf_compile_check(const char* ir_text, uint32_t flags, char* buf, size_t cap)
         std::string desc;
         std::vector<rf::KernelSource> kss;
         // Preludes (computed vectors read as tables) first, then the function.
-        std::function<void(const rf::N&)> add = [&](const rf::N& f0) {
+        std::function<void(const rf::N&, int)> add = [&](const rf::N& f0, int depth) {
+            need(depth < 8, "computed vectors nested more than 8 deep");
             std::vector<rf::Prelude> pre;
             rf::N f = rf::split_materialized(f0, pre);
             for (auto& p : pre) {
-                add(p.func);
+                add(p.func, depth + 1);
                 desc += std::string("  materialized as an input") +
                         (p.index ? " with an index" : "") + ": " + rf::to_text(p.func->args[0]) +
                         "\n";
@@ -289,7 +293,7 @@ int rf_compile_check(const char* ir_text, uint32_t flags, char* buf, size_t cap)
             }
             desc += plan.describe();
         };
-        add(rf::parse_func(ir_text));
+        add(rf::parse_func(ir_text), 0);
         size_t bytes = 0;
         std::string text;
         for (auto& ks : kss) {

@@ -1633,6 +1633,7 @@ static void run_preludes(rf_query* q, const rf_arg* args, int64_t n_args,
     for (size_t i = 0; i < q->pre.size(); i++) {
         rf_table* t = nullptr;
         run_query(q->pre[i].get(), args, n_args, nullptr, &t);
+        if (!t) throw IRError("a computed vector produced no table");
         mats.emplace_back(t);
         if (q->pre_index[i]) table_build_index(t);
         rf_arg a{};

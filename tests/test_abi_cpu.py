@@ -228,3 +228,23 @@ def test_nbody_splits_into_three_plans():
     assert text.count("materialized as an input") == 2
     assert "ranged fold: for $key in" in text
     assert "readdata" in text
+
+
+def test_self_referencing_computed_vector_is_rejected():
+    """A computed vector whose definition reads itself is refused (a status
+    error), not recursed into without end."""
+    from reffine_amd.reffiner import _func, _get, _i64_t, _in, _op, _sym, _vec_t
+    A = _sym("A", _vec_t(1, _i64_t, _i64_t))
+    t = _sym("t", _i64_t)
+    V = _sym("V", _vec_t(1, _i64_t, _i64_t))
+    op = _op([t], _in(t, A) & _in(t, V), [_get(A[t], 0)])
+    fn = _func("selfref", V, [A])
+    fn.tbl[V] = op
+    u = _sym("u", _i64_t)
+    top = _op([u], _in(u, A) & _in(u, V), [_get(V[u], 0)])
+    ts = _sym("top", top)
+    fn2 = _func("selfref2", ts, [A])
+    fn2.tbl[V] = op
+    fn2.tbl[ts] = top
+    with pytest.raises(runtime.ReffineError):
+        runtime.compile_check(fn2)
