@@ -280,7 +280,9 @@ int rf_query_free(rf_query* q);
  * launch over the list of all its units' (unit, wave-tile) items (each wave
  * a contiguous range), then per-unit completion and selection placement.
  * A group join unit runs its probe scan in the round; after the round one
- * sort and one fold launch serve all its group join units. Each unit's
+ * sort and one fold launch serve all its group join units. A unit whose
+ * function folds the runs of a run-end table or reads a computed vector
+ * as a table runs on the plain path when it is added. Each unit's
  * result equals the deterministic-mode run of the same query alone
  * (RF_COMPILE_DETERMINISTIC
  * queries give identical bits in both). `max_units` units per round;

@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <thread>
@@ -84,7 +85,12 @@ struct rf_batch {
         std::shared_ptr<rf::GjBuild> build;
         const rf::Spec* fold = nullptr;
         std::vector<void*> gj_outs;
-        bool no_scan = false;  // a group join without qualifying build keys
+        bool no_scan = false;  // no scan in the round (an empty group join, a plain unit)
+        // A function the round's kernels do not take (run folds, computed
+        // vectors): run on the plain path when added; its output kept here.
+        bool plain = false;
+        std::vector<char> plain_out;
+        std::shared_ptr<rf_table> plain_tbl;
     };
     std::vector<Unit> units;
     int64_t join_groups = 0;  // groups of the round's group join units
@@ -153,8 +159,22 @@ static void* arena_take(rf_batch* b, size_t bytes, size_t& used) {
 static void plan_unit(rf_batch::Unit& u, rf_query* q, const rf_arg* args, int64_t n_args) {
     u.q = q;
     u.args.assign(args, args + n_args);
-    if (q->plan.seg || q->plan.runs) throw Unsupported("run reductions do not run in a batch");
-    if (!q->pre.empty()) throw Unsupported("computed vectors read as tables do not run in a batch");
+    if (q->plan.seg || q->plan.runs || !q->pre.empty()) {
+        // Run folds and computed vectors are not batch kernels: the unit
+        // runs on the plain path when added (batch_add).
+        std::function<bool(const Type&)> holds_table = [&](const Type& t) {
+            if (t.is_vector()) return true;
+            for (auto& d : t.dtypes)
+                if (holds_table(d)) return true;
+            return false;
+        };
+        const Type& ot = q->plan.output_type;
+        if (!ot.is_vector() && holds_table(ot))
+            throw Unsupported("a struct output holding a table from a run fold in a batch");
+        u.plain = true;
+        u.no_scan = true;
+        return;
+    }
     // The query's spec cache (and a group join's build cache) may be used by
     // a plain run on another thread.
     std::lock_guard<std::mutex> lk(q->mu);
@@ -178,6 +198,7 @@ static void plan_unit(rf_batch::Unit& u, rf_query* q, const rf_arg* args, int64_
 }
 
 static size_t unit_footprint(const rf_batch::Unit& u) {
+    if (u.plain) return 0;
     size_t f = 0;
     if (u.build) {
         if (u.no_scan) return 0;
@@ -210,6 +231,12 @@ bool batch_add(rf_batch* b, rf_query* q, const rf_arg* args, int64_t n_args, int
     rf_batch::Unit u;
     plan_unit(u, q, args, n_args);
     const int idx = (int)b->units.size();
+    if (u.plain) {
+        u.plain_out.assign(4096, 0);
+        rf_table* t = nullptr;
+        run_query(q, args, n_args, u.plain_out.data(), &t);
+        u.plain_tbl.reset(t, [](rf_table* x) { table_free(x); });
+    }
     if (u.no_scan) {
         b->units.push_back(std::move(u));
         *unit = idx;
@@ -557,6 +584,25 @@ void batch_result(rf_batch* b, int32_t unit, void* output_addr, rf_table** out_t
     if (b->post_pending) throw IRError("the round has not finished");
     if (unit < 0 || unit >= (int32_t)b->units.size()) throw IRError("no such unit");
     auto& u = b->units[unit];
+    if (u.plain) {
+        // A view sharing the unit's output buffers (valid after the reset).
+        rf_table* t = u.plain_tbl ? new rf_table(*u.plain_tbl) : nullptr;
+        if (t && !out_table) {
+            delete t;
+            t = nullptr;
+        }
+        if (output_addr) {
+            if (t)
+                *(void**)output_addr = t;
+            else {
+                size_t sz = 0, al = 0;
+                c_layout(u.q->plan.output_type, sz, al);  // the caller's struct size
+                std::memcpy(output_addr, u.plain_out.data(), std::min(sz, u.plain_out.size()));
+            }
+        }
+        if (out_table) *out_table = t;
+        return;
+    }
     if (u.build) {
         if (!out_table) throw IRError("an Op output needs out_table");
         const GroupJoinPlan& gp = *u.q->plan.gj;

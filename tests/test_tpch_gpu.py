@@ -208,21 +208,37 @@ def test_run_stage_nulls_and_a_second_table(device):
     np.testing.assert_array_equal(out.column(3), ew)
 
 
-def test_run_stage_deterministic_and_not_batched(device, tables):
-    """Deterministic compilation gives the same results; a batch refuses
-    a run stage with an unsupported-status error."""
-    from reffine_amd.batch import Batch
-    from reffine_amd.runtime import ReffineError
+def test_run_stage_deterministic_and_batched(device, tables):
+    """Deterministic compilation gives the same results; in a batch (and
+    served), run-stage units run on the plain path beside the round's scan
+    units and return the same outputs."""
+    from reffine_amd.batch import Batch, serve
+    from reffine_amd.reffiner import _f64, _func, _red, _subvec, _sym, _idx, _len, _add, _get
     t, dev = tables
     assert list(compile_op(tpch.q4(), deterministic=True)(*_args(dev, "q4"))) == ot.q4(t)
     out = compile_op(tpch.q18(150), deterministic=True)(*_args(dev, "q18"))
     k, v = ot.q18(t, 150)
     np.testing.assert_array_equal(out.column(0), k)
     np.testing.assert_array_equal(out.column(1), v)
-    q = compile_op(tpch.q18(150), deterministic=True)
-    b = Batch()
-    with pytest.raises(ReffineError):
-        b.add(q, *_args(dev, "q18"))
+    q18 = compile_op(tpch.q18(150), deterministic=True)
+    q4 = compile_op(tpch.q4(), deterministic=True)
+    from reffine_amd.reffiner import _vec_t, _i64_t, _i8_t, _f64_t, _i32_t
+    o = _sym("orders", _vec_t(2, _i64_t, _i64_t, _i8_t, _f64_t, _i64_t, _i8_t, _i64_t, _i32_t,
+                              _i64_t))
+    tot = _red(_subvec(o, _idx(0), _len(o, 1)), lambda: _f64(0), lambda s, r: _add(s, _get(r, 2)))
+    ts = _sym("tot", tot)
+    fn = _func("total_price", ts, [o])
+    fn.tbl[ts] = tot
+    qs = compile_op(fn, deterministic=True)
+    o2 = dev["orders"].view(2)
+    work = [(q18, tuple(_args(dev, "q18"))), (qs, (o2,)), (q4, tuple(_args(dev, "q4")))]
+    res = Batch().run(work)  # a table output comes back as its host columns
+    np.testing.assert_array_equal(res[0][0], k)
+    np.testing.assert_array_equal(res[0][1], v)
+    assert res[1] == qs(o2)
+    assert list(res[2]) == ot.q4(t)
+    sr = serve([(q4, tuple(_args(dev, "q4")), 0.0), (qs, (o2,), 0.0)] * 3)
+    assert (sr.status == 0).all()
 
 
 def test_run_stage_empty_lineitem(device, tables):
