@@ -62,12 +62,14 @@ def main() -> int:
     ap.add_argument("--cpu-rows", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "tpch"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "tpch", "programs"],
+                    default="c2",
                     help="c2 (default): the 60M-row scan-filter-aggregate + selection; c3: the "
                          "60M-row grouped aggregate; c4: the 1.5M x 60M hash join + grouped "
                          "aggregate; c5: 4096 queries over 64 record batches, batched/served "
                          "against plain per-query launches; tpch: TPC-H Q3/Q4/Q12/Q18 over a "
-                         "run-end encoded lineitem")
+                         "run-end encoded lineitem; programs: reffine's PageRank and Nbody "
+                         "steps and TPC-DS Q9 (benchmark/include/misc.h, tpch.h)")
     ap.add_argument("--orders", type=int, default=15_000_000,
                     help="tpch: orders (lineitem has about 3.75 lines per order)")
     ap.add_argument("--parts", type=int, default=2_000_000,
@@ -97,6 +99,8 @@ def main() -> int:
         return bench_c5(args)
     if args.workload == "tpch":
         return bench_tpch(args)
+    if args.workload == "programs":
+        return bench_programs(args)
     n = args.rows
     log(f"generating {n} rows")
     c = datagen.scan_table(n, args.seed, with_q=True, rowid_dtype=np.int32)
@@ -392,6 +396,87 @@ def bench_tpch(args) -> int:
                                 "kind": "port",
                                 "sample": f"the four queries over {sample} orders, numpy "
                                           "restatement folding runs position by position"}
+    print(json.dumps(line), flush=True)
+    return 0 if ok_all else 1
+
+
+def bench_programs(args) -> int:
+    """reffine's PageRank step (misc.h:212-291; a materialised, indexed
+    contribution table and a run fold over in-edges), Nbody step (misc.h:63-210;
+    pair table, ranged window folds, indexed reads) and TPC-DS Q9 (tpch.h:
+    218-313; a 5-bucket SubVector fold), each verified at size."""
+    import math
+    from oracle import graph as og
+    from oracle import tpch as ot
+    from reffine_amd import DeviceTable, compile_op, datagen, graph, tpch
+    per = {}
+    ok_all = True
+    tot_bytes, tot_ms = 0, 0.0
+    # PageRank: 2M nodes, 32M edge draws.
+    n, m = 2_000_000, 32_000_000
+    log(f"pagerank: {n} nodes, {m} edge draws")
+    t = datagen.graph_tables(n, m, seed=5)
+    rb = graph.record_batches(t)
+    E = DeviceTable.from_arrow(rb["edges"], dim=2)
+    R = DeviceTable.from_arrow(rb["rev_edges"], dim=2)
+    P = DeviceTable.from_arrow(rb["pr"]).build_index()
+    del rb
+    q = compile_op(graph.pagerank(n))
+    out, ms = _timed(lambda: q(E, R, P), args.steps, args.warmup)
+    k, v = og.pagerank_np(t, n)
+    ok = np.array_equal(out.column(0), k) and np.array_equal(out.column(1).view(np.uint32),
+                                                           v.view(np.uint32))
+    ne = len(t["edges"]["src"])
+    nb = (12 * (len(np.unique(t["edges"]["src"])) + len(k)) + 8 * ne + 20 * ne + 12 * n
+          + 12 * len(k))
+    per["pagerank"] = {"ms": round(ms, 4), "edges": ne, "gbs": round(nb / (ms * 1e-3) / 1e9, 2),
+                       "edges_per_s": round(ne / (ms * 1e-3)), "verified": bool(ok)}
+    log(f"pagerank: {per['pagerank']}")
+    tot_bytes, tot_ms, ok_all = tot_bytes + nb, tot_ms + ms, ok_all and ok
+    del E, R, P, t, out
+    # Nbody: 2048 bodies (4.19M pairs), the reference's size.
+    nb_n = 2048
+    b = datagen.bodies(nb_n, seed=3)
+    T = DeviceTable.from_columns([b[c] for c in ("id", "m", "x", "y", "z", "vx", "vy", "vz")],
+                                 dim=1)
+    q = compile_op(graph.nbody(nb_n))
+    log("nbody: timing")
+    out, ms = _timed(lambda: q(T), args.steps, args.warmup)
+    want = og.nbody_np(b, nb_n)
+    ok = all(np.array_equal(out.column(c).view(np.uint64), np.asarray(want[c]).view(np.uint64))
+             for c in range(8))
+    pairs = nb_n * nb_n - nb_n
+    per["nbody"] = {"ms": round(ms, 4), "pairs": pairs, "pairs_per_s": round(pairs / (ms * 1e-3)),
+                    "verified": bool(ok)}
+    log(f"nbody: {per['nbody']}")
+    tot_ms, ok_all = tot_ms + ms, ok_all and ok
+    # TPC-DS Q9: 30M store_sales rows.
+    ns = 30_000_000
+    log(f"tpcds q9: {ns} rows")
+    ss = datagen.store_sales(ns, seed=7)
+    T = DeviceTable.from_arrow(tpch.store_sales_batch(ss), dim=2)
+    q = compile_op(tpch.tpcds_q9())
+    got, ms = _timed(lambda: q(T), args.steps, args.warmup)
+    want = ot.tpcds_q9(ss)
+    flat = []
+    for x in got:
+        flat += list(x) if isinstance(x, (list, tuple)) else [x]
+    ok = all(flat[3 * i] == w[0] and math.isclose(flat[3 * i + 1], w[1], rel_tol=1e-12) and
+             math.isclose(flat[3 * i + 2], w[2], rel_tol=1e-12) for i, w in enumerate(want))
+    run_ms, scan_ms = q.time_kernels(T, iters=max(args.steps, 10))
+    by = 20 * ns
+    per["tpcds_q9"] = {"ms": round(ms, 4), "kernel_ms": round(scan_ms, 4), "rows": ns,
+                       "gbs": round(by / (ms * 1e-3) / 1e9, 2),
+                       "kernel_gbs": round(by / (scan_ms * 1e-3) / 1e9, 2), "verified": bool(ok)}
+    log(f"tpcds_q9: {per['tpcds_q9']}")
+    tot_bytes, tot_ms, ok_all = tot_bytes + by, tot_ms + ms, ok_all and ok
+    cfg = {"workload": "reffine's PageRank step (2M nodes), Nbody step (2048 bodies) and "
+                       "TPC-DS Q9 (30M store_sales rows), as written; value: algorithmic bytes of "
+                       "PageRank and Q9 / the three programs' summed wall time",
+           "programs": per, "verified": bool(ok_all)}
+    line = _line(cfg, tot_bytes / (tot_ms * 1e-3) / 1e9, tot_ms / 3, args,
+                 per["tpcds_q9"]["kernel_gbs"], "rf_q (TPC-DS Q9: 5-bucket fold, 20 B/row)",
+                 per["tpcds_q9"]["kernel_ms"], run_ms)
     print(json.dumps(line), flush=True)
     return 0 if ok_all else 1
 

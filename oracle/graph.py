@@ -75,3 +75,51 @@ def nbody(b: dict, n: int, g: float = 1.0, dt: float = 0.01):
              b["vz"] + (force[2] / m) * dt]
     out_p = [x + out_v[0] * dt, y + out_v[1] * dt, z + out_v[2] * dt]
     return [np.arange(n, dtype=np.int64), m.copy(), *out_p, *out_v]
+
+
+def pagerank_np(t: dict, n: int, alpha: float = 0.85):
+    """pagerank() vectorised over runs (each run still folded in row
+    order, position by position, in float32) for large graphs."""
+    e, r, pr = t["edges"], t["rev_edges"], t["pr"]
+    s = e["src"]
+    cut = np.flatnonzero(np.diff(s)) + 1
+    lo = np.concatenate([[0], cut]) if len(s) else np.zeros(0, np.int64)
+    deg = np.add.reduceat(e["count"], lo) if len(s) else np.zeros(0, np.int64)
+    keys = s[lo]
+    contrib = np.zeros(n, dtype=np.float32)
+    contrib[keys] = pr["rank"][keys] / deg.astype(np.float32)
+    d = r["dst"]
+    cut = np.flatnonzero(np.diff(d)) + 1
+    lo = np.concatenate([[0], cut])
+    hi = np.concatenate([cut, [len(d)]])
+    acc = np.zeros(len(lo), dtype=np.float32)
+    term = r["count"].astype(np.float32) * contrib[r["src"]]
+    ln = hi - lo
+    for j in range(int(ln.max()) if len(ln) else 0):
+        m = ln > j
+        acc[m] = acc[m] + term[lo[m] + j]
+    out = np.float32(alpha) * acc + np.float32((1 - alpha) / n)
+    return d[lo], out.astype(np.float32)
+
+
+def nbody_np(b: dict, n: int, g: float = 1.0, dt: float = 0.01):
+    """nbody() with the window sums folded position by position."""
+    m, x, y, z = b["m"], b["x"], b["y"], b["z"]
+    a_i, b_i = np.divmod(np.arange(n * n, dtype=np.int64), n)
+    keep = a_i != b_i
+    a_i, b_i = a_i[keep], b_i[keep]
+    dx, dy, dz = x[a_i] - x[b_i], y[a_i] - y[b_i], z[a_i] - z[b_i]
+    dist2 = (dx * dx + dy * dy) + dz * dz
+    dist = np.sqrt(dist2)
+    f = (g * (m[a_i] * m[b_i])) / dist2
+    pf = np.stack([(f * dx) / dist, (f * dy) / dist, (f * dz) / dist])
+    rows = pf.shape[1]
+    force = np.zeros((3, n))
+    for j in range(n):
+        r = np.arange(n) * n + j
+        ok = r < rows
+        force[:, ok] = force[:, ok] + pf[:, r[ok]]
+    out_v = [b["vx"] + (force[0] / m) * dt, b["vy"] + (force[1] / m) * dt,
+             b["vz"] + (force[2] / m) * dt]
+    out_p = [x + out_v[0] * dt, y + out_v[1] * dt, z + out_v[2] * dt]
+    return [np.arange(n, dtype=np.int64), m.copy(), *out_p, *out_v]

@@ -342,3 +342,16 @@ def test_pagerank_oracle_against_sparse_matrix():
     bad = {**t, "rev_edges": {**t["rev_edges"], "count": np.ones_like(t["rev_edges"]["count"])}}
     if (t["rev_edges"]["count"] > 1).any():
         assert not np.allclose(og.pagerank(bad, n)[1], want[k], rtol=1e-5)
+
+
+def test_graph_vectorised_oracles_match_the_loops():
+    """oracle.graph's numpy forms equal its row loops bit for bit."""
+    from oracle import graph as og
+    t = datagen.graph_tables(700, 5000, seed=4)
+    k, v = og.pagerank(t, 700)
+    kn, vn = og.pagerank_np(t, 700)
+    np.testing.assert_array_equal(k, kn)
+    np.testing.assert_array_equal(v.view(np.uint32), vn.view(np.uint32))
+    b = datagen.bodies(48, seed=2)
+    for x, y in zip(og.nbody(b, 48), og.nbody_np(b, 48)):
+        np.testing.assert_array_equal(np.asarray(x).view(np.uint64), np.asarray(y).view(np.uint64))
