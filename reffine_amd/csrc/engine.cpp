@@ -477,6 +477,8 @@ Module& jit_module(const KernelSource& ks) {
     auto m = std::make_unique<Module>();
     RF_HIP(hipModuleLoadData(&m->mod, code.data()));
     RF_HIP(hipModuleGetFunction(&m->fn, m->mod, ks.entry.c_str()));
+    if (ks.entry == "rf_seg_fold" && ks.alt_cap > 0)
+        RF_HIP(hipModuleGetFunction(&m->seg_long, m->mod, "rf_seg_fold_long"));
     if (ks.entry == "rf_gj_fold") {
         RF_HIP(hipModuleGetFunction(&m->fold_b, m->mod, "rf_gj_fold_b"));
         RF_HIP(hipModuleGetFunction(&m->fold_w, m->mod, "rf_gj_fold_w"));
@@ -1578,7 +1580,12 @@ static void launch_seg(rf_query* q, const SegReducePlan& sp, Spec& spec, const r
     fill_params(spec.ks, c, A.p);
     void* params[] = {&A};
     const int g = (int)std::min<int64_t>((runs + 255) / 256, 16384);
-    RF_HIP(hipModuleLaunchKernel(spec.mod->fn, g, 1, 1, 256, 1, 1, 0, s, params, nullptr));
+    // The wide copy when a block's 256 runs would not fit the small one on
+    // average (with a quarter to spare).
+    hipFunction_t fn = spec.mod->fn;
+    if (spec.mod->seg_long && !sp.ranged && 5 * 256 * A.rows > 4 * (int64_t)spec.ks.cap * runs)
+        fn = spec.mod->seg_long;
+    RF_HIP(hipModuleLaunchKernel(fn, g, 1, 1, 256, 1, 1, 0, s, params, nullptr));
 }
 
 // A new table of one row per run of T, or per key of a ranged fold (no

@@ -828,8 +828,11 @@ KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
     // average), at most 40 KiB over the staged columns (4 blocks per CU).
     int row_bytes = 0;
     for (int sl : g.lds_slots) row_bytes += ks.cols[sl].type.width();
-    int cap = 0;
-    if (row_bytes > 0) cap = std::min(1536, (40960 / row_bytes) / 256 * 256);
+    int cap = 0, cap_long = 0;
+    if (row_bytes > 0) {
+        cap = std::min(1536, (40960 / row_bytes) / 256 * 256);
+        cap_long = std::min(8192, (65536 / row_bytes) / 256 * 256);
+    }
     const bool staged = cap >= 256 && !sp.ranged;
     std::string lo_e, hi_e;
     std::ostringstream bnd;
@@ -841,9 +844,9 @@ KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
         hi_e = er.expr(sp.hi);
     }
 
-    std::ostringstream o;
-    o << "// Generated run reduction for '" << sp.sym_name << "'.\n";
-    o << "extern \"C\" __global__ __launch_bounds__(256) void rf_seg_fold(RfSegArgs L) {\n";
+    for (size_t j = 0; j < ov.size(); j++) ks.out_types.push_back(sp.out_types[j]);
+    auto emit_kernel = [&](std::ostringstream& o, const char* kname, int cap, bool staged) {
+    o << "extern \"C\" __global__ __launch_bounds__(256) void " << kname << "(RfSegArgs L) {\n";
     if (staged) {
         o << "    constexpr int64_t CAP = " << cap << ";\n";
         for (int sl : g.lds_slots)
@@ -914,9 +917,20 @@ KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
     for (size_t j = 0; j < ov.size(); j++) {
         const std::string T = cty(sp.out_types[j]);
         o << "            ((" << T << "*)L.out[" << j << "])[run] = (" << T << ")" << ov[j] << ";\n";
-        ks.out_types.push_back(sp.out_types[j]);
     }
     o << "        }\n    }\n}\n";
+    };
+    std::ostringstream o;
+    o << "// Generated run reduction for '" << sp.sym_name << "'.\n";
+    emit_kernel(o, "rf_seg_fold", cap, staged);
+    // For runs longer than 6 rows on average (graph adjacency runs): the same
+    // fold with the copy sized to 64 KiB (at most 32 rows per run), chosen
+    // at launch from the table's rows per run.
+    if (staged && cap_long > cap) {
+        emit_kernel(o, "rf_seg_fold_long", cap_long, true);
+        ks.alt_cap = cap_long;
+    }
+    ks.cap = cap;
     ks.code = o.str();
     return ks;
 }

@@ -58,6 +58,10 @@ struct KernelSource {
     std::vector<Type> stage_types; // staged per selected row: row index, carried probe results
     Tuning tune;
     bool det = false;
+    // Run folds: rows per block the LDS copy holds (0: not staged), and the
+    // capacity of the rf_seg_fold_long variant (0: none).
+    int cap = 0;
+    int alt_cap = 0;
 };
 
 // `nullable[t][c]` says whether column c of func input t has a bitmap.
