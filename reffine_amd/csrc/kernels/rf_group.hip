@@ -468,12 +468,29 @@ extern "C" __global__ void rf_seg_bounds(const uint32_t* __restrict__ rank, int6
 
 // Rank map of sorted distinct keys: mark bit (key - base); then the popcount
 // of each 64-bit word (scanned afterwards into the words' first ranks).
+// Lanes of a wave whose keys fall in the same 64-bit word combine their bits
+// first (sorted keys put a wave's 64 keys in one or two words), so one lane
+// per word issues the atomic. The loop steps whole waves: every lane takes
+// part in the shuffles.
 extern "C" __global__ void rf_rank_bits(const int64_t* __restrict__ keys, int64_t n, int64_t base,
                                         unsigned long long* bits) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t off = (uint64_t)keys[i] - (uint64_t)base;
-        atomicOr(&bits[off >> 6], 1ull << (off & 63));
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); i0 < n;
+         i0 += stride) {
+        const int64_t i = i0 + lane;
+        const bool act = i < n;
+        const uint64_t off = act ? (uint64_t)keys[i] - (uint64_t)base : 0;
+        // Inactive lanes get word numbers no key has (distinct per lane).
+        const uint64_t w = act ? (off >> 6) : ~(uint64_t)lane;
+        uint64_t b = act ? (1ull << (off & 63)) : 0;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint64_t ow = __shfl_down(w, d, 64);
+            const uint64_t ob = __shfl_down(b, d, 64);
+            if (lane + d < 64 && ow == w) b |= ob;
+        }
+        const uint64_t pw = __shfl_up(w, 1, 64);
+        if (act && (lane == 0 || pw != w)) atomicOr(&bits[w], b);
     }
 }
 
