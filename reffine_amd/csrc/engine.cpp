@@ -479,6 +479,8 @@ Module& jit_module(const KernelSource& ks) {
     RF_HIP(hipModuleGetFunction(&m->fn, m->mod, ks.entry.c_str()));
     if (ks.entry == "rf_seg_fold" && ks.alt_cap > 0)
         RF_HIP(hipModuleGetFunction(&m->seg_long, m->mod, "rf_seg_fold_long"));
+    if (ks.entry == "rf_seg_fold")
+        RF_HIP(hipModuleGetFunction(&m->seg_wave, m->mod, "rf_seg_fold_wave"));
     if (ks.entry == "rf_gj_fold") {
         RF_HIP(hipModuleGetFunction(&m->fold_b, m->mod, "rf_gj_fold_b"));
         RF_HIP(hipModuleGetFunction(&m->fold_w, m->mod, "rf_gj_fold_w"));
@@ -1583,6 +1585,12 @@ static void launch_seg(rf_query* q, const SegReducePlan& sp, Spec& spec, const r
     // The wide copy when a block's 256 runs would not fit the small one on
     // average (with a quarter to spare).
     hipFunction_t fn = spec.mod->fn;
+    if (spec.mod->seg_wave && A.rows >= 32 * runs) {
+        // 32 rows per run or window and more: a wave per fold.
+        const int gw = (int)std::min<int64_t>((runs + 3) / 4, 65536);
+        RF_HIP(hipModuleLaunchKernel(spec.mod->seg_wave, gw, 1, 1, 256, 1, 1, 0, s, params, nullptr));
+        return;
+    }
     if (spec.mod->seg_long && !sp.ranged && 5 * 256 * A.rows > 4 * (int64_t)spec.ks.cap * runs)
         fn = spec.mod->seg_long;
     RF_HIP(hipModuleLaunchKernel(fn, g, 1, 1, 256, 1, 1, 0, s, params, nullptr));

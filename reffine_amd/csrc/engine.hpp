@@ -142,6 +142,7 @@ struct Module {
     hipFunction_t fold_w = nullptr;  // rf_gj_fold_w / _wb: a wave per group (large groups)
     hipFunction_t fold_wb = nullptr;
     hipFunction_t seg_long = nullptr; // rf_seg_fold_long: run folds with long runs
+    hipFunction_t seg_wave = nullptr; // rf_seg_fold_wave: a wave per run or window
     int max_blocks_per_cu = 1;
     int bscan_blocks_per_cu = 1;
 };

@@ -931,6 +931,86 @@ KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
         ks.alt_cap = cap_long;
     }
     ks.cap = cap;
+    // A wave per fold, for runs or windows of 32 rows and more (a window of
+    // Nbody's pair table holds 2047): the 64 lanes load 64 rows at a time
+    // (coalesced) and evaluate their predicates and contributions; every lane
+    // then folds the 64 in row order from lane broadcasts -- the same
+    // operations in the same order as one thread's loop.
+    {
+        std::ostringstream wb;
+        g.lds_rows = false;
+        Emitter ew(g, Mode::MOVE, wb, "                ");
+        ew.ntmp = nt + 2000;
+        ew.probes_done = ei.probes_done;
+        ew.memo = ei.memo;
+        std::vector<std::string> wp(sp.fields.size()), wc(sp.fields.size());
+        for (size_t f = 0; f < sp.fields.size(); f++) {
+            if (!sp.fields[f].updated) continue;
+            wp[f] = ew.expr(sp.fields[f].pred);
+            wc[f] = ew.expr(sp.fields[f].contrib);
+        }
+        o << "extern \"C\" __global__ __launch_bounds__(256) void rf_seg_fold_wave(RfSegArgs L) {\n";
+        o << "    const int lane = threadIdx.x & 63;\n";
+        o << "    const int64_t nw = (int64_t)gridDim.x * 4;\n";
+        o << "    for (int64_t run = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); run < L.runs;\n"
+             "         run += nw) {\n";
+        o << "        const int64_t rr = run;\n";
+        if (sp.ranged)
+            o << "        const int64_t key = L.key0 + rr;\n        (void)key;\n";
+        else
+            o << "        const " << cty(sp.key_type) << " key = ((const " << cty(sp.key_type)
+              << "*)L.keys)[rr];\n        (void)key;\n";
+        o << init.str();
+        o << "        {\n";
+        if (sp.ranged) {
+            o << bnd.str();
+            o << "            int64_t lo = (int64_t)" << lo_e << ";\n";
+            o << "            int64_t hi = (int64_t)" << hi_e << ";\n";
+            o << "            lo = lo < 0 ? 0 : lo;\n";
+            o << "            hi = hi < L.rows ? hi : L.rows;\n";
+        } else {
+            o << "            const int64_t lo = run > 0 ? (int64_t)L.run_ends[run - 1] : 0;\n";
+            o << "            const int64_t hi0 = (int64_t)L.run_ends[run];\n";
+            o << "            const int64_t hi = hi0 < L.rows ? hi0 : L.rows;\n";
+        }
+        for (size_t f = 0; f < sp.fields.size(); f++) {
+            const std::string T = cty(sp.fields[f].type);
+            o << "            " << T << " s" << f << " = (" << T << ")" << iv[f] << ";\n";
+        }
+        o << "            for (int64_t c = lo; c < hi; c += 64) {\n";
+        o << "                const int64_t row0 = c + lane;\n";
+        o << "                const int64_t row = row0 < hi ? row0 : lo;  // lanes past the end read row lo\n";
+        o << wb.str();
+        for (size_t f = 0; f < sp.fields.size(); f++) {
+            if (!sp.fields[f].updated) continue;
+            const std::string T = cty(sp.fields[f].type), F = std::to_string(f);
+            o << "                const bool wp" << F << " = (bool)(" << wp[f] << ");\n";
+            o << "                const " << T << " wv" << F << " = (" << T << ")(" << wc[f] << ");\n";
+        }
+        o << "                const int cnt = (int)(hi - c < 64 ? hi - c : 64);\n";
+        o << "                for (int k = 0; k < cnt; k++) {\n";
+        for (size_t f = 0; f < sp.fields.size(); f++) {
+            if (!sp.fields[f].updated) continue;
+            const std::string T = cty(sp.fields[f].type), S = "s" + std::to_string(f);
+            const std::string F = std::to_string(f);
+            o << "                    {\n"
+              << "                        const bool pk = rf_readlane(wp" << F << ", k);\n"
+              << "                        const " << T << " vk = rf_readlane(wv" << F << ", k);\n"
+              << "                        " << S << " = pk ? "
+              << combine_expr(sp.fields[f].op, sp.fields[f].type, S, "vk") << " : " << S << ";\n"
+              << "                    }\n";
+        }
+        o << "                }\n";
+        o << "            }\n";
+        o << ob.str();
+        o << "            if (lane == 0) {\n";
+        for (size_t j = 0; j < ov.size(); j++) {
+            const std::string T = cty(sp.out_types[j]);
+            o << "                ((" << T << "*)L.out[" << j << "])[run] = (" << T << ")" << ov[j]
+              << ";\n";
+        }
+        o << "            }\n        }\n    }\n}\n";
+    }
     ks.code = o.str();
     return ks;
 }
