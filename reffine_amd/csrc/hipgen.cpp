@@ -873,13 +873,31 @@ KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
         o << "        bhi = bhi > blo ? bhi : blo;\n";
         o << "        const bool in_lds = bhi - blo <= CAP;\n";
         o << "        __syncthreads();  // the previous runs' LDS reads are done\n";
+        // U rows per lane per pass, all loads issued before the LDS stores
+        // (a store waits for its own load; one row a pass serialises them).
+        const int U = std::max(2, std::min(8, 64 / row_bytes));
         o << "        if (in_lds) {\n";
-        o << "            for (int64_t i = threadIdx.x; i < bhi - blo; i += 256) {\n";
+        o << "            const int64_t nb = bhi - blo;\n";
+        o << "            for (int64_t i0 = threadIdx.x; i0 < nb; i0 += 256 * " << U << ") {\n";
+        for (int sl : g.lds_slots)
+            o << "                " << cty(ks.cols[sl].type) << " r" << sl << "[" << U << "];\n";
+        o << "#pragma unroll\n";
+        o << "                for (int u = 0; u < " << U << "; u++) {\n";
+        o << "                    const int64_t i = i0 + u * 256;\n";
+        o << "                    if (i < nb) {\n";
         for (int sl : g.lds_slots) {
             const std::string T = cty(ks.cols[sl].type);
-            o << "                sm" << sl << "[i] = rf_g((const " << T << "*)L.col[" << sl
-              << "])[blo + i];\n";
+            o << "                        r" << sl << "[u] = rf_g((const " << T << "*)L.col["
+              << sl << "])[blo + i];\n";
         }
+        o << "                    }\n                }\n";
+        o << "#pragma unroll\n";
+        o << "                for (int u = 0; u < " << U << "; u++) {\n";
+        o << "                    const int64_t i = i0 + u * 256;\n";
+        o << "                    if (i < nb) {\n";
+        for (int sl : g.lds_slots)
+            o << "                        sm" << sl << "[i] = r" << sl << "[u];\n";
+        o << "                    }\n                }\n";
         o << "            }\n";
         o << "        }\n";
         o << "        __syncthreads();\n";
