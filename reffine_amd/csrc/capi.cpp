@@ -233,6 +233,7 @@ static std::unique_ptr<rf_query> make_query(const rf::N& f0, uint32_t flags,
                                                  : q->plan.inputs[i]->type.dtypes.size();
             rsig += std::string(nc, '0') + "|";
         }
+        rsig += std::string(rs.seg.sub.probes.size(), 'h');  // hash-probed tables
         q->run_specs.emplace(rsig, std::move(rsp));
     }
     return q;

@@ -843,10 +843,21 @@ static std::vector<N> run_stage_inputs(const Plan& plan) {
 static Spec& get_run_spec(rf_query* q, const rf_arg* args, int64_t n_args) {
     const std::vector<N> inputs = run_stage_inputs(q->plan);
     std::string sig = null_signature(inputs, args, n_args);
+    // And by each probed table's index form: a packed rank map over
+    // ascending keys ('p') is probed without branches.
+    std::vector<bool> pack;
+    for (const Probe& pr : q->plan.runs->seg.sub.probes) {
+        rf_table* pt = args[pr.table].u.table;
+        if (!pt->has_index) table_build_index(pt);
+        const bool p = pt->index.dense_pack && !pt->index.dense_rows && pt->index.dense_len > 0;
+        pack.push_back(p);
+        sig += p ? 'p' : 'h';
+    }
     auto it = q->run_specs.find(sig);
     if (it != q->run_specs.end()) return it->second;
     Spec sp;
-    sp.ks = gen_seg_fold(q->plan.runs->seg, q->plan, nullable_of(inputs.size(), args, inputs));
+    sp.ks = gen_seg_fold(q->plan.runs->seg, q->plan, nullable_of(inputs.size(), args, inputs),
+                         pack);
     sp.mod = &jit_module(sp.ks);
     return q->run_specs.emplace(sig, std::move(sp)).first->second;
 }

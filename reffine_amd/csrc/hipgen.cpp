@@ -46,6 +46,13 @@ struct Gen {
     // (sm<slot>[row - blo]) instead of global memory; the slots so read.
     bool lds_rows = false;
     std::set<int> lds_slots;
+    // Run folds: probes whose table is a packed rank map with ascending keys
+    // and at least one key; their row code probes (rf_probe_pack) and reads
+    // the found row (clamped to row 0, then selected) without branches.
+    std::vector<bool> pack_probes;
+    bool packed(int j) const {
+        return j < (int)pack_probes.size() && pack_probes[j];
+    }
 
     Gen(const Plan& p, const std::vector<std::vector<bool>>& nl, KernelSource& k)
         : plan(p), nullable(nl), ks(k) {}
@@ -130,7 +137,11 @@ struct Emitter {
         if (!probes_done.count(j)) {
             const Probe& pr = g.plan.probes[j];
             std::string k = expr(pr.key);
-            if (pr.external && g.plan.dense_probes) {
+            if (mode == Mode::MOVE && g.packed(j)) {
+                const std::string gd = pr.guard ? expr(pr.guard) : std::string("true");
+                os << indent << "const int64_t " << h << " = rf_probe_pack(L.probe[" << j
+                   << "], (int64_t)" << k << ", " << gd << ");\n";
+            } else if (pr.external && g.plan.dense_probes) {
                 const std::string gd = pr.guard ? expr(pr.guard) : std::string("true");
                 os << indent << "const int64_t " << h << " = rf_probe_dense(L.probe[" << j
                    << "], (int64_t)" << k << ", " << gd << ");\n";
@@ -180,6 +191,11 @@ struct Emitter {
             int s = g.col_slot(t, c, false);
             std::string h = probe(j);
             const std::string T = cty(g.ks.cols[s].type);
+            if (mode == Mode::MOVE && g.packed(j)) {
+                const std::string v = tmp("rf_g((const " + T + "*)L.col[" + std::to_string(s) +
+                                          "])[" + h + " < 0 ? 0 : " + h + "]");
+                return tmp("(" + h + " >= 0 ? " + v + " : (" + T + ")0)");
+            }
             return tmp("(" + h + " >= 0 ? rf_g((const " + T + "*)L.col[" + std::to_string(s) +
                        "])[" + h + "] : (" + T + ")0)");
         }
@@ -736,13 +752,17 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
 }
 
 KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
-                          const std::vector<std::vector<bool>>& nullable) {
+                          const std::vector<std::vector<bool>>& nullable,
+                          const std::vector<bool>& pack_probes) {
     (void)plan;
     KernelSource ks;
     ks.entry = "rf_seg_fold";
     ks.tune.block = 256;
     if (sp.sub.probes.size() > 4) throw Unsupported("more than 4 probed tables");
     Gen g(sp.sub, nullable, ks);
+    g.pack_probes = pack_probes;
+    bool any_pack = false;
+    for (bool b : pack_probes) any_pack = any_pack || b;
     // A block folds 256 consecutive runs, a thread per run, each run's rows
     // in row order (the reference's SubVector loop). The rows the block's
     // runs cover are first copied to LDS with coalesced loads (a thread's
@@ -803,6 +823,9 @@ KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
             ct[f] = eb.expr(sp.fields[f].contrib);
         }
         g.lds_rows = false;
+        // Branch-free probes: four rows unrolled issue their reads together
+        // (the fold itself stays in row order).
+        if (any_pack) out << "#pragma unroll 4\n";
         out << "            for (int64_t row = lo; row < hi; row++) {\n" << b.str();
         for (size_t f = 0; f < sp.fields.size(); f++) {
             if (!sp.fields[f].updated) continue;

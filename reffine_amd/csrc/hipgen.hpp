@@ -75,8 +75,11 @@ KernelSource gen_group_fold(const GroupJoinPlan& gj,
                             const std::vector<std::vector<bool>>& nullable);
 
 // Per-run fold and output kernel (rf_seg_fold) of a run reduction.
+// `pack_probes[j]`: probe j's table is a packed rank map over ascending keys
+// with at least one key (the row loop then probes it without branches).
 KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
-                          const std::vector<std::vector<bool>>& nullable);
+                          const std::vector<std::vector<bool>>& nullable,
+                          const std::vector<bool>& pack_probes = {});
 
 // Picks ITEMS from the narrowest driver column the row code loads.
 Tuning default_tuning(const Plan& plan);

@@ -185,6 +185,22 @@ __device__ __forceinline__ int64_t rf_probe_dense(const RfProbeDev& h, int64_t k
     return (int64_t)rf_g(h.dense_cum)[off >> 6] + (int64_t)__popcll(wd & (bit - 1));
 }
 
+// The packed rank-map probe without branches, for a map with ascending keys
+// (no row table) and at least one key: a key outside the map reads word 0
+// and is then rejected. Straight-line code lets a fold's consecutive rows
+// issue their probes together.
+__device__ __forceinline__ int64_t rf_probe_pack(const RfProbeDev& h, int64_t key, bool guard) {
+    const uint64_t off = (uint64_t)key - (uint64_t)h.dense_base;
+    const bool in = guard && off < h.dense_len;
+    const uint64_t o = in ? off : 0;
+    uint64_t wd;
+    uint32_t below;
+    rf_rank_word(h, o >> 6, wd, below);
+    const uint64_t bit = 1ull << (o & 63);
+    const int64_t r = (int64_t)below + (int64_t)__popcll(wd & (bit - 1));
+    return (in && (wd & bit) != 0) ? r : (int64_t)-1;
+}
+
 // ------------------------------------------------------------ helpers
 
 template <typename T>
