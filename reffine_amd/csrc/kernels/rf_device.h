@@ -216,10 +216,11 @@ __device__ __forceinline__ T rf_shfl_xor(T v, int o) {
         int64_t x = __builtin_bit_cast(int64_t, v);
         x = __shfl_xor(x, o, 64);
         return __builtin_bit_cast(T, x);
+    } else if constexpr (sizeof(T) == 4) {
+        // Bits, not a value conversion: a float partial moves unchanged.
+        return __builtin_bit_cast(T, __shfl_xor(__builtin_bit_cast(int, v), o, 64));
     } else {
-        int x = (int)v;
-        x = __shfl_xor(x, o, 64);
-        return (T)x;
+        return (T)__shfl_xor((int)v, o, 64);
     }
 }
 

@@ -148,3 +148,28 @@ def test_micro_benchmark_programs(device):
     assert compile_op(rq.micro_sum())(L2) == int(lv.sum())
     with pytest.raises(ReffineError):
         compile_op(rq.micro_join(outer=True))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("det", [False, True])
+def test_float32_sum_exact(device, det):
+    """A float32 SubVector sum (micro.h:104-115's shape): the wave reduction
+    moves float partials between lanes as bits. Quarters sum exactly in any
+    order below 2^22, so the result is exact; a value conversion of the
+    partials at any butterfly level would drop their fractions."""
+    from reffine_amd.reffiner import (_add, _f32, _f32_t, _func, _get, _i64_t, _idx, _len,
+                                      _red, _subvec, _sym, _vec_t)
+    vec_in = _sym("vec_in", _vec_t(2, _i64_t, _f32_t))
+    red = _red(_subvec(vec_in, _idx(0), _len(vec_in, 1)), lambda: _f32(0.0),
+               lambda s, v: _add(s, _get(v, 0)))
+    rs = _sym("red", red)
+    fn = _func("sum_f32", rs, [vec_in])
+    fn.tbl[rs] = red
+    q = compile_op(fn, deterministic=det)
+    for n in (1, 63, 4096, 100_003):
+        x = np.full(n, 0.25, dtype=np.float32)
+        got = q(DeviceTable.from_columns([np.arange(n, dtype=np.int64), x], dim=2))
+        assert float(got) == 0.25 * n, (n, got)
+    y = np.random.default_rng(11).integers(0, 64, 50_000).astype(np.float32) * np.float32(0.75)
+    yk = np.arange(len(y), dtype=np.int64)
+    assert float(q(DeviceTable.from_columns([yk, y], dim=2))) == float(y.astype(np.float64).sum())
