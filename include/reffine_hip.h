@@ -280,12 +280,14 @@ int rf_query_free(rf_query* q);
  * launch over the list of all its units' (unit, wave-tile) items (each wave
  * a contiguous range), then per-unit completion and selection placement.
  * A group join unit runs its probe scan in the round; after the round one
- * sort and one fold launch serve all its group join units. A unit whose
- * function folds the runs of a run-end table or reads a computed vector
- * as a table runs on the plain path when it is added. Each unit's
+ * sort and one fold launch serve all its group join units. Each unit's
  * result equals the deterministic-mode run of the same query alone
- * (RF_COMPILE_DETERMINISTIC
- * queries give identical bits in both). `max_units` units per round;
+ * (RF_COMPILE_DETERMINISTIC queries give identical bits in both), with one
+ * exception: a unit whose function folds the runs of a run-end table or
+ * reads a computed vector as a table runs on the plain path, synchronously
+ * inside rf_batch_add (under rf_serve: at admission, while the serving loop
+ * waits for it), with the query's own flags, so its result is that of
+ * rf_query_run. `max_units` units per round;
  * `arena_bytes` of device memory for the units' partials and selection
  * buffers (admission by footprint: rf_batch_add returns RF_ERR_BUSY when a
  * unit does not fit); `use_graph` != 0 captures a round into a hipGraph once
@@ -309,7 +311,8 @@ int rf_batch_poll(rf_batch* b, int32_t* done);
 int rf_batch_wait(rf_batch* b);
 /* Output of a unit of a finished round, as rf_query_run gives it; a
  * selection (or group join) table is a view of the runner's arena, valid until
- * rf_batch_reset (free it with rf_table_free). */
+ * rf_batch_reset (free it with rf_table_free). A unit whose output is a
+ * table needs `out_table` (RF_ERR_INVALID without it). */
 int rf_batch_result(rf_batch* b, int32_t unit, void* output_addr,
                     rf_table** out_table);
 /* Waits for the round if needed and starts an empty one. */

@@ -232,7 +232,10 @@ bool batch_add(rf_batch* b, rf_query* q, const rf_arg* args, int64_t n_args, int
     plan_unit(u, q, args, n_args);
     const int idx = (int)b->units.size();
     if (u.plain) {
-        u.plain_out.assign(4096, 0);
+        // Room for the output struct (or the table handle an Op output writes).
+        size_t sz = sizeof(void*), al = 0;
+        if (!q->plan.output_type.is_vector()) c_layout(q->plan.output_type, sz, al);
+        u.plain_out.assign(std::max(sz, sizeof(void*)), 0);
         rf_table* t = nullptr;
         run_query(q, args, n_args, u.plain_out.data(), &t);
         u.plain_tbl.reset(t, [](rf_table* x) { table_free(x); });
@@ -585,22 +588,18 @@ void batch_result(rf_batch* b, int32_t unit, void* output_addr, rf_table** out_t
     if (unit < 0 || unit >= (int32_t)b->units.size()) throw IRError("no such unit");
     auto& u = b->units[unit];
     if (u.plain) {
-        // A view sharing the unit's output buffers (valid after the reset).
-        rf_table* t = u.plain_tbl ? new rf_table(*u.plain_tbl) : nullptr;
-        if (t && !out_table) {
-            delete t;
-            t = nullptr;
+        if (u.q->plan.output_type.is_vector()) {
+            // The unit's table stays owned by the batch (plain_tbl); the caller
+            // gets a view sharing its buffers (valid after the reset), never
+            // the internal handle.
+            if (!out_table) throw IRError("an Op output needs out_table");
+            rf_table* t = u.plain_tbl ? new rf_table(*u.plain_tbl) : nullptr;
+            *out_table = t;
+            if (output_addr) *(void**)output_addr = t;
+            return;
         }
-        if (output_addr) {
-            if (t)
-                *(void**)output_addr = t;
-            else {
-                size_t sz = 0, al = 0;
-                c_layout(u.q->plan.output_type, sz, al);  // the caller's struct size
-                std::memcpy(output_addr, u.plain_out.data(), std::min(sz, u.plain_out.size()));
-            }
-        }
-        if (out_table) *out_table = t;
+        if (output_addr) std::memcpy(output_addr, u.plain_out.data(), u.plain_out.size());
+        if (out_table) *out_table = nullptr;
         return;
     }
     if (u.build) {

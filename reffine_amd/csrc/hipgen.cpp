@@ -561,11 +561,19 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
       << "        Acc b;";
     for (size_t i = 0; i < ks.acc.size(); i++)
         o << " b.a" << i << " = rf_shfl_xor(a.a" << i << ", o);";
-    o << "\n        acc_combine(a, b);\n    }\n";    o << "    template <int CTRL>\n"
+    o << "\n        acc_combine(a, b);\n    }\n";    // XOR: the butterfly level a mirror control stands for (0 for the quad
+    // permutations, which are that xor). A float min/max field is not
+    // commutative once a NaN is involved (lanes of a quad may then differ), so
+    // it moves by the xor shuffle itself at the mirror levels.
+    o << "    template <int CTRL, int XOR>\n"
       << "    static __device__ __forceinline__ void acc_dpp(Acc& a) {\n"
       << "        Acc b;";
-    for (size_t i = 0; i < ks.acc.size(); i++)
-        o << " b.a" << i << " = rf_dpp<CTRL>(a.a" << i << ");";
+    for (size_t i = 0; i < ks.acc.size(); i++) {
+        const bool exact = ks.acc[i].type.is_float() &&
+                           (ks.acc[i].op == AggOp::MIN || ks.acc[i].op == AggOp::MAX);
+        o << " b.a" << i << " = " << (exact ? "rf_dpp_x<CTRL, XOR>" : "rf_dpp<CTRL>") << "(a.a" << i
+          << ");";
+    }
     o << "\n        acc_combine(a, b);\n    }\n";
 
     // Vector-loaded columns become parameters of row_block.

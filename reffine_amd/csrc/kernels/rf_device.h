@@ -242,6 +242,16 @@ __device__ __forceinline__ T rf_dpp(T v) {
     }
 }
 
+// A butterfly level that must pair lanes exactly as xor XOR does: the DPP
+// control for XOR == 0 (the quad permutations are xor 1 and 2), else the
+// xor shuffle (the mirrors pair the same lanes only when every lane of a
+// quad holds the same bits, which a non-commutative combine breaks).
+template <int CTRL, int XOR, typename T>
+__device__ __forceinline__ T rf_dpp_x(T v) {
+    if constexpr (XOR == 0) return rf_dpp<CTRL>(v);
+    else return rf_shfl_xor(v, XOR);
+}
+
 // Lane k's value (k wave-uniform), for every lane.
 template <typename T>
 __device__ __forceinline__ T rf_readlane(T v, int k) {
@@ -268,11 +278,13 @@ __device__ __forceinline__ void rf_wave_reduce(typename Q::Acc& a) {
     // quad permutations; then, with every lane of a quad holding the quad's
     // value, the half-row mirror pairs quads as xor 4 does, and the row
     // mirror pairs half-rows as xor 8 does -- for a commutative combine the
-    // lanes hold exactly the butterfly's values. xor 16 and 32 cross rows.
-    Q::template acc_dpp<0xB1>(a);   // quad_perm(1,0,3,2): xor 1
-    Q::template acc_dpp<0x4E>(a);   // quad_perm(2,3,0,1): xor 2
-    Q::template acc_dpp<0x141>(a);  // row_half_mirror
-    Q::template acc_dpp<0x140>(a);  // row_mirror
+    // lanes hold exactly the butterfly's values (float min/max fields, not
+    // commutative with NaNs, take xor shuffles there). xor 16 and 32 cross
+    // rows.
+    Q::template acc_dpp<0xB1, 0>(a);   // quad_perm(1,0,3,2): xor 1
+    Q::template acc_dpp<0x4E, 0>(a);   // quad_perm(2,3,0,1): xor 2
+    Q::template acc_dpp<0x141, 4>(a);  // row_half_mirror (xor 4)
+    Q::template acc_dpp<0x140, 8>(a);  // row_mirror (xor 8)
     Q::acc_shfl_xor(a, 16);
     Q::acc_shfl_xor(a, 32);
 }

@@ -369,13 +369,17 @@ struct Lowerer {
             bool a_s = a->kind == K::Sym && a->name == fname;
             bool b_s = b->kind == K::Sym && b->name == fname;
             AggOp op;
-            bool comm = true;
+            bool comm = true;  // the state may be either operand
+            bool neg = false;  // s - x folds as s + (-x)
             switch (E->op) {
                 case MathOp::ADD: op = AggOp::ADD; break;
-                case MathOp::SUB: op = AggOp::ADD; comm = false; break;
+                case MathOp::SUB: op = AggOp::ADD; comm = false; neg = true; break;
                 case MathOp::MUL: op = AggOp::MUL; break;
-                case MathOp::MAX: op = AggOp::MAX; break;
-                case MathOp::MIN: op = AggOp::MIN; break;
+                // std::max(x, s) / std::min(x, s) with the state second keep
+                // x on a NaN or a tie of zeros: a float fold of that form is
+                // not the state-first fold the kernels re-associate.
+                case MathOp::MAX: op = AggOp::MAX; comm = !E->type.is_float(); break;
+                case MathOp::MIN: op = AggOp::MIN; comm = !E->type.is_float(); break;
                 case MathOp::AND: op = AggOp::AND; break;
                 case MathOp::OR: op = AggOp::OR; break;
                 default:
@@ -384,7 +388,7 @@ struct Lowerer {
             if (a_s && !mentions_state(b)) {
                 d.upd = true;
                 d.op = op;
-                d.contrib = comm ? b : mk_nary(MathOp::NEG, {b});
+                d.contrib = neg ? mk_nary(MathOp::NEG, {b}) : b;
                 return d;
             }
             if (b_s && comm && !mentions_state(a)) {

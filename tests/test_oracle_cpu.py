@@ -104,6 +104,42 @@ def test_detorder_integers_exact_and_fp_close():
     assert brute.rel_err(got, brute.fsum(xf[3:n - 5][p[3:n - 5]])) < 1e-14
 
 
+def test_detorder_minmax_values_and_tie_order():
+    """MAX / MIN in the fixed order: the value is the max / min over the
+    selected non-NaN rows (std::max(s, x) with the state first drops a NaN
+    row); among equal values (both signs of zero) the one that wins is the
+    first row by (wave-tile, lane, row) when the tiles fit one block -- each
+    combine keeps its left operand on a tie and the left operand holds the
+    lower tiles, lanes and rows."""
+    rng = np.random.default_rng(3)
+    n = 50_000
+    x = rng.normal(size=n)
+    x[rng.random(n) < 0.1] = np.nan
+    p = rng.random(n) < 0.4
+    xs = x[p]
+    assert detorder.det_fold(x, p, 0, n, 256, 4, 4, "max") == np.nanmax(xs)
+    assert detorder.det_fold(x, p, 0, n, 256, 4, 4, "min") == np.nanmin(xs)
+    assert detorder.det_fold(x, p, 5, n - 7, 256, 2, 8, "max") == np.nanmax(
+        x[5:n - 7][p[5:n - 7]])
+    # Ties: all selected values are zeros of random sign.
+    block, items, sub = 256, 2, 4
+    tile = sub * 64 * items
+    for seed in range(6):
+        r = np.random.default_rng(seed)
+        m = 40 * tile + 77  # 41 tiles, fewer than the block's threads
+        z = np.where(r.random(m) < 0.5, 0.0, -0.0)
+        pz = r.random(m) < 0.01
+        if not pz.any():
+            continue
+        rows = np.nonzero(pz)[0]
+        tile_of = rows // tile
+        lane_of = (rows % tile) // items % 64
+        first = rows[np.lexsort((rows, lane_of, tile_of))[0]]
+        for op in ("max", "min"):
+            got = detorder.det_fold(z, pz, 0, m, block, items, sub, op)
+            assert np.signbit(got) == np.signbit(z[first])
+
+
 def test_datagen_is_a_pure_function_of_seed_col_row():
     a = datagen.uniform(3, 2, 1000)
     b = datagen.uniform(3, 2, 600, start=400)

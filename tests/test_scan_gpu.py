@@ -110,3 +110,46 @@ def test_autotune_keeps_results(device):
     assert np.array_equal(sel.column(0), sel0.column(0))
     again, t2 = autotune.autotune(fn, [t])
     assert t2 == {} and again.tuning() == best.tuning()
+
+
+def _max_query(det_init: float, use_min: bool = False):
+    """MAX (or MIN) of x over rows with d < 100, state first in the fold
+    (std::max(s, x)), over T(rowid i64, x f64, d i32), dim 2."""
+    from reffine_amd.reffiner import (_f64, _f64_t, _func, _get, _i32, _i32_t, _i64_t, _idx,
+                                      _len, _lt, _max, _min, _red, _sel, _subvec, _sym, _vec_t)
+    T = _sym("T", _vec_t(2, _i64_t, _f64_t, _i32_t))
+    f = _min if use_min else _max
+
+    def acc(s, v):
+        return _sel(_lt(_get(v, 1), _i32(100)), f(s, _get(v, 0)), s)
+
+    red = _red(_subvec(T, _idx(0), _len(T, 1)), lambda: _f64(det_init), acc)
+    rs = _sym("m", red)
+    fn = _func("maxfold", rs, [T])
+    fn.tbl[rs] = red
+    return fn
+
+
+@pytest.mark.parametrize("use_min", [False, True])
+def test_deterministic_minmax_nan_and_signed_zero(device, use_min):
+    """Float MAX / MIN folds in the deterministic mode against the fixed
+    order (butterfly levels as exact xor pairs): rows hold NaNs and both
+    signs of zero, on which std::max / std::min are not commutative."""
+    n = 300_000
+    rng = np.random.default_rng(11)
+    x = np.where(rng.random(n) < 0.5, 0.0, -0.0)
+    x = np.where(rng.random(n) < 0.05, np.nan, x)
+    x = np.where(rng.random(n) < 0.3, (-1.0 if not use_min else 1.0) * rng.random(n), x)
+    d = rng.integers(0, 200, n).astype(np.int32)
+    t = DeviceTable.from_columns([np.arange(n, dtype=np.int64), x, d], dim=2)
+    init = -5.0 if not use_min else 5.0
+    for det in (True, False):
+        q = compile_op(_max_query(init, use_min), deterministic=det)
+        got = q(t)
+        m = d < 100
+        want_val = (np.max if not use_min else np.min)(np.where(np.isnan(x[m]), -np.inf if not use_min else np.inf, x[m]))
+        assert got == want_val  # value (either sign of zero)
+        if det:
+            want = detorder.det_fold(x, m, 0, n, *q.tuning()[:3], op="min" if use_min else "max",
+                                     init=init)
+            assert np.float64(got).view(np.uint64) == np.float64(want).view(np.uint64)

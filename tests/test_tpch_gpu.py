@@ -241,6 +241,34 @@ def test_run_stage_deterministic_and_batched(device, tables):
     assert (sr.status == 0).all()
 
 
+def test_plain_batch_unit_table_output_needs_out_table(device, tables):
+    """A run-stage unit (plain path inside the batch) whose output is a
+    table: rf_batch_result with only output_addr is refused (the batch's own
+    table is never handed out), and with out_table it gives a view that
+    stays valid after the reset, freed by the caller."""
+    import ctypes
+    from reffine_amd.batch import Batch
+    from reffine_amd.runtime import lib
+    _, dev = tables
+    q18 = compile_op(tpch.q18(150))
+    want = q18(*_args(dev, "q18"))
+    b = Batch()
+    u = b.add(q18, *_args(dev, "q18"))
+    b.launch()
+    b.wait()
+    buf = ctypes.create_string_buffer(16)
+    assert lib().rf_batch_result(b.handle, u, buf, None) == 1  # RF_ERR_INVALID
+    assert b"out_table" in lib().rf_last_error()
+    tbl = ctypes.c_void_p()
+    assert lib().rf_batch_result(b.handle, u, buf, ctypes.byref(tbl)) == 0
+    assert ctypes.c_void_p.from_buffer(buf).value == tbl.value
+    b.reset()
+    view = DeviceTable(tbl.value)
+    np.testing.assert_array_equal(view.column(0), want.column(0))
+    np.testing.assert_array_equal(view.column(1), want.column(1))
+    view.free()
+
+
 def test_run_stage_empty_lineitem(device, tables):
     """No lineitem rows: no runs, empty outputs and zero counts."""
     t0 = datagen.tpch_tables(5, seed=2)
@@ -315,3 +343,53 @@ def test_q11(device):
         assert len(k) > 100
         np.testing.assert_array_equal(out.column(0), k)
         np.testing.assert_array_equal(out.column(1).view(np.uint64), v.view(np.uint64))
+
+
+def test_example_probe_misses(device):
+    """The run fold's row-keyed probe over a packed rank map (supplier keys
+    strictly ascending, with gaps): partsupp rows name suppliers below the
+    key range, past it and in interior gaps. A missing supplier's columns
+    read as 0 (its nation is then 0), folded in row order per part."""
+    t = datagen.tpch_parts(20000, seed=8)
+    sup, ps = t["supplier"], t["partsupp"]
+    n_s = len(sup["s_suppkey"])
+    i = np.arange(n_s, dtype=np.int64)
+    new_keys = 100 + 3 * i + (i % 2)  # strictly ascending, every gap >= 2
+    supp = new_keys[ps["ps_suppkey"] - 1]
+    r = np.arange(len(supp))
+    supp = np.where(r % 17 == 3, 7, supp)                      # below the range
+    supp = np.where(r % 19 == 5, new_keys[-1] + 11, supp)       # past the range
+    supp = np.where(r % 13 == 2, new_keys[(r * 7) % n_s] + 1, supp)  # interior gaps
+    assert not np.isin(supp[r % 13 == 2], new_keys).any()
+    t["supplier"] = dict(sup, s_suppkey=new_keys)
+    t["partsupp"] = dict(ps, ps_suppkey=supp)
+    rb = tpch.part_batches(t)
+    dev = {"part": DeviceTable.from_arrow(rb["part"]).build_index(),
+           "supplier": DeviceTable.from_arrow(rb["supplier"]).build_index(),
+           "partsupp": DeviceTable.from_arrow(rb["partsupp"], dim=2)}
+    srow = {int(k): j for j, k in enumerate(new_keys.tolist())}
+    nation = t["supplier"]["s_nationkey"]
+    for nation_key, size in ((0, 10), (4, 0)):
+        out = compile_op(tpch.example(nation_key, size))(
+            *[dev[a] for a in tpch.ARGS["example"]])
+        keys, vals = [], []
+        pk, qty = t["partsupp"]["ps_partkey"], t["partsupp"]["ps_availqty"]
+        psize = dict(zip(t["part"]["p_partkey"].tolist(), t["part"]["p_size"].tolist()))
+        lo = 0
+        while lo < len(pk):
+            hi = lo
+            while hi < len(pk) and pk[hi] == pk[lo]:
+                hi += 1
+            v = np.int32(0)
+            for j in range(lo, hi):
+                row = srow.get(int(supp[j]), -1)
+                nk = int(nation[row]) if row >= 0 else 0
+                if nk == nation_key:
+                    v = np.int32(v + np.int32(qty[j]))
+            if psize[int(pk[lo])] > size:
+                keys.append(int(pk[lo]))
+                vals.append(int(v))
+            lo = hi
+        assert len(keys) > 1000
+        np.testing.assert_array_equal(out.column(0), np.array(keys, np.int64))
+        np.testing.assert_array_equal(out.column(1), np.array(vals, np.int32))
