@@ -262,6 +262,28 @@ int rf_query_clear_cache(rf_query* q);
 int rf_query_run(rf_query* q, const rf_arg* args, int64_t n_args,
                  void* output_addr, rf_table** out_table);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): runs a scan query (reductions and at most one selection
+ * over one table, probes of indexed tables; not group joins, run folds or
+ * computed vectors) as rf_query_run does, with no hipMalloc, hipFree or
+ * other allocating call. The kernels' scratch (partials, counter, staging)
+ * lives in one fixed device array of RF_NOALLOC_SCRATCH_BYTES per GPU; a run
+ * that needs more fails with RF_ERR_NOMEM before launching. `output_addr`
+ * receives the reduction struct (host memory; a field holding the selection
+ * table gets NULL). The selection's columns (rf_query_output_info order:
+ * iterators then outputs) are placed into the device buffers `out_cols[j]`,
+ * each with room for `out_cap` rows; `*out_rows` receives the row count
+ * (RF_ERR_NOMEM when it exceeds out_cap; the first out_cap rows are placed).
+ * The kernel for the tables' null layout must be loaded (rf_compile loads
+ * the no-null one; a run through rf_query_run loads others) and probed
+ * tables indexed, else RF_ERR_INVALID. Calls share the device's scratch and
+ * synchronise before returning; calls that overlap on one GPU race. */
+#define RF_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int rf_query_run_noalloc(rf_query* q, const rf_arg* args, int64_t n_args,
+                         void* output_addr, void* const* out_cols, int64_t out_cap,
+                         int64_t* out_rows);
+
 /* Times `iters` back-to-back runs of the query's kernels with HIP events on
  * the library stream (no host work): `ms_per_run` is the mean device time of
  * a whole run (scan + selection gather), `ms_scan_kernel` (optional) that of

@@ -24,7 +24,7 @@ ARCH = "gfx950"
 
 HOST_SOURCES = ["ir.cpp", "lower.cpp", "hipgen.cpp", "engine.cpp", "engine_batch.cpp",
                 "arrow_device.cpp", "capi.cpp"]
-DEVICE_SOURCES = ["kernels/rf_aot.hip", "kernels/rf_group.hip"]
+DEVICE_SOURCES = ["kernels/rf_aot.hip", "kernels/rf_group.hip", "noalloc.hip"]
 EMBED = {"kRfAbiSrc": "kernels/rf_abi.h", "kRfDeviceSrc": "kernels/rf_device.h"}
 
 COMMON = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",

@@ -361,6 +361,14 @@ int rf_query_run(rf_query* q, const rf_arg* args, int64_t n_args, void* output_a
     });
 }
 
+int rf_query_run_noalloc(rf_query* q, const rf_arg* args, int64_t n_args, void* output_addr,
+                         void* const* out_cols, int64_t out_cap, int64_t* out_rows) {
+    return guard([&] {
+        need(q != nullptr, "query is NULL");
+        rf::run_query_noalloc(q, args, n_args, output_addr, out_cols, out_cap, out_rows);
+    });
+}
+
 int rf_query_time(rf_query* q, const rf_arg* args, int64_t n_args, int32_t iters,
                   double* ms_per_run, double* ms_scan_kernel) {
     return guard([&] {

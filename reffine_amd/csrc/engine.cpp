@@ -47,6 +47,7 @@ Device& init_device(int id) {
     g_dev.num_cus = prop.multiProcessorCount;
     g_dev.arch = "gfx950";
     RF_HIP(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+    noalloc_setup(id);
     g_dev.id = id;
     return g_dev;
 }
@@ -862,6 +863,11 @@ static Spec& get_run_spec(rf_query* q, const rf_arg* args, int64_t n_args) {
     return q->run_specs.emplace(sig, std::move(sp)).first->second;
 }
 
+Spec* find_spec(rf_query* q, const rf_arg* args, int64_t n_args) {
+    auto it = q->specs.find(null_signature(q->plan.inputs, args, n_args));
+    return it == q->specs.end() ? nullptr : &it->second;
+}
+
 static Spec& get_spec(rf_query* q, const rf_arg* args, int64_t n_args) {
     std::string sig = null_signature(q->plan.inputs, args, n_args);
     auto it = q->specs.find(sig);
@@ -1028,6 +1034,19 @@ Launch prepare_args(rf_query* q, const rf_arg* args, int64_t n_args) {
     return l;
 }
 
+int scan_grid(const Launch& l) {
+    // Launch width: waves_per_cu resident waves per CU (bounded by occupancy),
+    // each owning a contiguous range of wave-tiles.
+    const KernelSource& ks = l.spec->ks;
+    const Module& m = *l.spec->mod;
+    const int waves_per_block = ks.tune.block / 64;
+    const int64_t blocks_per_cu = std::max<int64_t>(
+        1, std::min<int64_t>(m.max_blocks_per_cu, ks.tune.waves_per_cu / waves_per_block));
+    int64_t grid = std::min<int64_t>((l.args.ntiles + waves_per_block - 1) / waves_per_block,
+                                     (int64_t)device().num_cus * blocks_per_cu);
+    return (int)std::max<int64_t>(grid, 1);
+}
+
 Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/) {
     Launch l = prepare_args(q, args, n_args);
     const KernelSource& ks = l.spec->ks;
@@ -1035,18 +1054,8 @@ Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* /*unused*/
     const int64_t tile = (int64_t)ks.tune.sub * 64 * ks.tune.items;
     const int64_t lo = l.lo, end = l.end;
     // Workspace.
-    Device& d = device();
-    const Module& m = *l.spec->mod;
-    // Launch width: waves_per_cu resident waves per CU (bounded by occupancy),
-    // each owning a contiguous range of wave-tiles.
-    const int waves_per_block = ks.tune.block / 64;
-    const int64_t blocks_per_cu = std::max<int64_t>(
-        1, std::min<int64_t>(m.max_blocks_per_cu, ks.tune.waves_per_cu / waves_per_block));
-    int64_t grid = std::min<int64_t>((A.ntiles + waves_per_block - 1) / waves_per_block,
-                                     (int64_t)d.num_cus * blocks_per_cu);
-    if (grid < 1) grid = 1;
-    l.grid = (int)grid;
-    const int64_t nw = grid * (ks.tune.block / 64);  // waves in the launch
+    l.grid = scan_grid(l);
+    const int64_t nw = (int64_t)l.grid * (ks.tune.block / 64);  // waves in the launch
     size_t nparts = ks.det ? (size_t)std::max<int64_t>(A.ntiles, 1) : (size_t)nw;
     size_t need = std::max<size_t>(nparts * std::max(ks.acc_bytes, 8), 256);
     if (need > q->partials_bytes) {

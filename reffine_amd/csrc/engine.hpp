@@ -219,6 +219,17 @@ struct Launch {
 
 // Prepares arguments and workspace for one run (no launch).
 Launch prepare(rf_query* q, const rf_arg* args, int64_t n_args, bool* ok);
+// The compiled scan kernel of `q` for the arguments' null layout, or null
+// when none was compiled for it (nothing is compiled here).
+Spec* find_spec(rf_query* q, const rf_arg* args, int64_t n_args);
+// Blocks of a scan launch: waves_per_cu resident waves per CU (bounded by
+// occupancy), no more than the wave-tiles need.
+int scan_grid(const Launch& l);
+// The no-allocation run (noalloc.hip): per-device scratch set up once, with
+// the device current, by init_device; then rf_query_run_noalloc's body.
+void noalloc_setup(int device);
+void run_query_noalloc(rf_query* q, const rf_arg* args, int64_t n_args, void* output_addr,
+                       void* const* out_cols, int64_t out_cap, int64_t* out_rows);
 // Arguments of one run (columns, probes, constants, rows, tiles) without
 // the query's workspace pointers.
 Launch prepare_args(rf_query* q, const rf_arg* args, int64_t n_args);

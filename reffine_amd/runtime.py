@@ -140,6 +140,10 @@ _SIGS = [
     ("rf_query_clear_cache", ctypes.c_int, [ctypes.c_void_p]),
     ("rf_query_run", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RfArg), ctypes.c_int64,
                                     ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    ("rf_query_run_noalloc", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RfArg),
+                                            ctypes.c_int64, ctypes.c_void_p,
+                                            ctypes.POINTER(ctypes.c_void_p), ctypes.c_int64,
+                                            ctypes.POINTER(ctypes.c_int64)]),
     ("rf_query_time", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RfArg), ctypes.c_int64,
                                      ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_double)]),
