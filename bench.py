@@ -279,8 +279,11 @@ def bench_c4(args) -> int:
     out, ms = _timed(cold, args.steps, args.warmup)
     _, warm_ms = _timed(lambda: q(B, P), args.steps, args.warmup)
     keys, sums, counts, _, _ = brute.group_join_c4(b, p)
+    s = out.column(1)
+    # The default mode folds groups in LDS in any order: each group's FP64 sum
+    # within 1e-12 of the sequential fold (the deterministic mode is that fold).
     ok = (np.array_equal(out.column(0), keys) and np.array_equal(out.column(2), counts) and
-          np.array_equal(out.column(1).view(np.int64), sums.view(np.int64)))
+          bool(np.all(np.abs(s - sums) <= 1e-12 * np.abs(sums))))
     run_ms, scan_ms = q.time_kernels(B, P, iters=max(args.steps, 5))
     alg = workloads.c4_bytes(nb, n, len(keys))
     joined = int(counts.sum())

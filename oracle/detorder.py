@@ -44,7 +44,8 @@ OPS = {"add": _add, "max": _max, "min": _min}
 
 def _identity(op: str, dtype):
     if op == "add":
-        return np.zeros((), dtype)
+        # -0.0 for floats: the exact additive identity the kernels fold from.
+        return np.array(-0.0 if np.issubdtype(dtype, np.floating) else 0, dtype)
     if np.issubdtype(dtype, np.floating):
         return np.array(-np.inf if op == "max" else np.inf, dtype)
     info = np.iinfo(dtype)

@@ -486,6 +486,8 @@ Module& jit_module(const KernelSource& ks) {
         RF_HIP(hipModuleGetFunction(&m->fold_b, m->mod, "rf_gj_fold_b"));
         RF_HIP(hipModuleGetFunction(&m->fold_w, m->mod, "rf_gj_fold_w"));
         RF_HIP(hipModuleGetFunction(&m->fold_wb, m->mod, "rf_gj_fold_wb"));
+        if (ks.code.find("void rf_gj_agg(") != std::string::npos)
+            RF_HIP(hipModuleGetFunction(&m->agg, m->mod, "rf_gj_agg"));
     }
     if (ks.entry == "rf_q") {
         RF_HIP(hipModuleGetFunction(&m->gather, m->mod, "rf_q_gather"));
@@ -1167,6 +1169,9 @@ GroupJoinState::~GroupJoinState() {
     dev_free(rank2);
     for (void* p : pay2) dev_free(p);
     dev_free(seg);
+    dev_free(part_small);
+    dev_free(prank);
+    for (void* p : ppay) dev_free(p);
 }
 
 GjBuild::~GjBuild() {
@@ -1479,6 +1484,99 @@ void gj_fold_args(rf_query* q, const Spec& fs, const GjBuild& b, const rf_arg* a
     fill_params(fs.ks, c, A.p);
 }
 
+// The default mode's group fold after the probe scan (no gather, no sort):
+// the staged joined rows are moved into partitions of 2^shift consecutive
+// ranks (rf_part_hist, rf_part_scatter) and a block per partition folds its
+// groups with LDS atomics and writes their outputs (rf_gj_agg). Returns
+// null (nothing launched) when the groups do not suit it: the caller then
+// sorts.
+static rf_table* group_agg(rf_query* q, const Spec& fs, const GjBuild& B, const rf_arg* args,
+                           int64_t n_args, rf_query* pq, Launch& lp) {
+    Device& d = device();
+    hipStream_t s = d.stream;
+    GroupJoinState& G = *q->gj;
+    const GroupJoinPlan& gp = *q->plan.gj;
+    const int64_t K = B.groups;
+    int nslot = 0;
+    for (auto& f : gp.fields) nslot += f.updated ? 1 : 0;
+    // 2^shift groups per partition: about 512 partitions or more (a block
+    // each), at least 16 groups per partition (LDS atomics on one slot
+    // serialise), at most 64 KiB of LDS slots and 4096 partitions.
+    int max_shift = 0;
+    while (((size_t)std::max(nslot, 1) * 8) << (max_shift + 1) <= 65536) max_shift++;
+    int shift = 4;
+    while (shift < max_shift && (K >> (shift + 1)) >= 512) shift++;
+    while (shift < max_shift && ((K + ((int64_t)1 << shift) - 1) >> shift) > 4096) shift++;
+    const int64_t nparts = std::max<int64_t>((K + ((int64_t)1 << shift) - 1) >> shift, 1);
+    const int np = (int)gp.pay_types.size();
+    size_t row = 4;
+    for (auto& t : gp.pay_types) row += (size_t)t.width();
+    // rf_part_scatter's LDS: 16 bytes per partition and a 4096-row chunk.
+    if (nparts > 4096 || np > 8 || K >= ((int64_t)1 << 31) ||
+        (size_t)nparts * 16 + 4096 * row > 160 * 1024)
+        return nullptr;
+    const RfScanArgs& A = lp.args;
+    const size_t cap = lp.empty ? 1 : (size_t)std::max<int64_t>(A.stage_rows * A.nwaves, 1);
+    const int64_t wpb = 4;  // wave regions per partition block
+    if (!lp.empty)
+        grow(G.hist, G.hist_cap, (size_t)aot_part_entries(nparts, A.nwaves, wpb) * 4);
+    if (!G.part_small) G.part_small = dev_alloc(256);
+    if (cap > G.part_cap || G.ppay.size() != (size_t)np) {
+        dev_free(G.prank);
+        G.prank = nullptr;
+        for (void* p : G.ppay) dev_free(p);
+        G.ppay.clear();
+        G.prank = dev_alloc(cap * 4);
+        for (int j = 0; j < np; j++) G.ppay.push_back(dev_alloc(cap * gp.pay_types[j].width()));
+        G.part_cap = cap;
+    }
+    // Partition starts at pstart[p * stride]: the scanned block counts, or
+    // zeros (one entry per partition) when the probe scan ran no rows.
+    const uint32_t* pstart = (const uint32_t*)G.part_small;
+    int64_t stride = 0;
+    if (!lp.empty) {
+        pstart = (const uint32_t*)G.hist;
+        stride = (A.nwaves + wpb - 1) / wpb;
+    } else {
+        dev_fill_u64(G.part_small, 0, 1);
+    }
+    if (!lp.empty) {
+        RfPartArgs P{};
+        P.rank = (const uint32_t*)pq->stages[0];
+        for (int j = 0; j < np; j++) {
+            P.pay[j] = pq->stages[1 + j];
+            P.width[j] = gp.pay_types[j].width();
+            P.ppay[j] = G.ppay[j];
+        }
+        P.np = np;
+        P.shift = shift;
+        P.nparts = nparts;
+        P.stage_rows = A.stage_rows;
+        P.nwaves = A.nwaves;
+        P.waves_per_block = wpb;
+        P.wave_cnt = A.wave_cnt;
+        P.boff = (uint32_t*)G.hist;
+        P.prank = (uint32_t*)G.prank;
+        RF_HIP(aot_partition(P, s));
+    }
+    std::unique_ptr<rf_table> out(alloc_table(gp.out_types, K, 1));
+    RfFoldArgs F;
+    gj_fold_args(q, fs, B, args, n_args, F);
+    for (int j = 0; j < np; j++) F.pay[j] = G.ppay[j];
+    for (size_t j = 0; j < out->cols.size(); j++) F.out[j] = out->cols[j].values;
+    RfAggArgs GA{};
+    GA.prank = (const uint32_t*)G.prank;
+    GA.pstart = pstart;
+    GA.stride = stride;
+    GA.shift = shift;
+    void* params[] = {&F, &GA};
+    const size_t lds = (size_t)std::max(nslot, 1) * 8 << shift;
+    RF_HIP(hipModuleLaunchKernel(fs.mod->agg, (unsigned)nparts, 1, 1, 256, 1, 1, (unsigned)lds, s,
+                                 params, nullptr));
+    RF_HIP(hipStreamSynchronize(s));
+    return out.release();
+}
+
 // Runs a group join (see GroupJoinPlan): the build side from the cache (or
 // built), then the probe scan, a stable sort of the joined rows by group
 // rank, segment bounds and the per-group fold. Two host synchronisations
@@ -1505,6 +1603,10 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
     if (tm) RF_HIP(hipEventRecord(tm->scan0, s));
     launch(pq, lp, s, true);
     if (tm) RF_HIP(hipEventRecord(tm->scan1, s));
+    if (!q->det && fs.mod->agg && lp.spec->ks.stage_types == lp.spec->ks.out_types) {
+        rf_table* t = group_agg(q, fs, *B, args, n_args, pq, lp);
+        if (t) return t;
+    }
     launch_gather(lp, s);
     RF_HIP(hipMemcpyAsync(hs, pq->out_count, 8, hipMemcpyDeviceToHost, s));
     RF_HIP(hipStreamSynchronize(s));

@@ -141,6 +141,7 @@ struct Module {
     hipFunction_t fold_b = nullptr;  // rf_gj_fold_b: folds of several runs (fold modules)
     hipFunction_t fold_w = nullptr;  // rf_gj_fold_w / _wb: a wave per group (large groups)
     hipFunction_t fold_wb = nullptr;
+    hipFunction_t agg = nullptr;     // rf_gj_agg: partitioned LDS fold (default mode)
     hipFunction_t seg_long = nullptr; // rf_seg_fold_long: run folds with long runs
     hipFunction_t seg_wave = nullptr; // rf_seg_fold_wave: a wave per run or window
     int max_blocks_per_cu = 1;
@@ -303,6 +304,13 @@ struct GroupJoinState {
     size_t join_cap = 0;
     void* seg = nullptr;                     // lo | hi per group
     size_t seg_cap = 0;
+    // Partitioned aggregation (default mode): zero partition starts for an
+    // empty probe, and the partitioned joined rows (sized to the probe
+    // scan's staging; the block counts go in `hist`).
+    void* part_small = nullptr;
+    void* prank = nullptr;
+    std::vector<void*> ppay;
+    size_t part_cap = 0;
     // Build sides kept resident, by build key (the build table's uid and the
     // build plan's scalar arguments); the oldest is dropped beyond kBuilds.
     static constexpr size_t kBuilds = 16;
@@ -335,6 +343,8 @@ hipError_t aot_rank_build(const int64_t* keys, int64_t n, int64_t base, void* bi
                           void* cum, hipStream_t s);
 hipError_t aot_gj_concat(const RfCatSeg* segs, int32_t nseg, int64_t max_rows,
                          const RfCatArgs& A, hipStream_t s);
+hipError_t aot_partition(const RfPartArgs& P, hipStream_t s);
+int64_t aot_part_entries(int64_t nparts, int64_t nwaves, int64_t waves_per_block);
 
 // Batched execution (engine_batch.cpp).
 rf_batch* batch_create(int cap, int64_t arena_bytes, bool use_graph);

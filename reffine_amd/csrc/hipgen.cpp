@@ -318,7 +318,9 @@ std::string combine_expr(AggOp op, const Type& t, const std::string& a,
 std::string fold_identity(AggOp op, const Type& t) {
     const std::string T = t.cstr();
     switch (op) {
-        case AggOp::ADD: return "(" + T + ")0";
+        // -0.0 is the exact additive identity of IEEE floats (+0.0 + -0.0 is
+        // +0.0, so a fold from +0.0 would lose a sum's -0.0).
+        case AggOp::ADD: return t.is_float() ? "(" + T + ")(-0.0)" : "(" + T + ")0";
         case AggOp::MUL: return "(" + T + ")1";
         case AggOp::AND: return "true";
         case AggOp::OR: return "false";
@@ -1064,6 +1066,17 @@ KernelSource gen_seg_fold(const SegReducePlan& sp, const Plan& plan,
     return ks;
 }
 
+bool gj_agg_ok(const GroupJoinPlan& gj) {
+    for (const GField& gf : gj.fields) {
+        if (!gf.updated) continue;
+        const bool integer = gf.type.is_int() || gf.type.is_idx();
+        if (gf.op == AggOp::ADD && (integer || gf.type.is_float())) continue;
+        if ((gf.op == AggOp::MIN || gf.op == AggOp::MAX) && integer) continue;
+        return false;
+    }
+    return gj.fields.size() <= 16;
+}
+
 KernelSource gen_group_fold(const GroupJoinPlan& gj, const std::vector<std::vector<bool>>& nullable) {
     KernelSource ks;
     ks.entry = "rf_gj_fold";
@@ -1157,9 +1170,118 @@ KernelSource gen_group_fold(const GroupJoinPlan& gj, const std::vector<std::vect
         o << "    }\n}\n";
         return o.str();
     };
+    // rf_gj_agg (the default mode, gj_agg_ok fields): a block per partition of
+    // 2^shift groups folds the partition's joined rows into LDS accumulators
+    // with LDS atomics (one 8-byte slot per group and updated field; float
+    // sums from -0.0, the exact additive identity), then evaluates each
+    // group's state as init (+) accumulated total and its outputs.
+    auto agg_fn = [&]() {
+        std::ostringstream o;
+        o << "extern \"C\" __global__ __launch_bounds__(256) void rf_gj_agg(RfFoldArgs L, RfAggArgs G) {\n"
+             "    extern __shared__ unsigned long long rf_acc[];\n"
+             "    const int64_t gpp = (int64_t)1 << G.shift;\n"
+             "    const int64_t g0 = (int64_t)blockIdx.x << G.shift;\n"
+             "    const int64_t ng = L.groups - g0 < gpp ? L.groups - g0 : gpp;\n";
+        std::vector<int> slot(gj.fields.size(), -1);
+        int nslot = 0;
+        for (size_t f = 0; f < gj.fields.size(); f++)
+            if (gj.fields[f].updated) slot[f] = nslot++;
+        auto ident_bits = [&](const GField& gf) -> std::string {
+            const std::string T = cty(gf.type);
+            if (gf.op == AggOp::ADD) {
+                if (gf.type.bt == BT::FLOAT64) return "0x8000000000000000ull";
+                if (gf.type.bt == BT::FLOAT32) return "0x80000000ull";
+                return "0ull";
+            }
+            const std::string id = fold_identity(gf.op, gf.type);
+            return gf.type.is_unsigned() ? "(unsigned long long)" + id
+                                         : "(unsigned long long)(long long)" + id;
+        };
+        o << "    for (int64_t i = threadIdx.x; i < gpp; i += 256) {\n";
+        for (size_t f = 0; f < gj.fields.size(); f++)
+            if (slot[f] >= 0)
+                o << "        rf_acc[" << slot[f] << " * gpp + i] = " << ident_bits(gj.fields[f])
+                  << ";\n";
+        o << "    }\n    __syncthreads();\n";
+        // Constant contributions (COUNT's 1) are kernel parameters.
+        std::ostringstream cst;
+        Emitter ec(g, Mode::FINAL, cst, "    ");
+        std::vector<std::string> cv(gj.fields.size());
+        for (size_t f = 0; f < gj.fields.size(); f++)
+            if (gj.fields[f].updated && gj.fields[f].const_contrib)
+                cv[f] = ec.expr(gj.fields[f].const_contrib);
+        o << cst.str();
+        o << "    const int64_t r0 = G.pstart[blockIdx.x * G.stride];\n"
+             "    const int64_t r1 = G.pstart[(blockIdx.x + 1) * G.stride];\n"
+             "    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) {\n"
+             "        const int64_t g = (int64_t)G.prank[i] - g0;\n";
+        auto val = [&](int j) {
+            return "((const " + cty(gj.pay_types[j]) + "*)L.pay[" + std::to_string(j) + "])[i]";
+        };
+        for (size_t f = 0; f < gj.fields.size(); f++) {
+            const GField& gf = gj.fields[f];
+            if (slot[f] < 0) continue;
+            const std::string T = cty(gf.type);
+            const std::string v = gf.const_contrib ? "(" + T + ")" + cv[f] : "(" + T + ")" + val(gf.pay);
+            const std::string a = "rf_acc[" + std::to_string(slot[f]) + " * gpp + g]";
+            std::string upd;
+            if (gf.op == AggOp::ADD && gf.type.bt == BT::FLOAT64)
+                upd = "atomicAdd((double*)&" + a + ", (double)" + v + ")";
+            else if (gf.op == AggOp::ADD && gf.type.bt == BT::FLOAT32)
+                upd = "atomicAdd((float*)&" + a + ", (float)" + v + ")";
+            else if (gf.op == AggOp::ADD)
+                upd = "atomicAdd(&" + a + ", (unsigned long long)(long long)" + v + ")";
+            else {
+                const char* fn = gf.op == AggOp::MIN ? "atomicMin" : "atomicMax";
+                upd = gf.type.is_unsigned()
+                          ? std::string(fn) + "(&" + a + ", (unsigned long long)" + v + ")"
+                          : std::string(fn) + "((long long*)&" + a + ", (long long)" + v + ")";
+            }
+            if (gf.pred_pay >= 0)
+                o << "        if (" << val(gf.pred_pay) << ") " << upd << ";\n";
+            else
+                o << "        " << upd << ";\n";
+        }
+        o << "    }\n    __syncthreads();\n"
+             "    for (int64_t gl = threadIdx.x; gl < ng; gl += 256) {\n"
+             "        const int64_t grp = g0 + gl;\n";
+        std::ostringstream init;
+        Emitter ei(g, Mode::FINAL, init, "        ");
+        ei.ntmp = ec.ntmp;
+        std::vector<std::string> iv;
+        for (auto& f : gj.fields) iv.push_back(ei.expr(f.init));
+        o << init.str();
+        for (size_t f = 0; f < gj.fields.size(); f++) {
+            const GField& gf = gj.fields[f];
+            const std::string T = cty(gf.type), S = "s" + std::to_string(f);
+            o << "        " << T << " " << S << " = (" << T << ")" << iv[f] << ";\n";
+            if (slot[f] < 0) continue;
+            const std::string a = "rf_acc[" + std::to_string(slot[f]) + " * gpp + gl]";
+            std::string tot;
+            if (gf.type.bt == BT::FLOAT64) tot = "((const double*)rf_acc)[" + std::to_string(slot[f]) + " * gpp + gl]";
+            else if (gf.type.bt == BT::FLOAT32) tot = "*(const float*)&" + a;
+            else tot = "(" + T + ")(long long)" + a;
+            o << "        " << S << " = " << combine_expr(gf.op, gf.type, S, tot) << ";\n";
+        }
+        o << "        const int64_t key = L.keys[grp];\n        const int64_t brow = L.brow[grp];\n"
+             "        (void)key; (void)brow;\n";
+        std::ostringstream body;
+        Emitter em(g, Mode::FINAL, body, "        ");
+        em.ntmp = ei.ntmp;
+        std::vector<std::string> ov;
+        for (auto& v : gj.out_values) ov.push_back(em.expr(v));
+        o << body.str();
+        for (size_t j = 0; j < ov.size(); j++) {
+            const std::string T = cty(gj.out_types[j]);
+            o << "        ((" << T << "*)L.out[" << j << "])[grp] = (" << T << ")" << ov[j] << ";\n";
+        }
+        o << "    }\n}\n";
+        return o.str();
+    };
     std::ostringstream o;
     o << "// Generated group fold and outputs for group join '" << gj.sym_name << "'.\n";
     o << fold_fn(false) << fold_fn(true);
+    if (gj_agg_ok(gj)) o << agg_fn();
     // A run's groups (a thread, or a wave, per group) and several runs' groups
     // in one launch (the run by binary search over cumulative group counts).
     const char* find_run =

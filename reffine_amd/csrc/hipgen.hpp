@@ -71,6 +71,9 @@ KernelSource gen_scan(const Plan& plan,
 
 // Per-group fold and output kernel (rf_gj_fold) of a group join;
 // `nullable` describes the function inputs as for gen_scan.
+// Whether a group join's fields fold with LDS atomics in the default mode
+// (rf_gj_agg): sums of integers or floats, integer min / max.
+bool gj_agg_ok(const GroupJoinPlan& gj);
 KernelSource gen_group_fold(const GroupJoinPlan& gj,
                             const std::vector<std::vector<bool>>& nullable);
 

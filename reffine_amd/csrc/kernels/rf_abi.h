@@ -130,6 +130,41 @@ struct RfCatArgs {
     int32_t np;
 };
 
+// Partitioned aggregation of a group join's joined rows (the default mode):
+// the probe scan's staged rows (per-wave regions, rank then payload
+// columns) are counted per partition (rank >> shift) and scattered into
+// contiguous partitions; rf_gj_agg then folds each partition's groups in LDS.
+struct RfPartArgs {
+    const uint32_t* rank;      // staged ranks: wave w's rows at [w*stage_rows, +wave_cnt[w])
+    const void* pay[8];        // staged payload columns, same layout
+    int32_t width[8];          // payload widths in bytes (1, 2, 4, 8)
+    int32_t np;
+    int32_t shift;             // partition of a row: rank >> shift
+    int64_t nparts;
+    int64_t stage_rows;
+    int64_t nwaves;
+    int64_t waves_per_block;   // wave regions one block handles
+    const int64_t* wave_cnt;
+    int64_t nblocks;           // blocks of the partition kernels
+    // [nparts * nblocks + 1]: rows of partition p in block b at p*nblocks + b,
+    // scanned in place into offsets (the last entry then holds the total), so
+    // partition p starts at boff[p * nblocks].
+    uint32_t* boff;
+    uint32_t* prank;           // partitioned ranks
+    void* ppay[8];             // partitioned payload columns
+};
+
+// The partition of one rf_gj_agg block p: joined rows [pstart[p * stride],
+// pstart[(p + 1) * stride]) of prank / RfFoldArgs::pay, groups
+// [p << shift, (p + 1) << shift).
+struct RfAggArgs {
+    const uint32_t* prank;
+    const uint32_t* pstart;
+    int64_t stride;
+    int32_t shift;
+    int32_t pad;
+};
+
 // Batched execution: a persistent launch over (work unit, wave-tile) items.
 struct RfBatchHdr {
     int64_t nunits;      // work units of this launch

@@ -590,6 +590,213 @@ extern "C" __global__ __launch_bounds__(256) void rf_gj_concat(const RfCatSeg* _
     }
 }
 
+// ------------------------------------------- partitioned aggregation
+//
+// The default mode's group fold does not need the joined rows in probe-row
+// order (only the deterministic mode does): the rows are counted per
+// partition of 2^shift consecutive group ranks and moved into contiguous
+// partitions, each block handling the staged rows of `waves_per_block` wave
+// regions of the probe scan; the generated rf_gj_agg then folds a partition's
+// groups in LDS. A block reserves its run of each partition with one atomic,
+// so its rows of a partition land contiguously (runs of ~rows/nparts rows).
+
+template <typename T>
+__device__ __forceinline__ void rf_move(void* dst, const void* src, int64_t d, int64_t i) {
+    ((T*)dst)[d] = ((const T*)src)[i];
+}
+
+#define RF_PART_BLOCK 1024
+#define RF_PART_U 4
+
+// The block's rows: the concatenation of its wave regions (at most 8);
+// row k of the block is element rf_part_elem(k) of the staged columns. The
+// region table lives in LDS (a private array indexed in a loop would be
+// scratch memory).
+struct RfPartRegions {
+    int64_t base[8];
+    int64_t end[8];  // cumulative row counts
+    int n;
+    int64_t total;
+};
+
+__device__ __forceinline__ void rf_part_regions(const RfPartArgs& P, RfPartRegions& R) {
+    if (threadIdx.x == 0) {
+        const int64_t w0 = (int64_t)blockIdx.x * P.waves_per_block;
+        const int64_t w1 = w0 + P.waves_per_block < P.nwaves ? w0 + P.waves_per_block : P.nwaves;
+        R.n = (int)(w1 - w0);
+        int64_t tot = 0;
+        for (int i = 0; i < R.n; i++) {
+            R.base[i] = (w0 + i) * P.stage_rows;
+            tot += P.wave_cnt[w0 + i];
+            R.end[i] = tot;
+        }
+        R.total = tot;
+    }
+}
+
+__device__ __forceinline__ int64_t rf_part_elem(const RfPartRegions& R, int64_t k) {
+    int64_t prev = 0;
+    for (int i = 0; i < R.n; i++) {
+        if (k < R.end[i]) return R.base[i] + (k - prev);
+        prev = R.end[i];
+    }
+    return -1;
+}
+
+// Rows per partition of each block: an LDS histogram per block, written to
+// the block's column of P.boff (no global atomics: many blocks adding to the
+// same few addresses serialise). Each thread issues RF_PART_U rank loads
+// before using them.
+extern "C" __global__ __launch_bounds__(RF_PART_BLOCK) void rf_part_hist(RfPartArgs P) {
+    extern __shared__ uint32_t rf_ph[];
+    const int t = threadIdx.x;
+    for (int64_t i = t; i < P.nparts; i += RF_PART_BLOCK) rf_ph[i] = 0;
+    __shared__ RfPartRegions R;
+    rf_part_regions(P, R);
+    __syncthreads();
+    for (int64_t k0 = 0; k0 < R.total; k0 += (int64_t)RF_PART_BLOCK * RF_PART_U) {
+        uint32_t r[RF_PART_U];
+#pragma unroll
+        for (int u = 0; u < RF_PART_U; u++) {
+            const int64_t k = k0 + t + (int64_t)u * RF_PART_BLOCK;
+            r[u] = k < R.total ? P.rank[rf_part_elem(R, k)] : 0xffffffffu;
+        }
+#pragma unroll
+        for (int u = 0; u < RF_PART_U; u++)
+            if (r[u] != 0xffffffffu) atomicAdd(&rf_ph[r[u] >> P.shift], 1u);
+    }
+    __syncthreads();
+    for (int64_t i = t; i < P.nparts; i += RF_PART_BLOCK)
+        P.boff[i * P.nblocks + blockIdx.x] = rf_ph[i];
+    if (blockIdx.x == 0 && t == 0) P.boff[P.nparts * P.nblocks] = 0;
+}
+
+// Moves the block's rows into their partitions, RF_PART_CHUNK rows at a
+// time staged in LDS in partition order, so that consecutive threads write
+// consecutive rows of a partition's run:
+//  * the block's run of each partition is its scanned count in P.boff;
+//  * per chunk: every row takes a slot in its partition (LDS counters), the
+//    counters are scanned into the chunk's partition starts, the rows are
+//    written to LDS in that order, and thread i writes LDS row i to its
+//    run: run[p] + rows of p placed by earlier chunks + its place in p.
+// Dynamic LDS: 4 * nparts counters and the chunk's rank + payload columns.
+#define RF_PART_CHUNK 4096
+
+#define RF_PART_PREG 2
+
+__device__ __forceinline__ uint64_t rf_part_load(const void* src, int width, int64_t e) {
+    switch (width) {
+        case 8: return ((const uint64_t*)src)[e];
+        case 4: return ((const uint32_t*)src)[e];
+        case 2: return ((const uint16_t*)src)[e];
+        default: return ((const uint8_t*)src)[e];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void rf_part_from_lds(void* dst, const unsigned char* s, int64_t d, int i) {
+    ((T*)dst)[d] = ((const T*)s)[i];
+}
+
+extern "C" __global__ __launch_bounds__(RF_PART_BLOCK) void rf_part_scatter(RfPartArgs P) {
+    extern __shared__ uint32_t rf_ps[];
+    const int64_t np_ = P.nparts;
+    uint32_t* cnt = rf_ps;              // block rows per partition, then rows placed so far
+    uint32_t* run = rf_ps + np_;        // the block's run of each partition
+    uint32_t* lc = rf_ps + 2 * np_;     // chunk rows per partition, then slots taken
+    uint32_t* ls = rf_ps + 3 * np_;     // chunk start of each partition (LDS order)
+    uint32_t* srank = rf_ps + 4 * np_;  // chunk ranks in partition order
+    unsigned char* spay[8];
+    {
+        unsigned char* q = (unsigned char*)(srank + RF_PART_CHUNK);
+        for (int j = 0; j < P.np; j++) {
+            spay[j] = q;
+            q += (size_t)RF_PART_CHUNK * P.width[j];
+        }
+    }
+    __shared__ uint64_t s_scan[RF_PART_BLOCK / 64];
+    const int t = threadIdx.x;
+    __shared__ RfPartRegions R;
+    rf_part_regions(P, R);
+    // The block's run of each partition: its scanned offset.
+    for (int64_t i = t; i < np_; i += RF_PART_BLOCK) {
+        run[i] = P.boff[i * P.nblocks + blockIdx.x];
+        cnt[i] = 0;
+    }
+    __syncthreads();
+    const int64_t per = (np_ + RF_PART_BLOCK - 1) / RF_PART_BLOCK;
+    const int64_t lo = (int64_t)t * per;
+    const int64_t hi = lo + per < np_ ? lo + per : np_;
+    uint64_t mine, off, total;
+    constexpr int CU = RF_PART_CHUNK / RF_PART_BLOCK;
+    for (int64_t k0 = 0; k0 < R.total; k0 += RF_PART_CHUNK) {
+        for (int64_t i = t; i < np_; i += RF_PART_BLOCK) lc[i] = 0;
+        __syncthreads();
+        int64_t e[CU];
+        uint32_t r[CU], slot[CU];
+        // The first RF_PART_PREG payload columns are loaded with the ranks
+        // (their latency overlaps); further ones when written to LDS.
+        uint64_t pv[CU][RF_PART_PREG];
+#pragma unroll
+        for (int u = 0; u < CU; u++) {
+            const int64_t k = k0 + t + (int64_t)u * RF_PART_BLOCK;
+            e[u] = k < R.total ? rf_part_elem(R, k) : -1;
+            r[u] = e[u] >= 0 ? P.rank[e[u]] : 0u;
+#pragma unroll
+            for (int j = 0; j < RF_PART_PREG; j++) {
+                pv[u][j] = 0;
+                if (j < P.np && e[u] >= 0) pv[u][j] = rf_part_load(P.pay[j], P.width[j], e[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CU; u++)
+            slot[u] = e[u] >= 0 ? atomicAdd(&lc[r[u] >> P.shift], 1u) : 0u;
+        __syncthreads();
+        mine = 0;
+        for (int64_t i = lo; i < hi; i++) mine += lc[i];
+        off = rf_block_excl_scan<RF_PART_BLOCK>(mine, s_scan, total);  // has barriers
+        for (int64_t i = lo; i < hi; i++) {
+            ls[i] = (uint32_t)off;
+            off += lc[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < CU; u++) {
+            if (e[u] < 0) continue;
+            const int i = (int)(ls[r[u] >> P.shift] + slot[u]);
+            srank[i] = r[u];
+            for (int j = 0; j < P.np; j++) {
+                const uint64_t v = j < RF_PART_PREG ? pv[u][j < RF_PART_PREG ? j : 0]
+                                                    : rf_part_load(P.pay[j], P.width[j], e[u]);
+                switch (P.width[j]) {
+                    case 8: ((uint64_t*)spay[j])[i] = v; break;
+                    case 4: ((uint32_t*)spay[j])[i] = (uint32_t)v; break;
+                    case 2: ((uint16_t*)spay[j])[i] = (uint16_t)v; break;
+                    default: spay[j][i] = (uint8_t)v; break;
+                }
+            }
+        }
+        __syncthreads();
+        const int n = (int)(R.total - k0 < RF_PART_CHUNK ? R.total - k0 : RF_PART_CHUNK);
+        for (int i = t; i < n; i += RF_PART_BLOCK) {
+            const uint32_t rr = srank[i];
+            const uint32_t part = rr >> P.shift;
+            const int64_t d = (int64_t)run[part] + cnt[part] + (uint32_t)(i - (int)ls[part]);
+            P.prank[d] = rr;
+            for (int j = 0; j < P.np; j++) {
+                switch (P.width[j]) {
+                    case 8: rf_part_from_lds<uint64_t>(P.ppay[j], spay[j], d, i); break;
+                    case 4: rf_part_from_lds<uint32_t>(P.ppay[j], spay[j], d, i); break;
+                    case 2: rf_part_from_lds<uint16_t>(P.ppay[j], spay[j], d, i); break;
+                    default: rf_part_from_lds<uint8_t>(P.ppay[j], spay[j], d, i); break;
+                }
+            }
+        }
+        __syncthreads();
+        for (int64_t i = t; i < np_; i += RF_PART_BLOCK) cnt[i] += lc[i];
+    }
+}
+
 // ------------------------------------------------------- host launchers
 
 namespace rf {
@@ -767,6 +974,45 @@ hipError_t aot_rank_rows(const int64_t* keys, int64_t n, int64_t base, const voi
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(rf_rank_rows, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, n, base,
                        (const uint64_t*)bits, (const uint32_t*)cum, (unsigned long long*)rows);
+    return hipGetLastError();
+}
+
+// Counts and moves a probe scan's staged joined rows into partitions of
+// 2^shift ranks (P.hist and P.cursor zeroed before; at most 8192 partitions).
+// Entries of RfPartArgs::boff (and its scan's chunk sums) for nparts
+// partitions of the joined rows of nwaves wave regions.
+int64_t aot_part_entries(int64_t nparts, int64_t nwaves, int64_t waves_per_block) {
+    const int64_t nb = (nwaves + waves_per_block - 1) / waves_per_block;
+    const int64_t m = nparts * nb + 1;
+    return m + (m + RF_SCAN_CHUNK - 1) / RF_SCAN_CHUNK;
+}
+
+hipError_t aot_partition(const RfPartArgs& P0, hipStream_t s) {
+    RfPartArgs P = P0;
+    if (P.nparts <= 0 || P.nparts > 4096 || P.nwaves <= 0) return hipErrorInvalidValue;
+    if (P.waves_per_block < 1 || P.waves_per_block > 8) return hipErrorInvalidValue;
+    const int64_t nb = (P.nwaves + P.waves_per_block - 1) / P.waves_per_block;
+    P.nblocks = nb;
+    hipLaunchKernelGGL(rf_part_hist, dim3((unsigned)nb), dim3(RF_PART_BLOCK),
+                       (size_t)P.nparts * 4, s, P);
+    // Exclusive scan of the counts, partition-major: each block's offset in
+    // each partition, and the partitions' starts.
+    const int64_t m = P.nparts * nb + 1;
+    const int64_t nch = (m + RF_SCAN_CHUNK - 1) / RF_SCAN_CHUNK;
+    uint32_t* part = P.boff + m;
+    hipLaunchKernelGGL(rf_scan_chunks, dim3((unsigned)nch), dim3(256), 0, s, P.boff, m, part);
+    hipLaunchKernelGGL(rf_sort_scan, dim3(1), dim3(1024), 0, s, part, nch);
+    hipLaunchKernelGGL(rf_scan_apply, dim3((unsigned)nch), dim3(256), 0, s, P.boff, m, part);
+    size_t row = 4;
+    for (int j = 0; j < P.np; j++) row += (size_t)P.width[j];
+    const size_t lds = (size_t)P.nparts * 16 + (size_t)RF_PART_CHUNK * row;
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)rf_part_scatter,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(rf_part_scatter, dim3((unsigned)nb), dim3(RF_PART_BLOCK), lds, s, P);
     return hipGetLastError();
 }
 
