@@ -1530,13 +1530,15 @@ static rf_table* group_agg(rf_query* q, const Spec& fs, const GjBuild& B, const 
         for (int j = 0; j < np; j++) G.ppay.push_back(dev_alloc(cap * gp.pay_types[j].width()));
         G.part_cap = cap;
     }
-    // Partition starts at pstart[p * stride]: the scanned block counts, or
-    // zeros (one entry per partition) when the probe scan ran no rows.
+    // Partition starts at pstart[p * stride]: the starts rf_part_scatter
+    // publishes, or one zero for every partition when the probe scan ran no
+    // rows.
     const uint32_t* pstart = (const uint32_t*)G.part_small;
     int64_t stride = 0;
     if (!lp.empty) {
-        pstart = (const uint32_t*)G.hist;
-        stride = (A.nwaves + wpb - 1) / wpb;
+        const int64_t nb = (A.nwaves + wpb - 1) / wpb;
+        pstart = (const uint32_t*)G.hist + nb * nparts + nparts;  // aot_partition's layout
+        stride = 1;
     } else {
         dev_fill_u64(G.part_small, 0, 1);
     }

@@ -146,10 +146,11 @@ struct RfPartArgs {
     int64_t waves_per_block;   // wave regions one block handles
     const int64_t* wave_cnt;
     int64_t nblocks;           // blocks of the partition kernels
-    // [nparts * nblocks + 1]: rows of partition p in block b at p*nblocks + b,
-    // scanned in place into offsets (the last entry then holds the total), so
-    // partition p starts at boff[p * nblocks].
+    // [nparts * nblocks]: rows of partition p in block b at p*nblocks + b,
+    // then scanned in place along each partition (b's offset in p).
     uint32_t* boff;
+    uint32_t* ptot;            // [nparts] rows per partition
+    uint32_t* pstart;          // [nparts + 1] first row of each partition
     uint32_t* prank;           // partitioned ranks
     void* ppay[8];             // partitioned payload columns
 };

@@ -668,7 +668,27 @@ extern "C" __global__ __launch_bounds__(RF_PART_BLOCK) void rf_part_hist(RfPartA
     __syncthreads();
     for (int64_t i = t; i < P.nparts; i += RF_PART_BLOCK)
         P.boff[i * P.nblocks + blockIdx.x] = rf_ph[i];
-    if (blockIdx.x == 0 && t == 0) P.boff[P.nparts * P.nblocks] = 0;
+}
+
+// Exclusive scan of one partition's block counts in place (block p), and
+// the partition's row count.
+extern "C" __global__ __launch_bounds__(256) void rf_part_rowscan(RfPartArgs P) {
+    __shared__ uint64_t s_scan[4];
+    const int64_t nb = P.nblocks;
+    uint32_t* row = P.boff + (int64_t)blockIdx.x * nb;
+    const int64_t per = (nb + 255) / 256;
+    const int64_t lo = (int64_t)threadIdx.x * per;
+    const int64_t hi = lo + per < nb ? lo + per : nb;
+    uint64_t mine = 0;
+    for (int64_t i = lo; i < hi; i++) mine += row[i];
+    uint64_t total;
+    uint64_t off = rf_block_excl_scan<256>(mine, s_scan, total);
+    for (int64_t i = lo; i < hi; i++) {
+        const uint32_t c = row[i];
+        row[i] = (uint32_t)off;
+        off += c;
+    }
+    if (threadIdx.x == 0) P.ptot[blockIdx.x] = (uint32_t)total;
 }
 
 // Moves the block's rows into their partitions, RF_PART_CHUNK rows at a
@@ -718,16 +738,23 @@ extern "C" __global__ __launch_bounds__(RF_PART_BLOCK) void rf_part_scatter(RfPa
     const int t = threadIdx.x;
     __shared__ RfPartRegions R;
     rf_part_regions(P, R);
-    // The block's run of each partition: its scanned offset.
-    for (int64_t i = t; i < np_; i += RF_PART_BLOCK) {
-        run[i] = P.boff[i * P.nblocks + blockIdx.x];
-        cnt[i] = 0;
-    }
-    __syncthreads();
+    // The block's run of each partition: the partition's start (a scan of
+    // the partition totals, thread t taking a contiguous chunk of them) plus
+    // the block's offset in it; block 0 publishes the starts for rf_gj_agg.
     const int64_t per = (np_ + RF_PART_BLOCK - 1) / RF_PART_BLOCK;
     const int64_t lo = (int64_t)t * per;
     const int64_t hi = lo + per < np_ ? lo + per : np_;
-    uint64_t mine, off, total;
+    uint64_t mine = 0, off, total;
+    for (int64_t i = lo; i < hi; i++) mine += P.ptot[i];
+    off = rf_block_excl_scan<RF_PART_BLOCK>(mine, s_scan, total);  // has barriers
+    for (int64_t i = lo; i < hi; i++) {
+        if (blockIdx.x == 0) P.pstart[i] = (uint32_t)off;
+        run[i] = (uint32_t)off + P.boff[i * P.nblocks + blockIdx.x];
+        cnt[i] = 0;
+        off += P.ptot[i];
+    }
+    if (blockIdx.x == 0 && t == 0) P.pstart[np_] = (uint32_t)total;
+    __syncthreads();
     constexpr int CU = RF_PART_CHUNK / RF_PART_BLOCK;
     for (int64_t k0 = 0; k0 < R.total; k0 += RF_PART_CHUNK) {
         for (int64_t i = t; i < np_; i += RF_PART_BLOCK) lc[i] = 0;
@@ -979,30 +1006,27 @@ hipError_t aot_rank_rows(const int64_t* keys, int64_t n, int64_t base, const voi
 
 // Counts and moves a probe scan's staged joined rows into partitions of
 // 2^shift ranks (P.hist and P.cursor zeroed before; at most 8192 partitions).
-// Entries of RfPartArgs::boff (and its scan's chunk sums) for nparts
-// partitions of the joined rows of nwaves wave regions.
+// Entries (uint32) of the block counts, partition totals and starts for
+// nparts partitions of the joined rows of nwaves wave regions: boff is at
+// 0, ptot at nparts * nblocks, pstart after it.
 int64_t aot_part_entries(int64_t nparts, int64_t nwaves, int64_t waves_per_block) {
     const int64_t nb = (nwaves + waves_per_block - 1) / waves_per_block;
-    const int64_t m = nparts * nb + 1;
-    return m + (m + RF_SCAN_CHUNK - 1) / RF_SCAN_CHUNK;
+    return nparts * nb + 2 * nparts + 1;
 }
 
 hipError_t aot_partition(const RfPartArgs& P0, hipStream_t s) {
     RfPartArgs P = P0;
+    P.nblocks = (P.nwaves + P.waves_per_block - 1) / P.waves_per_block;
+    P.ptot = P.boff + P.nparts * P.nblocks;
+    P.pstart = P.ptot + P.nparts;
     if (P.nparts <= 0 || P.nparts > 4096 || P.nwaves <= 0) return hipErrorInvalidValue;
     if (P.waves_per_block < 1 || P.waves_per_block > 8) return hipErrorInvalidValue;
     const int64_t nb = (P.nwaves + P.waves_per_block - 1) / P.waves_per_block;
     P.nblocks = nb;
     hipLaunchKernelGGL(rf_part_hist, dim3((unsigned)nb), dim3(RF_PART_BLOCK),
                        (size_t)P.nparts * 4, s, P);
-    // Exclusive scan of the counts, partition-major: each block's offset in
-    // each partition, and the partitions' starts.
-    const int64_t m = P.nparts * nb + 1;
-    const int64_t nch = (m + RF_SCAN_CHUNK - 1) / RF_SCAN_CHUNK;
-    uint32_t* part = P.boff + m;
-    hipLaunchKernelGGL(rf_scan_chunks, dim3((unsigned)nch), dim3(256), 0, s, P.boff, m, part);
-    hipLaunchKernelGGL(rf_sort_scan, dim3(1), dim3(1024), 0, s, part, nch);
-    hipLaunchKernelGGL(rf_scan_apply, dim3((unsigned)nch), dim3(256), 0, s, P.boff, m, part);
+    // Each block's offset in each partition and the partition totals.
+    hipLaunchKernelGGL(rf_part_rowscan, dim3((unsigned)P.nparts), dim3(256), 0, s, P);
     size_t row = 4;
     for (int j = 0; j < P.np; j++) row += (size_t)P.width[j];
     const size_t lds = (size_t)P.nparts * 16 + (size_t)RF_PART_CHUNK * row;
