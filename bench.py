@@ -33,10 +33,14 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic():
-    """HBM bytes per rf_q launch from the committed, copy-calibrated
-    FETCH_SIZE + WRITE_SIZE passes (profiles/scan_c2_60m_rocprof.md)."""
-    path = os.path.join(ROOT, "profiles", "scan_c2_60m_rocprof.md")
+PROFILES = {"c2": "scan_c2_60m_rocprof.md", "c3": "group_agg_c3_60m_rocprof.md",
+            "c4": "group_join_c4_rocprof.md"}
+
+
+def measured_traffic(workload: str = "c2"):
+    """HBM bytes per launch of the workload's dominant kernel from its
+    committed, copy-calibrated FETCH_SIZE + WRITE_SIZE passes (profiles/)."""
+    path = os.path.join(ROOT, "profiles", PROFILES[workload])
     try:
         with open(path) as f:
             for line in f:
@@ -78,6 +82,8 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true",
                     help="c2: leave out the batched/served measurement (4096 queries over 64 "
                          "record batches)")
+    ap.add_argument("--c2-only", action="store_true",
+                    help="default line: leave out the C3 and C4 sub-objects")
     ap.add_argument("--counter-pass", action="store_true",
                     help="untimed: run the query 3 times and a 1 GiB calibration copy 3 times "
                          "(the program profiled by the FETCH_SIZE / WRITE_SIZE passes)")
@@ -160,11 +166,23 @@ def main() -> int:
                      "traffic": measured_traffic(), "kernel": "rf_q (fused scan-filter-aggregate + staging)",
                      "kernel_ms": round(kernel_ms, 5), "device_ms_per_query": round(run_ms, 5)},
     }
+    out["roofline"]["traffic_source"] = "profiles/" + PROFILES["c2"]
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_rows)
+        from oracle import cpu_baseline as cb
+        out["cpu_baseline_c1"] = cb.time_c1()
+    del t, c, res, sel
+    if not args.c2_only:
+        # The other 60M-row configurations, each timed like C2 (same steps).
+        for name, fn in (("c3", measure_c3), ("c4", measure_c4)):
+            line = fn(args)
+            ok = ok and line["config"]["verified"]
+            out[name] = {k: line[k] for k in ("value", "unit", "ms_per_step", "config", "roofline")
+                         if k in line}
+            if "cpu_baseline" in line:
+                out[name]["cpu_baseline"] = line["cpu_baseline"]
     if not args.no_batched:
-        del t, c
         out["batched"] = c5_summary(args.queries)
         ok = ok and out["batched"]["identical_results"]
     print(json.dumps(out), flush=True)
@@ -212,6 +230,15 @@ def _line(metric_cfg: dict, value: float, ms: float, args, achieved: float, kern
 
 def bench_c3(args) -> int:
     """60M-row grouped aggregate (6 groups, TPC-H Q1 idiom)."""
+    if args.counter_pass:
+        return measure_c3(args)
+    out = measure_c3(args)
+    print(json.dumps(out), flush=True)
+    return 0 if out["config"]["verified"] else 1
+
+
+def measure_c3(args):
+    """The C3 line (or, with --counter-pass, the counter program's status)."""
     from oracle import brute
     from reffine_amd import DeviceTable, compile_op, datagen, workloads
     n = args.rows
@@ -232,11 +259,12 @@ def bench_c3(args) -> int:
                  "rows_per_s": round(n / (ms * 1e-3)), "launch_shape": list(q.tuning())},
                 alg / (ms * 1e-3) / 1e9, ms, args, alg / (kernel_ms * 1e-3) / 1e9,
                 "rf_q (fused grouped scan)", kernel_ms, run_ms)
+    out["roofline"]["traffic"] = measured_traffic("c3")
+    out["roofline"]["traffic_source"] = "profiles/" + PROFILES["c3"]
     if not args.no_cpu:
         from oracle import cpu_baseline as cb
-        out["cpu_baseline"] = cb.time_c3(min(args.cpu_rows, 10_000_000))
-    print(json.dumps(out), flush=True)
-    return 0 if ok else 1
+        out["cpu_baseline"] = cb.time_c3(min(args.cpu_rows, 5_000_000))
+    return out
 
 
 def _c3_ok(got, want) -> bool:
@@ -262,6 +290,15 @@ def _flatten(x):
 
 def bench_c4(args) -> int:
     """Hash join (1.5M build, 60M probe) + grouped aggregate (~300K groups)."""
+    if args.counter_pass:
+        return measure_c4(args)
+    out = measure_c4(args)
+    print(json.dumps(out), flush=True)
+    return 0 if out["config"]["verified"] else 1
+
+
+def measure_c4(args):
+    """The C4 line (or, with --counter-pass, the counter program's status)."""
     from oracle import brute
     from reffine_amd import DeviceTable, compile_op, datagen, workloads
     nb, n = 1_500_000, args.rows
@@ -297,11 +334,12 @@ def bench_c4(args) -> int:
                      alg / (ms * 1e-3) / 1e9, ms, args, 24 * n / (scan_ms * 1e-3) / 1e9,
                      "rf_q (probe scan: filter, hash probe, compaction of joined rows)", scan_ms,
                      run_ms)
+    out_line["roofline"]["traffic"] = measured_traffic("c4")
+    out_line["roofline"]["traffic_source"] = "profiles/" + PROFILES["c4"]
     if not args.no_cpu:
         from oracle import cpu_baseline as cb
-        out_line["cpu_baseline"] = cb.time_c4(nb, min(args.cpu_rows, 10_000_000))
-    print(json.dumps(out_line), flush=True)
-    return 0 if ok else 1
+        out_line["cpu_baseline"] = cb.time_c4(nb, min(args.cpu_rows, 5_000_000))
+    return out_line
 
 
 def _run_fold_bytes(name: str, t: dict) -> int:

@@ -49,6 +49,24 @@ def time_c2(rows: int, seed: int = 2, reps: int = 3) -> dict:
             "rows_per_s": round(rows / s)}
 
 
+def time_c1(rows: int = 1_048_576, seed: int = 42, reps: int = 7) -> dict:
+    """The C1 configuration itself: COUNT(*), SUM(v*k) WHERE 365 <= d < 730
+    AND v < 0.5 over 1,048,576 rows, through the CPU JIT restatement
+    (CEmitter-style C, clang -O3) on one thread; median of `reps` runs
+    after one warm-up, the query call alone timed (compile excluded)."""
+    from reffine_amd import datagen, workloads
+    from . import cjit
+    c = datagen.scan_table(rows, seed, with_q=False, rowid_dtype=np.int64)
+    fn = cjit.compile_op(workloads.scan_agg_c1())
+    inp = {"cols": [c["rowid"], c["k"], c["v"], c["d"]]}
+    s = _median_time(lambda: fn(inp), reps)
+    return {"value": round(20.0 * rows / s / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "ms_per_query": round(s * 1e3, 4), "rows_per_s": round(rows / s),
+            "sample": f"the whole C1 workload: {rows} rows (rowid, k i64, v f64, d i32), median of "
+                      f"{reps} runs; oracle/cjit: CEmitter-style C, clang -O3, single thread; "
+                      f"host {_host()} ({os.cpu_count()} logical CPUs)"}
+
+
 def _median_time(run, reps: int) -> float:
     run()
     times = []
