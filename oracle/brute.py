@@ -66,6 +66,17 @@ def scan_c2(v, d, q, d0=1000, vlo=0.05, vhi=0.07, qmax=24, rowid=None):
             fsum(v[m] * q[m].astype(np.float64)), sel)
 
 
+def q6_batch(c, d0, dlo, dhi, qmax):
+    """The Q6-shaped batch template (reffine_amd.workloads.q6_template) over
+    a grouped-layout batch: COUNT, SUM(qty) i64 and SUM(price*disc) f64 of
+    the rows with d0 <= d < d0+365, dlo <= disc <= dhi, qty < qmax, and their
+    ascending row ids."""
+    d, disc, qty = c["d"], c["disc"], c["qty"]
+    m = (d >= d0) & (d < d0 + 365) & (disc >= dlo) & (disc <= dhi) & (qty < qmax)
+    return (int(m.sum()), int(qty[m].sum(dtype=np.int64)), fsum(c["price"][m] * disc[m]),
+            np.asarray(c["rowid"])[m].astype(np.int64))
+
+
 def group_join_c4(build, probe, bf_eq=1, pq_gt=10):
     """Hash join + grouped aggregate, brute force. For every build key with
     bf == bf_eq, in ascending key order: SUM(pv*pq) over probe rows with

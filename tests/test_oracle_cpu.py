@@ -140,6 +140,26 @@ def test_detorder_minmax_values_and_tie_order():
             assert np.signbit(got) == np.signbit(z[first])
 
 
+def test_q6_batch_oracle_against_loops():
+    """brute.q6_batch against a row loop (every bound checked one row at a
+    time), and a planted mistake (an exclusive disc bound) is caught."""
+    c = datagen.group_table(3000, 77)
+    d0, dlo, dhi, qmax = 700, 0.03, 0.05, 30
+    cnt, sq, spd, rows = brute.q6_batch(c, d0, dlo, dhi, qmax)
+    want = []
+    for i in range(3000):
+        d, disc, q = int(c["d"][i]), float(c["disc"][i]), int(c["qty"][i])
+        if d0 <= d < d0 + 365 and dlo <= disc <= dhi and q < qmax:
+            want.append(i)
+    assert cnt == len(want) > 10
+    assert rows.tolist() == want
+    assert sq == sum(int(c["qty"][i]) for i in want)
+    assert brute.rel_err(spd, math.fsum(float(c["price"][i]) * float(c["disc"][i])
+                                        for i in want)) <= 1e-15
+    wrong = [i for i in want if float(c["disc"][i]) < dhi]
+    assert len(wrong) != cnt
+
+
 def test_datagen_is_a_pure_function_of_seed_col_row():
     a = datagen.uniform(3, 2, 1000)
     b = datagen.uniform(3, 2, 600, start=400)
