@@ -1040,6 +1040,7 @@ int scan_grid(const Launch& l) {
     // Launch width: waves_per_cu resident waves per CU (bounded by occupancy),
     // each owning a contiguous range of wave-tiles.
     const KernelSource& ks = l.spec->ks;
+    if (ks.sequential) return 1;  // one wave folds every row in order
     const Module& m = *l.spec->mod;
     const int waves_per_block = ks.tune.block / 64;
     const int64_t blocks_per_cu = std::max<int64_t>(
@@ -1135,7 +1136,10 @@ static HostVal finish_reduce(const Plan& plan, const KernelSource& ks, size_t co
     for (size_t f = 0; f < rc.fields.size(); f++) {
         const Field& fl = rc.fields[f];
         HostVal v = convert(eval(fl.init, c), fl.type);
-        if (fl.updated && !empty) {
+        if (ks.sequential && !empty) {
+            // The kernel folded from the init: its state is the result.
+            v = read_prim(fl.type, acc_bytes + ks.acc_offset[f]);
+        } else if (fl.updated && !empty) {
             for (size_t a = 0; a < ks.acc.size(); a++) {
                 if (ks.acc[a].comp == (int)comp && ks.acc[a].field == (int)f) {
                     HostVal dv = read_prim(fl.type, acc_bytes + ks.acc_offset[a]);

@@ -33,7 +33,11 @@ std::vector<std::string> split(const std::string& s, char d) {
 // one qualifying row, read by index where the placement kernel moves the
 // row (probe results the row code computed come from staging); FINAL: one
 // group of a group join (build-table columns at the group's build row).
-enum class Mode { ROWS, MOVE, FINAL };
+// ROWS: a scan's row block (columns in registers per item); MOVE: one row
+// read by index (placement, run folds); FINAL: a group's outputs; SEQ: the
+// row of a sequential fold, its columns broadcast from the lane that loaded
+// them (c<slot> / v<slot>), the state in s<field>.
+enum class Mode { ROWS, MOVE, FINAL, SEQ };
 
 struct Gen {
     const Plan& plan;
@@ -50,6 +54,8 @@ struct Gen {
     // and at least one key; their row code probes (rf_probe_pack) and reads
     // the found row (clamped to row 0, then selected) without branches.
     std::vector<bool> pack_probes;
+    // Sequential folds: the column slots (values, validity) the row reads.
+    std::set<int> seq_cols, seq_valid;
     bool packed(int j) const {
         return j < (int)pack_probes.size() && pack_probes[j];
     }
@@ -169,6 +175,10 @@ struct Emitter {
             int s = g.col_slot(t, c, mode == Mode::ROWS);
             const std::string T = cty(g.ks.cols[s].type);
             if (mode == Mode::ROWS) return "c" + std::to_string(s) + ".v[k]";
+            if (mode == Mode::SEQ) {
+                g.seq_cols.insert(s);
+                return "c" + std::to_string(s);
+            }
             if (mode == Mode::FINAL) return "rf_g((const " + T + "*)L.col[" + std::to_string(s) + "])[brow]";
             if (g.lds_rows) {
                 g.lds_slots.insert(s);
@@ -182,6 +192,10 @@ struct Emitter {
             if (!g.is_nullable(t, c)) return "true";
             int s = g.col_slot(t, c, mode == Mode::ROWS);
             if (mode == Mode::ROWS) return "(((vb" + std::to_string(s) + " >> k) & 1u) != 0u)";
+            if (mode == Mode::SEQ) {
+                g.seq_valid.insert(s);
+                return "v" + std::to_string(s);
+            }
             if (mode == Mode::FINAL) return "rf_valid_bit(L.valid[" + std::to_string(s) + "], brow)";
             return "rf_valid_bit(L.valid[" + std::to_string(s) + "], row)";
         }
@@ -210,6 +224,7 @@ struct Emitter {
         }
         if (kind == "$key") return "key";
         if (kind == "$acc") return "s" + parts[1];
+        if (kind == "$s" && mode == Mode::SEQ) return "s" + parts[1];
         if (kind == "$arg") {
             ParamSlot p{ParamSlot::ARG, e->type, 0, std::stoi(parts[1]), 0};
             return g.param(p, name + ":" + e->type.str());
@@ -471,8 +486,103 @@ Tuning default_tuning(const Plan& plan) {
     return t;
 }
 
+// A fold the scan cannot re-associate, in the reference's order
+// (src/pass/loopgen.cpp:253-262: s = pred(row) ? next(s, row) : s over rows
+// [lo, end) ascending): one wave; its 64 lanes load 64 consecutive rows'
+// columns at a time (coalesced), then every lane folds the 64 rows in order
+// from lane broadcasts, the state held identically in every lane. Lane 0
+// writes the final state into the result struct.
+static KernelSource gen_seq_scan(const Plan& plan, const std::vector<std::vector<bool>>& nullable,
+                                 bool det) {
+    KernelSource ks;
+    ks.entry = "rf_qs";
+    ks.sequential = true;
+    ks.det = det;
+    ks.tune.block = 64;
+    ks.tune.items = 1;
+    ks.tune.sub = 1;
+    ks.tune.waves_per_cu = 1;
+    Gen g(plan, nullable, ks);
+    const ReduceComp& rc = plan.reduces.at(0);
+    for (size_t f = 0; f < rc.fields.size(); f++)
+        ks.acc.push_back({0, (int)f, rc.fields[f].type, AggOp::ADD});
+    int off = 0, maxal = 1;
+    for (auto& a : ks.acc) {
+        const int w = a.type.width();
+        off = (off + w - 1) / w * w;
+        ks.acc_offset.push_back(off);
+        off += w;
+        maxal = std::max(maxal, w);
+    }
+    ks.acc_bytes = (off + maxal - 1) / maxal * maxal;
+
+    std::ostringstream init;
+    Emitter ei(g, Mode::SEQ, init, "    ");
+    std::vector<std::string> iv;
+    for (auto& f : rc.fields) iv.push_back(ei.expr(f.init));
+    std::ostringstream body;
+    Emitter em(g, Mode::SEQ, body, "            ");
+    em.ntmp = ei.ntmp;
+    const std::string pred = rc.seq_pred ? em.expr(rc.seq_pred) : std::string("true");
+    std::vector<std::string> nx;
+    for (auto& e : rc.seq_next) nx.push_back(em.expr(e));
+
+    std::ostringstream o;
+    o << "// Generated sequential fold for reffine function '" << plan.func_name << "'.\n";
+    o << "extern \"C\" __global__ __launch_bounds__(64) void rf_qs(RfScanArgs L) {\n"
+         "    const int lane = (int)threadIdx.x;\n";
+    o << init.str();
+    for (size_t f = 0; f < rc.fields.size(); f++) {
+        const std::string T = cty(rc.fields[f].type);
+        o << "    " << T << " s" << f << " = (" << T << ")" << iv[f] << ";\n";
+    }
+    o << "    for (int64_t r0 = L.lo; r0 < L.end; r0 += 64) {\n"
+         "        const int64_t rl = r0 + lane;\n"
+         "        const bool inl = rl < L.end;\n";
+    for (int sl : g.seq_cols) {
+        const std::string T = cty(ks.cols[sl].type), S = std::to_string(sl);
+        o << "        const " << T << " cl" << S << " = inl ? rf_g((const " << T << "*)L.col[" << S
+          << "])[rl] : (" << T << ")0;\n";
+    }
+    for (int sl : g.seq_valid) {
+        const std::string S = std::to_string(sl);
+        o << "        const int vl" << S << " = inl ? (int)rf_valid_bit(L.valid[" << S
+          << "], rl) : 0;\n";
+    }
+    o << "        const int n = (int)(L.end - r0 < 64 ? L.end - r0 : 64);\n"
+         "        for (int k = 0; k < n; k++) {\n"
+         "            const int64_t row = r0 + k;\n"
+         "            (void)row;\n";
+    for (int sl : g.seq_cols) {
+        const std::string T = cty(ks.cols[sl].type), S = std::to_string(sl);
+        o << "            const " << T << " c" << S << " = rf_readlane(cl" << S << ", k);\n";
+    }
+    for (int sl : g.seq_valid) {
+        const std::string S = std::to_string(sl);
+        o << "            const bool v" << S << " = rf_readlane(vl" << S << ", k) != 0;\n";
+    }
+    o << body.str();
+    o << "            if (" << pred << ") {\n";
+    for (size_t f = 0; f < nx.size(); f++) {
+        const std::string T = cty(rc.fields[f].type);
+        o << "                const " << T << " n" << f << " = (" << T << ")" << nx[f] << ";\n";
+    }
+    for (size_t f = 0; f < nx.size(); f++) o << "                s" << f << " = n" << f << ";\n";
+    o << "            }\n        }\n    }\n";
+    o << "    if (lane == 0) {\n";
+    for (size_t f = 0; f < rc.fields.size(); f++) {
+        const std::string T = cty(rc.fields[f].type);
+        o << "        *rf_gw((" << T << "*)((char*)L.result + " << ks.acc_offset[f] << ")) = s" << f
+          << ";\n";
+    }
+    o << "    }\n}\n";
+    ks.code = o.str();
+    return ks;
+}
+
 KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nullable,
                       const Tuning& tune, bool det) {
+    if (plan.sequential) return gen_seq_scan(plan, nullable, det);
     KernelSource ks;
     ks.tune = tune;
     ks.det = det;

@@ -58,6 +58,9 @@ struct KernelSource {
     std::vector<Type> stage_types; // staged per selected row: row index, carried probe results
     Tuning tune;
     bool det = false;
+    // A sequential fold (rf_qs, one wave, grid 1): the Acc fields are the
+    // final state, not partial folds to combine with the init.
+    bool sequential = false;
     // Run folds: rows per block the LDS copy holds (0: not staged), and the
     // capacity of the rf_seg_fold_long variant (0: none).
     int cap = 0;

@@ -81,8 +81,14 @@ struct Lowerer {
         const Type& vt = plan.inputs[table]->type;
         if (col < 0 || col >= (int)vt.dtypes.size())
             throw IRError("column index out of range");
-        if (col < (int)vt.enc.size() && vt.enc[col] == Enc::RUNEND)
-            throw Unsupported("run-end encoded column in a row expression");
+        if (col < (int)vt.enc.size() && vt.enc[col] == Enc::RUNEND) {
+            // ReadData of a run-end child reads its values child at the row
+            // index (src/pass/readwritepass.cpp:16-29): the value of run
+            // `row`. Rows at or past the number of runs read 0 here (the
+            // reference reads them unchecked).
+            return mk_readdata(leaf("$tab." + std::to_string(table), vt),
+                               leaf("$row", types::IDX_T), col);
+        }
         return leaf("$col." + std::to_string(table) + "." + std::to_string(col),
                     vt.dtypes[col]);
     }
@@ -96,13 +102,7 @@ struct Lowerer {
     N row_struct(int table, int first_col) {
         const Type& vt = plan.inputs[table]->type;
         std::vector<N> cols;
-        for (int c = first_col; c < (int)vt.dtypes.size(); c++) {
-            if (c < (int)vt.enc.size() && vt.enc[c] == Enc::RUNEND) {
-                cols.push_back(leaf("$ree", vt.dtypes[c]));  // poisoned lazily
-            } else {
-                cols.push_back(col_leaf(table, c));
-            }
-        }
+        for (int c = first_col; c < (int)vt.dtypes.size(); c++) cols.push_back(col_leaf(table, c));
         return mk_new(cols);
     }
 
@@ -282,9 +282,7 @@ struct Lowerer {
 
     N get_of(N v, int64_t k) {
         if (v->kind == K::New) {
-            N r = v->args.at(k);
-            if (is_leaf(r, "$ree")) throw Unsupported("run-end encoded column read per row");
-            return r;
+            return v->args.at(k);
         }
         if (v->kind == K::Select) return simplify_select(v->args[0], get_of(v->args[1], k),
                                                          get_of(v->args[2], k));
@@ -705,21 +703,34 @@ struct Lowerer {
         if (flat.size() != rc.fields.size()) throw IRError("accumulator shape");
         std::vector<N> inits;
         flatten(fold_host(init), inits);
-        for (size_t f = 0; f < rc.fields.size(); f++) {
-            Dec d = decompose(flat[f], "$s." + std::to_string(f));
-            Field& fl = rc.fields[f];
-            fl.init = inits.at(f);
-            fl.updated = d.upd;
-            if (d.upd) {
-                fl.op = d.op;
-                // vectorize: state = body_cond ? acc : state; otherwise
-                // if (body_cond) state = acc -- the same fold
-                // (loopgen.cpp:253-262).
-                fl.pred = mk_and(src.pred, d.pred);
-                if (!fl.pred) fl.pred = true_c();
-                fl.contrib = d.contrib;
-                if (mentions_state(fl.contrib) || mentions_state(fl.pred))
-                    throw Unsupported("state field depends on another field");
+        for (size_t f = 0; f < rc.fields.size(); f++) rc.fields[f].init = inits.at(f);
+        try {
+            for (size_t f = 0; f < rc.fields.size(); f++) {
+                Dec d = decompose(flat[f], "$s." + std::to_string(f));
+                Field& fl = rc.fields[f];
+                fl.updated = d.upd;
+                if (d.upd) {
+                    fl.op = d.op;
+                    // vectorize: state = body_cond ? acc : state; otherwise
+                    // if (body_cond) state = acc -- the same fold
+                    // (loopgen.cpp:253-262).
+                    fl.pred = mk_and(src.pred, d.pred);
+                    if (!fl.pred) fl.pred = true_c();
+                    fl.contrib = d.contrib;
+                    if (mentions_state(fl.contrib) || mentions_state(fl.pred))
+                        throw Unsupported("state field depends on another field");
+                }
+            }
+        } catch (const Unsupported&) {
+            // Not re-associable: the reference's own loop, row by row
+            // (loopgen.cpp:253-262), run by one wave (rf_qs).
+            rc.sequential = true;
+            rc.seq_pred = src.pred;
+            rc.seq_next = flat;
+            for (auto& fl : rc.fields) {
+                fl.updated = true;
+                fl.pred = nullptr;
+                fl.contrib = nullptr;
             }
         }
         return rc;
@@ -1343,6 +1354,13 @@ struct Lowerer {
         }
         if (staged && plan.driver != plan.runs->table)
             throw Unsupported("reductions over runs outside an Op over In(t, T)");
+        for (auto& r : plan.reduces) {
+            if (!r.sequential) continue;
+            if (plan.reduces.size() != 1 || !plan.ops.empty() || staged)
+                throw Unsupported("a fold that cannot be re-associated must be the function's "
+                                  "only component");
+            plan.sequential = true;
+        }
         plan.output = out;
         return plan;
     }
@@ -1460,7 +1478,15 @@ std::string Plan::describe() const {
         o << "  probe " << j << ": table input " << probes[j].table << " key "
           << to_text(probes[j].key) << "\n";
     for (auto& r : reduces) {
-        o << "  reduce " << r.sym_name << " -> " << r.state_type.str() << "\n";
+        o << "  reduce " << r.sym_name << " -> " << r.state_type.str()
+          << (r.sequential ? " (sequential fold, in row order)" : "") << "\n";
+        if (r.sequential) {
+            o << "    if " << (r.seq_pred ? to_text(r.seq_pred) : std::string("true")) << "\n";
+            for (size_t f = 0; f < r.seq_next.size(); f++)
+                o << "    field " << f << " " << r.fields[f].type.str() << " := "
+                  << to_text(r.seq_next[f]) << "\n";
+            continue;
+        }
         for (size_t f = 0; f < r.fields.size(); f++) {
             auto& fl = r.fields[f];
             o << "    field " << f << " " << fl.type.str();

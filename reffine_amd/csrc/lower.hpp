@@ -49,6 +49,13 @@ struct ReduceComp {
     Type state_type;
     std::vector<Field> fields;
     std::string sym_name;  // table symbol holding this reduction
+    // A fold the kernels cannot re-associate (a field's update is not
+    // `pred ? s (+) x : s`, or reads another field): the state is folded row
+    // by row in order, s = pred(row) ? next(s, row) : s, every field at once
+    // (`$s.F` leaves are the current state).
+    bool sequential = false;
+    N seq_pred;                 // row predicate (null: every row)
+    std::vector<N> seq_next;    // the next value of each flattened field
 };
 
 struct OpComp {
@@ -93,6 +100,9 @@ struct Plan {
     // the kernel calls the branch-free rank lookup; it must only be given
     // rank maps.
     bool dense_probes = false;
+    // The function is one sequential fold (ReduceComp::sequential): a
+    // single wave folds the rows in order (rf_qs).
+    bool sequential = false;
     std::shared_ptr<GroupJoinPlan> gj;  // set: the function is a group join
     std::shared_ptr<SegReducePlan> seg; // set: a reduction per run of a run-end column
     std::shared_ptr<RunStagePlan> runs; // set: a runs table is built first (T's place)

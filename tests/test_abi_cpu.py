@@ -42,16 +42,30 @@ def test_errors_are_status_codes_without_device():
     assert lib.rf_compile_check(b"(func f (sym o#1 i64) () ())", 0, buf, 256) == 2
 
 
-def test_unsupported_fold_is_rejected():
+def test_non_associative_fold_lowers_to_a_sequential_fold():
+    """s * 0.5 + v is not a fold the kernels can re-associate: it lowers to
+    the sequential fold (one wave, rows in order: the reference's loop),
+    which must be the function's only component."""
     T = _sym("T", _vec_t(2, _i64_t, _i64_t, _f64_t))
-    # s * 0.5 + v is not an associative fold of the state.
     red = _red(_subvec(T, _idx(0), _len(T, 1)), lambda: _f64(0),
                lambda s, v: _add(_mul(s, _f64(0.5)), _get(v, 1)))
     rs = _sym("r", red)
-    fn = _func("bad", rs, [T])
+    fn = _func("seq", rs, [T])
     fn.tbl[rs] = red
+    text = runtime.compile_check(fn)
+    assert "sequential fold" in text and "rf_qs" in text
+    # Beside an associative fold in one output: refused.
+    red2 = _red(_subvec(T, _idx(0), _len(T, 1)), lambda: _i64(0),
+                lambda s, v: _add(s, _get(v, 0)))
+    rs2 = _sym("r2", red2)
+    both = _new([rs, rs2])
+    bs = _sym("both", both)
+    fn2 = _func("mixed", bs, [T])
+    fn2.tbl[rs] = red
+    fn2.tbl[rs2] = red2
+    fn2.tbl[bs] = both
     with pytest.raises(runtime.ReffineError) as e:
-        runtime.compile_check(fn)
+        runtime.compile_check(fn2)
     assert e.value.status == 2
 
 

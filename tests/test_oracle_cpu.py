@@ -160,6 +160,34 @@ def test_q6_batch_oracle_against_loops():
     assert len(wrong) != cnt
 
 
+def test_cjit_sequential_folds_against_row_loops():
+    """The CPU JIT restatement folds the non-associative folds of
+    tests/seq_queries.py exactly as plain Python row loops do (bit for bit),
+    with and without null rows; a planted mistake (the decayed sum with the
+    operands of + swapped, equal in IEEE arithmetic) must agree, one with the
+    order of the last two rows swapped must not."""
+    import seq_queries as sq
+    rng = np.random.default_rng(8)
+    n = 3000
+    k = rng.integers(-1000, 1000, n).astype(np.int64)
+    v = rng.random(n)
+    for name, build in sq.BUILDERS.items():
+        vv = v.copy()
+        if name == "maxv":
+            vv[rng.random(n) < 0.05] = np.nan
+        (got,) = cjit.compile_op(build())({"cols": [np.arange(n), k, vv]})
+        want = sq.loop(name, k, vv)
+        assert [np.float64(x).view(np.uint64) if isinstance(x, float) else x for x in got] == \
+            [np.float64(x).view(np.uint64) if isinstance(x, float) else x for x in want], name
+    valid = rng.random(n) > 0.3
+    (got,) = cjit.compile_op(sq.decay())({"cols": [np.arange(n), k, v],
+                                         "valid": [None, valid, None]})
+    assert got == sq.loop("decay", k, v, valid)
+    vs = v.copy()
+    vs[[n - 2, n - 1]] = vs[[n - 1, n - 2]]
+    assert sq.loop("decay", k, vs) != sq.loop("decay", k, v)
+
+
 def test_datagen_is_a_pure_function_of_seed_col_row():
     a = datagen.uniform(3, 2, 1000)
     b = datagen.uniform(3, 2, 600, start=400)
