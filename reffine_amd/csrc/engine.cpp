@@ -489,6 +489,7 @@ Module& jit_module(const KernelSource& ks) {
         if (ks.code.find("void rf_gj_agg(") != std::string::npos)
             RF_HIP(hipModuleGetFunction(&m->agg, m->mod, "rf_gj_agg"));
     }
+    if (ks.entry == "rf_qu_count") RF_HIP(hipModuleGetFunction(&m->emit, m->mod, "rf_qu_emit"));
     if (ks.entry == "rf_q") {
         RF_HIP(hipModuleGetFunction(&m->gather, m->mod, "rf_q_gather"));
         RF_HIP(hipModuleGetFunction(&m->bscan, m->mod, "rf_qb"));
@@ -1765,6 +1766,61 @@ static rf_table* run_stage_table(rf_query* q, const rf_arg* args, int64_t n_args
     return out.release();
 }
 
+// Runs a union merge (Plan::MERGE): both key columns are checked strictly
+// ascending, the count kernel and a scan give each thread's first output row
+// and the row count (one host synchronisation), the emit kernel writes the
+// rows into a table of exactly that many rows.
+static rf_table* run_merge(rf_query* q, const rf_arg* args, int64_t n_args) {
+    Device& d = device();
+    Launch l = prepare_args(q, args, n_args);
+    const KernelSource& ks = l.spec->ks;
+    const Plan& plan = q->plan;
+    const rf_table* ta = args[plan.merge_a].u.table;
+    const rf_table* tb = args[plan.merge_b].u.table;
+    for (const rf_table* t : {ta, tb}) {
+        const Column& k = t->cols.at(0);
+        if (k.enc != Enc::FLAT || k.type.bt != BT::INT64)
+            throw ArrowError("a union merge needs a flat int64 key column");
+    }
+    const int64_t na = ta->rows(), nb = tb->rows(), M = na + nb;
+    if (M >= (int64_t(1) << 32) - RF_MERGE_IT)
+        throw Unsupported("a union merge of 2^32 rows or more");
+    if (M == 0) return alloc_table(ks.out_types, 0, 1);
+    const int64_t nthr = (M + RF_MERGE_IT - 1) / RF_MERGE_IT;
+    const int64_t entries = aot_scan_entries(nthr + 1);
+    OwnedBuffers ws;
+    char* w = (char*)dev_alloc(256 + (size_t)entries * 4);
+    ws.p.push_back(w);
+    uint32_t* flag = (uint32_t*)w;
+    uint32_t* tc = (uint32_t*)(w + 256);
+    dev_fill_u64(flag, 0, 1);
+    RF_HIP(aot_not_ascending((const int64_t*)ta->cols[0].values, na, flag, d.stream));
+    RF_HIP(aot_not_ascending((const int64_t*)tb->cols[0].values, nb, flag, d.stream));
+    l.args.partials = tc;
+    void* params[] = {&l.args};
+    const unsigned grid = (unsigned)((nthr + 1 + 255) / 256);
+    RF_HIP(hipModuleLaunchKernel(l.spec->mod->fn, grid, 1, 1, 256, 1, 1, 0, d.stream, params,
+                                 nullptr));
+    RF_HIP(aot_scan_u32(tc, nthr + 1, d.stream));
+    if (!q->host_stage) RF_HIP(hipHostMalloc(&q->host_stage, 8192, hipHostMallocDefault));
+    uint32_t* host = (uint32_t*)q->host_stage;
+    RF_HIP(hipMemcpyAsync(host, tc + nthr, 4, hipMemcpyDeviceToHost, d.stream));
+    RF_HIP(hipMemcpyAsync(host + 1, flag, 4, hipMemcpyDeviceToHost, d.stream));
+    RF_HIP(hipStreamSynchronize(d.stream));
+    if (host[1] != 0)
+        throw IRError("a union merge needs the keys of both vectors strictly ascending");
+    const int64_t total = host[0];
+    std::unique_ptr<rf_table> out(alloc_table(ks.out_types, total, 1));
+    if (total > 0) {
+        for (size_t k = 0; k < ks.out_types.size(); k++) l.args.out[k] = out->cols[k].values;
+        l.args.out_cap = total;
+        RF_HIP(hipModuleLaunchKernel(l.spec->mod->emit, grid, 1, 1, 256, 1, 1, 0, d.stream,
+                                     params, nullptr));
+        RF_HIP(hipStreamSynchronize(d.stream));
+    }
+    return out.release();
+}
+
 // Runs a query's preludes on the caller's arguments: `full` gets those
 // arguments and then each prelude's result table (owned by `mats`).
 static void run_preludes(rf_query* q, const rf_arg* args, int64_t n_args,
@@ -1818,6 +1874,12 @@ void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_add
         rf_table* t = run_group_join(q, args, n_args, nullptr);
         *out_table = t;
         if (output_addr) *(void**)output_addr = t;
+        return;
+    }
+    if (q->plan.source == Plan::MERGE) {
+        rf_table* t = run_merge(q, args, n_args);
+        finish_output(q, get_spec(q, args, n_args).ks, args, n_args, nullptr, true, t,
+                      output_addr, out_table);
         return;
     }
     Device& d = device();
@@ -1978,13 +2040,17 @@ void time_query(rf_query* q, const rf_arg* args, int64_t n_args, int iters, doub
         staged[q->plan.runs->table].u.table = runs.get();
         args = staged.data();
     }
-    if (q->plan.seg) {
+    if (q->plan.seg || q->plan.source == Plan::MERGE) {
+        // Whole runs (each synchronises).
         if (iters <= 0) return;
+        auto run = [&] {
+            return q->plan.seg ? run_seg_reduce(q, args, n_args) : run_merge(q, args, n_args);
+        };
         hipEvent_t e[2];
         for (auto& x : e) RF_HIP(hipEventCreate(&x));
-        table_free(run_seg_reduce(q, args, n_args));
+        table_free(run());
         RF_HIP(hipEventRecord(e[0], d.stream));
-        for (int i = 0; i < iters; i++) table_free(run_seg_reduce(q, args, n_args));
+        for (int i = 0; i < iters; i++) table_free(run());
         RF_HIP(hipEventRecord(e[1], d.stream));
         RF_HIP(hipEventSynchronize(e[1]));
         float a = 0;

@@ -59,6 +59,8 @@ hipError_t aot_minmax_i64(const int64_t* keys, const int64_t* n_dev, int64_t n_m
                           hipStream_t s);
 hipError_t aot_adjacent_dups(const int64_t* keys, int64_t n, void* flag, hipStream_t s);
 hipError_t aot_not_ascending(const int64_t* keys, int64_t n, void* flag, hipStream_t s);
+int64_t aot_scan_entries(int64_t m);
+hipError_t aot_scan_u32(void* h, int64_t m, hipStream_t s);
 // words x 16 bytes: word w's bits, then its count (RfProbeDev::dense_pack).
 hipError_t aot_rank_pack(const void* bits, const void* cum, int64_t words, void* pack,
                          hipStream_t s);
@@ -142,6 +144,7 @@ struct Module {
     hipFunction_t fold_w = nullptr;  // rf_gj_fold_w / _wb: a wave per group (large groups)
     hipFunction_t fold_wb = nullptr;
     hipFunction_t agg = nullptr;     // rf_gj_agg: partitioned LDS fold (default mode)
+    hipFunction_t emit = nullptr;    // rf_qu_emit: a union merge's rows (entry rf_qu_count)
     hipFunction_t seg_long = nullptr; // rf_seg_fold_long: run folds with long runs
     hipFunction_t seg_wave = nullptr; // rf_seg_fold_wave: a wave per run or window
     int max_blocks_per_cu = 1;

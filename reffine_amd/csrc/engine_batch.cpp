@@ -159,9 +159,11 @@ static void* arena_take(rf_batch* b, size_t bytes, size_t& used) {
 static void plan_unit(rf_batch::Unit& u, rf_query* q, const rf_arg* args, int64_t n_args) {
     u.q = q;
     u.args.assign(args, args + n_args);
-    if (q->plan.seg || q->plan.runs || !q->pre.empty() || q->plan.sequential) {
-        // Run folds, computed vectors and sequential folds are not batch
-        // kernels: the unit runs on the plain path when added (batch_add).
+    if (q->plan.seg || q->plan.runs || !q->pre.empty() || q->plan.sequential ||
+        q->plan.source == Plan::MERGE) {
+        // Run folds, computed vectors, sequential folds and union merges are
+        // not batch kernels: the unit runs on the plain path when added
+        // (batch_add).
         std::function<bool(const Type&)> holds_table = [&](const Type& t) {
             if (t.is_vector()) return true;
             for (auto& d : t.dtypes)

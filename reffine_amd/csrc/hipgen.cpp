@@ -1,6 +1,8 @@
 // Row-code emitter for the scan skeleton (see hipgen.hpp).
 #include "hipgen.hpp"
 
+#include "kernels/rf_abi.h"
+
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -37,7 +39,9 @@ std::vector<std::string> split(const std::string& s, char d) {
 // read by index (placement, run folds); FINAL: a group's outputs; SEQ: the
 // row of a sequential fold, its columns broadcast from the lane that loaded
 // them (c<slot> / v<slot>), the state in s<field>.
-enum class Mode { ROWS, MOVE, FINAL, SEQ };
+// MERGE: the current rows of a union merge (side 0 at row i, side 1 at row
+// j, ok0 / ok1 whether each is inside its table; key the iterator).
+enum class Mode { ROWS, MOVE, FINAL, SEQ, MERGE };
 
 struct Gen {
     const Plan& plan;
@@ -169,6 +173,25 @@ struct Emitter {
         const std::string name = leaf_name(e);
         auto parts = split(name, '.');
         const std::string& kind = parts[0];
+        if (kind == "$mcol" || kind == "$mvalid") {
+            if (mode != Mode::MERGE) throw IRError("merge leaf outside a union merge");
+            const int S = std::stoi(parts[1]), c = std::stoi(parts[2]);
+            const int t = S == 0 ? g.plan.merge_a : g.plan.merge_b;
+            const std::string I = S == 0 ? "i" : "j", OK = S == 0 ? "ok0" : "ok1";
+            if (kind == "$mvalid" && !g.is_nullable(t, c)) return OK;
+            const int sl = g.col_slot(t, c, false);
+            const std::string SL = std::to_string(sl);
+            if (kind == "$mvalid")
+                return tmp("(" + OK + " && rf_valid_bit(L.valid[" + SL + "], " + I + "))");
+            // A side past its last row reads 0.
+            const std::string T = cty(g.ks.cols[sl].type);
+            return tmp("(" + OK + " ? rf_g((const " + T + "*)L.col[" + SL + "])[" + I + "] : (" +
+                       T + ")0)");
+        }
+        if (kind == "$mkey") {
+            if (mode != Mode::MERGE) throw IRError("merge key outside a union merge");
+            return "key";
+        }
         if (kind == "$col") {
             int t = std::stoi(parts[1]), c = std::stoi(parts[2]);
             if (t != g.plan.driver) throw IRError("column leaf of a non-scanned table");
@@ -220,6 +243,7 @@ struct Emitter {
         if (kind == "$prow") return probe(std::stoi(parts[1]));
         if (kind == "$row") {
             if (mode == Mode::FINAL) throw IRError("row index in a group join output");
+            if (mode == Mode::MERGE) throw IRError("row index in a union merge");
             return "row";
         }
         if (kind == "$key") return "key";
@@ -580,8 +604,108 @@ static KernelSource gen_seq_scan(const Plan& plan, const std::vector<std::vector
     return ks;
 }
 
+// Union merge (an Op over _in(t, A) | _in(t, B)): rf_qu_count and
+// rf_qu_emit walk the same merge positions, RF_MERGE_IT per thread. Merge
+// position m is the m-th step of the merge of A's and B's keys that takes A's
+// key first on a tie; the state (i, j) before the step is the reference
+// loop's state (LoopGen::build_loop, loopgen.cpp:44-110, over UnionSpace,
+// iter_space.cpp:216-316: t = min of the two current keys, each side
+// advances when its key is <= the other's) except when A's previous key
+// equals B's current one: the reference advanced both sides at once there,
+// the merge takes B's key as a step of its own, which emits nothing. The
+// count kernel writes each thread's qualifying rows to partials[thread] (and
+// 0 at partials[threads]); after their exclusive scan the emit kernel writes
+// the rows from there, in merge order.
+static KernelSource gen_merge(const Plan& plan, const std::vector<std::vector<bool>>& nullable) {
+    if (plan.ops.size() != 1 || !plan.reduces.empty())
+        throw IRError("a union merge plan holds one Op");
+    KernelSource ks;
+    ks.entry = "rf_qu_count";
+    ks.has_sel = true;
+    ks.tune.block = 256;
+    ks.tune.items = RF_MERGE_IT;
+    ks.tune.sub = 1;
+    Gen g(plan, nullable, ks);
+    const OpComp& op = plan.ops[0];
+    const int ka = g.col_slot(plan.merge_a, 0, false), kb = g.col_slot(plan.merge_b, 0, false);
+    ParamSlot pa{ParamSlot::LEN, types::IDX_T, 0, plan.merge_a, 0};
+    ParamSlot pb{ParamSlot::LEN, types::IDX_T, 0, plan.merge_b, 0};
+    const std::string na = g.param(pa, "$len." + std::to_string(plan.merge_a) + ".0");
+    const std::string nb = g.param(pb, "$len." + std::to_string(plan.merge_b) + ".0");
+    std::ostringstream body;
+    Emitter em(g, Mode::MERGE, body, "            ");
+    const std::string pred = em.expr(op.pred);
+    std::vector<std::string> vals;
+    for (auto& v : op.values) vals.push_back(em.expr(v));
+    if (vals.size() > RF_MAXOUT) throw Unsupported("more than 16 output columns");
+    ks.out_types = op.types;
+
+    std::ostringstream o;
+    o << "// Generated union merge for reffine function '" << plan.func_name << "'.\n";
+    o << "template <bool EMIT>\n"
+         "__device__ __forceinline__ void rf_qu_body(const RfScanArgs& L) {\n"
+      << "    const int64_t na = " << na << ", nb = " << nb << ";\n"
+      << "    const int64_t* __restrict__ KA = (const int64_t*)L.col[" << ka << "];\n"
+      << "    const int64_t* __restrict__ KB = (const int64_t*)L.col[" << kb << "];\n"
+      << "    uint32_t* __restrict__ tc = (uint32_t*)L.partials;\n"
+         "    const int64_t M = na + nb;\n"
+         "    const int64_t nthr = (M + RF_MERGE_IT - 1) / RF_MERGE_IT;\n"
+         "    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n"
+         "    if (gt >= nthr) {\n"
+         "        if (!EMIT && gt == nthr) tc[nthr] = 0u;\n"
+         "        return;\n"
+         "    }\n"
+         "    const int64_t d0 = gt * RF_MERGE_IT;\n"
+         "    const int64_t d1 = d0 + RF_MERGE_IT < M ? d0 + RF_MERGE_IT : M;\n"
+         "    int64_t i = rf_merge_split(KA, na, KB, nb, d0);\n"
+         "    int64_t j = d0 - i;\n"
+         "    int64_t pa = i > 0 ? rf_g(KA)[i - 1] : 0;\n"
+         "    uint32_t cnt = 0;\n"
+         "    int64_t pos = EMIT ? (int64_t)tc[gt] : 0;\n"
+         "    (void)pos;\n"
+         "    for (int64_t m = d0; m < d1; m++) {\n"
+         "        // i + j = m < na + nb: one side at least is inside its table.\n"
+         "        const bool ok0 = i < na, ok1 = j < nb;\n"
+         "        const int64_t ka = ok0 ? rf_g(KA)[i] : 0;\n"
+         "        const int64_t kb = ok1 ? rf_g(KB)[j] : 0;\n"
+         "        const bool take0 = ok0 && (!ok1 || ka <= kb);\n"
+         "        if (!(ok1 && i > 0 && pa == kb)) {\n"
+         "            const int64_t key = take0 ? ka : kb;\n"
+         "            (void)key;\n";
+    o << body.str();
+    o << "            if (" << pred << ") {\n"
+         "                if (EMIT) {\n";
+    for (size_t k = 0; k < vals.size(); k++) {
+        const std::string T = cty(op.types[k]);
+        o << "                    ((" << T << "*)L.out[" << k << "])[pos] = (" << T << ")" << vals[k]
+          << ";\n";
+    }
+    o << "                    pos++;\n"
+         "                } else {\n"
+         "                    cnt++;\n"
+         "                }\n"
+         "            }\n"
+         "        }\n"
+         "        if (take0) {\n"
+         "            pa = ka;\n"
+         "            i++;\n"
+         "        } else {\n"
+         "            j++;\n"
+         "        }\n"
+         "    }\n"
+         "    if (!EMIT) tc[gt] = cnt;\n"
+         "}\n\n"
+         "extern \"C\" __global__ __launch_bounds__(256) void rf_qu_count(RfScanArgs L) { "
+         "rf_qu_body<false>(L); }\n"
+         "extern \"C\" __global__ __launch_bounds__(256) void rf_qu_emit(RfScanArgs L) { "
+         "rf_qu_body<true>(L); }\n";
+    ks.code = o.str();
+    return ks;
+}
+
 KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nullable,
                       const Tuning& tune, bool det) {
+    if (plan.source == Plan::MERGE) return gen_merge(plan, nullable);
     if (plan.sequential) return gen_seq_scan(plan, nullable, det);
     KernelSource ks;
     ks.tune = tune;

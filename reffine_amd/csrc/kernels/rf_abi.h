@@ -20,6 +20,8 @@ typedef __UINT64_TYPE__ uint64_t;
 #define RF_MAXPARAMS 96
 #define RF_MAXPROBES 4
 #define RF_MAXOUT 16
+// Merge positions per thread of a union merge (rf_qu_count / rf_qu_emit).
+#define RF_MERGE_IT 8
 
 #define RF_EMPTY_KEY ((int64_t)0x8000000000000000ull)
 

@@ -989,6 +989,22 @@ hipError_t aot_rank_pack(const void* bits, const void* cum, int64_t words, void*
     return hipGetLastError();
 }
 
+// Entries the exclusive scan of m uint32 counts needs (the counts and the
+// scan's chunk sums).
+int64_t aot_scan_entries(int64_t m) { return m + (m + RF_SCAN_CHUNK - 1) / RF_SCAN_CHUNK; }
+
+// Exclusive scan in place of h[0, m) (h holds aot_scan_entries(m) entries).
+hipError_t aot_scan_u32(void* h, int64_t m, hipStream_t s) {
+    if (m <= 0) return hipSuccess;
+    uint32_t* c = (uint32_t*)h;
+    const int64_t nch = (m + RF_SCAN_CHUNK - 1) / RF_SCAN_CHUNK;
+    uint32_t* part = c + m;
+    hipLaunchKernelGGL(rf_scan_chunks, dim3((unsigned)nch), dim3(256), 0, s, c, m, part);
+    hipLaunchKernelGGL(rf_sort_scan, dim3(1), dim3(1024), 0, s, part, nch);
+    hipLaunchKernelGGL(rf_scan_apply, dim3((unsigned)nch), dim3(256), 0, s, c, m, part);
+    return hipGetLastError();
+}
+
 hipError_t aot_not_ascending(const int64_t* keys, int64_t n, void* flag, hipStream_t s) {
     if (n <= 1) return hipSuccess;
     hipLaunchKernelGGL(rf_not_ascending, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, n,

@@ -51,6 +51,8 @@ static N mk_and(N a, N b) {
     return mk_nary(MathOp::AND, {a, b});
 }
 
+static N mk_or(N a, N b) { return mk_nary(MathOp::OR, {a, b}); }
+
 static N true_c() { return mk_const(types::BOOL_T, 1); }
 
 // ---------------------------------------------------------------- context
@@ -304,6 +306,18 @@ struct Lowerer {
         int t = input_index(vec);
         if (t < 0) throw Unsupported("Element of a computed vector");
         const N& it = e->args[1];
+        if (plan.source == Plan::MERGE && iter_sym && it->kind == K::Sym &&
+            it->name == iter_sym->name && (t == plan.merge_a || t == plan.merge_b)) {
+            // A side's element at the iterator reads that side's current row
+            // (JointSpace::_vec_iter_idxs, iter_space.cpp:252-262; LoopGen
+            // Element, loopgen.cpp:12-20), whatever that row's key.
+            const std::string S = t == plan.merge_a ? "0" : "1";
+            const Type& vt = plan.inputs[t]->type;
+            std::vector<N> cols;
+            for (size_t c = 1; c < vt.dtypes.size(); c++)
+                cols.push_back(leaf("$mcol." + S + "." + std::to_string(c), vt.dtypes[c]));
+            return mk_new(cols);
+        }
         if (iter_sym && it->kind == K::Sym && it->name == iter_sym->name &&
             t == driver) {
             // Iterator-bound element: direct row (loopgen.cpp:16-17).
@@ -570,6 +584,62 @@ struct Lowerer {
         return s;
     }
 
+    // Finds the conjunct _in(t, A) | _in(t, B) (t the Op's iterator, A and B
+    // two input tables keyed by int64): sets the plan's merge source, binds
+    // t to the merge's current key and removes the conjunct. Other unions of
+    // iteration spaces are refused by name.
+    bool bind_merge(const N& op, std::vector<N>& cj) {
+        int found = -1;
+        for (size_t k = 0; k < cj.size(); k++) {
+            N c = resolve_bool_sym(cj[k]);
+            if (!(c->kind == K::Nary && c->op == MathOp::OR)) continue;
+            bool has_in = false;
+            std::function<void(const N&)> f = [&](const N& x) {
+                N y = resolve_bool_sym(x);
+                if (y->kind == K::In) has_in = true;
+                if (y->kind == K::Nary && y->op == MathOp::OR)
+                    for (auto& a : y->args) f(a);
+            };
+            f(c);
+            if (!has_in) continue;  // a plain boolean filter
+            auto side = [&](const N& x) -> int {
+                N y = resolve_bool_sym(x);
+                if (y->kind != K::In || y->args[0]->kind != K::Sym ||
+                    y->args[0]->name != iter_sym->name)
+                    return -1;
+                return input_index(resolve_vec(y->args[1]));
+            };
+            const int a = side(c->args[0]), b = side(c->args[1]);
+            if (a < 0 || b < 0)
+                throw Unsupported("a union of iteration spaces other than _in(t, A) | _in(t, B) "
+                                  "over two input tables (t the Op's iterator)");
+            if (a == b) throw Unsupported("a union of a vector with itself");
+            if (found >= 0) throw Unsupported("an Op over more than one union");
+            for (int t : {a, b}) {
+                const Type& vt = plan.inputs[t]->type;
+                if (vt.dim != 1) throw Unsupported("a union merge over a multi-dimensional vector");
+                if (vt.dtypes[0].bt != BT::INT64 || iter_sym->type.bt != BT::INT64)
+                    throw Unsupported("a union merge needs int64 keys and iterator");
+                for (size_t col = 0; col < vt.enc.size(); col++)
+                    if (vt.enc[col] == Enc::RUNEND)
+                        throw Unsupported("a run-end encoded column in a union merge");
+            }
+            found = (int)k;
+            plan.merge_a = a;
+            plan.merge_b = b;
+        }
+        if (found < 0) return false;
+        if (op->iters.size() != 1) throw Unsupported("a union merge with more than one iterator");
+        cj.erase(cj.begin() + found);
+        plan.source = Plan::MERGE;
+        plan.start = mk_const(types::IDX_T, 0);
+        plan.end = mk_nary(MathOp::ADD,
+                           {leaf("$len." + std::to_string(plan.merge_a) + ".0", types::IDX_T),
+                            leaf("$len." + std::to_string(plan.merge_b) + ".0", types::IDX_T)});
+        subst[iter_sym->name] = leaf("$mkey", iter_sym->type);
+        return true;
+    }
+
     Source bind_op(const N& op) {
         if (op->iters.size() == 2) return bind_nested(op);
         if (op->iters.size() != 1)
@@ -580,24 +650,21 @@ struct Lowerer {
         // The first In(t, T) over an input table drives the scan (VecSpace,
         // iter_space.cpp:55-91); further In's are hash semi-joins
         // (InterSpace, iter_space.cpp:318-364).
-        // _in(t, A) | _in(t, B) is the reference's UnionSpace outer merge
-        // (iter_space.cpp:216-316), whose reads past the shorter vector have
-        // no defined result: refused by name.
-        for (auto& c0 : cj) {
-            N c = resolve_bool_sym(c0);
-            if (c->kind == K::Nary && c->op == MathOp::OR) {
-                bool has_in = false;
-                std::function<void(const N&)> f = [&](const N& x) {
-                    N y = resolve_bool_sym(x);
-                    if (y->kind == K::In) has_in = true;
-                    if (y->kind == K::Nary && y->op == MathOp::OR)
-                        for (auto& a : y->args) f(a);
-                };
-                f(c);
-                if (has_in)
-                    throw Unsupported("an Op over _in(t, A) | _in(t, B) (UnionSpace, the outer "
-                                      "merge join) is not supported");
-            }
+        if (bind_merge(op, cj)) {
+            // _in(t, A) | _in(t, B): UnionSpace (iter_space.cpp:216-316), the
+            // outer merge of two vectors ascending by key. The loop's body
+            // condition is UnionSpace::_iter_cond (either side's current row
+            // is in range with a valid key); further conjuncts filter
+            // (FilteredSpace over the union).
+            N pred = mk_or(leaf("$mvalid.0.0", types::BOOL_T), leaf("$mvalid.1.0", types::BOOL_T));
+            for (auto& c : cj) pred = mk_and(pred, rw(c));
+            Source s;
+            s.pred = pred;
+            std::vector<N> vals;
+            for (auto& it : op->iters) vals.push_back(rw(it));
+            for (size_t i = 1; i < op->args.size(); i++) vals.push_back(rw(op->args[i]));
+            s.row_value = mk_new(vals);
+            return s;
         }
         int drv = -1;
         for (auto& c0 : cj) {
@@ -1352,6 +1419,10 @@ struct Lowerer {
                 plan.ops.push_back(lower_op(node, name));
             }
         }
+        if (plan.source == Plan::MERGE &&
+            (!plan.reduces.empty() || plan.ops.size() != 1 || staged))
+            throw Unsupported("a union merge must be the function's only component (an Op "
+                              "output)");
         if (staged && plan.driver != plan.runs->table)
             throw Unsupported("reductions over runs outside an Op over In(t, T)");
         for (auto& r : plan.reduces) {
@@ -1471,9 +1542,14 @@ static const char* aggop_name(AggOp o) {
 
 std::string Plan::describe() const {
     std::ostringstream o;
-    o << "func " << func_name << ": scan "
-      << (source == RANGE ? "range" : "table input " + std::to_string(driver))
-      << " rows [" << to_text(start) << ", " << to_text(end) << ")\n";
+    if (source == MERGE)
+        o << "func " << func_name << ": union merge of table inputs " << merge_a << " and "
+          << merge_b << " (keys ascending), merge positions [" << to_text(start) << ", "
+          << to_text(end) << ")\n";
+    else
+        o << "func " << func_name << ": scan "
+          << (source == RANGE ? "range" : "table input " + std::to_string(driver))
+          << " rows [" << to_text(start) << ", " << to_text(end) << ")\n";
     for (size_t j = 0; j < probes.size(); j++)
         o << "  probe " << j << ": table input " << probes[j].table << " key "
           << to_text(probes[j].key) << "\n";

@@ -79,7 +79,11 @@ struct SegReducePlan;
 struct RunStagePlan;
 
 struct Plan {
-    enum Source { TABLE_ROWS, RANGE } source = TABLE_ROWS;
+    enum Source { TABLE_ROWS, RANGE, MERGE } source = TABLE_ROWS;
+    // MERGE: an Op over _in(t, A) | _in(t, B) walks the merge of the two
+    // tables' key columns (row leaves $mkey, $mcol.S.c, $mvalid.S.c with
+    // S = 0 for A, 1 for B).
+    int merge_a = -1, merge_b = -1;
     std::string func_name;
     std::vector<N> inputs;     // func inputs (syms)
     int driver = -1;           // func input index of the scanned table

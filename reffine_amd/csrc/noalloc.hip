@@ -47,10 +47,11 @@ static size_t up256(size_t x) { return (x + 255) & ~size_t(255); }
 
 void run_query_noalloc(rf_query* q, const rf_arg* args, int64_t n_args, void* output_addr,
                        void* const* out_cols, int64_t out_cap, int64_t* out_rows) {
-    if (!q->pre.empty() || q->plan.runs || q->plan.seg || q->gj)
+    if (!q->pre.empty() || q->plan.runs || q->plan.seg || q->gj ||
+        q->plan.source == Plan::MERGE)
         throw Unsupported("the no-allocation run takes scan queries only (reductions and one "
-                          "selection over one table): not group joins, run folds or computed "
-                          "vectors");
+                          "selection over one table): not group joins, run folds, union merges "
+                          "or computed vectors");
     Device& d = device();
     char* base = (char*)g_scratch[d.id];
     if (!base) throw HipError("the scratch array of this device is not set up");

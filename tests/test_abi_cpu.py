@@ -121,21 +121,28 @@ def test_run_reduction_compiles():
     assert "rf_seg_fold" in text
 
 
-def test_union_space_is_rejected():
+def test_union_space_lowers_to_a_merge():
     """An Op over _in(t, A) | _in(t, B) (the reference's UnionSpace merge
-    loop) is refused as unsupported at compile time, not lowered wrongly."""
-    from reffine_amd.reffiner import _func, _get, _i64_t, _in, _op, _sub, _sym, _vec_t
+    loop) lowers to the union merge kernels; a union of other iteration
+    spaces (a shifted iterator, AlgoTrading's _in(t - 50, B)) is refused as
+    unsupported at compile time, not lowered wrongly."""
+    import ref_queries as rq
+    from reffine_amd.reffiner import _add, _func, _get, _i64, _i64_t, _in, _op, _sub, _sym, _vec_t
+    text = runtime.compile_check(rq.micro_join(outer=True))
+    assert "union merge of table inputs 0 and 1" in text
+    assert "rf_qu_count" in text and "rf_qu_emit" in text
     t = _sym("t", _i64_t)
     A = _sym("left", _vec_t(1, _i64_t, _i64_t))
     B = _sym("right", _vec_t(1, _i64_t, _i64_t))
-    op = _op([t], _in(t, A) | _in(t, B), [_sub(_get(A[t], 0), _get(B[t], 0))])
+    t50 = _add(t, _i64(-50))
+    op = _op([t], _in(t, A) | _in(t50, B), [_sub(_get(A[t], 0), _get(B[t50], 0))])
     o = _sym("op", op)
-    fn = _func("ojoin", o, [A, B])
+    fn = _func("algotrading", o, [A, B])
     fn.tbl[o] = op
     with pytest.raises(runtime.ReffineError) as e:
         runtime.compile_check(fn)
     assert e.value.status == 2, e.value  # RF_ERR_UNSUPPORTED
-    assert "UnionSpace" in str(e.value)
+    assert "union of iteration spaces" in str(e.value)
 
 
 def test_output_struct_decoder_matches_c_layout():

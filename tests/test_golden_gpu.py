@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import ref_queries as rq
+from oracle import merge
 from reffine_amd import DeviceTable, compile_op
 
 pytestmark = pytest.mark.gpu
@@ -128,9 +129,8 @@ def test_run_reduction_random(device):
 
 def test_micro_benchmark_programs(device):
     """benchmark/include/micro.h: select (even keys), inner join (keys of
-    both tables, value difference), sum; the outer join (UnionSpace) is
-    refused with an unsupported-status error."""
-    from reffine_amd.runtime import ReffineError
+    both tables, value difference), sum, and the outer join (UnionSpace)
+    against the oracle's merge."""
     rng = np.random.default_rng(3)
     lk = np.unique(rng.integers(0, 400_000, 150_000)).astype(np.int64)
     rk = np.unique(rng.integers(0, 400_000, 150_000)).astype(np.int64)
@@ -146,8 +146,10 @@ def test_micro_benchmark_programs(device):
     np.testing.assert_array_equal(inner.column(1), lv[li] - rv[ri])
     L2 = DeviceTable.from_columns([lk, lv], dim=2)
     assert compile_op(rq.micro_sum())(L2) == int(lv.sum())
-    with pytest.raises(ReffineError):
-        compile_op(rq.micro_join(outer=True))
+    outer = compile_op(rq.micro_join(outer=True))(L, R)
+    t, d = merge.ojoin(lk, lv, rk, rv)
+    np.testing.assert_array_equal(outer.column(0), t)
+    np.testing.assert_array_equal(outer.column(1), d)
 
 
 @pytest.mark.gpu

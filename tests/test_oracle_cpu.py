@@ -7,7 +7,7 @@ import math
 import numpy as np
 import pytest
 
-from oracle import brute, cjit, detorder
+from oracle import brute, cjit, detorder, merge
 from reffine_amd import datagen, workloads
 import ref_queries as rq
 
@@ -439,3 +439,88 @@ def test_graph_vectorised_oracles_match_the_loops():
     b = datagen.bodies(48, seed=2)
     for x, y in zip(og.nbody(b, 48), og.nbody_np(b, 48)):
         np.testing.assert_array_equal(np.asarray(x).view(np.uint64), np.asarray(y).view(np.uint64))
+
+
+def _subsets(universe):
+    for mask in range(1 << len(universe)):
+        yield np.array([u for b, u in enumerate(universe) if mask >> b & 1], np.int64)
+
+
+def test_union_merge_closed_form_against_the_loop_by_brute_force():
+    """oracle/merge.py's vectorised states (sorted union of the keys, rows by
+    searchsorted) equal its row-by-row merge loop on every pair of subsets of
+    a 5-key universe, with random key validity; a loop that advances only A
+    on a tie (emitting the key twice) must disagree somewhere."""
+    rng = np.random.default_rng(11)
+    subsets = list(_subsets([-3, 0, 2, 7, 9]))
+    for ka in subsets:
+        for kb in subsets:
+            for trial in range(2):
+                va = rng.random(len(ka)) > 0.3 if trial else None
+                vb = rng.random(len(kb)) > 0.3 if trial else None
+                loop = merge.union_states_loop(ka, kb, va, vb)
+                t, i, j = merge.union_states_np(ka, kb, va, vb)
+                assert loop == list(zip(t.tolist(), i.tolist(), j.tolist())), (ka, kb, va, vb)
+
+    def tie_advances_a_only(ka, kb):
+        out, i, j = [], 0, 0
+        while i < len(ka) or j < len(kb):
+            x = ka[i] if i < len(ka) else None
+            y = kb[j] if j < len(kb) else None
+            t = x if y is None or (x is not None and x <= y) else y
+            out.append((int(t), i, j))
+            if x is not None and (y is None or x <= y):
+                i += 1
+            else:
+                j += 1
+        return out
+    bad = 0
+    for ka in subsets:
+        for kb in subsets:
+            t, i, j = merge.union_states_np(ka, kb)
+            bad += tie_advances_a_only(ka, kb) != list(zip(t.tolist(), i.tolist(), j.tolist()))
+    assert bad > 0
+
+
+def _reference_loop_defined_prefix(ka, kb):
+    """The reference's loop transcribed literally (locate through a first-
+    occurrence index, min of the current keys, lc / rc advance), stopped at
+    the first read outside either vector: the states it defines."""
+    ia = {}
+    for r, k in enumerate(ka.tolist()):
+        ia.setdefault(k, r)
+    ib = {}
+    for r, k in enumerate(kb.tolist()):
+        ib.setdefault(k, r)
+    if not len(ka) or not len(kb):
+        return []
+    lb = min(int(ka[0]), int(kb[0]))
+    i, j = ia.get(lb, -1), ib.get(lb, -1)
+    out = []
+    while i < len(ka) or j < len(kb):
+        if not (0 <= i < len(ka) and 0 <= j < len(kb)):
+            break
+        t = min(int(ka[i]), int(kb[j]))
+        out.append((t, i, j))
+        lc, rc = ka[i] <= kb[j], kb[j] <= ka[i]
+        i, j = i + lc, j + rc
+    return out
+
+
+def test_union_merge_matches_the_reference_loop_where_it_is_defined():
+    """Where the reference's loop stays inside both vectors (equal first
+    keys), its states are the first states of oracle/merge.py's loop; and the
+    micro.h case (the same table on both sides) emits every key once with
+    diff 0."""
+    rng = np.random.default_rng(12)
+    for _ in range(300):
+        u = np.sort(rng.choice(60, size=30, replace=False)).astype(np.int64)
+        ka = np.sort(np.unique(np.concatenate([[u[0]], rng.choice(u, 12)])))
+        kb = np.sort(np.unique(np.concatenate([[u[0]], rng.choice(u, 12)])))
+        ref = _reference_loop_defined_prefix(ka, kb)
+        ours = merge.union_states_loop(ka, kb)
+        assert len(ref) >= 1 and ours[:len(ref)] == ref
+    k = np.arange(0, 50, 3, dtype=np.int64)
+    v = rng.integers(-9, 9, len(k))
+    t, d = merge.ojoin(k, v, k, v)
+    assert t.tolist() == k.tolist() and not d.any()
