@@ -66,14 +66,15 @@ def main() -> int:
     ap.add_argument("--cpu-rows", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "tpch", "programs"],
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "tpch", "programs", "merge"],
                     default="c2",
                     help="c2 (default): the 60M-row scan-filter-aggregate + selection; c3: the "
                          "60M-row grouped aggregate; c4: the 1.5M x 60M hash join + grouped "
                          "aggregate; c5: 4096 queries over 64 record batches, batched/served "
                          "against plain per-query launches; tpch: TPC-H Q3/Q4/Q12/Q18 over a "
                          "run-end encoded lineitem; programs: reffine's PageRank and Nbody "
-                         "steps and TPC-DS Q9 (benchmark/include/misc.h, tpch.h)")
+                         "steps and TPC-DS Q9 (benchmark/include/misc.h, tpch.h); merge: "
+                         "micro.h's outer join (a union merge) of two 30M-key vectors")
     ap.add_argument("--orders", type=int, default=15_000_000,
                     help="tpch: orders (lineitem has about 3.75 lines per order)")
     ap.add_argument("--parts", type=int, default=2_000_000,
@@ -107,6 +108,8 @@ def main() -> int:
         return bench_tpch(args)
     if args.workload == "programs":
         return bench_programs(args)
+    if args.workload == "merge":
+        return bench_merge(args)
     n = args.rows
     log(f"generating {n} rows")
     c = datagen.scan_table(n, args.seed, with_q=True, rowid_dtype=np.int32)
@@ -536,6 +539,43 @@ def bench_programs(args) -> int:
                  per["tpcds_q9"]["kernel_ms"], run_ms)
     print(json.dumps(line), flush=True)
     return 0 if ok_all else 1
+
+
+def bench_merge(args) -> int:
+    """benchmark/include/micro.h:79-95's outer join, an Op over
+    _in(t, left) | _in(t, right) (UnionSpace): two vectors of 30M ascending
+    keys, half of right's keys also in left; verified against the oracle's
+    merge."""
+    from oracle import merge
+    from reffine_amd import DeviceTable, compile_op, workloads
+    n = 30_000_000
+    rng = np.random.default_rng(args.seed)
+    log(f"merge: two vectors of {n} keys")
+    ka = np.sort(rng.choice(4 * n, n, replace=False)).astype(np.int64)
+    kb = np.unique(np.concatenate([rng.choice(ka, n // 2, replace=False),
+                                   rng.choice(np.arange(4 * n, 6 * n), n // 2, replace=False)]))
+    va = rng.integers(-10**9, 10**9, len(ka))
+    vb = rng.integers(-10**9, 10**9, len(kb))
+    A = DeviceTable.from_columns([ka, va])
+    B = DeviceTable.from_columns([kb, vb])
+    q = compile_op(workloads.outer_join())
+    if args.counter_pass:
+        return _counter_pass(lambda: q(A, B), q)
+    out, ms = _timed(lambda: q(A, B), args.steps, args.warmup)
+    t, d = merge.ojoin(ka, va, kb, vb)
+    ok = np.array_equal(out.column(0), t) and np.array_equal(out.column(1), d)
+    run_ms, emit_ms = q.time_kernels(A, B, iters=max(args.steps, 10))
+    nbytes = workloads.merge_bytes(len(ka), len(kb), len(t))
+    # The emit kernel reads both sides' keys and values and writes the rows.
+    cfg = {"workload": "micro.h outer join (union merge) of two int64-keyed vectors",
+           "left_rows": len(ka), "right_rows": len(kb), "out_rows": int(len(t)),
+           "bytes_per_query": nbytes, "verified": bool(ok)}
+    line = _line(cfg, nbytes / (ms * 1e-3) / 1e9, ms, args, nbytes / (emit_ms * 1e-3) / 1e9,
+                 "rf_qu_emit (keys + values read, rows written: 16 B per row each)", emit_ms,
+                 run_ms)
+    line["metric"] = "union merge (outer join) throughput"
+    print(json.dumps(line), flush=True)
+    return 0 if ok else 1
 
 
 def c5_setup(n_queries: int, seed: int = 5):

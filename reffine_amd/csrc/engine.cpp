@@ -52,7 +52,7 @@ Device& init_device(int id) {
     return g_dev;
 }
 
-// Device memory. Freed blocks up to 512 MiB are kept (8 GiB in total at
+// Device memory. Freed blocks up to 2 GiB are kept (8 GiB in total at
 // most) and handed out again for the same size class: hipFree synchronises
 // the device and hipMalloc costs tens of microseconds, and a repeated query
 // allocates and frees its result table on every run. A block is freed only
@@ -63,7 +63,7 @@ std::mutex g_mem_mu;
 std::unordered_map<void*, size_t> g_live;  // block -> size class
 std::multimap<size_t, void*> g_cache;      // free blocks by size class
 size_t g_cached = 0;
-constexpr size_t kCacheBlockMax = size_t(512) << 20;
+constexpr size_t kCacheBlockMax = size_t(2) << 30;
 constexpr size_t kCacheMax = size_t(8) << 30;
 
 size_t size_class(size_t b) {
@@ -1766,11 +1766,14 @@ static rf_table* run_stage_table(rf_query* q, const rf_arg* args, int64_t n_args
     return out.release();
 }
 
-// Runs a union merge (Plan::MERGE): both key columns are checked strictly
-// ascending, the count kernel and a scan give each thread's first output row
-// and the row count (one host synchronisation), the emit kernel writes the
-// rows into a table of exactly that many rows.
-static rf_table* run_merge(rf_query* q, const rf_arg* args, int64_t n_args) {
+// Runs a union merge (Plan::MERGE): merge-path splits per tile, the count
+// kernel (which also checks both key columns strictly ascending) and a scan
+// of its per-tile counts give each tile's first output row and the row count
+// (one host synchronisation); the emit kernel writes the rows into a table
+// of exactly that many rows.
+// ev: when set, events recorded around the emit kernel (timing).
+static rf_table* run_merge(rf_query* q, const rf_arg* args, int64_t n_args,
+                           hipEvent_t* ev = nullptr) {
     Device& d = device();
     Launch l = prepare_args(q, args, n_args);
     const KernelSource& ks = l.spec->ks;
@@ -1783,28 +1786,32 @@ static rf_table* run_merge(rf_query* q, const rf_arg* args, int64_t n_args) {
             throw ArrowError("a union merge needs a flat int64 key column");
     }
     const int64_t na = ta->rows(), nb = tb->rows(), M = na + nb;
-    if (M >= (int64_t(1) << 32) - RF_MERGE_IT)
+    if (M >= (int64_t(1) << 32) - RF_MERGE_TILE)
         throw Unsupported("a union merge of 2^32 rows or more");
     if (M == 0) return alloc_table(ks.out_types, 0, 1);
-    const int64_t nthr = (M + RF_MERGE_IT - 1) / RF_MERGE_IT;
-    const int64_t entries = aot_scan_entries(nthr + 1);
+    const int64_t tiles = (M + RF_MERGE_TILE - 1) / RF_MERGE_TILE;
+    const int64_t entries = aot_scan_entries(tiles + 1);
     OwnedBuffers ws;
-    char* w = (char*)dev_alloc(256 + (size_t)entries * 4);
+    const size_t split_bytes = ((size_t)(tiles + 1) * 8 + 255) & ~size_t(255);
+    char* w = (char*)dev_alloc(256 + split_bytes + (size_t)entries * 4);
     ws.p.push_back(w);
     uint32_t* flag = (uint32_t*)w;
-    uint32_t* tc = (uint32_t*)(w + 256);
+    int64_t* splits = (int64_t*)(w + 256);
+    uint32_t* bcnt = (uint32_t*)(w + 256 + split_bytes);
+    const int64_t* ka = (const int64_t*)ta->cols[0].values;
+    const int64_t* kb = (const int64_t*)tb->cols[0].values;
     dev_fill_u64(flag, 0, 1);
-    RF_HIP(aot_not_ascending((const int64_t*)ta->cols[0].values, na, flag, d.stream));
-    RF_HIP(aot_not_ascending((const int64_t*)tb->cols[0].values, nb, flag, d.stream));
-    l.args.partials = tc;
+    RF_HIP(aot_merge_splits(ka, na, kb, nb, RF_MERGE_TILE, tiles + 1, splits, d.stream));
+    l.args.partials = bcnt;
+    l.args.counters = flag;
+    l.args.wave_dst = splits;
     void* params[] = {&l.args};
-    const unsigned grid = (unsigned)((nthr + 1 + 255) / 256);
-    RF_HIP(hipModuleLaunchKernel(l.spec->mod->fn, grid, 1, 1, 256, 1, 1, 0, d.stream, params,
-                                 nullptr));
-    RF_HIP(aot_scan_u32(tc, nthr + 1, d.stream));
+    RF_HIP(hipModuleLaunchKernel(l.spec->mod->fn, (unsigned)(tiles + 1), 1, 1, RF_MERGE_BLOCK, 1,
+                                 1, 0, d.stream, params, nullptr));
+    RF_HIP(aot_scan_u32(bcnt, tiles + 1, d.stream));
     if (!q->host_stage) RF_HIP(hipHostMalloc(&q->host_stage, 8192, hipHostMallocDefault));
     uint32_t* host = (uint32_t*)q->host_stage;
-    RF_HIP(hipMemcpyAsync(host, tc + nthr, 4, hipMemcpyDeviceToHost, d.stream));
+    RF_HIP(hipMemcpyAsync(host, bcnt + tiles, 4, hipMemcpyDeviceToHost, d.stream));
     RF_HIP(hipMemcpyAsync(host + 1, flag, 4, hipMemcpyDeviceToHost, d.stream));
     RF_HIP(hipStreamSynchronize(d.stream));
     if (host[1] != 0)
@@ -1814,8 +1821,10 @@ static rf_table* run_merge(rf_query* q, const rf_arg* args, int64_t n_args) {
     if (total > 0) {
         for (size_t k = 0; k < ks.out_types.size(); k++) l.args.out[k] = out->cols[k].values;
         l.args.out_cap = total;
-        RF_HIP(hipModuleLaunchKernel(l.spec->mod->emit, grid, 1, 1, 256, 1, 1, 0, d.stream,
-                                     params, nullptr));
+        if (ev) RF_HIP(hipEventRecord(ev[0], d.stream));
+        RF_HIP(hipModuleLaunchKernel(l.spec->mod->emit, (unsigned)tiles, 1, 1, RF_MERGE_BLOCK, 1, 1,
+                                     0, d.stream, params, nullptr));
+        if (ev) RF_HIP(hipEventRecord(ev[1], d.stream));
         RF_HIP(hipStreamSynchronize(d.stream));
     }
     return out.release();
@@ -2040,12 +2049,35 @@ void time_query(rf_query* q, const rf_arg* args, int64_t n_args, int iters, doub
         staged[q->plan.runs->table].u.table = runs.get();
         args = staged.data();
     }
-    if (q->plan.seg || q->plan.source == Plan::MERGE) {
+    if (q->plan.source == Plan::MERGE) {
+        // Whole runs (each synchronises) and the emit kernel of each.
+        if (iters <= 0) return;
+        hipEvent_t e[4];
+        for (auto& x : e) RF_HIP(hipEventCreate(&x));
+        table_free(run_merge(q, args, n_args));
+        double emit = 0;
+        RF_HIP(hipEventRecord(e[0], d.stream));
+        for (int i = 0; i < iters; i++) {
+            rf_table* t = run_merge(q, args, n_args, e + 2);
+            const bool emitted = t->rows() > 0;  // the events are recorded
+            table_free(t);
+            float ms = 0;
+            if (emitted) RF_HIP(hipEventElapsedTime(&ms, e[2], e[3]));
+            emit += ms;
+        }
+        RF_HIP(hipEventRecord(e[1], d.stream));
+        RF_HIP(hipEventSynchronize(e[1]));
+        float a = 0;
+        RF_HIP(hipEventElapsedTime(&a, e[0], e[1]));
+        for (auto& x : e) (void)hipEventDestroy(x);
+        *ms_all = a / iters;
+        *ms_scan = emit / iters;
+        return;
+    }
+    if (q->plan.seg) {
         // Whole runs (each synchronises).
         if (iters <= 0) return;
-        auto run = [&] {
-            return q->plan.seg ? run_seg_reduce(q, args, n_args) : run_merge(q, args, n_args);
-        };
+        auto run = [&] { return run_seg_reduce(q, args, n_args); };
         hipEvent_t e[2];
         for (auto& x : e) RF_HIP(hipEventCreate(&x));
         table_free(run());

@@ -60,6 +60,8 @@ hipError_t aot_minmax_i64(const int64_t* keys, const int64_t* n_dev, int64_t n_m
 hipError_t aot_adjacent_dups(const int64_t* keys, int64_t n, void* flag, hipStream_t s);
 hipError_t aot_not_ascending(const int64_t* keys, int64_t n, void* flag, hipStream_t s);
 int64_t aot_scan_entries(int64_t m);
+hipError_t aot_merge_splits(const int64_t* a, int64_t na, const int64_t* b, int64_t nb,
+                            int64_t tile, int64_t n, int64_t* out, hipStream_t s);
 hipError_t aot_scan_u32(void* h, int64_t m, hipStream_t s);
 // words x 16 bytes: word w's bits, then its count (RfProbeDev::dense_pack).
 hipError_t aot_rank_pack(const void* bits, const void* cum, int64_t words, void* pack,

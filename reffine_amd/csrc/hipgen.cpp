@@ -604,25 +604,35 @@ static KernelSource gen_seq_scan(const Plan& plan, const std::vector<std::vector
     return ks;
 }
 
-// Union merge (an Op over _in(t, A) | _in(t, B)): rf_qu_count and
-// rf_qu_emit walk the same merge positions, RF_MERGE_IT per thread. Merge
-// position m is the m-th step of the merge of A's and B's keys that takes A's
-// key first on a tie; the state (i, j) before the step is the reference
-// loop's state (LoopGen::build_loop, loopgen.cpp:44-110, over UnionSpace,
+// Union merge (an Op over _in(t, A) | _in(t, B)). Merge position m is the
+// m-th step of the merge of A's and B's keys that takes A's key first on a
+// tie; the state (i, j) before the step is the reference loop's state
+// (LoopGen::build_loop, loopgen.cpp:44-110, over UnionSpace,
 // iter_space.cpp:216-316: t = min of the two current keys, each side
 // advances when its key is <= the other's) except when A's previous key
 // equals B's current one: the reference advanced both sides at once there,
-// the merge takes B's key as a step of its own, which emits nothing. The
-// count kernel writes each thread's qualifying rows to partials[thread] (and
-// 0 at partials[threads]); after their exclusive scan the emit kernel writes
-// the rows from there, in merge order.
+// the merge takes B's key as a step of its own, which emits nothing.
+//
+// Block b owns merge positions [b * TILE, (b + 1) * TILE): the A rows
+// [splits[b], splits[b + 1]) and the B rows in between (merge-path splits,
+// rf_merge_splits). It copies both key ranges, with the key before each,
+// into LDS (coalesced; padded one slot in 8 so threads RF_MERGE_IT keys
+// apart hit different banks), and each thread finds its own start in LDS and
+// walks RF_MERGE_IT positions evaluating the predicate. rf_qu_count checks
+// the keys strictly ascending (its range against the key before it; a split
+// outside the tile's bounds also means unsorted keys: counters[0] is set)
+// and writes the block's row count to partials[b] (0 for the extra block
+// b = tiles). rf_qu_emit, after the exclusive scan of those counts, walks
+// again, places the block's emitting states (tile-local rows and side taken)
+// in LDS in merge order, then evaluates the outputs of row r of the block
+// in thread r % BLOCK, so consecutive threads write consecutive rows.
 static KernelSource gen_merge(const Plan& plan, const std::vector<std::vector<bool>>& nullable) {
     if (plan.ops.size() != 1 || !plan.reduces.empty())
         throw IRError("a union merge plan holds one Op");
     KernelSource ks;
     ks.entry = "rf_qu_count";
     ks.has_sel = true;
-    ks.tune.block = 256;
+    ks.tune.block = RF_MERGE_BLOCK;
     ks.tune.items = RF_MERGE_IT;
     ks.tune.sub = 1;
     Gen g(plan, nullable, ks);
@@ -632,72 +642,131 @@ static KernelSource gen_merge(const Plan& plan, const std::vector<std::vector<bo
     ParamSlot pb{ParamSlot::LEN, types::IDX_T, 0, plan.merge_b, 0};
     const std::string na = g.param(pa, "$len." + std::to_string(plan.merge_a) + ".0");
     const std::string nb = g.param(pb, "$len." + std::to_string(plan.merge_b) + ".0");
-    std::ostringstream body;
-    Emitter em(g, Mode::MERGE, body, "            ");
-    const std::string pred = em.expr(op.pred);
+    std::ostringstream pbody;
+    Emitter ep(g, Mode::MERGE, pbody, "                ");
+    const std::string pred = ep.expr(op.pred);
+    std::ostringstream vbody;
+    Emitter ev(g, Mode::MERGE, vbody, "            ");
+    ev.ntmp = ep.ntmp;
     std::vector<std::string> vals;
-    for (auto& v : op.values) vals.push_back(em.expr(v));
+    for (auto& v : op.values) vals.push_back(ev.expr(v));
     if (vals.size() > RF_MAXOUT) throw Unsupported("more than 16 output columns");
     ks.out_types = op.types;
 
     std::ostringstream o;
-    o << "// Generated union merge for reffine function '" << plan.func_name << "'.\n";
-    o << "template <bool EMIT>\n"
+    o << "// Generated union merge for reffine function '" << plan.func_name << "'.\n"
+         "#define RF_MP(x) ((x) + ((x) >> 3))\n"
+         "#define RF_MERGE_LDS (RF_MP(RF_MERGE_TILE) + 1)\n"
+         "template <bool EMIT>\n"
          "__device__ __forceinline__ void rf_qu_body(const RfScanArgs& L) {\n"
+         "    __shared__ int64_t sA[RF_MERGE_LDS], sB[RF_MERGE_LDS];\n"
+         "    __shared__ uint32_t srec[EMIT ? RF_MERGE_TILE : 1];\n"
+         "    __shared__ uint64_t swave[RF_MERGE_BLOCK / 64];\n"
       << "    const int64_t na = " << na << ", nb = " << nb << ";\n"
       << "    const int64_t* __restrict__ KA = (const int64_t*)L.col[" << ka << "];\n"
       << "    const int64_t* __restrict__ KB = (const int64_t*)L.col[" << kb << "];\n"
-      << "    uint32_t* __restrict__ tc = (uint32_t*)L.partials;\n"
+      << "    const int64_t* __restrict__ splits = (const int64_t*)L.wave_dst;\n"
+         "    uint32_t* __restrict__ bcnt = (uint32_t*)L.partials;\n"
          "    const int64_t M = na + nb;\n"
-         "    const int64_t nthr = (M + RF_MERGE_IT - 1) / RF_MERGE_IT;\n"
-         "    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;\n"
-         "    if (gt >= nthr) {\n"
-         "        if (!EMIT && gt == nthr) tc[nthr] = 0u;\n"
+         "    const int64_t blk = blockIdx.x;\n"
+         "    const int64_t d0 = blk * RF_MERGE_TILE;\n"
+         "    if (d0 >= M) {\n"
+         "        if (!EMIT && threadIdx.x == 0) bcnt[blk] = 0u;\n"
          "        return;\n"
          "    }\n"
-         "    const int64_t d0 = gt * RF_MERGE_IT;\n"
-         "    const int64_t d1 = d0 + RF_MERGE_IT < M ? d0 + RF_MERGE_IT : M;\n"
-         "    int64_t i = rf_merge_split(KA, na, KB, nb, d0);\n"
-         "    int64_t j = d0 - i;\n"
-         "    int64_t pa = i > 0 ? rf_g(KA)[i - 1] : 0;\n"
-         "    uint32_t cnt = 0;\n"
-         "    int64_t pos = EMIT ? (int64_t)tc[gt] : 0;\n"
-         "    (void)pos;\n"
-         "    for (int64_t m = d0; m < d1; m++) {\n"
-         "        // i + j = m < na + nb: one side at least is inside its table.\n"
-         "        const bool ok0 = i < na, ok1 = j < nb;\n"
-         "        const int64_t ka = ok0 ? rf_g(KA)[i] : 0;\n"
-         "        const int64_t kb = ok1 ? rf_g(KB)[j] : 0;\n"
-         "        const bool take0 = ok0 && (!ok1 || ka <= kb);\n"
-         "        if (!(ok1 && i > 0 && pa == kb)) {\n"
-         "            const int64_t key = take0 ? ka : kb;\n"
-         "            (void)key;\n";
-    o << body.str();
-    o << "            if (" << pred << ") {\n"
-         "                if (EMIT) {\n";
-    for (size_t k = 0; k < vals.size(); k++) {
-        const std::string T = cty(op.types[k]);
-        o << "                    ((" << T << "*)L.out[" << k << "])[pos] = (" << T << ")" << vals[k]
-          << ";\n";
-    }
-    o << "                    pos++;\n"
-         "                } else {\n"
-         "                    cnt++;\n"
+         "    const int64_t d1 = d0 + RF_MERGE_TILE < M ? d0 + RF_MERGE_TILE : M;\n"
+         "    const int len = (int)(d1 - d0);\n"
+         "    const int64_t a0 = splits[blk];\n"
+         "    int64_t a1 = splits[blk + 1];\n"
+         "    bool bad = false;\n"
+         "    if (a1 < a0) { a1 = a0; bad = true; }\n"
+         "    if (a1 > a0 + len) { a1 = a0 + len; bad = true; }\n"
+         "    const int64_t b0 = d0 - a0;\n"
+         "    const int nA = (int)(a1 - a0), nB = len - nA;\n"
+         "    // LDS slot x + 1 of a side holds its key at tile row x (slot 0: the\n"
+         "    // key before the tile, 0 at row -1).\n"
+         "    for (int x = threadIdx.x; x <= nA; x += RF_MERGE_BLOCK) {\n"
+         "        const int64_t r = a0 - 1 + x;\n"
+         "        sA[RF_MP(x)] = r >= 0 ? rf_g(KA)[r] : 0;\n"
+         "    }\n"
+         "    for (int x = threadIdx.x; x <= nB; x += RF_MERGE_BLOCK) {\n"
+         "        const int64_t r = b0 - 1 + x;\n"
+         "        sB[RF_MP(x)] = r >= 0 ? rf_g(KB)[r] : 0;\n"
+         "    }\n"
+         "    __syncthreads();\n"
+         "    if (!EMIT) {\n"
+         "        for (int x = threadIdx.x + 1; x <= nA; x += RF_MERGE_BLOCK)\n"
+         "            if (a0 - 2 + x >= 0 && !(sA[RF_MP(x - 1)] < sA[RF_MP(x)])) bad = true;\n"
+         "        for (int x = threadIdx.x + 1; x <= nB; x += RF_MERGE_BLOCK)\n"
+         "            if (b0 - 2 + x >= 0 && !(sB[RF_MP(x - 1)] < sB[RF_MP(x)])) bad = true;\n"
+         "        if (bad) atomicOr(L.counters, 1u);\n"
+         "    }\n"
+         "    const int q0 = (int)threadIdx.x * RF_MERGE_IT < len ? (int)threadIdx.x * RF_MERGE_IT : len;\n"
+         "    int lo = q0 > nB ? q0 - nB : 0, hi = q0 < nA ? q0 : nA;\n"
+         "    while (lo < hi) {\n"
+         "        const int mid = (lo + hi) >> 1;\n"
+         "        if (sA[RF_MP(mid + 1)] <= sB[RF_MP(q0 - mid)]) lo = mid + 1;\n"
+         "        else hi = mid;\n"
+         "    }\n"
+         "    int ia = lo, jb = q0 - lo;\n"
+         "    uint32_t rec[RF_MERGE_IT];\n"
+         "    uint32_t emask = 0;\n"
+         "#pragma unroll\n"
+         "    for (int s = 0; s < RF_MERGE_IT; s++) {\n"
+         "        rec[s] = 0u;\n"
+         "        if (q0 + s < len) {\n"
+         "            // ia + jb < nA + nB: one side at least has a row left in the tile.\n"
+         "            const bool ain = ia < nA, bin = jb < nB;\n"
+         "            const int64_t ka = ain ? sA[RF_MP(ia + 1)] : 0;\n"
+         "            const int64_t kb = bin ? sB[RF_MP(jb + 1)] : 0;\n"
+         "            const bool take0 = ain && (!bin || ka <= kb);\n"
+         "            const int64_t i = a0 + ia, j = b0 + jb;\n"
+         "            if (!(bin && i > 0 && sA[RF_MP(ia)] == kb)) {\n"
+         "                const bool ok0 = i < na, ok1 = j < nb;\n"
+         "                const int64_t key = take0 ? ka : kb;\n"
+         "                (void)ok0; (void)ok1; (void)key;\n";
+    o << pbody.str();
+    o << "                if (" << pred << ") {\n"
+         "                    rec[s] = (uint32_t)ia | ((uint32_t)jb << 12) | ((uint32_t)take0 << 24);\n"
+         "                    emask |= 1u << s;\n"
          "                }\n"
          "            }\n"
-         "        }\n"
-         "        if (take0) {\n"
-         "            pa = ka;\n"
-         "            i++;\n"
-         "        } else {\n"
-         "            j++;\n"
+         "            if (take0) ia++;\n"
+         "            else jb++;\n"
          "        }\n"
          "    }\n"
-         "    if (!EMIT) tc[gt] = cnt;\n"
+         "    uint64_t total = 0;\n"
+         "    const uint32_t base = (uint32_t)rf_block_excl_scan<RF_MERGE_BLOCK>(\n"
+         "        (uint64_t)__builtin_popcount(emask), swave, total);\n"
+         "    if (!EMIT) {\n"
+         "        if (threadIdx.x == 0) bcnt[blk] = (uint32_t)total;\n"
+         "        return;\n"
+         "    }\n"
+         "#pragma unroll\n"
+         "    for (int s = 0; s < RF_MERGE_IT; s++)\n"
+         "        if ((emask >> s) & 1u)\n"
+         "            srec[base + __builtin_popcount(emask & ((1u << s) - 1u))] = rec[s];\n"
+         "    __syncthreads();\n"
+         "    const int64_t off = bcnt[blk];\n"
+         "    for (int r = threadIdx.x; r < (int)total; r += RF_MERGE_BLOCK) {\n"
+         "        const uint32_t e = srec[r];\n"
+         "        const int ia = (int)(e & 4095u), jb = (int)((e >> 12) & 4095u);\n"
+         "        const bool take0 = ((e >> 24) & 1u) != 0u;\n"
+         "        const int64_t i = a0 + ia, j = b0 + jb;\n"
+         "        const bool ok0 = i < na, ok1 = j < nb;\n"
+         "        const int64_t key = take0 ? sA[RF_MP(ia + 1)] : sB[RF_MP(jb + 1)];\n"
+         "        (void)ok0; (void)ok1; (void)key;\n";
+    o << vbody.str();
+    for (size_t k = 0; k < vals.size(); k++) {
+        const std::string T = cty(op.types[k]);
+        o << "            ((" << T << "*)L.out[" << k << "])[off + r] = (" << T << ")" << vals[k]
+          << ";\n";
+    }
+    o << "    }\n"
          "}\n\n"
-         "extern \"C\" __global__ __launch_bounds__(256) void rf_qu_count(RfScanArgs L) { "
+         "extern \"C\" __global__ __launch_bounds__(RF_MERGE_BLOCK) void rf_qu_count(RfScanArgs L) { "
          "rf_qu_body<false>(L); }\n"
-         "extern \"C\" __global__ __launch_bounds__(256) void rf_qu_emit(RfScanArgs L) { "
+         "extern \"C\" __global__ __launch_bounds__(RF_MERGE_BLOCK) void rf_qu_emit(RfScanArgs L) { "
          "rf_qu_body<true>(L); }\n";
     ks.code = o.str();
     return ks;

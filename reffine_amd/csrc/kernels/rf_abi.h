@@ -20,8 +20,11 @@ typedef __UINT64_TYPE__ uint64_t;
 #define RF_MAXPARAMS 96
 #define RF_MAXPROBES 4
 #define RF_MAXOUT 16
-// Merge positions per thread of a union merge (rf_qu_count / rf_qu_emit).
-#define RF_MERGE_IT 8
+// A union merge (rf_qu_count / rf_qu_emit): RF_MERGE_TILE merge positions
+// per block of RF_MERGE_BLOCK threads, RF_MERGE_IT per thread.
+#define RF_MERGE_IT 4
+#define RF_MERGE_BLOCK 256
+#define RF_MERGE_TILE (RF_MERGE_IT * RF_MERGE_BLOCK)
 
 #define RF_EMPTY_KEY ((int64_t)0x8000000000000000ull)
 

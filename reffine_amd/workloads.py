@@ -176,6 +176,33 @@ def c4_bytes(n_build: int, n_probe: int, groups: int) -> int:
     return 9 * n_build + 24 * n_probe + 24 * groups
 
 
+# Two keyed vectors for the union merge (micro.h's fake_data: key, value).
+MERGE_TYPE = _vec_t(1, _i64_t, _i64_t)
+
+
+def outer_join(vt=MERGE_TYPE):
+    """benchmark/include/micro.h:79-95's outer join: an Op over
+    `_in(t, left) | _in(t, right)` (UnionSpace) whose outputs are t and
+    left[t].0 - right[t].0, each side read at its current row of the merge."""
+    left = _sym("left", vt)
+    right = _sym("right", vt)
+    t = _sym("t", _i64_t)
+    diff = _sub(_get(left[t], 0), _get(right[t], 0))
+    dsym = _sym("diff", diff)
+    op = _op([t], _in(t, left) | _in(t, right), [dsym])
+    osym = _sym("op", op)
+    fn = _func("ojoinbench", osym, [left, right])
+    fn.tbl[osym] = op
+    fn.tbl[dsym] = diff
+    return fn
+
+
+def merge_bytes(na: int, nb: int, n_out: int) -> int:
+    """Algorithmic bytes of the outer join: both sides' keys and values
+    (16 B/row) read, (t, diff) written (16 B per output row)."""
+    return 16 * (na + nb) + 16 * n_out
+
+
 # The grouped layout iterated by row id (an Op over `_in(t, T)` needs a
 # one-dimensional vector; the same device columns serve both views).
 C3_DIM1_TYPE = _vec_t(1, _i64_t, _i8_t, _i8_t, _i64_t, _f64_t, _f64_t, _f64_t, _i32_t)

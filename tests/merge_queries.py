@@ -1,26 +1,17 @@
 """Ops over a union of iteration spaces, _in(t, A) | _in(t, B) (the
 reference's UnionSpace): the outer join of benchmark/include/micro.h:79-95
 and variants. Shared by the CPU oracle checks and the GPU parity tests."""
+from reffine_amd import workloads
 from reffine_amd.reffiner import (_and, _eq, _f64_t, _func, _get, _i64, _i64_t, _in, _mod, _op,
                                   _or, _sub, _sym, _vec_t)
 
-MERGE_T = _vec_t(1, _i64_t, _i64_t)   # key, value (micro.h's fake_data, dim 1)
+MERGE_T = workloads.MERGE_TYPE        # key, value (micro.h's fake_data, dim 1)
 MERGE_F = _vec_t(1, _i64_t, _f64_t)
 
 
 def ojoin(vt=MERGE_T):
     """op(t | t in left or t in right) -> left[t].0 - right[t].0 (micro.h:79-95)."""
-    left = _sym("left", vt)
-    right = _sym("right", vt)
-    t = _sym("t", _i64_t)
-    diff = _sub(_get(left[t], 0), _get(right[t], 0))
-    dsym = _sym("diff", diff)
-    op = _op([t], _or(_in(t, left), _in(t, right)), [dsym])
-    osym = _sym("op", op)
-    fn = _func("ojoinbench", osym, [left, right])
-    fn.tbl[osym] = op
-    fn.tbl[dsym] = diff
-    return fn
+    return workloads.outer_join(vt)
 
 
 def ojoin_even():
