@@ -34,7 +34,7 @@ def log(msg: str) -> None:
 
 
 PROFILES = {"c2": "scan_c2_60m_rocprof.md", "c3": "group_agg_c3_60m_rocprof.md",
-            "c4": "group_join_c4_rocprof.md"}
+            "c4": "group_join_c4_rocprof.md", "merge": "union_merge_30m_rocprof.md"}
 
 
 def measured_traffic(workload: str = "c2"):
@@ -574,6 +574,8 @@ def bench_merge(args) -> int:
                  "rf_qu_emit (keys + values read, rows written: 16 B per row each)", emit_ms,
                  run_ms)
     line["metric"] = "union merge (outer join) throughput"
+    line["roofline"]["traffic"] = measured_traffic("merge")
+    line["roofline"]["traffic_source"] = "profiles/" + PROFILES["merge"]
     print(json.dumps(line), flush=True)
     return 0 if ok else 1
 
