@@ -1,14 +1,17 @@
-// Ahead-of-time kernels of the hash-join + grouped-aggregation path:
-// key range, a stable LSD radix sort with payload columns, adjacent-duplicate
-// detection and segment bounds of a sorted group column.
+// Ahead-of-time kernels of the hash-join + grouped-aggregation path: key
+// range, the rank map, a stable LSD radix sort with payload columns,
+// adjacent-duplicate detection, segment bounds of a sorted group column, the
+// partitioning of joined rows by group rank, and the union merge's tile
+// splits.
 //
-// The grouped fold is sort-based, not atomic: joined records are ordered by
-// group rank with a stable sort (ties keep ascending probe-row order), and
-// each group is then folded sequentially. This is the fold order of the
-// reference's nested loop (an outer Op over the build table whose output
-// reduces the probe rows with a matching key, src/pass/loopgen.cpp:206-269),
-// and on gfx950 it is also faster: random global atomics saturate near
-// 23 G updates/s, while the sort streams.
+// The deterministic mode orders joined records by group rank with a stable
+// sort (ties keep ascending probe-row order) and folds each group
+// sequentially: the fold order of the reference's nested loop (an outer Op
+// over the build table whose output reduces the probe rows with a matching
+// key, src/pass/loopgen.cpp:206-269). The default mode partitions the rows by
+// rank range and folds each partition with LDS atomics (generated
+// rf_gj_agg). Neither uses random global atomics, which saturate near
+// 23 G updates/s on gfx950.
 #include <hip/hip_runtime.h>
 
 #include "rf_device.h"
