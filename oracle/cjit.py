@@ -50,6 +50,7 @@ PRELUDE = r"""
 #include <math.h>
 #include <algorithm>
 #include <cmath>
+#include <limits>
 struct RfTab { const void* const* data; const uint16_t* const* valid; int64_t n; };
 struct RfIdx { const int64_t* keys; const int64_t* rows; uint64_t mask; int64_t min_row; };
 static inline bool get_null_bit(const uint16_t* bm, int64_t i) {
@@ -84,10 +85,38 @@ def cpp_const(t: R.DataType, v: float) -> str:
     return f"{ty}({int(v)})"
 
 
+_FIXED_IDENT = {
+    ("add", True): "-0.0", ("add", False): "0",
+    ("max", True): "-INFINITY", ("min", True): "INFINITY",
+}
+
+
+def _ident(op: str, t: R.DataType) -> str:
+    """The identity a fixed-order fold starts each lane and thread from
+    (-0.0 for float sums: the exact additive identity)."""
+    ct = _CTYPE[t.btype]
+    if (op, t.is_float()) in _FIXED_IDENT:
+        return f"({ct})({_FIXED_IDENT[(op, t.is_float())]})"
+    lim = "min" if op == "max" else "max"
+    return f"std::numeric_limits<{ct}>::{lim}()"
+
+
+def _combine(op: str, a: str, b: str) -> str:
+    """a (+) b with std::max / std::min semantics for max / min."""
+    if op == "add":
+        return f"({a} + {b})"
+    if op == "max":
+        return f"(({a} < {b}) ? {b} : {a})"
+    return f"(({b} < {a}) ? {b} : {a})"
+
+
 class _Gen:
-    def __init__(self, fn: R.Func, vectorize: bool):
+    def __init__(self, fn: R.Func, vectorize: bool, fixed=None):
         self.fn = fn
         self.vectorize = vectorize
+        # (BLOCK, ITEMS, SUB, ops per state field or None for all "add"):
+        # emit the reductions in the GPU kernels' deterministic order.
+        self.fixed = fixed
         self.tbl = {s: e for s, e in fn.tbl.items()}
         self.inputs = list(fn.inputs)
         self.lines: list[str] = []
@@ -363,6 +392,18 @@ class _Gen:
                 env[comp.state] = self.nest(comp.type, svars)
                 env[comp.val] = vals
                 accv = self.flatten(self.val(comp.acc, env))
+                if self.fixed is not None:
+                    blk += self.fixed_fold(fields, svars, init_vals, accv, c, start, end)
+                    off = 0
+                    for (path, t), sv in zip(fields, svars):
+                        w = np.dtype(_NP[t.btype]).itemsize
+                        off = (off + w - 1) // w * w
+                        blk.append(f"    *({_CTYPE[t.btype]}*)((char*)state{ci} + {off}) = {sv};")
+                        off += w
+                    meta["comps"].append(("reduce", comp, fields))
+                    blocks.append(blk)
+                    meta.setdefault("probed", set()).update(t for t, _ in self.probes)
+                    continue
                 for (path, t), sv, iv in zip(fields, svars, init_vals):
                     blk.append(f"    {_CTYPE[t.btype]} {sv} = {iv};")
                 blk += [f"    int64_t i = {start};", f"    const int64_t end = {end};",
@@ -411,12 +452,95 @@ class _Gen:
         return text, meta
 
 
+def _fixed_fold(self, fields, svars, init_vals, accv, c, start, end):
+    """C text of a reduction in the GPU scan's deterministic order: rows
+    [base, end) in wave-tiles of SUB*64*ITEMS rows (base = start rounded
+    down to a multiple of ITEMS); in sub-step s lane l folds rows
+    tile0 + (s*64 + l)*ITEMS + [0, ITEMS) in ascending order from the
+    identity, applying the query's own accumulator to its lane state; the
+    lanes combine by xor butterflies (lane_i = lane_i (+) lane_{i^o},
+    o = 1, 2, ..., 32) and lane 0 is the tile's partial; thread t of a block of
+    BLOCK threads folds partials t, t + BLOCK, ... from the identity; each
+    wave combines its threads by the same butterflies; the waves fold in
+    order; the result is init (+) total."""
+    block, items, sub, ops = self.fixed
+    ops = ops or ["add"] * len(fields)
+    if len(ops) != len(fields):
+        raise ValueError("one combine operator per state field")
+    L = []
+    ty = [_CTYPE[t.btype] for _, t in fields]
+    idn = [_ident(op, t) for op, (_, t) in zip(ops, fields)]
+    L += [f"    const int64_t lo = {start};", f"    const int64_t end = {end};",
+          f"    const int64_t TILE = (int64_t){sub} * 64 * {items};",
+          f"    const int64_t base = lo >= 0 ? lo - lo % {items} : "
+          f"-(((-lo) + {items} - 1) / {items}) * {items};",
+          "    const int64_t ntiles = end > lo ? (end - base + TILE - 1) / TILE : 0;"]
+    for k, (t_, idk) in enumerate(zip(ty, idn)):
+        L.append(f"    {t_}* part{k} = new {t_}[ntiles > 0 ? ntiles : 1];")
+    L.append("    for (int64_t tile = 0; tile < ntiles; tile++) {")
+    for k, (t_, idk) in enumerate(zip(ty, idn)):
+        L.append(f"        {t_} ln{k}[64];")
+        L.append(f"        for (int l = 0; l < 64; l++) ln{k}[l] = {idk};")
+    L += [f"        for (int s = 0; s < {sub}; s++) {{",
+          "        for (int l = 0; l < 64; l++) {"]
+    for k, (sv, t_) in enumerate(zip(svars, ty)):
+        L.append(f"            {t_} {sv} = ln{k}[l];")
+    L += [f"            for (int k = 0; k < {items}; k++) {{",
+          f"                const int64_t i = base + tile * TILE + ((int64_t)s * 64 + l) * {items} + k;",
+          "                if (i < lo || i >= end) continue;"]
+    L += ["        " + x for x in self.lines]
+    L.append(f"                if ({c}) {{")
+    L += [f"                    const {t_} n{k} = {a};" for k, (a, t_) in enumerate(zip(accv, ty))]
+    L += [f"                    {sv} = n{k};" for k, sv in enumerate(svars)]
+    L += ["                }", "            }"]
+    for k, sv in enumerate(svars):
+        L.append(f"            ln{k}[l] = {sv};")
+    L += ["        }", "        }"]
+    for k, (t_, op) in enumerate(zip(ty, ops)):
+        L += ["        for (int o = 1; o < 64; o <<= 1) {",
+              f"            {t_} nx[64];",
+              f"            for (int l = 0; l < 64; l++) nx[l] = {_combine(op, f'ln{k}[l]', f'ln{k}[l ^ o]')};",
+              f"            for (int l = 0; l < 64; l++) ln{k}[l] = nx[l];",
+              "        }",
+              f"        part{k}[tile] = ln{k}[0];"]
+    L.append("    }")
+    for k, (sv, t_, op, idk, iv) in enumerate(zip(svars, ty, ops, idn, init_vals)):
+        L += ["    {",
+              f"        {t_} th[{block}];",
+              f"        for (int t = 0; t < {block}; t++) th[t] = {idk};",
+              f"        for (int64_t p = 0; p < ntiles; p++) th[p % {block}] = "
+              f"{_combine(op, f'th[p % {block}]', f'part{k}[p]')};",
+              f"        for (int w = 0; w < {block} / 64; w++) {{",
+              "            for (int o = 1; o < 64; o <<= 1) {",
+              f"                {t_} nx[64];",
+              f"                for (int l = 0; l < 64; l++) nx[l] = "
+              f"{_combine(op, 'th[w * 64 + l]', 'th[w * 64 + (l ^ o)]')};",
+              "                for (int l = 0; l < 64; l++) th[w * 64 + l] = nx[l];",
+              "            }",
+              "        }",
+              f"        {t_} tot = th[0];",
+              f"        for (int w = 1; w < {block} / 64; w++) tot = {_combine(op, 'tot', 'th[w * 64]')};",
+              f"        {t_} {sv} = ({t_})({iv});",
+              f"        {sv} = {_combine(op, sv, 'tot')};",
+              f"        part{k}[0] = {sv};",
+              "    }"]
+    # the final states, declared at block scope for the state write
+    for k, (sv, t_) in enumerate(zip(svars, ty)):
+        L.append(f"    const {t_} {sv} = part{k}[0];")
+    for k in range(len(fields)):
+        L.append(f"    delete[] part{k};")
+    return L
+
+
+_Gen.fixed_fold = _fixed_fold
+
+
 class CompiledQuery:
     """A query compiled by the restated CPU JIT. Call with numpy tables."""
 
-    def __init__(self, fn: R.Func, vectorize: bool = False):
+    def __init__(self, fn: R.Func, vectorize: bool = False, fixed_order=None):
         self.fn = fn
-        gen = _Gen(fn, vectorize)
+        gen = _Gen(fn, vectorize, fixed_order)
         self.src, self.meta = gen.gen()
         os.makedirs(BUILD, exist_ok=True)
         h = hashlib.sha1(self.src.encode()).hexdigest()[:16]
@@ -570,8 +694,13 @@ def build_index(keys: np.ndarray, valid=None):
     return tk, tr, cap - 1, mn
 
 
-def compile_op(fn: R.Func, vectorize: bool = False) -> CompiledQuery:
-    return CompiledQuery(fn, vectorize)
+def compile_op(fn: R.Func, vectorize: bool = False, fixed_order=None) -> CompiledQuery:
+    """fixed_order=(BLOCK, ITEMS, SUB[, ops]): the reductions in the GPU
+    deterministic mode's order (ops: each state field's combine, "add" by
+    default)."""
+    if fixed_order is not None and len(fixed_order) == 3:
+        fixed_order = tuple(fixed_order) + (None,)
+    return CompiledQuery(fn, vectorize, fixed_order)
 
 
 def build() -> None:

@@ -4,7 +4,7 @@ bit-exact against the fixed-order restatement."""
 import numpy as np
 import pytest
 
-from oracle import brute, detorder
+from oracle import brute, cjit, detorder
 from reffine_amd import DeviceTable, compile_op, datagen, workloads
 
 pytestmark = pytest.mark.gpu
@@ -47,6 +47,20 @@ def test_c3_deterministic_bit_exact(device):
             assert got[i] == want.item(), (i, got[i], want)
             i += 1
     assert q(t) == got
+
+
+def test_c3_deterministic_equals_cjit_fixed_order(device):
+    """The deterministic mode's 36 Q1 fields equal the CPU JIT
+    restatement's fixed-order emission at the kernel's launch shape, bit
+    for bit."""
+    n = 700_001
+    c, t = c3_table(n, seed=10)
+    q = compile_op(workloads.group_agg_c3(), deterministic=True)
+    (want,) = cjit.compile_op(workloads.group_agg_c3(), fixed_order=q.tuning()[:3])(
+        {"cols": [c[k] for k in COLS]})
+    got = q(t)
+    assert [np.float64(x).view(np.uint64) if isinstance(x, float) else x for x in got] == \
+        [np.float64(x).view(np.uint64) if isinstance(x, float) else x for x in want]
 
 
 def test_c3_full_size_60m(device):

@@ -524,3 +524,124 @@ def test_union_merge_matches_the_reference_loop_where_it_is_defined():
     v = rng.integers(-9, 9, len(k))
     t, d = merge.ojoin(k, v, k, v)
     assert t.tolist() == k.tolist() and not d.any()
+
+
+def _bits(vals):
+    return [np.float64(x).view(np.uint64) if isinstance(x, float) else x for x in vals]
+
+
+def test_cjit_fixed_order_equals_detorder_on_workloads():
+    """oracle/cjit.py's fixed-order emission (C text in the GPU deterministic
+    mode's tile / butterfly / completion order, compiled by clang) equals the
+    numpy restatement of that order bit for bit on the C1, C2 and C3 folds,
+    for unaligned sizes and a few launch shapes; on contributions where the
+    order shows, two shapes give different bits, each matching its own numpy
+    order, so the order is what is pinned."""
+    n = 70_001
+    c1 = datagen.scan_table(n, 6, with_q=False)
+    m1 = (c1["d"] >= 365) & (c1["d"] < 730) & (c1["v"] < 0.5)
+    x1 = c1["v"] * c1["k"].astype(np.float64)
+    for shape in [(256, 2, 8), (512, 4, 2), (256, 1, 1)]:
+        (st,) = cjit.compile_op(workloads.scan_agg_c1(), fixed_order=shape)(
+            {"cols": [c1["rowid"], c1["k"], c1["v"], c1["d"]]})
+        assert st[0] == int(m1.sum())
+        assert _bits([st[1]]) == _bits([float(detorder.det_sum(x1, m1, 0, n, *shape))]), shape
+    # Contributions of both signs spread over 15 decades make the order visible.
+    rng = np.random.default_rng(1)
+    kw = (rng.choice([-1, 1], n) * (10.0 ** rng.uniform(0, 15, n))).astype(np.int64)
+    xw = c1["v"] * kw.astype(np.float64)
+    cols = {"cols": [c1["rowid"], kw, c1["v"], c1["d"]]}
+    (a,) = cjit.compile_op(workloads.scan_agg_c1(), fixed_order=(256, 1, 1))(cols)
+    (b,) = cjit.compile_op(workloads.scan_agg_c1(), fixed_order=(256, 2, 8))(cols)
+    assert _bits([a[1]]) == _bits([float(detorder.det_sum(xw, m1, 0, n, 256, 1, 1))])
+    assert _bits([b[1]]) == _bits([float(detorder.det_sum(xw, m1, 0, n, 256, 2, 8))])
+    assert _bits([a[1]]) != _bits([b[1]])
+
+    c2 = datagen.scan_table(n, 5, with_q=True, rowid_dtype=np.int32)
+    cq = cjit.compile_op(workloads.scan_select_c2(), fixed_order=(256, 4, 4))
+    (st2, sel) = cq({"cols": [c2["rowid"], c2["k"], c2["v"], c2["d"], c2["q"]]})
+    m2 = (c2["d"] >= 1000) & (c2["d"] < 1365) & (c2["v"] >= 0.05) & (c2["v"] <= 0.07) & \
+        (c2["q"] < 24)
+    assert st2[0] == int(m2.sum()) and st2[1] == int(c2["q"][m2].sum())
+    x2 = c2["v"] * c2["q"].astype(np.float64)
+    assert _bits([st2[2]]) == _bits([float(detorder.det_sum(x2, m2, 0, n, 256, 4, 4))])
+    np.testing.assert_array_equal(sel[0], c2["rowid"][m2])
+
+    c3 = datagen.group_table(n, 3)
+    (st3,) = cjit.compile_op(workloads.group_agg_c3(), fixed_order=(256, 2, 16))(
+        {"cols": [c3[k] for k in ("rowid", "f1", "f2", "qty", "price", "disc", "tax", "d")]})
+    keep = c3["d"] <= 2526
+    vals = brute.group_c3_contribs(c3)
+    want = []
+    for a, b in workloads.C3_GROUPS:
+        m = keep & (c3["f1"] == a) & (c3["f2"] == b)
+        for x in vals + [np.ones(n, np.int64)]:
+            want.append(detorder.det_sum(x, m, 0, n, 256, 2, 16).item())
+    assert _bits(list(st3)) == _bits(want)
+
+
+def _nbody_loops(b, n, g=1.0, dt=0.01, assoc="left", compact=True):
+    """reffine's Nbody step (benchmark/include/misc.h:77-202) as plain
+    Python loops over Python floats: the pair Op's nested loops (id_a outer,
+    id_b inner, a == b skipped), each expression one IEEE operation in the
+    C text's order, the per-body window sums over pair rows
+    [idx*n, (idx+1)*n) that exist, then the velocity and position update.
+    `assoc` and `compact` plant mistakes for the check below."""
+    m = [float(v) for v in b["m"]]
+    x, y, z = ([float(v) for v in b[c]] for c in ("x", "y", "z"))
+    pairs = []
+    for ia in range(n):
+        for ib in range(n):
+            if ia == ib and compact:
+                continue
+            dx, dy, dz = x[ia] - x[ib], y[ia] - y[ib], z[ia] - z[ib]
+            if assoc == "left":
+                dist2 = (dx * dx + dy * dy) + dz * dz
+            else:
+                dist2 = dx * dx + (dy * dy + dz * dz)
+            if dist2 == 0.0:
+                pairs.append((0.0, 0.0, 0.0))
+                continue
+            dist = math.sqrt(dist2)
+            f = (g * (m[ia] * m[ib])) / dist2
+            pairs.append(((f * dx) / dist, (f * dy) / dist, (f * dz) / dist))
+    out = {c: [] for c in ("X", "Y", "Z", "VX", "VY", "VZ")}
+    for idx in range(n):
+        s = [0.0, 0.0, 0.0]
+        for r in range(idx * n, (idx + 1) * n):
+            if r < len(pairs):
+                for c in range(3):
+                    s[c] = s[c] + pairs[r][c]
+        v = [float(b[k][idx]) + (s[c] / m[idx]) * dt for c, k in enumerate(("vx", "vy", "vz"))]
+        p = [(x, y, z)[c][idx] + v[c] * dt for c in range(3)]
+        for c, k in enumerate(("X", "Y", "Z")):
+            out[k].append(p[c])
+        for c, k in enumerate(("VX", "VY", "VZ")):
+            out[k].append(v[c])
+    return out, pairs
+
+
+def test_nbody_oracle_against_python_loops():
+    """oracle/graph.py's nbody (and its vectorised form) equal independent
+    Python loops written from misc.h bit for bit on n <= 6 bodies; the pair
+    forces are antisymmetric (F(a, b) = -F(b, a) exactly: IEEE subtraction
+    is, and the rest is symmetric in a and b); a different association of
+    dist2 or pair rows kept for a == b give other bits, so both are pinned."""
+    from oracle import graph as og
+    caught = {"right": 0, "nocompact": 0}
+    for n in range(2, 7):
+        for seed in range(4):
+            b = datagen.bodies(n, seed=100 + seed)
+            want, pairs = _nbody_loops(b, n)
+            for got in (og.nbody(b, n), og.nbody_np(b, n)):
+                for k, col in zip(("X", "Y", "Z", "VX", "VY", "VZ"), got[2:]):
+                    assert np.asarray(col, np.float64).view(np.uint64).tolist() == \
+                        np.asarray(want[k], np.float64).view(np.uint64).tolist(), (n, seed, k)
+            rows = [(ia, ib) for ia in range(n) for ib in range(n) if ia != ib]
+            at = {ab: f for ab, f in zip(rows, pairs)}
+            for (ia, ib), f in at.items():
+                assert all(u == -v for u, v in zip(f, at[(ib, ia)]))
+            for plant, kw in (("right", {"assoc": "right"}), ("nocompact", {"compact": False})):
+                bad, _ = _nbody_loops(b, n, **kw)
+                caught[plant] += any(bad[k] != want[k] for k in want)
+    assert caught["right"] > 0 and caught["nocompact"] > 0

@@ -7,7 +7,7 @@ deterministic mode bit-exact against oracle.detorder."""
 import numpy as np
 import pytest
 
-from oracle import brute, detorder
+from oracle import brute, cjit, detorder
 from reffine_amd import DeviceTable, compile_op, datagen, workloads
 
 FP_TOL = 1e-12
@@ -46,6 +46,30 @@ def test_c1_deterministic_bit_exact(device, n):
     assert s == float(want)  # bit-exact
     # and repeatable
     assert q(t) == (cnt, s)
+
+
+def _bits(vals):
+    return [np.float64(x).view(np.uint64) if isinstance(x, float) else x for x in vals]
+
+
+@pytest.mark.parametrize("n", [4097, 1 << 20])
+def test_deterministic_mode_equals_cjit_fixed_order(device, n):
+    """The deterministic mode's C1 and C2 results equal the CPU JIT
+    restatement's fixed-order emission (oracle/cjit.py, C text in the
+    kernels' tile / butterfly / completion order) at the kernel's launch
+    shape, bit for bit; C2's selection too."""
+    c, t = c1_table(n, seed=5)
+    q = compile_op(workloads.scan_agg_c1(), deterministic=True)
+    (want,) = cjit.compile_op(workloads.scan_agg_c1(), fixed_order=q.tuning()[:3])(
+        {"cols": [c["rowid"], c["k"], c["v"], c["d"]]})
+    assert _bits(q(t)) == _bits(want)
+    c2, t2 = c2_table(n)
+    q2 = compile_op(workloads.scan_select_c2(), deterministic=True)
+    st, sel = q2(t2)
+    wst, wsel = cjit.compile_op(workloads.scan_select_c2(), fixed_order=q2.tuning()[:3])(
+        {"cols": [c2["rowid"], c2["k"], c2["v"], c2["d"], c2["q"]]})
+    assert _bits(st) == _bits(wst)
+    np.testing.assert_array_equal(sel.column(0), wsel[0])
 
 
 @pytest.mark.parametrize("n", [0, 17, 4096, 4097, 200_003])
