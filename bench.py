@@ -85,6 +85,10 @@ def main() -> int:
                          "record batches)")
     ap.add_argument("--c2-only", action="store_true",
                     help="default line: leave out the C3 and C4 sub-objects")
+    ap.add_argument("--tune", action="store_true",
+                    help="c5: also tune each template's launch shape (and a group join's index "
+                         "load factor) on the median batch, persist the choices, and measure the "
+                         "batched QPS with the tuned shapes against the defaults")
     ap.add_argument("--counter-pass", action="store_true",
                     help="untimed: run the query 3 times and a 1 GiB calibration copy 3 times "
                          "(the program profiled by the FETCH_SIZE / WRITE_SIZE passes)")
@@ -580,9 +584,18 @@ def bench_merge(args) -> int:
     return 0 if ok else 1
 
 
-def c5_setup(n_queries: int, seed: int = 5):
-    """64 record batches (log-normal row counts) in the grouped layout, two
-    prepared query templates and the request mix."""
+C5_TEMPLATES = {"q6": "q6_template", "q1": "group_agg_c3", "join": "dim_join_template"}
+
+
+def _c5_func(name):
+    from reffine_amd import workloads
+    return workloads.group_agg_c3(None) if name == "q1" else getattr(workloads, C5_TEMPLATES[name])()
+
+
+def c5_setup(n_queries: int, seed: int = 5, queries=None):
+    """64 record batches (log-normal row counts) in the grouped layout, the
+    three prepared query templates (deterministic; `queries` overrides them)
+    and the request mix."""
     from reffine_amd import DeviceTable, compile_op, datagen, workloads
     sizes = datagen.lognormal_sizes(64, seed)
     tables = []
@@ -593,9 +606,8 @@ def c5_setup(n_queries: int, seed: int = 5):
         tables.append((t1, t1.view(2)))
     dim = datagen.dim_table()
     D = DeviceTable.from_columns([dim["dk"], dim["dflag"]], dim=1)
-    queries = {"q6": compile_op(workloads.q6_template(), deterministic=True),
-               "q1": compile_op(workloads.group_agg_c3(None), deterministic=True),
-               "join": compile_op(workloads.dim_join_template(), deterministic=True)}
+    if queries is None:
+        queries = {name: compile_op(_c5_func(name), deterministic=True) for name in C5_TEMPLATES}
     mix = workloads.c5_mix(n_queries, len(tables), seed=17)
     work = []
     nbytes = 0
@@ -618,8 +630,64 @@ def bench_c5(args) -> int:
     """Batched/served form against plain per-query launches: closed-loop
     throughput of both, then latency at a fixed Poisson arrival rate."""
     out = c5_measure(args.queries)
+    ok = out["config"]["identical_results"]
+    if args.tune:
+        out["tuned"] = c5_tuned(args.queries, out["batched"]["qps"], out["plain"]["qps"])
+        ok = ok and out["tuned"]["identical_results"]
     print(json.dumps(out), flush=True)
-    return 0 if out["config"]["identical_results"] else 1
+    return 0 if ok else 1
+
+
+C5_TUNE_CANDS = [(256, 2, 4, 16, 0, 0), (256, 2, 8, 16, 0, 0), (256, 4, 4, 16, 0, 0),
+                 (256, 2, 8, 8, 1, 0), (256, 4, 4, 8, 1, 0), (256, 2, 16, 8, 1, 4),
+                 (512, 2, 8, 16, 0, 0), (256, 1, 8, 16, 0, 0)]
+
+
+def c5_tuned(n_queries: int, default_qps: float, default_plain_qps: float) -> dict:
+    """Tunes each template on the batch of median size (launch shape among
+    C5_TUNE_CANDS, and the index load factor for the join; persisted per
+    query shape), then measures the batched closed loop with the tuned
+    queries; the results are checked against the tuned queries' plain runs."""
+    from reffine_amd import autotune, batch, compile_op, workloads
+    sizes, tables, queries, work, nbytes = c5_setup(n_queries)
+    mix = workloads.c5_mix(n_queries, len(sizes), seed=17)
+    med = int(np.argsort(sizes)[len(sizes) // 2])
+    tuned, shapes = {}, {}
+    for name in C5_TEMPLATES:
+        i = next(k for k, (nm, _, _) in enumerate(mix) if nm == name)
+        _, b, params = mix[i]
+        inp = list(work[i][1])
+        for k, x in enumerate(inp):  # the same inputs over the median batch
+            if x is tables[b][0]:
+                inp[k] = tables[med][0]
+            elif x is tables[b][1]:
+                inp[k] = tables[med][1]
+        log(f"tuning {name} on a {int(sizes[med])}-row batch")
+        fn = _c5_func(name)
+        q, times = autotune.autotune(fn, inp, deterministic=True, cands=C5_TUNE_CANDS,
+                                     scales=(2, 4) if name == "join" else None)
+        tuned[name] = autotune.tuned(fn, inp, deterministic=True)
+        d_ms = compile_op(fn, deterministic=True).time(*inp, iters=10)
+        shapes[name] = {"tuning": list(tuned[name].tuning()),
+                        "index_scale": tuned[name].index_scale,
+                        "ms_tuned": round(min(times.values()), 5), "ms_default": round(d_ms, 5)}
+        log(f"{name}: {shapes[name]}")
+    _, _, _, work_t, _ = c5_setup(n_queries, queries=tuned)
+    reqs = [(q, inp, 0.0) for q, inp in work_t]
+    batch.serve(reqs)
+    served = batch.serve(reqs)
+    batch.serve_plain(reqs)
+    plain = batch.serve_plain(reqs)
+    same = bool(np.array_equal(plain.out_rows, served.out_rows) and all(
+        batch.reduction_bytes(q, a) == batch.reduction_bytes(q, b)
+        for (q, _), a, b in zip(work_t, plain.outputs, served.outputs)) and
+        (served.status == 0).all() and (plain.status == 0).all())
+    return {"shapes": shapes, "tuning_cache": autotune.cache_path(),
+            "batched_qps_tuned": round(served.qps, 1), "batched_qps_default": default_qps,
+            "speedup_vs_default": round(served.qps / default_qps, 3),
+            "plain_qps_tuned": round(plain.qps, 1), "plain_qps_default": default_plain_qps,
+            "plain_speedup_vs_default": round(plain.qps / default_plain_qps, 3),
+            "identical_results": same}
 
 
 def c5_summary(n_queries: int) -> dict:

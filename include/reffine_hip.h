@@ -211,8 +211,12 @@ int rf_table_free(rf_table* t);
  * specialises the hand-written kernels for the query's row expressions.
  * Replaces compile_op<T>(func, vectorize) (include/reffine/utils/utils.h:
  * 47-54). `flags`: RF_COMPILE_DETERMINISTIC selects the fixed-order
- * reduction mode. */
+ * reduction mode; RF_COMPILE_INDEX_SCALE(s), s in 2..15, sizes a group
+ * join's hash index over sparse build keys at >= s slots per key (the
+ * load factor the autotuner picks; default 2; a rank map over dense keys
+ * does not use it). */
 #define RF_COMPILE_DETERMINISTIC 1u
+#define RF_COMPILE_INDEX_SCALE(s) ((uint32_t)(s) << 8)
 int rf_compile(const char* ir_text, uint32_t flags, rf_query** out);
 
 /* rf_compile with an explicit launch shape `tuning` = {BLOCK (64..1024,

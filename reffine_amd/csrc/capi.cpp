@@ -199,8 +199,12 @@ static std::unique_ptr<rf_query> make_query(const rf::N& f0, uint32_t flags,
         need(t.waves_per_cu >= 1 && t.waves_per_cu <= 64, "waves_per_cu must be 1..64");
         q->tune = t;
     }
+    const int scale = (int)((flags >> 8) & 15u);
+    need(scale == 0 || scale >= 2, "the index scale must be 2..15 (0: 2)");
     if (q->plan.gj) {
         rf::setup_group_join(q.get(), tuning ? &q->tune : nullptr);
+        if (scale) q->gj->table_scale = scale;
+        q->tune.index_scale = q->gj->table_scale;
         return q;
     }
     // Compile the all-valid specialisation now so compile errors surface
@@ -346,6 +350,7 @@ int rf_query_describe(const rf_query* q, char* buf, size_t cap) {
         s += "tuning: block " + std::to_string(tn.block) + " items " + std::to_string(tn.items) +
              " sub " + std::to_string(tn.sub) + " waves/CU " + std::to_string(tn.waves_per_cu) +
              (tn.pipeline ? " pipelined" : "") +
+             (q->gj ? " index scale " + std::to_string(q->gj->table_scale) : std::string()) +
              (q->det ? " deterministic" : "") + "\n";
         if (!q->specs.empty()) s += "--- kernel ---\n" + q->specs.begin()->second.ks.code;
         std::strncpy(buf, s.c_str(), cap - 1);

@@ -43,6 +43,9 @@ struct Tuning {
     int waves_per_cu = 16;  // launch width: resident waves aimed at per CU
     int pipeline = 0;       // 1: overlap the next load group's loads with this one's rows
     int lsub = 0;           // sub-steps per load group (divides sub); 0: from the load budget
+    // A group join's hash index over its build keys (sparse keys): capacity
+    // >= index_scale * keys, rounded up to a power of two (load <= 1/scale).
+    int index_scale = 2;
 };
 
 struct KernelSource {

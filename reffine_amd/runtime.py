@@ -519,13 +519,16 @@ class Query:
     """A compiled query: the GPU counterpart of compile_op's function pointer."""
 
     def __init__(self, func: Func, vectorize: bool = False, deterministic: bool = False,
-                 tuning: tuple | None = None):
+                 tuning: tuple | None = None, index_scale: int | None = None):
         self.func = func
         self.vectorize = vectorize
         self.deterministic = deterministic
+        self.index_scale = index_scale
         self.text = func.ir()
         h = ctypes.c_void_p()
         flags = RF_COMPILE_DETERMINISTIC if deterministic else 0
+        if index_scale:
+            flags |= int(index_scale) << 8  # RF_COMPILE_INDEX_SCALE
         if tuning is None:
             _ck(lib().rf_compile(self.text.encode(), flags, ctypes.byref(h)))
         else:
@@ -620,11 +623,12 @@ class Query:
 
 
 def compile_op(func: Func, vectorize: bool = False, deterministic: bool = False,
-               tuning: tuple | None = None) -> Query:
+               tuning: tuple | None = None, index_scale: int | None = None) -> Query:
     """compile_op<T>(func, vectorize) (include/reffine/utils/utils.h:47-54).
     `tuning` = (BLOCK, ITEMS, SUB[, WAVES_PER_CU[, PIPELINE[, LSUB]]]) overrides the
-    default launch shape."""
-    return Query(func, vectorize, deterministic, tuning)
+    default launch shape; `index_scale` (2..15) the slots per key of a group
+    join's hash index over sparse build keys."""
+    return Query(func, vectorize, deterministic, tuning, index_scale)
 
 
 def calib_copy(nbytes: int, iters: int = 20) -> float:

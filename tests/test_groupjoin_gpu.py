@@ -137,6 +137,59 @@ def test_group_join_keys_not_dense(device):
     _check(compile_op(workloads.group_join_c4())(B, P), b, p)
 
 
+def _sparse_join(seed, nb=3000, n=200_000):
+    rng = np.random.default_rng(seed)
+    keys = rng.permutation(np.unique(rng.integers(-2**62, 2**62, size=nb, dtype=np.int64)))
+    nb = len(keys)
+    b = {"bk": keys, "bd": np.zeros(nb, np.int32), "bf": rng.integers(0, 2, nb).astype(np.int8)}
+    pk = keys[rng.integers(0, nb, n)]
+    pk[::5] = rng.integers(-2**62, 2**62, size=len(pk[::5]), dtype=np.int64)
+    p = {"pk": pk, "pv": rng.random(n), "pq": rng.integers(1, 51, n).astype(np.int64)}
+    return b, p
+
+
+def test_group_join_index_scale(device):
+    """The hash index over sparse build keys at 2, 4 and 8 slots per key
+    gives the same bits in the deterministic mode (the load factor changes
+    where keys sit, not which row a key finds); scales below 2 are refused
+    (a full table would leave misses no empty slot to stop at)."""
+    b, p = _sparse_join(6)
+    B, P = _upload(b, p)
+    outs = []
+    for sc in (2, 4, 8):
+        q = compile_op(workloads.group_join_c4(), deterministic=True, index_scale=sc)
+        assert f"index scale {sc}" in q.describe()
+        out = q(B, P)
+        _check(out, b, p, det=True)
+        outs.append([out.column(k) for k in range(3)])
+    for o in outs[1:]:
+        assert all(np.array_equal(x.view(np.uint8), y.view(np.uint8)) for x, y in zip(o, outs[0]))
+    with pytest.raises(ReffineError):
+        compile_op(workloads.group_join_c4(), index_scale=1)
+
+
+def test_group_join_autotune_picks_index_scale(device, tmp_path, monkeypatch):
+    """The autotuner sweeps the index load factor of a group join at its
+    best launch shape and persists both; `tuned` compiles them again."""
+    import json
+
+    from reffine_amd import autotune
+    monkeypatch.setenv("RF_TUNING_CACHE", str(tmp_path / "tuning.json"))
+    monkeypatch.setattr(autotune, "_mem", {})
+    b, p = _sparse_join(7, nb=20_000, n=1_000_000)
+    B, P = _upload(b, p)
+    fn = workloads.group_join_c4()
+    best, times = autotune.autotune(fn, [B, P], cands=[(256, 2, 2, 16, 0, 0)], scales=(2, 8))
+    assert {k[1] for k in times} == {None, 2, 8}
+    _check(best(B, P), b, p)
+    e = next(iter(json.loads((tmp_path / "tuning.json").read_text()).values()))
+    assert e["index_scale"] in (None, 2, 8)
+    monkeypatch.setattr(autotune, "_mem", {})
+    again = autotune.tuned(fn, [B, P])
+    assert again.tuning() == best.tuning() and again.index_scale == e["index_scale"]
+    _check(again(B, P), b, p)
+
+
 def test_group_join_nulls(device):
     """Null build keys are not iterated; null probe keys join nothing
     (the validity of the iterator column, iter_space.cpp:60-66)."""

@@ -116,24 +116,34 @@ def test_c2_with_nulls(device):
     np.testing.assert_array_equal(sel.column(0), c["rowid"][m])
 
 
-def test_autotune_keeps_results(device):
+def test_autotune_keeps_results(device, tmp_path, monkeypatch):
     """The autotuner's choice (a few candidates on a 2M-row C2 table)
-    computes the same count, integer sum and selection as the default shape,
-    and is reused from its cache for the same shape of inputs."""
+    computes the same count, integer sum and selection as the default shape;
+    it is persisted per query shape and `tuned` compiles it again from the
+    file (another table of a similar row count shares it)."""
+    import json
+
     from reffine_amd import autotune
+    monkeypatch.setenv("RF_TUNING_CACHE", str(tmp_path / "tuning.json"))
+    monkeypatch.setattr(autotune, "_mem", {})
     n = 2_000_000
     c = datagen.scan_table(n, 31, with_q=True, rowid_dtype=np.int32)
     t = DeviceTable.from_columns([c["rowid"], c["k"], c["v"], c["d"], c["q"]], dim=1)
     fn = workloads.scan_select_c2()
     cands = [(256, 2, 4, 16, 0, 0), (256, 2, 16, 8, 1, 0), (256, 4, 8, 8, 1, 4)]
     best, times = autotune.autotune(fn, [t], cands=cands)
-    assert set(times) == set(cands) and all(ms > 0 for ms in times.values())
+    assert {k[0] for k in times} == set(cands) and all(ms > 0 for ms in times.values())
     (cnt, sq, _), sel = best(t)
     (cnt0, sq0, _), sel0 = compile_op(fn)(t)
     assert (cnt, sq) == (cnt0, sq0)
     assert np.array_equal(sel.column(0), sel0.column(0))
-    again, t2 = autotune.autotune(fn, [t])
-    assert t2 == {} and again.tuning() == best.tuning()
+    saved = json.loads((tmp_path / "tuning.json").read_text())
+    assert len(saved) == 1 and tuple(next(iter(saved.values()))["tuning"]) == best.tuning()
+    monkeypatch.setattr(autotune, "_mem", {})  # from the file
+    c2 = datagen.scan_table(n + 12345, 32, with_q=True, rowid_dtype=np.int32)
+    t2 = DeviceTable.from_columns([c2["rowid"], c2["k"], c2["v"], c2["d"], c2["q"]], dim=1)
+    again = autotune.tuned(fn, [t2])
+    assert again.tuning() == best.tuning()
 
 
 def _max_query(det_init: float, use_min: bool = False):
