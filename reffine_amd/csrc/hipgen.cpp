@@ -485,7 +485,17 @@ Tuning default_tuning(const Plan& plan) {
     // unpipelined on 30M rows.
     t.pipeline = nacc <= 16 ? 1 : 0;
     t.waves_per_cu = t.pipeline ? 8 : 16;
-    const int budget = t.pipeline ? (nacc <= 8 ? 640 : 320) : 320;  // load bytes per lane per tile
+    int budget = t.pipeline ? (nacc <= 8 ? 640 : 320) : 320;  // load bytes per lane per tile
+    if (t.pipeline && nacc <= 8) {
+        // A small state streams fastest with three waves per CU (one block
+        // of 192 threads, ~480 load bytes per lane per group, two register
+        // sets): on the 60M-row C2 scan 194-196 us against 212 us at four
+        // waves (256 threads) and 235 us at two; five and more waves per CU
+        // measured slower still (more concurrent streams, 281 us at eight).
+        t.block = 192;
+        t.waves_per_cu = 3;
+        budget = 480;
+    }
     t.sub = 4;
     if (row_bytes > 0) t.sub = std::max(1, std::min(16, budget / (t.items * row_bytes)));
     // At most 32 rows per lane per wave-tile: narrow columns would otherwise
@@ -496,6 +506,7 @@ Tuning default_tuning(const Plan& plan) {
         // bounds the scan. Few rows per lane and one register set; 8 waves
         // per CU (measured on the 60M-row probe of the join workload against
         // a direct map: 0.61 ms against 0.75 ms at 16 waves per CU).
+        t.block = 256;
         t.pipeline = 0;
         t.sub = std::max(1, 8 / t.items);
         t.waves_per_cu = 8;
