@@ -1303,7 +1303,12 @@ int bits_of(uint64_t range) {
 
 // Builds the resident build side (two host synchronisations: the group
 // count and key range, then the duplicate-key check).
-static std::shared_ptr<GjBuild> make_build(rf_query* q, const rf_arg* args, int64_t n_args) {
+static const char* kDupMsg =
+    "group join: the build table holds a qualifying key more than once "
+    "(each group key must occur in one build row)";
+
+static std::shared_ptr<GjBuild> make_build(rf_query* q, const rf_arg* args, int64_t n_args,
+                                           bool defer_check) {
     Device& d = device();
     hipStream_t s = d.stream;
     GroupJoinState& G = *q->gj;
@@ -1313,33 +1318,33 @@ static std::shared_ptr<GjBuild> make_build(rf_query* q, const rf_arg* args, int6
     uint64_t* small = (uint64_t*)G.small;  // [0..1] key range, [2] duplicate flag, [3] scratch
     auto B = std::make_shared<GjBuild>();
 
-    // 1. Build selection: (key, build row) of the qualifying build rows.
+    // 1. Build selection: (key, build row) of the qualifying build rows, and
+    // their key range folded in the same scan; the completing block writes
+    // the count and the range straight into pinned host memory.
     Launch lb = prepare(bq, args, n_args, nullptr);
     char* hs = (char*)bq->host_stage;  // pinned, allocated by prepare
-    dev_fill_u64(small, ~0ull, 1);
-    dev_fill_u64(small + 1, 0, 3);
+    if (!bq->host_stage_dev) RF_HIP(hipHostGetDevicePointer(&bq->host_stage_dev, bq->host_stage, 0));
+    dev_fill_u64(small + 2, 0, 2);
+    lb.args.result = bq->host_stage_dev;
+    lb.args.out_count = (int64_t*)((char*)bq->host_stage_dev + 4096);
     launch(bq, lb, s, false);
-    uint64_t mm[2] = {~0ull, 0};
-    int64_t K = 0;
+    int64_t K = 0, kmin = 0, kmax = 0;
     if (!lb.empty) {
-        const int64_t nb = args[gp.build].u.table->rows();
-        RF_HIP(aot_minmax_i64((const int64_t*)bq->outs[0], (const int64_t*)bq->out_count, nb, small,
-                              s));
-        RF_HIP(hipMemcpyAsync(hs, bq->out_count, 8, hipMemcpyDeviceToHost, s));
-        RF_HIP(hipMemcpyAsync(hs + 8, small, 16, hipMemcpyDeviceToHost, s));
         RF_HIP(hipStreamSynchronize(s));
-        K = *(int64_t*)hs;
-        std::memcpy(mm, hs + 8, 16);
+        K = *(int64_t*)(hs + 4096);
+        const KernelSource& bks = lb.spec->ks;
+        for (size_t a = 0; a < bks.acc.size(); a++) {
+            int64_t v;
+            std::memcpy(&v, hs + bks.acc_offset[a], 8);
+            if (bks.acc[a].op == AggOp::MIN) kmin = v;
+            else if (bks.acc[a].op == AggOp::MAX) kmax = v;
+        }
     }
     if (K == 0) return B;
     if (K >= ((int64_t)1 << 31)) throw Unsupported("more than 2^31 groups");
-    const int64_t kmin = (int64_t)(mm[0] ^ 0x8000000000000000ull);
-    const int64_t kmax = (int64_t)(mm[1] ^ 0x8000000000000000ull);
 
     const uint64_t range = (uint64_t)kmax - (uint64_t)kmin + 1;
-    const char* dup_msg =
-        "group join: the build table holds a qualifying key more than once "
-        "(each group key must occur in one build row)";
+    const char* dup_msg = kDupMsg;
 
     // 2-3 with dense keys: a rank map (a bit per key of the range and a
     // count per 64 bits: 1.5 bits per slot) when the key range is dense --
@@ -1357,6 +1362,29 @@ static std::shared_ptr<GjBuild> make_build(rf_query* q, const rf_arg* args, int6
         B->keys = dev_alloc((size_t)K * 8);
         B->rows = dev_alloc((size_t)K * 8);
         dev_fill_u64(B->dense_bits, 0, words);
+        if (words <= kRankFusedWords) {
+            // Marking, one-block counts + packed form, placement by rank:
+            // three launches. The count of marked keys goes to pinned host
+            // memory; with defer_check the caller reads it after its own
+            // synchronisation.
+            B->dense_pack = dev_alloc((size_t)words * 16);
+            if (!bq->host_stage_dev)
+                RF_HIP(hipHostGetDevicePointer(&bq->host_stage_dev, bq->host_stage, 0));
+            uint64_t* total_dev = (uint64_t*)((char*)bq->host_stage_dev + 6144);
+            RF_HIP(aot_rank_build_small((const int64_t*)bq->outs[0], K, kmin, B->dense_bits, words,
+                                        B->dense_cum, B->dense_pack, total_dev, s));
+            RF_HIP(aot_rank_place((const int64_t*)bq->outs[0], (const int64_t*)bq->outs[1], K, kmin,
+                                  B->dense_bits, B->dense_cum, (int64_t*)B->keys,
+                                  (int64_t*)B->rows, s));
+            B->groups = K;
+            B->kmin = kmin;
+            B->pending_total = (const volatile uint64_t*)(hs + 6144);
+            if (!defer_check) {
+                RF_HIP(hipStreamSynchronize(s));
+                gj_verify(q, *B);
+            }
+            return B;
+        }
         RF_HIP(aot_rank_build((const int64_t*)bq->outs[0], K, kmin, B->dense_bits, words,
                               B->dense_cum, s));
         RF_HIP(aot_rank_scatter((const int64_t*)bq->outs[0], (const int64_t*)bq->outs[1], K, kmin,
@@ -1421,7 +1449,20 @@ static std::shared_ptr<GjBuild> make_build(rf_query* q, const rf_arg* args, int6
     return B;
 }
 
-std::shared_ptr<GjBuild> gj_build(rf_query* q, const rf_arg* args, int64_t n_args) {
+void gj_verify(rf_query* q, GjBuild& b) {
+    if (!b.pending_total) return;
+    const uint64_t total = *b.pending_total;
+    b.pending_total = nullptr;
+    if (total != (uint64_t)b.groups) {
+        // The caller holds q->mu (gj_clear_cache would take it again).
+        q->gj->builds.clear();
+        q->gj->build_order.clear();
+        throw IRError(kDupMsg);
+    }
+}
+
+std::shared_ptr<GjBuild> gj_build(rf_query* q, const rf_arg* args, int64_t n_args,
+                                  bool defer_check) {
     GroupJoinState& G = *q->gj;
     const GroupJoinPlan& gp = *q->plan.gj;
     // The build side depends only on the build table and the build plan's
@@ -1437,8 +1478,15 @@ std::shared_ptr<GjBuild> gj_build(rf_query* q, const rf_arg* args, int64_t n_arg
             }
     }
     auto it = G.builds.find(bkey);
-    if (it != G.builds.end()) return it->second;
-    std::shared_ptr<GjBuild> B = make_build(q, args, n_args);
+    if (it != G.builds.end()) {
+        if (it->second->pending_total) {  // a run that built it ended early
+            RF_HIP(hipStreamSynchronize(device().stream));
+            std::shared_ptr<GjBuild> B = it->second;
+            gj_verify(q, *B);
+        }
+        return it->second;
+    }
+    std::shared_ptr<GjBuild> B = make_build(q, args, n_args, defer_check);
     if (G.build_order.size() >= GroupJoinState::kBuilds) {
         G.builds.erase(G.build_order.front());
         G.build_order.erase(G.build_order.begin());
@@ -1595,9 +1643,20 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
     GroupJoinState& G = *q->gj;
     const GroupJoinPlan& gp = *q->plan.gj;
     check_args(q->plan, args, n_args);
-    const std::shared_ptr<GjBuild> B = gj_build(q, args, n_args);
+    // A new dense build's duplicate-key check is read after this run's last
+    // synchronisation (no synchronisation between the build and the probe).
+    const std::shared_ptr<GjBuild> B = gj_build(q, args, n_args, true);
     const int64_t K = B->groups;
     if (K == 0) return alloc_table(gp.out_types, 0, 1);
+    auto verified = [&](rf_table* t) {
+        try {
+            gj_verify(q, *B);
+        } catch (...) {
+            table_free(t);
+            throw;
+        }
+        return t;
+    };
     rf_query* pq = gj_probe_query(q, *B);
     const Spec& fs = gj_fold_spec(q, args, n_args);
     const RfProbeDev h = B->probe();
@@ -1612,11 +1671,12 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
     if (tm) RF_HIP(hipEventRecord(tm->scan1, s));
     if (!q->det && fs.mod->agg && lp.spec->ks.stage_types == lp.spec->ks.out_types) {
         rf_table* t = group_agg(q, fs, *B, args, n_args, pq, lp);
-        if (t) return t;
+        if (t) return verified(t);
     }
     launch_gather(lp, s);
     RF_HIP(hipMemcpyAsync(hs, pq->out_count, 8, hipMemcpyDeviceToHost, s));
     RF_HIP(hipStreamSynchronize(s));
+    gj_verify(q, *B);
     const int64_t J = lp.empty ? 0 : *(int64_t*)hs;
 
     // 5. Stable sort of the joined rows by rank.

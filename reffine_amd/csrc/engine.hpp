@@ -60,6 +60,12 @@ hipError_t aot_minmax_i64(const int64_t* keys, const int64_t* n_dev, int64_t n_m
 hipError_t aot_adjacent_dups(const int64_t* keys, int64_t n, void* flag, hipStream_t s);
 hipError_t aot_not_ascending(const int64_t* keys, int64_t n, void* flag, hipStream_t s);
 int64_t aot_scan_entries(int64_t m);
+constexpr int64_t kRankFusedWords = int64_t(1) << 18;
+hipError_t aot_rank_build_small(const int64_t* keys, int64_t n, int64_t base, void* bits,
+                                int64_t words, void* cum, void* pack, void* total, hipStream_t s);
+hipError_t aot_rank_place(const int64_t* keys, const int64_t* rows, int64_t n, int64_t base,
+                          const void* bits, const void* cum, int64_t* keys_out, int64_t* rows_out,
+                          hipStream_t s);
 hipError_t aot_merge_splits(const int64_t* a, int64_t na, const int64_t* b, int64_t nb,
                             int64_t tile, int64_t n, int64_t* out, hipStream_t s);
 hipError_t aot_scan_u32(void* h, int64_t m, hipStream_t s);
@@ -285,6 +291,10 @@ struct GjBuild {
     void* dense_cum = nullptr;
     void* dense_pack = nullptr;  // bits and counts interleaved, for probes
     uint64_t dense_len = 0;
+    // Duplicate-key check not yet read: the count of marked keys, written by
+    // the build into pinned host memory, must equal `groups` once the
+    // stream has been synchronised (then the build is usable).
+    const volatile uint64_t* pending_total = nullptr;
     RfProbeDev probe() const;  // the probe plan's index for the external probes
     ~GjBuild();
 };
@@ -326,7 +336,13 @@ struct GroupJoinState {
 
 // The build side of group join `q` for these arguments: from the cache, or
 // built on the library stream (synchronous). Caller holds q->mu.
-std::shared_ptr<GjBuild> gj_build(rf_query* q, const rf_arg* args, int64_t n_args);
+// defer_check: a dense build's duplicate-key check is left for the caller's
+// own synchronisation (gj_verify after it) instead of a synchronisation here.
+std::shared_ptr<GjBuild> gj_build(rf_query* q, const rf_arg* args, int64_t n_args,
+                                  bool defer_check = false);
+// After the stream is synchronised, with q->mu held: throws (and drops the
+// cached builds) when a deferred duplicate-key check failed.
+void gj_verify(rf_query* q, GjBuild& b);
 // The probe sub-query for a build side (direct map or hash index).
 rf_query* gj_probe_query(rf_query* q, const GjBuild& b);
 // Drops the cached build sides (in-flight batch rounds keep theirs).

@@ -559,6 +559,44 @@ extern "C" __global__ void rf_rank_total(const uint64_t* __restrict__ bits,
         *out = (uint64_t)cum[words - 1] + (uint64_t)__popcll(bits[words - 1]);
 }
 
+// A rank map's counts in one launch (one block of 1024 threads, rounds of
+// 1024 words): cum[w] = marked keys in words before w, the packed probe form
+// pack[2w] = bits[w], pack[2w + 1] = cum[w], and *total = all marked keys.
+extern "C" __global__ __launch_bounds__(1024) void rf_rank_finish(
+    const uint64_t* __restrict__ bits, int64_t words, uint32_t* __restrict__ cum,
+    unsigned long long* __restrict__ pack, uint64_t* total) {
+    __shared__ uint32_t s_w[16];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    uint32_t base = 0;
+    for (int64_t r0 = 0; r0 < words; r0 += 1024) {
+        const int64_t i = r0 + t;
+        const uint64_t b = i < words ? bits[i] : 0ull;
+        const uint32_t c = (uint32_t)__popcll(b);
+        uint32_t inc = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += y;
+        }
+        if (lane == 63) s_w[wv] = inc;
+        __syncthreads();
+        uint32_t wb = 0, tot = 0;
+        for (int k = 0; k < 16; k++) {
+            const uint32_t v = s_w[k];
+            if (k < wv) wb += v;
+            tot += v;
+        }
+        const uint32_t ex = base + wb + inc - c;
+        if (i < words) {
+            cum[i] = ex;
+            pack[2 * i] = b;
+            pack[2 * i + 1] = ex;
+        }
+        base += tot;
+        __syncthreads();  // s_w is rewritten next round
+    }
+    if (t == 0) *total = base;
+}
+
 extern "C" __global__ void rf_word_popc(const uint64_t* __restrict__ bits, int64_t words,
                                         uint32_t* cnt) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < words;
@@ -980,6 +1018,32 @@ hipError_t aot_rank_build(const int64_t* keys, int64_t n, int64_t base, void* bi
     hipLaunchKernelGGL(rf_scan_chunks, dim3((unsigned)nch), dim3(256), 0, s, c, words, part);
     hipLaunchKernelGGL(rf_sort_scan, dim3(1), dim3(1024), 0, s, part, nch);
     hipLaunchKernelGGL(rf_scan_apply, dim3((unsigned)nch), dim3(256), 0, s, c, words, part);
+    return hipGetLastError();
+}
+
+// The rank map of n keys in words zeroed words, counts and packed form in
+// two launches (marking, then rf_rank_finish): for maps of at most
+// kRankFusedWords words, where one block scans them quickly. *total gets
+// the number of marked keys.
+hipError_t aot_rank_build_small(const int64_t* keys, int64_t n, int64_t base, void* bits,
+                                int64_t words, void* cum, void* pack, void* total,
+                                hipStream_t s) {
+    if (words <= 0) return hipSuccess;
+    if (n > 0)
+        hipLaunchKernelGGL(rf_rank_bits, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, n,
+                           base, (unsigned long long*)bits);
+    hipLaunchKernelGGL(rf_rank_finish, dim3(1), dim3(1024), 0, s, (const uint64_t*)bits, words,
+                       (uint32_t*)cum, (unsigned long long*)pack, (uint64_t*)total);
+    return hipGetLastError();
+}
+
+// Places (key, row) at each key's rank (rf_rank_scatter alone).
+hipError_t aot_rank_place(const int64_t* keys, const int64_t* rows, int64_t n, int64_t base,
+                          const void* bits, const void* cum, int64_t* keys_out, int64_t* rows_out,
+                          hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rf_rank_scatter, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, rows, n,
+                       base, (const uint64_t*)bits, (const uint32_t*)cum, keys_out, rows_out);
     return hipGetLastError();
 }
 

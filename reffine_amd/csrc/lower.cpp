@@ -882,6 +882,24 @@ struct Lowerer {
         bo.types = {types::I64, types::I64};
         bo.vec_type = Type(BT::VECTOR, {types::I64, types::I64}, 1);
         Lb.plan.ops = {bo};
+        // The qualifying keys' range, folded in the same scan (MIN, MAX of
+        // the key under the selection's predicate): the rank map is sized
+        // from it.
+        ReduceComp kr;
+        kr.sym_name = name + ".keyrange";
+        kr.state_type = Type(BT::STRUCT, {types::I64, types::I64});
+        for (AggOp op : {AggOp::MIN, AggOp::MAX}) {
+            Field f;
+            f.type = types::I64;
+            f.op = op;
+            f.updated = true;
+            f.pred = sb.pred;
+            f.contrib = Lb.col_leaf(B, 0);
+            f.init = mk_const(types::I64, 0);
+            f.path = {(int)kr.fields.size()};
+            kr.fields.push_back(f);
+        }
+        Lb.plan.reduces = {kr};
         Lb.plan.func_name = func->name + "_build";
         Lb.plan.output_type = bo.vec_type;
         gj->build = B;
