@@ -560,28 +560,32 @@ extern "C" __global__ void rf_rank_total(const uint64_t* __restrict__ bits,
 }
 
 // A rank map's counts in one launch (one block of 1024 threads, rounds of
-// 1024 words): cum[w] = marked keys in words before w, the packed probe form
+// 1024 words, the next round's words loaded before this round's scan):
+// cum[w] = marked keys in words before w, the packed probe form
 // pack[2w] = bits[w], pack[2w + 1] = cum[w], and *total = all marked keys.
 extern "C" __global__ __launch_bounds__(1024) void rf_rank_finish(
     const uint64_t* __restrict__ bits, int64_t words, uint32_t* __restrict__ cum,
     unsigned long long* __restrict__ pack, uint64_t* total) {
-    __shared__ uint32_t s_w[16];
+    __shared__ uint32_t s_w[2][16];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     uint32_t base = 0;
+    uint64_t nb = t < words ? bits[t] : 0ull;
+    int par = 0;
     for (int64_t r0 = 0; r0 < words; r0 += 1024) {
         const int64_t i = r0 + t;
-        const uint64_t b = i < words ? bits[i] : 0ull;
+        const uint64_t b = nb;
+        nb = i + 1024 < words ? bits[i + 1024] : 0ull;
         const uint32_t c = (uint32_t)__popcll(b);
         uint32_t inc = c;
         for (int o = 1; o < 64; o <<= 1) {
             const uint32_t y = __shfl_up(inc, o, 64);
             if (lane >= o) inc += y;
         }
-        if (lane == 63) s_w[wv] = inc;
+        if (lane == 63) s_w[par][wv] = inc;
         __syncthreads();
         uint32_t wb = 0, tot = 0;
         for (int k = 0; k < 16; k++) {
-            const uint32_t v = s_w[k];
+            const uint32_t v = s_w[par][k];
             if (k < wv) wb += v;
             tot += v;
         }
@@ -592,7 +596,7 @@ extern "C" __global__ __launch_bounds__(1024) void rf_rank_finish(
             pack[2 * i + 1] = ex;
         }
         base += tot;
-        __syncthreads();  // s_w is rewritten next round
+        par ^= 1;  // the next round writes the other slots: one barrier a round
     }
     if (t == 0) *total = base;
 }
