@@ -379,6 +379,14 @@ def bench_tpch(args) -> int:
            "orders": DeviceTable.from_arrow(rb["orders"]).build_index(),
            "customer": DeviceTable.from_arrow(rb["customer"]).build_index()}
     del rb
+    if args.counter_pass:
+        from reffine_amd import runtime
+        for name in ("q3", "q4", "q12", "q18"):
+            compile_op(tpch.QUERIES[name]())(*[dev[x] for x in tpch.ARGS[name]])
+        runtime.lib().rf_sync()
+        runtime.calib_copy(1 << 30, 3)
+        log("counter pass done (Q3, Q4, Q12, Q18 once each)")
+        return 0
     per = {}
     tot_bytes, tot_ms, ok_all = 0, 0.0, True
     fold_best = None
@@ -629,6 +637,16 @@ def c5_setup(n_queries: int, seed: int = 5, queries=None):
 def bench_c5(args) -> int:
     """Batched/served form against plain per-query launches: closed-loop
     throughput of both, then latency at a fixed Poisson arrival rate."""
+    if args.counter_pass:
+        from reffine_amd import batch, runtime
+        _, _, _, work, _ = c5_setup(args.queries)
+        reqs = [(q, inp, 0.0) for q, inp in work]
+        batch.serve(reqs)  # first pass: compiles, build sides, runner buffers
+        batch.serve(reqs)
+        runtime.lib().rf_sync()
+        runtime.calib_copy(1 << 30, 3)
+        log("counter pass done (two batched closed-loop passes)")
+        return 0
     out = c5_measure(args.queries)
     ok = out["config"]["identical_results"]
     if args.tune:
