@@ -179,11 +179,33 @@ static std::unique_ptr<rf_query> make_query(const rf::N& f0, uint32_t flags,
     std::vector<rf::Prelude> pre;
     rf::N f = rf::split_materialized(f0, pre);
     auto q = std::make_unique<rf_query>();
-    q->plan = rf::lower_func(f, false);
+    std::vector<rf::N> parts;
+    try {
+        q->plan = rf::lower_func(f, false);
+    } catch (const rf::Unsupported&) {
+        // Components that cannot share one scan run as queries of their own.
+        parts = rf::split_components(f);
+        if (parts.size() < 2) throw;
+        q->plan = rf::Plan();
+        q->plan.composite = true;
+        q->plan.func_name = f->name;
+        q->plan.inputs.assign(f->args.begin() + 1, f->args.end());
+        q->plan.tbl = f->tbl;
+        q->plan.output = f->args[0];
+        q->plan.output_type = f->args[0]->type;
+    }
     q->n_user_args = (int64_t)f0->args.size() - 1;
     for (auto& p : pre) {
         q->pre.push_back(make_query(p.func, flags, nullptr, depth + 1));
         q->pre_index.push_back(p.index);
+    }
+    if (q->plan.composite) {
+        for (auto& pf : parts) {
+            q->parts.push_back(make_query(pf, flags, nullptr, depth + 1));
+            q->part_names.push_back(pf->args[0]->name);
+        }
+        q->det = (flags & RF_COMPILE_DETERMINISTIC) != 0;
+        return q;
     }
     q->det = (flags & RF_COMPILE_DETERMINISTIC) != 0;
     q->tune = rf::default_tuning(q->plan, q->det);
@@ -277,7 +299,17 @@ int rf_compile_check(const char* ir_text, uint32_t flags, char* buf, size_t cap)
                         (p.index ? " with an index" : "") + ": " + rf::to_text(p.func->args[0]) +
                         "\n";
             }
-            rf::Plan plan = rf::lower_func(f, false);
+            rf::Plan plan;
+            try {
+                plan = rf::lower_func(f, false);
+            } catch (const rf::Unsupported&) {
+                std::vector<rf::N> parts = rf::split_components(f);
+                if (parts.size() < 2) throw;
+                desc += "func " + f->name + ": independent components, each a query of its own, "
+                        "run one after another\n";
+                for (auto& pf : parts) add(pf, depth + 1);
+                return;
+            }
             rf::Tuning tune = rf::default_tuning(plan, (flags & RF_COMPILE_DETERMINISTIC) != 0);
             std::vector<std::vector<bool>> nl(plan.inputs.size());
             for (size_t i = 0; i < plan.inputs.size(); i++)
@@ -346,6 +378,8 @@ int rf_query_describe(const rf_query* q, char* buf, size_t cap) {
     return guard([&] {
         need(q && buf && cap, "NULL argument");
         std::string s = q->plan.describe();
+        for (size_t i = 0; i < q->parts.size(); i++)
+            s += "component " + q->part_names[i] + ":\n" + q->parts[i]->plan.describe();
         auto tn = q->tune;
         s += "tuning: block " + std::to_string(tn.block) + " items " + std::to_string(tn.items) +
              " sub " + std::to_string(tn.sub) + " waves/CU " + std::to_string(tn.waves_per_cu) +

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <set>
 #include <sstream>
 
 namespace rf {
@@ -756,6 +757,29 @@ void write_prim(const HostVal& v, const Type& t, char* p) {
         case BT::FLOAT64: *(double*)p = c.f; break;
         default: throw IRError("bad primitive");
     }
+}
+
+HostVal read_value(const Type& t, const char* src) {
+    if (t.is_struct()) {
+        HostVal v;
+        v.type = t;
+        size_t off = 0;
+        for (size_t i = 0; i < t.dtypes.size(); i++) {
+            size_t sz, al;
+            c_layout(t.dtypes[i], sz, al);
+            off = (off + al - 1) / al * al;
+            v.fields.push_back(read_value(t.dtypes[i], src + off));
+            off += sz;
+        }
+        return v;
+    }
+    if (t.is_vector()) {
+        HostVal v;
+        v.type = t;
+        v.table = *(void* const*)src;
+        return v;
+    }
+    return read_prim(t, src);
 }
 
 void write_value(const HostVal& v, const Type& t, char* dst) {
@@ -1911,6 +1935,59 @@ static void run_preludes(rf_query* q, const rf_arg* args, int64_t n_args,
     }
 }
 
+// A composite query (Plan::composite): each component's query on the same
+// arguments, then the output over their results. Tables the output does not
+// hold are freed.
+static void run_composite(rf_query* q, const rf_arg* args, int64_t n_args, void* output_addr,
+                          rf_table** out_table) {
+    const Plan& plan = q->plan;
+    EvalCtx c{&plan, args, n_args, {}, {}};
+    for (auto& kv : plan.tbl) c.tbl[kv.first->name] = kv.second;
+    std::vector<rf_table*> tables;
+    try {
+        for (size_t i = 0; i < q->parts.size(); i++) {
+            rf_query* part = q->parts[i].get();
+            const Type& ty = part->plan.output_type;
+            if (ty.is_vector()) {
+                rf_table* t = nullptr;
+                run_query(part, args, n_args, nullptr, &t);
+                tables.push_back(t);
+                HostVal hv;
+                hv.type = ty;
+                hv.table = t;
+                c.comps[q->part_names[i]] = hv;
+            } else {
+                size_t sz = 0, al = 0;
+                c_layout(ty, sz, al);
+                std::vector<char> buf(std::max<size_t>(sz, 8));
+                run_query(part, args, n_args, buf.data(), nullptr);
+                c.comps[q->part_names[i]] = read_value(ty, buf.data());
+            }
+        }
+        HostVal out = eval(plan.output, c);
+        std::set<void*> kept;
+        std::function<void(const HostVal&)> held = [&](const HostVal& v) {
+            if (v.table) kept.insert(v.table);
+            for (auto& f : v.fields) held(f);
+        };
+        held(out);
+        if (plan.output_type.is_vector()) {
+            if (!out_table) throw IRError("an Op output needs out_table");
+            *out_table = (rf_table*)out.table;
+            if (output_addr) *(void**)output_addr = out.table;
+        } else if (output_addr) {
+            write_value(out, plan.output_type, (char*)output_addr);
+        } else {
+            throw IRError("no output address");
+        }
+        for (rf_table* t : tables)
+            if (!kept.count(t)) table_free(t);
+    } catch (...) {
+        for (rf_table* t : tables) table_free(t);
+        throw;
+    }
+}
+
 void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_addr,
                rf_table** out_table) {
     std::vector<std::unique_ptr<rf_table>> mats;
@@ -1919,6 +1996,10 @@ void run_query(rf_query* q, const rf_arg* args, int64_t n_args, void* output_add
         run_preludes(q, args, n_args, mats, full);
         args = full.data();
         n_args = (int64_t)full.size();
+    }
+    if (q->plan.composite) {
+        run_composite(q, args, n_args, output_addr, out_table);
+        return;
     }
     std::lock_guard<std::mutex> lk(q->mu);
     // A run stage first folds T's runs; the plan then scans that table in
@@ -2043,7 +2124,7 @@ void time_query(rf_query* q, const rf_arg* args, int64_t n_args, int iters, doub
                 double* ms_scan) {
     Device& d = device();
     *ms_all = *ms_scan = 0.0;
-    if (!q->pre.empty()) {
+    if (!q->pre.empty() || q->plan.composite) {
         // Whole runs, preludes included (each run synchronises).
         if (iters <= 0) return;
         hipEvent_t e[2];

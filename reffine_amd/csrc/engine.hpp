@@ -187,6 +187,10 @@ struct rf_query {
     // their results appended to the arguments (indexed where pre_index).
     std::vector<std::unique_ptr<rf_query>> pre;
     std::vector<bool> pre_index;
+    // Plan::composite: one query per component (named by its symbol), run
+    // on the same arguments; the output is evaluated over their results.
+    std::vector<std::unique_ptr<rf_query>> parts;
+    std::vector<std::string> part_names;
     int64_t n_user_args = -1;
     // Workspace (grow-only, reused across runs).
     void* partials = nullptr;

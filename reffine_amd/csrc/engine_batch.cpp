@@ -160,7 +160,7 @@ static void plan_unit(rf_batch::Unit& u, rf_query* q, const rf_arg* args, int64_
     u.q = q;
     u.args.assign(args, args + n_args);
     if (q->plan.seg || q->plan.runs || !q->pre.empty() || q->plan.sequential ||
-        q->plan.source == Plan::MERGE) {
+        q->plan.source == Plan::MERGE || q->plan.composite) {
         // Run folds, computed vectors, sequential folds and union merges are
         // not batch kernels: the unit runs on the plain path when added
         // (batch_add).

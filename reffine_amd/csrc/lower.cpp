@@ -1538,6 +1538,41 @@ N split_materialized(const N& func, std::vector<Prelude>& pre) {
     return mk_func(func->name, func->args[0], in, main_tbl);
 }
 
+std::vector<N> output_components(const N& func) {
+    if (!func || func->kind != K::Func) throw IRError("expected a Func");
+    std::map<std::string, N> tbl, tsym;
+    for (auto& kv : func->tbl) {
+        tbl[kv.first->name] = kv.second;
+        tsym[kv.first->name] = kv.first;
+    }
+    std::vector<N> comps;
+    std::set<std::string> seen;
+    std::function<void(const N&)> walk = [&](const N& e) {
+        if (e->kind == K::Sym) {
+            auto t = tbl.find(e->name);
+            if (t == tbl.end()) return;
+            N d = strip_initval(t->second);
+            if (d->kind == K::Reduce || d->kind == K::Op) {
+                if (seen.insert(e->name).second) comps.push_back(tsym[e->name]);
+                return;
+            }
+            walk(t->second);
+            return;
+        }
+        for (auto& a : e->args) walk(a);
+    };
+    walk(func->args[0]);
+    return comps;
+}
+
+std::vector<N> split_components(const N& func) {
+    std::vector<N> in(func->args.begin() + 1, func->args.end());
+    std::vector<N> out;
+    for (const N& c : output_components(func))
+        out.push_back(mk_func(func->name + "." + c->name, c, in, func->tbl));
+    return out;
+}
+
 Plan lower_func(const N& func, bool vectorize) {
     if (!func || func->kind != K::Func) throw IRError("expected a Func");
     Lowerer L;
@@ -1560,6 +1595,11 @@ static const char* aggop_name(AggOp o) {
 
 std::string Plan::describe() const {
     std::ostringstream o;
+    if (composite) {
+        o << "func " << func_name << ": independent components, each a query of its own, "
+          << "run one after another; output " << to_text(output) << "\n";
+        return o.str();
+    }
     if (source == MERGE)
         o << "func " << func_name << ": union merge of table inputs " << merge_a << " and "
           << merge_b << " (keys ascending), merge positions [" << to_text(start) << ", "

@@ -107,6 +107,10 @@ struct Plan {
     // The function is one sequential fold (ReduceComp::sequential): a
     // single wave folds the rows in order (rf_qs).
     bool sequential = false;
+    // Independent components run as queries of their own
+    // (split_components); this plan holds only the inputs, the symbol
+    // table and the output expression over the components.
+    bool composite = false;
     std::shared_ptr<GroupJoinPlan> gj;  // set: the function is a group join
     std::shared_ptr<SegReducePlan> seg; // set: a reduction per run of a run-end column
     std::shared_ptr<RunStagePlan> runs; // set: a runs table is built first (T's place)
@@ -166,6 +170,17 @@ N split_materialized(const N& func, std::vector<Prelude>& pre);
 // Lowers `func` (a parsed IR Func) into a plan; throws Unsupported for
 // shapes the kernels do not cover and IRError for ill-formed IR.
 Plan lower_func(const N& func, bool vectorize);
+
+// The components (Reduce / Op symbols, through the symbol table) the
+// function's output combines, in the order the output names them.
+std::vector<N> output_components(const N& func);
+
+// A function whose components cannot share one scan (they read different
+// tables, or one is a sequential fold): one function per component, each
+// with that component as its output, on the same inputs and symbol table.
+// The reference emits each component as a loop of its own
+// (src/pass/loopgen.cpp:113-269), one after another.
+std::vector<N> split_components(const N& func);
 
 // Evaluation of host-side expressions (bounds, init values, the func output
 // over reduction results).

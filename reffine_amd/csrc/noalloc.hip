@@ -48,7 +48,7 @@ static size_t up256(size_t x) { return (x + 255) & ~size_t(255); }
 void run_query_noalloc(rf_query* q, const rf_arg* args, int64_t n_args, void* output_addr,
                        void* const* out_cols, int64_t out_cap, int64_t* out_rows) {
     if (!q->pre.empty() || q->plan.runs || q->plan.seg || q->gj ||
-        q->plan.source == Plan::MERGE)
+        q->plan.source == Plan::MERGE || q->plan.composite)
         throw Unsupported("the no-allocation run takes scan queries only (reductions and one "
                           "selection over one table): not group joins, run folds, union merges "
                           "or computed vectors");

@@ -79,3 +79,62 @@ def loop(name, k, v, valid=None):
 
 
 BUILDERS = {"decay": decay, "poly": poly_hash, "maxv": max_state_second, "pair": running_pair}
+
+
+def _red_sum(T, col):
+    return _red(_subvec(T, _idx(0), _len(T, 1)), lambda: _f64(0.0),
+                lambda s, r: _add(s, _get(r, col)))
+
+
+def sum_and_count_of_two_tables():
+    """{SUM(T.v), COUNT(U)}: two reductions over different tables (each its
+    own query: they cannot share a scan)."""
+    T = _sym("T", SEQ_T)
+    U = _sym("U", _vec_t(2, _i64_t, _f64_t))
+    a = _red_sum(T, 1)
+    b = _red(_subvec(U, _idx(0), _len(U, 1)), lambda: _i64(0),
+             lambda s, r: _add(s, _i64(1)))
+    sa, sb = _sym("sa", a), _sym("sb", b)
+    out = _new([sa, sb])
+    so = _sym("out", out)
+    fn = _func("two_tables", so, [T, U])
+    fn.tbl[sa] = a
+    fn.tbl[sb] = b
+    fn.tbl[so] = out
+    return fn
+
+
+def decay_and_sum():
+    """{decayed sum (sequential), SUM(v)} over one table: a sequential fold
+    beside an associative one."""
+    T = _sym("T", SEQ_T)
+    d = _red(_subvec(T, _idx(0), _len(T, 1)), lambda: _f64(0.25),
+             lambda s, r: _add(_mul(s, _f64(0.5)), _get(r, 1)))
+    a = _red_sum(T, 1)
+    sd, sa = _sym("sd", d), _sym("sa", a)
+    out = _new([sd, sa])
+    so = _sym("out", out)
+    fn = _func("decay_and_sum", so, [T])
+    fn.tbl[sd] = d
+    fn.tbl[sa] = a
+    fn.tbl[so] = out
+    return fn
+
+
+def selection_and_decay():
+    """{rows with v > 0.9 (an Op), decayed sum (sequential)} over one table."""
+    from reffine_amd.reffiner import _op, _in
+    T = _sym("T1", _vec_t(1, _i64_t, _i64_t, _f64_t))
+    t = _sym("t", _i64_t)
+    op = _op([t], _in(t, T) & _gt(T[t][1], _f64(0.9)), [T[t][1]])
+    t2 = _sym("t2", _i64_t)
+    rows = _op([t2], _in(t2, T), [T[t2][1]])
+    d = _red(rows, lambda: _f64(0.25), lambda s, r: _add(_mul(s, _f64(0.5)), _get(r, 1)))
+    so_, sd = _sym("sel", op), _sym("sd", d)
+    out = _new([so_, sd])
+    so = _sym("out", out)
+    fn = _func("selection_and_decay", so, [T])
+    fn.tbl[so_] = op
+    fn.tbl[sd] = d
+    fn.tbl[so] = out
+    return fn
