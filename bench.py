@@ -475,8 +475,9 @@ def bench_tpch(args) -> int:
 def bench_programs(args) -> int:
     """reffine's PageRank step (misc.h:212-291; a materialised, indexed
     contribution table and a run fold over in-edges), Nbody step (misc.h:63-210;
-    pair table, ranged window folds, indexed reads) and TPC-DS Q9 (tpch.h:
-    218-313; a 5-bucket SubVector fold), each verified at size."""
+    pair table, ranged window folds, indexed reads), TPC-DS Q9 (tpch.h:
+    218-313; a 5-bucket SubVector fold) and AlgoTrading (misc.h:15-61; a union
+    merge with a shifted side), each verified at size."""
     import math
     from oracle import graph as og
     from oracle import tpch as ot
@@ -542,11 +543,31 @@ def bench_programs(args) -> int:
                        "kernel_gbs": round(by / (scan_ms * 1e-3) / 1e9, 2), "verified": bool(ok)}
     log(f"tpcds_q9: {per['tpcds_q9']}")
     tot_bytes, tot_ms, ok_all = tot_bytes + by, tot_ms + ms, ok_all and ok
-    cfg = {"workload": "reffine's PageRank step (2M nodes), Nbody step (2048 bodies) and "
-                       "TPC-DS Q9 (30M store_sales rows), as written; value: algorithmic bytes of "
-                       "PageRank and Q9 / the three programs' summed wall time",
+    # AlgoTrading: the 1M-row stock_price table merged with itself shifted by -50.
+    from oracle import merge
+    from reffine_amd import workloads
+    na = 1_000_000
+    log(f"algotrading: {na} rows")
+    rng = np.random.default_rng(args.seed)
+    ts = np.arange(na, dtype=np.int64)
+    val = (rng.standard_normal(na) * 10 + 100).astype(np.float32)
+    S = DeviceTable.from_columns([ts, val])
+    q = compile_op(workloads.algotrading())
+    out, ms = _timed(lambda: q(S, S), args.steps, args.warmup)
+    t_, d_ = merge.algotrading(ts, val, ts, val)
+    ok = np.array_equal(out.column(0), t_) and np.array_equal(out.column(1).view(np.uint32),
+                                                              d_.view(np.uint32))
+    by = 24 * na + 12 * len(t_)
+    per["algotrading"] = {"ms": round(ms, 4), "rows": na, "out_rows": len(t_),
+                          "gbs": round(by / (ms * 1e-3) / 1e9, 2), "verified": bool(ok)}
+    log(f"algotrading: {per['algotrading']}")
+    tot_bytes, tot_ms, ok_all = tot_bytes + by, tot_ms + ms, ok_all and ok
+    cfg = {"workload": "reffine's PageRank step (2M nodes), Nbody step (2048 bodies), "
+                       "TPC-DS Q9 (30M store_sales rows) and AlgoTrading (1M prices), as written; "
+                       "value: algorithmic bytes of PageRank, Q9 and AlgoTrading / the four "
+                       "programs' summed wall time",
            "programs": per, "verified": bool(ok_all)}
-    line = _line(cfg, tot_bytes / (tot_ms * 1e-3) / 1e9, tot_ms / 3, args,
+    line = _line(cfg, tot_bytes / (tot_ms * 1e-3) / 1e9, tot_ms / 4, args,
                  per["tpcds_q9"]["kernel_gbs"], "rf_q (TPC-DS Q9: 5-bucket fold, 20 B/row)",
                  per["tpcds_q9"]["kernel_ms"], run_ms)
     print(json.dumps(line), flush=True)

@@ -29,15 +29,28 @@ For strictly ascending keys the emitting states have a closed form: one per
 key k of the sorted union of the two key sets, at A row = #(A keys < k) and
 B row = #(B keys < k). `union_states_np` uses it at full sizes; it is pinned
 against the loop by brute force over all small inputs.
+
+A side written _in(t + c, X), c a constant, is a ShiftedSpace
+(src/pass/reffinepass.cpp:67-71): its iterator at row i is X.key[i] + c
+(ShiftedSpace::_idx_to_iter, src/iter/iter_space.cpp:136-140), so the loop
+above runs on the keys plus c (`off_a`, `off_b`; wrapping int64 addition).
+AlgoTrading (benchmark/include/misc.h:15-61) merges a table with itself
+shifted by -50.
 """
 from __future__ import annotations
 
 import numpy as np
 
 
-def union_states_loop(ka, kb, valid_a=None, valid_b=None):
+def _shift(keys, off):
+    with np.errstate(over="ignore"):
+        return np.asarray(keys, np.int64) + np.int64(off)
+
+
+def union_states_loop(ka, kb, valid_a=None, valid_b=None, off_a=0, off_b=0):
     """The merge loop above, row by row: (t, i, j) of every emitting state in
     loop order (i, j: the current rows of A and B, past the end when out)."""
+    ka, kb = _shift(ka, off_a), _shift(kb, off_b)
     na, nb = len(ka), len(kb)
     va = np.ones(na, bool) if valid_a is None else np.asarray(valid_a, bool)
     vb = np.ones(nb, bool) if valid_b is None else np.asarray(valid_b, bool)
@@ -60,11 +73,11 @@ def union_states_loop(ka, kb, valid_a=None, valid_b=None):
     return out
 
 
-def union_states_np(ka, kb, valid_a=None, valid_b=None):
+def union_states_np(ka, kb, valid_a=None, valid_b=None, off_a=0, off_b=0):
     """The same states for strictly ascending keys, vectorised:
     (t, i, j) arrays in loop order."""
-    ka = np.asarray(ka, np.int64)
-    kb = np.asarray(kb, np.int64)
+    ka = _shift(ka, off_a)
+    kb = _shift(kb, off_b)
     t = np.union1d(ka, kb)
     i = np.searchsorted(ka, t, "left")
     j = np.searchsorted(kb, t, "left")
@@ -98,3 +111,14 @@ def ojoin(ka, va, kb, vb, valid_a=None, valid_b=None):
     t, i, j = union_states_np(ka, kb, valid_a, valid_b)
     with np.errstate(over="ignore"):
         return t, read_at(va, i) - read_at(vb, j)
+
+
+def algotrading(ka, va, kb, vb, shift=-50):
+    """benchmark/include/misc.h:15-61's AlgoTrading: an Op over
+    _in(t, A) | _in(t + shift, B) with outputs (t, A[t].0 - B[t + shift].0),
+    each side read at its current row; values float32 (the stock_price
+    table's `val`), the difference rounded to float32."""
+    t, i, j = union_states_np(ka, kb, off_b=shift)
+    a = read_at(np.asarray(va, np.float32), i)
+    b = read_at(np.asarray(vb, np.float32), j)
+    return t, (a - b).astype(np.float32)

@@ -1885,7 +1885,8 @@ static rf_table* run_merge(rf_query* q, const rf_arg* args, int64_t n_args,
     const int64_t* ka = (const int64_t*)ta->cols[0].values;
     const int64_t* kb = (const int64_t*)tb->cols[0].values;
     dev_fill_u64(flag, 0, 1);
-    RF_HIP(aot_merge_splits(ka, na, kb, nb, RF_MERGE_TILE, tiles + 1, splits, d.stream));
+    RF_HIP(aot_merge_splits(ka, na, kb, nb, plan.merge_off[0], plan.merge_off[1], RF_MERGE_TILE,
+                            tiles + 1, splits, d.stream));
     l.args.partials = bcnt;
     l.args.counters = flag;
     l.args.wave_dst = splits;
@@ -1899,7 +1900,8 @@ static rf_table* run_merge(rf_query* q, const rf_arg* args, int64_t n_args,
     RF_HIP(hipMemcpyAsync(host + 1, flag, 4, hipMemcpyDeviceToHost, d.stream));
     RF_HIP(hipStreamSynchronize(d.stream));
     if (host[1] != 0)
-        throw IRError("a union merge needs the keys of both vectors strictly ascending");
+        throw IRError("a union merge needs the keys of both vectors strictly ascending"
+                      " (after the shift of a shifted side)");
     const int64_t total = host[0];
     std::unique_ptr<rf_table> out(alloc_table(ks.out_types, total, 1));
     if (total > 0) {

@@ -67,7 +67,8 @@ hipError_t aot_rank_place(const int64_t* keys, const int64_t* rows, int64_t n, i
                           const void* bits, const void* cum, int64_t* keys_out, int64_t* rows_out,
                           hipStream_t s);
 hipError_t aot_merge_splits(const int64_t* a, int64_t na, const int64_t* b, int64_t nb,
-                            int64_t tile, int64_t n, int64_t* out, hipStream_t s);
+                            int64_t off_a, int64_t off_b, int64_t tile, int64_t n, int64_t* out,
+                            hipStream_t s);
 hipError_t aot_scan_u32(void* h, int64_t m, hipStream_t s);
 // words x 16 bytes: word w's bits, then its count (RfProbeDev::dense_pack).
 hipError_t aot_rank_pack(const void* bits, const void* cum, int64_t words, void* pack,

@@ -664,6 +664,11 @@ static KernelSource gen_merge(const Plan& plan, const std::vector<std::vector<bo
     if (vals.size() > RF_MAXOUT) throw Unsupported("more than 16 output columns");
     ks.out_types = op.types;
 
+    // A shifted side's keys: key + offset in wrapping int64 arithmetic.
+    auto shifted = [](const std::string& load, int64_t off) {
+        if (off == 0) return load;
+        return "(int64_t)((uint64_t)" + load + " + " + std::to_string((uint64_t)off) + "ull)";
+    };
     std::ostringstream o;
     o << "// Generated union merge for reffine function '" << plan.func_name << "'.\n"
          "#define RF_MP(x) ((x) + ((x) >> 3))\n"
@@ -698,11 +703,13 @@ static KernelSource gen_merge(const Plan& plan, const std::vector<std::vector<bo
          "    // key before the tile, 0 at row -1).\n"
          "    for (int x = threadIdx.x; x <= nA; x += RF_MERGE_BLOCK) {\n"
          "        const int64_t r = a0 - 1 + x;\n"
-         "        sA[RF_MP(x)] = r >= 0 ? rf_g(KA)[r] : 0;\n"
+      << "        sA[RF_MP(x)] = r >= 0 ? " << shifted("rf_g(KA)[r]", plan.merge_off[0])
+      << " : 0;\n"
          "    }\n"
          "    for (int x = threadIdx.x; x <= nB; x += RF_MERGE_BLOCK) {\n"
          "        const int64_t r = b0 - 1 + x;\n"
-         "        sB[RF_MP(x)] = r >= 0 ? rf_g(KB)[r] : 0;\n"
+      << "        sB[RF_MP(x)] = r >= 0 ? " << shifted("rf_g(KB)[r]", plan.merge_off[1])
+      << " : 0;\n"
          "    }\n"
          "    __syncthreads();\n"
          "    if (!EMIT) {\n"

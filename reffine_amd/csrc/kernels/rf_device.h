@@ -104,15 +104,17 @@ __device__ __forceinline__ bool rf_valid_bit(const uint16_t* __restrict__ bm,
 // ------------------------------------------------------------ merges
 
 // Merge-path split: how many of A's keys are among the first d keys of the
-// merge of ascending A (na keys) and B (nb keys) that takes A's key first on
-// a tie. Reads A and B only inside [0, na) and [0, nb) for 0 <= d <= na + nb.
+// merge of ascending A (na keys, each + oa) and B (nb keys, each + ob) that
+// takes A's key first on a tie. Reads A and B only inside [0, na) and
+// [0, nb) for 0 <= d <= na + nb. The offsets add in wrapping arithmetic.
 __device__ __forceinline__ int64_t rf_merge_split(const int64_t* __restrict__ A, int64_t na,
                                                   const int64_t* __restrict__ B, int64_t nb,
-                                                  int64_t d) {
+                                                  int64_t d, uint64_t oa, uint64_t ob) {
     int64_t lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
     while (lo < hi) {
         const int64_t mid = (lo + hi) >> 1;
-        if (rf_g(A)[mid] <= rf_g(B)[d - mid - 1]) lo = mid + 1;
+        if ((int64_t)((uint64_t)rf_g(A)[mid] + oa) <= (int64_t)((uint64_t)rf_g(B)[d - mid - 1] + ob))
+            lo = mid + 1;
         else hi = mid;
     }
     return lo;

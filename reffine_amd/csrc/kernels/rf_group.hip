@@ -1064,18 +1064,20 @@ hipError_t aot_rank_pack(const void* bits, const void* cum, int64_t words, void*
 // first min(b * tile, na + nb) keys of the merge of ascending A and B.
 extern "C" __global__ void rf_merge_splits(const int64_t* __restrict__ A, int64_t na,
                                            const int64_t* __restrict__ B, int64_t nb,
-                                           int64_t tile, int64_t n, int64_t* __restrict__ out) {
+                                           uint64_t oa, uint64_t ob, int64_t tile, int64_t n,
+                                           int64_t* __restrict__ out) {
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= n) return;
     const int64_t m = na + nb;
-    out[b] = rf_merge_split(A, na, B, nb, b * tile < m ? b * tile : m);
+    out[b] = rf_merge_split(A, na, B, nb, b * tile < m ? b * tile : m, oa, ob);
 }
 
 hipError_t aot_merge_splits(const int64_t* a, int64_t na, const int64_t* b, int64_t nb,
-                            int64_t tile, int64_t n, int64_t* out, hipStream_t s) {
+                            int64_t off_a, int64_t off_b, int64_t tile, int64_t n, int64_t* out,
+                            hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(rf_merge_splits, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, na,
-                       b, nb, tile, n, out);
+                       b, nb, (uint64_t)off_a, (uint64_t)off_b, tile, n, out);
     return hipGetLastError();
 }
 

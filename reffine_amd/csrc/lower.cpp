@@ -29,6 +29,21 @@ static bool same(const N& a, const N& b) {
     return false;
 }
 
+// Structural equality of two expressions.
+static bool same_tree(const N& a, const N& b) {
+    if (a == b) return true;
+    if (!a || !b) return false;
+    if (a->kind != b->kind || !(a->type == b->type) || a->op != b->op || a->name != b->name ||
+        a->val != b->val || a->idx != b->idx || a->args.size() != b->args.size() ||
+        a->iters.size() != b->iters.size())
+        return false;
+    for (size_t i = 0; i < a->args.size(); i++)
+        if (!same_tree(a->args[i], b->args[i])) return false;
+    for (size_t i = 0; i < a->iters.size(); i++)
+        if (!same_tree(a->iters[i], b->iters[i])) return false;
+    return true;
+}
+
 static bool mentions(const N& e, const std::function<bool(const N&)>& p) {
     if (!e) return false;
     if (p(e)) return true;
@@ -67,6 +82,7 @@ struct Lowerer {
     // Row binding: what the scan's row means.
     N row_val_sym;      // Reduce value symbol bound to a SubVector row
     N iter_sym;         // Op iterator symbol (VecSpace / range form)
+    N merge_iter[2];    // MERGE: each side's _in iterator expression (t or t + c)
     int driver = -1;
     int driver_dim = 1;
     // An In(t, T) over this input drives the scan when present (the runs
@@ -306,12 +322,19 @@ struct Lowerer {
         int t = input_index(vec);
         if (t < 0) throw Unsupported("Element of a computed vector");
         const N& it = e->args[1];
-        if (plan.source == Plan::MERGE && iter_sym && it->kind == K::Sym &&
-            it->name == iter_sym->name && (t == plan.merge_a || t == plan.merge_b)) {
-            // A side's element at the iterator reads that side's current row
+        if (plan.source == Plan::MERGE && (t == plan.merge_a || t == plan.merge_b)) {
+            // A side's element at its iteration expression (t, or t + c for
+            // a shifted side) reads that side's current row
             // (JointSpace::_vec_iter_idxs, iter_space.cpp:252-262; LoopGen
-            // Element, loopgen.cpp:12-20), whatever that row's key.
-            const std::string S = t == plan.merge_a ? "0" : "1";
+            // Element, loopgen.cpp:12-20: the index map is keyed by that
+            // expression), whatever that row's key. At another expression
+            // the reference's map lookup fails.
+            const int side = t == plan.merge_a ? 0 : 1;
+            N w = resolve_bool_sym(it);
+            if (!same_tree(w, merge_iter[side]))
+                throw Unsupported("an element of a merged vector read at an expression other "
+                                  "than its _in's iterator expression");
+            const std::string S = side == 0 ? "0" : "1";
             const Type& vt = plan.inputs[t]->type;
             std::vector<N> cols;
             for (size_t c = 1; c < vt.dtypes.size(); c++)
@@ -602,17 +625,35 @@ struct Lowerer {
             };
             f(c);
             if (!has_in) continue;  // a plain boolean filter
-            auto side = [&](const N& x) -> int {
+            // A side is _in(t, X) or _in(t + c, X) with c a constant: the
+            // Reffine pass makes the latter a ShiftedSpace (reffinepass.cpp:
+            // 67-71) whose iterator at X's row i is X.key[i] + c
+            // (iter_space.cpp:120-140).
+            int64_t offs[2] = {0, 0};
+            N its[2];
+            auto side = [&](const N& x, int s) -> int {
                 N y = resolve_bool_sym(x);
-                if (y->kind != K::In || y->args[0]->kind != K::Sym ||
-                    y->args[0]->name != iter_sym->name)
+                if (y->kind != K::In) return -1;
+                N e = resolve_bool_sym(y->args[0]);
+                auto is_iter = [&](const N& v) {
+                    return v->kind == K::Sym && v->name == iter_sym->name;
+                };
+                if (e->kind == K::Nary && e->op == MathOp::ADD && e->args.size() == 2 &&
+                    is_iter(e->args[0])) {
+                    N cst = resolve_bool_sym(e->args[1]);
+                    if (cst->kind != K::Const || cst->type.bt != BT::INT64) return -1;
+                    offs[s] = (int64_t)cst->val;  // constants are held as doubles
+                } else if (!is_iter(e)) {
                     return -1;
+                }
+                its[s] = e;
                 return input_index(resolve_vec(y->args[1]));
             };
-            const int a = side(c->args[0]), b = side(c->args[1]);
+            const int a = side(c->args[0], 0), b = side(c->args[1], 1);
             if (a < 0 || b < 0)
                 throw Unsupported("a union of iteration spaces other than _in(t, A) | _in(t, B) "
-                                  "over two input tables (t the Op's iterator)");
+                                  "over two input tables (t the Op's iterator, or t + c for a "
+                                  "constant c)");
             if (a == b) throw Unsupported("a union of a vector with itself");
             if (found >= 0) throw Unsupported("an Op over more than one union");
             for (int t : {a, b}) {
@@ -627,6 +668,10 @@ struct Lowerer {
             found = (int)k;
             plan.merge_a = a;
             plan.merge_b = b;
+            plan.merge_off[0] = offs[0];
+            plan.merge_off[1] = offs[1];
+            merge_iter[0] = its[0];
+            merge_iter[1] = its[1];
         }
         if (found < 0) return false;
         if (op->iters.size() != 1) throw Unsupported("a union merge with more than one iterator");
@@ -1602,8 +1647,12 @@ std::string Plan::describe() const {
     }
     if (source == MERGE)
         o << "func " << func_name << ": union merge of table inputs " << merge_a << " and "
-          << merge_b << " (keys ascending), merge positions [" << to_text(start) << ", "
-          << to_text(end) << ")\n";
+          << merge_b << " (keys ascending"
+          << (merge_off[0] || merge_off[1]
+                  ? ", shifted by " + std::to_string(merge_off[0]) + " and " +
+                        std::to_string(merge_off[1])
+                  : std::string())
+          << "), merge positions [" << to_text(start) << ", " << to_text(end) << ")\n";
     else
         o << "func " << func_name << ": scan "
           << (source == RANGE ? "range" : "table input " + std::to_string(driver))

@@ -82,8 +82,10 @@ struct Plan {
     enum Source { TABLE_ROWS, RANGE, MERGE } source = TABLE_ROWS;
     // MERGE: an Op over _in(t, A) | _in(t, B) walks the merge of the two
     // tables' key columns (row leaves $mkey, $mcol.S.c, $mvalid.S.c with
-    // S = 0 for A, 1 for B).
+    // S = 0 for A, 1 for B). A side given as _in(t + c, X) (c constant) is
+    // merged by key + c (merge_off[S]; wrapping int64 addition).
     int merge_a = -1, merge_b = -1;
+    int64_t merge_off[2] = {0, 0};
     std::string func_name;
     std::vector<N> inputs;     // func inputs (syms)
     int driver = -1;           // func input index of the scanned table

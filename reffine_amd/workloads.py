@@ -9,7 +9,7 @@ id, so its output column 0 is the int32 selection vector.
 """
 from __future__ import annotations
 
-from .reffiner import (_add, _cast, _eq, _f64, _f64_t, _func, _gt, _gte, _i8, _i8_t, _i32, _i32_t, _i64,
+from .reffiner import (_add, _cast, _eq, _f32_t, _f64, _f64_t, _func, _gt, _gte, _i8, _i8_t, _i32, _i32_t, _i64,
                        _i64_t, _sub, _idx, _in, _len, _lt, _lte, _mul, _new, _op, _red, _sel, _subvec,
                        _sym, _vec_t, _get)
 
@@ -192,6 +192,29 @@ def outer_join(vt=MERGE_TYPE):
     op = _op([t], _in(t, left) | _in(t, right), [dsym])
     osym = _sym("op", op)
     fn = _func("ojoinbench", osym, [left, right])
+    fn.tbl[osym] = op
+    fn.tbl[dsym] = diff
+    return fn
+
+
+# benchmark/include/misc.h's stock_price table: ts (int64, ascending), val (float32).
+STOCK_TYPE = _vec_t(1, _i64_t, _f32_t)
+
+
+def algotrading(shift: int = -50):
+    """benchmark/include/misc.h:15-61's AlgoTrading: an Op over
+    `_in(t, stock_price1) | _in(t + shift, stock_price2)` (a union with a
+    shifted side) whose output is stock_price1[t].0 - stock_price2[t + shift].0,
+    each side read at its current row of the merge."""
+    sp1 = _sym("stock_price1", STOCK_TYPE)
+    sp2 = _sym("stock_price2", STOCK_TYPE)
+    t = _sym("t", _i64_t)
+    t50 = _add(t, _i64(shift))
+    diff = _sub(_get(sp1[t], 0), _get(sp2[t50], 0))
+    dsym = _sym("diff", diff)
+    op = _op([t], _in(t, sp1) | _in(t50, sp2), [dsym])
+    osym = _sym("op", op)
+    fn = _func("algotrading", osym, [sp1, sp2])
     fn.tbl[osym] = op
     fn.tbl[dsym] = diff
     return fn

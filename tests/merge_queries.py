@@ -2,8 +2,8 @@
 reference's UnionSpace): the outer join of benchmark/include/micro.h:79-95
 and variants. Shared by the CPU oracle checks and the GPU parity tests."""
 from reffine_amd import workloads
-from reffine_amd.reffiner import (_and, _eq, _f64_t, _func, _get, _i64, _i64_t, _in, _mod, _op,
-                                  _or, _sub, _sym, _vec_t)
+from reffine_amd.reffiner import (_add, _and, _eq, _f64_t, _func, _get, _i64, _i64_t, _in, _mod,
+                                  _op, _or, _sub, _sym, _vec_t)
 
 MERGE_T = workloads.MERGE_TYPE        # key, value (micro.h's fake_data, dim 1)
 MERGE_F = _vec_t(1, _i64_t, _f64_t)
@@ -29,4 +29,24 @@ def ojoin_even():
     fn.tbl[osym] = op
     fn.tbl[dsym] = diff
     fn.tbl[csym] = cond
+    return fn
+
+
+def shifted(off_a: int, off_b: int, vt=MERGE_T):
+    """A union with shifted sides, _in(t + off_a, left) | _in(t + off_b, right)
+    (a side with offset 0 written _in(t, X)), outputs left[..].0 - right[..].0
+    read at each side's iteration expression (ShiftedSpace,
+    src/iter/iter_space.cpp:120-150)."""
+    left = _sym("left", vt)
+    right = _sym("right", vt)
+    t = _sym("t", _i64_t)
+    ia = _add(t, _i64(off_a)) if off_a else t
+    ib = _add(t, _i64(off_b)) if off_b else t
+    diff = _sub(_get(left[ia], 0), _get(right[ib], 0))
+    dsym = _sym("diff", diff)
+    op = _op([t], _in(ia, left) | _in(ib, right), [dsym])
+    osym = _sym("op", op)
+    fn = _func("ojoinshift", osym, [left, right])
+    fn.tbl[osym] = op
+    fn.tbl[dsym] = diff
     return fn

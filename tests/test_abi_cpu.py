@@ -122,26 +122,37 @@ def test_run_reduction_compiles():
 
 def test_union_space_lowers_to_a_merge():
     """An Op over _in(t, A) | _in(t, B) (the reference's UnionSpace merge
-    loop) lowers to the union merge kernels; a union of other iteration
-    spaces (a shifted iterator, AlgoTrading's _in(t - 50, B)) is refused as
-    unsupported at compile time, not lowered wrongly."""
+    loop) lowers to the union merge kernels, and so does AlgoTrading's union
+    with a shifted side, _in(t + (-50), B) (a ShiftedSpace); unions the
+    Reffine pass cannot form (an iterator scaled or shifted by a non-constant)
+    and an element read at another expression than its side's iterator are
+    refused as unsupported at compile time, not lowered wrongly."""
     import ref_queries as rq
-    from reffine_amd.reffiner import _add, _func, _get, _i64, _i64_t, _in, _op, _sub, _sym, _vec_t
+    from reffine_amd.reffiner import _add, _func, _get, _i64, _i64_t, _in, _mul, _op, _sub, _sym, _vec_t
     text = runtime.compile_check(rq.micro_join(outer=True))
-    assert "union merge of table inputs 0 and 1" in text
+    assert "union merge of table inputs 0 and 1 (keys ascending)" in text
     assert "rf_qu_count" in text and "rf_qu_emit" in text
+    text = runtime.compile_check(workloads.algotrading())
+    assert "union merge of table inputs 0 and 1 (keys ascending, shifted by 0 and -50)" in text
     t = _sym("t", _i64_t)
+    x = _sym("x", _i64_t)
     A = _sym("left", _vec_t(1, _i64_t, _i64_t))
     B = _sym("right", _vec_t(1, _i64_t, _i64_t))
-    t50 = _add(t, _i64(-50))
-    op = _op([t], _in(t, A) | _in(t50, B), [_sub(_get(A[t], 0), _get(B[t50], 0))])
-    o = _sym("op", op)
-    fn = _func("algotrading", o, [A, B])
-    fn.tbl[o] = op
-    with pytest.raises(runtime.ReffineError) as e:
-        runtime.compile_check(fn)
-    assert e.value.status == 2, e.value  # RF_ERR_UNSUPPORTED
-    assert "union of iteration spaces" in str(e.value)
+
+    def union(side_b, read_b):
+        op = _op([t], _in(t, A) | _in(side_b, B), [_sub(_get(A[t], 0), _get(B[read_b], 0))])
+        o = _sym("op", op)
+        fn = _func("u", o, [A, B, x])
+        fn.tbl[o] = op
+        return fn
+    for side_b, read_b, why in [(_mul(t, _i64(2)), t, "union of iteration spaces"),
+                                (_add(t, x), t, "union of iteration spaces"),
+                                (_add(t, _i64(3)), t, "other than its _in"),
+                                (t, _add(t, _i64(3)), "other than its _in")]:
+        with pytest.raises(runtime.ReffineError) as e:
+            runtime.compile_check(union(side_b, read_b))
+        assert e.value.status == 2, e.value  # RF_ERR_UNSUPPORTED
+        assert why in str(e.value)
 
 
 def test_output_struct_decoder_matches_c_layout():

@@ -120,3 +120,49 @@ def test_outer_join_refuses_keys_not_strictly_ascending(device, bad):
         q(DeviceTable.from_columns([ka, ka]), DeviceTable.from_columns([kb, kb]))
     # the query still runs on good input afterwards
     _check(q, np.arange(5, dtype=np.int64), np.arange(5, dtype=np.int64), kb, kb)
+
+
+@pytest.mark.parametrize("na,nb,oa,ob", [(0, 7, 0, -5), (9, 0, 3, 0), (100, 37, 0, -50),
+                                         (4097, 5000, 12, -7), (100_000, 80_000, -1000, 250)])
+def test_shifted_sides_against_oracle(device, na, nb, oa, ob):
+    """_in(t + c, X) sides (ShiftedSpace): the merge runs on the keys plus
+    their side's constant; outputs read each side's current row."""
+    rng = np.random.default_rng(na + 3 * nb)
+    span = max(2 * (na + nb), 4)
+    ka, kb = _keys(rng, na, span), _keys(rng, nb, span)
+    va = rng.integers(-10**12, 10**12, na)
+    vb = rng.integers(-10**12, 10**12, nb)
+    out = compile_op(mq.shifted(oa, ob))(DeviceTable.from_columns([ka, va]),
+                                         DeviceTable.from_columns([kb, vb]))
+    t, i, j = merge.union_states_np(ka, kb, off_a=oa, off_b=ob)
+    assert np.array_equal(out.column(0), t)
+    assert np.array_equal(out.column(1), merge.read_at(va, i) - merge.read_at(vb, j))
+
+
+def test_algotrading_against_oracle(device):
+    """benchmark/include/misc.h:15-61 as written: one stock_price table
+    (ts = 0..n-1, val float32 ~ 100 + 10 N(0,1), the shape of
+    benchmark/python/pandasbench.py's StockPrice) passed as both inputs, the
+    second shifted by -50; t and the float32 differences bit for bit."""
+    from reffine_amd import workloads
+    n = 1_000_000
+    rng = np.random.default_rng(21)
+    ts = np.arange(n, dtype=np.int64)
+    val = (rng.standard_normal(n) * 10 + 100).astype(np.float32)
+    sp = DeviceTable.from_columns([ts, val])
+    out = compile_op(workloads.algotrading())(sp, sp)
+    t, d = merge.algotrading(ts, val, ts, val)
+    assert out.nrows == n + 50
+    assert np.array_equal(out.column(0), t)
+    assert np.array_equal(out.column(1).view(np.uint32), d.view(np.uint32))
+
+
+def test_shift_that_wraps_the_key_order_is_refused(device):
+    """Keys + c wrap in int64 arithmetic; a wrap that breaks the ascending
+    order is refused like unsorted keys."""
+    big = np.iinfo(np.int64).max
+    ka = np.array([big - 30, big - 5], np.int64)
+    kb = np.array([0, 1], np.int64)
+    q = compile_op(mq.shifted(20, 0))
+    with pytest.raises(ReffineError, match="strictly ascending"):
+        q(DeviceTable.from_columns([ka, ka]), DeviceTable.from_columns([kb, kb]))

@@ -482,6 +482,32 @@ def test_union_merge_closed_form_against_the_loop_by_brute_force():
     assert bad > 0
 
 
+def test_shifted_union_merge_against_the_loop_and_a_closed_form():
+    """Shifted sides (_in(t + c, X): the loop runs on X's keys + c) -- the
+    vectorised states equal the row loop for every pair of subsets of a
+    4-key universe and offsets on both sides; and AlgoTrading's states over
+    keys 0..n-1 on both sides with B shifted by -50 follow from hand
+    counting: t = -50..-1 at A row 0 and B row t + 50, t = 0..n-51 at rows
+    t and t + 50, t = n-50..n-1 at A row t with B run out (B reads 0)."""
+    subsets = list(_subsets([-2, 0, 3, 5]))
+    for oa, ob in [(0, -3), (2, 0), (-1, 4), (7, 7)]:
+        for ka in subsets:
+            for kb in subsets:
+                loop = merge.union_states_loop(ka, kb, off_a=oa, off_b=ob)
+                t, i, j = merge.union_states_np(ka, kb, off_a=oa, off_b=ob)
+                assert loop == list(zip(t.tolist(), i.tolist(), j.tolist())), (ka, kb, oa, ob)
+    n = 200
+    k = np.arange(n, dtype=np.int64)
+    t, i, j = merge.union_states_np(k, k, off_b=-50)
+    assert t.tolist() == list(range(-50, n))
+    assert i.tolist() == [0] * 50 + list(range(n))
+    assert j.tolist() == list(range(50)) + [r + 50 for r in range(n - 50)] + [n] * 50
+    v = np.linspace(90, 110, n).astype(np.float32)
+    t2, d = merge.algotrading(k, v, k, v)
+    want = np.concatenate([v[0] - v[:50], v[:n - 50] - v[50:], v[n - 50:]]).astype(np.float32)
+    assert np.array_equal(t2, t) and np.array_equal(d.view(np.uint32), want.view(np.uint32))
+
+
 def _reference_loop_defined_prefix(ka, kb):
     """The reference's loop transcribed literally (locate through a first-
     occurrence index, min of the current keys, lc / rc advance), stopped at
