@@ -151,6 +151,7 @@ struct RfPartArgs {
     int64_t waves_per_block;   // wave regions one block handles
     const int64_t* wave_cnt;
     int64_t nblocks;           // blocks of the partition kernels
+    int64_t chunk;             // rows rf_part_scatter stages in LDS at a time (4096 or 8192)
     // [nparts * nblocks]: rows of partition p in block b at p*nblocks + b,
     // then scanned in place along each partition (b's offset in p).
     uint32_t* boff;

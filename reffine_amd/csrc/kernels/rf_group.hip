@@ -736,9 +736,9 @@ extern "C" __global__ __launch_bounds__(256) void rf_part_rowscan(RfPartArgs P) 
     if (threadIdx.x == 0) P.ptot[blockIdx.x] = (uint32_t)total;
 }
 
-// Moves the block's rows into their partitions, RF_PART_CHUNK rows at a
-// time staged in LDS in partition order, so that consecutive threads write
-// consecutive rows of a partition's run:
+// Moves the block's rows into their partitions, CHUNK rows (RF_PART_CHUNK or
+// twice that, P.chunk) at a time staged in LDS in partition order, so that
+// consecutive threads write consecutive rows of a partition's run:
 //  * the block's run of each partition is its scanned count in P.boff;
 //  * per chunk: every row takes a slot in its partition (LDS counters), the
 //    counters are scanned into the chunk's partition starts, the rows are
@@ -763,7 +763,8 @@ __device__ __forceinline__ void rf_part_from_lds(void* dst, const unsigned char*
     ((T*)dst)[d] = ((const T*)s)[i];
 }
 
-extern "C" __global__ __launch_bounds__(RF_PART_BLOCK) void rf_part_scatter(RfPartArgs P) {
+template <int CHUNK>
+__device__ __forceinline__ void rf_part_scatter_body(const RfPartArgs& P) {
     extern __shared__ uint32_t rf_ps[];
     const int64_t np_ = P.nparts;
     uint32_t* cnt = rf_ps;              // block rows per partition, then rows placed so far
@@ -773,10 +774,10 @@ extern "C" __global__ __launch_bounds__(RF_PART_BLOCK) void rf_part_scatter(RfPa
     uint32_t* srank = rf_ps + 4 * np_;  // chunk ranks in partition order
     unsigned char* spay[8];
     {
-        unsigned char* q = (unsigned char*)(srank + RF_PART_CHUNK);
+        unsigned char* q = (unsigned char*)(srank + CHUNK);
         for (int j = 0; j < P.np; j++) {
             spay[j] = q;
-            q += (size_t)RF_PART_CHUNK * P.width[j];
+            q += (size_t)CHUNK * P.width[j];
         }
     }
     __shared__ uint64_t s_scan[RF_PART_BLOCK / 64];
@@ -800,8 +801,8 @@ extern "C" __global__ __launch_bounds__(RF_PART_BLOCK) void rf_part_scatter(RfPa
     }
     if (blockIdx.x == 0 && t == 0) P.pstart[np_] = (uint32_t)total;
     __syncthreads();
-    constexpr int CU = RF_PART_CHUNK / RF_PART_BLOCK;
-    for (int64_t k0 = 0; k0 < R.total; k0 += RF_PART_CHUNK) {
+    constexpr int CU = CHUNK / RF_PART_BLOCK;
+    for (int64_t k0 = 0; k0 < R.total; k0 += CHUNK) {
         for (int64_t i = t; i < np_; i += RF_PART_BLOCK) lc[i] = 0;
         __syncthreads();
         int64_t e[CU];
@@ -849,7 +850,7 @@ extern "C" __global__ __launch_bounds__(RF_PART_BLOCK) void rf_part_scatter(RfPa
             }
         }
         __syncthreads();
-        const int n = (int)(R.total - k0 < RF_PART_CHUNK ? R.total - k0 : RF_PART_CHUNK);
+        const int n = (int)(R.total - k0 < CHUNK ? R.total - k0 : CHUNK);
         for (int i = t; i < n; i += RF_PART_BLOCK) {
             const uint32_t rr = srank[i];
             const uint32_t part = rr >> P.shift;
@@ -867,6 +868,13 @@ extern "C" __global__ __launch_bounds__(RF_PART_BLOCK) void rf_part_scatter(RfPa
         __syncthreads();
         for (int64_t i = t; i < np_; i += RF_PART_BLOCK) cnt[i] += lc[i];
     }
+}
+
+extern "C" __global__ __launch_bounds__(RF_PART_BLOCK) void rf_part_scatter(RfPartArgs P) {
+    rf_part_scatter_body<RF_PART_CHUNK>(P);
+}
+extern "C" __global__ __launch_bounds__(RF_PART_BLOCK) void rf_part_scatter8k(RfPartArgs P) {
+    rf_part_scatter_body<2 * RF_PART_CHUNK>(P);
 }
 
 // ------------------------------------------------------- host launchers
@@ -1137,14 +1145,19 @@ hipError_t aot_partition(const RfPartArgs& P0, hipStream_t s) {
     hipLaunchKernelGGL(rf_part_rowscan, dim3((unsigned)P.nparts), dim3(256), 0, s, P);
     size_t row = 4;
     for (int j = 0; j < P.np; j++) row += (size_t)P.width[j];
-    const size_t lds = (size_t)P.nparts * 16 + (size_t)RF_PART_CHUNK * row;
+    const int chunk = P.chunk == 2 * RF_PART_CHUNK ? 2 * RF_PART_CHUNK : RF_PART_CHUNK;
+    const void* kern = chunk == RF_PART_CHUNK ? (const void*)rf_part_scatter
+                                              : (const void*)rf_part_scatter8k;
+    const size_t lds = (size_t)P.nparts * 16 + (size_t)chunk * row;
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void*)rf_part_scatter,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(rf_part_scatter, dim3((unsigned)nb), dim3(RF_PART_BLOCK), lds, s, P);
+    if (chunk == RF_PART_CHUNK)
+        hipLaunchKernelGGL(rf_part_scatter, dim3((unsigned)nb), dim3(RF_PART_BLOCK), lds, s, P);
+    else
+        hipLaunchKernelGGL(rf_part_scatter8k, dim3((unsigned)nb), dim3(RF_PART_BLOCK), lds, s, P);
     return hipGetLastError();
 }
 
