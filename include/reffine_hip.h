@@ -366,16 +366,17 @@ typedef struct rf_request {
 } rf_request;
 
 /* Serves `n` requests as they arrive (a real-time loop against the arrival
- * times) with continuous batching over two batch runners on two streams:
- * requests that arrive while a round is on the GPU are admitted into the
- * other runner as units and arena memory allow, and it launches at once, so
- * rounds overlap on the device. Latency of request i = done_us - arrival_us.
- * `wall_us` receives the time from start to the last completion. The two
- * runners (arena, buffers, captured graphs) are kept for the next call with
- * the same max_units, arena_bytes and use_graph (at most four runners in
- * all, held until the process ends). No reference counterpart. */
+ * times) with continuous batching over `runners` batch runners (2..4; 0
+ * selects 3), each on its own stream: requests that arrive while rounds are
+ * on the GPU are admitted into an idle runner as units and arena memory
+ * allow, and it launches at once, so up to `runners` rounds overlap on the
+ * device. Latency of request i = done_us - arrival_us. `wall_us` receives
+ * the time from start to the last completion. The runners (arena, buffers,
+ * captured graphs) are kept for the next call with the same max_units,
+ * arena_bytes and use_graph (at most four runners in all, held until the
+ * process ends). No reference counterpart. */
 int rf_serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes,
-             int32_t use_graph, double* wall_us);
+             int32_t use_graph, int32_t runners, double* wall_us);
 
 /* The plain counterpart of rf_serve for the same requests: each one, once
  * it has arrived, runs alone through rf_query_run (its own launches and

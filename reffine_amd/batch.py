@@ -203,13 +203,14 @@ def _sel_width(q: Query) -> int:
 
 
 def serve(reqs, max_units: int = 256, arena_bytes: int = 1 << 30, use_graph: bool = True,
-          sel_cap_rows=0) -> ServeResult:
-    """Serves (query, inputs, arrival_us) requests with continuous batching.
+          sel_cap_rows=0, runners: int = 0) -> ServeResult:
+    """Serves (query, inputs, arrival_us) requests with continuous batching
+    over `runners` batch runners (2..4 rounds in flight; 0: the default, 3).
     The first selection column of each request is returned as bytes (at most
     sel_cap_rows rows of up to 8 bytes)."""
     arr, keep = _requests(reqs, sel_cap_rows)
     wall = ctypes.c_double()
-    _ck(lib().rf_serve(arr, len(reqs), max_units, arena_bytes, 1 if use_graph else 0,
+    _ck(lib().rf_serve(arr, len(reqs), max_units, arena_bytes, 1 if use_graph else 0, runners,
                        ctypes.byref(wall)))
     return _collect(reqs, arr, keep, wall.value)
 

@@ -485,10 +485,10 @@ int rf_batch_free(rf_batch* b) {
 }
 
 int rf_serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes, int32_t use_graph,
-             double* wall_us) {
+             int32_t runners, double* wall_us) {
     return guard([&] {
         need(reqs || n == 0, "NULL argument");
-        const double w = rf::serve(reqs, n, max_units, arena_bytes, use_graph != 0);
+        const double w = rf::serve(reqs, n, max_units, arena_bytes, use_graph != 0, runners);
         if (wall_us) *wall_us = w;
     });
 }

@@ -382,7 +382,8 @@ void batch_wait(rf_batch* b);
 void batch_result(rf_batch* b, int32_t unit, void* output_addr, rf_table** out_table);
 void batch_reset(rf_batch* b);
 int batch_units(const rf_batch* b);
-double serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes, bool use_graph);
+double serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes, bool use_graph,
+             int32_t runners);
 double serve_plain(rf_request* reqs, int64_t n);
 
 // Creates the sub-queries of a group join and compiles its all-valid

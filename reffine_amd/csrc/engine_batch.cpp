@@ -714,25 +714,36 @@ struct RunnerLease {
 };
 
 // Runs `n` requests (arrival times in us from the start, ascending) through
-// two runners with continuous batching; fills each request's completion
+// `runners` runners (2..4; 0: 3) with continuous batching: arrivals are
+// admitted into an idle runner, which launches at once, so up to `runners`
+// rounds are in flight on their streams. Fills each request's completion
 // time, status and outputs. Returns the wall time in us.
-double serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes, bool use_graph) {
+double serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes, bool use_graph,
+             int32_t n_runners) {
     for (int64_t i = 1; i < n; i++)
         if (reqs[i].arrival_us < reqs[i - 1].arrival_us)
             throw IRError("requests must be ordered by arrival time");
     if (max_units < 1 || max_units > 65536) throw IRError("units per round must be 1..65536");
     if (arena_bytes < 4096) throw IRError("arena must hold at least 4 KiB");
-    RunnerLease r0{runner_take(max_units, arena_bytes, use_graph)};
-    RunnerLease r1{runner_take(max_units, arena_bytes, use_graph)};
-    rf_batch* runners[2] = {r0.b.get(), r1.b.get()};
-    std::vector<std::vector<int64_t>> members(2);  // request index per unit
+    // Three rounds in flight served the 4096-query mix 20 % faster than two
+    // (151-158K against 123-131K queries/s closed loop); four were slower.
+    if (n_runners == 0) n_runners = 3;
+    if (n_runners < 2 || n_runners > (int32_t)kPoolRunners) throw IRError("runners must be 2..4");
+    const int R = n_runners;
+    std::vector<RunnerLease> leases(R);
+    std::vector<rf_batch*> runners(R);
+    for (int k = 0; k < R; k++) {
+        leases[k].b = runner_take(max_units, arena_bytes, use_graph);
+        runners[k] = leases[k].b.get();
+    }
+    std::vector<std::vector<int64_t>> members(R);  // request index per unit
     int64_t next = 0, finished = 0;
     int filling = 0;
     const auto t0 = std::chrono::steady_clock::now();
     while (finished < n) {
         bool progress = false;
         // Collect finished rounds.
-        for (int k = 0; k < 2; k++) {
+        for (int k = 0; k < R; k++) {
             rf_batch* b = runners[k];
             if (!b->launched || !batch_poll(b)) continue;
             const double t = now_us(t0);
@@ -754,7 +765,9 @@ double serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes
             batch_reset(b);
             progress = true;
         }
-        // Admit arrived requests into the filling runner.
+        // Admit arrived requests into the filling runner: the next idle one
+        // after the last launched.
+        for (int k = 0; k < R && runners[filling]->launched; k++) filling = (filling + 1) % R;
         const double t = now_us(t0);
         rf_batch* f = runners[filling];
         if (!f->launched) {
@@ -787,7 +800,7 @@ double serve(rf_request* reqs, int64_t n, int32_t max_units, int64_t arena_bytes
             }
             if (batch_units(f) > 0) {
                 batch_launch(f);
-                filling ^= 1;
+                filling = (filling + 1) % R;
                 progress = true;
             }
         }

@@ -164,7 +164,7 @@ _SIGS = [
     ("rf_batch_reset", ctypes.c_int, [ctypes.c_void_p]),
     ("rf_batch_free", ctypes.c_int, [ctypes.c_void_p]),
     ("rf_serve", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64,
-                                ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
+                                ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
     ("rf_serve_plain", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64,
                                       ctypes.POINTER(ctypes.c_double)]),
 ]

@@ -42,6 +42,16 @@ def test_errors_are_status_codes_without_device():
     assert lib.rf_compile_check(b"(func f (sym o#1 i64) () ())", 0, buf, 256) == 2
 
 
+def test_serve_checks_its_settings_before_any_device_work():
+    """rf_serve refuses a runner count outside 2..4 (0 picks the default)
+    with RF_ERR_INVALID, before it needs a device."""
+    lib = runtime.lib()
+    wall = ctypes.c_double()
+    for runners in (1, 5, -2):
+        assert lib.rf_serve(None, 0, 16, 1 << 20, 0, runners, ctypes.byref(wall)) == 1
+        assert b"runners must be 2..4" in lib.rf_last_error()
+
+
 def test_non_associative_fold_lowers_to_a_sequential_fold():
     """s * 0.5 + v is not a fold the kernels can re-associate: it lowers to
     the sequential fold (one wave, rows in order: the reference's loop),

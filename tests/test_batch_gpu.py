@@ -118,14 +118,18 @@ def test_batch_same_query_many_units(queries, batches):
     _cmp(_plain(work), Batch(max_units=32, use_graph=True).run(work))
 
 
-def test_serve_matches_plain_and_reports_latency(queries, batches, dim):
+@pytest.mark.parametrize("runners", [0, 2, 4])
+def test_serve_matches_plain_and_reports_latency(queries, batches, dim, runners):
+    """Served results (2, 3 -- the default -- or 4 rounds in flight) equal
+    the plain runs of the same requests."""
     mix = workloads.c5_mix(400, len(SIZES), seed=5)
     assert {m[0] for m in mix} == {"q6", "q1", "join"}
     work = _work(queries, batches, mix, dim[1])
     arrivals = datagen.poisson_arrivals(len(work), qps=20_000.0, seed=6)
     reqs = [(q, inp, t) for (q, inp), t in zip(work, arrivals)]
     caps = [int(inp[0].nrows) for _, inp, _ in reqs]
-    s = serve(reqs, max_units=128, arena_bytes=1 << 30, use_graph=True, sel_cap_rows=caps)
+    s = serve(reqs, max_units=128, arena_bytes=1 << 30, use_graph=True, sel_cap_rows=caps,
+              runners=runners)
     p = serve_plain(reqs, sel_cap_rows=caps)
     assert (s.status == 0).all() and (p.status == 0).all()
     assert (s.latency_us >= 0).all() and (p.latency_us >= 0).all()
