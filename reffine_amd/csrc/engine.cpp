@@ -336,6 +336,34 @@ static uint64_t next_pow2(uint64_t x) {
     return p;
 }
 
+// Column 0's key range (see rf_table::key_range_state); false when column
+// 0 is not a flat int64 column without nulls, or has no rows.
+static bool table_key_range(rf_table* t, int64_t& kmin, int64_t& kmax) {
+    if (t->key_range_state == 0) {
+        t->key_range_state = -1;
+        const Column* c = t->cols.empty() ? nullptr : &t->cols[0];
+        if (c && c->type == types::I64 && c->enc == Enc::FLAT && !c->validity && c->length > 0) {
+            Device& d = device();
+            uint64_t* small = (uint64_t*)dev_alloc(32);
+            const uint64_t init[4] = {~0ull, 0ull, (uint64_t)c->length, 0ull};
+            RF_HIP(hipMemcpyAsync(small, init, 32, hipMemcpyHostToDevice, d.stream));
+            RF_HIP(aot_minmax_i64((const int64_t*)c->values, (const int64_t*)(small + 2), c->length,
+                                  small, d.stream));
+            uint64_t mm[2];
+            RF_HIP(hipMemcpyAsync(mm, small, 16, hipMemcpyDeviceToHost, d.stream));
+            RF_HIP(hipStreamSynchronize(d.stream));
+            dev_free(small);
+            t->key_min = (int64_t)(mm[0] ^ 0x8000000000000000ull);
+            t->key_max = (int64_t)(mm[1] ^ 0x8000000000000000ull);
+            t->key_range_state = 1;
+        }
+    }
+    if (t->key_range_state != 1) return false;
+    kmin = t->key_min;
+    kmax = t->key_max;
+    return true;
+}
+
 void table_build_index(rf_table* t) {
     if (t->cols.empty()) throw ArrowError("empty table");
     // Over the values of column 0: its rows, or for a run-end column its run
@@ -1348,6 +1376,55 @@ static std::shared_ptr<GjBuild> make_build(rf_query* q, const rf_arg* args, int6
     Launch lb = prepare(bq, args, n_args, nullptr);
     char* hs = (char*)bq->host_stage;  // pinned, allocated by prepare
     if (!bq->host_stage_dev) RF_HIP(hipHostGetDevicePointer(&bq->host_stage_dev, bq->host_stage, 0));
+
+    // Dense keys without a host synchronisation. When the key range of the
+    // whole build table (column 0, cached per table) is dense, the rank map
+    // covers that range, the selection's count stays in device memory and
+    // the rank-map kernels read it there; the count and the duplicate check
+    // reach the host at the run's last synchronisation. This saves the build
+    // its host round trip: waiting for the selection, then issuing the
+    // rank-map launches to an idle GPU (about 35 us of the C4 query with the
+    // build).
+    {
+        rf_table* bt = args[gp.build].u.table;
+        const int64_t nb = bt->rows();
+        int64_t tmin = 0, tmax = 0;
+        if (!lb.empty && nb < ((int64_t)1 << 31) && table_key_range(bt, tmin, tmax)) {
+            const uint64_t range = (uint64_t)tmax - (uint64_t)tmin + 1;
+            const int64_t words = (int64_t)((range + 63) / 64);
+            if (range != 0 && range <= 64 * (uint64_t)nb + ((uint64_t)1 << 20) &&
+                words <= kRankFusedWords) {
+                // The selection first: the GPU runs it while the host issues
+                // the rest.
+                launch(bq, lb, s, false);
+                B->dense_bits = dev_alloc((size_t)words * 8);
+                B->dense_cum = dev_alloc((size_t)aot_rank_cum_entries(words) * 4);
+                B->dense_pack = dev_alloc((size_t)words * 16);
+                B->dense_len = range;
+                B->keys = dev_alloc((size_t)nb * 8);
+                B->rows = dev_alloc((size_t)nb * 8);
+                dev_fill_u64(B->dense_bits, 0, words);
+                const int64_t* kdev = lb.args.out_count;
+                char* hd = (char*)bq->host_stage_dev;
+                RF_HIP(aot_rank_build_small((const int64_t*)bq->outs[0], nb, kdev, tmin,
+                                            B->dense_bits, words, B->dense_cum, B->dense_pack,
+                                            hd + 6144, kdev, (int64_t*)(hd + 6160), s));
+                RF_HIP(aot_rank_place((const int64_t*)bq->outs[0], (const int64_t*)bq->outs[1], nb,
+                                      kdev, tmin, B->dense_bits, B->dense_cum, (int64_t*)B->keys,
+                                      (int64_t*)B->rows, s));
+                B->kmin = tmin;
+                B->groups_ub = nb;
+                B->groups_dev = kdev;
+                B->pending_groups = (const volatile int64_t*)(hs + 6160);
+                B->pending_total = (const volatile uint64_t*)(hs + 6144);
+                if (!defer_check) {
+                    RF_HIP(hipStreamSynchronize(s));
+                    gj_verify(q, *B);
+                }
+                return B;
+            }
+        }
+    }
     dev_fill_u64(small + 2, 0, 2);
     lb.args.result = bq->host_stage_dev;
     lb.args.out_count = (int64_t*)((char*)bq->host_stage_dev + 4096);
@@ -1395,10 +1472,11 @@ static std::shared_ptr<GjBuild> make_build(rf_query* q, const rf_arg* args, int6
             if (!bq->host_stage_dev)
                 RF_HIP(hipHostGetDevicePointer(&bq->host_stage_dev, bq->host_stage, 0));
             uint64_t* total_dev = (uint64_t*)((char*)bq->host_stage_dev + 6144);
-            RF_HIP(aot_rank_build_small((const int64_t*)bq->outs[0], K, kmin, B->dense_bits, words,
-                                        B->dense_cum, B->dense_pack, total_dev, s));
-            RF_HIP(aot_rank_place((const int64_t*)bq->outs[0], (const int64_t*)bq->outs[1], K, kmin,
-                                  B->dense_bits, B->dense_cum, (int64_t*)B->keys,
+            RF_HIP(aot_rank_build_small((const int64_t*)bq->outs[0], K, nullptr, kmin,
+                                        B->dense_bits, words, B->dense_cum, B->dense_pack,
+                                        total_dev, nullptr, nullptr, s));
+            RF_HIP(aot_rank_place((const int64_t*)bq->outs[0], (const int64_t*)bq->outs[1], K,
+                                  nullptr, kmin, B->dense_bits, B->dense_cum, (int64_t*)B->keys,
                                   (int64_t*)B->rows, s));
             B->groups = K;
             B->kmin = kmin;
@@ -1474,6 +1552,11 @@ static std::shared_ptr<GjBuild> make_build(rf_query* q, const rf_arg* args, int6
 }
 
 void gj_verify(rf_query* q, GjBuild& b) {
+    if (b.pending_groups) {
+        b.groups = *b.pending_groups;
+        b.pending_groups = nullptr;
+        b.groups_dev = nullptr;
+    }
     if (!b.pending_total) return;
     const uint64_t total = *b.pending_total;
     b.pending_total = nullptr;
@@ -1503,7 +1586,7 @@ std::shared_ptr<GjBuild> gj_build(rf_query* q, const rf_arg* args, int64_t n_arg
     }
     auto it = G.builds.find(bkey);
     if (it != G.builds.end()) {
-        if (it->second->pending_total) {  // a run that built it ended early
+        if (it->second->pending_total || it->second->pending_groups) {  // a run that built it ended early
             RF_HIP(hipStreamSynchronize(device().stream));
             std::shared_ptr<GjBuild> B = it->second;
             gj_verify(q, *B);
@@ -1573,16 +1656,21 @@ static rf_table* group_agg(rf_query* q, const Spec& fs, const GjBuild& B, const 
     hipStream_t s = d.stream;
     GroupJoinState& G = *q->gj;
     const GroupJoinPlan& gp = *q->plan.gj;
-    const int64_t K = B.groups;
+    // A build whose key count is still on the device is sized by its bound
+    // (the build table's rows); the fold reads the count itself.
+    const bool pend = B.pending_groups != nullptr;
+    const int64_t K = pend ? B.groups_ub : B.groups;
     int nslot = 0;
     for (auto& f : gp.fields) nslot += f.updated ? 1 : 0;
     // 2^shift groups per partition: about 512 partitions or more (a block
     // each), at least 16 groups per partition (LDS atomics on one slot
-    // serialise), at most 64 KiB of LDS slots and 4096 partitions.
+    // serialise), at most 64 KiB of LDS slots and 4096 partitions. Under a
+    // bound, partitions of at most 512 groups: the partitions past the
+    // actual count are empty, and the ones below keep the fold's width.
     int max_shift = 0;
     while (((size_t)std::max(nslot, 1) * 8) << (max_shift + 1) <= 65536) max_shift++;
     int shift = 4;
-    while (shift < max_shift && (K >> (shift + 1)) >= 512) shift++;
+    while (shift < max_shift && (!pend || shift < 9) && (K >> (shift + 1)) >= 512) shift++;
     while (shift < max_shift && ((K + ((int64_t)1 << shift) - 1) >> shift) > 4096) shift++;
     const int64_t nparts = std::max<int64_t>((K + ((int64_t)1 << shift) - 1) >> shift, 1);
     const int np = (int)gp.pay_types.size();
@@ -1639,11 +1727,13 @@ static rf_table* group_agg(rf_query* q, const Spec& fs, const GjBuild& B, const 
         P.wave_cnt = A.wave_cnt;
         P.boff = (uint32_t*)G.hist;
         P.prank = (uint32_t*)G.prank;
+        P.groups_dev = pend ? B.groups_dev : nullptr;
         RF_HIP(aot_partition(P, s));
     }
     std::unique_ptr<rf_table> out(alloc_table(gp.out_types, K, 1));
     RfFoldArgs F;
     gj_fold_args(q, fs, B, args, n_args, F);
+    F.groups = K;
     for (int j = 0; j < np; j++) F.pay[j] = G.ppay[j];
     for (size_t j = 0; j < out->cols.size(); j++) F.out[j] = out->cols[j].values;
     RfAggArgs GA{};
@@ -1651,6 +1741,7 @@ static rf_table* group_agg(rf_query* q, const Spec& fs, const GjBuild& B, const 
     GA.pstart = pstart;
     GA.stride = stride;
     GA.shift = shift;
+    GA.groups_dev = pend ? B.groups_dev : nullptr;
     void* params[] = {&F, &GA};
     const size_t lds = (size_t)std::max(nslot, 1) * 8 << shift;
     RF_HIP(hipModuleLaunchKernel(fs.mod->agg, (unsigned)nparts, 1, 1, 256, 1, 1, (unsigned)lds, s,
@@ -1673,8 +1764,7 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
     // A new dense build's duplicate-key check is read after this run's last
     // synchronisation (no synchronisation between the build and the probe).
     const std::shared_ptr<GjBuild> B = gj_build(q, args, n_args, true);
-    const int64_t K = B->groups;
-    if (K == 0) return alloc_table(gp.out_types, 0, 1);
+    if (!B->pending_groups && B->groups == 0) return alloc_table(gp.out_types, 0, 1);
     auto verified = [&](rf_table* t) {
         try {
             gj_verify(q, *B);
@@ -1682,6 +1772,8 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
             table_free(t);
             throw;
         }
+        // A table sized by a pending count's bound gets the count's rows.
+        for (Column& c : t->cols) c.length = c.logical_length = B->groups;
         return t;
     };
     rf_query* pq = gj_probe_query(q, *B);
@@ -1704,6 +1796,8 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
     RF_HIP(hipMemcpyAsync(hs, pq->out_count, 8, hipMemcpyDeviceToHost, s));
     RF_HIP(hipStreamSynchronize(s));
     gj_verify(q, *B);
+    const int64_t K = B->groups;
+    if (K == 0) return alloc_table(gp.out_types, 0, 1);
     const int64_t J = lp.empty ? 0 : *(int64_t*)hs;
 
     // 5. Stable sort of the joined rows by rank.

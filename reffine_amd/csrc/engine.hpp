@@ -61,11 +61,12 @@ hipError_t aot_adjacent_dups(const int64_t* keys, int64_t n, void* flag, hipStre
 hipError_t aot_not_ascending(const int64_t* keys, int64_t n, void* flag, hipStream_t s);
 int64_t aot_scan_entries(int64_t m);
 constexpr int64_t kRankFusedWords = int64_t(1) << 18;
-hipError_t aot_rank_build_small(const int64_t* keys, int64_t n, int64_t base, void* bits,
-                                int64_t words, void* cum, void* pack, void* total, hipStream_t s);
-hipError_t aot_rank_place(const int64_t* keys, const int64_t* rows, int64_t n, int64_t base,
-                          const void* bits, const void* cum, int64_t* keys_out, int64_t* rows_out,
-                          hipStream_t s);
+hipError_t aot_rank_build_small(const int64_t* keys, int64_t n, const int64_t* n_dev, int64_t base,
+                                void* bits, int64_t words, void* cum, void* pack, void* total,
+                                const int64_t* k_in, int64_t* k_out, hipStream_t s);
+hipError_t aot_rank_place(const int64_t* keys, const int64_t* rows, int64_t n, const int64_t* n_dev,
+                          int64_t base, const void* bits, const void* cum, int64_t* keys_out,
+                          int64_t* rows_out, hipStream_t s);
 hipError_t aot_merge_splits(const int64_t* a, int64_t na, const int64_t* b, int64_t nb,
                             int64_t off_a, int64_t off_b, int64_t tile, int64_t n, int64_t* out,
                             hipStream_t s);
@@ -134,6 +135,11 @@ struct rf_table {
     std::vector<std::shared_ptr<void>> keep;   // borrowed inputs kept alive (imports)
     rf::HashIndex index;
     bool has_index = false;
+    // The smallest and largest key of an int64 column 0 without nulls,
+    // computed on first use (a table's contents do not change once it
+    // exists): 0 not yet computed, 1 known, -1 not applicable or no rows.
+    int key_range_state = 0;
+    int64_t key_min = 0, key_max = 0;
     rf::Type type() const;
     // Rows of the table: the logical length (a run-end column 0 holds fewer
     // runs than rows).
@@ -300,6 +306,13 @@ struct GjBuild {
     // the build into pinned host memory, must equal `groups` once the
     // stream has been synchronised (then the build is usable).
     const volatile uint64_t* pending_total = nullptr;
+    // A build launched without a host synchronisation: the count of
+    // qualifying keys is still on the device (groups_dev, at most
+    // groups_ub); the build's last kernel copies it to pending_groups
+    // (pinned memory), which gj_verify reads into `groups`.
+    int64_t groups_ub = 0;
+    const int64_t* groups_dev = nullptr;
+    const volatile int64_t* pending_groups = nullptr;
     RfProbeDev probe() const;  // the probe plan's index for the external probes
     ~GjBuild();
 };

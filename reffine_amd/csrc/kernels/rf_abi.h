@@ -159,6 +159,10 @@ struct RfPartArgs {
     uint32_t* pstart;          // [nparts + 1] first row of each partition
     uint32_t* prank;           // partitioned ranks
     void* ppay[8];             // partitioned payload columns
+    // When set, the number of groups, a count an earlier kernel wrote:
+    // partitions from ceil(groups / 2^shift) on are empty (nparts, a bound,
+    // still lays out boff, ptot, pstart and the LDS).
+    const int64_t* groups_dev;
 };
 
 // The partition of one rf_gj_agg block p: joined rows [pstart[p * stride],
@@ -170,6 +174,9 @@ struct RfAggArgs {
     int64_t stride;
     int32_t shift;
     int32_t pad;
+    // When set, the number of groups (RfFoldArgs::groups is then a bound):
+    // a count an earlier kernel wrote.
+    const int64_t* groups_dev;
 };
 
 // Batched execution: a persistent launch over (work unit, wave-tile) items.

@@ -432,6 +432,49 @@ __device__ __forceinline__ void rf_tile_emit(const RfScanArgs& L, int64_t row0, 
     }
 }
 
+// rf_tile_emit through the wave's LDS buffer: the load group's qualifying
+// rows are written to LDS at their place among the group's rows (the same
+// row order), then the wave writes them to the staging columns 64 rows per
+// store. LDS operations of one wave complete in issue order, so lanes read
+// what other lanes of the wave wrote before; the wave barriers keep the
+// compiler from moving the buffer's reads and writes across the hand-over.
+template <class Q>
+__device__ __forceinline__ void rf_tile_emit_lds(const RfScanArgs& L, int64_t row0, uint32_t lane,
+                                                 const uint32_t* bits, const typename Q::Carry& cy,
+                                                 int64_t base, int64_t& cursor,
+                                                 typename Q::EmitBuf& buf) {
+    constexpr int ITEMS = Q::ITEMS, LSUB = Q::LSUB;
+    uint32_t n = 0;  // the group's qualifying rows so far (wave-uniform)
+#pragma unroll
+    for (int s = 0; s < LSUB; s++) {
+        const uint32_t b = bits[s];
+        uint32_t below = 0, total = 0;
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const uint64_t m = __ballot((b >> k) & 1u);
+            below += rf_lanes_below(m);
+            total += (uint32_t)__builtin_popcountll(m);
+        }
+        if (b) {
+            const int64_t r = row0 + ((int64_t)s * 64 + lane) * ITEMS;
+            int i = (int)(n + below);
+#pragma unroll
+            for (int k = 0; k < ITEMS; k++) {
+                if ((b >> k) & 1u) {
+                    Q::emit_lds(buf, i, r + k, cy, s, k);
+                    i++;
+                }
+            }
+        }
+        n += total;
+    }
+    if (n == 0) return;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t i = lane; i < n; i += 64) Q::flush(L, buf, (int)i, base + cursor + i);
+    __builtin_amdgcn_wave_barrier();
+    cursor += n;
+}
+
 // Scan kernel. Wave w (of W = gridDim.x * BLOCK/64) owns wave-tiles
 // [w*per, min((w+1)*per, ntiles)) with per = ceil(ntiles / W), processed as
 // a sequence of load groups.
@@ -453,6 +496,7 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
     __shared__ Acc s_waves[WAVES];
     __shared__ uint64_t s_scan[WAVES];
     __shared__ uint32_t s_last;
+    __shared__ typename Q::EmitBuf s_emit[Q::LDS_EMIT ? WAVES : 1];  // per-wave emit buffers
 
     const uint32_t lane = __lane_id();
     const int64_t gw = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
@@ -479,7 +523,13 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
         uint32_t bits[LSUB];
         typename Q::Carry cy;  // per-row values the selection's emit reuses
         rf_tile_rows<Q>(L, T, row0, interior, lane, a, bits, cy);
-        if constexpr (Q::HAS_SEL) rf_tile_emit<Q>(L, row0, lane, bits, cy, gw * L.stage_rows, cursor);
+        if constexpr (Q::HAS_SEL) {
+            if constexpr (Q::LDS_EMIT)
+                rf_tile_emit_lds<Q>(L, row0, lane, bits, cy, gw * L.stage_rows, cursor,
+                                    s_emit[threadIdx.x >> 6]);
+            else
+                rf_tile_emit<Q>(L, row0, lane, bits, cy, gw * L.stage_rows, cursor);
+        }
         if constexpr (Q::NACC > 0 && Q::DET) {
             if (grp % NG == NG - 1) {
                 Acc t = a_tile;

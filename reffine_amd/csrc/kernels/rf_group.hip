@@ -475,8 +475,12 @@ extern "C" __global__ void rf_seg_bounds(const uint32_t* __restrict__ rank, int6
 // first (sorted keys put a wave's 64 keys in one or two words), so one lane
 // per word issues the atomic. The loop steps whole waves: every lane takes
 // part in the shuffles.
-extern "C" __global__ void rf_rank_bits(const int64_t* __restrict__ keys, int64_t n, int64_t base,
+// With n_dev set, the keys are the first min(*n_dev, n) (a count an earlier
+// kernel wrote).
+extern "C" __global__ void rf_rank_bits(const int64_t* __restrict__ keys, int64_t n,
+                                        const int64_t* __restrict__ n_dev, int64_t base,
                                         unsigned long long* bits) {
+    if (n_dev && *n_dev < n) n = *n_dev;
     const int lane = threadIdx.x & 63;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); i0 < n;
@@ -536,10 +540,12 @@ extern "C" __global__ void rf_rank_rows(const int64_t* __restrict__ keys, int64_
 // slot rank. Keys must be distinct (checked by the caller through the count
 // of marked keys).
 extern "C" __global__ void rf_rank_scatter(const int64_t* __restrict__ keys,
-                                           const int64_t* __restrict__ rows, int64_t n, int64_t base,
+                                           const int64_t* __restrict__ rows, int64_t n,
+                                           const int64_t* __restrict__ n_dev, int64_t base,
                                            const uint64_t* __restrict__ bits,
                                            const uint32_t* __restrict__ cum, int64_t* keys_out,
                                            int64_t* rows_out) {
+    if (n_dev && *n_dev < n) n = *n_dev;  // as in rf_rank_bits
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t k = keys[i];
@@ -559,46 +565,80 @@ extern "C" __global__ void rf_rank_total(const uint64_t* __restrict__ bits,
         *out = (uint64_t)cum[words - 1] + (uint64_t)__popcll(bits[words - 1]);
 }
 
-// A rank map's counts in one launch (one block of 1024 threads, rounds of
-// 1024 words, the next round's words loaded before this round's scan):
-// cum[w] = marked keys in words before w, the packed probe form
-// pack[2w] = bits[w], pack[2w + 1] = cum[w], and *total = all marked keys.
+// A rank map's counts in one launch (one block of 1024 threads): cum[w] =
+// marked keys in words before w, the packed probe form pack[2w] = bits[w],
+// pack[2w + 1] = cum[w], and *total = all marked keys. A round covers
+// 16 x 1024 words; thread t loads words r0 + j*1024 + t for every j before
+// using any (coalesced loads, one memory wait per round), scans each j's
+// 1024 counts within its wave, and wave 0 scans the 16 x 16 wave totals in
+// word order: two barriers a round. (One word per thread per round, each
+// round waiting on its loads, took 23 us for the 23K words of a 1.5M-key
+// range; 16 consecutive words per thread, 50 us: uncoalesced, all on one CU.)
+#define RF_RANK_PER 16
 extern "C" __global__ __launch_bounds__(1024) void rf_rank_finish(
     const uint64_t* __restrict__ bits, int64_t words, uint32_t* __restrict__ cum,
-    unsigned long long* __restrict__ pack, uint64_t* total) {
-    __shared__ uint32_t s_w[2][16];
+    unsigned long long* __restrict__ pack, uint64_t* total, const int64_t* __restrict__ k_in,
+    int64_t* k_out) {
+    __shared__ uint32_t s_tot[RF_RANK_PER * 16];  // wave totals, then their prefixes
+    __shared__ uint32_t s_round;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     uint32_t base = 0;
-    uint64_t nb = t < words ? bits[t] : 0ull;
-    int par = 0;
-    for (int64_t r0 = 0; r0 < words; r0 += 1024) {
-        const int64_t i = r0 + t;
-        const uint64_t b = nb;
-        nb = i + 1024 < words ? bits[i + 1024] : 0ull;
-        const uint32_t c = (uint32_t)__popcll(b);
-        uint32_t inc = c;
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(inc, o, 64);
-            if (lane >= o) inc += y;
+    for (int64_t r0 = 0; r0 < words; r0 += (int64_t)1024 * RF_RANK_PER) {
+        uint64_t b[RF_RANK_PER];
+#pragma unroll
+        for (int j = 0; j < RF_RANK_PER; j++) {
+            const int64_t w = r0 + (int64_t)j * 1024 + t;
+            b[j] = w < words ? bits[w] : 0ull;
         }
-        if (lane == 63) s_w[par][wv] = inc;
+        uint32_t inc[RF_RANK_PER];
+#pragma unroll
+        for (int j = 0; j < RF_RANK_PER; j++) {
+            uint32_t x = (uint32_t)__popcll(b[j]);
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o, 64);
+                if (lane >= o) x += y;
+            }
+            inc[j] = x;
+            if (lane == 63) s_tot[j * 16 + wv] = x;
+        }
         __syncthreads();
-        uint32_t wb = 0, tot = 0;
-        for (int k = 0; k < 16; k++) {
-            const uint32_t v = s_w[par][k];
-            if (k < wv) wb += v;
-            tot += v;
+        if (wv == 0) {
+            // 256 wave totals in word order (j major), four per lane.
+            const uint32_t v0 = s_tot[4 * lane], v1 = s_tot[4 * lane + 1],
+                           v2 = s_tot[4 * lane + 2], v3 = s_tot[4 * lane + 3];
+            const uint32_t sum = v0 + v1 + v2 + v3;
+            uint32_t x = sum;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o, 64);
+                if (lane >= o) x += y;
+            }
+            const uint32_t ex = base + x - sum;
+            s_tot[4 * lane] = ex;
+            s_tot[4 * lane + 1] = ex + v0;
+            s_tot[4 * lane + 2] = ex + v0 + v1;
+            s_tot[4 * lane + 3] = ex + v0 + v1 + v2;
+            if (lane == 63) s_round = x;
         }
-        const uint32_t ex = base + wb + inc - c;
-        if (i < words) {
-            cum[i] = ex;
-            pack[2 * i] = b;
-            pack[2 * i + 1] = ex;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RF_RANK_PER; j++) {
+            const int64_t w = r0 + (int64_t)j * 1024 + t;
+            const uint32_t below = s_tot[j * 16 + wv] + inc[j] - (uint32_t)__popcll(b[j]);
+            if (w < words) {
+                cum[w] = below;
+                pack[2 * w] = b[j];
+                pack[2 * w + 1] = below;
+            }
         }
-        base += tot;
-        par ^= 1;  // the next round writes the other slots: one barrier a round
+        base += s_round;
+        __syncthreads();  // s_tot and s_round are rewritten next round
     }
-    if (t == 0) *total = base;
+    if (t == 0) {
+        *total = base;
+        if (k_in) *k_out = *k_in;  // the keys' count beside the marked keys' (pinned memory)
+    }
 }
 
 extern "C" __global__ void rf_word_popc(const uint64_t* __restrict__ bits, int64_t words,
@@ -679,6 +719,14 @@ __device__ __forceinline__ void rf_part_regions(const RfPartArgs& P, RfPartRegio
     }
 }
 
+// Partitions that can hold rows: all nparts, or those below the count of
+// groups when it is read from the device.
+__device__ __forceinline__ int64_t rf_part_live(const RfPartArgs& P) {
+    if (!P.groups_dev) return P.nparts;
+    const int64_t n = (*P.groups_dev + ((int64_t)1 << P.shift) - 1) >> P.shift;
+    return n < P.nparts ? n : P.nparts;
+}
+
 __device__ __forceinline__ int64_t rf_part_elem(const RfPartRegions& R, int64_t k) {
     int64_t prev = 0;
     for (int i = 0; i < R.n; i++) {
@@ -695,7 +743,8 @@ __device__ __forceinline__ int64_t rf_part_elem(const RfPartRegions& R, int64_t 
 extern "C" __global__ __launch_bounds__(RF_PART_BLOCK) void rf_part_hist(RfPartArgs P) {
     extern __shared__ uint32_t rf_ph[];
     const int t = threadIdx.x;
-    for (int64_t i = t; i < P.nparts; i += RF_PART_BLOCK) rf_ph[i] = 0;
+    const int64_t nl = rf_part_live(P);
+    for (int64_t i = t; i < nl; i += RF_PART_BLOCK) rf_ph[i] = 0;
     __shared__ RfPartRegions R;
     rf_part_regions(P, R);
     __syncthreads();
@@ -711,7 +760,7 @@ extern "C" __global__ __launch_bounds__(RF_PART_BLOCK) void rf_part_hist(RfPartA
             if (r[u] != 0xffffffffu) atomicAdd(&rf_ph[r[u] >> P.shift], 1u);
     }
     __syncthreads();
-    for (int64_t i = t; i < P.nparts; i += RF_PART_BLOCK)
+    for (int64_t i = t; i < nl; i += RF_PART_BLOCK)
         P.boff[i * P.nblocks + blockIdx.x] = rf_ph[i];
 }
 
@@ -719,6 +768,7 @@ extern "C" __global__ __launch_bounds__(RF_PART_BLOCK) void rf_part_hist(RfPartA
 // the partition's row count.
 extern "C" __global__ __launch_bounds__(256) void rf_part_rowscan(RfPartArgs P) {
     __shared__ uint64_t s_scan[4];
+    if (blockIdx.x >= rf_part_live(P)) return;  // the whole block: an empty partition
     const int64_t nb = P.nblocks;
     uint32_t* row = P.boff + (int64_t)blockIdx.x * nb;
     const int64_t per = (nb + 255) / 256;
@@ -787,9 +837,10 @@ __device__ __forceinline__ void rf_part_scatter_body(const RfPartArgs& P) {
     // The block's run of each partition: the partition's start (a scan of
     // the partition totals, thread t taking a contiguous chunk of them) plus
     // the block's offset in it; block 0 publishes the starts for rf_gj_agg.
-    const int64_t per = (np_ + RF_PART_BLOCK - 1) / RF_PART_BLOCK;
+    const int64_t nl = rf_part_live(P);  // partitions that can hold rows
+    const int64_t per = (nl + RF_PART_BLOCK - 1) / RF_PART_BLOCK;
     const int64_t lo = (int64_t)t * per;
-    const int64_t hi = lo + per < np_ ? lo + per : np_;
+    const int64_t hi = lo + per < nl ? lo + per : nl;
     uint64_t mine = 0, off, total;
     for (int64_t i = lo; i < hi; i++) mine += P.ptot[i];
     off = rf_block_excl_scan<RF_PART_BLOCK>(mine, s_scan, total);  // has barriers
@@ -799,11 +850,11 @@ __device__ __forceinline__ void rf_part_scatter_body(const RfPartArgs& P) {
         cnt[i] = 0;
         off += P.ptot[i];
     }
-    if (blockIdx.x == 0 && t == 0) P.pstart[np_] = (uint32_t)total;
+    if (blockIdx.x == 0 && t == 0) P.pstart[nl] = (uint32_t)total;
     __syncthreads();
     constexpr int CU = CHUNK / RF_PART_BLOCK;
     for (int64_t k0 = 0; k0 < R.total; k0 += CHUNK) {
-        for (int64_t i = t; i < np_; i += RF_PART_BLOCK) lc[i] = 0;
+        for (int64_t i = t; i < nl; i += RF_PART_BLOCK) lc[i] = 0;
         __syncthreads();
         int64_t e[CU];
         uint32_t r[CU], slot[CU];
@@ -866,7 +917,7 @@ __device__ __forceinline__ void rf_part_scatter_body(const RfPartArgs& P) {
             }
         }
         __syncthreads();
-        for (int64_t i = t; i < np_; i += RF_PART_BLOCK) cnt[i] += lc[i];
+        for (int64_t i = t; i < nl; i += RF_PART_BLOCK) cnt[i] += lc[i];
     }
 }
 
@@ -1004,7 +1055,8 @@ hipError_t aot_rank_scatter(const int64_t* keys, const int64_t* rows, int64_t n,
                             int64_t* rows_out, void* total, hipStream_t s) {
     if (n <= 0 || words <= 0) return hipSuccess;
     hipLaunchKernelGGL(rf_rank_scatter, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, rows, n,
-                       base, (const uint64_t*)bits, (const uint32_t*)cum, keys_out, rows_out);
+                       (const int64_t*)nullptr, base, (const uint64_t*)bits, (const uint32_t*)cum,
+                       keys_out, rows_out);
     hipLaunchKernelGGL(rf_rank_total, dim3(1), dim3(64), 0, s, (const uint64_t*)bits,
                        (const uint32_t*)cum, words, (uint64_t*)total);
     return hipGetLastError();
@@ -1020,8 +1072,8 @@ int64_t aot_rank_cum_entries(int64_t words) {
 hipError_t aot_rank_build(const int64_t* keys, int64_t n, int64_t base, void* bits, int64_t words,
                           void* cum, hipStream_t s) {
     if (n <= 0 || words <= 0) return hipSuccess;
-    hipLaunchKernelGGL(rf_rank_bits, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, n, base,
-                       (unsigned long long*)bits);
+    hipLaunchKernelGGL(rf_rank_bits, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, n,
+                       (const int64_t*)nullptr, base, (unsigned long long*)bits);
     hipLaunchKernelGGL(rf_word_popc, dim3(grid_cap(words, 256, 4096)), dim3(256), 0, s,
                        (const uint64_t*)bits, words, (uint32_t*)cum);
     uint32_t* c = (uint32_t*)cum;
@@ -1036,26 +1088,31 @@ hipError_t aot_rank_build(const int64_t* keys, int64_t n, int64_t base, void* bi
 // The rank map of n keys in words zeroed words, counts and packed form in
 // two launches (marking, then rf_rank_finish): for maps of at most
 // kRankFusedWords words, where one block scans them quickly. *total gets
-// the number of marked keys.
-hipError_t aot_rank_build_small(const int64_t* keys, int64_t n, int64_t base, void* bits,
-                                int64_t words, void* cum, void* pack, void* total,
-                                hipStream_t s) {
+// the number of marked keys. With n_dev set, the keys are the first
+// min(*n_dev, n) (a count a kernel before wrote: no host synchronisation),
+// and with k_out set, rf_rank_finish copies *k_in there.
+hipError_t aot_rank_build_small(const int64_t* keys, int64_t n, const int64_t* n_dev, int64_t base,
+                                void* bits, int64_t words, void* cum, void* pack, void* total,
+                                const int64_t* k_in, int64_t* k_out, hipStream_t s) {
     if (words <= 0) return hipSuccess;
     if (n > 0)
         hipLaunchKernelGGL(rf_rank_bits, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, n,
-                           base, (unsigned long long*)bits);
+                           n_dev, base, (unsigned long long*)bits);
     hipLaunchKernelGGL(rf_rank_finish, dim3(1), dim3(1024), 0, s, (const uint64_t*)bits, words,
-                       (uint32_t*)cum, (unsigned long long*)pack, (uint64_t*)total);
+                       (uint32_t*)cum, (unsigned long long*)pack, (uint64_t*)total,
+                       k_out ? k_in : (const int64_t*)nullptr, k_out);
     return hipGetLastError();
 }
 
-// Places (key, row) at each key's rank (rf_rank_scatter alone).
-hipError_t aot_rank_place(const int64_t* keys, const int64_t* rows, int64_t n, int64_t base,
-                          const void* bits, const void* cum, int64_t* keys_out, int64_t* rows_out,
-                          hipStream_t s) {
+// Places (key, row) at each key's rank (rf_rank_scatter alone); n_dev as in
+// aot_rank_build_small.
+hipError_t aot_rank_place(const int64_t* keys, const int64_t* rows, int64_t n, const int64_t* n_dev,
+                          int64_t base, const void* bits, const void* cum, int64_t* keys_out,
+                          int64_t* rows_out, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(rf_rank_scatter, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, rows, n,
-                       base, (const uint64_t*)bits, (const uint32_t*)cum, keys_out, rows_out);
+                       n_dev, base, (const uint64_t*)bits, (const uint32_t*)cum, keys_out,
+                       rows_out);
     return hipGetLastError();
 }
 
