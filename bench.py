@@ -34,7 +34,8 @@ def log(msg: str) -> None:
 
 
 PROFILES = {"c2": "scan_c2_60m_rocprof.md", "c3": "group_agg_c3_60m_rocprof.md",
-            "c4": "group_join_c4_rocprof.md", "merge": "union_merge_30m_rocprof.md"}
+            "c4": "group_join_c4_rocprof.md", "merge": "union_merge_30m_rocprof.md",
+            "dimjoin": "group_join_dim_60m_rocprof.md"}
 
 
 def measured_traffic(workload: str = "c2"):
@@ -66,7 +67,8 @@ def main() -> int:
     ap.add_argument("--cpu-rows", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "tpch", "programs", "merge"],
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "tpch", "programs", "merge",
+                                           "dimjoin"],
                     default="c2",
                     help="c2 (default): the 60M-row scan-filter-aggregate + selection; c3: the "
                          "60M-row grouped aggregate; c4: the 1.5M x 60M hash join + grouped "
@@ -74,7 +76,9 @@ def main() -> int:
                          "against plain per-query launches; tpch: TPC-H Q3/Q4/Q12/Q18 over a "
                          "run-end encoded lineitem; programs: reffine's PageRank and Nbody "
                          "steps and TPC-DS Q9 (benchmark/include/misc.h, tpch.h); merge: "
-                         "micro.h's outer join (a union merge) of two 30M-key vectors")
+                         "micro.h's outer join (a union merge) of two 30M-key vectors; "
+                         "dimjoin: a 60M-row batch joined with the 2556-row date dimension, "
+                         "grouped by day (pre-aggregation in LDS against the sort + fold)")
     ap.add_argument("--orders", type=int, default=15_000_000,
                     help="tpch: orders (lineitem has about 3.75 lines per order)")
     ap.add_argument("--parts", type=int, default=2_000_000,
@@ -114,6 +118,8 @@ def main() -> int:
         return bench_programs(args)
     if args.workload == "merge":
         return bench_merge(args)
+    if args.workload == "dimjoin":
+        return bench_dimjoin(args)
     n = args.rows
     log(f"generating {n} rows")
     c = datagen.scan_table(n, args.seed, with_q=True, rowid_dtype=np.int32)
@@ -611,6 +617,64 @@ def bench_merge(args) -> int:
     line["roofline"]["traffic_source"] = "profiles/" + PROFILES["merge"]
     print(json.dumps(line), flush=True)
     return 0 if ok else 1
+
+
+def bench_dimjoin(args) -> int:
+    """The C5 join template at full size: a 60M-row batch (grouped layout)
+    joined with the 2556-row date dimension on d = dk for the days with
+    dflag == 1, per day SUM(price*(1-disc)) and COUNT over qty < 25. The
+    default mode pre-aggregates the joined rows in LDS (few groups); the
+    deterministic mode sorts them and folds each day in row order. Both
+    verified against the oracle (the default within 1e-12, the
+    deterministic one bit for bit)."""
+    from oracle import brute
+    from reffine_amd import DeviceTable, compile_op, datagen, workloads
+    n = args.rows
+    log(f"dimjoin: {n} rows x 2556 days")
+    c = datagen.group_table(n, 7)
+    dim = datagen.dim_table()
+    T = DeviceTable.from_columns([c["rowid"], c["f1"], c["f2"], c["qty"], c["price"], c["disc"],
+                                  c["tax"], c["d"]], dim=1)
+    D = DeviceTable.from_columns([dim["dk"], dim["dflag"]], dim=1)
+    f, qmax = 1, 25
+    keys, sums, counts = brute.dim_join(dim, c, f, qmax)
+    fn = workloads.dim_join_template()
+    out_line = None
+    ok_all = True
+    for det in (False, True):
+        q = compile_op(fn, deterministic=det)
+
+        def cold():
+            q.clear_cache()
+            return q(D, T, f, qmax)
+        if args.counter_pass and not det:
+            return _counter_pass(cold, q)
+        out, ms = _timed(cold, args.steps, args.warmup)
+        _, warm_ms = _timed(lambda: q(D, T, f, qmax), args.steps, args.warmup)
+        s_ = out.column(1)
+        ok = np.array_equal(out.column(0), keys) and np.array_equal(out.column(2), counts)
+        ok = ok and (bool(np.array_equal(s_.view(np.int64), sums.view(np.int64))) if det else
+                     bool(np.all(np.abs(s_ - sums) <= 1e-12 * np.abs(sums))))
+        ok_all = ok_all and ok
+        run_ms, scan_ms = q.time_kernels(D, T, f, qmax, iters=max(args.steps, 10))
+        nbytes = 28 * n + 9 * len(dim["dk"]) + 24 * len(keys)
+        cfg = {"workload": f"date-dimension join, {n} rows x 2556 days (dflag == {f}), per day "
+                           f"SUM(price*(1-disc)), COUNT over qty < {qmax}",
+               "mode": "deterministic (sort + fold)" if det else "default (LDS pre-aggregation)",
+               "groups": int(len(keys)), "joined_rows": int(counts.sum()), "verified": bool(ok),
+               "ms_per_query_build_resident": round(warm_ms, 4), "probe_scan_ms": round(scan_ms, 5)}
+        line = _line(cfg, nbytes / (ms * 1e-3) / 1e9, ms, args, 28 * n / (scan_ms * 1e-3) / 1e9,
+                     "rf_q (probe scan: qty, price, disc, d read, 28 B per row)", scan_ms, run_ms)
+        line["metric"] = "grouped join throughput"
+        if out_line is None:
+            line["roofline"]["traffic"] = measured_traffic("dimjoin")
+            line["roofline"]["traffic_source"] = "profiles/" + PROFILES["dimjoin"]
+            out_line = line
+        else:
+            out_line["deterministic"] = {k: line[k] for k in ("value", "ms_per_step", "config",
+                                                             "roofline")}
+    print(json.dumps(out_line), flush=True)
+    return 0 if ok_all else 1
 
 
 C5_TEMPLATES = {"q6": "q6_template", "q1": "group_agg_c3", "join": "dim_join_template"}

@@ -517,6 +517,8 @@ Module& jit_module(const KernelSource& ks) {
         RF_HIP(hipModuleGetFunction(&m->fold_wb, m->mod, "rf_gj_fold_wb"));
         if (ks.code.find("void rf_gj_agg(") != std::string::npos)
             RF_HIP(hipModuleGetFunction(&m->agg, m->mod, "rf_gj_agg"));
+        if (ks.code.find("void rf_gj_lds_done(") != std::string::npos)
+            RF_HIP(hipModuleGetFunction(&m->lds_done, m->mod, "rf_gj_lds_done"));
     }
     if (ks.entry == "rf_qu_count") RF_HIP(hipModuleGetFunction(&m->emit, m->mod, "rf_qu_emit"));
     if (ks.entry == "rf_q") {
@@ -1229,6 +1231,7 @@ GroupJoinState::~GroupJoinState() {
     dev_free(part_small);
     dev_free(prank);
     for (void* p : ppay) dev_free(p);
+    dev_free(lds_part);
 }
 
 GjBuild::~GjBuild() {
@@ -1321,10 +1324,16 @@ std::vector<KernelSource> group_join_sources(const Plan& plan,
     const GroupJoinPlan& gp = *plan.gj;
     Plan dp = gp.probe_plan;
     dp.dense_probes = true;
-    return {gen_scan(gp.build_plan, nullable, default_tuning(gp.build_plan), false),
-            gen_scan(dp, nullable, default_tuning(dp), false),
-            gen_scan(gp.probe_plan, nullable, default_tuning(gp.probe_plan), false),
-            gen_group_fold(gp, nullable)};
+    std::vector<KernelSource> v = {gen_scan(gp.build_plan, nullable, default_tuning(gp.build_plan), false),
+                                   gen_scan(dp, nullable, default_tuning(dp), false),
+                                   gen_scan(gp.probe_plan, nullable, default_tuning(gp.probe_plan), false),
+                                   gen_group_fold(gp, nullable)};
+    if (gj_agg_ok(gp)) {
+        Plan lp = dp;
+        lp.lds_agg = &gp;
+        v.push_back(gen_scan(lp, nullable, default_tuning(dp), false));
+    }
+    return v;
 }
 
 Spec& gj_fold_spec(rf_query* q, const rf_arg* args, int64_t n_args) {
@@ -1750,6 +1759,74 @@ static rf_table* group_agg(rf_query* q, const Spec& fs, const GjBuild& B, const 
     return out.release();
 }
 
+// LDS bytes a probe block may give its pre-aggregation slots (with the
+// scan's own LDS, within the 64 KiB a block gets without opting in).
+static constexpr int64_t kLdsAggBytes = 48 * 1024;
+
+// Updated fields of a group join's state (LDS slots per group).
+static int gj_slots(const GroupJoinPlan& gp) {
+    int n = 0;
+    for (auto& f : gp.fields) n += f.updated ? 1 : 0;
+    return n;
+}
+
+// The default mode's group fold for few groups (at most kLdsAggBytes of
+// slots per block, a rank-map build): the probe scan folds every joined row
+// into its block's LDS slots with LDS atomics and writes the slots once per
+// block; rf_gj_lds_done combines the blocks' slots per group and writes the
+// outputs. No joined row is staged, partitioned or sorted. Float sums are
+// in the atomics' order (the default mode's 1e-12 bar).
+static rf_table* group_lds_agg(rf_query* q, const Spec& fs, const GjBuild& B, const rf_arg* args,
+                               int64_t n_args, int64_t Kb, GjTiming* tm) {
+    Device& d = device();
+    hipStream_t s = d.stream;
+    GroupJoinState& G = *q->gj;
+    const GroupJoinPlan& gp = *q->plan.gj;
+    const int nslot = gj_slots(gp);
+    if (!G.probe_q_lds) {
+        Plan lp = gp.probe_plan;
+        lp.dense_probes = true;
+        lp.lds_agg = q->plan.gj.get();
+        G.probe_q_lds = sub_query(lp, G.probe_q->tune);
+        G.probe_q_lds->probe_override.resize(gp.probe_plan.probes.size());
+    }
+    rf_query* pl = G.probe_q_lds.get();
+    const RfProbeDev h = B.probe();
+    for (size_t j = 0; j < gp.probe_plan.probes.size(); j++)
+        if (gp.probe_plan.probes[j].external) pl->probe_override[j] = h;
+    Launch lp = prepare(pl, args, n_args, nullptr);
+    const size_t lds = (size_t)nslot * (size_t)Kb * 8;
+    if (!lp.empty) {
+        grow(G.lds_part, G.lds_part_cap, (size_t)lp.grid * lds);
+        lp.args.agg_gpp = Kb;
+        lp.args.agg_part = (unsigned long long*)G.lds_part;
+        void* params[] = {&lp.args};
+        if (tm) RF_HIP(hipEventRecord(tm->scan0, s));
+        RF_HIP(hipModuleLaunchKernel(lp.spec->mod->fn, lp.grid, 1, 1, lp.spec->ks.tune.block, 1, 1,
+                                     (unsigned)lds, s, params, nullptr));
+        if (tm) RF_HIP(hipEventRecord(tm->scan1, s));
+    } else if (tm) {
+        RF_HIP(hipEventRecord(tm->scan0, s));
+        RF_HIP(hipEventRecord(tm->scan1, s));
+    }
+    std::unique_ptr<rf_table> out(alloc_table(gp.out_types, Kb, 1));
+    RfFoldArgs F;
+    gj_fold_args(q, fs, B, args, n_args, F);
+    F.groups = Kb;
+    for (size_t j = 0; j < out->cols.size(); j++) F.out[j] = out->cols[j].values;
+    RfAggArgs GA{};
+    GA.groups_dev = B.pending_groups ? B.groups_dev : nullptr;
+    GA.bpart = (const unsigned long long*)G.lds_part;
+    GA.nblk = lp.empty ? 0 : lp.grid;
+    GA.gpp = Kb;
+    void* params[] = {&F, &GA};
+    const int W = gj_lds_done_waves(nslot);
+    RF_HIP(hipModuleLaunchKernel(fs.mod->lds_done, (unsigned)((Kb + 15) / 16), 1, 1, W * 64, 1, 1, 0,
+                                 s, params, nullptr));
+    RF_HIP(hipStreamSynchronize(s));
+    return out.release();
+}
+
 // Runs a group join (see GroupJoinPlan): the build side from the cache (or
 // built), then the probe scan, a stable sort of the joined rows by group
 // rank, segment bounds and the per-group fold. Two host synchronisations
@@ -1776,8 +1853,16 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
         for (Column& c : t->cols) c.length = c.logical_length = B->groups;
         return t;
     };
-    rf_query* pq = gj_probe_query(q, *B);
     const Spec& fs = gj_fold_spec(q, args, n_args);
+    {
+        // Few groups (by the count or its bound): pre-aggregation in LDS.
+        const int64_t Kb = B->pending_groups ? B->groups_ub : B->groups;
+        const int nslot = gj_slots(gp);
+        if (!q->det && fs.mod->lds_done && B->dense_bits && nslot > 0 && Kb > 0 &&
+            Kb * nslot * 8 <= kLdsAggBytes)
+            return verified(group_lds_agg(q, fs, *B, args, n_args, Kb, tm));
+    }
+    rf_query* pq = gj_probe_query(q, *B);
     const RfProbeDev h = B->probe();
     for (size_t j = 0; j < gp.probe_plan.probes.size(); j++)
         if (gp.probe_plan.probes[j].external) pq->probe_override[j] = h;

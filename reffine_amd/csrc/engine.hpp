@@ -159,6 +159,7 @@ struct Module {
     hipFunction_t fold_w = nullptr;  // rf_gj_fold_w / _wb: a wave per group (large groups)
     hipFunction_t fold_wb = nullptr;
     hipFunction_t agg = nullptr;     // rf_gj_agg: partitioned LDS fold (default mode)
+    hipFunction_t lds_done = nullptr; // rf_gj_lds_done: outputs after LDS pre-aggregation
     hipFunction_t emit = nullptr;    // rf_qu_emit: a union merge's rows (entry rf_qu_count)
     hipFunction_t seg_long = nullptr; // rf_seg_fold_long: run folds with long runs
     hipFunction_t seg_wave = nullptr; // rf_seg_fold_wave: a wave per run or window
@@ -323,6 +324,9 @@ struct GroupJoinState {
     // another launch shape than against a direct map: compiled on first use
     // unless the caller fixed the tuning.
     std::unique_ptr<rf_query> probe_q_hash;
+    // The probe scan against a rank map that pre-aggregates the joined rows
+    // in LDS (default mode, few groups): compiled on first use.
+    std::unique_ptr<rf_query> probe_q_lds;
     bool probe_tuned = false;
     std::map<std::string, Spec> fold_specs;  // by nullability signature
     int table_scale = 2;                     // index capacity >= scale * groups
@@ -344,6 +348,8 @@ struct GroupJoinState {
     void* prank = nullptr;
     std::vector<void*> ppay;
     size_t part_cap = 0;
+    void* lds_part = nullptr;                // pre-aggregation: the blocks' LDS slots
+    size_t lds_part_cap = 0;
     // Build sides kept resident, by build key (the build table's uid and the
     // build plan's scalar arguments); the oldest is dropped beyond kBuilds.
     static constexpr size_t kBuilds = 16;

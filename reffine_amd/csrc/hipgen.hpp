@@ -57,6 +57,7 @@ struct KernelSource {
     std::vector<int> acc_offset;   // byte offset of each Acc field
     int acc_bytes = 0;
     bool has_sel = false;
+    int lds_agg_slots = 0;         // a probe that pre-aggregates in LDS: slots per group
     std::vector<Type> out_types;   // selection output columns
     std::vector<Type> stage_types; // staged per selected row: row index, carried probe results
     Tuning tune;
@@ -80,6 +81,8 @@ KernelSource gen_scan(const Plan& plan,
 // Whether a group join's fields fold with LDS atomics in the default mode
 // (rf_gj_agg): sums of integers or floats, integer min / max.
 bool gj_agg_ok(const GroupJoinPlan& gj);
+std::string gj_slot_identity(const GField& gf);
+int gj_lds_done_waves(int nslot);
 KernelSource gen_group_fold(const GroupJoinPlan& gj,
                             const std::vector<std::vector<bool>>& nullable);
 

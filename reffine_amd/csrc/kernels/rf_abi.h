@@ -79,6 +79,10 @@ struct RfScanArgs {
     void* out[RF_MAXOUT];
     int64_t* out_count;
     int64_t out_cap;      // rows the out columns hold (placement stops there)
+    // Group join pre-aggregation in LDS: slots per field (groups), and where
+    // each block writes its slots ([block][field slot][group]).
+    int64_t agg_gpp;
+    unsigned long long* agg_part;
     union RfParam p[RF_MAXPARAMS];
 };
 
@@ -177,6 +181,10 @@ struct RfAggArgs {
     // When set, the number of groups (RfFoldArgs::groups is then a bound):
     // a count an earlier kernel wrote.
     const int64_t* groups_dev;
+    // rf_gj_lds_done: the probe scan's block slots ([nblk][field slot][gpp]).
+    const unsigned long long* bpart;
+    int64_t nblk;
+    int64_t gpp;
 };
 
 // Batched execution: a persistent launch over (work unit, wave-tile) items.

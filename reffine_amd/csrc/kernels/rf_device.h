@@ -475,6 +475,22 @@ __device__ __forceinline__ void rf_tile_emit_lds(const RfScanArgs& L, int64_t ro
     cursor += n;
 }
 
+// A load group's joined rows folded into the block's LDS slots (a group
+// join's probe pre-aggregating in LDS): no staging.
+template <class Q>
+__device__ __forceinline__ void rf_tile_agg(const RfScanArgs& L, const uint32_t* bits,
+                                            const typename Q::Carry& cy, unsigned long long* acc) {
+    constexpr int ITEMS = Q::ITEMS, LSUB = Q::LSUB;
+#pragma unroll
+    for (int s = 0; s < LSUB; s++) {
+        const uint32_t b = bits[s];
+        if (!b) continue;
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++)
+            if ((b >> k) & 1u) Q::agg_row(L, acc, L.agg_gpp, cy, s, k);
+    }
+}
+
 // Scan kernel. Wave w (of W = gridDim.x * BLOCK/64) owns wave-tiles
 // [w*per, min((w+1)*per, ntiles)) with per = ceil(ntiles / W), processed as
 // a sequence of load groups.
@@ -497,6 +513,12 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
     __shared__ uint64_t s_scan[WAVES];
     __shared__ uint32_t s_last;
     __shared__ typename Q::EmitBuf s_emit[Q::LDS_EMIT ? WAVES : 1];  // per-wave emit buffers
+    extern __shared__ unsigned long long rf_agg_lds[];  // LDS_AGG: NSLOT x agg_gpp slots
+    if constexpr (Q::LDS_AGG) {
+        for (int64_t i = threadIdx.x; i < (int64_t)Q::NSLOT * L.agg_gpp; i += BLOCK)
+            rf_agg_lds[i] = Q::agg_ident((int)(i / L.agg_gpp));
+        __syncthreads();
+    }
 
     const uint32_t lane = __lane_id();
     const int64_t gw = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
@@ -523,6 +545,7 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
         uint32_t bits[LSUB];
         typename Q::Carry cy;  // per-row values the selection's emit reuses
         rf_tile_rows<Q>(L, T, row0, interior, lane, a, bits, cy);
+        if constexpr (Q::LDS_AGG) rf_tile_agg<Q>(L, bits, cy, rf_agg_lds);
         if constexpr (Q::HAS_SEL) {
             if constexpr (Q::LDS_EMIT)
                 rf_tile_emit_lds<Q>(L, row0, lane, bits, cy, gw * L.stage_rows, cursor,
@@ -571,6 +594,13 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
 
     if constexpr (Q::HAS_SEL) {
         if (lane == 0) L.wave_cnt[gw] = cursor;
+    }
+    if constexpr (Q::LDS_AGG) {
+        // The block's slots, for rf_gj_lds_done to combine across blocks.
+        __syncthreads();
+        unsigned long long* dst = L.agg_part + (int64_t)blockIdx.x * Q::NSLOT * L.agg_gpp;
+        for (int64_t i = threadIdx.x; i < (int64_t)Q::NSLOT * L.agg_gpp; i += BLOCK)
+            dst[i] = rf_agg_lds[i];
     }
     if constexpr (Q::NACC > 0 && !Q::DET) {
         rf_wave_reduce<Q>(run);

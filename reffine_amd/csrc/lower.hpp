@@ -114,6 +114,10 @@ struct Plan {
     // table and the output expression over the components.
     bool composite = false;
     std::shared_ptr<GroupJoinPlan> gj;  // set: the function is a group join
+    // Set on a group join's probe plan compiled for pre-aggregation in LDS:
+    // the scan folds each joined row into its block's LDS slots (one per
+    // group and updated field of *lds_agg) instead of staging it.
+    const GroupJoinPlan* lds_agg = nullptr;
     std::shared_ptr<SegReducePlan> seg; // set: a reduction per run of a run-end column
     std::shared_ptr<RunStagePlan> runs; // set: a runs table is built first (T's place)
 

@@ -111,8 +111,11 @@ def test_group_join_lowers_and_compiles():
     assert "group join: build table input 0, probe table input 1" in text
     assert "group field 0 f64 add of payload 0" in text
     assert "group field 1 i64 add of (const i64 1)" in text
-    # build scan, probe scan against a rank map and against a hash index, fold
-    assert text.count("--- kernel ---") == 4
+    # build scan, probe scan against a rank map and against a hash index, fold,
+    # and the rank-map probe that pre-aggregates in LDS (few groups)
+    assert text.count("--- kernel ---") == 5
+    assert "enum { LDS_AGG = 1, NSLOT = 2 };" in text
+    assert "void rf_gj_lds_done(" in text
     assert "? rf_probe(" in text  # probed only where the probe predicate holds
     assert runtime.compile_check(workloads.group_join_c4(), nullable=True)
 

@@ -238,3 +238,90 @@ def test_group_join_build_cache_is_keyed_and_clearable(device):
     q.clear_cache()
     q.clear_cache()
     _check(q(B1, P), b1, p)
+
+
+def _five_field_join():
+    """group_join_c4 with a five-field state: SUM(pv*pq) f64, COUNT, MAX(pq)
+    from -1, MIN(pq) from 1000 and COUNT of the rows with pv < 0.5 (a
+    predicated field)."""
+    from reffine_amd.reffiner import (_cast, _eq, _f64, _f64_t, _func, _get, _gt, _i8, _i64,
+                                      _i64_t, _in, _lt, _max, _min, _mul, _new, _op, _red,
+                                      _sel, _sym, _add)
+    B = _sym("B", workloads.C4_BUILD_TYPE)
+    P = _sym("P", workloads.C4_PROBE_TYPE)
+    t = _sym("t", _i64_t)
+    i = _sym("i", _i64_t)
+    pe = P[i]
+    rows = _op([i], _in(i, P) & _eq(i, t) & _gt(pe[1], _i64(10)),
+               [_mul(pe[0], _cast(_f64_t, pe[1])), pe[1], pe[0]])
+
+    def acc(s, r):
+        return _new([_add(_get(s, 0), r[1]), _add(_get(s, 1), _i64(1)), _max(_get(s, 2), r[2]),
+                     _min(_get(s, 3), r[2]),
+                     _sel(_lt(r[3], _f64(0.5)), _add(_get(s, 4), _i64(1)), _get(s, 4))])
+
+    red = _red(rows, lambda: _new([_f64(0), _i64(0), _i64(-1), _i64(1000), _i64(0)]), acc)
+    grp = _sym("grp", red)
+    be = B[t]
+    out = _op([t], _in(t, B) & _eq(be[1], _i8(1)),
+              [_get(grp, 0), _get(grp, 1), _get(grp, 2), _get(grp, 3), _get(grp, 4)])
+    os_ = _sym("out", out)
+    fn = _func("five_field_join", os_, [B, P])
+    fn.tbl[grp] = red
+    fn.tbl[os_] = out
+    return fn
+
+
+def _five_field_oracle(b, p):
+    keys = np.sort(np.asarray(b["bk"])[np.asarray(b["bf"]) == 1])
+    pk, pv, pq = (np.asarray(p[c]) for c in ("pk", "pv", "pq"))
+    m = pq > 10
+    pos = np.searchsorted(keys, pk[m])
+    ok = (pos < len(keys)) & (keys[np.minimum(pos, max(len(keys) - 1, 0))] == pk[m])
+    g, v, q = pos[ok], pv[m][ok], pq[m][ok]
+    sums = np.zeros(len(keys))
+    np.add.at(sums, g, v * q.astype(np.float64))
+    counts = np.bincount(g, minlength=len(keys)).astype(np.int64)
+    mx = np.full(len(keys), -1, dtype=np.int64)
+    np.maximum.at(mx, g, q)
+    mn = np.full(len(keys), 1000, dtype=np.int64)
+    np.minimum.at(mn, g, q)
+    low = np.bincount(g[v < 0.5], minlength=len(keys)).astype(np.int64)
+    return keys, sums, counts, mx, mn, low
+
+
+# 300 and 1000 build rows: few groups (five 8-byte slots per group within
+# 48 KiB of LDS per block), the probe scan pre-aggregates in LDS (default
+# mode); 20000: the partitioned fold. Deterministic: sort + fold.
+@pytest.mark.parametrize("nb", [300, 1000, 20_000])
+@pytest.mark.parametrize("det", [False, True])
+def test_group_join_five_fields(device, nb, det):
+    b, p = datagen.join_tables(nb, 400_000, seed=nb + 17)
+    B, P = _upload(b, p)
+    q = compile_op(_five_field_join(), deterministic=det)
+    want = _five_field_oracle(b, p)
+    for _ in range(2):  # the second run: the build side from the cache
+        out = q(B, P)
+        got = [out.column(j) for j in range(6)]
+        assert np.array_equal(got[0], want[0])
+        for j in (2, 3, 4, 5):
+            assert np.array_equal(got[j], want[j]), j
+        if det:
+            assert np.array_equal(got[1].view(np.int64), want[1].view(np.int64))
+        else:
+            assert _close(got[1], want[1])
+
+
+def test_group_join_lds_preaggregation_empty_probe(device):
+    """Few groups and no joined row (no probe row passes pq > 10): every
+    qualifying build key gets the reduction's init, as its inner loop over
+    zero rows gives."""
+    b, p = datagen.join_tables(500, 10_000, seed=29)
+    p["pq"] = np.minimum(p["pq"], 10)
+    B, P = _upload(b, p)
+    out = compile_op(_five_field_join())(B, P)
+    keys = np.sort(np.asarray(b["bk"])[np.asarray(b["bf"]) == 1])
+    assert np.array_equal(out.column(0), keys)
+    assert np.all(out.column(1) == 0.0) and np.all(out.column(2) == 0)
+    assert np.all(out.column(3) == -1) and np.all(out.column(4) == 1000)
+    assert np.all(out.column(5) == 0)
