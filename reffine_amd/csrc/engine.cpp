@@ -1771,7 +1771,7 @@ static int gj_slots(const GroupJoinPlan& gp) {
 }
 
 // The default mode's group fold for few groups (at most kLdsAggBytes of
-// slots per block, a rank-map build): the probe scan folds every joined row
+// slots per block): the probe scan folds every joined row
 // into its block's LDS slots with LDS atomics and writes the slots once per
 // block; rf_gj_lds_done combines the blocks' slots per group and writes the
 // outputs. No joined row is staged, partitioned or sorted. Float sums are
@@ -1783,14 +1783,17 @@ static rf_table* group_lds_agg(rf_query* q, const Spec& fs, const GjBuild& B, co
     GroupJoinState& G = *q->gj;
     const GroupJoinPlan& gp = *q->plan.gj;
     const int nslot = gj_slots(gp);
-    if (!G.probe_q_lds) {
+    // The kernel that reads a rank map only ever meets a rank map (as the
+    // staging probe scans, gj_probe_query).
+    std::unique_ptr<rf_query>& slot = B.dense_bits ? G.probe_q_lds : G.probe_q_lds_hash;
+    if (!slot) {
         Plan lp = gp.probe_plan;
-        lp.dense_probes = true;
+        lp.dense_probes = B.dense_bits != nullptr;
         lp.lds_agg = q->plan.gj.get();
-        G.probe_q_lds = sub_query(lp, G.probe_q->tune);
-        G.probe_q_lds->probe_override.resize(gp.probe_plan.probes.size());
+        slot = sub_query(lp, gj_probe_query(q, B)->tune);
+        slot->probe_override.resize(gp.probe_plan.probes.size());
     }
-    rf_query* pl = G.probe_q_lds.get();
+    rf_query* pl = slot.get();
     const RfProbeDev h = B.probe();
     for (size_t j = 0; j < gp.probe_plan.probes.size(); j++)
         if (gp.probe_plan.probes[j].external) pl->probe_override[j] = h;
@@ -1858,8 +1861,7 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
         // Few groups (by the count or its bound): pre-aggregation in LDS.
         const int64_t Kb = B->pending_groups ? B->groups_ub : B->groups;
         const int nslot = gj_slots(gp);
-        if (!q->det && fs.mod->lds_done && B->dense_bits && nslot > 0 && Kb > 0 &&
-            Kb * nslot * 8 <= kLdsAggBytes)
+        if (!q->det && fs.mod->lds_done && nslot > 0 && Kb > 0 && Kb * nslot * 8 <= kLdsAggBytes)
             return verified(group_lds_agg(q, fs, *B, args, n_args, Kb, tm));
     }
     rf_query* pq = gj_probe_query(q, *B);

@@ -324,9 +324,10 @@ struct GroupJoinState {
     // another launch shape than against a direct map: compiled on first use
     // unless the caller fixed the tuning.
     std::unique_ptr<rf_query> probe_q_hash;
-    // The probe scan against a rank map that pre-aggregates the joined rows
-    // in LDS (default mode, few groups): compiled on first use.
-    std::unique_ptr<rf_query> probe_q_lds;
+    // The probe scans that pre-aggregate the joined rows in LDS (default
+    // mode, few groups), against a rank map and against a hash index:
+    // compiled on first use.
+    std::unique_ptr<rf_query> probe_q_lds, probe_q_lds_hash;
     bool probe_tuned = false;
     std::map<std::string, Spec> fold_specs;  // by nullability signature
     int table_scale = 2;                     // index capacity >= scale * groups
