@@ -421,7 +421,10 @@ static void driver_row_shape(const Plan& plan, int& wmax, int& row_bytes, int& n
 int load_cap(const Plan& plan, int items, bool pipelined) {
     int wmax, row_bytes, nacc;
     driver_row_shape(plan, wmax, row_bytes, nacc);
-    const int budget = pipelined ? 640 : 320;
+    // As default_tuning's budget: a pipelined state of more than 8
+    // accumulators keeps half the load bytes in flight (two register sets
+    // of a 36-accumulator state at 640 bytes spilled).
+    const int budget = pipelined ? (nacc <= 8 ? 640 : 320) : 320;
     int sub = 4;
     if (row_bytes > 0) sub = std::max(1, std::min(16, budget / (items * row_bytes)));
     sub = std::min(sub, std::max(1, 32 / items));
@@ -474,16 +477,19 @@ Tuning default_tuning(const Plan& plan) {
     // that saturates HBM on gfx950.
     t.items = wmax >= 8 ? 2 : (wmax == 4 ? 4 : (wmax == 2 ? 8 : 16));
     if (wmax == 0 || plan.source == Plan::RANGE) t.items = 4;
-    // With a small state, wave-tiles are double-buffered (the next tile's
-    // loads overlap this tile's rows) at 8 waves per CU; a large state
-    // (many accumulators) keeps one register set at 16 waves per CU.
+    // Wave-tiles are double-buffered (the next tile's loads overlap this
+    // tile's rows) at 8 waves per CU; a very large state (more than 48
+    // accumulators) keeps one register set at 16 waves per CU.
     int nacc = 0;
     for (auto& r : plan.reduces)
         for (auto& f : r.fields) nacc += f.updated ? 1 : 0;
-    // Up to 16 accumulators (the five 3-field buckets of TPC-DS Q9) still
-    // pipeline, with half the load bytes in flight: 0.143 ms against 0.188 ms
-    // unpipelined on 30M rows.
-    t.pipeline = nacc <= 16 ? 1 : 0;
+    // Up to 16 accumulators (the five 3-field buckets of TPC-DS Q9) pipeline
+    // with half the load bytes in flight: 0.143 ms against 0.188 ms
+    // unpipelined on 30M rows. So does the 36-accumulator grouped scan (six
+    // Q1 groups of six fields, 38-byte rows): 349 us against 369 us at 16
+    // waves per CU unpipelined on 60M rows (one predicate branch per group
+    // and DPP reductions made room for the second register set).
+    t.pipeline = nacc <= 48 ? 1 : 0;
     t.waves_per_cu = t.pipeline ? 8 : 16;
     int budget = t.pipeline ? (nacc <= 8 ? 640 : 320) : 320;  // load bytes per lane per tile
     if (t.pipeline && nacc <= 8) {
