@@ -204,14 +204,36 @@ def test_group_join_nulls(device):
     _check(out, b2, p2)
 
 
-def test_group_join_duplicate_build_keys_rejected(device):
-    b, p = datagen.join_tables(1000, 10_000, seed=2)
+# 1000 build rows: the probe pre-aggregates in LDS; 20000: the partitioned
+# fold. Both builds are dense (no host synchronisation): the duplicate is
+# found at the run's last synchronisation.
+@pytest.mark.parametrize("nb", [1000, 20_000])
+def test_group_join_duplicate_build_keys_rejected(device, nb):
+    b, p = datagen.join_tables(nb, 10_000, seed=2)
     b["bk"] = b["bk"].copy()
-    b["bf"] = np.ones(1000, np.int8)
+    b["bf"] = np.ones(nb, np.int8)
     b["bk"][10] = b["bk"][20]
     B, P = _upload(b, p)
+    q = compile_op(workloads.group_join_c4())
     with pytest.raises(ReffineError, match="more than once"):
-        compile_op(workloads.group_join_c4())(B, P)
+        q(B, P)
+    with pytest.raises(ReffineError, match="more than once"):
+        q(B, P)  # the failed build is not kept
+    good, _ = datagen.join_tables(nb, 10, seed=3)
+    G = DeviceTable.from_columns([good["bk"], good["bd"], good["bf"]], dim=1)
+    _check(q(G, P), good, p)  # the query stays usable
+
+
+@pytest.mark.parametrize("nb", [2000, 20_000])
+def test_group_join_nothing_qualifies(device, nb):
+    """No qualifying build key (an empty output), and no qualifying probe row
+    (every group COUNT 0, SUM 0.0), on both default-mode folds."""
+    b, p = datagen.join_tables(nb, 100_000, seed=nb)
+    B, P = _upload(b, p)
+    assert compile_op(workloads.group_join_c4(bf_eq=9))(B, P).nrows == 0
+    k, s, c = _cols(compile_op(workloads.group_join_c4(pq_gt=100))(B, P))
+    assert np.array_equal(k, np.sort(b["bk"][b["bf"] == 1]))
+    assert not c.any() and not s.any()
 
 
 def test_group_join_timing(device):
