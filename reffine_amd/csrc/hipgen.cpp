@@ -1493,6 +1493,29 @@ std::string gj_slot_identity(const GField& gf) {
                                  : "(unsigned long long)(long long)" + id;
 }
 
+// Payload columns the group fold reads.
+static std::set<int> gj_fold_cols(const GroupJoinPlan& gj) {
+    std::set<int> used;
+    for (auto& gf : gj.fields) {
+        if (!gf.updated) continue;
+        if (!gf.const_contrib) used.insert(gf.pay);
+        if (gf.pred_pay >= 0) used.insert(gf.pred_pay);
+    }
+    return used;
+}
+
+// Rows per block of the wave fold's LDS buffer: 256, or 128 with more than
+// four payload columns (four waves' buffers within 32 KiB).
+int gj_fold_wave_rows(const GroupJoinPlan& gj) { return gj_fold_cols(gj).size() <= 4 ? 256 : 128; }
+
+// Bytes of one wave's buffer (each column's block, 8-byte aligned).
+static size_t gj_fold_wave_bytes(const GroupJoinPlan& gj) {
+    size_t off = 0;
+    const int R = gj_fold_wave_rows(gj);
+    for (int j : gj_fold_cols(gj)) off += ((size_t)R * gj.pay_types[j].width() + 7) / 8 * 8;
+    return std::max<size_t>(off, 8);
+}
+
 // Waves per block of rf_gj_lds_done: its LDS totals (8 bytes per slot,
 // 4 per wave, and group of 16) within 32 KiB.
 int gj_lds_done_waves(int nslot) { return nslot <= 4 ? 16 : (nslot <= 8 ? 8 : 4); }
@@ -1526,8 +1549,12 @@ KernelSource gen_group_fold(const GroupJoinPlan& gj, const std::vector<std::vect
     auto fold_fn = [&](bool wave) {
         std::ostringstream o;
         o << "__device__ __forceinline__ void " << (wave ? "rf_gj_fold_wave" : "rf_gj_fold_one")
-          << "(const RfFoldArgs& L, const int64_t grp) {\n    {\n";
-        o << "        const int64_t i0 = L.seg_lo[grp], i1 = L.seg_hi[grp];\n";
+          << "(const RfFoldArgs& L, const int64_t grp" << (wave ? ", unsigned char* wbuf" : "")
+          << ") {\n    {\n";
+        if (wave)
+            o << "        const int64_t i0 = rf_scalar64(L.seg_lo[grp]), i1 = rf_scalar64(L.seg_hi[grp]);\n";
+        else
+            o << "        const int64_t i0 = L.seg_lo[grp], i1 = L.seg_hi[grp];\n";
         std::ostringstream init;
         Emitter ei(g, Mode::FINAL, init, "        ");
         std::vector<std::string> iv;
@@ -1572,42 +1599,49 @@ KernelSource gen_group_fold(const GroupJoinPlan& gj, const std::vector<std::vect
                 if (!gf.const_contrib) used.insert(gf.pay);
                 if (gf.pred_pay >= 0) used.insert(gf.pred_pay);
             }
-            // Chunks of 64 rows, four at a time: the next four chunks' loads are
-            // issued before the current four are folded (a lone chunk per
-            // iteration waited on its own load: 576 us for 512 groups of 11K
-            // rows). The additions and their order do not change.
+            // Blocks of R rows through the wave's LDS buffer: the lanes load a
+            // block coalesced (64 rows per load), store it to LDS and issue
+            // the next block's loads; then every lane folds the block in row
+            // order, each row an LDS broadcast (same address in every lane).
+            // The group's bounds are wave-uniform, so the row loop is a
+            // scalar loop. (Rows broadcast from lanes with readlane, in a
+            // loop whose bound the compiler took for a per-lane one: 513 us
+            // for 512 groups of 11K rows.) Same additions, same order.
+            const int R = gj_fold_wave_rows(gj);
+            const std::string RS = std::to_string(R), DS = std::to_string(R / 64);
             o << "        const int lane = (int)(threadIdx.x & 63);\n";
+            size_t off = 0;
             for (int j : used) {
                 const std::string T = cty(gj.pay_types[j]), J = std::to_string(j);
-                o << "        " << T << " v" << J << "[4], n" << J << "[4];\n";
+                o << "        " << T << "* b" << J << " = (" << T << "*)(wbuf + " << off << ");\n";
+                o << "        " << T << " r" << J << "[" << DS << "];\n";
+                off += ((size_t)R * gj.pay_types[j].width() + 7) / 8 * 8;
             }
-            auto load4 = [&](const std::string& dst, const std::string& at) {
-                o << "            #pragma unroll\n            for (int d = 0; d < 4; d++) {\n"
-                  << "                const int64_t i = " << at << " + d * 64 + lane;\n";
+            auto loadR = [&](const std::string& at, const std::string& ind) {
+                o << ind << "#pragma unroll\n" << ind << "for (int d = 0; d < " << DS << "; d++) {\n"
+                  << ind << "    const int64_t i = " << at << " + d * 64 + lane;\n";
                 for (int j : used) {
                     const std::string T = cty(gj.pay_types[j]), J = std::to_string(j);
-                    o << "                " << dst << J << "[d] = i < i1 ? ((const " << T
-                      << "*)L.pay[" << J << "])[i] : (" << T << ")0;\n";
+                    o << ind << "    r" << J << "[d] = i < i1 ? ((const " << T << "*)L.pay[" << J
+                      << "])[i] : (" << T << ")0;\n";
                 }
-                o << "            }\n";
+                o << ind << "}\n";
             };
-            o << "        {\n";
-            load4("v", "i0");
-            o << "        }\n";
-            o << "        for (int64_t c0 = i0; c0 < i1; c0 += 256) {\n";
-            load4("n", "c0 + 256");
-            o << "            #pragma unroll\n            for (int d = 0; d < 4; d++) {\n"
-                 "                const int64_t b0 = c0 + d * 64;\n"
-                 "                if (b0 >= i1) break;\n"
-                 "                const int n = (int)(i1 - b0 < 64 ? i1 - b0 : 64);\n"
-                 "                for (int k = 0; k < n; k++) {\n";
-            updates("                    ", [&](int j) {
-                return "rf_readlane(v" + std::to_string(j) + "[d], k)";
-            });
-            o << "                }\n            }\n";
-            o << "            #pragma unroll\n            for (int d = 0; d < 4; d++) {\n";
-            for (int j : used) o << "                v" << j << "[d] = n" << j << "[d];\n";
-            o << "            }\n        }\n";
+            loadR("i0", "        ");
+            o << "        for (int64_t c0 = i0; c0 < i1; c0 += " << RS << ") {\n";
+            o << "            #pragma unroll\n            for (int d = 0; d < " << DS << "; d++) {\n";
+            for (int j : used) o << "                b" << j << "[d * 64 + lane] = r" << j << "[d];\n";
+            o << "            }\n";
+            loadR("c0 + " + RS, "            ");
+            o << "            __builtin_amdgcn_wave_barrier();\n";
+            o << "            const int n = (int)(i1 - c0 < " << RS << " ? i1 - c0 : " << RS << ");\n";
+            o << "            #pragma unroll 16\n            for (int k = 0; k < n; k++) {\n";
+            for (int j : used) {
+                const std::string T = cty(gj.pay_types[j]), J = std::to_string(j);
+                o << "                const " << T << " v" << J << " = b" << J << "[k];\n";
+            }
+            updates("                ", [&](int j) { return "v" + std::to_string(j); });
+            o << "            }\n            __builtin_amdgcn_wave_barrier();\n        }\n";
         }
         o << "        const int64_t key = L.keys[grp];\n        const int64_t brow = L.brow[grp];\n"
              "        (void)key; (void)brow;\n";
@@ -1825,10 +1859,15 @@ KernelSource gen_group_fold(const GroupJoinPlan& gj, const std::vector<std::vect
          "         grp += (int64_t)gridDim.x * 256)\n"
          "        rf_gj_fold_one(L, grp);\n"
          "}\n";
+    // The wave folds' LDS buffers: one per wave of the block.
+    const std::string wlds = "    __shared__ unsigned long long rf_fold_lds[4][" +
+                             std::to_string(gj_fold_wave_bytes(gj) / 8) +
+                             "];\n    unsigned char* wbuf = (unsigned char*)rf_fold_lds[threadIdx.x >> 6];\n";
     o << "extern \"C\" __global__ __launch_bounds__(256) void rf_gj_fold_w(RfFoldArgs L) {\n"
-         "    for (int64_t grp = blockIdx.x * (int64_t)4 + (threadIdx.x >> 6); grp < L.groups;\n"
+      << wlds
+      << "    for (int64_t grp = rf_scalar64(blockIdx.x * (int64_t)4 + (threadIdx.x >> 6)); grp < L.groups;\n"
          "         grp += (int64_t)gridDim.x * 4)\n"
-         "        rf_gj_fold_wave(L, grp);\n"
+         "        rf_gj_fold_wave(L, grp, wbuf);\n"
          "}\n";
     o << "extern \"C\" __global__ __launch_bounds__(256) void rf_gj_fold_b(RfFoldBatch B) {\n"
          "    const int64_t total = B.gstart[B.n];\n"
@@ -1839,11 +1878,12 @@ KernelSource gen_group_fold(const GroupJoinPlan& gj, const std::vector<std::vect
          "    }\n"
          "}\n";
     o << "extern \"C\" __global__ __launch_bounds__(256) void rf_gj_fold_wb(RfFoldBatch B) {\n"
-         "    const int64_t total = B.gstart[B.n];\n"
+      << wlds
+      << "    const int64_t total = B.gstart[B.n];\n"
          "    for (int64_t g = rf_scalar64(blockIdx.x * (int64_t)4 + (threadIdx.x >> 6)); g < total;\n"
          "         g += (int64_t)gridDim.x * 4) {\n"
       << find_run
-      << "        rf_gj_fold_wave(*(const RfFoldArgs*)(B.list + rf_scalar64(lo)), g - B.gstart[lo]);\n"
+      << "        rf_gj_fold_wave(*(const RfFoldArgs*)(B.list + rf_scalar64(lo)), g - B.gstart[lo], wbuf);\n"
          "    }\n"
          "}\n";
     ks.code = o.str();
