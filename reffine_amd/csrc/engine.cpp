@@ -1653,18 +1653,15 @@ void gj_fold_args(rf_query* q, const Spec& fs, const GjBuild& b, const rf_arg* a
     fill_params(fs.ks, c, A.p);
 }
 
-// The default mode's group fold after the probe scan (no gather, no sort):
-// the staged joined rows are moved into partitions of 2^shift consecutive
-// ranks (rf_part_hist, rf_part_scatter) and a block per partition folds its
-// groups with LDS atomics and writes their outputs (rf_gj_agg). Returns
-// null (nothing launched) when the groups do not suit it: the caller then
-// sorts.
-static rf_table* group_agg(rf_query* q, const Spec& fs, const GjBuild& B, const rf_arg* args,
-                           int64_t n_args, rf_query* pq, Launch& lp) {
-    Device& d = device();
-    hipStream_t s = d.stream;
-    GroupJoinState& G = *q->gj;
-    const GroupJoinPlan& gp = *q->plan.gj;
+// The partitioned fold's layout: 2^shift groups per partition, nparts
+// partitions (sized by the group count's bound while it is on the device).
+struct PartLayout {
+    int shift = 0;
+    int64_t nparts = 0;
+};
+
+// False when the groups do not suit the partitioned fold (the caller sorts).
+static bool part_layout(const GroupJoinPlan& gp, const GjBuild& B, PartLayout& lay) {
     // A build whose key count is still on the device is sized by its bound
     // (the build table's rows); the fold reads the count itself.
     const bool pend = B.pending_groups != nullptr;
@@ -1688,11 +1685,37 @@ static rf_table* group_agg(rf_query* q, const Spec& fs, const GjBuild& B, const 
     // rf_part_scatter's LDS: 16 bytes per partition and a 4096- or 8192-row chunk.
     if (nparts > 4096 || np > 8 || K >= ((int64_t)1 << 31) ||
         (size_t)nparts * 16 + 4096 * row > 160 * 1024)
-        return nullptr;
+        return false;
+    lay.shift = shift;
+    lay.nparts = nparts;
+    return true;
+}
+
+// The default mode's group fold after the probe scan (no gather, no sort):
+// the staged joined rows are moved into partitions of 2^shift consecutive
+// ranks (rf_part_hist unless the probe scan counted them, rf_part_scatter)
+// and a block per partition folds its groups with LDS atomics and writes
+// their outputs (rf_gj_agg).
+static rf_table* group_agg(rf_query* q, const Spec& fs, const GjBuild& B, const rf_arg* args,
+                           int64_t n_args, rf_query* pq, Launch& lp, const PartLayout& lay,
+                           bool counted) {
+    Device& d = device();
+    hipStream_t s = d.stream;
+    GroupJoinState& G = *q->gj;
+    const GroupJoinPlan& gp = *q->plan.gj;
+    const bool pend = B.pending_groups != nullptr;
+    const int64_t K = pend ? B.groups_ub : B.groups;
+    int nslot = 0;
+    for (auto& f : gp.fields) nslot += f.updated ? 1 : 0;
+    const int shift = lay.shift;
+    const int64_t nparts = lay.nparts;
+    const int np = (int)gp.pay_types.size();
+    size_t row = 4;  // bytes of a joined row: rank and payload
+    for (auto& t : gp.pay_types) row += (size_t)t.width();
     const RfScanArgs& A = lp.args;
     const size_t cap = lp.empty ? 1 : (size_t)std::max<int64_t>(A.stage_rows * A.nwaves, 1);
     const int64_t wpb = 4;  // wave regions per partition block
-    if (!lp.empty)
+    if (!lp.empty)  // (the caller grew it before a probe scan that counts)
         grow(G.hist, G.hist_cap, (size_t)aot_part_entries(nparts, A.nwaves, wpb) * 4);
     if (!G.part_small) G.part_small = dev_alloc(256);
     if (cap > G.part_cap || G.ppay.size() != (size_t)np) {
@@ -1737,6 +1760,7 @@ static rf_table* group_agg(rf_query* q, const Spec& fs, const GjBuild& B, const 
         P.boff = (uint32_t*)G.hist;
         P.prank = (uint32_t*)G.prank;
         P.groups_dev = pend ? B.groups_dev : nullptr;
+        P.counted = counted ? 1 : 0;
         RF_HIP(aot_partition(P, s));
     }
     std::unique_ptr<rf_table> out(alloc_table(gp.out_types, K, 1));
@@ -1869,16 +1893,34 @@ static rf_table* run_group_join(rf_query* q, const rf_arg* args, int64_t n_args,
     for (size_t j = 0; j < gp.probe_plan.probes.size(); j++)
         if (gp.probe_plan.probes[j].external) pq->probe_override[j] = h;
 
-    // 4. Probe scan: (rank, payload) of joined probe rows, in row order.
+    // 4. Probe scan: (rank, payload) of joined probe rows, in row order. For
+    // the default mode's partitioned fold, a probe scan that compacts in LDS
+    // also counts its joined rows per partition (blocks of four waves are the
+    // partition kernels' blocks), so no counting pass follows.
     Launch lp = prepare(pq, args, n_args, nullptr);
     char* hs = (char*)pq->host_stage;
-    if (tm) RF_HIP(hipEventRecord(tm->scan0, s));
-    launch(pq, lp, s, true);
-    if (tm) RF_HIP(hipEventRecord(tm->scan1, s));
-    if (!q->det && fs.mod->agg && lp.spec->ks.stage_types == lp.spec->ks.out_types) {
-        rf_table* t = group_agg(q, fs, *B, args, n_args, pq, lp);
-        if (t) return verified(t);
+    PartLayout lay;
+    const bool part = !q->det && fs.mod->agg &&
+                      lp.spec->ks.stage_types == lp.spec->ks.out_types && part_layout(gp, *B, lay);
+    bool counted = false;
+    size_t dyn_lds = 0;
+    if (part && !lp.empty && lp.spec->ks.lds_emit && lp.spec->ks.tune.block == 256) {
+        grow(G.hist, G.hist_cap, (size_t)aot_part_entries(lay.nparts, lp.args.nwaves, 4) * 4);
+        lp.args.part_boff = (uint32_t*)G.hist;
+        lp.args.part_nparts = lay.nparts;
+        lp.args.part_shift = lay.shift;
+        lp.args.part_groups_dev = B->pending_groups ? B->groups_dev : nullptr;
+        dyn_lds = (size_t)lay.nparts * 4;
+        counted = true;
     }
+    if (tm) RF_HIP(hipEventRecord(tm->scan0, s));
+    if (!lp.empty) {
+        void* params[] = {&lp.args};
+        RF_HIP(hipModuleLaunchKernel(lp.spec->mod->fn, lp.grid, 1, 1, lp.spec->ks.tune.block, 1, 1,
+                                     (unsigned)dyn_lds, s, params, nullptr));
+    }
+    if (tm) RF_HIP(hipEventRecord(tm->scan1, s));
+    if (part) return verified(group_agg(q, fs, *B, args, n_args, pq, lp, lay, counted));
     launch_gather(lp, s);
     RF_HIP(hipMemcpyAsync(hs, pq->out_count, 8, hipMemcpyDeviceToHost, s));
     RF_HIP(hipStreamSynchronize(s));

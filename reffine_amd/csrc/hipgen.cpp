@@ -1133,11 +1133,16 @@ KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nu
          "const Carry& cy, int s, int k) {\n        (void)cy; (void)s; (void)k;\n"
       << es.str() << "    }\n";
     o << "    enum { LDS_EMIT = " << (lds_emit ? 1 : 0) << " };\n";
+    ks.lds_emit = lds_emit;
     o << "    struct EmitBuf {" << (lds_emit ? eb.str() : std::string(" int unused;")) << " };\n";
     o << "    static __device__ __forceinline__ void emit_lds(EmitBuf& b, int i, int64_t row, "
          "const Carry& cy, int s, int k) {\n        (void)b; (void)i; (void)row; (void)cy; (void)s; "
          "(void)k;\n"
       << (lds_emit ? el.str() : std::string()) << "    }\n";
+    // The partition key of a staged row (a group join's rank, staged first).
+    o << "    static __device__ __forceinline__ int32_t part_key(const EmitBuf& b, int i) {\n"
+         "        (void)b; (void)i;\n"
+      << (lds_emit && early ? "        return (int32_t)b.c0[i];\n" : "        return 0;\n") << "    }\n";
     o << "    static __device__ __forceinline__ void flush(const RfScanArgs& L, const EmitBuf& b, int i, "
          "int64_t pos) {\n        (void)L; (void)b; (void)i; (void)pos;\n"
       << (lds_emit ? fl.str() : std::string()) << "    }\n";

@@ -58,6 +58,7 @@ struct KernelSource {
     int acc_bytes = 0;
     bool has_sel = false;
     int lds_agg_slots = 0;         // a probe that pre-aggregates in LDS: slots per group
+    bool lds_emit = false;         // selection compacted in LDS (can count partitions)
     std::vector<Type> out_types;   // selection output columns
     std::vector<Type> stage_types; // staged per selected row: row index, carried probe results
     Tuning tune;

@@ -83,6 +83,16 @@ struct RfScanArgs {
     // each block writes its slots ([block][field slot][group]).
     int64_t agg_gpp;
     unsigned long long* agg_part;
+    // A group join's probe scan that also counts its joined rows per
+    // partition (rank >> part_shift) for the partitioned fold: block b
+    // writes its count of partition p at part_boff[p * gridDim.x + b], for
+    // the partitions below ceil(groups / 2^part_shift) (groups: part_groups_dev
+    // when set, else part_nparts is exact). Null: no counting.
+    uint32_t* part_boff;
+    int64_t part_nparts;
+    const int64_t* part_groups_dev;
+    int32_t part_shift;
+    int32_t part_pad;
     union RfParam p[RF_MAXPARAMS];
 };
 
@@ -167,6 +177,8 @@ struct RfPartArgs {
     // partitions from ceil(groups / 2^shift) on are empty (nparts, a bound,
     // still lays out boff, ptot, pstart and the LDS).
     const int64_t* groups_dev;
+    int32_t counted;           // the probe scan already wrote boff's counts
+    int32_t pad;
 };
 
 // The partition of one rf_gj_agg block p: joined rows [pstart[p * stride],

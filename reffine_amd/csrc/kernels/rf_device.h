@@ -426,6 +426,13 @@ __device__ __forceinline__ void rf_tile_emit(const RfScanArgs& L, int64_t row0, 
     }
 }
 
+// Partitions the fused histogram of a group join's probe scan counts.
+__device__ __forceinline__ int64_t rf_scan_parts(const RfScanArgs& L) {
+    if (!L.part_groups_dev) return L.part_nparts;
+    const int64_t n = (*L.part_groups_dev + ((int64_t)1 << L.part_shift) - 1) >> L.part_shift;
+    return n < L.part_nparts ? n : L.part_nparts;
+}
+
 // rf_tile_emit through the wave's LDS buffer: the load group's qualifying
 // rows are written to LDS at their place among the group's rows (the same
 // row order), then the wave writes them to the staging columns 64 rows per
@@ -436,7 +443,7 @@ template <class Q>
 __device__ __forceinline__ void rf_tile_emit_lds(const RfScanArgs& L, int64_t row0, uint32_t lane,
                                                  const uint32_t* bits, const typename Q::Carry& cy,
                                                  int64_t base, int64_t& cursor,
-                                                 typename Q::EmitBuf& buf) {
+                                                 typename Q::EmitBuf& buf, uint32_t* hist) {
     constexpr int ITEMS = Q::ITEMS, LSUB = Q::LSUB;
     uint32_t n = 0;  // the group's qualifying rows so far (wave-uniform)
 #pragma unroll
@@ -464,7 +471,10 @@ __device__ __forceinline__ void rf_tile_emit_lds(const RfScanArgs& L, int64_t ro
     }
     if (n == 0) return;
     __builtin_amdgcn_wave_barrier();
-    for (uint32_t i = lane; i < n; i += 64) Q::flush(L, buf, (int)i, base + cursor + i);
+    for (uint32_t i = lane; i < n; i += 64) {
+        Q::flush(L, buf, (int)i, base + cursor + i);
+        if (L.part_boff) atomicAdd(&hist[(uint32_t)Q::part_key(buf, (int)i) >> L.part_shift], 1u);
+    }
     __builtin_amdgcn_wave_barrier();
     cursor += n;
 }
@@ -507,11 +517,21 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
     __shared__ uint64_t s_scan[WAVES];
     __shared__ uint32_t s_last;
     __shared__ typename Q::EmitBuf s_emit[Q::LDS_EMIT ? WAVES : 1];  // per-wave emit buffers
-    extern __shared__ unsigned long long rf_agg_lds[];  // LDS_AGG: NSLOT x agg_gpp slots
+    // Dynamic LDS: the LDS_AGG slots (NSLOT x agg_gpp), or the fused
+    // partition histogram of an LDS_EMIT probe scan (part_nparts counters).
+    extern __shared__ unsigned long long rf_agg_lds[];
+    uint32_t* const rf_hist = reinterpret_cast<uint32_t*>(rf_agg_lds);
     if constexpr (Q::LDS_AGG) {
         for (int64_t i = threadIdx.x; i < (int64_t)Q::NSLOT * L.agg_gpp; i += BLOCK)
             rf_agg_lds[i] = Q::agg_ident((int)(i / L.agg_gpp));
         __syncthreads();
+    }
+    if constexpr (Q::LDS_EMIT) {
+        if (L.part_boff) {  // uniform: a kernel argument
+            const int64_t np = rf_scan_parts(L);
+            for (int64_t i = threadIdx.x; i < np; i += BLOCK) rf_hist[i] = 0u;
+            __syncthreads();
+        }
     }
 
     const uint32_t lane = __lane_id();
@@ -543,7 +563,7 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
         if constexpr (Q::HAS_SEL) {
             if constexpr (Q::LDS_EMIT)
                 rf_tile_emit_lds<Q>(L, row0, lane, bits, cy, gw * L.stage_rows, cursor,
-                                    s_emit[threadIdx.x >> 6]);
+                                    s_emit[threadIdx.x >> 6], rf_hist);
             else
                 rf_tile_emit<Q>(L, row0, lane, bits, cy, gw * L.stage_rows, cursor);
         }
@@ -588,6 +608,15 @@ __device__ __forceinline__ void rf_scan(const RfScanArgs& L) {
 
     if constexpr (Q::HAS_SEL) {
         if (lane == 0) L.wave_cnt[gw] = cursor;
+    }
+    if constexpr (Q::LDS_EMIT) {
+        if (L.part_boff) {
+            // The block's column of the partition count matrix.
+            __syncthreads();
+            const int64_t np = rf_scan_parts(L);
+            for (int64_t i = threadIdx.x; i < np; i += BLOCK)
+                L.part_boff[i * (int64_t)gridDim.x + blockIdx.x] = rf_hist[i];
+        }
     }
     if constexpr (Q::LDS_AGG) {
         // The block's slots, for rf_gj_lds_done to combine across blocks.

@@ -1196,8 +1196,9 @@ hipError_t aot_partition(const RfPartArgs& P0, hipStream_t s) {
     if (P.waves_per_block < 1 || P.waves_per_block > 8) return hipErrorInvalidValue;
     const int64_t nb = (P.nwaves + P.waves_per_block - 1) / P.waves_per_block;
     P.nblocks = nb;
-    hipLaunchKernelGGL(rf_part_hist, dim3((unsigned)nb), dim3(RF_PART_BLOCK),
-                       (size_t)P.nparts * 4, s, P);
+    if (!P.counted)
+        hipLaunchKernelGGL(rf_part_hist, dim3((unsigned)nb), dim3(RF_PART_BLOCK),
+                           (size_t)P.nparts * 4, s, P);
     // Each block's offset in each partition and the partition totals.
     hipLaunchKernelGGL(rf_part_rowscan, dim3((unsigned)P.nparts), dim3(256), 0, s, P);
     size_t row = 4;
