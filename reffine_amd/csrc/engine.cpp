@@ -1366,7 +1366,8 @@ int bits_of(uint64_t range) {
 // count and key range, then the duplicate-key check).
 static const char* kDupMsg =
     "group join: the build table holds a qualifying key more than once "
-    "(each group key must occur in one build row)";
+    "(each group key must occur in one build row), or its key column changed "
+    "after the table's key range was taken";
 
 static std::shared_ptr<GjBuild> make_build(rf_query* q, const rf_arg* args, int64_t n_args,
                                            bool defer_check) {
@@ -1419,8 +1420,8 @@ static std::shared_ptr<GjBuild> make_build(rf_query* q, const rf_arg* args, int6
                                             B->dense_bits, words, B->dense_cum, B->dense_pack,
                                             hd + 6144, kdev, (int64_t*)(hd + 6160), s));
                 RF_HIP(aot_rank_place((const int64_t*)bq->outs[0], (const int64_t*)bq->outs[1], nb,
-                                      kdev, tmin, B->dense_bits, B->dense_cum, (int64_t*)B->keys,
-                                      (int64_t*)B->rows, s));
+                                      kdev, tmin, B->dense_bits, words, B->dense_cum,
+                                      (int64_t*)B->keys, (int64_t*)B->rows, s));
                 B->kmin = tmin;
                 B->groups_ub = nb;
                 B->groups_dev = kdev;
@@ -1485,8 +1486,8 @@ static std::shared_ptr<GjBuild> make_build(rf_query* q, const rf_arg* args, int6
                                         B->dense_bits, words, B->dense_cum, B->dense_pack,
                                         total_dev, nullptr, nullptr, s));
             RF_HIP(aot_rank_place((const int64_t*)bq->outs[0], (const int64_t*)bq->outs[1], K,
-                                  nullptr, kmin, B->dense_bits, B->dense_cum, (int64_t*)B->keys,
-                                  (int64_t*)B->rows, s));
+                                  nullptr, kmin, B->dense_bits, words, B->dense_cum,
+                                  (int64_t*)B->keys, (int64_t*)B->rows, s));
             B->groups = K;
             B->kmin = kmin;
             B->pending_total = (const volatile uint64_t*)(hs + 6144);

@@ -65,8 +65,8 @@ hipError_t aot_rank_build_small(const int64_t* keys, int64_t n, const int64_t* n
                                 void* bits, int64_t words, void* cum, void* pack, void* total,
                                 const int64_t* k_in, int64_t* k_out, hipStream_t s);
 hipError_t aot_rank_place(const int64_t* keys, const int64_t* rows, int64_t n, const int64_t* n_dev,
-                          int64_t base, const void* bits, const void* cum, int64_t* keys_out,
-                          int64_t* rows_out, hipStream_t s);
+                          int64_t base, const void* bits, int64_t words, const void* cum,
+                          int64_t* keys_out, int64_t* rows_out, hipStream_t s);
 hipError_t aot_merge_splits(const int64_t* a, int64_t na, const int64_t* b, int64_t nb,
                             int64_t off_a, int64_t off_b, int64_t tile, int64_t n, int64_t* out,
                             hipStream_t s);

@@ -476,18 +476,21 @@ extern "C" __global__ void rf_seg_bounds(const uint32_t* __restrict__ rank, int6
 // per word issues the atomic. The loop steps whole waves: every lane takes
 // part in the shuffles.
 // With n_dev set, the keys are the first min(*n_dev, n) (a count an earlier
-// kernel wrote).
+// kernel wrote). A key outside the map's `words` words marks nothing (it
+// cannot happen when the range came from the keys; the caller's count of
+// marked keys then falls short of the keys' and reports it).
 extern "C" __global__ void rf_rank_bits(const int64_t* __restrict__ keys, int64_t n,
                                         const int64_t* __restrict__ n_dev, int64_t base,
-                                        unsigned long long* bits) {
+                                        unsigned long long* bits, int64_t words) {
     if (n_dev && *n_dev < n) n = *n_dev;
     const int lane = threadIdx.x & 63;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); i0 < n;
          i0 += stride) {
         const int64_t i = i0 + lane;
-        const bool act = i < n;
-        const uint64_t off = act ? (uint64_t)keys[i] - (uint64_t)base : 0;
+        const uint64_t off0 = i < n ? (uint64_t)keys[i] - (uint64_t)base : 0;
+        const bool act = i < n && (off0 >> 6) < (uint64_t)words;
+        const uint64_t off = act ? off0 : 0;
         // Inactive lanes get word numbers no key has (distinct per lane).
         const uint64_t w = act ? (off >> 6) : ~(uint64_t)lane;
         uint64_t b = act ? (1ull << (off & 63)) : 0;
@@ -544,12 +547,13 @@ extern "C" __global__ void rf_rank_scatter(const int64_t* __restrict__ keys,
                                            const int64_t* __restrict__ n_dev, int64_t base,
                                            const uint64_t* __restrict__ bits,
                                            const uint32_t* __restrict__ cum, int64_t* keys_out,
-                                           int64_t* rows_out) {
+                                           int64_t* rows_out, int64_t words) {
     if (n_dev && *n_dev < n) n = *n_dev;  // as in rf_rank_bits
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t k = keys[i];
         const uint64_t off = (uint64_t)k - (uint64_t)base;
+        if ((off >> 6) >= (uint64_t)words) continue;  // as in rf_rank_bits
         const uint64_t w = bits[off >> 6];
         const int64_t r = (int64_t)cum[off >> 6] + (int64_t)__popcll(w & ((1ull << (off & 63)) - 1));
         keys_out[r] = k;
@@ -1056,7 +1060,7 @@ hipError_t aot_rank_scatter(const int64_t* keys, const int64_t* rows, int64_t n,
     if (n <= 0 || words <= 0) return hipSuccess;
     hipLaunchKernelGGL(rf_rank_scatter, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, rows, n,
                        (const int64_t*)nullptr, base, (const uint64_t*)bits, (const uint32_t*)cum,
-                       keys_out, rows_out);
+                       keys_out, rows_out, words);
     hipLaunchKernelGGL(rf_rank_total, dim3(1), dim3(64), 0, s, (const uint64_t*)bits,
                        (const uint32_t*)cum, words, (uint64_t*)total);
     return hipGetLastError();
@@ -1073,7 +1077,7 @@ hipError_t aot_rank_build(const int64_t* keys, int64_t n, int64_t base, void* bi
                           void* cum, hipStream_t s) {
     if (n <= 0 || words <= 0) return hipSuccess;
     hipLaunchKernelGGL(rf_rank_bits, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, n,
-                       (const int64_t*)nullptr, base, (unsigned long long*)bits);
+                       (const int64_t*)nullptr, base, (unsigned long long*)bits, words);
     hipLaunchKernelGGL(rf_word_popc, dim3(grid_cap(words, 256, 4096)), dim3(256), 0, s,
                        (const uint64_t*)bits, words, (uint32_t*)cum);
     uint32_t* c = (uint32_t*)cum;
@@ -1097,7 +1101,7 @@ hipError_t aot_rank_build_small(const int64_t* keys, int64_t n, const int64_t* n
     if (words <= 0) return hipSuccess;
     if (n > 0)
         hipLaunchKernelGGL(rf_rank_bits, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, n,
-                           n_dev, base, (unsigned long long*)bits);
+                           n_dev, base, (unsigned long long*)bits, words);
     hipLaunchKernelGGL(rf_rank_finish, dim3(1), dim3(1024), 0, s, (const uint64_t*)bits, words,
                        (uint32_t*)cum, (unsigned long long*)pack, (uint64_t*)total,
                        k_out ? k_in : (const int64_t*)nullptr, k_out);
@@ -1107,12 +1111,12 @@ hipError_t aot_rank_build_small(const int64_t* keys, int64_t n, const int64_t* n
 // Places (key, row) at each key's rank (rf_rank_scatter alone); n_dev as in
 // aot_rank_build_small.
 hipError_t aot_rank_place(const int64_t* keys, const int64_t* rows, int64_t n, const int64_t* n_dev,
-                          int64_t base, const void* bits, const void* cum, int64_t* keys_out,
-                          int64_t* rows_out, hipStream_t s) {
+                          int64_t base, const void* bits, int64_t words, const void* cum,
+                          int64_t* keys_out, int64_t* rows_out, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(rf_rank_scatter, dim3(grid_cap(n, 256, 4096)), dim3(256), 0, s, keys, rows, n,
                        n_dev, base, (const uint64_t*)bits, (const uint32_t*)cum, keys_out,
-                       rows_out);
+                       rows_out, words);
     return hipGetLastError();
 }
 
