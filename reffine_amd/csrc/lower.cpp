@@ -1495,8 +1495,61 @@ struct Lowerer {
                                   "only component");
             plan.sequential = true;
         }
+        guard_probes();
         plan.output = out;
         return plan;
+    }
+
+    // A leaf of the scanned row that no probe feeds: its columns and their
+    // validity, the row index, scalar arguments and lengths.
+    bool probe_free(const N& e) const {
+        const std::string drv = std::to_string(plan.driver) + ".";
+        return !mentions(e, [&](const N& n) {
+            if (n->kind != K::Sym) return false;
+            const std::string ln = leaf_name(n);
+            if (ln.empty()) return true;
+            if (ln == "$row" || ln.rfind("$arg.", 0) == 0 || ln.rfind("$len.", 0) == 0)
+                return false;
+            if (ln.rfind("$col." + drv, 0) == 0 || ln.rfind("$valid." + drv, 0) == 0)
+                return false;
+            return true;
+        });
+    }
+
+    // Every output of the scan (the Op's rows, each updated reduction field)
+    // is taken only on rows whose predicate holds. The predicate's conjuncts
+    // that read no probe and appear in every output's predicate guard the
+    // probes: a row failing them is probed as "key absent" (-1), which no
+    // output can see, and skips the index lookup's dependent loads.
+    void guard_probes() {
+        if (plan.probes.empty() || plan.sequential || plan.source != Plan::TABLE_ROWS) return;
+        for (auto& pr : plan.probes)
+            if (pr.table == plan.driver) return;
+        std::vector<N> preds;
+        for (auto& o : plan.ops) preds.push_back(o.pred);
+        for (auto& r : plan.reduces)
+            for (auto& f : r.fields)
+                if (f.updated) preds.push_back(f.pred);
+        if (preds.empty()) return;
+        std::vector<std::vector<N>> cs(preds.size());
+        for (size_t i = 0; i < preds.size(); i++) {
+            if (!preds[i]) return;
+            conjuncts(preds[i], cs[i]);
+        }
+        N guard;
+        for (auto& c : cs[0]) {
+            if (!probe_free(c)) continue;
+            bool everywhere = true;
+            for (size_t i = 1; i < cs.size() && everywhere; i++) {
+                bool found = false;
+                for (auto& d : cs[i]) found = found || same_tree(c, d);
+                everywhere = found;
+            }
+            if (everywhere) guard = guard ? mk_nary(MathOp::AND, {guard, c}) : c;
+        }
+        if (!guard) return;
+        for (auto& pr : plan.probes)
+            if (!pr.external && !pr.guard) pr.guard = guard;
     }
 };
 
