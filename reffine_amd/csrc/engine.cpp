@@ -920,13 +920,36 @@ static Spec& get_run_spec(rf_query* q, const rf_arg* args, int64_t n_args) {
     return q->run_specs.emplace(sig, std::move(sp)).first->second;
 }
 
+// The scan kernel's key: the inputs' nullability, then each probed table's
+// index form (a packed rank map over ascending keys, 'p', is probed without
+// branches, so a tile's rows issue their probes together). `build` builds a
+// missing index first (a table without one keys as 'h').
+static std::string scan_signature(rf_query* q, const rf_arg* args, int64_t n_args, bool build,
+                                  std::vector<bool>* pack) {
+    std::string sig = null_signature(q->plan.inputs, args, n_args);
+    if (q->plan.seg || q->plan.source != Plan::TABLE_ROWS) return sig;
+    for (const Probe& pr : q->plan.probes) {
+        bool p = false;
+        if (!pr.external && pr.table >= 0 && pr.table < n_args) {
+            rf_table* pt = args[pr.table].u.table;
+            if (build && !pt->has_index) table_build_index(pt);
+            p = pt->has_index && pt->index.dense_pack && !pt->index.dense_rows &&
+                pt->index.dense_len > 0;
+        }
+        if (pack) pack->push_back(p);
+        sig += p ? 'p' : 'h';
+    }
+    return sig;
+}
+
 Spec* find_spec(rf_query* q, const rf_arg* args, int64_t n_args) {
-    auto it = q->specs.find(null_signature(q->plan.inputs, args, n_args));
+    auto it = q->specs.find(scan_signature(q, args, n_args, false, nullptr));
     return it == q->specs.end() ? nullptr : &it->second;
 }
 
 static Spec& get_spec(rf_query* q, const rf_arg* args, int64_t n_args) {
-    std::string sig = null_signature(q->plan.inputs, args, n_args);
+    std::vector<bool> pack;
+    std::string sig = scan_signature(q, args, n_args, true, &pack);
     auto it = q->specs.find(sig);
     if (it != q->specs.end()) return it->second;
     std::vector<std::vector<bool>> nl(q->plan.inputs.size());
@@ -936,7 +959,7 @@ static Spec& get_spec(rf_query* q, const rf_arg* args, int64_t n_args) {
     }
     Spec sp;
     sp.ks = q->plan.seg ? gen_seg_fold(*q->plan.seg, q->plan, nl)
-                        : gen_scan(q->plan, nl, q->tune, q->det);
+                        : gen_scan(q->plan, nl, q->tune, q->det, pack);
     sp.mod = &jit_module(sp.ks);
     auto r = q->specs.emplace(sig, std::move(sp));
     return r.first->second;

@@ -147,7 +147,7 @@ struct Emitter {
         if (!probes_done.count(j)) {
             const Probe& pr = g.plan.probes[j];
             std::string k = expr(pr.key);
-            if (mode == Mode::MOVE && g.packed(j)) {
+            if (g.packed(j)) {
                 const std::string gd = pr.guard ? expr(pr.guard) : std::string("true");
                 os << indent << "const int64_t " << h << " = rf_probe_pack(L.probe[" << j
                    << "], (int64_t)" << k << ", " << gd << ");\n";
@@ -228,7 +228,7 @@ struct Emitter {
             int s = g.col_slot(t, c, false);
             std::string h = probe(j);
             const std::string T = cty(g.ks.cols[s].type);
-            if (mode == Mode::MOVE && g.packed(j)) {
+            if (g.packed(j)) {
                 const std::string v = tmp("rf_g((const " + T + "*)L.col[" + std::to_string(s) +
                                           "])[" + h + " < 0 ? 0 : " + h + "]");
                 return tmp("(" + h + " >= 0 ? " + v + " : (" + T + ")0)");
@@ -831,13 +831,14 @@ static KernelSource gen_merge(const Plan& plan, const std::vector<std::vector<bo
 }
 
 KernelSource gen_scan(const Plan& plan, const std::vector<std::vector<bool>>& nullable,
-                      const Tuning& tune, bool det) {
+                      const Tuning& tune, bool det, const std::vector<bool>& pack_probes) {
     if (plan.source == Plan::MERGE) return gen_merge(plan, nullable);
     if (plan.sequential) return gen_seq_scan(plan, nullable, det);
     KernelSource ks;
     ks.tune = tune;
     ks.det = det;
     Gen g(plan, nullable, ks);
+    g.pack_probes = pack_probes;
     if (tune.block % 64 != 0 || tune.block > 1024) throw IRError("block size");
     if (16 % tune.items != 0) throw IRError("ITEMS must divide 16");
     ks.has_sel = !plan.ops.empty();

@@ -73,9 +73,12 @@ struct KernelSource {
 };
 
 // `nullable[t][c]` says whether column c of func input t has a bitmap.
+// `pack_probes[j]` as for gen_seg_fold: probe j's table is a packed rank map
+// over ascending keys with at least one key (probed without branches).
 KernelSource gen_scan(const Plan& plan,
                       const std::vector<std::vector<bool>>& nullable,
-                      const Tuning& tune, bool det);
+                      const Tuning& tune, bool det,
+                      const std::vector<bool>& pack_probes = {});
 
 // Per-group fold and output kernel (rf_gj_fold) of a group join;
 // `nullable` describes the function inputs as for gen_scan.

@@ -187,3 +187,55 @@ def test_deterministic_minmax_nan_and_signed_zero(device, use_min):
             want = detorder.det_fold(x, m, 0, n, *q.tuning()[:3], op="min" if use_min else "max",
                                      init=init)
             assert np.float64(got).view(np.uint64) == np.float64(want).view(np.uint64)
+
+
+def _guarded_join(thr):
+    """_op over t in A with A[t].v > thr and t in B: (t, A[t].v + B[t].v).
+    The probe-free conjunct A[t].v > thr guards the probe of B's index
+    (joins through _in as in test/src/test_join.cpp:9-35)."""
+    from reffine_amd.reffiner import _add, _func, _gt, _i64, _i64_t, _in, _op, _sym, _vec_t
+    ty = _vec_t(1, _i64_t, _i64_t)
+    a, b = _sym("avec", ty), _sym("bvec", ty)
+    t = _sym("t", _i64_t)
+    ae, be = a[t][0], b[t][0]
+    a_s, b_s = _sym("a", ae), _sym("b", be)
+    op = _op([t], _in(t, a) & _gt(a_s, _i64(thr)) & _in(t, b), [_add(a_s, b_s)])
+    op_sym = _sym("op", op)
+    fn = _func("guarded_join", op_sym, [a, b])
+    fn.tbl[a_s] = ae
+    fn.tbl[b_s] = be
+    fn.tbl[op_sym] = op
+    return fn
+
+
+@pytest.mark.parametrize("ascending", [True, False])
+def test_guarded_probe_misses(device, ascending):
+    """A scan's probe of a rank map: B's keys strictly ascending with gaps
+    (probed without branches) or the same rows shuffled (a row table
+    behind the map). A's keys fall below B's range, past it and into its
+    interior gaps, and the guard A[t].v > thr is false on about half the
+    rows; the output equals the numpy join."""
+    rng = np.random.default_rng(11)
+    m = 300_000
+    i = np.arange(m, dtype=np.int64)
+    bk = 100 + 3 * i + (i % 2)
+    bv = rng.integers(-1000, 1000, m, dtype=np.int64)
+    ak = np.arange(0, int(bk[-1]) + 200, dtype=np.int64)
+    av = rng.integers(-1000, 1000, len(ak), dtype=np.int64)
+    order = np.arange(m) if ascending else rng.permutation(m)
+    A = DeviceTable.from_columns([ak, av], dim=1)
+    B = DeviceTable.from_columns([bk[order], bv[order]], dim=1)
+    q = compile_op(_guarded_join(0))
+    out = q(A, B)
+    pos = np.searchsorted(bk, ak)
+    hit = (pos < m) & (bk[np.minimum(pos, m - 1)] == ak)
+    keep = hit & (av > 0)
+    assert keep.sum() > 100_000 and (hit & ~(av > 0)).sum() > 100_000
+    np.testing.assert_array_equal(out.column(0), ak[keep])
+    np.testing.assert_array_equal(out.column(1), av[keep] + bv[np.minimum(pos, m - 1)][keep])
+    # A second table of the other index form through the same query.
+    B2 = DeviceTable.from_columns([bk[::-1].copy(), bv[::-1].copy()], dim=1) if ascending \
+        else DeviceTable.from_columns([bk, bv], dim=1)
+    out2 = q(A, B2)
+    np.testing.assert_array_equal(out2.column(0), ak[keep])
+    np.testing.assert_array_equal(out2.column(1), av[keep] + bv[np.minimum(pos, m - 1)][keep])
