@@ -279,9 +279,10 @@ int rf_query_run(rf_query* q, const rf_arg* args, int64_t n_args,
  * iterators then outputs) are placed into the device buffers `out_cols[j]`,
  * each with room for `out_cap` rows; `*out_rows` receives the row count
  * (RF_ERR_NOMEM when it exceeds out_cap; the first out_cap rows are placed).
- * The kernel for the tables' null layout must be loaded (rf_compile loads
- * the no-null one; a run through rf_query_run loads others) and probed
- * tables indexed, else RF_ERR_INVALID. Calls share the device's scratch and
+ * The kernel for the tables' null layout and index forms must be loaded
+ * (rf_compile loads the no-null one for probed tables whose index is not a
+ * packed rank map over ascending keys; a run through rf_query_run loads the
+ * others) and probed tables indexed, else RF_ERR_INVALID. Calls share the device's scratch and
  * synchronise before returning; calls that overlap on one GPU race. */
 #define RF_NOALLOC_SCRATCH_BYTES (64ll << 20)
 int rf_query_run_noalloc(rf_query* q, const rf_arg* args, int64_t n_args,

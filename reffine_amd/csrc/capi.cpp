@@ -243,6 +243,10 @@ static std::unique_ptr<rf_query> make_query(const rf::N& f0, uint32_t flags,
         if (!q->plan.inputs[i]->type.is_vector()) continue;
         sig += std::string(q->plan.inputs[i]->type.dtypes.size(), '0') + "|";
     }
+    // Its probes take any index (the form keyed 'h'; a packed rank map over
+    // ascending keys gets its own kernel at its first run).
+    if (!q->plan.seg && q->plan.source == rf::Plan::TABLE_ROWS)
+        for (size_t j = 0; j < q->plan.probes.size(); j++) sig += 'h';
     q->specs.emplace(sig, std::move(sp));
     if (q->plan.runs) {
         // The run stage's fold over T with its own (run-end) type.

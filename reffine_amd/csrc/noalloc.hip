@@ -60,14 +60,15 @@ void run_query_noalloc(rf_query* q, const rf_arg* args, int64_t n_args, void* ou
     check_args(q->plan, args, n_args);
     // Nothing is compiled or built here: the kernel for this null layout and
     // the probed tables' indexes must exist.
-    if (!find_spec(q, args, n_args))
-        throw IRError("no kernel is loaded for these tables' null layout: run the query once "
-                      "through rf_query_run on tables with the same validity buffers");
     for (const Probe& pr : q->plan.probes) {
         if (pr.external) throw Unsupported("external probes need a group join's build side");
         if (!args[pr.table].u.table->has_index)
             throw IRError("a probed table has no index: call rf_table_build_index first");
     }
+    if (!find_spec(q, args, n_args))
+        throw IRError("no kernel is loaded for these tables' null layout and index forms: run "
+                      "the query once through rf_query_run on tables with the same validity "
+                      "buffers and index forms (packed rank map over ascending keys, or not)");
     Launch l = prepare_args(q, args, n_args);
     const KernelSource& ks = l.spec->ks;
     if ((size_t)ks.acc_bytes > 4096) throw Unsupported("reduction state larger than 4 KiB");

@@ -98,3 +98,31 @@ def test_limits_are_status_codes(device):
     st, _, rows = run_noalloc(q, [tn], out2, 50_000)
     assert st == RF_OK and rows == cnt_n
     np.testing.assert_array_equal(out2.column(0)[:rows], sel_n.column(0))
+
+
+def test_probe_query_over_both_index_forms(device):
+    """A selection that probes a second table: its kernel variant follows
+    the probed table's index form (a packed rank map over ascending keys,
+    or the same keys shuffled, with a row table). Each form runs once
+    through rf_query_run, then the no-allocation call gives the same rows;
+    the second call is over a smaller scanned table."""
+    from test_scan_gpu import _guarded_join
+    rng = np.random.default_rng(5)
+    m = 50_000
+    i = np.arange(m, dtype=np.int64)
+    bk = 100 + 3 * i + (i % 2)
+    bv = rng.integers(-1000, 1000, m, dtype=np.int64)
+    q = compile_op(_guarded_join(0))
+    cap = 200_000
+    out = DeviceTable.from_columns([np.zeros(cap, np.int64), np.zeros(cap, np.int64)], dim=1)
+    for order, na in ((np.arange(m), 160_000), (rng.permutation(m), 120_001)):
+        B = DeviceTable.from_columns([bk[order], bv[order]], dim=1)
+        ak = np.arange(0, na, dtype=np.int64)
+        av = rng.integers(-1000, 1000, na, dtype=np.int64)
+        A = DeviceTable.from_columns([ak, av], dim=1)
+        want = q(A, B)
+        st, raw, rows = run_noalloc(q, [A, B], out, cap)
+        assert st == RF_OK, lib().rf_last_error()
+        assert rows == want.nrows > 10_000
+        for c in range(2):
+            np.testing.assert_array_equal(out.column(c)[:rows], want.column(c))
